@@ -1,0 +1,7 @@
+"""gravitas_amd -- MI355X (gfx950) Barnes-Hut force step for the gravitas engine.
+
+The product is libgravitas_hip.so (C ABI in include/gravitas_hip.h); this
+package builds it (``gravitas_amd.build``) and mirrors the reference's
+``bh.Solver`` interface on top of it (``gravitas_amd.solver``).
+"""
+from .solver import DeviceState, Params, Solver, gpu_available, gpu_name  # noqa: F401

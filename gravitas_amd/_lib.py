@@ -1,0 +1,93 @@
+"""ctypes binding of libgravitas_hip.so (the C ABI declared in include/gravitas_hip.h).
+
+There is no CPU fallback: if the library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgravitas_hip.so")
+
+GX_OK = 0
+GX_ERR_NO_DEVICE = 1
+GX_ERR_CODE_OBJECT = 2
+GX_ERR_PIPELINE = 3
+GX_ERR_ARG = 4
+GX_ERR_HIP = 5
+GX_ERR_CAPACITY = 6
+GX_ERR_UNSUPPORTED = 7
+GX_WALK_FP32 = 0
+GX_WALK_FP64 = 1
+
+# Every symbol include/gravitas_hip.h declares.
+EXPORTS = (
+    "gx_init", "gx_name", "gx_error", "gx_solver_new", "gx_solver_free", "gx_solver_config",
+    "gx_solver_set_stream", "gx_accelerations", "gx_accelerations_dev", "gx_enqueue_accelerations_dev",
+    "gx_sync", "gx_potential_energy_dev", "gx_direct_sample_dev", "gx_last_seconds", "gx_get_stats",
+    "gx_phase_ms", "gx_export",
+)
+
+
+class GxStats(ctypes.Structure):
+    _fields_ = [
+        ("n_stars", ctypes.c_int64), ("n_holes", ctypes.c_int64), ("n_nodes", ctypes.c_int64),
+        ("pp_interactions", ctypes.c_int64), ("pc_interactions", ctypes.c_int64),
+        ("max_stack", ctypes.c_int32), ("flags", ctypes.c_int32), ("depth", ctypes.c_int32),
+        ("walk", ctypes.c_int32), ("fallbacks", ctypes.c_int64),
+    ]
+
+
+class GxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gravitas_hip error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m gravitas_amd.build` "
+                           "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    D = ctypes.c_double
+    I64 = ctypes.c_int64
+    I = ctypes.c_int
+    L.gx_init.argtypes = [ctypes.c_char_p, I]
+    L.gx_init.restype = I
+    L.gx_name.argtypes = [ctypes.c_char_p, I]
+    L.gx_name.restype = None
+    L.gx_error.restype = ctypes.c_char_p
+    L.gx_solver_new.restype = P
+    L.gx_solver_free.argtypes = [P]
+    L.gx_solver_config.argtypes = [P, D, I, I, I]
+    L.gx_solver_set_stream.argtypes = [P, P]
+    step = [P, P, P, P, P, P, P, I64, D, I, P, P, P]
+    for f in ("gx_accelerations", "gx_accelerations_dev", "gx_enqueue_accelerations_dev"):
+        getattr(L, f).argtypes = step
+        getattr(L, f).restype = I
+    L.gx_sync.argtypes = [P]
+    L.gx_potential_energy_dev.argtypes = [P, P, P, P, P, P, P, I64, D, I, P]
+    L.gx_direct_sample_dev.argtypes = [P, P, P, P, P, P, I64, P, I64, D, I, P]
+    L.gx_last_seconds.argtypes = [P]
+    L.gx_last_seconds.restype = D
+    L.gx_get_stats.argtypes = [P, ctypes.POINTER(GxStats)]
+    L.gx_phase_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float), I]
+    L.gx_export.argtypes = [P, I, P, I64]
+    L.gx_export.restype = I64
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != GX_OK:
+        raise GxError(rc, load().gx_error().decode(errors="replace"))
+    return rc

@@ -1,0 +1,583 @@
+// gx_build.hip -- tree construction on gfx950.
+//
+// Compiled with -ffp-contract=off: every FP64 expression here is evaluated
+// with the same operations in the same order as the reference, so keys,
+// permutation, node set and moments are bit-identical to it.
+//
+//   collect          internal/sim/bh/build.go:62-113
+//   keys             internal/sim/bh/build.go:19-31, morton.go:7-21
+//   radix sort       internal/sim/bh/morton.go:25-49 (stable LSD, 8-bit digits)
+//   sorted copies    internal/sim/bh/build.go:34-41, internal/sim/sim.go:93-98
+//   topology         internal/sim/bh/build.go:134-172, lowerBound :340-350
+//   moments          internal/sim/bh/build.go:176-217, quadrupole.add :326-337
+//
+// The reference builds top-down recursively; here each tree level is one
+// launch (one wavefront per node finds the octant boundaries with a 10-ary
+// search on the sorted keys), and moments are one launch per level from the
+// deepest level up.  Children of a node are allocated contiguously and in
+// octant order, and skip links follow the reference's rule, so the walk order
+// -- which alone determines the forces -- equals the reference's.  Only the
+// numbering of nodes differs (breadth-first instead of depth-first).
+#include "gx_common.h"
+#include "gx_kernels.h"
+
+namespace gx {
+
+// ---------------------------------------------------------------------------
+// collect
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_collect_count(const double* __restrict__ x, const double* __restrict__ y,
+                                                       const double* __restrict__ z,
+                                                       const uint8_t* __restrict__ kind,
+                                                       const uint8_t* __restrict__ alive, long long n,
+                                                       int* __restrict__ blockStar, int* __restrict__ blockHole,
+                                                       double* __restrict__ blockBox) {
+    __shared__ double sbox[6][4];
+    __shared__ int scnt[2][4];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    long long base = (long long)blockIdx.x * kCollectTile;
+    int cnt = 0, hcnt = 0;
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int j = 0; j < kCollectTile / 256; j++) {
+        long long i = base + j * 256 + tid;
+        if (i >= n) break;
+        bool a = alive[i] != 0;
+        uint8_t k = kind[i];
+        if (a && k == kHole) hcnt++;
+        if (a && k != kHole) {
+            cnt++;
+            double px = x[i], py = y[i], pz = z[i];
+            mn[0] = fmin(mn[0], px); mx[0] = fmax(mx[0], px);
+            mn[1] = fmin(mn[1], py); mx[1] = fmax(mx[1], py);
+            mn[2] = fmin(mn[2], pz); mx[2] = fmax(mx[2], pz);
+        }
+    }
+    cnt = wave_sum_i(cnt);
+    hcnt = wave_sum_i(hcnt);
+    for (int c = 0; c < 3; c++) {
+        mn[c] = wave_min_d(mn[c]);
+        mx[c] = wave_max_d(mx[c]);
+    }
+    if (lane == 0) {
+        scnt[0][w] = cnt;
+        scnt[1][w] = hcnt;
+        for (int c = 0; c < 3; c++) {
+            sbox[c][w] = mn[c];
+            sbox[3 + c][w] = mx[c];
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int c0 = 0, c1 = 0;
+        double b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (int ww = 0; ww < 4; ww++) {
+            c0 += scnt[0][ww];
+            c1 += scnt[1][ww];
+            for (int c = 0; c < 3; c++) {
+                b[c] = fmin(b[c], sbox[c][ww]);
+                b[3 + c] = fmax(b[3 + c], sbox[3 + c][ww]);
+            }
+        }
+        blockStar[blockIdx.x] = c0;
+        blockHole[blockIdx.x] = c1;
+        for (int c = 0; c < 6; c++) blockBox[blockIdx.x * 6 + c] = b[c];
+    }
+}
+
+// Single workgroup: exclusive scans of the per-block counts, box reduction.
+__global__ __launch_bounds__(1024) void k_collect_scan(int nb, int* __restrict__ blockStar,
+                                                       int* __restrict__ blockHole,
+                                                       const double* __restrict__ blockBox, Meta* meta) {
+    __shared__ long long ssum[2][1024];
+    __shared__ double sbox[6][16];
+    const int tid = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    int lo = tid * per, hi = min(nb, lo + per);
+    long long s0 = 0, s1 = 0;
+    double b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int i = lo; i < hi; i++) {
+        s0 += blockStar[i];
+        s1 += blockHole[i];
+        for (int c = 0; c < 3; c++) {
+            b[c] = fmin(b[c], blockBox[i * 6 + c]);
+            b[3 + c] = fmax(b[3 + c], blockBox[i * 6 + 3 + c]);
+        }
+    }
+    ssum[0][tid] = s0;
+    ssum[1][tid] = s1;
+    for (int c = 0; c < 3; c++) {
+        b[c] = wave_min_d(b[c]);
+        b[3 + c] = wave_max_d(b[3 + c]);
+    }
+    if ((tid & 63) == 0)
+        for (int c = 0; c < 6; c++) sbox[c][tid >> 6] = b[c];
+    __syncthreads();
+    // Hillis-Steele inclusive scan over the 1024 thread sums.
+    for (int d = 1; d < 1024; d <<= 1) {
+        long long a0 = tid >= d ? ssum[0][tid - d] : 0, a1 = tid >= d ? ssum[1][tid - d] : 0;
+        __syncthreads();
+        ssum[0][tid] += a0;
+        ssum[1][tid] += a1;
+        __syncthreads();
+    }
+    long long run0 = ssum[0][tid] - s0, run1 = ssum[1][tid] - s1;
+    for (int i = lo; i < hi; i++) {
+        int c0 = blockStar[i], c1 = blockHole[i];
+        blockStar[i] = (int)run0;
+        blockHole[i] = (int)run1;
+        run0 += c0;
+        run1 += c1;
+    }
+    if (tid == 0) {
+        double bb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (int ww = 0; ww < 16; ww++)
+            for (int c = 0; c < 3; c++) {
+                bb[c] = fmin(bb[c], sbox[c][ww]);
+                bb[3 + c] = fmax(bb[3 + c], sbox[3 + c][ww]);
+            }
+        for (int c = 0; c < 6; c++) meta->box[c] = bb[c];
+        meta->nstars = ssum[0][1023];
+        meta->nholes = ssum[1][1023];
+    }
+}
+
+// Stable compaction in state order: order[] gets the stars, holes[] the holes.
+__global__ __launch_bounds__(256) void k_collect_scatter(const uint8_t* __restrict__ kind,
+                                                         const uint8_t* __restrict__ alive, long long n,
+                                                         const int* __restrict__ starOff,
+                                                         const int* __restrict__ holeOff,
+                                                         int32_t* __restrict__ order, int32_t* __restrict__ holes) {
+    __shared__ int ssum[2][256];
+    const int tid = threadIdx.x;
+    constexpr int per = kCollectTile / 256;
+    long long base = (long long)blockIdx.x * kCollectTile + (long long)tid * per;
+    uint32_t smask = 0, hmask = 0;
+    for (int j = 0; j < per; j++) {
+        long long i = base + j;
+        if (i < n && alive[i]) {
+            if (kind[i] == kHole)
+                hmask |= 1u << j;
+            else
+                smask |= 1u << j;
+        }
+    }
+    int cs = __popc(smask), ch = __popc(hmask);
+    ssum[0][tid] = cs;
+    ssum[1][tid] = ch;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        int a0 = tid >= d ? ssum[0][tid - d] : 0, a1 = tid >= d ? ssum[1][tid - d] : 0;
+        __syncthreads();
+        ssum[0][tid] += a0;
+        ssum[1][tid] += a1;
+        __syncthreads();
+    }
+    int os = starOff[blockIdx.x] + ssum[0][tid] - cs;
+    int oh = holeOff[blockIdx.x] + ssum[1][tid] - ch;
+    for (int j = 0; j < per; j++) {
+        if (smask >> j & 1) order[os++] = (int32_t)(base + j);
+        if (hmask >> j & 1) holes[oh++] = (int32_t)(base + j);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// keys: build.go:19-31
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double box_side(const Meta* meta) {
+    double side = fmax(meta->box[3] - meta->box[0], fmax(meta->box[4] - meta->box[1], meta->box[5] - meta->box[2]));
+    return fmax(side * (1 + 1e-9), 1e-12);
+}
+
+__device__ __forceinline__ uint64_t cell_of(double v, double lo, double scale) {
+    double t = fmax(0.0, (v - lo) * scale);
+    return (uint64_t)fmin(2097151.0, t);
+}
+
+__global__ __launch_bounds__(256) void k_keys(const double* __restrict__ x, const double* __restrict__ y,
+                                              const double* __restrict__ z, const int32_t* __restrict__ order,
+                                              const Meta* __restrict__ meta, uint64_t* __restrict__ keys) {
+    long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= meta->nstars) return;
+    double side = box_side(meta);
+    double scale = (double)(1 << kMortonBits) / side;
+    int32_t i = order[k];
+    uint64_t cx = cell_of(x[i], meta->box[0], scale);
+    uint64_t cy = cell_of(y[i], meta->box[1], scale);
+    uint64_t cz = cell_of(z[i], meta->box[2], scale);
+    keys[k] = spread21(cx) << 2 | spread21(cy) << 1 | spread21(cz);
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (key, order) pairs, 8-bit digits.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, const Meta* __restrict__ meta,
+                                                   int shift, uint32_t* __restrict__ hist, int ntiles) {
+    __shared__ uint32_t h[256];
+    const int tid = threadIdx.x;
+    h[tid] = 0;
+    __syncthreads();
+    long long nst = meta->nstars;
+    long long base = (long long)blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortTile / 256; r++) {
+        long long i = base + r * 256 + tid;
+        if (i < nst) atomicAdd(&h[(keys[i] >> shift) & 255], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)tid * ntiles + blockIdx.x] = h[tid];
+}
+
+// One workgroup per digit: exclusive scan over the tiles, total per digit.
+__global__ __launch_bounds__(256) void k_sort_scan(uint32_t* __restrict__ hist, int ntiles,
+                                                   uint32_t* __restrict__ digitTotal) {
+    __shared__ uint32_t s[256];
+    const int tid = threadIdx.x;
+    uint32_t* row = hist + (size_t)blockIdx.x * ntiles;
+    const int per = (ntiles + 255) / 256;
+    int lo = tid * per, hi = min(ntiles, lo + per);
+    uint32_t sum = 0;
+    for (int i = lo; i < hi; i++) sum += row[i];
+    s[tid] = sum;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        uint32_t a = tid >= d ? s[tid - d] : 0;
+        __syncthreads();
+        s[tid] += a;
+        __syncthreads();
+    }
+    uint32_t run = s[tid] - sum;
+    for (int i = lo; i < hi; i++) {
+        uint32_t c = row[i];
+        row[i] = run;
+        run += c;
+    }
+    if (tid == 255) digitTotal[blockIdx.x] = s[255];
+}
+
+__global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict__ kin,
+                                                      const int32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+                                                      int32_t* __restrict__ vout, const Meta* __restrict__ meta,
+                                                      int shift, const uint32_t* __restrict__ hist,
+                                                      const uint32_t* __restrict__ digitTotal, int ntiles) {
+    __shared__ uint32_t tileBase[256];
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t waveCnt[4][256];
+    __shared__ uint32_t waveOff[2][4][256];
+    __shared__ uint32_t scan[256];
+    const int tid = threadIdx.x, w = tid >> 6;
+    const long long nst = meta->nstars;
+    // digit bases: exclusive scan of the digit totals
+    uint32_t tot = digitTotal[tid];
+    scan[tid] = tot;
+    for (int ww = 0; ww < 4; ww++) waveCnt[ww][tid] = 0;
+    running[tid] = 0;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        uint32_t a = tid >= d ? scan[tid - d] : 0;
+        __syncthreads();
+        scan[tid] += a;
+        __syncthreads();
+    }
+    tileBase[tid] = scan[tid] - tot + hist[(size_t)tid * ntiles + blockIdx.x];
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortTile / 256; r++) {
+        long long idx = base + r * 256 + tid;
+        bool valid = idx < nst;
+        uint64_t key = valid ? kin[idx] : 0;
+        int32_t val = valid ? vin[idx] : 0;
+        uint32_t dig = (uint32_t)(key >> shift) & 255u;
+        uint64_t match = __ballot(valid);
+        for (int b = 0; b < 8; b++) {
+            bool bit = (dig >> b) & 1u;
+            uint64_t bal = __ballot(bit);
+            match &= bit ? bal : ~bal;
+        }
+        int rank = mbcnt(match);
+        if (valid && rank == 0) waveCnt[w][dig] = (uint32_t)__popcll(match);
+        __syncthreads();
+        {
+            uint32_t run = running[tid];
+            for (int ww = 0; ww < 4; ww++) {
+                uint32_t c = waveCnt[ww][tid];
+                waveOff[r & 1][ww][tid] = run;
+                run += c;
+                waveCnt[ww][tid] = 0;
+            }
+            running[tid] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t dst = tileBase[dig] + waveOff[r & 1][w][dig] + (uint32_t)rank;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// sorted copies: build.go:34-41 with SourceMass (sim.go:93-98)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gather(const double* __restrict__ x, const double* __restrict__ y,
+                                                const double* __restrict__ z, const double* __restrict__ m,
+                                                const uint8_t* __restrict__ kind, const int32_t* __restrict__ order,
+                                                const Meta* __restrict__ meta, int dmOff,
+                                                double4* __restrict__ spos) {
+    long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= meta->nstars) return;
+    int32_t i = order[k];
+    double sm = (dmOff && kind[i] == kDark) ? 0.0 : m[i];
+    spos[k] = make_double4(x[i], y[i], z[i], sm);
+}
+
+// ---------------------------------------------------------------------------
+// topology
+// ---------------------------------------------------------------------------
+__global__ void k_tree_init(Meta* meta, TravNode* __restrict__ trav) {
+    if (threadIdx.x != 0) return;
+    for (int l = 0; l <= kMaxLevels; l++) {
+        meta->levelStart[l] = 0;
+        meta->levelCount[l] = 0;
+    }
+    meta->pp = 0;
+    meta->pc = 0;
+    meta->flags = 0;
+    meta->maxStack = 0;
+    meta->nnodes = 0;
+    long long nst = meta->nstars;
+    if (nst <= 0) return;
+    double side = box_side(meta);
+    double half = side / 2;
+    TravNode t;
+    t.x = meta->box[0] + half;
+    t.y = meta->box[1] + half;
+    t.z = meta->box[2] + half;
+    t.crit2 = half;
+    t.first = -1;
+    t.next = -1;
+    t.lo = 0;
+    t.hi = (int32_t)nst;
+    t.nchild = 0;
+    t.level = 0;
+    t.pad0 = t.pad1 = 0;
+    trav[0] = t;
+    meta->levelCount[0] = 1;
+}
+
+// Octant boundaries of the sorted range [lo, hi) at digit shift: b[o] is the
+// first index whose digit is >= o (lowerBound, build.go:340-350).  Result in
+// lane o (o = 1..7) of the returned value.  Whole wavefront cooperates.
+__device__ int octant_bounds(const uint64_t* __restrict__ keys, int lo, int hi, int shift) {
+    const int lane = lane_id();
+    const int R = hi - lo;
+    int result = 0;
+    if (R <= 64) {
+        int d = 8;
+        if (lane < R) d = (int)((keys[lo + lane] >> shift) & 7);
+        for (int o = 1; o < 8; o++) {
+            int b = lo + __popcll(__ballot(d < o));
+            if (lane == o) result = b;
+        }
+        return result;
+    }
+    // 7 groups of 9 lanes; group g searches octant o = g + 1 with a 10-ary search.
+    const int g = lane / 9, s = lane % 9, o = g + 1;
+    int a = lo, b = hi;  // answer in [a, b]
+    bool active = lane < 63;
+    for (;;) {
+        bool open = active && a < b;
+        if (__ballot(open) == 0) break;
+        long long span = b - a;
+        int q = a + (int)((span * (s + 1)) / 10);
+        bool pred = open && (int)((keys[q] >> shift) & 7) >= o;
+        uint64_t bal = __ballot(pred);
+        if (open) {
+            uint32_t gb = (uint32_t)(bal >> (g * 9)) & 0x1ffu;
+            int na, nb;
+            if (gb) {
+                int ss = __builtin_ctz(gb);
+                nb = a + (int)((span * (ss + 1)) / 10);
+                na = ss > 0 ? a + (int)((span * ss) / 10) + 1 : a;
+            } else {
+                na = a + (int)((span * 9) / 10) + 1;
+                nb = b;
+            }
+            a = na;
+            b = nb;
+        }
+    }
+    // lane o needs the answer of group o-1 (held by its lane 9*(o-1))
+    int src = lane >= 1 && lane <= 7 ? 9 * (lane - 1) : 0;
+    int v = __shfl(a, src, kWave);
+    return lane >= 1 && lane <= 7 ? v : 0;
+}
+
+__global__ __launch_bounds__(256) void k_topo_level(int L, const uint64_t* __restrict__ keys,
+                                                    TravNode* __restrict__ trav, Meta* meta, long long cap) {
+    const long long start = meta->levelStart[L];
+    const long long cnt = meta->levelCount[L];
+    const long long nextBase = start + cnt;
+    if (blockIdx.x == 0 && threadIdx.x == 0) meta->levelStart[L + 1] = nextBase;
+    if (L >= kMortonBits || (meta->flags & kFlagNodeOverflow)) return;
+    const int lane = lane_id();
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    const int shift = 3 * (kMortonBits - 1 - L);
+    for (long long wv = wave; wv < cnt; wv += nwaves) {
+        const long long ni = start + wv;
+        const int lo = trav[ni].lo, hi = trav[ni].hi;
+        if (hi - lo <= kLeafSize) continue;  // leaf: first = -1 already
+        int b = octant_bounds(keys, lo, hi, shift);
+        // bounds[0] = lo, bounds[8] = hi
+        int bo = lane == 0 ? lo : (lane == 8 ? hi : b);
+        int bnext = __shfl_down(bo, 1, kWave);
+        bool nonempty = lane < 8 && bnext > bo;
+        uint64_t ne = __ballot(nonempty);
+        int nchild = __popcll(ne);
+        long long off = 0;
+        if (lane == 0) off = (long long)atomicAdd((unsigned long long*)&meta->levelCount[L + 1], (unsigned long long)nchild);
+        off = __shfl(off, 0, kWave);
+        long long firstc = nextBase + off;
+        if (firstc + nchild > cap) {
+            if (lane == 0) atomicOr(&meta->flags, (int)kFlagNodeOverflow);
+            continue;
+        }
+        const double cx = trav[ni].x, cy = trav[ni].y, cz = trav[ni].z, half = trav[ni].crit2;
+        const int pnext = trav[ni].next;
+        if (nonempty) {
+            int rank = __popcll(ne & ((1ull << lane) - 1));
+            long long c = firstc + rank;
+            double q = half / 2;
+            int oct = lane;
+            TravNode t;
+            t.x = cx + (((oct >> 2) & 1) ? 1.0 : -1.0) * q;
+            t.y = cy + (((oct >> 1) & 1) ? 1.0 : -1.0) * q;
+            t.z = cz + ((oct & 1) ? 1.0 : -1.0) * q;
+            t.crit2 = q;
+            t.first = -1;
+            t.next = rank < nchild - 1 ? (int32_t)(c + 1) : pnext;
+            t.lo = bo;
+            t.hi = bnext;
+            t.nchild = 0;
+            t.level = L + 1;
+            t.pad0 = t.pad1 = 0;
+            trav[c] = t;
+        }
+        if (lane == 0) {
+            trav[ni].first = (int32_t)firstc;
+            trav[ni].nchild = nchild;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// moments: finish, build.go:176-217
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void quad_add(double q[6], double m, double dx, double dy, double dz) {
+    double r2 = dx * dx + dy * dy + dz * dz;
+    q[0] += m * (3 * dx * dx - r2);
+    q[1] += m * (3 * dy * dy - r2);
+    q[2] += m * (3 * dz * dz - r2);
+    q[3] += m * 3 * dx * dy;
+    q[4] += m * 3 * dx * dz;
+    q[5] += m * 3 * dy * dz;
+}
+
+__global__ __launch_bounds__(256) void k_moments_level(int L, const double4* __restrict__ spos,
+                                                       TravNode* __restrict__ trav, Mult64* __restrict__ m64,
+                                                       Mult32* __restrict__ m32, const Meta* __restrict__ meta,
+                                                       double theta) {
+    const long long start = meta->levelStart[L];
+    const long long cnt = meta->levelCount[L];
+    if (meta->flags & kFlagNodeOverflow) return;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long ni = start + t;
+        TravNode nd = trav[ni];
+        const double cx = nd.x, cy = nd.y, cz = nd.z, half = nd.crit2;
+        double x = 0, y = 0, z = 0, m = 0;
+        if (nd.first < 0) {
+            for (int j = nd.lo; j < nd.hi; j++) {
+                double4 p = spos[j];
+                double mj = p.w;
+                x = x + mj * p.x;
+                y = y + mj * p.y;
+                z = z + mj * p.z;
+                m = m + mj;
+            }
+        } else {
+            for (int c = nd.first; c < nd.first + nd.nchild; c++) {
+                double cm = m64[c].m;
+                x = x + cm * trav[c].x;
+                y = y + cm * trav[c].y;
+                z = z + cm * trav[c].z;
+                m = m + cm;
+            }
+        }
+        if (m > 0) {
+            x = x / m;
+            y = y / m;
+            z = z / m;
+        } else {
+            x = cx;
+            y = cy;
+            z = cz;
+        }
+        double q[6] = {0, 0, 0, 0, 0, 0};
+        if (nd.first < 0) {
+            for (int j = nd.lo; j < nd.hi; j++) {
+                double4 p = spos[j];
+                quad_add(q, p.w, p.x - x, p.y - y, p.z - z);
+            }
+        } else {
+            for (int c = nd.first; c < nd.first + nd.nchild; c++) {
+                const Mult64 cm = m64[c];
+                q[0] = q[0] + cm.qxx;
+                q[1] = q[1] + cm.qyy;
+                q[2] = q[2] + cm.qzz;
+                q[3] = q[3] + cm.qxy;
+                q[4] = q[4] + cm.qxz;
+                q[5] = q[5] + cm.qyz;
+                quad_add(q, cm.m, trav[c].x - x, trav[c].y - y, trav[c].z - z);
+            }
+        }
+        double delta = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
+        double crit = 2 * half / theta + delta;
+        nd.x = x;
+        nd.y = y;
+        nd.z = z;
+        nd.crit2 = crit * crit;
+        trav[ni] = nd;
+        Mult64 mm;
+        mm.m = m;
+        mm.qxx = q[0]; mm.qyy = q[1]; mm.qzz = q[2];
+        mm.qxy = q[3]; mm.qxz = q[4]; mm.qyz = q[5];
+        mm.pad = 0;
+        m64[ni] = mm;
+        Mult32 mf;
+        mf.m = (float)m;
+        mf.qxx = (float)q[0]; mf.qyy = (float)q[1]; mf.qzz = (float)q[2];
+        mf.qxy = (float)q[3]; mf.qxz = (float)q[4]; mf.qyz = (float)q[5];
+        mf.pad = 0;
+        m32[ni] = mf;
+    }
+}
+
+__global__ void k_tree_finish(Meta* meta) {
+    if (threadIdx.x != 0) return;
+    long long total = 0;
+    for (int l = 0; l <= kMaxLevels; l++) total += meta->levelCount[l];
+    meta->nnodes = total;
+}
+
+__global__ void k_zero_holes(const int32_t* __restrict__ holes, const Meta* __restrict__ meta, double* ax,
+                             double* ay, double* az) {
+    long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; h < meta->nholes; h += (long long)gridDim.x * blockDim.x) {
+        int32_t i = holes[h];
+        ax[i] = 0;
+        ay[i] = 0;
+        az[i] = 0;
+    }
+}
+
+}  // namespace gx

@@ -1,0 +1,111 @@
+// gx_common.h -- device records and wavefront helpers shared by the gfx950
+// Barnes-Hut kernels.  All kernels are written for CDNA4 wave64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gx {
+
+constexpr int kMortonBits = 21;   // internal/sim/bh/morton.go:4
+constexpr int kLeafSize = 24;     // internal/sim/bh/bh.go:19
+constexpr int kMaxLevels = 22;    // levels 0..21
+constexpr int kWave = 64;
+
+enum Kind : uint8_t { kStar = 0, kHole = 1, kDark = 2, kGas = 3 };
+
+// Per-node traversal record (64 B, one cache line).  During the topology
+// build x,y,z hold the cell centre and crit2 holds the half side; the moment
+// pass replaces them with the centre of mass and the squared opening radius.
+struct alignas(64) TravNode {
+    double x, y, z, crit2;
+    int32_t first;   // first child, -1 for leaves
+    int32_t next;    // skip link: next sibling or the parent's next, -1 at the end
+    int32_t lo, hi;  // star range in sorted order
+    int32_t nchild;  // children first .. first+nchild-1 (octant order)
+    int32_t level;
+    int32_t pad0, pad1;
+};
+
+// FP64 multipole (m and the traceless quadrupole about the centre of mass).
+struct alignas(64) Mult64 {
+    double m, qxx, qyy, qzz, qxy, qxz, qyz, pad;
+};
+
+// FP32 multipole for the FP32 walk.
+struct alignas(32) Mult32 {
+    float m, qxx, qyy, qzz, qxy, qxz, qyz, pad;
+};
+
+// Device-side metadata of one build.
+struct Meta {
+    double box[6];          // minX, minY, minZ, maxX, maxY, maxZ of the tree stars
+    long long nstars;
+    long long nholes;
+    long long nnodes;
+    long long levelStart[kMaxLevels + 1];
+    long long levelCount[kMaxLevels + 1];
+    unsigned long long pp;  // particle-particle interactions (incl. self pairs)
+    unsigned long long pc;  // particle-cell interactions
+    int flags;              // bit 0: node capacity overflow, bit 1: walk stack overflow
+    int maxStack;
+};
+
+enum Flag : int { kFlagNodeOverflow = 1, kFlagStackOverflow = 2 };
+
+// ---------------------------------------------------------------------------
+// wavefront helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ int uniform32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, kWave); }
+
+__device__ __forceinline__ double wave_min_d(double v) {
+    for (int m = 32; m >= 1; m >>= 1) v = fmin(v, shfl_xor_d(v, m));
+    return v;
+}
+__device__ __forceinline__ double wave_max_d(double v) {
+    for (int m = 32; m >= 1; m >>= 1) v = fmax(v, shfl_xor_d(v, m));
+    return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+// Inclusive prefix sum across the wave (Hillis-Steele with shuffles).
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    int lane = lane_id();
+    for (int d = 1; d < kWave; d <<= 1) {
+        int t = __shfl_up(v, d, kWave);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Morton interleave: morton.go:7-21.
+__device__ __forceinline__ uint64_t spread21(uint64_t v) {
+    v &= (1ull << kMortonBits) - 1;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+}  // namespace gx

@@ -1,0 +1,195 @@
+// gx_walk64.hip -- FP64 kernels: the reference's per-particle walk, the tree
+// potential walk and the direct-sum oracle kernel.
+//
+// Compiled with -ffp-contract=off; the walk repeats internal/sim/bh/bh.go:106-155
+// operation for operation, so its results are bit-identical to the reference's
+// CPU walk on the same tree.  It is the "FP64 GPU walk" baseline: one thread per
+// star in Morton order, stackless skip-pointer traversal.
+#include "gx_common.h"
+#include "gx_kernels.h"
+
+namespace gx {
+
+__global__ __launch_bounds__(256) void k_walk64(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
+                                                const Mult64* __restrict__ m64, const int32_t* __restrict__ order,
+                                                Meta* meta, double eps2, int quad, double* __restrict__ ax,
+                                                double* __restrict__ ay, double* __restrict__ az) {
+    const long long nst = meta->nstars;
+    if (meta->flags & kFlagNodeOverflow) return;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long npp = 0, npc = 0;
+    if (k < nst) {
+        const double4 p = spos[k];
+        const double x = p.x, y = p.y, z = p.z;
+        double sx = 0, sy = 0, sz = 0;
+        for (int32_t i = 0; i >= 0;) {
+            const TravNode nd = trav[i];
+            double dx = nd.x - x, dy = nd.y - y, dz = nd.z - z;
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 >= nd.crit2) {
+                const Mult64 mm = m64[i];
+                double inv = 1 / (d2 + eps2);
+                double inv3 = inv * sqrt(inv);
+                double f = mm.m * inv3;
+                if (quad) {
+                    double qx = mm.qxx * dx + mm.qxy * dy + mm.qxz * dz;
+                    double qy = mm.qxy * dx + mm.qyy * dy + mm.qyz * dz;
+                    double qz = mm.qxz * dx + mm.qyz * dy + mm.qzz * dz;
+                    double inv5 = inv3 * inv;
+                    f += 2.5 * (dx * qx + dy * qy + dz * qz) * inv5 * inv;
+                    sx -= qx * inv5;
+                    sy -= qy * inv5;
+                    sz -= qz * inv5;
+                }
+                sx += f * dx;
+                sy += f * dy;
+                sz += f * dz;
+                npc++;
+                i = nd.next;
+                continue;
+            }
+            if (nd.first >= 0) {
+                i = nd.first;
+                continue;
+            }
+            for (int32_t j = nd.lo; j < nd.hi; j++) {
+                if (j == (int32_t)k) continue;
+                const double4 q = spos[j];
+                double ddx = q.x - x, ddy = q.y - y, ddz = q.z - z;
+                double r2 = ddx * ddx + ddy * ddy + ddz * ddz + eps2;
+                double f = q.w / (r2 * sqrt(r2));
+                sx += f * ddx;
+                sy += f * ddy;
+                sz += f * ddz;
+            }
+            npp += (unsigned long long)(nd.hi - nd.lo);
+            i = nd.next;
+        }
+        const int32_t s = order[k];
+        ax[s] = sx;
+        ay[s] = sy;
+        az[s] = sz;
+    }
+    npp = wave_sum_u64(npp);
+    npc = wave_sum_u64(npc);
+    if (lane_id() == 0 && (npp | npc)) {
+        atomicAdd(&meta->pp, npp);
+        atomicAdd(&meta->pc, npc);
+    }
+}
+
+// Tree potential: walkPotential, internal/sim/bh/potential.go:45-81.  Writes
+// sm_k * psi_k per star into partial[k] (sorted order); the host-side library
+// sums them in a fixed order.
+__global__ __launch_bounds__(256) void k_potential64(const double4* __restrict__ spos,
+                                                     const TravNode* __restrict__ trav,
+                                                     const Mult64* __restrict__ m64, Meta* meta, double eps2,
+                                                     int quad, double* __restrict__ partial) {
+    const long long nst = meta->nstars;
+    if (meta->flags & kFlagNodeOverflow) return;
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nst) return;
+    const double4 p = spos[k];
+    const double x = p.x, y = p.y, z = p.z;
+    double psi = 0;
+    for (int32_t i = 0; i >= 0;) {
+        const TravNode nd = trav[i];
+        double dx = nd.x - x, dy = nd.y - y, dz = nd.z - z;
+        double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 >= nd.crit2) {
+            const Mult64 mm = m64[i];
+            double inv = 1 / (d2 + eps2);
+            psi += mm.m * sqrt(inv);
+            if (quad) {
+                double dqd = mm.qxx * dx * dx + mm.qyy * dy * dy + mm.qzz * dz * dz +
+                             2 * (mm.qxy * dx * dy + mm.qxz * dx * dz + mm.qyz * dy * dz);
+                psi += 0.5 * dqd * inv * inv * sqrt(inv);
+            }
+            i = nd.next;
+            continue;
+        }
+        if (nd.first >= 0) {
+            i = nd.first;
+            continue;
+        }
+        for (int32_t j = nd.lo; j < nd.hi; j++) {
+            if (j == (int32_t)k) continue;
+            const double4 q = spos[j];
+            double ddx = q.x - x, ddy = q.y - y, ddz = q.z - z;
+            psi += q.w / sqrt(ddx * ddx + ddy * ddy + ddz * ddz + eps2);
+        }
+        i = nd.next;
+    }
+    partial[k] = p.w * psi;
+}
+
+// FP64 direct sum for a sample of targets (direct.go:25-48): every workgroup
+// owns 256 targets and one slice of the sources; slices are summed in a fixed
+// order by k_direct_reduce, so the result is deterministic.
+constexpr int kDirTile = 256;
+
+__global__ __launch_bounds__(256) void k_direct_sample(const double* __restrict__ x, const double* __restrict__ y,
+                                                       const double* __restrict__ z, const double* __restrict__ m,
+                                                       const uint8_t* __restrict__ kind, long long n,
+                                                       const long long* __restrict__ targets, int nt, double eps2,
+                                                       int dmOff, int splits, double* __restrict__ partial) {
+    __shared__ double4 src[kDirTile];
+    const int tid = threadIdx.x;
+    const int t = blockIdx.x * 256 + tid;
+    const int split = blockIdx.y;
+    const long long per = (n + splits - 1) / splits;
+    const long long jlo = (long long)split * per;
+    const long long jhi = min(n, jlo + per);
+    long long ti = t < nt ? targets[t] : -1;
+    if (ti >= n) ti = -1;  // out-of-range targets give zeros instead of a fault
+    double xi = 0, yi = 0, zi = 0;
+    bool hole = true;
+    if (ti >= 0) {
+        xi = x[ti];
+        yi = y[ti];
+        zi = z[ti];
+        hole = kind[ti] == kHole;
+    }
+    double sx = 0, sy = 0, sz = 0;
+    for (long long j0 = jlo; j0 < jhi; j0 += kDirTile) {
+        long long j = j0 + tid;
+        double4 s = make_double4(0, 0, 0, 0);
+        if (j < jhi) {
+            uint8_t kj = kind[j];
+            bool skip = kj == kHole || (dmOff && kj == kDark);
+            s = make_double4(x[j], y[j], z[j], skip ? -1.0 : m[j]);
+        }
+        __syncthreads();
+        src[tid] = s;
+        __syncthreads();
+        int cnt = (int)min((long long)kDirTile, jhi - j0);
+        for (int u = 0; u < cnt; u++) {
+            double4 q = src[u];
+            if (q.w < 0 || j0 + u == ti) continue;
+            double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+            double re2 = dx * dx + dy * dy + dz * dz + eps2;
+            double f = q.w / (re2 * sqrt(re2));
+            sx += f * dx;
+            sy += f * dy;
+            sz += f * dz;
+        }
+    }
+    if (t < nt) {
+        if (hole) sx = sy = sz = 0;
+        size_t o = ((size_t)split * nt + t) * 3;
+        partial[o] = sx;
+        partial[o + 1] = sy;
+        partial[o + 2] = sz;
+    }
+}
+
+__global__ void k_direct_reduce(const double* __restrict__ partial, int nt, int splits, double* __restrict__ out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double a[3] = {0, 0, 0};
+    for (int s = 0; s < splits; s++)
+        for (int c = 0; c < 3; c++) a[c] += partial[((size_t)s * nt + t) * 3 + c];
+    for (int c = 0; c < 3; c++) out[(size_t)t * 3 + c] = a[c];
+}
+
+}  // namespace gx

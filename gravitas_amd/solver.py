@@ -1,0 +1,222 @@
+"""Host-side mirror of the reference's force-provider interface.
+
+``Solver`` mirrors ``bh.Solver`` (internal/sim/bh/bh.go:29-38) with its
+``Accelerations`` (bh.go:76-102) and ``PotentialEnergy`` (potential.go:12-42);
+``gpu_available`` / ``gpu_name`` / ``last_gpu_seconds`` mirror
+``GPUAvailable`` / ``GPUName`` / ``LastGPUSeconds`` (gpu_darwin.go:132-155).
+The computation runs in libgravitas_hip.so on the GPU; states live in HBM as
+hipMalloc'd arrays (gravitas_amd.hip.DeviceArray).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .hip import DeviceArray
+
+KIND_STAR, KIND_HOLE, KIND_DARK, KIND_GAS = 0, 1, 2, 3
+DEFAULT_THETA = 0.9
+
+
+@dataclass
+class Params:
+    """The subset of sim.Params (internal/sim/sim.go:9-23) the solver reads."""
+    softening: float = 0.0
+    dark_matter_off: bool = False
+    dt: float = 0.0
+
+
+_avail = None
+
+
+def gpu_available():
+    """None if the GPU backend can be used, else the reason (GPUAvailable)."""
+    global _avail
+    if _avail is None:
+        try:
+            L = _lib.load()
+        except (RuntimeError, OSError) as e:
+            _avail = str(e)
+            return _avail
+        buf = ctypes.create_string_buffer(512)
+        rc = L.gx_init(buf, 512)
+        _avail = None if rc == _lib.GX_OK else f"hip: {buf.value.decode()} (code {rc})"
+    return _avail
+
+
+def gpu_name() -> str:
+    if gpu_available() is not None:
+        return ""
+    buf = ctypes.create_string_buffer(256)
+    _lib.load().gx_name(buf, 256)
+    return buf.value.decode()
+
+
+class DeviceState:
+    """Struct-of-arrays state resident in HBM (sim.State, internal/sim/state.go:33-52):
+    float64 px, py, pz, vx, vy, vz, ax, ay, az, m; uint8 kind, alive."""
+
+    FIELDS = ("px", "py", "pz", "vx", "vy", "vz", "ax", "ay", "az", "m")
+
+    def __init__(self, **arrays):
+        for k, v in arrays.items():
+            setattr(self, k, v)
+
+    @classmethod
+    def from_host(cls, s: dict):
+        n = len(s["px"])
+        out = {}
+        for f in cls.FIELDS:
+            a = s.get(f)
+            out[f] = DeviceArray.from_numpy(np.zeros(n) if a is None else np.asarray(a, dtype=np.float64))
+        kind, alive = s.get("kind"), s.get("alive")
+        out["kind"] = DeviceArray.from_numpy(np.zeros(n, np.uint8) if kind is None else np.asarray(kind, np.uint8))
+        out["alive"] = DeviceArray.from_numpy(np.ones(n, np.uint8) if alive is None else np.asarray(alive, np.uint8))
+        return cls(**out)
+
+    def __len__(self):
+        return len(self.px)
+
+    def acc_host(self):
+        return np.stack([self.ax.numpy(), self.ay.numpy(), self.az.numpy()], axis=1)
+
+
+def _ptr(t):
+    return t.ptr
+
+
+class Solver:
+    """bh.Solver on the GPU.
+
+    theta: opening angle, 0 means DefaultTheta (bh.go:69-74).
+    monopole_only: drop the quadrupole term (bh.go:33-35).
+    walk: "fp32" (group walk, the default) or "fp64" (per-particle walk,
+          bit-identical to the reference's CPU walk).
+    """
+
+    def __init__(self, theta: float = 0.0, monopole_only: bool = False, walk: str = "fp32", timing: bool = False):
+        reason = gpu_available()
+        if reason is not None:
+            raise RuntimeError(f"GPU backend unavailable: {reason}")
+        self._L = _lib.load()
+        h = self._L.gx_solver_new()
+        if not h:
+            raise _lib.GxError(_lib.GX_ERR_HIP, self._L.gx_error().decode())
+        self._h = ctypes.c_void_p(h)
+        self.theta = theta
+        self.monopole_only = monopole_only
+        self.walk = walk
+        self.timing = timing
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                self._L.gx_solver_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def _configure(self):
+        w = {"fp32": _lib.GX_WALK_FP32, "fp64": _lib.GX_WALK_FP64}[self.walk]
+        _lib.check(self._L.gx_solver_config(self._h, float(self.theta), int(self.monopole_only), w,
+                                            int(self.timing)))
+
+    def set_stream(self, stream):
+        """Launch on the given hipStream_t (a pointer-sized int or None)."""
+        _lib.check(self._L.gx_solver_set_stream(self._h, ctypes.c_void_p(stream) if stream else None))
+
+    # -- Accelerations (bh.go:76-102) ------------------------------------------------
+    def accelerations(self, s: DeviceState, p: Params):
+        """Fill s.ax/ay/az (device tensors) from star-star gravity."""
+        self._configure()
+        n = len(s)
+        _lib.check(self._L.gx_accelerations_dev(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.m), _ptr(s.kind), _ptr(s.alive), n,
+            float(p.softening), int(p.dark_matter_off), _ptr(s.ax), _ptr(s.ay), _ptr(s.az)))
+
+    def enqueue(self, s: DeviceState, p: Params):
+        """Asynchronous Accelerations; call sync() before reading results."""
+        self._configure()
+        _lib.check(self._L.gx_enqueue_accelerations_dev(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.m), _ptr(s.kind), _ptr(s.alive), len(s),
+            float(p.softening), int(p.dark_matter_off), _ptr(s.ax), _ptr(s.ay), _ptr(s.az)))
+
+    def sync(self):
+        _lib.check(self._L.gx_sync(self._h))
+
+    def accelerations_host(self, x, y, z, m, kind=None, alive=None, eps=0.0, dm_off=False, out=None):
+        """Accelerations on host (numpy) arrays; the data crosses PCIe."""
+        self._configure()
+        n = len(x)
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        x, y, z, m = f64(x), f64(y), f64(z), f64(m)
+        kind = np.zeros(n, np.uint8) if kind is None else np.ascontiguousarray(kind, np.uint8)
+        alive = np.ones(n, np.uint8) if alive is None else np.ascontiguousarray(alive, np.uint8)
+        ax, ay, az = (np.zeros(n), np.zeros(n), np.zeros(n)) if out is None else out
+        c = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        _lib.check(self._L.gx_accelerations(self._h, c(x), c(y), c(z), c(m), c(kind), c(alive), n, float(eps),
+                                            int(dm_off), c(ax), c(ay), c(az)))
+        return np.stack([ax, ay, az], axis=1)
+
+    # -- PotentialEnergy (potential.go:12-42) ----------------------------------------
+    def potential_energy(self, s: DeviceState, p: Params) -> float:
+        self._configure()
+        out = ctypes.c_double()
+        _lib.check(self._L.gx_potential_energy_dev(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.m), _ptr(s.kind), _ptr(s.alive), len(s),
+            float(p.softening), int(p.dark_matter_off), ctypes.byref(out)))
+        return out.value
+
+    # -- FP64 direct-sum oracle kernel (direct.go:25-48) ------------------------------
+    def direct_sample(self, s: DeviceState, p: Params, targets):
+        """FP64 direct-sum accelerations of the given target indices (host array
+        or int64 DeviceArray); returns an (nt, 3) float64 DeviceArray."""
+        if not isinstance(targets, DeviceArray):
+            t = np.asarray(targets, dtype=np.int64)
+            if len(t) and (t.min() < 0 or t.max() >= len(s)):
+                raise IndexError("target index out of range")
+            targets = DeviceArray.from_numpy(t)
+        nt = len(targets)
+        out = DeviceArray((nt, 3), np.float64)
+        _lib.check(self._L.gx_direct_sample_dev(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.m), _ptr(s.kind), len(s), _ptr(targets), nt,
+            float(p.softening), int(p.dark_matter_off), _ptr(out)))
+        return out
+
+    # -- diagnostics -------------------------------------------------------------------
+    def last_gpu_seconds(self) -> float:
+        """LastGPUSeconds (gpu_darwin.go:155)."""
+        return float(self._L.gx_last_seconds(self._h))
+
+    def stats(self) -> dict:
+        st = _lib.GxStats()
+        _lib.check(self._L.gx_get_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    _EXPORT = {"keys": (0, np.uint64, ()), "order": (1, np.int32, ()), "spos": (2, np.float64, (4,)),
+               "nodes": (3, None, ()), "mult": (4, np.float64, (8,))}
+    NODE_DTYPE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("crit2", "f8"), ("first", "i4"),
+                           ("next", "i4"), ("lo", "i4"), ("hi", "i4"), ("nchild", "i4"), ("level", "i4"),
+                           ("pad0", "i4"), ("pad1", "i4")])
+
+    def export(self, what: str) -> np.ndarray:
+        """Copy part of the last build to host (test hook): keys, order, spos,
+        nodes, mult."""
+        code, dt, shape = self._EXPORT[what]
+        st = self.stats()
+        count = st["n_nodes"] if what in ("nodes", "mult") else st["n_stars"]
+        dt = self.NODE_DTYPE if dt is None else np.dtype(dt)
+        buf = np.zeros((max(count, 0),) + shape, dtype=dt)
+        k = self._L.gx_export(self._h, code, buf.ctypes.data_as(ctypes.c_void_p), count)
+        if k < 0:
+            raise _lib.GxError(_lib.GX_ERR_ARG, self._L.gx_error().decode())
+        return buf[:k]
+
+    def phase_ms(self) -> dict:
+        buf = (ctypes.c_float * 8)()
+        k = self._L.gx_phase_ms(self._h, buf, 8)
+        names = ("collect", "keys", "sort", "gather", "topology", "moments", "walk", "total")
+        return {names[i]: float(buf[i]) for i in range(k)}

@@ -1,0 +1,143 @@
+/*
+ * gravitas_hip.h -- C ABI of the MI355X (gfx950) Barnes-Hut backend.
+ *
+ * This library replaces the GPU bridge of the gravitas reference, i.e. the C
+ * functions in the cgo preamble of internal/sim/bh/gpu_darwin.go:18-62 and
+ * the Go stub internal/sim/bh/gpu_other.go:8-18, and moves the whole force
+ * step bh.Solver.Accelerations (internal/sim/bh/bh.go:76-102) onto the GPU:
+ * collect, Morton keys, stable radix sort, sorted copies, octree topology,
+ * moments and the walk.  Plain pointers and sizes only; no C++ or torch types.
+ *
+ * Threading: one gx_solver is used by one thread at a time, as one bh.Solver
+ * is (internal/sim/runner.go:177-226).  Different solvers may be used from
+ * different threads.
+ *
+ * Error handling: functions returning int return GX_OK (0) or a GX_ERR_*
+ * code; gx_error() returns the message of the calling thread's last error.
+ */
+#ifndef GRAVITAS_HIP_H
+#define GRAVITAS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  1-3 keep the meaning of gpu_init's return values
+ * (gpu_darwin.go:18-28: 1 no device, 2 library, 3 pipeline). */
+#define GX_OK 0
+#define GX_ERR_NO_DEVICE 1
+#define GX_ERR_CODE_OBJECT 2
+#define GX_ERR_PIPELINE 3
+#define GX_ERR_ARG 4
+#define GX_ERR_HIP 5
+#define GX_ERR_CAPACITY 6
+#define GX_ERR_UNSUPPORTED 7
+
+/* Walk variants. */
+#define GX_WALK_FP32 0 /* FP32 group walk, reference interaction set (default) */
+#define GX_WALK_FP64 1 /* FP64 per-particle walk, bit-identical to the reference */
+
+/* Body kinds, internal/sim/state.go:11-20. */
+#define GX_KIND_STAR 0
+#define GX_KIND_HOLE 1
+#define GX_KIND_DARK 2
+#define GX_KIND_GAS 3
+
+/* Replaces gpu_init (gpu_darwin.go:18-28): selects and initialises the GPU,
+ * returns GX_OK or GX_ERR_NO_DEVICE / GX_ERR_CODE_OBJECT and writes a message
+ * to err.  Idempotent; the first result is kept, like GPUAvailable's
+ * sync.Once (gpu_darwin.go:132-142). */
+int gx_init(char *err, int errlen);
+
+/* Replaces gpu_name (gpu_darwin.go:31-34): device name, "" before a
+ * successful gx_init. */
+void gx_name(char *out, int len);
+
+/* Message of the calling thread's last error ("" if none). */
+const char *gx_error(void);
+
+/* A solver owns a device arena and a stream; buffers grow by 1.25x like the
+ * Metal buffers (gpu_darwin.go:164-177) and are reused between calls. */
+typedef struct gx_solver gx_solver;
+gx_solver *gx_solver_new(void);
+void gx_solver_free(gx_solver *s);
+
+/* bh.Solver fields (bh.go:29-38): theta (0 means DefaultTheta 0.9, bh.go:69-74),
+ * MonopoleOnly, and the walk variant (GX_WALK_*).  timing != 0 records
+ * per-phase GPU times (gx_phase_ms). */
+int gx_solver_config(gx_solver *s, double theta, int monopole_only, int walk, int timing);
+
+/* Uses the given hipStream_t (passed as void*) instead of the solver's own
+ * stream; NULL restores the solver's stream. */
+int gx_solver_set_stream(gx_solver *s, void *stream);
+
+/* bh.Solver.Accelerations (bh.go:76-102) on host arrays of n bodies in state
+ * order.  Star-star accelerations with Plummer softening eps; alive holes get
+ * 0; dead bodies are left untouched; with dark_matter_off, dark matter pulls
+ * nothing but is accelerated (sim.go:93-98).  Synchronous, like gpu_walk. */
+int gx_accelerations(gx_solver *s, const double *x, const double *y, const double *z, const double *mass,
+                     const uint8_t *kind, const uint8_t *alive, int64_t n, double eps, int dark_matter_off,
+                     double *ax, double *ay, double *az);
+
+/* Same on device arrays (hipMalloc'd or HBM-resident tensors).  Synchronous;
+ * grows the node arena and repeats the step if the tree did not fit. */
+int gx_accelerations_dev(gx_solver *s, const double *x, const double *y, const double *z, const double *mass,
+                         const uint8_t *kind, const uint8_t *alive, int64_t n, double eps, int dark_matter_off,
+                         double *ax, double *ay, double *az);
+
+/* Enqueues the step on the solver's stream and returns at once; gx_sync
+ * waits and reports GX_ERR_CAPACITY if the node arena was too small (the
+ * outputs of such a step are not valid; gx_accelerations_dev regrows). */
+int gx_enqueue_accelerations_dev(gx_solver *s, const double *x, const double *y, const double *z,
+                                 const double *mass, const uint8_t *kind, const uint8_t *alive, int64_t n,
+                                 double eps, int dark_matter_off, double *ax, double *ay, double *az);
+int gx_sync(gx_solver *s);
+
+/* bh.Solver.PotentialEnergy (potential.go:12-42) on device arrays: theta
+ * capped at 0.9; FP64 tree walk.  Star-star pairs only: returns
+ * GX_ERR_UNSUPPORTED if alive holes are present. */
+int gx_potential_energy_dev(gx_solver *s, const double *x, const double *y, const double *z, const double *mass,
+                            const uint8_t *kind, const uint8_t *alive, int64_t n, double eps, int dark_matter_off,
+                            double *out);
+
+/* FP64 direct sum (internal/sim/direct/direct.go:25-48) for nt target
+ * indices (device int64) against all n bodies (device arrays); out is a
+ * device array of nt*3 doubles (ax, ay, az per target).  Synchronous. */
+int gx_direct_sample_dev(gx_solver *s, const double *x, const double *y, const double *z, const double *mass,
+                         const uint8_t *kind, int64_t n, const int64_t *targets, int64_t nt, double eps,
+                         int dark_matter_off, double *out);
+
+/* LastGPUSeconds (gpu_darwin.go:155): GPU time of the last step, 0 before. */
+double gx_last_seconds(const gx_solver *s);
+
+typedef struct {
+    int64_t n_stars, n_holes, n_nodes;
+    int64_t pp_interactions; /* star-star pairs, self pairs excluded */
+    int64_t pc_interactions; /* star-cell pairs */
+    int32_t max_stack, flags, depth, walk;
+    int64_t fallbacks;       /* FP32 steps recomputed by the FP64 walk */
+} gx_stats;
+
+/* Statistics of the last completed step. */
+int gx_get_stats(const gx_solver *s, gx_stats *out);
+
+/* Per-phase GPU milliseconds of the last step when timing is on:
+ * 0 collect, 1 keys, 2 sort, 3 gather, 4 topology, 5 moments, 6 walk,
+ * 7 total.  Returns the number of phases written. */
+int gx_phase_ms(const gx_solver *s, float *ms, int n);
+
+/* Test hook with no counterpart in the reference: copies count items of
+ * the last build to host memory.  what: 0 keys (uint64, sorted), 1 order
+ * (int32, sorted position -> state index), 2 sorted stars (4 doubles x, y, z,
+ * source mass), 3 nodes (64-byte records: double x, y, z, crit2; int32 first,
+ * next, lo, hi, nchild, level, pad, pad), 4 FP64 moments (8 doubles m, qxx,
+ * qyy, qzz, qxy, qxz, qyz, pad).  Returns the number of items copied. */
+int64_t gx_export(gx_solver *s, int what, void *host, int64_t count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

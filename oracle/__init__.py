@@ -1,0 +1,290 @@
+"""CPU oracle for the Barnes-Hut force step -- TEST INFRASTRUCTURE ONLY.
+
+This package wraps ``bh_oracle.c``, a C restatement of the gravitas
+reference's ``bh.Solver`` (internal/sim/bh/*.go) and ``direct.Solver``
+(internal/sim/direct/direct.go).  Only ``tests/``, ``__graft_entry__.smoke()``
+and ``bench.py``'s ``cpu_baseline`` leg may import it, and only as the
+checker or as the timed CPU baseline -- never inside the product path.
+
+Every function here cites the reference lines it follows in ``bh_oracle.c``.
+
+Pinning (what keeps the restatement honest, see tests/test_oracle.py):
+  * Morton keys: naive bit loop (bh_test.go:16-31), closed-form cell().
+  * Radix sort: equals numpy's stable argsort of (key) (bh_test.go:33-50).
+  * Tree: brute-force recursive definition on small inputs in Python; node
+    invariants (ranges partition, children in octant order, skip links).
+  * Moments: per-node direct sums over the node's stars; trace(Q) = 0.
+  * Walk: two-star known answer (bh_test.go:137-147); theta -> 1e-9 equals
+    the direct sum to 1e-10 (bh_test.go:88-95); force = gradient of the tree
+    potential (bh_test.go:332-351); quadrupole beats monopole in the far field
+    (bh_test.go:309-330); coincident stars give no NaN (bh_test.go:149-161).
+  * Potential: tiny theta equals the exact pair sum (bh_test.go:280-292).
+  * Direct sum: numpy float64 pair sum, Kepler/momentum invariants.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+_U8 = ctypes.POINTER(ctypes.c_uint8)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+
+NODE_DTYPE = np.dtype(
+    [("x", "f8"), ("y", "f8"), ("z", "f8"), ("m", "f8"),
+     ("qxx", "f8"), ("qyy", "f8"), ("qzz", "f8"), ("qxy", "f8"), ("qxz", "f8"), ("qyz", "f8"),
+     ("crit2", "f8"), ("first", "i4"), ("next", "i4"), ("lo", "i4"), ("hi", "i4")])
+
+KIND_STAR, KIND_HOLE, KIND_DARK, KIND_GAS = 0, 1, 2, 3
+DEFAULT_THETA = 0.9
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with oracle/Makefile (gcc, -ffp-contract=off)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.or_create.restype = ctypes.c_void_p
+        L.or_destroy.argtypes = [ctypes.c_void_p]
+        L.or_build.restype = ctypes.c_int64
+        L.or_build.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _U8, _U8, ctypes.c_int64, ctypes.c_double,
+                               ctypes.c_int]
+        L.or_accelerations.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _U8, _U8, ctypes.c_int64, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _D]
+        L.or_accelerations_built.argtypes = [ctypes.c_void_p, ctypes.c_double, _D, _D, _D]
+        L.or_accelerations_sorted.argtypes = [ctypes.c_void_p, ctypes.c_double, _D, _D, _D]
+        L.or_potential_energy.restype = ctypes.c_double
+        L.or_potential_energy.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _U8, _U8, ctypes.c_int64,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double]
+        L.or_walk.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                              ctypes.c_double, _D]
+        L.or_walk_potential.restype = ctypes.c_double
+        L.or_walk_potential.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int32, ctypes.c_double]
+        L.or_direct.argtypes = [_D, _D, _D, _D, _U8, ctypes.c_int64, _I64, ctypes.c_int64, ctypes.c_double,
+                                ctypes.c_int, ctypes.c_int, _D, _D, _D]
+        L.or_potential_exact.restype = ctypes.c_double
+        L.or_potential_exact.argtypes = [_D, _D, _D, _D, _U8, ctypes.c_int64, ctypes.c_double, ctypes.c_int]
+        L.or_radix_sort.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int32),
+                                    ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int32), ctypes.c_int64]
+        L.or_morton_key.restype = ctypes.c_uint64
+        L.or_morton_key.argtypes = [ctypes.c_uint64] * 3
+        for f in ("or_n_stars", "or_n_nodes"):
+            getattr(L, f).restype = ctypes.c_int64
+            getattr(L, f).argtypes = [ctypes.c_void_p]
+        L.or_nodes.restype = ctypes.c_void_p
+        L.or_nodes.argtypes = [ctypes.c_void_p]
+        L.or_keys.restype = ctypes.c_void_p
+        L.or_keys.argtypes = [ctypes.c_void_p]
+        L.or_order.restype = ctypes.c_void_p
+        L.or_order.argtypes = [ctypes.c_void_p]
+        L.or_sorted.restype = ctypes.c_void_p
+        L.or_sorted.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_set_monopole.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_set_workers.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_node_size.restype = ctypes.c_int
+        L.or_kick.argtypes = [ctypes.c_int64, _D, _D, _D, _D, _D, _D, ctypes.c_double]
+        L.or_drift.argtypes = [ctypes.c_int64, _D, _D, _D, _D, _D, _D, ctypes.c_double]
+        assert L.or_node_size() == NODE_DTYPE.itemsize
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _u8(a):
+    return a.ctypes.data_as(_U8)
+
+
+def _state_arrays(x, y, z, m, kind=None, alive=None):
+    x, y, z, m = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, m))
+    n = len(x)
+    kind = np.zeros(n, np.uint8) if kind is None else np.ascontiguousarray(kind, dtype=np.uint8)
+    alive = np.ones(n, np.uint8) if alive is None else np.ascontiguousarray(alive, dtype=np.uint8)
+    return x, y, z, m, kind, alive
+
+
+class Solver:
+    """CPU ``bh.Solver`` restatement (internal/sim/bh/bh.go:29-38).
+
+    ``theta == 0`` means DefaultTheta = 0.9 (bh.go:69-74).  ``workers`` is the
+    thread count of the walk (ParallelFor, internal/sim/parallel.go:15-40).
+    """
+
+    def __init__(self, theta: float = 0.0, monopole_only: bool = False, workers: int = 1):
+        self.theta = theta
+        self.monopole_only = monopole_only
+        self.workers = workers
+        self._L = lib()
+        self._h = ctypes.c_void_p(self._L.or_create())
+        self._keep = None
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._L.or_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def theta_eff(self) -> float:
+        return DEFAULT_THETA if self.theta == 0 else self.theta
+
+    # -- build (build.go:11-51) ------------------------------------------------
+    def build(self, x, y, z, m, kind=None, alive=None, dm_off=False, theta=None):
+        arrs = _state_arrays(x, y, z, m, kind, alive)
+        self._keep = arrs
+        x, y, z, m, kind, alive = arrs
+        th = self.theta_eff() if theta is None else theta
+        self._L.or_set_monopole(self._h, int(self.monopole_only))
+        return int(self._L.or_build(self._h, _dp(x), _dp(y), _dp(z), _dp(m), _u8(kind), _u8(alive), len(x),
+                                    th, int(dm_off)))
+
+    @property
+    def n_stars(self) -> int:
+        return int(self._L.or_n_stars(self._h))
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self._L.or_n_nodes(self._h))
+
+    def nodes(self) -> np.ndarray:
+        n = self.n_nodes
+        if n == 0:
+            return np.zeros(0, NODE_DTYPE)
+        buf = (ctypes.c_char * (n * NODE_DTYPE.itemsize)).from_address(self._L.or_nodes(self._h))
+        return np.frombuffer(buf, dtype=NODE_DTYPE).copy()
+
+    def keys(self) -> np.ndarray:
+        n = self.n_stars
+        if n == 0:
+            return np.zeros(0, np.uint64)
+        buf = (ctypes.c_uint64 * n).from_address(self._L.or_keys(self._h))
+        return np.frombuffer(buf, dtype=np.uint64).copy()
+
+    def order(self) -> np.ndarray:
+        n = self.n_stars
+        if n == 0:
+            return np.zeros(0, np.int32)
+        buf = (ctypes.c_int32 * n).from_address(self._L.or_order(self._h))
+        return np.frombuffer(buf, dtype=np.int32).copy()
+
+    def sorted(self):
+        n = self.n_stars
+        out = []
+        for w in range(4):
+            if n == 0:
+                out.append(np.zeros(0))
+                continue
+            buf = (ctypes.c_double * n).from_address(self._L.or_sorted(self._h, w))
+            out.append(np.frombuffer(buf, dtype=np.float64).copy())
+        return out
+
+    # -- Accelerations (bh.go:76-102) ----------------------------------------
+    def accelerations(self, x, y, z, m, kind=None, alive=None, eps=0.0, dm_off=False, out=None):
+        x, y, z, m, kind, alive = _state_arrays(x, y, z, m, kind, alive)
+        n = len(x)
+        ax, ay, az = (np.zeros(n) for _ in range(3)) if out is None else out
+        self._keep = (x, y, z, m, kind, alive)
+        self._L.or_accelerations(self._h, _dp(x), _dp(y), _dp(z), _dp(m), _u8(kind), _u8(alive), n, self.theta,
+                                 eps, int(dm_off), int(self.monopole_only), int(self.workers), _dp(ax), _dp(ay),
+                                 _dp(az))
+        return np.stack([ax, ay, az], axis=1)
+
+    def accelerations_sorted(self, eps):
+        """Walk every star of the last build; results in sorted order."""
+        n = self.n_stars
+        ax, ay, az = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._L.or_accelerations_sorted(self._h, eps, _dp(ax), _dp(ay), _dp(az))
+        return np.stack([ax, ay, az], axis=1)
+
+    def walk(self, x, y, z, self_index=-1, eps2=0.0):
+        out = np.zeros(3)
+        self._L.or_walk(self._h, x, y, z, self_index, eps2, _dp(out))
+        return out
+
+    def walk_potential(self, x, y, z, self_index=-1, eps2=0.0):
+        return float(self._L.or_walk_potential(self._h, x, y, z, self_index, eps2))
+
+    # -- PotentialEnergy (potential.go:12-42) ---------------------------------
+    def potential_energy(self, x, y, z, m, kind=None, alive=None, eps=0.0, dm_off=False, c=0.0):
+        x, y, z, m, kind, alive = _state_arrays(x, y, z, m, kind, alive)
+        self._keep = (x, y, z, m, kind, alive)
+        return float(self._L.or_potential_energy(self._h, _dp(x), _dp(y), _dp(z), _dp(m), _u8(kind), _u8(alive),
+                                                 len(x), self.theta, eps, int(dm_off), int(self.monopole_only), c))
+
+
+def direct(x, y, z, m, kind=None, eps=0.0, dm_off=False, targets=None, workers=1):
+    """FP64 O(N^2) direct sum, direct.go:25-48.  Rows follow ``targets``."""
+    x, y, z, m, kind, _ = _state_arrays(x, y, z, m, kind, None)
+    n = len(x)
+    if targets is None:
+        nt, tp = n, None
+    else:
+        targets = np.ascontiguousarray(targets, dtype=np.int64)
+        nt, tp = len(targets), targets.ctypes.data_as(_I64)
+    ax, ay, az = np.zeros(nt), np.zeros(nt), np.zeros(nt)
+    lib().or_direct(_dp(x), _dp(y), _dp(z), _dp(m), _u8(kind), n, tp, nt, eps, int(dm_off), int(workers),
+                    _dp(ax), _dp(ay), _dp(az))
+    return np.stack([ax, ay, az], axis=1)
+
+
+def potential_exact(x, y, z, m, kind=None, eps=0.0, dm_off=False):
+    """Exact softened star-star potential energy (diag.go:44-64, no holes)."""
+    x, y, z, m, kind, _ = _state_arrays(x, y, z, m, kind, None)
+    return float(lib().or_potential_exact(_dp(x), _dp(y), _dp(z), _dp(m), _u8(kind), len(x), eps, int(dm_off)))
+
+
+def morton_key(x: int, y: int, z: int) -> int:
+    return int(lib().or_morton_key(x, y, z))
+
+
+def radix_sort(keys: np.ndarray):
+    """Serial stable LSD radix sort (morton.go:25-49); returns (keys, idx)."""
+    k = np.ascontiguousarray(keys, dtype=np.uint64).copy()
+    n = len(k)
+    idx = np.arange(n, dtype=np.int32)
+    tk, ti = np.zeros(n, np.uint64), np.zeros(n, np.int32)
+    lib().or_radix_sort(k.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                        idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                        tk.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                        ti.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n)
+    return k, idx
+
+
+def kick_drift_kick(state, acc_fn, dt):
+    """One KDK leapfrog step (sim.go:262-281 without holes/gas): kick(dt/2),
+    drift(dt), new forces, kick(dt/2).  ``state`` is a dict of float64 arrays
+    px..vz; ``acc_fn(px, py, pz)`` returns an (n, 3) acceleration array and is
+    assumed to hold the forces of the current positions on entry (state['acc'])."""
+    L = lib()
+    n = len(state["px"])
+    a = state["acc"]
+    ax, ay, az = (np.ascontiguousarray(a[:, i]) for i in range(3))
+    L.or_kick(n, _dp(state["vx"]), _dp(state["vy"]), _dp(state["vz"]), _dp(ax), _dp(ay), _dp(az), 0.5 * dt)
+    L.or_drift(n, _dp(state["px"]), _dp(state["py"]), _dp(state["pz"]), _dp(state["vx"]), _dp(state["vy"]),
+               _dp(state["vz"]), dt)
+    a = acc_fn(state["px"], state["py"], state["pz"])
+    ax, ay, az = (np.ascontiguousarray(a[:, i]) for i in range(3))
+    L.or_kick(n, _dp(state["vx"]), _dp(state["vy"]), _dp(state["vz"]), _dp(ax), _dp(ay), _dp(az), 0.5 * dt)
+    state["acc"] = a
+    return state

@@ -1,0 +1,667 @@
+/*
+ * bh_oracle.c -- CPU restatement of the gravitas Barnes-Hut force step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle the HIP backend is
+ * checked against (tests/, the smoke check and bench.py's cpu_baseline leg).
+ * No product code path links, loads or calls it.
+ *
+ * It restates, step by step and in the same floating-point order, the Go
+ * reference (paths relative to the gravitas repository):
+ *   collect              internal/sim/bh/build.go:62-113
+ *   keys                 internal/sim/bh/build.go:19-31, morton.go:7-21
+ *   radixSort            internal/sim/bh/morton.go:25-49
+ *   sorted copies        internal/sim/bh/build.go:34-41, internal/sim/sim.go:93-98
+ *   buildNode/split      internal/sim/bh/build.go:134-172, lowerBound :340-350,
+ *                        offset :353-358
+ *   finish/quadrupole    internal/sim/bh/build.go:176-217, :326-337
+ *   walk                 internal/sim/bh/bh.go:106-155
+ *   Accelerations        internal/sim/bh/bh.go:76-102 (CPU branch)
+ *   walkPotential        internal/sim/bh/potential.go:45-81, potentialWith :16-42
+ *   direct.accRange      internal/sim/direct/direct.go:25-48
+ *   ParallelFor          internal/sim/parallel.go:15-40
+ *   kick / drift         internal/sim/sim.go:302-343 (Newtonian branch, c <= 0)
+ *   PotentialEnergyParallel, kinetic, Momentum
+ *                        internal/sim/diag.go:44-64, relativity.go:24-26, diag.go:100-109
+ *   PairPotential        internal/sim/pair.go:37-47, SchwarzschildRadius
+ *                        internal/sim/relativity.go:49-54
+ *
+ * Build with -O2 -ffp-contract=off (oracle/Makefile): Go's gc compiler on
+ * amd64 does not fuse a*b+c, so the same operations in the same order give
+ * the same IEEE-754 doubles.  The reference builds its tree serially below
+ * 2^15 stars and in parallel above; both give the same tree and bit-identical
+ * forces (internal/sim/bh/bh_test.go:378-394), so the serial recursive build
+ * here stands for both.  The walk runs on pthreads with ParallelFor's dynamic
+ * chunking; its result is independent of the thread count.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MORTON_BITS 21
+#define LEAF_SIZE 24
+#define DEFAULT_THETA 0.9
+#define DIAG_THETA 0.9
+#define GRAIN 128
+
+enum { KIND_STAR = 0, KIND_HOLE = 1, KIND_DARK = 2, KIND_GAS = 3 };
+
+typedef struct {
+    double x, y, z, m;
+    double qxx, qyy, qzz, qxy, qxz, qyz;
+    double crit2;
+    int32_t first, next, lo, hi;
+} or_node;
+
+typedef struct {
+    int64_t n_stars, n_nodes, cap_nodes, cap_stars, n_holes, cap_holes;
+    or_node *nodes;
+    uint64_t *keys, *tmpk;
+    int32_t *order, *tmpi, *holes;
+    double *sx, *sy, *sz, *sm;
+    double theta;
+    int monopole;
+    int workers;
+} or_solver;
+
+/* Go's math.Min / math.Max: NaN propagates, -0 < +0. */
+static double go_min(double a, double b) {
+    if (isnan(a) || isnan(b)) return NAN;
+    if (a == 0 && b == 0) return signbit(a) ? a : b;
+    return a < b ? a : b;
+}
+static double go_max(double a, double b) {
+    if (isnan(a) || isnan(b)) return NAN;
+    if (a == 0 && b == 0) return signbit(a) ? b : a;
+    return a > b ? a : b;
+}
+
+/* ------------------------------------------------------------------ */
+/* Morton keys: morton.go:7-21                                          */
+/* ------------------------------------------------------------------ */
+uint64_t or_spread(uint64_t v) {
+    v &= (1ull << MORTON_BITS) - 1;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+uint64_t or_morton_key(uint64_t x, uint64_t y, uint64_t z) {
+    return or_spread(x) << 2 | or_spread(y) << 1 | or_spread(z);
+}
+
+/* cell(): build.go:23-25.  Go's builtin min on floats propagates NaN. */
+static uint64_t cell(double v, double lo, double scale) {
+    double t = go_max(0, (v - lo) * scale);
+    double c = isnan(t) ? t : (t < 2097151.0 ? t : 2097151.0);
+    return (uint64_t)c;
+}
+
+/* ------------------------------------------------------------------ */
+/* Stable LSD radix sort, 8-bit digits: morton.go:25-49                 */
+/* ------------------------------------------------------------------ */
+void or_radix_sort(uint64_t *keys, int32_t *idx, uint64_t *tmpk, int32_t *tmpi, int64_t n) {
+    if (n <= 0) return;
+    int64_t count[256];
+    for (unsigned shift = 0; shift < 64; shift += 8) {
+        memset(count, 0, sizeof count);
+        for (int64_t i = 0; i < n; i++) count[(keys[i] >> shift) & 0xff]++;
+        if (count[(keys[0] >> shift) & 0xff] == n) continue; /* all keys share this digit */
+        int64_t sum = 0;
+        for (int d = 0; d < 256; d++) {
+            int64_t c = count[d];
+            count[d] = sum;
+            sum += c;
+        }
+        for (int64_t i = 0; i < n; i++) {
+            uint64_t d = (keys[i] >> shift) & 0xff;
+            tmpk[count[d]] = keys[i];
+            tmpi[count[d]] = idx[i];
+            count[d]++;
+        }
+        memcpy(keys, tmpk, (size_t)n * sizeof *keys);
+        memcpy(idx, tmpi, (size_t)n * sizeof *idx);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Solver buffers                                                       */
+/* ------------------------------------------------------------------ */
+or_solver *or_create(void) {
+    or_solver *s = calloc(1, sizeof *s);
+    s->workers = 1;
+    return s;
+}
+
+void or_destroy(or_solver *s) {
+    if (!s) return;
+    free(s->nodes); free(s->keys); free(s->tmpk); free(s->order); free(s->tmpi);
+    free(s->holes); free(s->sx); free(s->sy); free(s->sz); free(s->sm);
+    free(s);
+}
+
+static void reserve_stars(or_solver *s, int64_t n) {
+    if (n <= s->cap_stars) return;
+    int64_t c = n + n / 4 + 16;
+    s->keys = realloc(s->keys, c * sizeof *s->keys);
+    s->tmpk = realloc(s->tmpk, c * sizeof *s->tmpk);
+    s->order = realloc(s->order, c * sizeof *s->order);
+    s->tmpi = realloc(s->tmpi, c * sizeof *s->tmpi);
+    s->sx = realloc(s->sx, c * sizeof *s->sx);
+    s->sy = realloc(s->sy, c * sizeof *s->sy);
+    s->sz = realloc(s->sz, c * sizeof *s->sz);
+    s->sm = realloc(s->sm, c * sizeof *s->sm);
+    s->cap_stars = c;
+}
+
+static int32_t new_node(or_solver *s) {
+    if (s->n_nodes == s->cap_nodes) {
+        s->cap_nodes = s->cap_nodes ? s->cap_nodes * 2 : 1024;
+        s->nodes = realloc(s->nodes, s->cap_nodes * sizeof *s->nodes);
+    }
+    memset(&s->nodes[s->n_nodes], 0, sizeof(or_node));
+    return (int32_t)s->n_nodes++;
+}
+
+/* ------------------------------------------------------------------ */
+/* Tree: build.go:134-217, 326-358                                      */
+/* ------------------------------------------------------------------ */
+static int32_t lower_bound(const uint64_t *keys, int32_t lo, int32_t hi, unsigned shift, uint64_t oct) {
+    while (lo < hi) {
+        int32_t mid = (int32_t)((uint32_t)(lo + hi) >> 1);
+        if (((keys[mid] >> shift) & 7) < oct)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+static int32_t split(const or_solver *s, int32_t lo, int32_t hi, int level, int32_t bounds[9]) {
+    unsigned shift = (unsigned)(3 * (MORTON_BITS - 1 - level));
+    bounds[0] = lo;
+    bounds[8] = hi;
+    for (uint64_t oct = 1; oct < 8; oct++) bounds[oct] = lower_bound(s->keys, bounds[oct - 1], hi, shift, oct);
+    int32_t children = 0;
+    for (int oct = 0; oct < 8; oct++)
+        if (bounds[oct + 1] > bounds[oct]) children++;
+    return children;
+}
+
+static double offset(int oct, int bit) { return ((oct >> bit) & 1) ? 1.0 : -1.0; }
+
+static void quad_add(double q[6], double m, double dx, double dy, double dz) {
+    double r2 = dx * dx + dy * dy + dz * dz;
+    q[0] += m * (3 * dx * dx - r2);
+    q[1] += m * (3 * dy * dy - r2);
+    q[2] += m * (3 * dz * dz - r2);
+    q[3] += m * 3 * dx * dy;
+    q[4] += m * 3 * dx * dz;
+    q[5] += m * 3 * dy * dz;
+}
+
+static or_node finish(const or_solver *s, int32_t lo, int32_t hi, int32_t first, int32_t children,
+                      double cx, double cy, double cz, double half, int32_t next, double theta) {
+    double x = 0, y = 0, z = 0, m = 0;
+    if (first < 0) {
+        for (int32_t j = lo; j < hi; j++) {
+            double mj = s->sm[j];
+            x = x + mj * s->sx[j];
+            y = y + mj * s->sy[j];
+            z = z + mj * s->sz[j];
+            m = m + mj;
+        }
+    } else {
+        for (int32_t c = first; c < first + children; c++) {
+            const or_node *ch = &s->nodes[c];
+            x = x + ch->m * ch->x;
+            y = y + ch->m * ch->y;
+            z = z + ch->m * ch->z;
+            m = m + ch->m;
+        }
+    }
+    if (m > 0) {
+        x = x / m; y = y / m; z = z / m;
+    } else {
+        x = cx; y = cy; z = cz;
+    }
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    if (first < 0) {
+        for (int32_t j = lo; j < hi; j++) quad_add(q, s->sm[j], s->sx[j] - x, s->sy[j] - y, s->sz[j] - z);
+    } else {
+        for (int32_t c = first; c < first + children; c++) {
+            const or_node *ch = &s->nodes[c];
+            q[0] = q[0] + ch->qxx; q[1] = q[1] + ch->qyy; q[2] = q[2] + ch->qzz;
+            q[3] = q[3] + ch->qxy; q[4] = q[4] + ch->qxz; q[5] = q[5] + ch->qyz;
+            quad_add(q, ch->m, ch->x - x, ch->y - y, ch->z - z);
+        }
+    }
+    double delta = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
+    double crit = 2 * half / theta + delta;
+    or_node nd;
+    nd.x = x; nd.y = y; nd.z = z; nd.m = m;
+    nd.qxx = q[0]; nd.qyy = q[1]; nd.qzz = q[2]; nd.qxy = q[3]; nd.qxz = q[4]; nd.qyz = q[5];
+    nd.crit2 = crit * crit;
+    nd.first = first; nd.next = next; nd.lo = lo; nd.hi = hi;
+    return nd;
+}
+
+static void build_node(or_solver *s, int32_t ni, int32_t lo, int32_t hi, int level,
+                       double cx, double cy, double cz, double half, int32_t next) {
+    int32_t first = -1, children = 0;
+    if (hi - lo > LEAF_SIZE && level < MORTON_BITS) {
+        int32_t bounds[9];
+        children = split(s, lo, hi, level, bounds);
+        first = (int32_t)s->n_nodes;
+        for (int32_t c = 0; c < children; c++) new_node(s);
+        int32_t c = first;
+        double q = half / 2;
+        for (int oct = 0; oct < 8; oct++) {
+            int32_t clo = bounds[oct], chi = bounds[oct + 1];
+            if (chi == clo) continue;
+            int32_t child_next = next;
+            if (c < first + children - 1) child_next = c + 1;
+            build_node(s, c, clo, chi, level + 1, cx + offset(oct, 2) * q, cy + offset(oct, 1) * q,
+                       cz + offset(oct, 0) * q, q, child_next);
+            c++;
+        }
+    }
+    s->nodes[ni] = finish(s, lo, hi, first, children, cx, cy, cz, half, next, s->theta);
+}
+
+/* ------------------------------------------------------------------ */
+/* ParallelFor: parallel.go:15-40 (dynamic chunks, 8 per worker)        */
+/* ------------------------------------------------------------------ */
+typedef void (*range_fn)(void *ctx, int64_t lo, int64_t hi);
+typedef struct {
+    range_fn fn;
+    void *ctx;
+    int64_t n, chunk, chunks;
+    atomic_llong next;
+} pf_job;
+
+static void *pf_worker(void *arg) {
+    pf_job *j = arg;
+    for (;;) {
+        int64_t c = atomic_fetch_add(&j->next, 1);
+        if (c >= j->chunks) return NULL;
+        int64_t lo = c * j->chunk, hi = lo + j->chunk < j->n ? lo + j->chunk : j->n;
+        j->fn(j->ctx, lo, hi);
+    }
+}
+
+static void parallel_for(int64_t n, int64_t grain, int workers, range_fn fn, void *ctx) {
+    if (workers <= 0) workers = 1;
+    if (n <= grain || workers == 1) {
+        fn(ctx, 0, n);
+        return;
+    }
+    pf_job j;
+    j.fn = fn; j.ctx = ctx; j.n = n;
+    j.chunk = n / ((int64_t)workers * 8);
+    if (j.chunk < grain) j.chunk = grain;
+    j.chunks = (n + j.chunk - 1) / j.chunk;
+    atomic_init(&j.next, 0);
+    int nt = workers < j.chunks ? workers : (int)j.chunks;
+    pthread_t th[256];
+    if (nt > 256) nt = 256;
+    for (int t = 1; t < nt; t++) pthread_create(&th[t], NULL, pf_worker, &j);
+    pf_worker(&j);
+    for (int t = 1; t < nt; t++) pthread_join(th[t], NULL);
+}
+
+/* ------------------------------------------------------------------ */
+/* build: build.go:11-51 with collect :62-113                           */
+/* ------------------------------------------------------------------ */
+/* Returns the number of tree stars; fills order/keys/sorted copies/nodes. */
+int64_t or_build(or_solver *s, const double *px, const double *py, const double *pz, const double *mass,
+                 const uint8_t *kind, const uint8_t *alive, int64_t n, double theta, int dm_off) {
+    s->theta = theta;
+    s->n_nodes = 0;
+    /* collect: holes, stars in state order, star bounding box */
+    s->n_holes = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (alive[i] && kind[i] == KIND_HOLE) {
+            if (s->n_holes == s->cap_holes) {
+                s->cap_holes = s->cap_holes ? 2 * s->cap_holes : 16;
+                s->holes = realloc(s->holes, s->cap_holes * sizeof *s->holes);
+            }
+            s->holes[s->n_holes++] = (int32_t)i;
+        }
+    }
+    reserve_stars(s, n);
+    double minX = INFINITY, minY = INFINITY, minZ = INFINITY;
+    double maxX = -INFINITY, maxY = -INFINITY, maxZ = -INFINITY;
+    int64_t ns = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (!(alive[i] && kind[i] != KIND_HOLE)) continue;
+        s->order[ns++] = (int32_t)i;
+        minX = go_min(minX, px[i]); maxX = go_max(maxX, px[i]);
+        minY = go_min(minY, py[i]); maxY = go_max(maxY, py[i]);
+        minZ = go_min(minZ, pz[i]); maxZ = go_max(maxZ, pz[i]);
+    }
+    s->n_stars = ns;
+    if (ns == 0) return 0;
+
+    double side = go_max(maxX - minX, go_max(maxY - minY, maxZ - minZ));
+    side = go_max(side * (1 + 1e-9), 1e-12);
+    double scale = (double)(1 << MORTON_BITS) / side;
+    for (int64_t k = 0; k < ns; k++) {
+        int32_t i = s->order[k];
+        s->keys[k] = or_morton_key(cell(px[i], minX, scale), cell(py[i], minY, scale), cell(pz[i], minZ, scale));
+    }
+    or_radix_sort(s->keys, s->order, s->tmpk, s->tmpi, ns);
+    for (int64_t k = 0; k < ns; k++) {
+        int32_t i = s->order[k];
+        s->sx[k] = px[i];
+        s->sy[k] = py[i];
+        s->sz[k] = pz[i];
+        s->sm[k] = (kind[i] == KIND_DARK && dm_off) ? 0.0 : mass[i];
+    }
+    double half = side / 2;
+    new_node(s);
+    build_node(s, 0, 0, (int32_t)ns, 0, minX + half, minY + half, minZ + half, half, -1);
+    return ns;
+}
+
+/* ------------------------------------------------------------------ */
+/* walk: bh.go:106-155                                                  */
+/* ------------------------------------------------------------------ */
+void or_walk(const or_solver *s, double x, double y, double z, int32_t self, double eps2, double out[3]) {
+    double ax = 0, ay = 0, az = 0;
+    if (s->n_nodes == 0) {
+        out[0] = out[1] = out[2] = 0;
+        return;
+    }
+    const or_node *nodes = s->nodes;
+    const double *sx = s->sx, *sy = s->sy, *sz = s->sz, *sm = s->sm;
+    int quad = !s->monopole;
+    for (int32_t i = 0; i >= 0;) {
+        const or_node *nd = &nodes[i];
+        double dx = nd->x - x, dy = nd->y - y, dz = nd->z - z;
+        double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 >= nd->crit2) {
+            double inv = 1 / (d2 + eps2);
+            double inv3 = inv * sqrt(inv);
+            double f = nd->m * inv3;
+            if (quad) {
+                double qx = nd->qxx * dx + nd->qxy * dy + nd->qxz * dz;
+                double qy = nd->qxy * dx + nd->qyy * dy + nd->qyz * dz;
+                double qz = nd->qxz * dx + nd->qyz * dy + nd->qzz * dz;
+                double inv5 = inv3 * inv;
+                f += 2.5 * (dx * qx + dy * qy + dz * qz) * inv5 * inv;
+                ax -= qx * inv5;
+                ay -= qy * inv5;
+                az -= qz * inv5;
+            }
+            ax += f * dx;
+            ay += f * dy;
+            az += f * dz;
+            i = nd->next;
+            continue;
+        }
+        if (nd->first >= 0) {
+            i = nd->first;
+            continue;
+        }
+        for (int32_t j = nd->lo; j < nd->hi; j++) {
+            if (j == self) continue;
+            double ddx = sx[j] - x, ddy = sy[j] - y, ddz = sz[j] - z;
+            double r2 = ddx * ddx + ddy * ddy + ddz * ddz + eps2;
+            double f = sm[j] / (r2 * sqrt(r2));
+            ax += f * ddx;
+            ay += f * ddy;
+            az += f * ddz;
+        }
+        i = nd->next;
+    }
+    out[0] = ax; out[1] = ay; out[2] = az;
+}
+
+/* walkPotential: potential.go:45-81 */
+double or_walk_potential(const or_solver *s, double x, double y, double z, int32_t self, double eps2) {
+    if (s->n_nodes == 0) return 0;
+    const or_node *nodes = s->nodes;
+    int quad = !s->monopole;
+    double psi = 0;
+    for (int32_t i = 0; i >= 0;) {
+        const or_node *nd = &nodes[i];
+        double dx = nd->x - x, dy = nd->y - y, dz = nd->z - z;
+        double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 >= nd->crit2) {
+            double inv = 1 / (d2 + eps2);
+            psi += nd->m * sqrt(inv);
+            if (quad) {
+                double dqd = nd->qxx * dx * dx + nd->qyy * dy * dy + nd->qzz * dz * dz +
+                             2 * (nd->qxy * dx * dy + nd->qxz * dx * dz + nd->qyz * dy * dz);
+                psi += 0.5 * dqd * inv * inv * sqrt(inv);
+            }
+            i = nd->next;
+            continue;
+        }
+        if (nd->first >= 0) {
+            i = nd->first;
+            continue;
+        }
+        for (int32_t j = nd->lo; j < nd->hi; j++) {
+            if (j == self) continue;
+            double ddx = s->sx[j] - x, ddy = s->sy[j] - y, ddz = s->sz[j] - z;
+            psi += s->sm[j] / sqrt(ddx * ddx + ddy * ddy + ddz * ddz + eps2);
+        }
+        i = nd->next;
+    }
+    return psi;
+}
+
+/* ------------------------------------------------------------------ */
+/* Accelerations (CPU branch): bh.go:76-102                             */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const or_solver *s;
+    double eps2;
+    double *ax, *ay, *az;
+} acc_ctx;
+
+static void acc_range(void *p, int64_t lo, int64_t hi) {
+    acc_ctx *c = p;
+    const or_solver *s = c->s;
+    for (int64_t k = lo; k < hi; k++) {
+        int32_t i = s->order[k];
+        double a[3];
+        or_walk(s, s->sx[k], s->sy[k], s->sz[k], (int32_t)k, c->eps2, a);
+        c->ax[i] = a[0]; c->ay[i] = a[1]; c->az[i] = a[2];
+    }
+}
+
+/* Accelerations of the last build into state-order arrays.  Holes get 0;
+ * dead bodies are left untouched, as in the reference. */
+void or_accelerations_built(or_solver *s, double eps, double *ax, double *ay, double *az) {
+    acc_ctx c = {s, eps * eps, ax, ay, az};
+    parallel_for(s->n_stars, GRAIN, s->workers, acc_range, &c);
+    for (int64_t h = 0; h < s->n_holes; h++) {
+        int32_t i = s->holes[h];
+        ax[i] = ay[i] = az[i] = 0;
+    }
+}
+
+/* Accelerations: theta == 0 means DefaultTheta (bh.go:69-74). */
+void or_accelerations(or_solver *s, const double *px, const double *py, const double *pz, const double *mass,
+                      const uint8_t *kind, const uint8_t *alive, int64_t n, double theta, double eps, int dm_off,
+                      int monopole, int workers, double *ax, double *ay, double *az) {
+    s->monopole = monopole;
+    s->workers = workers;
+    or_build(s, px, py, pz, mass, kind, alive, n, theta == 0 ? DEFAULT_THETA : theta, dm_off);
+    or_accelerations_built(s, eps, ax, ay, az);
+}
+
+/* Accelerations of the sorted stars (sorted order), for walk-level checks. */
+void or_accelerations_sorted(or_solver *s, double eps, double *ax, double *ay, double *az) {
+    double eps2 = eps * eps;
+    for (int64_t k = 0; k < s->n_stars; k++) {
+        double a[3];
+        or_walk(s, s->sx[k], s->sy[k], s->sz[k], (int32_t)k, eps2, a);
+        ax[k] = a[0]; ay[k] = a[1]; az[k] = a[2];
+    }
+}
+
+/* PairPotential / SchwarzschildRadius: pair.go:37-47, relativity.go:49-54 */
+static double pair_potential(double mi, double mj, double r2, double eps2, double rs) {
+    double re = sqrt(r2 + eps2);
+    if (rs == 0) return -mi * mj / re;
+    if (re >= 1.5 * rs) return -mi * mj / (re - rs);
+    return mi * mj * (-2 / rs + 4 / (rs * rs) * (re - 1.5 * rs));
+}
+static double schwarzschild(double m, double c) { return c <= 0 ? 0 : 2 * m / (c * c); }
+
+/* PotentialEnergy: potential.go:12-42, theta capped at diagTheta.  The
+ * per-star psi values are summed in sorted order (the reference sums chunk
+ * results in completion order, so only round-off may differ). */
+double or_potential_energy(or_solver *s, const double *px, const double *py, const double *pz, const double *mass,
+                           const uint8_t *kind, const uint8_t *alive, int64_t n, double theta, double eps, int dm_off,
+                           int monopole, double c) {
+    s->monopole = monopole;
+    double th = theta == 0 ? DEFAULT_THETA : theta;
+    if (th > DIAG_THETA) th = DIAG_THETA;
+    or_build(s, px, py, pz, mass, kind, alive, n, th, dm_off);
+    double eps2 = eps * eps, sum = 0;
+    for (int64_t k = 0; k < s->n_stars; k++) {
+        int32_t i = s->order[k];
+        double hp = 0;
+        for (int64_t h = 0; h < s->n_holes; h++) {
+            int32_t hi = s->holes[h];
+            if (hi == i) continue;
+            double dx = px[hi] - px[i], dy = py[hi] - py[i], dz = pz[hi] - pz[i];
+            double rs = (c <= 0) ? 0 : schwarzschild(mass[hi], c); /* PairRS: star-hole pair */
+            double mi = (kind[i] == KIND_DARK && dm_off) ? 0 : mass[i];
+            hp += pair_potential(mi, mass[hi], dx * dx + dy * dy + dz * dz, eps2, rs);
+        }
+        sum += s->sm[k] * or_walk_potential(s, s->sx[k], s->sy[k], s->sz[k], (int32_t)k, eps2) - hp;
+    }
+    for (int64_t h = 0; h < s->n_holes; h++) {
+        int32_t i = s->holes[h];
+        double rs = schwarzschild(mass[i], c);
+        for (int64_t k = 0; k < s->n_stars; k++) {
+            double dx = s->sx[k] - px[i], dy = s->sy[k] - py[i], dz = s->sz[k] - pz[i];
+            sum -= pair_potential(mass[i], s->sm[k], dx * dx + dy * dy + dz * dz, eps2, rs);
+        }
+        double hp = 0; /* hole-hole pairs: PairRS gives 0 for two holes */
+        for (int64_t g = 0; g < s->n_holes; g++) {
+            int32_t j = s->holes[g];
+            if (j == i) continue;
+            double dx = px[j] - px[i], dy = py[j] - py[i], dz = pz[j] - pz[i];
+            hp += pair_potential(mass[i], mass[j], dx * dx + dy * dy + dz * dz, eps2, 0);
+        }
+        sum -= hp;
+    }
+    return -sum / 2;
+}
+
+/* ------------------------------------------------------------------ */
+/* Direct sum: direct.go:25-48                                          */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const double *px, *py, *pz, *m;
+    const uint8_t *kind;
+    const int64_t *targets;
+    int64_t n;
+    double eps2;
+    int dm_off;
+    double *ax, *ay, *az;
+} dir_ctx;
+
+static void direct_one(const dir_ctx *c, int64_t i, double out[3]) {
+    double xi = c->px[i], yi = c->py[i], zi = c->pz[i];
+    double ax = 0, ay = 0, az = 0;
+    if (c->kind[i] == KIND_HOLE) {
+        out[0] = out[1] = out[2] = 0;
+        return;
+    }
+    for (int64_t j = 0; j < c->n; j++) {
+        if (j == i || c->kind[j] == KIND_HOLE || (c->dm_off && c->kind[j] == KIND_DARK)) continue;
+        double dx = c->px[j] - xi, dy = c->py[j] - yi, dz = c->pz[j] - zi;
+        double re2 = dx * dx + dy * dy + dz * dz + c->eps2;
+        double f = c->m[j] / (re2 * sqrt(re2));
+        ax += f * dx;
+        ay += f * dy;
+        az += f * dz;
+    }
+    out[0] = ax; out[1] = ay; out[2] = az;
+}
+
+static void direct_range(void *p, int64_t lo, int64_t hi) {
+    dir_ctx *c = p;
+    for (int64_t t = lo; t < hi; t++) {
+        int64_t i = c->targets ? c->targets[t] : t;
+        double a[3];
+        direct_one(c, i, a);
+        c->ax[t] = a[0]; c->ay[t] = a[1]; c->az[t] = a[2];
+    }
+}
+
+/* Direct-sum accelerations of targets[0..nt) (all bodies if targets is NULL,
+ * then nt must equal n); out[t] belongs to targets[t]. */
+void or_direct(const double *px, const double *py, const double *pz, const double *mass, const uint8_t *kind,
+               int64_t n, const int64_t *targets, int64_t nt, double eps, int dm_off, int workers,
+               double *ax, double *ay, double *az) {
+    dir_ctx c = {px, py, pz, mass, kind, targets, n, eps * eps, dm_off, ax, ay, az};
+    parallel_for(targets ? nt : n, 64, workers, direct_range, &c);
+}
+
+/* ------------------------------------------------------------------ */
+/* Integrator pieces (sim.go:302-343, Newtonian) and diagnostics        */
+/* ------------------------------------------------------------------ */
+void or_kick(int64_t n, double *vx, double *vy, double *vz, const double *ax, const double *ay, const double *az,
+             double h) {
+    for (int64_t i = 0; i < n; i++) {
+        vx[i] += h * ax[i];
+        vy[i] += h * ay[i];
+        vz[i] += h * az[i];
+    }
+}
+
+void or_drift(int64_t n, double *px, double *py, double *pz, const double *vx, const double *vy, const double *vz,
+              double dt) {
+    for (int64_t i = 0; i < n; i++) {
+        px[i] += dt * vx[i];
+        py[i] += dt * vy[i];
+        pz[i] += dt * vz[i];
+    }
+}
+
+/* Exact pair potential of stars (no holes, c <= 0): diag.go:44-64 restricted
+ * to the star-star pair model. */
+double or_potential_exact(const double *px, const double *py, const double *pz, const double *mass,
+                          const uint8_t *kind, int64_t n, double eps, int dm_off) {
+    double eps2 = eps * eps, sum = 0;
+    for (int64_t i = 0; i < n; i++) {
+        double mi = (kind[i] == KIND_DARK && dm_off) ? 0 : mass[i];
+        double part = 0;
+        for (int64_t j = i + 1; j < n; j++) {
+            double mj = (kind[j] == KIND_DARK && dm_off) ? 0 : mass[j];
+            double dx = px[j] - px[i], dy = py[j] - py[i], dz = pz[j] - pz[i];
+            part += pair_potential(mi, mj, dx * dx + dy * dy + dz * dz, eps2, 0);
+        }
+        sum += part;
+    }
+    return sum;
+}
+
+/* ------------------------------------------------------------------ */
+/* Accessors for the Python wrapper                                     */
+/* ------------------------------------------------------------------ */
+int64_t or_n_stars(const or_solver *s) { return s->n_stars; }
+int64_t or_n_nodes(const or_solver *s) { return s->n_nodes; }
+const or_node *or_nodes(const or_solver *s) { return s->nodes; }
+const uint64_t *or_keys(const or_solver *s) { return s->keys; }
+const int32_t *or_order(const or_solver *s) { return s->order; }
+const double *or_sorted(const or_solver *s, int which) {
+    return which == 0 ? s->sx : which == 1 ? s->sy : which == 2 ? s->sz : s->sm;
+}
+void or_set_monopole(or_solver *s, int monopole) { s->monopole = monopole; }
+void or_set_workers(or_solver *s, int workers) { s->workers = workers; }
+int or_node_size(void) { return (int)sizeof(or_node); }

@@ -1,0 +1,22 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """Skip unless the HIP backend finds a gfx950 device."""
+    from gravitas_amd import gpu_available
+    reason = gpu_available()
+    if reason is not None:
+        pytest.skip(f"no GPU: {reason}")
+    return 0

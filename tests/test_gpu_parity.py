@@ -1,0 +1,244 @@
+"""GPU parity: the HIP force step against the CPU oracle (bh_oracle.c).
+
+  * keys and permutation: bit-identical (build.go:19-32, morton.go:25-49);
+  * tree: identical node sequence in traversal order, bit-identical FP64
+    centres of mass, quadrupoles and opening radii (build.go:134-217);
+  * FP64 walk: bit-identical accelerations (bh.go:106-155);
+  * FP32 group walk: same interaction counts as the FP64 walk; relative error
+    vs the FP64 walk within the reference's own GPU bound (gpu_test.go:18:
+    median <= 1e-5, max <= 1e-3), and p99 <= 1e-4;
+  * contract: holes 0, dead bodies untouched, dark matter off (sim.go:93-98).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from gravitas_amd import DeviceState, Params, Solver, ics
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def dfs_sequence(nodes):
+    """Node indices in traversal order with every node opened: the order the
+    skip links define (first child if any, else next)."""
+    seq = []
+    i = 0
+    if len(nodes) == 0:
+        return seq
+    while i >= 0:
+        seq.append(i)
+        i = int(nodes["first"][i]) if nodes["first"][i] >= 0 else int(nodes["next"][i])
+        assert len(seq) <= len(nodes)
+    return seq
+
+
+def run_gpu(state, theta, eps, walk, dm_off=False, export=False):
+    solver = Solver(theta=theta, walk=walk)
+    ds = DeviceState.from_host(state)
+    solver.accelerations(ds, Params(softening=eps, dark_matter_off=dm_off))
+    out = {"acc": ds.acc_host(), "stats": solver.stats()}
+    if export:
+        for w in ("keys", "order", "spos", "nodes", "mult"):
+            out[w] = solver.export(w)
+    return out
+
+
+def run_oracle(state, theta, eps, dm_off=False):
+    o = oracle.Solver(theta=theta, workers=8)
+    acc = o.accelerations(state["px"], state["py"], state["pz"], state["m"], state["kind"],
+                          state.get("alive"), eps=eps, dm_off=dm_off,
+                          out=tuple(np.zeros(len(state["px"])) for _ in range(3)))
+    return o, acc
+
+
+def mixed_state(n, seed):
+    """Plummer stars + dark-matter halo + two holes + some dead bodies."""
+    s = ics.concat(ics.plummer(n // 2, seed), ics.hernquist_halo(n - n // 2, seed + 1, mass=2.0, a=2.0, rmax=20.0))
+    s["kind"][[3, 17]] = ics.KIND_HOLE
+    s["m"][[3, 17]] = 0.05
+    alive = np.ones(len(s["px"]), np.uint8)
+    alive[[5, 99, 1000]] = 0
+    s["alive"] = alive
+    return s
+
+
+CASES = {
+    "plummer-5k": (lambda: ics.plummer(5000, 11), 0.5, 0.01, False),
+    "disk-halo-20k": (lambda: ics.disk_halo(20000, 12), 0.7, 0.02, False),
+    "disk-halo-20k-dmoff": (lambda: ics.disk_halo(20000, 12), 0.5, 0.02, True),
+    "mixed-8k": (lambda: mixed_state(8000, 13), 0.6, 0.01, False),
+    "clumps-30k": (lambda: ics.clumps(30000, 12, 14, scale=0.005), 0.5, 1e-4, False),
+    "theta-1.0": (lambda: ics.plummer(3000, 15), 1.0, 0.01, False),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_fp64_walk_and_tree_bit_identical(gpu, case):
+    make, theta, eps, dm_off = CASES[case]
+    state = make()
+    say(f"[parity] case={case} n={len(state['px'])} theta={theta} eps={eps} walk=fp64")
+    g = run_gpu(state, theta, eps, "fp64", dm_off, export=True)
+    o, want = run_oracle(state, theta, eps, dm_off)
+    # keys and permutation
+    assert np.array_equal(g["keys"], o.keys())
+    assert np.array_equal(g["order"], o.order())
+    sx, sy, sz, sm = o.sorted()
+    assert np.array_equal(g["spos"][:, 0], sx) and np.array_equal(g["spos"][:, 3], sm)
+    # tree: same node sequence in traversal order, same moments
+    on = o.nodes()
+    gn, gm = g["nodes"], g["mult"]
+    assert g["stats"]["n_nodes"] == o.n_nodes
+    oseq, gseq = dfs_sequence(on), dfs_sequence(gn)
+    assert len(oseq) == len(gseq) == o.n_nodes
+    oo, gg = on[oseq], gn[gseq]
+    for f in ("lo", "hi"):
+        assert np.array_equal(oo[f], gg[f]), f
+    assert np.array_equal(oo["first"] < 0, gg["first"] < 0)
+    for f in ("x", "y", "z", "crit2"):
+        assert np.array_equal(oo[f], gg[f]), f
+    gmm = gm[gseq]
+    for c, f in enumerate(("m", "qxx", "qyy", "qzz", "qxy", "qxz", "qyz")):
+        assert np.array_equal(oo[f], gmm[:, c]), f
+    # forces, state order, bit for bit
+    assert np.array_equal(g["acc"], want)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_fp32_walk_matches_fp64_walk(gpu, case):
+    make, theta, eps, dm_off = CASES[case]
+    state = make()
+    say(f"[parity] case={case} n={len(state['px'])} theta={theta} eps={eps} walk=fp32 vs fp64")
+    g32 = run_gpu(state, theta, eps, "fp32", dm_off)
+    g64 = run_gpu(state, theta, eps, "fp64", dm_off)
+    # identical interaction set
+    assert g32["stats"]["pp_interactions"] == g64["stats"]["pp_interactions"]
+    assert g32["stats"]["pc_interactions"] == g64["stats"]["pc_interactions"]
+    assert g32["stats"]["fallbacks"] == 0
+    alive = state.get("alive", np.ones(len(state["px"]), np.uint8)).astype(bool)
+    star = alive & (state["kind"] != ics.KIND_HOLE)
+    e = np.sort(ics.rel_errors(g32["acc"][star], g64["acc"][star]))
+    med, p99, mx = e[len(e) // 2], e[int(0.99 * (len(e) - 1))], e[-1]
+    say(f"[parity] {case}: fp32 vs fp64 median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
+    assert med <= 1e-5 and p99 <= 1e-4 and mx <= 1e-3
+
+
+def test_two_stars_known_answer(gpu):
+    """bh_test.go:137-147: stars 1@x=1, 2@x=2 and a hole: (2,0,0), (-1,0,0), 0."""
+    s = {"px": np.array([1.0, 2.0, 0.0]), "py": np.zeros(3), "pz": np.zeros(3), "m": np.array([1.0, 2.0, 4.0]),
+         "kind": np.array([0, 0, 1], np.uint8)}
+    for walk in ("fp64", "fp32"):
+        say(f"[parity] two-star known answer walk={walk}")
+        acc = run_gpu(s, 0.0, 0.0, walk)["acc"]
+        assert np.array_equal(acc, [[2.0, 0, 0], [-1.0, 0, 0], [0, 0, 0]]), acc
+
+
+def test_coincident_stars_no_nan(gpu):
+    """bh_test.go:149-161: 100 coincident stars and one more, eps 0.01."""
+    n = 101
+    s = {"px": np.r_[np.ones(100), 0.0], "py": np.r_[np.full(100, 2.0), 0.0], "pz": np.r_[np.full(100, 3.0), 0.0],
+         "m": np.ones(n), "kind": np.zeros(n, np.uint8)}
+    o, want = run_oracle(s, 0.0, 0.01)
+    for walk in ("fp64", "fp32"):
+        say(f"[parity] coincident walk={walk}")
+        g = run_gpu(s, 0.0, 0.01, walk)
+        assert np.isfinite(g["acc"]).all()
+        assert g["stats"]["n_nodes"] == o.n_nodes
+        if walk == "fp64":
+            assert np.array_equal(g["acc"], want)
+        else:
+            assert np.allclose(g["acc"], want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
+
+
+def test_dead_bodies_untouched_and_holes_zero(gpu):
+    s = mixed_state(3000, 21)
+    n = len(s["px"])
+    say("[parity] dead/holes contract")
+    solver = Solver(theta=0.5, walk="fp32")
+    ds = DeviceState.from_host(s)
+    for t in (ds.ax, ds.ay, ds.az):
+        t.fill(7.0)
+    solver.accelerations(ds, Params(softening=0.01))
+    acc = ds.acc_host()
+    dead = s["alive"] == 0
+    holes = (s["kind"] == ics.KIND_HOLE) & ~dead
+    assert (acc[dead] == 7.0).all()
+    assert (acc[holes] == 0.0).all()
+    assert np.isfinite(acc).all()
+
+
+def test_empty_and_single(gpu):
+    for n in (0, 1):
+        say(f"[parity] n={n}")
+        s = {"px": np.zeros(n), "py": np.zeros(n), "pz": np.zeros(n), "m": np.ones(n), "kind": np.zeros(n, np.uint8)}
+        for walk in ("fp64", "fp32"):
+            if n == 0:
+                solver = Solver(walk=walk)
+                solver.accelerations_host(s["px"], s["py"], s["pz"], s["m"])
+                continue
+            acc = run_gpu(s, 0.5, 0.01, walk)["acc"]
+            assert np.array_equal(acc, np.zeros((1, 3)))
+    # only holes: no tree at all
+    s = {"px": np.arange(3.0), "py": np.zeros(3), "pz": np.zeros(3), "m": np.ones(3), "kind": np.ones(3, np.uint8)}
+    acc = run_gpu(s, 0.5, 0.01, "fp32")["acc"]
+    assert np.array_equal(acc, np.zeros((3, 3)))
+
+
+def test_host_pointer_entry_matches_device_entry(gpu):
+    s = ics.plummer(4000, 31)
+    say("[parity] host-pointer entry")
+    solver = Solver(theta=0.5, walk="fp64")
+    acc_h = solver.accelerations_host(s["px"], s["py"], s["pz"], s["m"], eps=0.01)
+    acc_d = run_gpu(s, 0.5, 0.01, "fp64")["acc"]
+    assert np.array_equal(acc_h, acc_d)
+
+
+def test_direct_sample_matches_oracle(gpu):
+    s = mixed_state(6000, 41)
+    targets = ics.sample_targets(len(s["px"]), 500, 3)
+    say("[parity] FP64 direct-sum sample")
+    solver = Solver()
+    ds = DeviceState.from_host(s)
+    got = solver.direct_sample(ds, Params(softening=0.01), targets).numpy()
+    want = oracle.direct(s["px"], s["py"], s["pz"], s["m"], s["kind"], eps=0.01, targets=targets, workers=8)
+    scale = np.linalg.norm(want, axis=1).max()
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-13 * scale)
+
+
+def test_tiny_theta_equals_direct_sum(gpu):
+    """bh_test.go:88-95: theta 1e-9 opens every node; equals direct sum to 1e-10."""
+    s = ics.plummer(3000, 51)
+    say("[parity] tiny theta vs direct")
+    solver = Solver(theta=1e-9, walk="fp64")
+    ds = DeviceState.from_host(s)
+    solver.accelerations(ds, Params(softening=0.01))
+    got = ds.acc_host()
+    want = oracle.direct(s["px"], s["py"], s["pz"], s["m"], s["kind"], eps=0.01, workers=8)
+    assert ics.rel_errors(got, want).max() <= 1e-10
+
+
+def test_potential_energy_matches_oracle(gpu):
+    s = ics.plummer(5000, 61)
+    say("[parity] potential energy")
+    solver = Solver(theta=0.5)
+    ds = DeviceState.from_host(s)
+    u = solver.potential_energy(ds, Params(softening=0.01))
+    o = oracle.Solver(theta=0.5)
+    want = o.potential_energy(s["px"], s["py"], s["pz"], s["m"], s["kind"], eps=0.01)
+    assert abs(u - want) <= 1e-12 * abs(want)
+    exact = oracle.potential_exact(s["px"], s["py"], s["pz"], s["m"], s["kind"], eps=0.01)
+    assert abs(u - exact) <= 1e-3 * abs(exact)
+
+
+def test_repeat_is_deterministic(gpu):
+    s = ics.clumps(20000, 8, 71, scale=0.01)
+    say("[parity] determinism")
+    a = run_gpu(s, 0.5, 1e-3, "fp32")["acc"]
+    b = run_gpu(s, 0.5, 1e-3, "fp32")["acc"]
+    assert np.array_equal(a, b)
