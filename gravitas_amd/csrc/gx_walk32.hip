@@ -27,7 +27,7 @@
 
 namespace gx {
 
-constexpr int kStack = 512;
+constexpr int kStack = 768;   // stack entries: (first child, child count, lane mask)
 constexpr int kCells = 64;
 constexpr int kParts = 256;
 constexpr double kMargin = 1e-12;  // >> FP64 rounding of d2; box tests stay conservative
@@ -38,8 +38,16 @@ struct WalkLDS {
     uint64_t cellMask[kCells];
     uint64_t partMask[kParts];
     uint64_t stkMask[kStack];
-    int stkNode[kStack];
+    uint32_t stkRange[kStack];  // first | (count - 1) << 28
+    uint64_t popMask[kWave];
+    int popNode[kWave];
 };
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
     float y = v - c;
@@ -132,24 +140,44 @@ __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos,
     int sp = 1, ncell = 0, npart = 0, maxsp = 1;
     bool overflow = false;
     if (lane == 0) {
-        L.stkNode[0] = 0;
+        L.stkRange[0] = 0u;  // root, one node
         L.stkMask[0] = gmask;
     }
     __syncthreads();
 
     while (sp > 0) {
-        int K = min(64, sp);
-        if (sp + 7 * K > kStack) K = max(1, (kStack - sp) / 7);
-        const int base = sp - K;
-        const bool has = lane < K;
+        // pop up to 64 nodes from the top ranges (top entry first)
+        int cnt = 0;
+        uint32_t rng = 0;
+        uint64_t rmask = 0;
+        if (lane < sp) {
+            rng = L.stkRange[sp - 1 - lane];
+            rmask = L.stkMask[sp - 1 - lane];
+            cnt = (int)(rng >> 28) + 1;
+        }
+        const int incl0 = wave_incl_scan(cnt);
+        const int excl0 = incl0 - cnt;
+        const int take = max(0, min(cnt, kWave - excl0));
+        const int rfirst = (int)(rng & 0x0fffffffu);
+        for (int j = 0; j < take; j++) {
+            L.popNode[excl0 + j] = rfirst + j;
+            L.popMask[excl0 + j] = rmask;
+        }
+        const uint64_t full = __ballot(cnt > 0 && take == cnt);
+        const int nfull = __popcll(full);
+        const int npop = min(kWave, __shfl(incl0, 63, kWave));
+        // a partly taken range stays on the stack with its remaining children
+        if (lane == nfull && take > 0 && take < cnt)
+            L.stkRange[sp - 1 - lane] = (uint32_t)(rfirst + take) | ((uint32_t)(cnt - take - 1) << 28);
+        sp -= nfull;
+        __syncthreads();
+        const bool has = lane < npop;
         int node = 0;
         uint64_t mask = 0;
         if (has) {
-            node = L.stkNode[base + lane];
-            mask = L.stkMask[base + lane];
+            node = L.popNode[lane];
+            mask = L.popMask[lane];
         }
-        __syncthreads();
-        sp = base;
         TravNode t;
         t.x = t.y = t.z = 0;
         t.crit2 = 0;
@@ -180,9 +208,10 @@ __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos,
         while (mixed) {
             const int src = __builtin_ctzll(mixed);
             mixed &= mixed - 1;
-            const double nx = __shfl(t.x, src, kWave), ny = __shfl(t.y, src, kWave), nz = __shfl(t.z, src, kWave);
-            const double ncrit2 = __shfl(t.crit2, src, kWave);
-            const uint64_t smask = uniform64(__shfl((unsigned long long)mask, src, kWave));
+            const double nx = readlane_d(t.x, src), ny = readlane_d(t.y, src), nz = readlane_d(t.z, src);
+            const double ncrit2 = readlane_d(t.crit2, src);
+            const uint64_t smask = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(mask >> 32), src) << 32) |
+                                   (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mask, src);
             // the reference's per-star opening test, bh.go:114-116 (FP64, unfused)
             const double dx = nx - p.x, dy = ny - p.y, dz = nz - p.z;
             const double d2 = dx * dx + dy * dy + dz * dz;
@@ -218,22 +247,21 @@ __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos,
         }
         ncell += nc;
 
-        // opened internal nodes -> children on the stack (octant order kept)
+        // opened internal nodes -> one stack entry with the whole child range
         const bool openInt = omask != 0 && t.first >= 0;
-        const int nch = openInt ? t.nchild : 0;
-        const int incl = wave_incl_scan(nch);
-        const int total = __shfl(incl, 63, kWave);
-        if (sp + total > kStack) {
+        const uint64_t pb = __ballot(openInt);
+        const int npush = __popcll(pb);
+        if (sp + npush > kStack) {
             overflow = true;
             break;
         }
-        // push so that the first child ends on top (pops take the top entries)
-        for (int c = 0; c < nch; c++) {
-            const int slot = sp + (total - incl) + (nch - 1 - c);
-            L.stkNode[slot] = t.first + c;
+        if (openInt) {
+            // later lanes end lower, so lane 0's children are popped first
+            const int slot = sp + npush - 1 - mbcnt(pb);
+            L.stkRange[slot] = (uint32_t)t.first | ((uint32_t)(t.nchild - 1) << 28);
             L.stkMask[slot] = omask;
         }
-        sp += total;
+        sp += npush;
         maxsp = max(maxsp, sp);
 
         // opened leaves -> particle list (large leaves are split over flushes)

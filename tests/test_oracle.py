@@ -1,0 +1,323 @@
+"""Pins the CPU oracle (oracle/bh_oracle.c) to what the reference's own tests
+and the mathematics fix, so that a plausible restatement mistake (a dropped
+quadrupole term, a wrong octant bit, a transposed index, a lost self-skip)
+fails here.  The reference holds no golden vectors for this path; its single
+exact known answer is the two-star case (bh_test.go:137-147).  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from gravitas_amd import ics
+
+# ---------------------------------------------------------------------------
+# keys and sort
+# ---------------------------------------------------------------------------
+
+
+def test_morton_key_interleaves_bits():
+    """bh_test.go:16-31: naive bit loop, x in bit 3b+2, y in 3b+1, z in 3b."""
+    rng = np.random.default_rng(12)
+    for _ in range(1000):
+        x, y, z = (int(v) for v in rng.integers(0, 1 << 21, 3))
+        want = 0
+        for b in range(21):
+            want |= ((x >> b) & 1) << (3 * b + 2) | ((y >> b) & 1) << (3 * b + 1) | ((z >> b) & 1) << (3 * b)
+        assert oracle.morton_key(x, y, z) == want
+
+
+def test_radix_sort_is_the_stable_sort():
+    """bh_test.go:33-50 and :353-367: sorted, consistent permutation, stable
+    (equal keys keep their input order), with many duplicates."""
+    rng = np.random.default_rng(34)
+    for keys in (rng.integers(0, 1 << 62, 5000, dtype=np.uint64),
+                 (rng.integers(0, 1 << 20, 200000, dtype=np.uint64) << np.uint64(40))):
+        k, idx = oracle.radix_sort(keys)
+        assert np.all(k[1:] >= k[:-1])
+        assert np.array_equal(keys[idx], k)
+        assert np.array_equal(idx, np.argsort(keys, kind="stable"))
+
+
+# ---------------------------------------------------------------------------
+# tree: brute-force geometric definition
+# ---------------------------------------------------------------------------
+
+
+def brute_tree(px, py, pz, m, theta):
+    """Octree built from the geometry alone: a cube around the star bounding
+    box (anchored at its min corner, side padded by 1e-9 as in build.go:19-21),
+    split while it holds more than 24 stars and level < 21, children in octant
+    order (bit 2 = upper x half, bit 1 = y, bit 0 = z).  Moments are direct
+    sums over each node's stars.  Returns nodes in traversal order."""
+    side = max(px.max() - px.min(), py.max() - py.min(), pz.max() - pz.min())
+    side = max(side * (1 + 1e-9), 1e-12)
+    half = side / 2
+    c0 = (px.min() + half, py.min() + half, pz.min() + half)
+    out = []
+
+    def rec(idx, c, h, level):
+        mm = m[idx].sum()
+        if mm > 0:
+            com = np.array([(m[idx] * px[idx]).sum(), (m[idx] * py[idx]).sum(), (m[idx] * pz[idx]).sum()]) / mm
+        else:
+            com = np.array(c)
+        d = np.stack([px[idx], py[idx], pz[idx]], axis=1) - com
+        r2 = (d * d).sum(axis=1)
+        Q = (3 * np.einsum("i,ij,ik->jk", m[idx], d, d) - np.eye(3) * (m[idx] * r2).sum())
+        crit = 2 * h / theta + math.dist(com, c)
+        node = dict(count=len(idx), com=com, m=mm, Q=Q, crit2=crit * crit, leaf=True)
+        out.append(node)
+        if len(idx) > 24 and level < 21:
+            node["leaf"] = False
+            ux = px[idx] >= c[0]
+            uy = py[idx] >= c[1]
+            uz = pz[idx] >= c[2]
+            octs = ux * 4 + uy * 2 + uz * 1
+            q = h / 2
+            for o in range(8):
+                sub = idx[octs == o]
+                if len(sub) == 0:
+                    continue
+                cc = (c[0] + (q if o & 4 else -q), c[1] + (q if o & 2 else -q), c[2] + (q if o & 1 else -q))
+                rec(sub, cc, q, level + 1)
+
+    rec(np.arange(len(px)), c0, half, 0)
+    return out
+
+
+def dfs_sequence(nodes):
+    seq, i = [], 0
+    while i >= 0:
+        seq.append(i)
+        i = int(nodes["first"][i]) if nodes["first"][i] >= 0 else int(nodes["next"][i])
+    return seq
+
+
+@pytest.mark.parametrize("n,theta,seed", [(3000, 0.5, 1), (5000, 0.9, 2), (700, 0.3, 3)])
+def test_tree_matches_geometric_definition(n, theta, seed):
+    s = ics.plummer(n, seed)
+    px, py, pz, m = s["px"], s["py"], s["pz"], s["m"] * (1 + np.arange(n) % 3)  # unequal masses
+    o = oracle.Solver(theta=theta)
+    o.build(px, py, pz, m)
+    nodes = o.nodes()
+    seq = dfs_sequence(nodes)
+    assert len(seq) == o.n_nodes  # every node reached exactly once
+    want = brute_tree(px, py, pz, m, theta)
+    assert len(want) == len(seq)
+    scale_q = max(abs(w["Q"]).max() for w in want)
+    for w, i in zip(want, seq):
+        nd = nodes[i]
+        assert nd["hi"] - nd["lo"] == w["count"]
+        assert (nd["first"] < 0) == w["leaf"]
+        assert np.allclose([nd["x"], nd["y"], nd["z"]], w["com"], rtol=0, atol=1e-12)
+        assert math.isclose(nd["m"], w["m"], rel_tol=1e-12)
+        q = [[nd["qxx"], nd["qxy"], nd["qxz"]], [nd["qxy"], nd["qyy"], nd["qyz"]], [nd["qxz"], nd["qyz"], nd["qzz"]]]
+        assert np.allclose(q, w["Q"], rtol=1e-9, atol=1e-11 * scale_q)
+        assert abs(nd["qxx"] + nd["qyy"] + nd["qzz"]) <= 1e-10 * scale_q  # traceless
+        assert math.isclose(nd["crit2"], w["crit2"], rel_tol=1e-12)
+    # skip links: a node's next is its next sibling or its parent's next
+    for i in range(o.n_nodes):
+        f = nodes["first"][i]
+        if f < 0:
+            continue
+        kids = [f]
+        while nodes["next"][kids[-1]] == kids[-1] + 1 and nodes["lo"][kids[-1] + 1] == nodes["hi"][kids[-1]] \
+                and nodes["hi"][kids[-1]] < nodes["hi"][i]:
+            kids.append(kids[-1] + 1)
+        assert nodes["lo"][kids[0]] == nodes["lo"][i] and nodes["hi"][kids[-1]] == nodes["hi"][i]
+        assert nodes["next"][kids[-1]] == nodes["next"][i]
+
+
+def test_sorted_copies_and_keys_are_consistent():
+    """Every star lies inside its leaf's cube; keys ascend; state order kept
+    among equal keys; dark matter with DarkMatterOff has source mass 0."""
+    s = ics.disk_halo(6000, 5)
+    o = oracle.Solver(theta=0.6)
+    o.build(s["px"], s["py"], s["pz"], s["m"], s["kind"], dm_off=True)
+    keys, order = o.keys(), o.order()
+    assert np.all(keys[1:] >= keys[:-1])
+    eq = keys[1:] == keys[:-1]
+    assert np.all(order[1:][eq] > order[:-1][eq])
+    sx, sy, sz, sm = o.sorted()
+    assert np.array_equal(sx, s["px"][order])
+    assert np.array_equal(sm, np.where(s["kind"][order] == ics.KIND_DARK, 0.0, s["m"][order]))
+
+
+# ---------------------------------------------------------------------------
+# walk
+# ---------------------------------------------------------------------------
+
+
+def numpy_direct(px, py, pz, m, eps):
+    p = np.stack([px, py, pz], axis=1)
+    d = p[None, :, :] - p[:, None, :]
+    r2 = (d * d).sum(-1) + eps * eps
+    np.fill_diagonal(r2, np.inf)
+    f = m[None, :] / (r2 * np.sqrt(r2))
+    return (f[:, :, None] * d).sum(1)
+
+
+def test_two_stars_known_answer():
+    """bh_test.go:137-147."""
+    acc = oracle.Solver().accelerations(np.array([1.0, 2.0, 0.0]), np.zeros(3), np.zeros(3),
+                                        np.array([1.0, 2.0, 4.0]), np.array([0, 0, 1], np.uint8))
+    assert np.array_equal(acc, [[2.0, 0, 0], [-1.0, 0, 0], [0, 0, 0]])
+
+
+def test_tiny_theta_equals_direct_sum():
+    """bh_test.go:88-95, against an independent numpy pair sum."""
+    s = ics.plummer(1500, 4)
+    got = oracle.Solver(theta=1e-9).accelerations(s["px"], s["py"], s["pz"], s["m"], eps=0.01)
+    want = numpy_direct(s["px"], s["py"], s["pz"], s["m"], 0.01)
+    assert ics.rel_errors(got, want).max() <= 1e-10
+
+
+def test_direct_sum_matches_numpy_and_conserves_momentum():
+    s = ics.plummer(1200, 6)
+    m = s["m"] * (0.5 + np.random.default_rng(1).random(1200))
+    got = oracle.direct(s["px"], s["py"], s["pz"], m, eps=0.05, workers=4)
+    want = numpy_direct(s["px"], s["py"], s["pz"], m, 0.05)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-14 * np.abs(want).max())
+    force = (m[:, None] * got).sum(0)
+    assert np.abs(force).max() <= 1e-12 * (m[:, None] * np.abs(got)).sum()
+
+
+def test_default_theta_accuracy():
+    """bh_test.go:124-135 bounds (median <= 4e-3, max <= 6e-2) at theta 0.9.
+    As there (withHolePull, bh_test.go:97-112), errors are relative to the
+    total force a star feels, which includes a central black hole of mass 2
+    that the solver itself leaves to the integrator."""
+    s = ics.disk_halo(5000, 7)
+    got = oracle.Solver().accelerations(s["px"], s["py"], s["pz"], s["m"], eps=0.02)
+    want = numpy_direct(s["px"], s["py"], s["pz"], s["m"], 0.02)
+    p = np.stack([s["px"], s["py"], s["pz"]], axis=1)
+    r2 = (p * p).sum(1) + 0.02 ** 2
+    hole = -2.0 * p / (r2 * np.sqrt(r2))[:, None]
+    e = np.sort(ics.rel_errors(got + hole, want + hole))
+    assert e[len(e) // 2] <= 4e-3 and e[-1] <= 6e-2
+
+
+def far_cluster():
+    """bh_test.go:294-307 in spirit: a clumpy cluster near (10, 3, -2), a
+    probe star at the origin."""
+    rng = np.random.default_rng([5, 6])
+    n = 400
+    i = np.arange(n)
+    px = np.r_[0.0, 10 + rng.standard_normal(n) * 0.4 + 0.5 * (i % 2)]
+    py = np.r_[0.0, 3 + rng.standard_normal(n) * 0.2 + 0.3 * (i % 3)]
+    pz = np.r_[0.0, -2 + rng.standard_normal(n) * 0.3]
+    m = np.r_[1e-9, 0.5 + rng.random(n)]
+    return px, py, pz, m
+
+
+def test_quadrupole_improves_far_field():
+    """bh_test.go:309-330: quadrupole error at least 5x below monopole."""
+    px, py, pz, m = far_cluster()
+    want = numpy_direct(px, py, pz, m, 1e-3)[0]
+    err = {}
+    for mono in (True, False):
+        got = oracle.Solver(theta=0.95, monopole_only=mono).accelerations(px, py, pz, m, eps=1e-3)[0]
+        err[mono] = np.linalg.norm(got - want) / np.linalg.norm(want)
+    assert err[False] <= err[True] / 5
+
+
+def test_quadrupole_force_is_gradient_of_potential():
+    """bh_test.go:332-351: a = grad(psi) with psi from walkPotential."""
+    px, py, pz, m = far_cluster()
+    o = oracle.Solver(theta=0.95)
+    o.build(px, py, pz, m)
+    eps2, h = 1e-6, 1e-6
+    x, y, z = 0.0, 0.5, 0.2
+    a = o.walk(x, y, z, -1, eps2)
+    g = [(o.walk_potential(x + h, y, z, -1, eps2) - o.walk_potential(x - h, y, z, -1, eps2)) / (2 * h),
+         (o.walk_potential(x, y + h, z, -1, eps2) - o.walk_potential(x, y - h, z, -1, eps2)) / (2 * h),
+         (o.walk_potential(x, y, z + h, -1, eps2) - o.walk_potential(x, y, z - h, -1, eps2)) / (2 * h)]
+    assert np.allclose(a, g, rtol=0, atol=1e-6 * abs(a[0]))
+
+
+def test_coincident_stars_do_not_recurse_forever():
+    """bh_test.go:149-161."""
+    px = np.r_[np.ones(100), 0.0]
+    py = np.r_[np.full(100, 2.0), 0.0]
+    pz = np.r_[np.full(100, 3.0), 0.0]
+    o = oracle.Solver()
+    acc = o.accelerations(px, py, pz, np.ones(101), eps=0.01)
+    assert np.isfinite(acc).all()
+    depth_chain = o.n_nodes
+    assert depth_chain >= 21  # single-child chain down to level 21
+
+
+def test_threads_do_not_change_results():
+    """bh_test.go:192-199: Workers 1 vs 8 bit-identical."""
+    s = ics.disk_halo(40000, 8)
+    a = oracle.Solver(theta=0.6, workers=1).accelerations(s["px"], s["py"], s["pz"], s["m"], eps=0.02)
+    b = oracle.Solver(theta=0.6, workers=8).accelerations(s["px"], s["py"], s["pz"], s["m"], eps=0.02)
+    assert np.array_equal(a, b)
+
+
+def test_holes_dead_and_dark_matter_contract():
+    s = ics.concat(ics.plummer(500, 9), ics.hernquist_halo(500, 10))
+    s["kind"][7] = ics.KIND_HOLE
+    alive = np.ones(1000, np.uint8)
+    alive[11] = 0
+    out = tuple(np.full(1000, 5.0) for _ in range(3))
+    acc = oracle.Solver(theta=0.5).accelerations(s["px"], s["py"], s["pz"], s["m"], s["kind"], alive, eps=0.01,
+                                                 dm_off=True, out=out)
+    assert np.array_equal(acc[7], [0, 0, 0])
+    assert np.array_equal(acc[11], [5.0, 5.0, 5.0])
+    # with dark matter off, only the 499 live stars (not the hole) pull
+    keep = (s["kind"] == ics.KIND_STAR) & (alive == 1)
+    mm = np.where(keep, s["m"], 0.0)
+    want = oracle.direct(s["px"], s["py"], s["pz"], mm, eps=0.01)
+    live = (alive == 1) & (s["kind"] != ics.KIND_HOLE)
+    tight = oracle.Solver(theta=1e-9).accelerations(s["px"], s["py"], s["pz"], s["m"], s["kind"], alive, eps=0.01,
+                                                    dm_off=True)
+    assert ics.rel_errors(tight[live], want[live]).max() <= 1e-10
+
+
+# ---------------------------------------------------------------------------
+# potential and integrator
+# ---------------------------------------------------------------------------
+
+
+def test_potential_energy_matches_exact():
+    """bh_test.go:280-292: tiny theta to 1e-10; default theta to 1e-3."""
+    s = ics.plummer(2000, 11)
+    exact = oracle.potential_exact(s["px"], s["py"], s["pz"], s["m"], eps=0.02)
+    p = s["px"], s["py"], s["pz"], s["m"]
+    tiny = oracle.Solver(theta=1e-9).potential_energy(*p, eps=0.02)
+    assert abs(tiny - exact) <= 1e-10 * abs(exact)
+    dflt = oracle.Solver().potential_energy(*p, eps=0.02)
+    assert abs(dflt - exact) <= 1e-3 * abs(exact)
+    # closed form for two bodies: U = -m1 m2 / sqrt(r^2 + eps^2)
+    u2 = oracle.potential_exact(np.array([0.0, 3.0]), np.zeros(2), np.array([0.0, 4.0]), np.array([2.0, 5.0]),
+                                eps=0.0)
+    assert u2 == -10.0 / 5.0
+
+
+def test_kepler_orbit_conserves_energy():
+    """direct_test.go:26-63 (shortened to 20 orbits): KDK with the direct sum."""
+    e = 0.3
+    m1, m2 = 0.999, 0.001
+    rp = 1 - e
+    vp = math.sqrt((1 + e) / rp)
+    st = {"px": np.array([-rp * m2, rp * m1]), "py": np.zeros(2), "pz": np.zeros(2),
+          "vx": np.zeros(2), "vy": np.array([-vp * m2, vp * m1]), "vz": np.zeros(2)}
+    m = np.array([m1, m2])
+    acc = lambda x, y, z: oracle.direct(x, y, z, m, eps=0.0)
+    st["acc"] = acc(st["px"], st["py"], st["pz"])
+
+    def energy():
+        k = 0.5 * (m * (st["vx"] ** 2 + st["vy"] ** 2 + st["vz"] ** 2)).sum()
+        return k + oracle.potential_exact(st["px"], st["py"], st["pz"], m, eps=0.0)
+
+    e0 = energy()
+    dt = 2 * math.pi / 2000
+    drift = 0.0
+    for i in range(2000 * 20):
+        oracle.kick_drift_kick(st, acc, dt)
+        if i % 100 == 0:
+            drift = max(drift, abs((energy() - e0) / e0))
+    assert drift <= 1e-4
+    assert abs(math.hypot(st["px"][1], st["py"][1]) - 0.7) <= 0.05
