@@ -152,6 +152,11 @@ def main():
         "interactions_per_s": (npp + npc) / (ms_per_step * 1e-3),
         "interactions_per_particle": {"pp": npp / n, "pc": npc / n},
         "tree": {"nodes": st["n_nodes"], "depth": st["depth"], "max_stack": st["max_stack"]},
+        "walk_lists": {"groups": (n + 63) // 64, "cell_entries_per_group": st["list_entries"] / ((n + 63) // 64),
+                       "cell_lane_utilization": npc / max(1, 64 * st["list_entries"]),
+                       "rounds_per_group": st["rounds"] / ((n + 63) // 64),
+                       "mixed_nodes_per_group": st["mixed_nodes"] / ((n + 63) // 64),
+                       "max_list": st["max_list"]},
         "phases_ms": phase_avg,
         "roofline": {"bound": "valu", "kernel": "k_walk32", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,

@@ -35,7 +35,9 @@ class GxStats(ctypes.Structure):
         ("n_stars", ctypes.c_int64), ("n_holes", ctypes.c_int64), ("n_nodes", ctypes.c_int64),
         ("pp_interactions", ctypes.c_int64), ("pc_interactions", ctypes.c_int64),
         ("max_stack", ctypes.c_int32), ("flags", ctypes.c_int32), ("depth", ctypes.c_int32),
-        ("walk", ctypes.c_int32), ("fallbacks", ctypes.c_int64),
+        ("walk", ctypes.c_int32), ("fallbacks", ctypes.c_int64), ("list_entries", ctypes.c_int64),
+        ("rounds", ctypes.c_int64), ("mixed_nodes", ctypes.c_int64),
+        ("max_list", ctypes.c_int32 * 4),
     ]
 
 

@@ -8,6 +8,7 @@
 // the Meta record; grids are sized from the body count n.
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -57,7 +58,7 @@ int ensure(DevBuf& b, size_t bytes) {
     return GX_OK;
 }
 
-constexpr int kPhases = 8;
+constexpr int kPhases = 9;
 constexpr int kTreeGrid = 1024;
 
 }  // namespace
@@ -74,6 +75,8 @@ struct gx_solver {
     // arena
     DevBuf keys, keys2, order, order2, holes, hist, digitTotal, blockStar, blockHole, blockBox, spos;
     DevBuf trav, m64, m32, meta, partial, targets;
+    DevBuf lfull, lpart, lleaf, lcounts;
+    int cf = 2048, cp = 512, cl = 256;  // FP32-walk list capacities per group (grow on demand)
     long long capNodes = 0;
     Meta* hmeta = nullptr;  // pinned mirror of the device Meta
     hipEvent_t ev[kPhases + 1] = {};
@@ -84,6 +87,10 @@ struct gx_solver {
     // the pending step (for gx_sync)
     bool pending = false;
     long long pendingN = 0;
+    int lastWalk = GX_WALK_FP32;
+    // FP32 walk as two kernels (traversal, evaluation) instead of one; set
+    // from GX_SPLIT_WALK=1 for profiling the two phases separately.
+    bool splitWalk = false;
 };
 
 extern "C" {
@@ -140,7 +147,9 @@ gx_solver* gx_solver_new(void) {
     }
     memset(s->hmeta, 0, sizeof(Meta));
     s->stream = s->own;
-    for (auto& e : s->ev) hipEventCreate(&e);
+    for (auto& e : s->ev) (void)hipEventCreate(&e);
+    const char* sw = getenv("GX_SPLIT_WALK");
+    s->splitWalk = sw && sw[0] == '1';
     return s;
 }
 
@@ -151,7 +160,7 @@ void gx_solver_free(gx_solver* s) {
                       &s->ax,    &s->ay,        &s->az,        &s->keys,     &s->keys2,     &s->order,
                       &s->order2, &s->holes,    &s->hist,      &s->digitTotal, &s->blockStar, &s->blockHole,
                       &s->blockBox, &s->spos,   &s->trav,      &s->m64,      &s->m32,       &s->meta,
-                      &s->partial, &s->targets};
+                      &s->partial, &s->targets, &s->lfull, &s->lpart, &s->lleaf, &s->lcounts};
     for (DevBuf* b : bufs)
         if (b->p) hipFree(b->p);
     for (auto& e : s->ev)
@@ -273,8 +282,22 @@ int enqueue_walk(gx_solver* s, int walk, long long n, double eps, double* ax, do
     Meta* meta = (Meta*)s->meta.p;
     const double eps2 = eps * eps;
     if (walk == GX_WALK_FP32) {
+        const long long ng = std::max(1LL, (n + 63) / 64);
+        int rc;
+        if ((rc = ensure(s->lfull, (size_t)ng * s->cf * 4)) || (rc = ensure(s->lpart, (size_t)ng * s->cp * 16)) ||
+            (rc = ensure(s->lleaf, (size_t)ng * s->cl * 16)) || (rc = ensure(s->lcounts, (size_t)ng * 16)))
+            return rc;
+        Lists lists;
+        lists.full = (uint32_t*)s->lfull.p;
+        lists.part = (uint4*)s->lpart.p;
+        lists.leaf = (uint4*)s->lleaf.p;
+        lists.counts = (int4*)s->lcounts.p;
+        lists.cf = s->cf;
+        lists.cp = s->cp;
+        lists.cl = s->cl;
         launch_walk32((const double4*)s->spos.p, (const TravNode*)s->trav.p, (const Mult32*)s->m32.p,
-                      (const int32_t*)s->order.p, meta, (n + 63) / 64, (float)eps2, !s->monopole, ax, ay, az, st);
+                      (const int32_t*)s->order.p, meta, (n + 63) / 64, (float)eps2, !s->monopole, lists, ax, ay, az,
+                      st, s->timing ? s->ev[7] : nullptr, s->splitWalk);
     } else {
         hipLaunchKernelGGL(k_walk64, dim3(blocks(n, 256)), dim3(256), 0, st, (const double4*)s->spos.p,
                            (const TravNode*)s->trav.p, (const Mult64*)s->m64.p, (const int32_t*)s->order.p, meta,
@@ -294,6 +317,7 @@ int enqueue_step(gx_solver* s, const double* x, const double* y, const double* z
     if (rc) return rc;
     if ((rc = enqueue_walk(s, s->walk, n, eps, ax, ay, az))) return rc;
     hipEventRecord(s->ev[kPhases], s->stream);
+    s->lastWalk = s->walk;
     HIPCHK(hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, s->stream));
     s->pending = true;
     s->pendingN = n;
@@ -313,6 +337,13 @@ int finish_step(gx_solver* s) {
         for (int i = 0; i < 6; i++) hipEventElapsedTime(&s->phase[i], s->ev[i], s->ev[i + 1]);
         hipEventElapsedTime(&s->phase[6], s->ev[6], s->ev[kPhases]);
         s->phase[7] = total;
+        s->phase[8] = 0;
+        if (s->lastWalk == GX_WALK_FP32 && s->splitWalk) {  // traversal | evaluation split
+            float trav = 0;
+            hipEventElapsedTime(&trav, s->ev[6], s->ev[7]);
+            s->phase[8] = s->phase[6] - trav;
+            s->phase[6] = trav;
+        }
     }
     int depth = 0;
     for (int l = 0; l < kMaxLevels; l++)
@@ -327,6 +358,11 @@ int finish_step(gx_solver* s) {
     s->stats.flags = h.flags;
     s->stats.depth = depth;
     s->stats.fallbacks = s->fallbacks;
+    s->stats.list_entries = (int64_t)h.listEntries;
+    s->stats.rounds = (int64_t)h.rounds;
+    s->stats.mixed_nodes = (int64_t)h.mixed;
+    for (int i = 0; i < 3; i++) s->stats.max_list[i] = h.maxList[i];
+    s->stats.max_list[3] = 0;
     if (h.flags & kFlagNodeOverflow)
         return set_err(GX_ERR_CAPACITY, "node arena of %lld nodes too small", s->capNodes);
     return GX_OK;
@@ -343,6 +379,14 @@ int run_dev(gx_solver* s, const double* x, const double* y, const double* z, con
             continue;
         }
         if (rc) return rc;
+        if (s->hmeta->flags & kFlagListOverflow) {
+            // some group's interaction list did not fit: grow and redo the walk
+            const Meta& h = *s->hmeta;
+            s->cf = std::max(s->cf, (int)(h.maxList[0] * 1.25) + 64);
+            s->cp = std::max(s->cp, (int)(h.maxList[1] * 1.25) + 64);
+            s->cl = std::max(s->cl, (int)(h.maxList[2] * 1.25) + 64);
+            continue;
+        }
         if (s->hmeta->flags & kFlagStackOverflow) {
             // the FP32 walk's LDS stack was too small for this tree: redo the
             // walk with the FP64 per-particle walk (same interaction set)
@@ -388,8 +432,8 @@ int gx_sync(gx_solver* s) {
     }
     int rc = finish_step(s);
     if (rc) return rc;
-    if (s->hmeta->flags & kFlagStackOverflow)
-        return set_err(GX_ERR_CAPACITY, "walk stack overflow; use gx_accelerations_dev");
+    if (s->hmeta->flags & (kFlagStackOverflow | kFlagListOverflow))
+        return set_err(GX_ERR_CAPACITY, "walk stack or list overflow; use gx_accelerations_dev");
     return GX_OK;
 }
 

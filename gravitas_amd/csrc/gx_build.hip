@@ -342,6 +342,10 @@ __global__ void k_tree_init(Meta* meta, TravNode* __restrict__ trav) {
     meta->pc = 0;
     meta->flags = 0;
     meta->maxStack = 0;
+    meta->maxList[0] = meta->maxList[1] = meta->maxList[2] = 0;
+    meta->listEntries = 0;
+    meta->rounds = 0;
+    meta->mixed = 0;
     meta->nnodes = 0;
     long long nst = meta->nstars;
     if (nst <= 0) return;

@@ -46,11 +46,15 @@ struct Meta {
     long long levelCount[kMaxLevels + 1];
     unsigned long long pp;  // particle-particle interactions (incl. self pairs)
     unsigned long long pc;  // particle-cell interactions
-    int flags;              // bit 0: node capacity overflow, bit 1: walk stack overflow
+    unsigned long long listEntries;  // cell list entries over all groups (FP32 walk)
+    unsigned long long rounds;       // FP32 walk traversal rounds over all groups
+    unsigned long long mixed;        // nodes decided per star (box test inconclusive)
+    int flags;              // bit 0: node arena, bit 1: walk stack, bit 2: walk lists overflowed
     int maxStack;
+    int maxList[3];         // longest full-cell, partial-cell and leaf list of a group
 };
 
-enum Flag : int { kFlagNodeOverflow = 1, kFlagStackOverflow = 2 };
+enum Flag : int { kFlagNodeOverflow = 1, kFlagStackOverflow = 2, kFlagListOverflow = 4 };
 
 // ---------------------------------------------------------------------------
 // wavefront helpers

@@ -40,8 +40,20 @@ __global__ void k_direct_sample(const double* x, const double* y, const double* 
 __global__ void k_direct_reduce(const double* partial, int nt, int splits, double* out);
 
 // gx_walk32.hip
+// Per-group interaction lists written by the FP32 walk's traversal and read by
+// its evaluation: full-mask cells as bare node indices, partial-mask cells and
+// opened leaves as (node, 0, mask lo, mask hi).  Group g owns entries
+// [g*cap, (g+1)*cap) of each list.
+struct Lists {
+    uint32_t* full;
+    uint4* part;
+    uint4* leaf;
+    int4* counts;  // per group: full, partial, leaves, stack overflow
+    int cf, cp, cl;  // capacities per group
+};
+
 void launch_walk32(const double4* spos, const TravNode* trav, const Mult32* m32, const int32_t* order, Meta* meta,
-                   long long ngroupsUpper, float eps2, int quad, double* ax, double* ay, double* az,
-                   hipStream_t stream);
+                   long long ngroupsUpper, float eps2, int quad, const Lists& lists, double* ax, double* ay,
+                   double* az, hipStream_t stream, hipEvent_t mid, bool split);
 
 }  // namespace gx

@@ -27,21 +27,10 @@
 
 namespace gx {
 
-constexpr int kStack = 768;   // stack entries: (first child, child count, lane mask)
-constexpr int kCells = 64;
-constexpr int kParts = 256;
+constexpr int kStack = 768;   // traversal stack entries: (first child, child count, lane mask)
+constexpr int kChunk = 64;    // cells staged per evaluation chunk
+constexpr int kParts = 256;   // stars staged per evaluation chunk
 constexpr double kMargin = 1e-12;  // >> FP64 rounding of d2; box tests stay conservative
-
-struct WalkLDS {
-    float4 cell[kCells * 3];
-    float4 part[kParts];
-    uint64_t cellMask[kCells];
-    uint64_t partMask[kParts];
-    uint64_t stkMask[kStack];
-    uint32_t stkRange[kStack];  // first | (count - 1) << 28
-    uint64_t popMask[kWave];
-    int popNode[kWave];
-};
 
 __device__ __forceinline__ double readlane_d(double v, int l) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
@@ -56,22 +45,22 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
     s = t;
 }
 
-template <bool QUAD>
-__device__ __forceinline__ void eval_cells(const WalkLDS& L, int ncell, float px, float py, float pz, float eps2,
-                                           float& ax, float& ay, float& az) {
+template <bool QUAD, bool MASKED>
+__device__ __forceinline__ void eval_cells(const float4* __restrict__ cell, const uint64_t* __restrict__ cmask,
+                                           int ncell, float px, float py, float pz, float eps2, float& ax,
+                                           float& ay, float& az) {
     for (int e = 0; e < ncell; e++) {
-        const uint64_t em = uniform64(L.cellMask[e]);
-        const float4 c0 = L.cell[3 * e];
+        const float4 c0 = cell[3 * e];
         const float dx = c0.x - px, dy = c0.y - py, dz = c0.z - pz;
         const float d2e = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
         float ir = __builtin_amdgcn_rsqf(d2e);
-        ir = __builtin_amdgcn_inverse_ballot_w64(em) ? ir : 0.0f;
+        if (MASKED) ir = __builtin_amdgcn_inverse_ballot_w64(uniform64(cmask[e])) ? ir : 0.0f;
         const float inv = ir * ir;
         const float inv3 = inv * ir;
         float f = c0.w * inv3;
         if (QUAD) {
-            const float4 c1 = L.cell[3 * e + 1];
-            const float4 c2 = L.cell[3 * e + 2];
+            const float4 c1 = cell[3 * e + 1];
+            const float4 c2 = cell[3 * e + 2];
             // Q = (qxx, qyy, qzz, qxy) in c1, (qxz, qyz) in c2.xy
             const float qx = fmaf(c1.x, dx, fmaf(c1.w, dy, c2.x * dz));
             const float qy = fmaf(c1.w, dx, fmaf(c1.y, dy, c2.y * dz));
@@ -90,11 +79,12 @@ __device__ __forceinline__ void eval_cells(const WalkLDS& L, int ncell, float px
 }
 
 template <bool EPS0>
-__device__ __forceinline__ void eval_parts(const WalkLDS& L, int npart, float px, float py, float pz, float eps2,
-                                           float& ax, float& ay, float& az) {
+__device__ __forceinline__ void eval_parts(const float4* __restrict__ part, const uint64_t* __restrict__ pmask,
+                                           int npart, float px, float py, float pz, float eps2, float& ax,
+                                           float& ay, float& az) {
     for (int e = 0; e < npart; e++) {
-        const uint64_t em = uniform64(L.partMask[e]);
-        const float4 q = L.part[e];
+        const uint64_t em = uniform64(pmask[e]);
+        const float4 q = part[e];
         const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
         float ir = __builtin_amdgcn_rsqf(r2);
@@ -109,39 +99,80 @@ __device__ __forceinline__ void eval_parts(const WalkLDS& L, int npart, float px
     }
 }
 
-template <bool QUAD, bool EPS0>
-__global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
-                                               const Mult32* __restrict__ m32, const int32_t* __restrict__ order,
-                                               Meta* meta, float eps2, double* __restrict__ oax,
-                                               double* __restrict__ oay, double* __restrict__ oaz) {
-    __shared__ WalkLDS L;
-    const long long nst = meta->nstars;
-    if (meta->flags & kFlagNodeOverflow) return;
+
+// Group prologue shared by both kernels: the group's stars, FP64 bounding box
+// and centre c_g.
+struct Group {
+    long long g, k;
+    bool valid;
+    double4 p;
+    uint64_t gmask;
+    double bminx, bminy, bminz, bmaxx, bmaxy, bmaxz, cgx, cgy, cgz;
+};
+
+__device__ __forceinline__ bool load_group(Group& G, const double4* __restrict__ spos, long long nst) {
     const long long ngroups = (nst + 63) / 64;
     // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so give
     // each XCD a contiguous run of (spatially adjacent) groups for its L2.
     const long long per = gridDim.x / 8;
-    const long long g = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (g >= ngroups) return;
+    G.g = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (G.g >= ngroups) return false;
+    G.k = G.g * 64 + lane_id();
+    G.valid = G.k < nst;
+    G.p = spos[G.valid ? G.k : G.g * 64];
+    G.gmask = __ballot(G.valid);
+    G.bminx = wave_min_d(G.valid ? G.p.x : INFINITY);
+    G.bmaxx = wave_max_d(G.valid ? G.p.x : -INFINITY);
+    G.bminy = wave_min_d(G.valid ? G.p.y : INFINITY);
+    G.bmaxy = wave_max_d(G.valid ? G.p.y : -INFINITY);
+    G.bminz = wave_min_d(G.valid ? G.p.z : INFINITY);
+    G.bmaxz = wave_max_d(G.valid ? G.p.z : -INFINITY);
+    G.cgx = 0.5 * (G.bminx + G.bmaxx);
+    G.cgy = 0.5 * (G.bminy + G.bmaxy);
+    G.cgz = 0.5 * (G.bminz + G.bmaxz);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// traversal: one wavefront per group, lists to HBM
+// ---------------------------------------------------------------------------
+struct TravLDS {
+    uint64_t stkMask[kStack];
+    uint32_t stkRange[kStack];  // first | (count - 1) << 28
+    uint64_t popMask[kWave];
+    int popNode[kWave];
+};
+
+struct EvalLDS {
+    float4 cell[kChunk * 3];
+    uint64_t cmask[kChunk];
+    float4 part[kParts];
+    uint64_t pmask[kParts];
+};
+
+union WalkLDS {
+    TravLDS t;
+    EvalLDS e;
+};
+
+// Writes group G's interaction lists; returns false on a stack overflow.
+__device__ bool trav_group(const Group& G, TravLDS& S, const TravNode* __restrict__ trav, Meta* meta,
+                           const Lists& lists) {
+    uint64_t* stkMask = S.stkMask;
+    uint32_t* stkRange = S.stkRange;
+    uint64_t* popMask = S.popMask;
+    int* popNode = S.popNode;
     const int lane = lane_id();
-    const long long k = g * 64 + lane;
-    const bool valid = k < nst;
-    const double4 p = spos[valid ? k : g * 64];
-    const uint64_t gmask = __ballot(valid);
-
-    const double bminx = wave_min_d(valid ? p.x : INFINITY), bmaxx = wave_max_d(valid ? p.x : -INFINITY);
-    const double bminy = wave_min_d(valid ? p.y : INFINITY), bmaxy = wave_max_d(valid ? p.y : -INFINITY);
-    const double bminz = wave_min_d(valid ? p.z : INFINITY), bmaxz = wave_max_d(valid ? p.z : -INFINITY);
-    const double cgx = 0.5 * (bminx + bmaxx), cgy = 0.5 * (bminy + bmaxy), cgz = 0.5 * (bminz + bmaxz);
-    const float px = (float)(p.x - cgx), py = (float)(p.y - cgy), pz = (float)(p.z - cgz);
-
-    float sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;  // Kahan sums
+    const double4 p = G.p;
+    uint32_t* __restrict__ fullOut = lists.full + (size_t)G.g * lists.cf;
+    uint4* __restrict__ partOut = lists.part + (size_t)G.g * lists.cp;
+    uint4* __restrict__ leafOut = lists.leaf + (size_t)G.g * lists.cl;
     unsigned long long npp = 0, npc = 0;
-    int sp = 1, ncell = 0, npart = 0, maxsp = 1;
+    int nf = 0, np = 0, nl = 0, sp = 1, maxsp = 1, nrounds = 0, nmixed = 0;
     bool overflow = false;
     if (lane == 0) {
-        L.stkRange[0] = 0u;  // root, one node
-        L.stkMask[0] = gmask;
+        stkRange[0] = 0u;  // the root, one node
+        stkMask[0] = G.gmask;
     }
     __syncthreads();
 
@@ -151,8 +182,8 @@ __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos,
         uint32_t rng = 0;
         uint64_t rmask = 0;
         if (lane < sp) {
-            rng = L.stkRange[sp - 1 - lane];
-            rmask = L.stkMask[sp - 1 - lane];
+            rng = stkRange[sp - 1 - lane];
+            rmask = stkMask[sp - 1 - lane];
             cnt = (int)(rng >> 28) + 1;
         }
         const int incl0 = wave_incl_scan(cnt);
@@ -160,44 +191,42 @@ __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos,
         const int take = max(0, min(cnt, kWave - excl0));
         const int rfirst = (int)(rng & 0x0fffffffu);
         for (int j = 0; j < take; j++) {
-            L.popNode[excl0 + j] = rfirst + j;
-            L.popMask[excl0 + j] = rmask;
+            popNode[excl0 + j] = rfirst + j;
+            popMask[excl0 + j] = rmask;
         }
-        const uint64_t full = __ballot(cnt > 0 && take == cnt);
-        const int nfull = __popcll(full);
+        const int nfull = __popcll(__ballot(cnt > 0 && take == cnt));
         const int npop = min(kWave, __shfl(incl0, 63, kWave));
         // a partly taken range stays on the stack with its remaining children
         if (lane == nfull && take > 0 && take < cnt)
-            L.stkRange[sp - 1 - lane] = (uint32_t)(rfirst + take) | ((uint32_t)(cnt - take - 1) << 28);
+            stkRange[sp - 1 - lane] = (uint32_t)(rfirst + take) | ((uint32_t)(cnt - take - 1) << 28);
         sp -= nfull;
+        nrounds++;
         __syncthreads();
         const bool has = lane < npop;
         int node = 0;
         uint64_t mask = 0;
         if (has) {
-            node = L.popNode[lane];
-            mask = L.popMask[lane];
+            node = popNode[lane];
+            mask = popMask[lane];
         }
-        TravNode t;
-        t.x = t.y = t.z = 0;
-        t.crit2 = 0;
-        t.first = -1;
-        t.lo = t.hi = 0;
-        t.nchild = 0;
+        double tx = 0, ty = 0, tz = 0, tc = 0;
+        int tfirst = -1, tnchild = 0, tlo = 0, thi = 0;
         int cls = 0;  // 1 accept for all, 2 open for all, 3 decide per star
         if (has) {
-            t = trav[node];
-            const double ex = fmax(0.0, fmax(bminx - t.x, t.x - bmaxx));
-            const double ey = fmax(0.0, fmax(bminy - t.y, t.y - bmaxy));
-            const double ez = fmax(0.0, fmax(bminz - t.z, t.z - bmaxz));
+            const TravNode t = trav[node];
+            tx = t.x; ty = t.y; tz = t.z; tc = t.crit2;
+            tfirst = t.first; tnchild = t.nchild; tlo = t.lo; thi = t.hi;
+            const double ex = fmax(0.0, fmax(G.bminx - tx, tx - G.bmaxx));
+            const double ey = fmax(0.0, fmax(G.bminy - ty, ty - G.bmaxy));
+            const double ez = fmax(0.0, fmax(G.bminz - tz, tz - G.bmaxz));
             const double dmin2 = ex * ex + ey * ey + ez * ez;
-            const double fx = fmax(fabs(t.x - bminx), fabs(t.x - bmaxx));
-            const double fy = fmax(fabs(t.y - bminy), fabs(t.y - bmaxy));
-            const double fz = fmax(fabs(t.z - bminz), fabs(t.z - bmaxz));
+            const double fx = fmax(fabs(tx - G.bminx), fabs(tx - G.bmaxx));
+            const double fy = fmax(fabs(ty - G.bminy), fabs(ty - G.bmaxy));
+            const double fz = fmax(fabs(tz - G.bminz), fabs(tz - G.bmaxz));
             const double dmax2 = fx * fx + fy * fy + fz * fz;
-            if (dmin2 >= t.crit2 * (1 + kMargin))
+            if (dmin2 >= tc * (1 + kMargin))
                 cls = 1;
-            else if (dmax2 * (1 + kMargin) < t.crit2)
+            else if (dmax2 * (1 + kMargin) < tc)
                 cls = 2;
             else
                 cls = 3;
@@ -205,11 +234,12 @@ __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos,
         uint64_t amask = cls == 1 ? mask : 0;
         uint64_t omask = cls == 2 ? mask : 0;
         uint64_t mixed = __ballot(cls == 3);
+        nmixed += __popcll(mixed);
         while (mixed) {
             const int src = __builtin_ctzll(mixed);
             mixed &= mixed - 1;
-            const double nx = readlane_d(t.x, src), ny = readlane_d(t.y, src), nz = readlane_d(t.z, src);
-            const double ncrit2 = readlane_d(t.crit2, src);
+            const double nx = readlane_d(tx, src), ny = readlane_d(ty, src), nz = readlane_d(tz, src);
+            const double ncrit2 = readlane_d(tc, src);
             const uint64_t smask = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(mask >> 32), src) << 32) |
                                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mask, src);
             // the reference's per-star opening test, bh.go:114-116 (FP64, unfused)
@@ -222,33 +252,34 @@ __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos,
             }
         }
 
-        // accepted cells -> cell list
+        // accepted cells -> full or partial list
         const bool isCell = amask != 0;
-        const uint64_t cb = __ballot(isCell);
-        const int nc = __popcll(cb);
-        if (ncell + nc > kCells) {
-            __syncthreads();
-            float ax = 0, ay = 0, az = 0;
-            eval_cells<QUAD>(L, ncell, px, py, pz, eps2, ax, ay, az);
-            kahan_add(sx, cx, ax);
-            kahan_add(sy, cy, ay);
-            kahan_add(sz, cz, az);
-            ncell = 0;
-            __syncthreads();
+        const bool isFull = isCell && amask == G.gmask;
+        const uint64_t bf = __ballot(isFull);
+        const uint64_t bp = __ballot(isCell && !isFull);
+        if (isFull) {
+            const int pos = nf + mbcnt(bf);
+            if (pos < lists.cf) fullOut[pos] = (uint32_t)node;
+        } else if (isCell) {
+            const int pos = np + mbcnt(bp);
+            if (pos < lists.cp) partOut[pos] = make_uint4((uint32_t)node, 0u, (uint32_t)amask, (uint32_t)(amask >> 32));
         }
-        if (isCell) {
-            const int pos = ncell + mbcnt(cb);
-            const Mult32 mm = m32[node];
-            L.cell[3 * pos] = make_float4((float)(t.x - cgx), (float)(t.y - cgy), (float)(t.z - cgz), mm.m);
-            L.cell[3 * pos + 1] = make_float4(mm.qxx, mm.qyy, mm.qzz, mm.qxy);
-            L.cell[3 * pos + 2] = make_float4(mm.qxz, mm.qyz, 0.0f, 0.0f);
-            L.cellMask[pos] = amask;
-            npc += (unsigned long long)__popcll(amask);
+        if (isCell) npc += (unsigned long long)__popcll(amask);
+        nf += __popcll(bf);
+        np += __popcll(bp);
+
+        // opened leaves -> leaf list
+        const bool isLeaf = omask != 0 && tfirst < 0;
+        const uint64_t bl = __ballot(isLeaf);
+        if (isLeaf) {
+            const int pos = nl + mbcnt(bl);
+            if (pos < lists.cl) leafOut[pos] = make_uint4((uint32_t)node, 0u, (uint32_t)omask, (uint32_t)(omask >> 32));
+            npp += (unsigned long long)__popcll(omask) * (unsigned long long)(thi - tlo);
         }
-        ncell += nc;
+        nl += __popcll(bl);
 
         // opened internal nodes -> one stack entry with the whole child range
-        const bool openInt = omask != 0 && t.first >= 0;
+        const bool openInt = omask != 0 && tfirst >= 0;
         const uint64_t pb = __ballot(openInt);
         const int npush = __popcll(pb);
         if (sp + npush > kStack) {
@@ -258,35 +289,143 @@ __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos,
         if (openInt) {
             // later lanes end lower, so lane 0's children are popped first
             const int slot = sp + npush - 1 - mbcnt(pb);
-            L.stkRange[slot] = (uint32_t)t.first | ((uint32_t)(t.nchild - 1) << 28);
-            L.stkMask[slot] = omask;
+            stkRange[slot] = (uint32_t)tfirst | ((uint32_t)(tnchild - 1) << 28);
+            stkMask[slot] = omask;
         }
         sp += npush;
         maxsp = max(maxsp, sp);
+        __syncthreads();
+    }
 
-        // opened leaves -> particle list (large leaves are split over flushes)
-        const bool openLeaf = omask != 0 && t.first < 0;
-        int lcur = openLeaf ? t.lo : 0;
-        int lcnt = openLeaf ? t.hi - t.lo : 0;
-        if (openLeaf) npp += (unsigned long long)__popcll(omask) * (unsigned long long)lcnt;
+    npp = wave_sum_u64(npp);
+    npc = wave_sum_u64(npc);
+    if (lane == 0) {
+        lists.counts[G.g] = make_int4(nf, np, nl, overflow ? 1 : 0);
+        atomicAdd(&meta->pp, npp);
+        atomicAdd(&meta->pc, npc);
+        atomicMax(&meta->maxStack, maxsp);
+        atomicMax(&meta->maxList[0], nf);
+        atomicMax(&meta->maxList[1], np);
+        atomicMax(&meta->maxList[2], nl);
+        atomicAdd(&meta->listEntries, (unsigned long long)(nf + np));
+        atomicAdd(&meta->rounds, (unsigned long long)nrounds);
+        atomicAdd(&meta->mixed, (unsigned long long)nmixed);
+        if (overflow) atomicOr(&meta->flags, (int)kFlagStackOverflow);
+        if (nf > lists.cf || np > lists.cp || nl > lists.cl) atomicOr(&meta->flags, (int)kFlagListOverflow);
+    }
+    return !overflow && nf <= lists.cf && np <= lists.cp && nl <= lists.cl;
+}
+
+// ---------------------------------------------------------------------------
+// evaluation: one wavefront per group, lists from HBM
+// ---------------------------------------------------------------------------
+template <bool QUAD, bool EPS0>
+__device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict__ spos,
+                           const TravNode* __restrict__ trav, const Mult32* __restrict__ m32,
+                           const int32_t* __restrict__ order, float eps2, const Lists& lists,
+                           double* __restrict__ oax, double* __restrict__ oay, double* __restrict__ oaz) {
+    float4* cell = S.cell;
+    uint64_t* cmask = S.cmask;
+    float4* part = S.part;
+    uint64_t* pmask = S.pmask;
+    const int lane = lane_id();
+    const float px = (float)(G.p.x - G.cgx), py = (float)(G.p.y - G.cgy), pz = (float)(G.p.z - G.cgz);
+    const int4 cnts = lists.counts[G.g];
+    const uint32_t* __restrict__ fullIn = lists.full + (size_t)G.g * lists.cf;
+    const uint4* __restrict__ partIn = lists.part + (size_t)G.g * lists.cp;
+    const uint4* __restrict__ leafIn = lists.leaf + (size_t)G.g * lists.cl;
+    float sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;  // Kahan sums
+
+    // Cell chunks are software-pipelined: while chunk i is evaluated from LDS,
+    // the node data of chunk i+1 and the indices of chunk i+2 are in flight.
+    struct Staged {
+        float4 a, b;
+        float2 c;
+        uint64_t m;
+    };
+    auto fetch = [&](int node, uint64_t m) {
+        Staged d;
+        const TravNode& t = trav[node];
+        const Mult32 mm = m32[node];
+        d.a = make_float4((float)(t.x - G.cgx), (float)(t.y - G.cgy), (float)(t.z - G.cgz), mm.m);
+        d.b = make_float4(mm.qxx, mm.qyy, mm.qzz, mm.qxy);
+        d.c = make_float2(mm.qxz, mm.qyz);
+        d.m = m;
+        return d;
+    };
+    auto index_of = [&](bool partial, int i, uint64_t& m) -> int {
+        if (partial) {
+            const uint4 e = partIn[i];
+            m = (uint64_t)e.z | ((uint64_t)e.w << 32);
+            return (int)e.x;
+        }
+        m = G.gmask;
+        return (int)fullIn[i];
+    };
+    for (int pass = 0; pass < 2; pass++) {
+        const bool partial = pass == 1;
+        const int n = partial ? cnts.y : cnts.x;
+        if (n == 0) continue;
+        uint64_t m0 = 0, m1 = 0;
+        int idx0 = lane < n ? index_of(partial, lane, m0) : 0;
+        int idx1 = kChunk + lane < n ? index_of(partial, kChunk + lane, m1) : 0;
+        Staged cur;
+        if (lane < n) cur = fetch(idx0, m0);
+        for (int c0 = 0; c0 < n; c0 += kChunk) {
+            const int cnt = min(kChunk, n - c0);
+            if (lane < cnt) {
+                cell[3 * lane] = cur.a;
+                cell[3 * lane + 1] = cur.b;
+                cell[3 * lane + 2] = make_float4(cur.c.x, cur.c.y, 0.0f, 0.0f);
+                cmask[lane] = cur.m;
+            }
+            __syncthreads();
+            Staged nxt;
+            const int c1 = c0 + kChunk;
+            if (c1 + lane < n) nxt = fetch(idx1, m1);
+            if (c1 + kChunk + lane < n) idx1 = index_of(partial, c1 + kChunk + lane, m1);
+            float ax = 0, ay = 0, az = 0;
+            if (partial)
+                eval_cells<QUAD, true>(cell, cmask, cnt, px, py, pz, eps2, ax, ay, az);
+            else
+                eval_cells<QUAD, false>(cell, cmask, cnt, px, py, pz, eps2, ax, ay, az);
+            kahan_add(sx, cx, ax);
+            kahan_add(sy, cy, ay);
+            kahan_add(sz, cz, az);
+            __syncthreads();
+            cur = nxt;
+        }
+    }
+    // stars of opened leaves (large leaves are split over chunks)
+    int npart = 0;
+    for (int c0 = 0; c0 < cnts.z; c0 += kWave) {
+        const int cnt = min(kWave, cnts.z - c0);
+        int lcur = 0, lcnt = 0;
+        uint64_t lmask = 0;
+        if (lane < cnt) {
+            const uint4 e = leafIn[c0 + lane];
+            lcur = trav[e.x].lo;
+            lcnt = trav[e.x].hi - lcur;
+            lmask = (uint64_t)e.z | ((uint64_t)e.w << 32);
+        }
         while (__ballot(lcnt > 0)) {
             const int room = kParts - npart;
-            const int incl2 = wave_incl_scan(lcnt);
-            const int excl2 = incl2 - lcnt;
-            const int wcount = max(0, min(lcnt, room - excl2));
-            for (int i = 0; i < wcount; i++) {
+            const int incl = wave_incl_scan(lcnt);
+            const int excl = incl - lcnt;
+            const int w = max(0, min(lcnt, room - excl));
+            for (int i = 0; i < w; i++) {
                 const double4 q = spos[lcur + i];
-                const int slot = npart + excl2 + i;
-                L.part[slot] = make_float4((float)(q.x - cgx), (float)(q.y - cgy), (float)(q.z - cgz), (float)q.w);
-                L.partMask[slot] = omask;
+                const int slot = npart + excl + i;
+                part[slot] = make_float4((float)(q.x - G.cgx), (float)(q.y - G.cgy), (float)(q.z - G.cgz), (float)q.w);
+                pmask[slot] = lmask;
             }
-            npart += wave_sum_i(wcount);
-            lcur += wcount;
-            lcnt -= wcount;
-            if (__ballot(lcnt > 0)) {
+            npart += wave_sum_i(w);
+            lcur += w;
+            lcnt -= w;
+            if (npart == kParts) {
                 __syncthreads();
                 float ax = 0, ay = 0, az = 0;
-                eval_parts<EPS0>(L, npart, px, py, pz, eps2, ax, ay, az);
+                eval_parts<EPS0>(part, pmask, npart, px, py, pz, eps2, ax, ay, az);
                 kahan_add(sx, cx, ax);
                 kahan_add(sy, cy, ay);
                 kahan_add(sz, cz, az);
@@ -294,56 +433,91 @@ __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos,
                 __syncthreads();
             }
         }
-        __syncthreads();
     }
-
     __syncthreads();
     {
         float ax = 0, ay = 0, az = 0;
-        eval_cells<QUAD>(L, ncell, px, py, pz, eps2, ax, ay, az);
-        eval_parts<EPS0>(L, npart, px, py, pz, eps2, ax, ay, az);
+        eval_parts<EPS0>(part, pmask, npart, px, py, pz, eps2, ax, ay, az);
         kahan_add(sx, cx, ax);
         kahan_add(sy, cy, ay);
         kahan_add(sz, cz, az);
     }
-    if (valid) {
-        const int32_t s = order[k];
-        oax[s] = overflow ? NAN : (double)sx - (double)cx;
-        oay[s] = overflow ? NAN : (double)sy - (double)cy;
-        oaz[s] = overflow ? NAN : (double)sz - (double)cz;
-    }
-    npp = wave_sum_u64(npp);
-    npc = wave_sum_u64(npc);
-    if (lane == 0) {
-        atomicAdd(&meta->pp, npp);
-        atomicAdd(&meta->pc, npc);
-        atomicMax(&meta->maxStack, maxsp);
-        if (overflow) atomicOr(&meta->flags, (int)kFlagStackOverflow);
+    if (G.valid) {
+        const int32_t s = order[G.k];
+        oax[s] = (double)sx - (double)cx;
+        oay[s] = (double)sy - (double)cy;
+        oaz[s] = (double)sz - (double)cz;
     }
 }
 
+__global__ __launch_bounds__(64) void k_trav32(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
+                                               Meta* meta, Lists lists) {
+    __shared__ TravLDS S;
+    if (meta->flags & kFlagNodeOverflow) return;
+    Group G;
+    if (!load_group(G, spos, meta->nstars)) return;
+    trav_group(G, S, trav, meta, lists);
+}
+
+template <bool QUAD, bool EPS0>
+__global__ __launch_bounds__(64) void k_eval32(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
+                                               const Mult32* __restrict__ m32, const int32_t* __restrict__ order,
+                                               Meta* meta, float eps2, Lists lists, double* __restrict__ oax,
+                                               double* __restrict__ oay, double* __restrict__ oaz) {
+    __shared__ EvalLDS S;
+    if (meta->flags & (kFlagNodeOverflow | kFlagStackOverflow | kFlagListOverflow)) return;
+    Group G;
+    if (!load_group(G, spos, meta->nstars)) return;
+    eval_group<QUAD, EPS0>(G, S, spos, trav, m32, order, eps2, lists, oax, oay, oaz);
+}
+
+// Traversal and evaluation of one group in one wavefront: the latency-bound
+// traversal of some waves overlaps the VALU-bound evaluation of others on the
+// same SIMD.  The lists still go through HBM (they stay in L2/MALL between
+// the two phases), so LDS holds only one phase's buffers at a time.
+template <bool QUAD, bool EPS0>
+__global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
+                                               const Mult32* __restrict__ m32, const int32_t* __restrict__ order,
+                                               Meta* meta, float eps2, Lists lists, double* __restrict__ oax,
+                                               double* __restrict__ oay, double* __restrict__ oaz) {
+    __shared__ WalkLDS S;
+    if (meta->flags & kFlagNodeOverflow) return;
+    Group G;
+    if (!load_group(G, spos, meta->nstars)) return;
+    const bool ok = trav_group(G, S.t, trav, meta, lists);
+    __syncthreads();  // lists written by this wave's lanes are read back by others
+    if (ok) eval_group<QUAD, EPS0>(G, S.e, spos, trav, m32, order, eps2, lists, oax, oay, oaz);
+}
+
 void launch_walk32(const double4* spos, const TravNode* trav, const Mult32* m32, const int32_t* order, Meta* meta,
-                   long long ngroupsUpper, float eps2, int quad, double* ax, double* ay, double* az,
-                   hipStream_t stream) {
+                   long long ngroupsUpper, float eps2, int quad, const Lists& lists, double* ax, double* ay,
+                   double* az, hipStream_t stream, hipEvent_t mid, bool split) {
     if (ngroupsUpper <= 0) return;
     const long long per = (ngroupsUpper + 7) / 8;
     dim3 grid((unsigned)(per * 8)), block(64);
     const bool eps0 = !(eps2 > 0.0f);
-    if (quad) {
-        if (eps0)
-            hipLaunchKernelGGL((k_walk32<true, true>), grid, block, 0, stream, spos, trav, m32, order, meta, eps2,
-                               ax, ay, az);
-        else
-            hipLaunchKernelGGL((k_walk32<true, false>), grid, block, 0, stream, spos, trav, m32, order, meta, eps2,
-                               ax, ay, az);
-    } else {
-        if (eps0)
-            hipLaunchKernelGGL((k_walk32<false, true>), grid, block, 0, stream, spos, trav, m32, order, meta, eps2,
-                               ax, ay, az);
-        else
-            hipLaunchKernelGGL((k_walk32<false, false>), grid, block, 0, stream, spos, trav, m32, order, meta, eps2,
-                               ax, ay, az);
+    if (!split) {
+#define GX_WALK(Q, E) \
+    hipLaunchKernelGGL((k_walk32<Q, E>), grid, block, 0, stream, spos, trav, m32, order, meta, eps2, lists, ax, ay, az)
+        if (quad) {
+            if (eps0) GX_WALK(true, true); else GX_WALK(true, false);
+        } else {
+            if (eps0) GX_WALK(false, true); else GX_WALK(false, false);
+        }
+#undef GX_WALK
+        if (mid) (void)hipEventRecord(mid, stream);
+        return;
     }
+    hipLaunchKernelGGL(k_trav32, grid, block, 0, stream, spos, trav, meta, lists);
+    if (mid) (void)hipEventRecord(mid, stream);
+#define GX_EVAL(Q, E) \
+    hipLaunchKernelGGL((k_eval32<Q, E>), grid, block, 0, stream, spos, trav, m32, order, meta, eps2, lists, ax, ay, az)
+    if (quad) {
+        if (eps0) GX_EVAL(true, true); else GX_EVAL(true, false);
+    } else {
+        if (eps0) GX_EVAL(false, true); else GX_EVAL(false, false);
+    }
+#undef GX_EVAL
 }
 
 }  // namespace gx

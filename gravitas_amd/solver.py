@@ -194,7 +194,9 @@ class Solver:
     def stats(self) -> dict:
         st = _lib.GxStats()
         _lib.check(self._L.gx_get_stats(self._h, ctypes.byref(st)))
-        return {f: getattr(st, f) for f, _ in st._fields_}
+        out = {f: getattr(st, f) for f, _ in st._fields_}
+        out["max_list"] = list(st.max_list)[:3]
+        return out
 
     _EXPORT = {"keys": (0, np.uint64, ()), "order": (1, np.int32, ()), "spos": (2, np.float64, (4,)),
                "nodes": (3, None, ()), "mult": (4, np.float64, (8,))}
@@ -216,7 +218,13 @@ class Solver:
         return buf[:k]
 
     def phase_ms(self) -> dict:
-        buf = (ctypes.c_float * 8)()
-        k = self._L.gx_phase_ms(self._h, buf, 8)
-        names = ("collect", "keys", "sort", "gather", "topology", "moments", "walk", "total")
-        return {names[i]: float(buf[i]) for i in range(k)}
+        """GPU milliseconds per phase of the last step (timing=True); for the
+        FP32 walk, "walk" is traversal + evaluation."""
+        buf = (ctypes.c_float * 9)()
+        k = self._L.gx_phase_ms(self._h, buf, 9)
+        names = ("collect", "keys", "sort", "gather", "topology", "moments", "walk", "total", "evaluate")
+        d = {names[i]: float(buf[i]) for i in range(k)}
+        if d.get("evaluate", 0.0) > 0.0:
+            d["traverse"] = d["walk"]
+            d["walk"] = d["walk"] + d["evaluate"]
+        return d

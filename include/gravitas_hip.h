@@ -119,14 +119,19 @@ typedef struct {
     int64_t pc_interactions; /* star-cell pairs */
     int32_t max_stack, flags, depth, walk;
     int64_t fallbacks;       /* FP32 steps recomputed by the FP64 walk */
+    int64_t list_entries;    /* FP32 walk: cell-list entries over all groups */
+    int64_t rounds;          /* FP32 walk: traversal rounds over all groups */
+    int64_t mixed_nodes;     /* FP32 walk: nodes decided star by star */
+    int32_t max_list[4];     /* FP32 walk: longest full-cell, partial-cell, leaf list; unused */
 } gx_stats;
 
 /* Statistics of the last completed step. */
 int gx_get_stats(const gx_solver *s, gx_stats *out);
 
 /* Per-phase GPU milliseconds of the last step when timing is on:
- * 0 collect, 1 keys, 2 sort, 3 gather, 4 topology, 5 moments, 6 walk,
- * 7 total.  Returns the number of phases written. */
+ * 0 collect, 1 keys, 2 sort, 3 gather, 4 topology, 5 moments, 6 walk (for
+ * the FP32 walk: its traversal kernel), 7 total, 8 FP32 walk evaluation
+ * kernel (0 for the FP64 walk).  Returns the number of phases written. */
 int gx_phase_ms(const gx_solver *s, float *ms, int n);
 
 /* Test hook with no counterpart in the reference: copies count items of
