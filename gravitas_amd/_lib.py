@@ -26,7 +26,7 @@ EXPORTS = (
     "gx_init", "gx_name", "gx_error", "gx_solver_new", "gx_solver_free", "gx_solver_config",
     "gx_solver_set_stream", "gx_accelerations", "gx_accelerations_dev", "gx_enqueue_accelerations_dev",
     "gx_sync", "gx_potential_energy_dev", "gx_direct_sample_dev", "gx_last_seconds", "gx_get_stats",
-    "gx_phase_ms", "gx_export",
+    "gx_phase_ms", "gx_export", "gx_step_dev", "gx_kick_dev", "gx_drift_dev",
 )
 
 
@@ -83,6 +83,9 @@ def load():
     L.gx_last_seconds.restype = D
     L.gx_get_stats.argtypes = [P, ctypes.POINTER(GxStats)]
     L.gx_phase_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float), I]
+    L.gx_step_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, I64, D, I, D, I, P, P, P]
+    L.gx_kick_dev.argtypes = [P, P, P, P, P, P, P, I64, D]
+    L.gx_drift_dev.argtypes = [P, P, P, P, P, P, P, I64, D]
     L.gx_export.argtypes = [P, I, P, I64]
     L.gx_export.restype = I64
     _lib = L

@@ -91,6 +91,7 @@ struct gx_solver {
     // FP32 walk as two kernels (traversal, evaluation) instead of one; set
     // from GX_SPLIT_WALK=1 for profiling the two phases separately.
     bool splitWalk = false;
+
 };
 
 extern "C" {
@@ -150,6 +151,7 @@ gx_solver* gx_solver_new(void) {
     for (auto& e : s->ev) (void)hipEventCreate(&e);
     const char* sw = getenv("GX_SPLIT_WALK");
     s->splitWalk = sw && sw[0] == '1';
+
     return s;
 }
 
@@ -469,6 +471,42 @@ int gx_accelerations(gx_solver* s, const double* x, const double* y, const doubl
     HIPCHK(hipMemcpyAsync(ay, s->ay.p, d, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(az, s->az.p, d, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return GX_OK;
+}
+
+int gx_kick_dev(gx_solver* s, double* vx, double* vy, double* vz, const double* ax, const double* ay,
+                const double* az, int64_t n, double h) {
+    if (!s || n < 0) return set_err(GX_ERR_ARG, "bad argument");
+    if (n == 0) return GX_OK;
+    hipLaunchKernelGGL(k_kick, dim3((unsigned)std::min<long long>(blocks(n, 256), 4096)), dim3(256), 0, s->stream,
+                       (long long)n, vx, vy, vz, ax, ay, az, h);
+    HIPCHK(hipGetLastError());
+    return GX_OK;
+}
+
+int gx_drift_dev(gx_solver* s, double* x, double* y, double* z, const double* vx, const double* vy,
+                 const double* vz, int64_t n, double dt) {
+    if (!s || n < 0) return set_err(GX_ERR_ARG, "bad argument");
+    if (n == 0) return GX_OK;
+    hipLaunchKernelGGL(k_drift, dim3((unsigned)std::min<long long>(blocks(n, 256), 4096)), dim3(256), 0, s->stream,
+                       (long long)n, x, y, z, vx, vy, vz, dt);
+    HIPCHK(hipGetLastError());
+    return GX_OK;
+}
+
+int gx_step_dev(gx_solver* s, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                const double* mass, const uint8_t* kind, const uint8_t* alive, int64_t n, double eps,
+                int dark_matter_off, double dt, int acc_valid, double* ax, double* ay, double* az) {
+    if (!s) return set_err(GX_ERR_ARG, "null solver");
+    int rc;
+    if (!acc_valid &&
+        (rc = gx_accelerations_dev(s, x, y, z, mass, kind, alive, n, eps, dark_matter_off, ax, ay, az)))
+        return rc;
+    if ((rc = gx_kick_dev(s, vx, vy, vz, ax, ay, az, n, 0.5 * dt))) return rc;
+    if ((rc = gx_drift_dev(s, x, y, z, vx, vy, vz, n, dt))) return rc;
+    if ((rc = gx_accelerations_dev(s, x, y, z, mass, kind, alive, n, eps, dark_matter_off, ax, ay, az))) return rc;
+    if ((rc = gx_kick_dev(s, vx, vy, vz, ax, ay, az, n, 0.5 * dt))) return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
     return GX_OK;
 }
 

@@ -362,7 +362,8 @@ __global__ void k_tree_init(Meta* meta, TravNode* __restrict__ trav) {
     t.hi = (int32_t)nst;
     t.nchild = 0;
     t.level = 0;
-    t.pad0 = t.pad1 = 0;
+    t.parent = -1;
+    t.pad = 0;
     trav[0] = t;
     meta->levelCount[0] = 1;
 }
@@ -415,8 +416,10 @@ __device__ int octant_bounds(const uint64_t* __restrict__ keys, int lo, int hi, 
     return lane >= 1 && lane <= 7 ? v : 0;
 }
 
-__global__ __launch_bounds__(256) void k_topo_level(int L, const uint64_t* __restrict__ keys,
-                                                    TravNode* __restrict__ trav, Meta* meta, long long cap) {
+// Splits every node of level L (one wavefront per node, grid-stride over the
+// level) and appends the children to level L+1.
+__device__ void topo_level(int L, const uint64_t* __restrict__ keys, TravNode* __restrict__ trav, Meta* meta,
+                           long long cap) {
     const long long start = meta->levelStart[L];
     const long long cnt = meta->levelCount[L];
     const long long nextBase = start + cnt;
@@ -463,7 +466,8 @@ __global__ __launch_bounds__(256) void k_topo_level(int L, const uint64_t* __res
             t.hi = bnext;
             t.nchild = 0;
             t.level = L + 1;
-            t.pad0 = t.pad1 = 0;
+            t.parent = (int32_t)ni;
+            t.pad = 0;
             trav[c] = t;
         }
         if (lane == 0) {
@@ -471,6 +475,11 @@ __global__ __launch_bounds__(256) void k_topo_level(int L, const uint64_t* __res
             trav[ni].nchild = nchild;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_topo_level(int L, const uint64_t* __restrict__ keys,
+                                                    TravNode* __restrict__ trav, Meta* meta, long long cap) {
+    topo_level(L, keys, trav, meta, cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -486,6 +495,80 @@ __device__ __forceinline__ void quad_add(double q[6], double m, double dx, doubl
     q[5] += m * 3 * dy * dz;
 }
 
+// Moments of node ni from its stars (leaf) or its finished children, in the
+// reference's order (finish, build.go:176-217).  While building, the node's
+// record holds the cell centre in x,y,z and the half side in crit2.
+__device__ void node_moments(long long ni, const double4* __restrict__ spos, TravNode* __restrict__ trav,
+                             Mult64* __restrict__ m64, Mult32* __restrict__ m32, double theta) {
+    TravNode nd = trav[ni];
+    const double cx = nd.x, cy = nd.y, cz = nd.z, half = nd.crit2;
+    double x = 0, y = 0, z = 0, m = 0;
+    if (nd.first < 0) {
+        for (int j = nd.lo; j < nd.hi; j++) {
+            double4 p = spos[j];
+            double mj = p.w;
+            x = x + mj * p.x;
+            y = y + mj * p.y;
+            z = z + mj * p.z;
+            m = m + mj;
+        }
+    } else {
+        for (int c = nd.first; c < nd.first + nd.nchild; c++) {
+            double cm = m64[c].m;
+            x = x + cm * trav[c].x;
+            y = y + cm * trav[c].y;
+            z = z + cm * trav[c].z;
+            m = m + cm;
+        }
+    }
+    if (m > 0) {
+        x = x / m;
+        y = y / m;
+        z = z / m;
+    } else {
+        x = cx;
+        y = cy;
+        z = cz;
+    }
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    if (nd.first < 0) {
+        for (int j = nd.lo; j < nd.hi; j++) {
+            double4 p = spos[j];
+            quad_add(q, p.w, p.x - x, p.y - y, p.z - z);
+        }
+    } else {
+        for (int c = nd.first; c < nd.first + nd.nchild; c++) {
+            const Mult64 cm = m64[c];
+            q[0] = q[0] + cm.qxx;
+            q[1] = q[1] + cm.qyy;
+            q[2] = q[2] + cm.qzz;
+            q[3] = q[3] + cm.qxy;
+            q[4] = q[4] + cm.qxz;
+            q[5] = q[5] + cm.qyz;
+            quad_add(q, cm.m, trav[c].x - x, trav[c].y - y, trav[c].z - z);
+        }
+    }
+    double delta = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
+    double crit = 2 * half / theta + delta;
+    Mult64 mm;
+    mm.m = m;
+    mm.qxx = q[0]; mm.qyy = q[1]; mm.qzz = q[2];
+    mm.qxy = q[3]; mm.qxz = q[4]; mm.qyz = q[5];
+    mm.pad = half;  // the cell's half side (scale of the FP16-stored quadrupoles)
+    m64[ni] = mm;
+    Mult32 mf;
+    mf.m = (float)m;
+    mf.qxx = (float)q[0]; mf.qyy = (float)q[1]; mf.qzz = (float)q[2];
+    mf.qxy = (float)q[3]; mf.qxz = (float)q[4]; mf.qyz = (float)q[5];
+    mf.pad = 0;
+    m32[ni] = mf;
+    nd.x = x;
+    nd.y = y;
+    nd.z = z;
+    nd.crit2 = crit * crit;
+    trav[ni] = nd;
+}
+
 __global__ __launch_bounds__(256) void k_moments_level(int L, const double4* __restrict__ spos,
                                                        TravNode* __restrict__ trav, Mult64* __restrict__ m64,
                                                        Mult32* __restrict__ m32, const Meta* __restrict__ meta,
@@ -494,76 +577,8 @@ __global__ __launch_bounds__(256) void k_moments_level(int L, const double4* __r
     const long long cnt = meta->levelCount[L];
     if (meta->flags & kFlagNodeOverflow) return;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
-         t += (long long)gridDim.x * blockDim.x) {
-        const long long ni = start + t;
-        TravNode nd = trav[ni];
-        const double cx = nd.x, cy = nd.y, cz = nd.z, half = nd.crit2;
-        double x = 0, y = 0, z = 0, m = 0;
-        if (nd.first < 0) {
-            for (int j = nd.lo; j < nd.hi; j++) {
-                double4 p = spos[j];
-                double mj = p.w;
-                x = x + mj * p.x;
-                y = y + mj * p.y;
-                z = z + mj * p.z;
-                m = m + mj;
-            }
-        } else {
-            for (int c = nd.first; c < nd.first + nd.nchild; c++) {
-                double cm = m64[c].m;
-                x = x + cm * trav[c].x;
-                y = y + cm * trav[c].y;
-                z = z + cm * trav[c].z;
-                m = m + cm;
-            }
-        }
-        if (m > 0) {
-            x = x / m;
-            y = y / m;
-            z = z / m;
-        } else {
-            x = cx;
-            y = cy;
-            z = cz;
-        }
-        double q[6] = {0, 0, 0, 0, 0, 0};
-        if (nd.first < 0) {
-            for (int j = nd.lo; j < nd.hi; j++) {
-                double4 p = spos[j];
-                quad_add(q, p.w, p.x - x, p.y - y, p.z - z);
-            }
-        } else {
-            for (int c = nd.first; c < nd.first + nd.nchild; c++) {
-                const Mult64 cm = m64[c];
-                q[0] = q[0] + cm.qxx;
-                q[1] = q[1] + cm.qyy;
-                q[2] = q[2] + cm.qzz;
-                q[3] = q[3] + cm.qxy;
-                q[4] = q[4] + cm.qxz;
-                q[5] = q[5] + cm.qyz;
-                quad_add(q, cm.m, trav[c].x - x, trav[c].y - y, trav[c].z - z);
-            }
-        }
-        double delta = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
-        double crit = 2 * half / theta + delta;
-        nd.x = x;
-        nd.y = y;
-        nd.z = z;
-        nd.crit2 = crit * crit;
-        trav[ni] = nd;
-        Mult64 mm;
-        mm.m = m;
-        mm.qxx = q[0]; mm.qyy = q[1]; mm.qzz = q[2];
-        mm.qxy = q[3]; mm.qxz = q[4]; mm.qyz = q[5];
-        mm.pad = 0;
-        m64[ni] = mm;
-        Mult32 mf;
-        mf.m = (float)m;
-        mf.qxx = (float)q[0]; mf.qyy = (float)q[1]; mf.qzz = (float)q[2];
-        mf.qxy = (float)q[3]; mf.qxz = (float)q[4]; mf.qyz = (float)q[5];
-        mf.pad = 0;
-        m32[ni] = mf;
-    }
+         t += (long long)gridDim.x * blockDim.x)
+        node_moments(start + t, spos, trav, m64, m32, theta);
 }
 
 __global__ void k_tree_finish(Meta* meta) {
@@ -581,6 +596,35 @@ __global__ void k_zero_holes(const int32_t* __restrict__ holes, const Meta* __re
         ax[i] = 0;
         ay[i] = 0;
         az[i] = 0;
+    }
+}
+
+}  // namespace gx
+
+namespace gx {
+
+// ---------------------------------------------------------------------------
+// KDK leapfrog pieces: kick and drift, internal/sim/sim.go:302-343 (the
+// Newtonian branch, c <= 0), over every body as in the reference.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_kick(long long n, double* __restrict__ vx, double* __restrict__ vy,
+                                              double* __restrict__ vz, const double* __restrict__ ax,
+                                              const double* __restrict__ ay, const double* __restrict__ az, double h) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        vx[i] += h * ax[i];
+        vy[i] += h * ay[i];
+        vz[i] += h * az[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_drift(long long n, double* __restrict__ px, double* __restrict__ py,
+                                               double* __restrict__ pz, const double* __restrict__ vx,
+                                               const double* __restrict__ vy, const double* __restrict__ vz,
+                                               double dt) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        px[i] += dt * vx[i];
+        py[i] += dt * vy[i];
+        pz[i] += dt * vz[i];
     }
 }
 

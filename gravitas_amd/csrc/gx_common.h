@@ -23,7 +23,8 @@ struct alignas(64) TravNode {
     int32_t lo, hi;  // star range in sorted order
     int32_t nchild;  // children first .. first+nchild-1 (octant order)
     int32_t level;
-    int32_t pad0, pad1;
+    int32_t parent;  // -1 for a root
+    int32_t pad;
 };
 
 // FP64 multipole (m and the traceless quadrupole about the centre of mass).

@@ -28,6 +28,10 @@ __global__ void k_moments_level(int L, const double4* spos, TravNode* trav, Mult
                                 const Meta* meta, double theta);
 __global__ void k_tree_finish(Meta* meta);
 __global__ void k_zero_holes(const int32_t* holes, const Meta* meta, double* ax, double* ay, double* az);
+__global__ void k_kick(long long n, double* vx, double* vy, double* vz, const double* ax, const double* ay,
+                       const double* az, double h);
+__global__ void k_drift(long long n, double* px, double* py, double* pz, const double* vx, const double* vy,
+                        const double* vz, double dt);
 
 // gx_walk64.hip
 __global__ void k_walk64(const double4* spos, const TravNode* trav, const Mult64* m64, const int32_t* order,

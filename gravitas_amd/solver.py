@@ -161,6 +161,19 @@ class Solver:
                                             int(dm_off), c(ax), c(ay), c(az)))
         return np.stack([ax, ay, az], axis=1)
 
+    # -- KDK step (sim.go:262-281, 302-343) -------------------------------------------
+    def step(self, s: DeviceState, p: Params, acc_valid: bool = True):
+        """One kick-drift-kick leapfrog step of p.dt on the device state; with
+        acc_valid False the accelerations are computed first (initForces)."""
+        self._configure()
+        _lib.check(self._L.gx_step_dev(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.vx), _ptr(s.vy), _ptr(s.vz), _ptr(s.m),
+            _ptr(s.kind), _ptr(s.alive), len(s), float(p.softening), int(p.dark_matter_off), float(p.dt),
+            int(bool(acc_valid)), _ptr(s.ax), _ptr(s.ay), _ptr(s.az)))
+
+    def host_state(self, s: DeviceState) -> dict:
+        return {f: getattr(s, f).numpy() for f in DeviceState.FIELDS}
+
     # -- PotentialEnergy (potential.go:12-42) ----------------------------------------
     def potential_energy(self, s: DeviceState, p: Params) -> float:
         self._configure()
@@ -202,7 +215,7 @@ class Solver:
                "nodes": (3, None, ()), "mult": (4, np.float64, (8,))}
     NODE_DTYPE = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("crit2", "f8"), ("first", "i4"),
                            ("next", "i4"), ("lo", "i4"), ("hi", "i4"), ("nchild", "i4"), ("level", "i4"),
-                           ("pad0", "i4"), ("pad1", "i4")])
+                           ("parent", "i4"), ("pad", "i4")])
 
     def export(self, what: str) -> np.ndarray:
         """Copy part of the last build to host (test hook): keys, order, spos,

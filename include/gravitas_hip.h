@@ -96,6 +96,22 @@ int gx_enqueue_accelerations_dev(gx_solver *s, const double *x, const double *y,
                                  double eps, int dark_matter_off, double *ax, double *ay, double *az);
 int gx_sync(gx_solver *s);
 
+/* KDK leapfrog on device arrays (internal/sim/sim.go:262-281 without holes
+ * and gas; kick/drift :302-343 with c <= 0): if acc_valid is 0 the
+ * accelerations are computed first (initForces, gas.go:14-26); then
+ * v += dt/2 a, x += dt v, new accelerations, v += dt/2 a, over every body.
+ * Synchronous. */
+int gx_step_dev(gx_solver *s, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                const double *mass, const uint8_t *kind, const uint8_t *alive, int64_t n, double eps,
+                int dark_matter_off, double dt, int acc_valid, double *ax, double *ay, double *az);
+
+/* The two halves of the step on device arrays, enqueued on the solver's
+ * stream: kick v += h a and drift x += dt v over n bodies. */
+int gx_kick_dev(gx_solver *s, double *vx, double *vy, double *vz, const double *ax, const double *ay,
+                const double *az, int64_t n, double h);
+int gx_drift_dev(gx_solver *s, double *x, double *y, double *z, const double *vx, const double *vy,
+                 const double *vz, int64_t n, double dt);
+
 /* bh.Solver.PotentialEnergy (potential.go:12-42) on device arrays: theta
  * capped at 0.9; FP64 tree walk.  Star-star pairs only: returns
  * GX_ERR_UNSUPPORTED if alive holes are present. */
@@ -138,7 +154,7 @@ int gx_phase_ms(const gx_solver *s, float *ms, int n);
  * the last build to host memory.  what: 0 keys (uint64, sorted), 1 order
  * (int32, sorted position -> state index), 2 sorted stars (4 doubles x, y, z,
  * source mass), 3 nodes (64-byte records: double x, y, z, crit2; int32 first,
- * next, lo, hi, nchild, level, pad, pad), 4 FP64 moments (8 doubles m, qxx,
+ * next, lo, hi, nchild, level, parent, pad), 4 FP64 moments (8 doubles m, qxx,
  * qyy, qzz, qxy, qxz, qyz, pad).  Returns the number of items copied. */
 int64_t gx_export(gx_solver *s, int what, void *host, int64_t count);
 
