@@ -1,17 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the gfx950 Barnes-Hut force step (bh.Solver.Accelerations).
 
-Workload: the single-galaxy model -- N = 1,048,576 (70 % exponential
-Toomre-Q disk, 30 % Hernquist halo with 5x the disk mass), theta 0.5, Plummer
-softening 0.02, inputs resident in HBM.  One step = one full force evaluation:
-collect, Morton keys, radix sort, sorted copies, octree topology, moments and
-the FP32 group walk, written back in state order.
+Default workload (the headline): the single-galaxy model -- N = 1,048,576
+(70 % exponential Toomre-Q disk, 30 % Hernquist halo with 5x the disk mass),
+theta 0.5, Plummer softening 0.02, inputs resident in HBM.  One step = one
+full force evaluation: collect, Morton keys, radix sort, sorted copies,
+octree topology, moments and the FP32 group walk, written back in state order.
 
 Printed (one JSON line): particles/s of the FP32 step; interactions/s; the
 FP32-vs-FP64-walk and FP32-vs-direct-sum relative force errors (p50/p99/max);
 the FP64 GPU walk's throughput; the walk kernel's achieved FP32 FLOP/s against
 the MI355X vector peak (HIP events around the walk on the solver's stream);
 and the CPU oracle's throughput on the host cores.
+
+Other workloads (--config): plummer-16k (force + one KDK step, FP64 and FP32),
+clumps-16m (precision ladder FP64 / FP32 / FP32 with FP16-stored quadrupoles).
 """
 from __future__ import annotations
 
@@ -31,6 +34,7 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector, datasheet
 HBM_PEAK_GBS = 8000.0
 FLOPS_PP = 18              # bh.go:145-150 counting rule (add/sub/mul/div/sqrt = 1, fma = 2)
 FLOPS_PC = 50              # bh.go:114-131
+CONFIGS = ("disk-halo-1m", "plummer-16k", "clumps-16m")
 
 
 def parse():
@@ -38,13 +42,18 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1048576)
-    ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--eps", type=float, default=0.02)
+    ap.add_argument("--config", default="disk-halo-1m", choices=CONFIGS)
+    ap.add_argument("--n", type=int, default=None, help="override the body count")
+    ap.add_argument("--theta", type=float, default=None)
+    ap.add_argument("--eps", type=float, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-errors", action="store_true", help="skip the error measurements")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def errors(got, want):
@@ -55,22 +64,19 @@ def errors(got, want):
 def cpu_baseline(state, theta, eps, budget):
     """The CPU oracle (C restatement of bh.Solver: serial build, ParallelFor
     walk) on the host cores: one warm-up call, then timed calls of the full
-    Accelerations until the budget is spent.  Sample: the first n_s bodies of
-    the workload in state order, n_s chosen so a call takes a few seconds."""
+    Accelerations until the budget is spent, on an evenly strided sample of
+    the workload sized so that one call takes about a quarter of the budget."""
     import oracle
     workers = min(16, os.cpu_count() or 1)
     n = len(state["px"])
-    # size the sample from a small probe
-    probe = min(n, 65536)
-    sl = lambda k: {f: state[f][:k] for f in ("px", "py", "pz", "m", "kind")}
     o = oracle.Solver(theta=theta, workers=workers)
-    s = sl(probe)
+    probe = min(n, 65536)
+    idx = np.linspace(0, n - 1, probe).astype(np.int64)
+    s = {f: np.ascontiguousarray(state[f][idx]) for f in ("px", "py", "pz", "m", "kind")}
     t0 = time.perf_counter()
     o.accelerations(s["px"], s["py"], s["pz"], s["m"], s["kind"], eps=eps)
     tp = time.perf_counter() - t0
     ns = int(min(n, max(probe, probe * (budget / 4.0) / max(tp, 1e-3))))
-    # sample of the same model: a state-order prefix mixes disk and halo only
-    # if it spans both, so take an evenly strided subset instead
     idx = np.linspace(0, n - 1, ns).astype(np.int64)
     s = {f: np.ascontiguousarray(state[f][idx]) for f in ("px", "py", "pz", "m", "kind")}
     o.accelerations(s["px"], s["py"], s["pz"], s["m"], s["kind"], eps=eps)  # warm-up
@@ -87,75 +93,36 @@ def cpu_baseline(state, theta, eps, budget):
                       f"C restatement of bh.Solver, -O2 -ffp-contract=off, {workers} threads"}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
-              "the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-    from gravitas_amd import DeviceState, Params, Solver, gpu_available, ics
+def timed_steps(solver, ds, params, steps, warmup):
+    """Warm-up, then `steps` synchronous force steps; per-phase GPU times from
+    HIP events recorded on the solver's stream."""
     from gravitas_amd.hip import synchronize
-    reason = gpu_available()
-    if reason is not None:
-        print(f"no GPU backend: {reason}", file=sys.stderr)
-        sys.exit(1)
-
-    n, theta, eps = args.n, args.theta, args.eps
-    print(f"[bench] building disk+halo model n={n}", file=sys.stderr, flush=True)
-    state = ics.disk_halo(n, seed=2, eps=eps)
-    ds = DeviceState.from_host(state)
-    params = Params(softening=eps)
-    solver = Solver(theta=theta, walk="fp32", timing=True)
-
-    print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
-    for _ in range(max(1, args.warmup)):
+    for _ in range(max(1, warmup)):
         solver.accelerations(ds, params)
     synchronize()
-
-    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
-    walk_ms, phases = [], []
-    synchronize()
+    phases = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         solver.accelerations(ds, params)
-        ph = solver.phase_ms()
-        walk_ms.append(ph["walk"])
-        phases.append(ph)
+        phases.append(solver.phase_ms())
     synchronize()
     wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    st = solver.stats()
-    acc32 = ds.acc_host()
+    phase_avg = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+    return wall * 1e3 / steps, phase_avg, solver.stats()
+
+
+def walk_summary(n, st, phase_avg):
     npp, npc = st["pp_interactions"], st["pc_interactions"]
-    walk_s = float(np.mean(walk_ms)) * 1e-3
+    walk_s = phase_avg["walk"] * 1e-3
     flops = FLOPS_PP * npp + FLOPS_PC * npc
     achieved = flops / walk_s / 1e12
-    phase_avg = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
-
-    out = {
-        "metric": METRIC,
-        "value": n / (ms_per_step * 1e-3),
-        "unit": "particles/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic",
-        "config": {"workload": "disk-halo-1m", "n": n, "theta": theta, "eps": eps,
-                   "model": "70% exponential disk (R_d 1, z0 0.1, Q 1.5) + 30% Hernquist halo (a 5, 5x disk mass)",
-                   "walk": "fp32 group walk (64 stars per wavefront), Kahan-compensated per-star sums",
-                   "parallelism": "single GPU"},
-        "interactions_per_s": (npp + npc) / (ms_per_step * 1e-3),
+    ng = (n + 63) // 64
+    return {
         "interactions_per_particle": {"pp": npp / n, "pc": npc / n},
         "tree": {"nodes": st["n_nodes"], "depth": st["depth"], "max_stack": st["max_stack"]},
-        "walk_lists": {"groups": (n + 63) // 64, "cell_entries_per_group": st["list_entries"] / ((n + 63) // 64),
+        "walk_lists": {"groups": ng, "cell_entries_per_group": st["list_entries"] / ng,
                        "cell_lane_utilization": npc / max(1, 64 * st["list_entries"]),
-                       "rounds_per_group": st["rounds"] / ((n + 63) // 64),
-                       "mixed_nodes_per_group": st["mixed_nodes"] / ((n + 63) // 64),
+                       "rounds_per_group": st["rounds"] / ng, "mixed_nodes_per_group": st["mixed_nodes"] / ng,
                        "max_list": st["max_list"]},
         "phases_ms": phase_avg,
         "roofline": {"bound": "valu", "kernel": "k_walk32", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
@@ -163,14 +130,39 @@ def main():
                      "flops_per_launch": flops, "walk_ms": walk_s * 1e3},
     }
 
+
+def base_line(args, n, ms_per_step, dtype, config):
+    return {
+        "metric": METRIC, "value": n / (ms_per_step * 1e-3), "unit": "particles/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config,
+    }
+
+
+def bench_disk_halo(args):
+    from gravitas_amd import DeviceState, Params, Solver, ics
+    n = args.n or 1048576
+    theta = 0.5 if args.theta is None else args.theta
+    eps = 0.02 if args.eps is None else args.eps
+    log(f"building disk+halo model n={n}")
+    state = ics.disk_halo(n, seed=2, eps=eps)
+    ds = DeviceState.from_host(state)
+    params = Params(softening=eps)
+    solver = Solver(theta=theta, walk="fp32", timing=True)
+    log(f"timing {args.steps} steps after {args.warmup} warm-up")
+    ms, phases, st = timed_steps(solver, ds, params, args.steps, args.warmup)
+    acc32 = ds.acc_host()
+    out = base_line(args, n, ms, "fp32", {
+        "workload": "disk-halo-1m", "n": n, "theta": theta, "eps": eps,
+        "model": "70% exponential disk (R_d 1, z0 0.1, Q 1.5) + 30% Hernquist halo (a 5, 5x disk mass)",
+        "walk": "fp32 group walk (64 stars per wavefront), Kahan-compensated per-star sums",
+        "parallelism": "single GPU"})
+    out["interactions_per_s"] = (st["pp_interactions"] + st["pc_interactions"]) / (ms * 1e-3)
+    out.update(walk_summary(n, st, phases))
     if not args.no_errors:
-        print("[bench] FP64 walk and direct-sum sample", file=sys.stderr, flush=True)
+        log("FP64 walk and direct-sum sample")
         s64 = Solver(theta=theta, walk="fp64", timing=True)
-        s64.accelerations(ds, params)  # warm-up
-        t64 = []
-        for _ in range(2):
-            s64.accelerations(ds, params)
-            t64.append(s64.phase_ms()["total"])
+        ms64, ph64, _ = timed_steps(s64, ds, params, 2, 1)
         acc64 = ds.acc_host()
         e64 = errors(acc32, acc64)
         targets = ics.sample_targets(n, 4096, 7)
@@ -178,19 +170,125 @@ def main():
         out["err_fp32_vs_fp64_walk"] = e64
         out["err_fp32_vs_direct"] = errors(acc32[targets], direct)
         out["err_fp64_walk_vs_direct"] = errors(acc64[targets], direct)
-        fp64_ps = n / (float(np.median(t64)) * 1e-3)
-        gpu32_ps = n / (phase_avg["total"] * 1e-3)
-        out["fp64_walk"] = {"particles_per_s": fp64_ps, "step_ms": float(np.median(t64)),
-                            "fp32_over_fp64": gpu32_ps / fp64_ps}
+        out["fp64_walk"] = {"particles_per_s": n / (ms64 * 1e-3), "step_ms": ms64, "walk_ms": ph64["walk"],
+                            "fp32_over_fp64_step": ms64 / ms, "fp32_over_fp64_walk": ph64["walk"] / phases["walk"]}
         out["gate_p99_le_1e-3"] = bool(e64["p99"] <= 1e-3)
-
+        if args.theta is None:
+            log("theta 0.7 line")
+            s7 = Solver(theta=0.7, walk="fp32", timing=True)
+            ms7, ph7, st7 = timed_steps(s7, ds, params, max(3, args.steps // 4), 1)
+            acc7 = ds.acc_host()
+            s7d = Solver(theta=0.7, walk="fp64")
+            s7d.accelerations(ds, params)
+            out["theta_0.7"] = {"particles_per_s": n / (ms7 * 1e-3), "ms_per_step": ms7, "walk_ms": ph7["walk"],
+                                "err_fp32_vs_fp64_walk": errors(acc7, ds.acc_host()),
+                                "err_fp32_vs_direct": errors(acc7[targets], direct),
+                                "interactions_per_particle": {"pp": st7["pp_interactions"] / n,
+                                                              "pc": st7["pc_interactions"] / n}}
     if not args.no_cpu:
-        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        log("CPU baseline")
         cb = cpu_baseline(state, theta, eps, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = out["value"] / cb["value"]
+    return out
 
-    print(json.dumps(out), flush=True)
+
+def bench_plummer(args):
+    """Plain reference method: 16,384 equal-mass Plummer stars, theta 0.5,
+    eps 0.01, one force evaluation plus one KDK step of dt 1e-3 (FP64 walk,
+    bit-identical to the reference algorithm; FP32 walk alongside)."""
+    from gravitas_amd import DeviceState, Params, Solver, ics
+    from gravitas_amd.hip import synchronize
+    n = args.n or 16384
+    theta = 0.5 if args.theta is None else args.theta
+    eps = 0.01 if args.eps is None else args.eps
+    state = ics.plummer_equal_mass(n, seed=1)
+    params = Params(softening=eps, dt=1e-3)
+    res = {}
+    for walk in ("fp64", "fp32"):
+        solver = Solver(theta=theta, walk=walk, timing=True)
+        ds = DeviceState.from_host(state)
+        solver.step(ds, params, acc_valid=False)  # warm-up
+        times = []
+        for _ in range(args.steps):
+            ds = DeviceState.from_host(state)
+            synchronize()
+            t0 = time.perf_counter()
+            solver.step(ds, params, acc_valid=False)  # force evaluation + one KDK step (two force calls)
+            synchronize()
+            times.append(time.perf_counter() - t0)
+        res[walk] = {"ms": float(np.median(times)) * 1e3, "acc": ds.acc_host(), "stats": solver.stats()}
+    ds = DeviceState.from_host(state)
+    direct = Solver().direct_sample(ds, params, np.arange(n)).numpy()
+    s64 = Solver(theta=theta, walk="fp64")
+    s64.accelerations(ds, params)
+    a0 = ds.acc_host()
+    ms = res["fp32"]["ms"]
+    out = base_line(args, n, ms, "fp32", {"workload": "plummer-16k", "n": n, "theta": theta, "eps": eps,
+                                          "step": "initial forces + one KDK step of dt 1e-3 (2 force evaluations)"})
+    out["fp64_step_ms"] = res["fp64"]["ms"]
+    out["err_fp64_tree_vs_direct"] = errors(a0, direct)
+    out["interactions_per_particle"] = {"pp": res["fp32"]["stats"]["pp_interactions"] / n,
+                                        "pc": res["fp32"]["stats"]["pc_interactions"] / n}
+    if not args.no_cpu:
+        import oracle
+        o = oracle.Solver(theta=theta, workers=min(16, os.cpu_count() or 1))
+        o.accelerations(state["px"], state["py"], state["pz"], state["m"], eps=eps)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            o.accelerations(state["px"], state["py"], state["pz"], state["m"], eps=eps)
+        t = (time.perf_counter() - t0) / 3
+        out["cpu_baseline"] = {"value": n / t, "unit": "particles/s", "cores": o.workers, "kind": "port",
+                               "sample": f"full {n}-body Accelerations, mean of 3 after 1 warm-up"}
+    return out
+
+
+def bench_clumps(args):
+    """Clustered stress case: 16,777,216 stars in 1,000 Plummer clumps, theta
+    0.5, eps 1e-4; precision ladder FP64 / FP32 / FP32 + FP16 quadrupoles."""
+    from gravitas_amd import DeviceState, Params, Solver, ics
+    n = args.n or 16777216
+    theta = 0.5 if args.theta is None else args.theta
+    eps = 1e-4 if args.eps is None else args.eps
+    log(f"building clumps n={n}")
+    state = ics.clumps(n, 1000, seed=5)
+    ds = DeviceState.from_host(state)
+    params = Params(softening=eps)
+    targets = ics.sample_targets(n, 4096, 7)
+    direct = Solver().direct_sample(ds, params, targets).numpy()
+    ladder, accs = {}, {}
+    for walk in ("fp64", "fp32", "fp32-q16"):
+        log(f"walk {walk}")
+        solver = Solver(theta=theta, walk=walk, timing=True)
+        steps = 2 if walk == "fp64" else args.steps
+        ms, ph, st = timed_steps(solver, ds, params, steps, 1)
+        accs[walk] = ds.acc_host()
+        ladder[walk] = {"ms_per_step": ms, "particles_per_s": n / (ms * 1e-3), "phases_ms": ph,
+                        "err_vs_direct": errors(accs[walk][targets], direct)}
+        if walk != "fp64":
+            ladder[walk]["err_vs_fp64_walk"] = errors(accs[walk], accs["fp64"])
+            ladder[walk].update(walk_summary(n, st, ph))
+    ms = ladder["fp32"]["ms_per_step"]
+    out = base_line(args, n, ms, "fp32", {"workload": "clumps-16m", "n": n, "theta": theta, "eps": eps,
+                                          "model": "1000 Plummer clumps (scale 0.005), log-normal masses, unit cube"})
+    out["ladder"] = ladder
+    out["roofline"] = ladder["fp32"]["roofline"]
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
+              "the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    from gravitas_amd import gpu_available
+    reason = gpu_available()
+    if reason is not None:
+        print(f"no GPU backend: {reason}", file=sys.stderr)
+        sys.exit(1)
+    fn = {"disk-halo-1m": bench_disk_halo, "plummer-16k": bench_plummer, "clumps-16m": bench_clumps}[args.config]
+    print(json.dumps(fn(args)), flush=True)
 
 
 if __name__ == "__main__":

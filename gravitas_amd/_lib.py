@@ -20,6 +20,7 @@ GX_ERR_CAPACITY = 6
 GX_ERR_UNSUPPORTED = 7
 GX_WALK_FP32 = 0
 GX_WALK_FP64 = 1
+GX_WALK_FP32_Q16 = 2
 
 # Every symbol include/gravitas_hip.h declares.
 EXPORTS = (

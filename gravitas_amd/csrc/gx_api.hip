@@ -74,7 +74,7 @@ struct gx_solver {
     DevBuf x, y, z, m, kind, alive, ax, ay, az;
     // arena
     DevBuf keys, keys2, order, order2, holes, hist, digitTotal, blockStar, blockHole, blockBox, spos;
-    DevBuf trav, m64, m32, meta, partial, targets;
+    DevBuf trav, m64, m32, m16, meta, partial, targets;
     DevBuf lfull, lpart, lleaf, lcounts;
     int cf = 2048, cp = 512, cl = 256;  // FP32-walk list capacities per group (grow on demand)
     long long capNodes = 0;
@@ -161,7 +161,7 @@ void gx_solver_free(gx_solver* s) {
     DevBuf* bufs[] = {&s->x,     &s->y,         &s->z,         &s->m,        &s->kind,      &s->alive,
                       &s->ax,    &s->ay,        &s->az,        &s->keys,     &s->keys2,     &s->order,
                       &s->order2, &s->holes,    &s->hist,      &s->digitTotal, &s->blockStar, &s->blockHole,
-                      &s->blockBox, &s->spos,   &s->trav,      &s->m64,      &s->m32,       &s->meta,
+                      &s->blockBox, &s->spos,   &s->trav,      &s->m64,      &s->m32,  &s->m16,     &s->meta,
                       &s->partial, &s->targets, &s->lfull, &s->lpart, &s->lleaf, &s->lcounts};
     for (DevBuf* b : bufs)
         if (b->p) hipFree(b->p);
@@ -174,7 +174,7 @@ void gx_solver_free(gx_solver* s) {
 
 int gx_solver_config(gx_solver* s, double theta, int monopole_only, int walk, int timing) {
     if (!s) return set_err(GX_ERR_ARG, "null solver");
-    if (!(theta >= 0) || walk < GX_WALK_FP32 || walk > GX_WALK_FP64)
+    if (!(theta >= 0) || walk < GX_WALK_FP32 || walk > GX_WALK_FP32_Q16)
         return set_err(GX_ERR_ARG, "bad theta %g or walk %d", theta, walk);
     s->theta = theta;
     s->monopole = monopole_only ? 1 : 0;
@@ -210,11 +210,12 @@ int reserve(gx_solver* s, long long n) {
     if (s->capNodes < wantNodes) s->capNodes = wantNodes;
     if ((rc = ensure(s->trav, (size_t)s->capNodes * sizeof(TravNode))) ||
         (rc = ensure(s->m64, (size_t)s->capNodes * sizeof(Mult64))) ||
-        (rc = ensure(s->m32, (size_t)s->capNodes * sizeof(Mult32))))
+        (rc = ensure(s->m32, (size_t)s->capNodes * sizeof(Mult32))) ||
+        (rc = ensure(s->m16, (size_t)s->capNodes * sizeof(Mult16))))
         return rc;
     // ensure() may have over-allocated: use all of it
     s->capNodes = (long long)std::min({s->trav.bytes / sizeof(TravNode), s->m64.bytes / sizeof(Mult64),
-                                       s->m32.bytes / sizeof(Mult32)});
+                                       s->m32.bytes / sizeof(Mult32), s->m16.bytes / sizeof(Mult16)});
     return GX_OK;
 }
 
@@ -272,7 +273,8 @@ int enqueue_build(gx_solver* s, const double* x, const double* y, const double* 
     mark(s, 5);
     for (int L = kMortonBits; L >= 0; L--)
         hipLaunchKernelGGL(k_moments_level, dim3(kTreeGrid), dim3(256), 0, st, L, (const double4*)s->spos.p,
-                           (TravNode*)s->trav.p, (Mult64*)s->m64.p, (Mult32*)s->m32.p, meta, theta);
+                           (TravNode*)s->trav.p, (Mult64*)s->m64.p, (Mult32*)s->m32.p, (Mult16*)s->m16.p, meta,
+                           theta);
     hipLaunchKernelGGL(k_tree_finish, dim3(1), dim3(64), 0, st, meta);
     mark(s, 6);
     HIPCHK(hipGetLastError());
@@ -283,7 +285,7 @@ int enqueue_walk(gx_solver* s, int walk, long long n, double eps, double* ax, do
     hipStream_t st = s->stream;
     Meta* meta = (Meta*)s->meta.p;
     const double eps2 = eps * eps;
-    if (walk == GX_WALK_FP32) {
+    if (walk == GX_WALK_FP32 || walk == GX_WALK_FP32_Q16) {
         const long long ng = std::max(1LL, (n + 63) / 64);
         int rc;
         if ((rc = ensure(s->lfull, (size_t)ng * s->cf * 4)) || (rc = ensure(s->lpart, (size_t)ng * s->cp * 16)) ||
@@ -298,8 +300,9 @@ int enqueue_walk(gx_solver* s, int walk, long long n, double eps, double* ax, do
         lists.cp = s->cp;
         lists.cl = s->cl;
         launch_walk32((const double4*)s->spos.p, (const TravNode*)s->trav.p, (const Mult32*)s->m32.p,
-                      (const int32_t*)s->order.p, meta, (n + 63) / 64, (float)eps2, !s->monopole, lists, ax, ay, az,
-                      st, s->timing ? s->ev[7] : nullptr, s->splitWalk);
+                      walk == GX_WALK_FP32_Q16 ? (const Mult16*)s->m16.p : nullptr, (const int32_t*)s->order.p, meta,
+                      (n + 63) / 64, (float)eps2, !s->monopole, lists, ax, ay, az, st, s->timing ? s->ev[7] : nullptr,
+                      s->splitWalk);
     } else {
         hipLaunchKernelGGL(k_walk64, dim3(blocks(n, 256)), dim3(256), 0, st, (const double4*)s->spos.p,
                            (const TravNode*)s->trav.p, (const Mult64*)s->m64.p, (const int32_t*)s->order.p, meta,
@@ -340,7 +343,7 @@ int finish_step(gx_solver* s) {
         hipEventElapsedTime(&s->phase[6], s->ev[6], s->ev[kPhases]);
         s->phase[7] = total;
         s->phase[8] = 0;
-        if (s->lastWalk == GX_WALK_FP32 && s->splitWalk) {  // traversal | evaluation split
+        if (s->lastWalk != GX_WALK_FP64 && s->splitWalk) {  // traversal | evaluation split
             float trav = 0;
             hipEventElapsedTime(&trav, s->ev[6], s->ev[7]);
             s->phase[8] = s->phase[6] - trav;

@@ -18,6 +18,8 @@
 // octant order, and skip links follow the reference's rule, so the walk order
 // -- which alone determines the forces -- equals the reference's.  Only the
 // numbering of nodes differs (breadth-first instead of depth-first).
+#include <hip/hip_fp16.h>
+
 #include "gx_common.h"
 #include "gx_kernels.h"
 
@@ -351,6 +353,7 @@ __global__ void k_tree_init(Meta* meta, TravNode* __restrict__ trav) {
     if (nst <= 0) return;
     double side = box_side(meta);
     double half = side / 2;
+    meta->rootHalf = half;
     TravNode t;
     t.x = meta->box[0] + half;
     t.y = meta->box[1] + half;
@@ -499,7 +502,8 @@ __device__ __forceinline__ void quad_add(double q[6], double m, double dx, doubl
 // reference's order (finish, build.go:176-217).  While building, the node's
 // record holds the cell centre in x,y,z and the half side in crit2.
 __device__ void node_moments(long long ni, const double4* __restrict__ spos, TravNode* __restrict__ trav,
-                             Mult64* __restrict__ m64, Mult32* __restrict__ m32, double theta) {
+                             Mult64* __restrict__ m64, Mult32* __restrict__ m32, Mult16* __restrict__ m16,
+                             double theta) {
     TravNode nd = trav[ni];
     const double cx = nd.x, cy = nd.y, cz = nd.z, half = nd.crit2;
     double x = 0, y = 0, z = 0, m = 0;
@@ -562,6 +566,11 @@ __device__ void node_moments(long long ni, const double4* __restrict__ spos, Tra
     mf.qxy = (float)q[3]; mf.qxz = (float)q[4]; mf.qyz = (float)q[5];
     mf.pad = 0;
     m32[ni] = mf;
+    Mult16 mh;
+    mh.m = (float)m;
+    const double inv = m > 0 ? 1.0 / (m * half * half) : 0.0;
+    for (int c = 0; c < 6; c++) mh.q[c] = __half_as_ushort(__float2half_rn((float)(q[c] * inv)));
+    m16[ni] = mh;
     nd.x = x;
     nd.y = y;
     nd.z = z;
@@ -571,14 +580,14 @@ __device__ void node_moments(long long ni, const double4* __restrict__ spos, Tra
 
 __global__ __launch_bounds__(256) void k_moments_level(int L, const double4* __restrict__ spos,
                                                        TravNode* __restrict__ trav, Mult64* __restrict__ m64,
-                                                       Mult32* __restrict__ m32, const Meta* __restrict__ meta,
-                                                       double theta) {
+                                                       Mult32* __restrict__ m32, Mult16* __restrict__ m16,
+                                                       const Meta* __restrict__ meta, double theta) {
     const long long start = meta->levelStart[L];
     const long long cnt = meta->levelCount[L];
     if (meta->flags & kFlagNodeOverflow) return;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
          t += (long long)gridDim.x * blockDim.x)
-        node_moments(start + t, spos, trav, m64, m32, theta);
+        node_moments(start + t, spos, trav, m64, m32, m16, theta);
 }
 
 __global__ void k_tree_finish(Meta* meta) {

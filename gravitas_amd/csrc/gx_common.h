@@ -37,9 +37,18 @@ struct alignas(32) Mult32 {
     float m, qxx, qyy, qzz, qxy, qxz, qyz, pad;
 };
 
+// FP16-stored multipole: m in FP32, the quadrupole scaled by 1/(m h^2) with h
+// the cell's half side (|Q_ij| <= 36 m h^2 because every star of a cell lies
+// within 2 sqrt(3) h of its centre of mass) and rounded to FP16.
+struct alignas(16) Mult16 {
+    float m;
+    uint16_t q[6];  // qxx, qyy, qzz, qxy, qxz, qyz as IEEE half
+};
+
 // Device-side metadata of one build.
 struct Meta {
     double box[6];          // minX, minY, minZ, maxX, maxY, maxZ of the tree stars
+    double rootHalf;        // half side of the root cell
     long long nstars;
     long long nholes;
     long long nnodes;

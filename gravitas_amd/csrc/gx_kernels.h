@@ -25,7 +25,7 @@ __global__ void k_gather(const double* x, const double* y, const double* z, cons
 __global__ void k_tree_init(Meta* meta, TravNode* trav);
 __global__ void k_topo_level(int L, const uint64_t* keys, TravNode* trav, Meta* meta, long long cap);
 __global__ void k_moments_level(int L, const double4* spos, TravNode* trav, Mult64* m64, Mult32* m32,
-                                const Meta* meta, double theta);
+                                Mult16* m16, const Meta* meta, double theta);
 __global__ void k_tree_finish(Meta* meta);
 __global__ void k_zero_holes(const int32_t* holes, const Meta* meta, double* ax, double* ay, double* az);
 __global__ void k_kick(long long n, double* vx, double* vy, double* vz, const double* ax, const double* ay,
@@ -56,8 +56,9 @@ struct Lists {
     int cf, cp, cl;  // capacities per group
 };
 
-void launch_walk32(const double4* spos, const TravNode* trav, const Mult32* m32, const int32_t* order, Meta* meta,
-                   long long ngroupsUpper, float eps2, int quad, const Lists& lists, double* ax, double* ay,
-                   double* az, hipStream_t stream, hipEvent_t mid, bool split);
+// m16 non-null selects the FP16-stored quadrupoles.
+void launch_walk32(const double4* spos, const TravNode* trav, const Mult32* m32, const Mult16* m16,
+                   const int32_t* order, Meta* meta, long long ngroupsUpper, float eps2, int quad, const Lists& lists,
+                   double* ax, double* ay, double* az, hipStream_t stream, hipEvent_t mid, bool split);
 
 }  // namespace gx

@@ -22,6 +22,8 @@
 // Interactions follow bh.go:117-133 (cells, monopole + quadrupole) and
 // bh.go:145-150 (stars, Plummer softening).  Each list flush is summed in
 // FP32 and added to the per-star total with Kahan compensation.
+#include <hip/hip_fp16.h>
+
 #include "gx_common.h"
 #include "gx_kernels.h"
 
@@ -319,11 +321,12 @@ __device__ bool trav_group(const Group& G, TravLDS& S, const TravNode* __restric
 // ---------------------------------------------------------------------------
 // evaluation: one wavefront per group, lists from HBM
 // ---------------------------------------------------------------------------
-template <bool QUAD, bool EPS0>
+template <bool QUAD, bool EPS0, bool Q16>
 __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict__ spos,
                            const TravNode* __restrict__ trav, const Mult32* __restrict__ m32,
-                           const int32_t* __restrict__ order, float eps2, const Lists& lists,
-                           double* __restrict__ oax, double* __restrict__ oay, double* __restrict__ oaz) {
+                           const Mult16* __restrict__ m16, double rootHalf, const int32_t* __restrict__ order,
+                           float eps2, const Lists& lists, double* __restrict__ oax, double* __restrict__ oay,
+                           double* __restrict__ oaz) {
     float4* cell = S.cell;
     uint64_t* cmask = S.cmask;
     float4* part = S.part;
@@ -346,10 +349,20 @@ __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict
     auto fetch = [&](int node, uint64_t m) {
         Staged d;
         const TravNode& t = trav[node];
-        const Mult32 mm = m32[node];
-        d.a = make_float4((float)(t.x - G.cgx), (float)(t.y - G.cgy), (float)(t.z - G.cgz), mm.m);
-        d.b = make_float4(mm.qxx, mm.qyy, mm.qzz, mm.qxy);
-        d.c = make_float2(mm.qxz, mm.qyz);
+        if (Q16) {
+            const Mult16 mh = m16[node];
+            const float h = (float)ldexp(rootHalf, -t.level);
+            const float sc = mh.m * h * h;
+            auto q = [&](int c) { return __half2float(__ushort_as_half(mh.q[c])) * sc; };
+            d.a = make_float4((float)(t.x - G.cgx), (float)(t.y - G.cgy), (float)(t.z - G.cgz), mh.m);
+            d.b = make_float4(q(0), q(1), q(2), q(3));
+            d.c = make_float2(q(4), q(5));
+        } else {
+            const Mult32 mm = m32[node];
+            d.a = make_float4((float)(t.x - G.cgx), (float)(t.y - G.cgy), (float)(t.z - G.cgz), mm.m);
+            d.b = make_float4(mm.qxx, mm.qyy, mm.qzz, mm.qxy);
+            d.c = make_float2(mm.qxz, mm.qyz);
+        }
         d.m = m;
         return d;
     };
@@ -459,65 +472,71 @@ __global__ __launch_bounds__(64) void k_trav32(const double4* __restrict__ spos,
     trav_group(G, S, trav, meta, lists);
 }
 
-template <bool QUAD, bool EPS0>
+template <bool QUAD, bool EPS0, bool Q16>
 __global__ __launch_bounds__(64) void k_eval32(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
-                                               const Mult32* __restrict__ m32, const int32_t* __restrict__ order,
-                                               Meta* meta, float eps2, Lists lists, double* __restrict__ oax,
-                                               double* __restrict__ oay, double* __restrict__ oaz) {
+                                               const Mult32* __restrict__ m32, const Mult16* __restrict__ m16,
+                                               const int32_t* __restrict__ order, Meta* meta, float eps2,
+                                               Lists lists, double* __restrict__ oax, double* __restrict__ oay,
+                                               double* __restrict__ oaz) {
     __shared__ EvalLDS S;
     if (meta->flags & (kFlagNodeOverflow | kFlagStackOverflow | kFlagListOverflow)) return;
     Group G;
     if (!load_group(G, spos, meta->nstars)) return;
-    eval_group<QUAD, EPS0>(G, S, spos, trav, m32, order, eps2, lists, oax, oay, oaz);
+    eval_group<QUAD, EPS0, Q16>(G, S, spos, trav, m32, m16, meta->rootHalf, order, eps2, lists, oax, oay, oaz);
 }
 
 // Traversal and evaluation of one group in one wavefront: the latency-bound
 // traversal of some waves overlaps the VALU-bound evaluation of others on the
 // same SIMD.  The lists still go through HBM (they stay in L2/MALL between
 // the two phases), so LDS holds only one phase's buffers at a time.
-template <bool QUAD, bool EPS0>
+template <bool QUAD, bool EPS0, bool Q16>
 __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
-                                               const Mult32* __restrict__ m32, const int32_t* __restrict__ order,
-                                               Meta* meta, float eps2, Lists lists, double* __restrict__ oax,
-                                               double* __restrict__ oay, double* __restrict__ oaz) {
+                                               const Mult32* __restrict__ m32, const Mult16* __restrict__ m16,
+                                               const int32_t* __restrict__ order, Meta* meta, float eps2,
+                                               Lists lists, double* __restrict__ oax, double* __restrict__ oay,
+                                               double* __restrict__ oaz) {
     __shared__ WalkLDS S;
     if (meta->flags & kFlagNodeOverflow) return;
     Group G;
     if (!load_group(G, spos, meta->nstars)) return;
     const bool ok = trav_group(G, S.t, trav, meta, lists);
     __syncthreads();  // lists written by this wave's lanes are read back by others
-    if (ok) eval_group<QUAD, EPS0>(G, S.e, spos, trav, m32, order, eps2, lists, oax, oay, oaz);
+    if (ok)
+        eval_group<QUAD, EPS0, Q16>(G, S.e, spos, trav, m32, m16, meta->rootHalf, order, eps2, lists, oax, oay,
+                                    oaz);
 }
 
-void launch_walk32(const double4* spos, const TravNode* trav, const Mult32* m32, const int32_t* order, Meta* meta,
-                   long long ngroupsUpper, float eps2, int quad, const Lists& lists, double* ax, double* ay,
-                   double* az, hipStream_t stream, hipEvent_t mid, bool split) {
+void launch_walk32(const double4* spos, const TravNode* trav, const Mult32* m32, const Mult16* m16,
+                   const int32_t* order, Meta* meta, long long ngroupsUpper, float eps2, int quad, const Lists& lists,
+                   double* ax, double* ay, double* az, hipStream_t stream, hipEvent_t mid, bool split) {
     if (ngroupsUpper <= 0) return;
     const long long per = (ngroupsUpper + 7) / 8;
     dim3 grid((unsigned)(per * 8)), block(64);
     const bool eps0 = !(eps2 > 0.0f);
-    if (!split) {
-#define GX_WALK(Q, E) \
-    hipLaunchKernelGGL((k_walk32<Q, E>), grid, block, 0, stream, spos, trav, m32, order, meta, eps2, lists, ax, ay, az)
-        if (quad) {
-            if (eps0) GX_WALK(true, true); else GX_WALK(true, false);
-        } else {
-            if (eps0) GX_WALK(false, true); else GX_WALK(false, false);
-        }
-#undef GX_WALK
+    const bool q16 = m16 != nullptr;
+    if (split) {
+        hipLaunchKernelGGL(k_trav32, grid, block, 0, stream, spos, trav, meta, lists);
         if (mid) (void)hipEventRecord(mid, stream);
-        return;
     }
-    hipLaunchKernelGGL(k_trav32, grid, block, 0, stream, spos, trav, meta, lists);
-    if (mid) (void)hipEventRecord(mid, stream);
-#define GX_EVAL(Q, E) \
-    hipLaunchKernelGGL((k_eval32<Q, E>), grid, block, 0, stream, spos, trav, m32, order, meta, eps2, lists, ax, ay, az)
+#define GX_LAUNCH(Q, E, H)                                                                                    \
+    do {                                                                                                      \
+        if (split)                                                                                            \
+            hipLaunchKernelGGL((k_eval32<Q, E, H>), grid, block, 0, stream, spos, trav, m32, m16, order, meta, eps2, \
+                               lists, ax, ay, az);                                                            \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_walk32<Q, E, H>), grid, block, 0, stream, spos, trav, m32, m16, order, meta, eps2, \
+                               lists, ax, ay, az);                                                            \
+    } while (0)
+#define GX_LAUNCH_E(Q, H) \
+    if (eps0) GX_LAUNCH(Q, true, H); else GX_LAUNCH(Q, false, H)
     if (quad) {
-        if (eps0) GX_EVAL(true, true); else GX_EVAL(true, false);
+        if (q16) { GX_LAUNCH_E(true, true); } else { GX_LAUNCH_E(true, false); }
     } else {
-        if (eps0) GX_EVAL(false, true); else GX_EVAL(false, false);
+        GX_LAUNCH_E(false, false);
     }
-#undef GX_EVAL
+#undef GX_LAUNCH_E
+#undef GX_LAUNCH
+    if (!split && mid) (void)hipEventRecord(mid, stream);
 }
 
 }  // namespace gx

@@ -93,8 +93,9 @@ class Solver:
 
     theta: opening angle, 0 means DefaultTheta (bh.go:69-74).
     monopole_only: drop the quadrupole term (bh.go:33-35).
-    walk: "fp32" (group walk, the default) or "fp64" (per-particle walk,
-          bit-identical to the reference's CPU walk).
+    walk: "fp32" (group walk, the default), "fp32-q16" (the same with
+          FP16-stored quadrupoles) or "fp64" (per-particle walk, bit-identical
+          to the reference's CPU walk).
     """
 
     def __init__(self, theta: float = 0.0, monopole_only: bool = False, walk: str = "fp32", timing: bool = False):
@@ -120,7 +121,7 @@ class Solver:
             pass
 
     def _configure(self):
-        w = {"fp32": _lib.GX_WALK_FP32, "fp64": _lib.GX_WALK_FP64}[self.walk]
+        w = {"fp32": _lib.GX_WALK_FP32, "fp64": _lib.GX_WALK_FP64, "fp32-q16": _lib.GX_WALK_FP32_Q16}[self.walk]
         _lib.check(self._L.gx_solver_config(self._h, float(self.theta), int(self.monopole_only), w,
                                             int(self.timing)))
 

@@ -39,6 +39,7 @@ extern "C" {
 /* Walk variants. */
 #define GX_WALK_FP32 0 /* FP32 group walk, reference interaction set (default) */
 #define GX_WALK_FP64 1 /* FP64 per-particle walk, bit-identical to the reference */
+#define GX_WALK_FP32_Q16 2 /* FP32 group walk with FP16-stored quadrupoles (scaled by 1/(m h^2)) */
 
 /* Body kinds, internal/sim/state.go:11-20. */
 #define GX_KIND_STAR 0

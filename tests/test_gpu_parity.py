@@ -110,12 +110,19 @@ def test_fp64_walk_and_tree_bit_identical(gpu, case):
     assert np.array_equal(g["acc"], want)
 
 
+# FP32 walk bounds vs the FP64 walk: the reference's Metal bound (median 1e-5,
+# max 1e-3, gpu_test.go:18) plus p99 1e-4; FP16-stored quadrupoles keep the
+# BASELINE gate p99 <= 1e-3.
+BOUNDS = {"fp32": (1e-5, 1e-4, 1e-3), "fp32-q16": (1e-4, 1e-3, 1e-2)}
+
+
+@pytest.mark.parametrize("walk", list(BOUNDS))
 @pytest.mark.parametrize("case", list(CASES))
-def test_fp32_walk_matches_fp64_walk(gpu, case):
+def test_fp32_walk_matches_fp64_walk(gpu, case, walk):
     make, theta, eps, dm_off = CASES[case]
     state = make()
-    say(f"[parity] case={case} n={len(state['px'])} theta={theta} eps={eps} walk=fp32 vs fp64")
-    g32 = run_gpu(state, theta, eps, "fp32", dm_off)
+    say(f"[parity] case={case} n={len(state['px'])} theta={theta} eps={eps} walk={walk} vs fp64")
+    g32 = run_gpu(state, theta, eps, walk, dm_off)
     g64 = run_gpu(state, theta, eps, "fp64", dm_off)
     # identical interaction set
     assert g32["stats"]["pp_interactions"] == g64["stats"]["pp_interactions"]
@@ -125,8 +132,9 @@ def test_fp32_walk_matches_fp64_walk(gpu, case):
     star = alive & (state["kind"] != ics.KIND_HOLE)
     e = np.sort(ics.rel_errors(g32["acc"][star], g64["acc"][star]))
     med, p99, mx = e[len(e) // 2], e[int(0.99 * (len(e) - 1))], e[-1]
-    say(f"[parity] {case}: fp32 vs fp64 median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
-    assert med <= 1e-5 and p99 <= 1e-4 and mx <= 1e-3
+    say(f"[parity] {case}: {walk} vs fp64 median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
+    bmed, bp99, bmax = BOUNDS[walk]
+    assert med <= bmed and p99 <= bp99 and mx <= bmax
 
 
 def test_two_stars_known_answer(gpu):
