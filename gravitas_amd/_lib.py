@@ -27,7 +27,8 @@ EXPORTS = (
     "gx_init", "gx_name", "gx_error", "gx_solver_new", "gx_solver_free", "gx_solver_config",
     "gx_solver_set_stream", "gx_accelerations", "gx_accelerations_dev", "gx_enqueue_accelerations_dev",
     "gx_sync", "gx_potential_energy_dev", "gx_direct_sample_dev", "gx_last_seconds", "gx_get_stats",
-    "gx_phase_ms", "gx_export", "gx_step_dev", "gx_kick_dev", "gx_drift_dev",
+    "gx_phase_ms", "gx_export", "gx_step_dev", "gx_kick_dev", "gx_drift_dev", "gx_solver_tune",
+    "gx_batch_accelerations_dev", "gx_batch_step_dev", "gx_graph_capture_step", "gx_graph_launch",
 )
 
 
@@ -87,6 +88,11 @@ def load():
     L.gx_step_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, I64, D, I, D, I, P, P, P]
     L.gx_kick_dev.argtypes = [P, P, P, P, P, P, P, I64, D]
     L.gx_drift_dev.argtypes = [P, P, P, P, P, P, P, I64, D]
+    L.gx_solver_tune.argtypes = [P, I, I]
+    L.gx_batch_accelerations_dev.argtypes = [P, P, P, P, P, P, P, P, ctypes.c_int32, P, P, I, P, P, P]
+    L.gx_batch_step_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, P, ctypes.c_int32, P, P, I, D, I, P, P, P]
+    L.gx_graph_capture_step.argtypes = [P, P, P, P, P, P, P, P, P, P, P, ctypes.c_int32, P, P, I, D, P, P, P]
+    L.gx_graph_launch.argtypes = [P, ctypes.c_int32]
     L.gx_export.argtypes = [P, I, P, I64]
     L.gx_export.restype = I64
     _lib = L

@@ -1,11 +1,12 @@
 // gx_api.hip -- C ABI and step orchestration (host side of the backend).
 //
-// One call of the force step enqueues, on one stream and with no host
-// synchronisation in between: collect (3 launches), keys, 8 radix passes
-// (3 launches each), gather, one topology launch per tree level, one moment
+// One force step enqueues, on one stream and with no host synchronisation in
+// between: step init, collect (3 launches), keys, the radix passes (3
+// launches each), gather, one topology launch per tree level, one moment
 // launch per level, the walk and the hole fix-up.  Sizes that only the device
-// knows (number of tree stars, nodes per level) are read by the kernels from
-// the Meta record; grids are sized from the body count n.
+// knows (stars per instance, nodes per level, walk groups) are read by the
+// kernels from the Meta record and the instance arrays; grids are sized from
+// the body count n and the instance count B.  A single system is B = 1.
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -13,6 +14,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <vector>
 
 #include "gravitas_hip.h"
 #include "gx_common.h"
@@ -32,9 +34,9 @@ int set_err(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIPCHK(x)                                                                       \
-    do {                                                                                \
-        hipError_t e_ = (x);                                                            \
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
         if (e_ != hipSuccess) return set_err(GX_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
     } while (0)
 
@@ -61,6 +63,15 @@ int ensure(DevBuf& b, size_t bytes) {
 constexpr int kPhases = 9;
 constexpr int kTreeGrid = 1024;
 
+// The arrays of one step.
+struct StepArgs {
+    const double *x, *y, *z, *m;
+    const uint8_t *kind, *alive;
+    long long n;
+    int dmOff;
+    double *ax, *ay, *az;
+};
+
 }  // namespace
 
 struct gx_solver {
@@ -68,30 +79,39 @@ struct gx_solver {
     int monopole = 0;
     int walk = GX_WALK_FP32;
     int timing = 0;
+    int gsize = 64;           // stars per walk group
+    int persistent = 0;       // FP32 walk as a work queue over resident workgroups
+    int persistentBlocks = 0;
+    bool splitWalk = false;   // GX_SPLIT_WALK=1: traversal and evaluation as two kernels
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     // host-API staging
     DevBuf x, y, z, m, kind, alive, ax, ay, az;
     // arena
-    DevBuf keys, keys2, order, order2, holes, hist, digitTotal, blockStar, blockHole, blockBox, spos;
-    DevBuf trav, m64, m32, m16, meta, partial, targets;
+    DevBuf keys, keys2, order, order2, holes, hist, digitTotal, blockStar, blockHole, spos;
+    DevBuf trav, m64, m32, m16, meta, partial;
     DevBuf lfull, lpart, lleaf, lcounts;
     int cf = 2048, cp = 512, cl = 256;  // FP32-walk list capacities per group (grow on demand)
     long long capNodes = 0;
+    // instances: device config (offsets | theta | eps2) and per-instance results
+    DevBuf instCfg, boxEnc, starCount, starOff, groupOff, root, half;
+    char* instHost = nullptr;  // pinned staging of the config
+    size_t instHostBytes = 0;
+    std::vector<char> instLast;  // last uploaded config
+    hipEvent_t instCopied = nullptr;
+    int B = 0;
+    bool eps0 = false;
     Meta* hmeta = nullptr;  // pinned mirror of the device Meta
     hipEvent_t ev[kPhases + 1] = {};
     float phase[kPhases] = {};
     double lastSeconds = 0;
     gx_stats stats = {};
     long long fallbacks = 0;
-    // the pending step (for gx_sync)
     bool pending = false;
-    long long pendingN = 0;
     int lastWalk = GX_WALK_FP32;
-    // FP32 walk as two kernels (traversal, evaluation) instead of one; set
-    // from GX_SPLIT_WALK=1 for profiling the two phases separately.
-    bool splitWalk = false;
-
+    // captured KDK step
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
 };
 
 extern "C" {
@@ -141,8 +161,9 @@ gx_solver* gx_solver_new(void) {
     }
     gx_solver* s = new gx_solver();
     if (hipStreamCreateWithFlags(&s->own, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void**)&s->hmeta, sizeof(Meta), hipHostMallocDefault) != hipSuccess) {
-        set_err(GX_ERR_HIP, "stream or pinned allocation failed");
+        hipHostMalloc((void**)&s->hmeta, sizeof(Meta), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&s->instCopied, hipEventDisableTiming) != hipSuccess) {
+        set_err(GX_ERR_HIP, "stream, event or pinned allocation failed");
         delete s;
         return nullptr;
     }
@@ -151,24 +172,32 @@ gx_solver* gx_solver_new(void) {
     for (auto& e : s->ev) (void)hipEventCreate(&e);
     const char* sw = getenv("GX_SPLIT_WALK");
     s->splitWalk = sw && sw[0] == '1';
-
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+        s->persistentBlocks = cus * std::max(walk32_blocks_per_cu(), 1);
     return s;
 }
 
 void gx_solver_free(gx_solver* s) {
     if (!s) return;
-    hipStreamSynchronize(s->stream);
-    DevBuf* bufs[] = {&s->x,     &s->y,         &s->z,         &s->m,        &s->kind,      &s->alive,
-                      &s->ax,    &s->ay,        &s->az,        &s->keys,     &s->keys2,     &s->order,
-                      &s->order2, &s->holes,    &s->hist,      &s->digitTotal, &s->blockStar, &s->blockHole,
-                      &s->blockBox, &s->spos,   &s->trav,      &s->m64,      &s->m32,  &s->m16,     &s->meta,
-                      &s->partial, &s->targets, &s->lfull, &s->lpart, &s->lleaf, &s->lcounts};
+    (void)hipStreamSynchronize(s->stream);
+    if (s->graphExec) (void)hipGraphExecDestroy(s->graphExec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    DevBuf* bufs[] = {&s->x,        &s->y,       &s->z,       &s->m,         &s->kind,      &s->alive,
+                      &s->ax,       &s->ay,      &s->az,      &s->keys,      &s->keys2,     &s->order,
+                      &s->order2,   &s->holes,   &s->hist,    &s->digitTotal, &s->blockStar, &s->blockHole,
+                      &s->spos,     &s->trav,    &s->m64,     &s->m32,       &s->m16,       &s->meta,
+                      &s->partial,  &s->lfull,   &s->lpart,   &s->lleaf,     &s->lcounts,   &s->instCfg,
+                      &s->boxEnc,   &s->starCount, &s->starOff, &s->groupOff, &s->root,      &s->half};
     for (DevBuf* b : bufs)
-        if (b->p) hipFree(b->p);
+        if (b->p) (void)hipFree(b->p);
     for (auto& e : s->ev)
-        if (e) hipEventDestroy(e);
-    if (s->hmeta) hipHostFree(s->hmeta);
-    if (s->own) hipStreamDestroy(s->own);
+        if (e) (void)hipEventDestroy(e);
+    if (s->instCopied) (void)hipEventDestroy(s->instCopied);
+    if (s->instHost) (void)hipHostFree(s->instHost);
+    if (s->hmeta) (void)hipHostFree(s->hmeta);
+    if (s->own) (void)hipStreamDestroy(s->own);
     delete s;
 }
 
@@ -183,6 +212,14 @@ int gx_solver_config(gx_solver* s, double theta, int monopole_only, int walk, in
     return GX_OK;
 }
 
+int gx_solver_tune(gx_solver* s, int group_size, int persistent) {
+    if (!s) return set_err(GX_ERR_ARG, "null solver");
+    if (group_size != 32 && group_size != 64) return set_err(GX_ERR_ARG, "group size %d not in {32, 64}", group_size);
+    s->gsize = group_size;
+    s->persistent = persistent ? 1 : 0;
+    return GX_OK;
+}
+
 int gx_solver_set_stream(gx_solver* s, void* stream) {
     if (!s) return set_err(GX_ERR_ARG, "null solver");
     s->stream = stream ? (hipStream_t)stream : s->own;
@@ -193,7 +230,69 @@ int gx_solver_set_stream(gx_solver* s, void* stream) {
 
 namespace {
 
-double theta_of(const gx_solver* s) { return s->theta == 0 ? 0.9 : s->theta; }
+double theta_of(double theta) { return theta == 0 ? 0.9 : theta; }  // bh.go:69-74
+
+inline unsigned blocks(long long n, int per) { return (unsigned)std::max(1LL, (n + per - 1) / per); }
+
+// Instance configuration: body offsets (B+1), theta (B), eps^2 (B).  Uploaded
+// only when it changes, so repeated steps (and graph replays) reuse it.
+int set_instances(gx_solver* s, int B, const int64_t* off, const double* theta, double thetaAll,
+                  const double* eps, double epsAll) {
+    if (B < 1) return set_err(GX_ERR_ARG, "need at least one instance");
+    if (B > 65536) return set_err(GX_ERR_ARG, "at most 65536 instances");
+    const size_t bytes = (size_t)(B + 1) * 8 + (size_t)B * 16;
+    HIPCHK(hipEventSynchronize(s->instCopied));  // the previous upload has read the staging
+    if (s->instHostBytes < bytes) {
+        if (s->instHost) HIPCHK(hipHostFree(s->instHost));
+        s->instHost = nullptr;
+        s->instHostBytes = 0;
+        HIPCHK(hipHostMalloc((void**)&s->instHost, bytes, hipHostMallocDefault));
+        s->instHostBytes = bytes;
+    }
+    long long* o = (long long*)s->instHost;
+    double* th = (double*)(o + B + 1);
+    double* e2 = th + B;
+    bool eps0 = false;
+    for (int b = 0; b <= B; b++) o[b] = off[b];
+    for (int b = 0; b < B; b++) {
+        if (o[b + 1] < o[b]) return set_err(GX_ERR_ARG, "instance offsets must ascend");
+        th[b] = theta_of(theta ? theta[b] : thetaAll);
+        if (!(th[b] >= 0)) return set_err(GX_ERR_ARG, "bad theta of instance %d", b);
+        const double e = eps ? eps[b] : epsAll;
+        e2[b] = e * e;
+        eps0 = eps0 || !((float)e2[b] > 0.0f);
+    }
+    int rc;
+    if ((rc = ensure(s->instCfg, bytes)) || (rc = ensure(s->boxEnc, (size_t)B * 48)) ||
+        (rc = ensure(s->starCount, (size_t)B * 8)) || (rc = ensure(s->starOff, (size_t)(B + 1) * 8)) ||
+        (rc = ensure(s->groupOff, (size_t)(B + 1) * 8)) || (rc = ensure(s->root, (size_t)B * 4)) ||
+        (rc = ensure(s->half, (size_t)B * 8)))
+        return rc;
+    if (s->B != B || s->instLast.size() != bytes || memcmp(s->instLast.data(), s->instHost, bytes) != 0) {
+        HIPCHK(hipMemcpyAsync(s->instCfg.p, s->instHost, bytes, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipEventRecord(s->instCopied, s->stream));
+        s->instLast.assign(s->instHost, s->instHost + bytes);
+    }
+    s->B = B;
+    s->eps0 = eps0;
+    return GX_OK;
+}
+
+Batch make_batch(gx_solver* s) {
+    Batch b;
+    b.B = s->B;
+    b.gsize = s->gsize;
+    b.bodyOff = (const long long*)s->instCfg.p;
+    b.theta = (const double*)((const long long*)s->instCfg.p + s->B + 1);
+    b.eps2 = b.theta + s->B;
+    b.boxEnc = (unsigned long long*)s->boxEnc.p;
+    b.starCount = (unsigned long long*)s->starCount.p;
+    b.starOff = (long long*)s->starOff.p;
+    b.groupOff = (long long*)s->groupOff.p;
+    b.root = (int*)s->root.p;
+    b.half = (double*)s->half.p;
+    return b;
+}
 
 int reserve(gx_solver* s, long long n) {
     const long long nb = (n + kCollectTile - 1) / kCollectTile;
@@ -203,70 +302,75 @@ int reserve(gx_solver* s, long long n) {
         (rc = ensure(s->order2, n * 4)) || (rc = ensure(s->holes, n * 4)) ||
         (rc = ensure(s->hist, (size_t)256 * std::max(nt, 1LL) * 4)) || (rc = ensure(s->digitTotal, 256 * 4)) ||
         (rc = ensure(s->blockStar, std::max(nb, 1LL) * 4)) || (rc = ensure(s->blockHole, std::max(nb, 1LL) * 4)) ||
-        (rc = ensure(s->blockBox, std::max(nb, 1LL) * 48)) || (rc = ensure(s->spos, std::max(n, 1LL) * 32)) ||
-        (rc = ensure(s->meta, sizeof(Meta))))
+        (rc = ensure(s->spos, std::max(n, 1LL) * 32)) || (rc = ensure(s->meta, sizeof(Meta))))
         return rc;
     long long wantNodes = std::max(4096LL, n);
     if (s->capNodes < wantNodes) s->capNodes = wantNodes;
+    if (s->capNodes >= (1LL << 28)) return set_err(GX_ERR_CAPACITY, "more than 2^28 nodes");
     if ((rc = ensure(s->trav, (size_t)s->capNodes * sizeof(TravNode))) ||
         (rc = ensure(s->m64, (size_t)s->capNodes * sizeof(Mult64))) ||
         (rc = ensure(s->m32, (size_t)s->capNodes * sizeof(Mult32))) ||
         (rc = ensure(s->m16, (size_t)s->capNodes * sizeof(Mult16))))
         return rc;
-    // ensure() may have over-allocated: use all of it
+    // ensure() may have over-allocated: use all of it (node indices stay below 2^28)
     s->capNodes = (long long)std::min({s->trav.bytes / sizeof(TravNode), s->m64.bytes / sizeof(Mult64),
                                        s->m32.bytes / sizeof(Mult32), s->m16.bytes / sizeof(Mult16)});
+    s->capNodes = std::min(s->capNodes, (1LL << 28) - 1);
     return GX_OK;
 }
 
-inline unsigned blocks(long long n, int per) { return (unsigned)std::max(1LL, (n + per - 1) / per); }
-
 void mark(gx_solver* s, int i) {
-    if (s->timing) hipEventRecord(s->ev[i], s->stream);
+    if (s->timing) (void)hipEventRecord(s->ev[i], s->stream);
 }
 
-// Build the tree for the device arrays (collect .. moments).  theta is baked
-// into crit2 (build.go:209-211).
-int enqueue_build(gx_solver* s, const double* x, const double* y, const double* z, const double* m,
-                  const uint8_t* kind, const uint8_t* alive, long long n, int dmOff, double theta) {
+// Build the trees of all instances (collect .. moments).  theta is baked into
+// crit2 (build.go:209-211).
+int enqueue_build(gx_solver* s, const StepArgs& a) {
+    const long long n = a.n;
     int rc = reserve(s, n);
     if (rc) return rc;
     hipStream_t st = s->stream;
     Meta* meta = (Meta*)s->meta.p;
+    const Batch batch = make_batch(s);
     const long long nb = (n + kCollectTile - 1) / kCollectTile;
-    const int ntiles = (int)((n + kSortTile - 1) / kSortTile);
-    hipEventRecord(s->ev[0], st);
-    hipLaunchKernelGGL(k_collect_count, dim3(blocks(n, kCollectTile)), dim3(256), 0, st, x, y, z, kind, alive, n,
-                       (int*)s->blockStar.p, (int*)s->blockHole.p, (double*)s->blockBox.p);
+    const int ntiles = std::max(1, (int)((n + kSortTile - 1) / kSortTile));
+    mark(s, 0);
+    hipLaunchKernelGGL(k_step_init, dim3(blocks(s->B, 256)), dim3(256), 0, st, meta, batch);
+    hipLaunchKernelGGL(k_collect_count, dim3(blocks(n, kCollectTile)), dim3(256), 0, st, a.x, a.y, a.z, a.kind,
+                       a.alive, n, (int*)s->blockStar.p, (int*)s->blockHole.p, batch);
     hipLaunchKernelGGL(k_collect_scan, dim3(1), dim3(1024), 0, st, (int)std::max(nb, 1LL), (int*)s->blockStar.p,
-                       (int*)s->blockHole.p, (const double*)s->blockBox.p, meta);
-    hipLaunchKernelGGL(k_collect_scatter, dim3(blocks(n, kCollectTile)), dim3(256), 0, st, kind, alive, n,
+                       (int*)s->blockHole.p, meta, batch);
+    hipLaunchKernelGGL(k_collect_scatter, dim3(blocks(n, kCollectTile)), dim3(256), 0, st, a.kind, a.alive, n,
                        (const int*)s->blockStar.p, (const int*)s->blockHole.p, (int32_t*)s->order.p,
                        (int32_t*)s->holes.p);
     mark(s, 1);
-    hipLaunchKernelGGL(k_keys, dim3(blocks(n, 256)), dim3(256), 0, st, x, y, z, (const int32_t*)s->order.p, meta,
-                       (uint64_t*)s->keys.p);
+    hipLaunchKernelGGL(k_keys, dim3(blocks(n, 256)), dim3(256), 0, st, a.x, a.y, a.z, (const int32_t*)s->order.p,
+                       meta, batch, (uint64_t*)s->keys.p);
     mark(s, 2);
+    // 8 passes over the 63-bit keys, then (several instances) 2 passes over the
+    // instance id, most significant last; an even count ends in keys/order.
     uint64_t* kin = (uint64_t*)s->keys.p;
     uint64_t* kout = (uint64_t*)s->keys2.p;
     int32_t* vin = (int32_t*)s->order.p;
     int32_t* vout = (int32_t*)s->order2.p;
-    for (int pass = 0; pass < 8; pass++) {
-        const int shift = 8 * pass;
-        hipLaunchKernelGGL(k_sort_hist, dim3(std::max(ntiles, 1)), dim3(256), 0, st, kin, meta, shift,
-                           (uint32_t*)s->hist.p, std::max(ntiles, 1));
-        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(256), 0, st, (uint32_t*)s->hist.p, std::max(ntiles, 1),
+    const int passes = s->B > 1 ? 10 : 8;
+    for (int pass = 0; pass < passes; pass++) {
+        const int inst = pass >= 8;
+        const int shift = inst ? 8 * (pass - 8) : 8 * pass;
+        hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(256), 0, st, kin, vin, meta, shift, inst, batch,
+                           (uint32_t*)s->hist.p, ntiles);
+        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(256), 0, st, (uint32_t*)s->hist.p, ntiles,
                            (uint32_t*)s->digitTotal.p);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(std::max(ntiles, 1)), dim3(256), 0, st, kin, vin, kout, vout, meta,
-                           shift, (const uint32_t*)s->hist.p, (const uint32_t*)s->digitTotal.p, std::max(ntiles, 1));
+        hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(256), 0, st, kin, vin, kout, vout, meta, shift, inst,
+                           batch, (const uint32_t*)s->hist.p, (const uint32_t*)s->digitTotal.p, ntiles);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
     mark(s, 3);
-    hipLaunchKernelGGL(k_gather, dim3(blocks(n, 256)), dim3(256), 0, st, x, y, z, m, kind,
-                       (const int32_t*)s->order.p, meta, dmOff, (double4*)s->spos.p);
+    hipLaunchKernelGGL(k_gather, dim3(blocks(n, 256)), dim3(256), 0, st, a.x, a.y, a.z, a.m, a.kind,
+                       (const int32_t*)s->order.p, meta, a.dmOff, (double4*)s->spos.p);
     mark(s, 4);
-    hipLaunchKernelGGL(k_tree_init, dim3(1), dim3(64), 0, st, meta, (TravNode*)s->trav.p);
+    hipLaunchKernelGGL(k_tree_init, dim3(blocks(s->B, 256)), dim3(256), 0, st, meta, batch, (TravNode*)s->trav.p);
     for (int L = 0; L <= kMortonBits; L++)
         hipLaunchKernelGGL(k_topo_level, dim3(kTreeGrid), dim3(256), 0, st, L, (const uint64_t*)s->keys.p,
                            (TravNode*)s->trav.p, meta, s->capNodes);
@@ -274,19 +378,20 @@ int enqueue_build(gx_solver* s, const double* x, const double* y, const double* 
     for (int L = kMortonBits; L >= 0; L--)
         hipLaunchKernelGGL(k_moments_level, dim3(kTreeGrid), dim3(256), 0, st, L, (const double4*)s->spos.p,
                            (TravNode*)s->trav.p, (Mult64*)s->m64.p, (Mult32*)s->m32.p, (Mult16*)s->m16.p, meta,
-                           theta);
+                           batch);
     hipLaunchKernelGGL(k_tree_finish, dim3(1), dim3(64), 0, st, meta);
     mark(s, 6);
     HIPCHK(hipGetLastError());
     return GX_OK;
 }
 
-int enqueue_walk(gx_solver* s, int walk, long long n, double eps, double* ax, double* ay, double* az) {
+int enqueue_walk(gx_solver* s, int walk, const StepArgs& a) {
     hipStream_t st = s->stream;
     Meta* meta = (Meta*)s->meta.p;
-    const double eps2 = eps * eps;
+    const Batch batch = make_batch(s);
+    const long long n = a.n;
     if (walk == GX_WALK_FP32 || walk == GX_WALK_FP32_Q16) {
-        const long long ng = std::max(1LL, (n + 63) / 64);
+        const long long ng = (n + s->gsize - 1) / s->gsize + s->B;
         int rc;
         if ((rc = ensure(s->lfull, (size_t)ng * s->cf * 4)) || (rc = ensure(s->lpart, (size_t)ng * s->cp * 16)) ||
             (rc = ensure(s->lleaf, (size_t)ng * s->cl * 16)) || (rc = ensure(s->lcounts, (size_t)ng * 16)))
@@ -301,31 +406,28 @@ int enqueue_walk(gx_solver* s, int walk, long long n, double eps, double* ax, do
         lists.cl = s->cl;
         launch_walk32((const double4*)s->spos.p, (const TravNode*)s->trav.p, (const Mult32*)s->m32.p,
                       walk == GX_WALK_FP32_Q16 ? (const Mult16*)s->m16.p : nullptr, (const int32_t*)s->order.p, meta,
-                      (n + 63) / 64, (float)eps2, !s->monopole, lists, ax, ay, az, st, s->timing ? s->ev[7] : nullptr,
-                      s->splitWalk);
+                      batch, ng, s->eps0, !s->monopole, lists, a.ax, a.ay, a.az, st,
+                      s->timing ? s->ev[7] : nullptr, s->splitWalk, s->persistent ? s->persistentBlocks : 0);
     } else {
         hipLaunchKernelGGL(k_walk64, dim3(blocks(n, 256)), dim3(256), 0, st, (const double4*)s->spos.p,
                            (const TravNode*)s->trav.p, (const Mult64*)s->m64.p, (const int32_t*)s->order.p, meta,
-                           eps2, !s->monopole, ax, ay, az);
+                           batch, !s->monopole, a.ax, a.ay, a.az);
     }
-    hipLaunchKernelGGL(k_zero_holes, dim3(64), dim3(256), 0, st, (const int32_t*)s->holes.p, meta, ax, ay, az);
+    hipLaunchKernelGGL(k_zero_holes, dim3(64), dim3(256), 0, st, (const int32_t*)s->holes.p, meta, a.ax, a.ay, a.az);
     HIPCHK(hipGetLastError());
     return GX_OK;
 }
 
-int enqueue_step(gx_solver* s, const double* x, const double* y, const double* z, const double* m,
-                 const uint8_t* kind, const uint8_t* alive, long long n, double eps, int dmOff, double* ax,
-                 double* ay, double* az) {
-    if (n < 0) return set_err(GX_ERR_ARG, "negative n");
-    if (n >= (1LL << 31) - 1) return set_err(GX_ERR_ARG, "n = %lld exceeds the int32 index range", n);
-    int rc = enqueue_build(s, x, y, z, m, kind, alive, n, dmOff, theta_of(s));
+int enqueue_step(gx_solver* s, const StepArgs& a) {
+    if (a.n < 0) return set_err(GX_ERR_ARG, "negative n");
+    if (a.n >= (1LL << 31) - 1) return set_err(GX_ERR_ARG, "n = %lld exceeds the int32 index range", a.n);
+    int rc = enqueue_build(s, a);
     if (rc) return rc;
-    if ((rc = enqueue_walk(s, s->walk, n, eps, ax, ay, az))) return rc;
-    hipEventRecord(s->ev[kPhases], s->stream);
-    s->lastWalk = s->walk;
+    if ((rc = enqueue_walk(s, s->walk, a))) return rc;
+    if (s->timing) HIPCHK(hipEventRecord(s->ev[kPhases], s->stream));
     HIPCHK(hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, s->stream));
+    s->lastWalk = s->walk;
     s->pending = true;
-    s->pendingN = n;
     return GX_OK;
 }
 
@@ -335,17 +437,17 @@ int finish_step(gx_solver* s) {
     s->pending = false;
     HIPCHK(hipStreamSynchronize(s->stream));
     const Meta& h = *s->hmeta;
-    float total = 0;
-    hipEventElapsedTime(&total, s->ev[0], s->ev[kPhases]);
-    s->lastSeconds = total * 1e-3;
     if (s->timing) {
-        for (int i = 0; i < 6; i++) hipEventElapsedTime(&s->phase[i], s->ev[i], s->ev[i + 1]);
-        hipEventElapsedTime(&s->phase[6], s->ev[6], s->ev[kPhases]);
+        float total = 0;
+        (void)hipEventElapsedTime(&total, s->ev[0], s->ev[kPhases]);
+        s->lastSeconds = total * 1e-3;
+        for (int i = 0; i < 6; i++) (void)hipEventElapsedTime(&s->phase[i], s->ev[i], s->ev[i + 1]);
+        (void)hipEventElapsedTime(&s->phase[6], s->ev[6], s->ev[kPhases]);
         s->phase[7] = total;
         s->phase[8] = 0;
         if (s->lastWalk != GX_WALK_FP64 && s->splitWalk) {  // traversal | evaluation split
             float trav = 0;
-            hipEventElapsedTime(&trav, s->ev[6], s->ev[7]);
+            (void)hipEventElapsedTime(&trav, s->ev[6], s->ev[7]);
             s->phase[8] = s->phase[6] - trav;
             s->phase[6] = trav;
         }
@@ -373,36 +475,103 @@ int finish_step(gx_solver* s) {
     return GX_OK;
 }
 
-int run_dev(gx_solver* s, const double* x, const double* y, const double* z, const double* m, const uint8_t* kind,
-            const uint8_t* alive, long long n, double eps, int dmOff, double* ax, double* ay, double* az) {
-    for (int attempt = 0; attempt < 6; attempt++) {
-        int rc = enqueue_step(s, x, y, z, m, kind, alive, n, eps, dmOff, ax, ay, az);
+// Grows the capacity that overflowed; true if the step must be repeated.
+bool regrow(gx_solver* s) {
+    const Meta& h = *s->hmeta;
+    if (h.flags & kFlagNodeOverflow) {
+        s->capNodes *= 2;
+        return true;
+    }
+    if (h.flags & kFlagListOverflow) {
+        s->cf = std::max(s->cf, (int)(h.maxList[0] * 1.25) + 64);
+        s->cp = std::max(s->cp, (int)(h.maxList[1] * 1.25) + 64);
+        s->cl = std::max(s->cl, (int)(h.maxList[2] * 1.25) + 64);
+        return true;
+    }
+    return false;
+}
+
+int run_dev(gx_solver* s, const StepArgs& a) {
+    if (s->pending) (void)finish_step(s);
+    for (int attempt = 0; attempt < 8; attempt++) {
+        int rc = enqueue_step(s, a);
         if (rc) return rc;
         rc = finish_step(s);
-        if (rc == GX_ERR_CAPACITY) {
-            s->capNodes *= 2;
-            continue;
-        }
-        if (rc) return rc;
-        if (s->hmeta->flags & kFlagListOverflow) {
-            // some group's interaction list did not fit: grow and redo the walk
-            const Meta& h = *s->hmeta;
-            s->cf = std::max(s->cf, (int)(h.maxList[0] * 1.25) + 64);
-            s->cp = std::max(s->cp, (int)(h.maxList[1] * 1.25) + 64);
-            s->cl = std::max(s->cl, (int)(h.maxList[2] * 1.25) + 64);
-            continue;
-        }
+        if (rc && rc != GX_ERR_CAPACITY) return rc;
+        if (regrow(s)) continue;
         if (s->hmeta->flags & kFlagStackOverflow) {
             // the FP32 walk's LDS stack was too small for this tree: redo the
             // walk with the FP64 per-particle walk (same interaction set)
             s->fallbacks++;
-            if ((rc = enqueue_walk(s, GX_WALK_FP64, n, eps, ax, ay, az))) return rc;
+            if ((rc = enqueue_walk(s, GX_WALK_FP64, a))) return rc;
             HIPCHK(hipStreamSynchronize(s->stream));
             s->stats.fallbacks = s->fallbacks;
         }
         return GX_OK;
     }
-    return set_err(GX_ERR_CAPACITY, "node arena did not fit after regrowing");
+    return set_err(GX_ERR_CAPACITY, "capacities did not fit after regrowing");
+}
+
+StepArgs step_args(const double* x, const double* y, const double* z, const double* m, const uint8_t* kind,
+                   const uint8_t* alive, long long n, int dmOff, double* ax, double* ay, double* az) {
+    StepArgs a;
+    a.x = x; a.y = y; a.z = z; a.m = m;
+    a.kind = kind; a.alive = alive;
+    a.n = n;
+    a.dmOff = dmOff;
+    a.ax = ax; a.ay = ay; a.az = az;
+    return a;
+}
+
+int check_arrays(const StepArgs& a) {
+    if (a.n < 0) return set_err(GX_ERR_ARG, "negative n");
+    if (a.n > 0 && (!a.x || !a.y || !a.z || !a.m || !a.kind || !a.alive || !a.ax || !a.ay || !a.az))
+        return set_err(GX_ERR_ARG, "null array");
+    return GX_OK;
+}
+
+int set_single(gx_solver* s, long long n, double theta, double eps) {
+    const int64_t off[2] = {0, n};
+    return set_instances(s, 1, off, nullptr, theta, nullptr, eps);
+}
+
+int check_batch(const int64_t* offsets, int32_t count, const double* theta, const double* eps) {
+    if (!offsets || !theta || !eps) return set_err(GX_ERR_ARG, "null argument");
+    if (count < 1 || offsets[0] != 0) return set_err(GX_ERR_ARG, "bad instance offsets");
+    return GX_OK;
+}
+
+int kick(gx_solver* s, double* vx, double* vy, double* vz, const double* ax, const double* ay, const double* az,
+         long long n, double h) {
+    if (n <= 0) return GX_OK;
+    hipLaunchKernelGGL(k_kick, dim3((unsigned)std::min<long long>(blocks(n, 256), 4096)), dim3(256), 0, s->stream, n,
+                       vx, vy, vz, ax, ay, az, h);
+    HIPCHK(hipGetLastError());
+    return GX_OK;
+}
+
+int drift(gx_solver* s, double* x, double* y, double* z, const double* vx, const double* vy, const double* vz,
+          long long n, double dt) {
+    if (n <= 0) return GX_OK;
+    hipLaunchKernelGGL(k_drift, dim3((unsigned)std::min<long long>(blocks(n, 256), 4096)), dim3(256), 0, s->stream,
+                       n, x, y, z, vx, vy, vz, dt);
+    HIPCHK(hipGetLastError());
+    return GX_OK;
+}
+
+// KDK step (sim.go:262-281): forces first if they are not valid, then
+// kick, drift, new forces, kick, over every body.
+int kdk(gx_solver* s, const StepArgs& a, double* vx, double* vy, double* vz, double dt, int accValid) {
+    int rc;
+    double* x = (double*)a.x;
+    double* y = (double*)a.y;
+    double* z = (double*)a.z;
+    if (!accValid && (rc = run_dev(s, a))) return rc;
+    if ((rc = kick(s, vx, vy, vz, a.ax, a.ay, a.az, a.n, 0.5 * dt)) || (rc = drift(s, x, y, z, vx, vy, vz, a.n, dt)) ||
+        (rc = run_dev(s, a)) || (rc = kick(s, vx, vy, vz, a.ax, a.ay, a.az, a.n, 0.5 * dt)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return GX_OK;
 }
 
 }  // namespace
@@ -413,20 +582,21 @@ int gx_accelerations_dev(gx_solver* s, const double* x, const double* y, const d
                          const uint8_t* kind, const uint8_t* alive, int64_t n, double eps, int dark_matter_off,
                          double* ax, double* ay, double* az) {
     if (!s) return set_err(GX_ERR_ARG, "null solver");
-    if (n > 0 && (!x || !y || !z || !mass || !kind || !alive || !ax || !ay || !az))
-        return set_err(GX_ERR_ARG, "null array");
-    if (s->pending) finish_step(s);
-    return run_dev(s, x, y, z, mass, kind, alive, n, eps, dark_matter_off, ax, ay, az);
+    const StepArgs a = step_args(x, y, z, mass, kind, alive, n, dark_matter_off, ax, ay, az);
+    int rc;
+    if ((rc = check_arrays(a)) || (rc = set_single(s, n, s->theta, eps))) return rc;
+    return run_dev(s, a);
 }
 
 int gx_enqueue_accelerations_dev(gx_solver* s, const double* x, const double* y, const double* z,
                                  const double* mass, const uint8_t* kind, const uint8_t* alive, int64_t n,
                                  double eps, int dark_matter_off, double* ax, double* ay, double* az) {
     if (!s) return set_err(GX_ERR_ARG, "null solver");
-    if (n > 0 && (!x || !y || !z || !mass || !kind || !alive || !ax || !ay || !az))
-        return set_err(GX_ERR_ARG, "null array");
+    const StepArgs a = step_args(x, y, z, mass, kind, alive, n, dark_matter_off, ax, ay, az);
+    int rc;
+    if ((rc = check_arrays(a)) || (rc = set_single(s, n, s->theta, eps))) return rc;
     s->pending = false;  // a new step supersedes an unsynchronised one
-    return enqueue_step(s, x, y, z, mass, kind, alive, n, eps, dark_matter_off, ax, ay, az);
+    return enqueue_step(s, a);
 }
 
 int gx_sync(gx_solver* s) {
@@ -477,39 +647,99 @@ int gx_accelerations(gx_solver* s, const double* x, const double* y, const doubl
     return GX_OK;
 }
 
+int gx_batch_accelerations_dev(gx_solver* s, const double* x, const double* y, const double* z, const double* mass,
+                               const uint8_t* kind, const uint8_t* alive, const int64_t* offsets, int32_t count,
+                               const double* theta, const double* eps, int dark_matter_off, double* ax, double* ay,
+                               double* az) {
+    if (!s) return set_err(GX_ERR_ARG, "null solver");
+    int rc;
+    if ((rc = check_batch(offsets, count, theta, eps))) return rc;
+    const StepArgs a = step_args(x, y, z, mass, kind, alive, offsets[count], dark_matter_off, ax, ay, az);
+    if ((rc = check_arrays(a)) || (rc = set_instances(s, count, offsets, theta, 0, eps, 0))) return rc;
+    return run_dev(s, a);
+}
+
 int gx_kick_dev(gx_solver* s, double* vx, double* vy, double* vz, const double* ax, const double* ay,
                 const double* az, int64_t n, double h) {
     if (!s || n < 0) return set_err(GX_ERR_ARG, "bad argument");
-    if (n == 0) return GX_OK;
-    hipLaunchKernelGGL(k_kick, dim3((unsigned)std::min<long long>(blocks(n, 256), 4096)), dim3(256), 0, s->stream,
-                       (long long)n, vx, vy, vz, ax, ay, az, h);
-    HIPCHK(hipGetLastError());
-    return GX_OK;
+    return kick(s, vx, vy, vz, ax, ay, az, n, h);
 }
 
 int gx_drift_dev(gx_solver* s, double* x, double* y, double* z, const double* vx, const double* vy,
                  const double* vz, int64_t n, double dt) {
     if (!s || n < 0) return set_err(GX_ERR_ARG, "bad argument");
-    if (n == 0) return GX_OK;
-    hipLaunchKernelGGL(k_drift, dim3((unsigned)std::min<long long>(blocks(n, 256), 4096)), dim3(256), 0, s->stream,
-                       (long long)n, x, y, z, vx, vy, vz, dt);
-    HIPCHK(hipGetLastError());
-    return GX_OK;
+    return drift(s, x, y, z, vx, vy, vz, n, dt);
 }
 
 int gx_step_dev(gx_solver* s, double* x, double* y, double* z, double* vx, double* vy, double* vz,
                 const double* mass, const uint8_t* kind, const uint8_t* alive, int64_t n, double eps,
                 int dark_matter_off, double dt, int acc_valid, double* ax, double* ay, double* az) {
     if (!s) return set_err(GX_ERR_ARG, "null solver");
+    const StepArgs a = step_args(x, y, z, mass, kind, alive, n, dark_matter_off, ax, ay, az);
     int rc;
-    if (!acc_valid &&
-        (rc = gx_accelerations_dev(s, x, y, z, mass, kind, alive, n, eps, dark_matter_off, ax, ay, az)))
-        return rc;
-    if ((rc = gx_kick_dev(s, vx, vy, vz, ax, ay, az, n, 0.5 * dt))) return rc;
-    if ((rc = gx_drift_dev(s, x, y, z, vx, vy, vz, n, dt))) return rc;
-    if ((rc = gx_accelerations_dev(s, x, y, z, mass, kind, alive, n, eps, dark_matter_off, ax, ay, az))) return rc;
-    if ((rc = gx_kick_dev(s, vx, vy, vz, ax, ay, az, n, 0.5 * dt))) return rc;
+    if ((rc = check_arrays(a)) || (rc = set_single(s, n, s->theta, eps))) return rc;
+    return kdk(s, a, vx, vy, vz, dt, acc_valid);
+}
+
+int gx_batch_step_dev(gx_solver* s, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                      const double* mass, const uint8_t* kind, const uint8_t* alive, const int64_t* offsets,
+                      int32_t count, const double* theta, const double* eps, int dark_matter_off, double dt,
+                      int acc_valid, double* ax, double* ay, double* az) {
+    if (!s) return set_err(GX_ERR_ARG, "null solver");
+    int rc;
+    if ((rc = check_batch(offsets, count, theta, eps))) return rc;
+    const StepArgs a = step_args(x, y, z, mass, kind, alive, offsets[count], dark_matter_off, ax, ay, az);
+    if ((rc = check_arrays(a)) || (rc = set_instances(s, count, offsets, theta, 0, eps, 0))) return rc;
+    return kdk(s, a, vx, vy, vz, dt, acc_valid);
+}
+
+int gx_graph_capture_step(gx_solver* s, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                          const double* mass, const uint8_t* kind, const uint8_t* alive, const int64_t* offsets,
+                          int32_t count, const double* theta, const double* eps, int dark_matter_off, double dt,
+                          double* ax, double* ay, double* az) {
+    if (!s) return set_err(GX_ERR_ARG, "null solver");
+    int rc;
+    if ((rc = check_batch(offsets, count, theta, eps))) return rc;
+    if (s->stream == nullptr) return set_err(GX_ERR_ARG, "capture needs a non-null stream");
+    const long long n = offsets[count];
+    const StepArgs a = step_args(x, y, z, mass, kind, alive, n, dark_matter_off, ax, ay, az);
+    if ((rc = check_arrays(a)) || (rc = set_instances(s, count, offsets, theta, 0, eps, 0))) return rc;
+    // an eager force evaluation sizes every arena for the captured step
+    if ((rc = run_dev(s, a))) return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
+    if (s->graphExec) HIPCHK(hipGraphExecDestroy(s->graphExec));
+    if (s->graph) HIPCHK(hipGraphDestroy(s->graph));
+    s->graphExec = nullptr;
+    s->graph = nullptr;
+    const int timing = s->timing;
+    s->timing = 0;  // no event records inside the graph
+    HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    rc = kick(s, vx, vy, vz, ax, ay, az, n, 0.5 * dt);
+    if (!rc) rc = drift(s, x, y, z, vx, vy, vz, n, dt);
+    if (!rc) rc = enqueue_build(s, a);
+    if (!rc) rc = enqueue_walk(s, s->walk, a);
+    if (!rc) rc = kick(s, vx, vy, vz, ax, ay, az, n, 0.5 * dt);
+    hipError_t mc = hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, s->stream);
+    hipGraph_t g = nullptr;
+    hipError_t ec = hipStreamEndCapture(s->stream, &g);
+    s->timing = timing;
+    s->pending = false;
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (mc != hipSuccess || ec != hipSuccess)
+        return set_err(GX_ERR_HIP, "graph capture failed: %s", hipGetErrorString(mc != hipSuccess ? mc : ec));
+    s->graph = g;
+    HIPCHK(hipGraphInstantiate(&s->graphExec, g, nullptr, nullptr, 0));
+    s->lastWalk = s->walk;
+    return GX_OK;
+}
+
+int gx_graph_launch(gx_solver* s, int32_t steps) {
+    if (!s || !s->graphExec || steps < 0) return set_err(GX_ERR_ARG, "no captured step");
+    for (int i = 0; i < steps; i++) HIPCHK(hipGraphLaunch(s->graphExec, s->stream));
+    if (steps > 0) s->pending = true;
     return GX_OK;
 }
 
@@ -518,16 +748,17 @@ int gx_potential_energy_dev(gx_solver* s, const double* x, const double* y, cons
                             double* out) {
     if (!s || !out) return set_err(GX_ERR_ARG, "null argument");
     if (n < 0) return set_err(GX_ERR_ARG, "negative n");
-    if (s->pending) finish_step(s);
+    if (s->pending) (void)finish_step(s);
     *out = 0;
     if (n == 0) return GX_OK;
-    const double theta = std::min(theta_of(s), 0.9);  // diagTheta, bh.go:20-23
+    const StepArgs a = step_args(x, y, z, mass, kind, alive, n, dark_matter_off, nullptr, nullptr, nullptr);
+    int rc;
+    if ((rc = set_single(s, n, std::min(theta_of(s->theta), 0.9), eps))) return rc;  // diagTheta, bh.go:20-23
     for (int attempt = 0; attempt < 6; attempt++) {
-        int rc = enqueue_build(s, x, y, z, mass, kind, alive, n, dark_matter_off, theta);
-        if (rc) return rc;
+        if ((rc = enqueue_build(s, a))) return rc;
         if ((rc = ensure(s->partial, (size_t)n * 8))) return rc;
         hipLaunchKernelGGL(k_potential64, dim3(blocks(n, 256)), dim3(256), 0, s->stream, (const double4*)s->spos.p,
-                           (const TravNode*)s->trav.p, (const Mult64*)s->m64.p, (Meta*)s->meta.p, eps * eps,
+                           (const TravNode*)s->trav.p, (const Mult64*)s->m64.p, (Meta*)s->meta.p, make_batch(s),
                            !s->monopole, (double*)s->partial.p);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, s->stream));
@@ -545,7 +776,7 @@ int gx_potential_energy_dev(gx_solver* s, const double* x, const double* y, cons
         hipError_t e = hipMemcpy(h, s->partial.p, (size_t)ns * 8, hipMemcpyDeviceToHost);
         double sum = 0;
         for (long long k = 0; k < ns; k++) sum += h[k];
-        hipHostFree(h);
+        (void)hipHostFree(h);
         if (e != hipSuccess) return set_err(GX_ERR_HIP, "copy of potential partials: %s", hipGetErrorString(e));
         *out = -sum / 2;
         return GX_OK;
@@ -559,7 +790,7 @@ int gx_direct_sample_dev(gx_solver* s, const double* x, const double* y, const d
     if (!s || !out || (nt > 0 && !targets)) return set_err(GX_ERR_ARG, "null argument");
     if (n < 0 || nt < 0 || nt > (1LL << 30)) return set_err(GX_ERR_ARG, "bad sizes");
     if (nt == 0) return GX_OK;
-    if (s->pending) finish_step(s);
+    if (s->pending) (void)finish_step(s);
     const int tblocks = (int)((nt + 255) / 256);
     int splits = std::max(1, 2048 / tblocks);
     splits = (int)std::min<long long>(splits, std::max(1LL, (long long)(n / 1024)));
@@ -586,7 +817,7 @@ int gx_get_stats(const gx_solver* s, gx_stats* out) {
 
 int64_t gx_export(gx_solver* s, int what, void* host, int64_t count) {
     if (!s || !host || count < 0) return set_err(GX_ERR_ARG, "bad argument"), -1;
-    if (s->pending) finish_step(s);
+    if (s->pending) (void)finish_step(s);
     if (hipStreamSynchronize(s->stream) != hipSuccess) return -1;
     const Meta& h = *s->hmeta;
     const void* src = nullptr;

@@ -28,12 +28,47 @@ namespace gx {
 // ---------------------------------------------------------------------------
 // collect
 // ---------------------------------------------------------------------------
+// Zeroes the step's counters and the per-instance accumulators.
+__global__ void k_step_init(Meta* meta, Batch batch) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) {
+        for (int l = 0; l <= kMaxLevels; l++) {
+            meta->levelStart[l] = 0;
+            meta->levelCount[l] = 0;
+        }
+        meta->nstars = meta->nholes = meta->nnodes = 0;
+        meta->pp = meta->pc = 0;
+        meta->flags = 0;
+        meta->maxStack = 0;
+        meta->maxList[0] = meta->maxList[1] = meta->maxList[2] = 0;
+        meta->listEntries = meta->rounds = meta->mixed = 0;
+        meta->groupNext = 0;
+    }
+    if (b < batch.B) {
+        for (int c = 0; c < 3; c++) {
+            batch.boxEnc[6 * b + c] = enc_d(INFINITY);
+            batch.boxEnc[6 * b + 3 + c] = enc_d(-INFINITY);
+        }
+        batch.starCount[b] = 0;
+    }
+}
+
+__device__ __forceinline__ void inst_accumulate(const Batch& batch, int b, int cnt, const double mn[3],
+                                                const double mx[3]) {
+    if (cnt == 0) return;
+    atomicAdd(&batch.starCount[b], (unsigned long long)cnt);
+    for (int c = 0; c < 3; c++) {
+        atomicMin(&batch.boxEnc[6 * b + c], enc_d(mn[c]));
+        atomicMax(&batch.boxEnc[6 * b + 3 + c], enc_d(mx[c]));
+    }
+}
+
 __global__ __launch_bounds__(256) void k_collect_count(const double* __restrict__ x, const double* __restrict__ y,
                                                        const double* __restrict__ z,
                                                        const uint8_t* __restrict__ kind,
                                                        const uint8_t* __restrict__ alive, long long n,
                                                        int* __restrict__ blockStar, int* __restrict__ blockHole,
-                                                       double* __restrict__ blockBox) {
+                                                       Batch batch) {
     __shared__ double sbox[6][4];
     __shared__ int scnt[2][4];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -52,6 +87,21 @@ __global__ __launch_bounds__(256) void k_collect_count(const double* __restrict_
             mn[0] = fmin(mn[0], px); mx[0] = fmax(mx[0], px);
             mn[1] = fmin(mn[1], py); mx[1] = fmax(mx[1], py);
             mn[2] = fmin(mn[2], pz); mx[2] = fmax(mx[2], pz);
+        }
+    }
+    // instances of the tile's first and last body: one instance -> block-wide
+    // reduction and one set of atomics; several -> atomics per star
+    const long long last = min(n, base + kCollectTile) - 1;
+    const int b0 = last >= base ? upper_index(batch.bodyOff, batch.B, base) : 0;
+    const int b1 = last >= base ? upper_index(batch.bodyOff, batch.B, last) : 0;
+    if (b0 != b1) {
+        for (int j = 0; j < kCollectTile / 256; j++) {
+            long long i = base + j * 256 + tid;
+            if (i >= n) break;
+            if (alive[i] && kind[i] != kHole) {
+                const double p[3] = {x[i], y[i], z[i]};
+                inst_accumulate(batch, upper_index(batch.bodyOff, batch.B, i), 1, p, p);
+            }
         }
     }
     cnt = wave_sum_i(cnt);
@@ -82,64 +132,91 @@ __global__ __launch_bounds__(256) void k_collect_count(const double* __restrict_
         }
         blockStar[blockIdx.x] = c0;
         blockHole[blockIdx.x] = c1;
-        for (int c = 0; c < 6; c++) blockBox[blockIdx.x * 6 + c] = b[c];
+        if (b0 == b1) inst_accumulate(batch, b0, c0, b, b + 3);
     }
 }
 
-// Single workgroup: exclusive scans of the per-block counts, box reduction.
-__global__ __launch_bounds__(1024) void k_collect_scan(int nb, int* __restrict__ blockStar,
-                                                       int* __restrict__ blockHole,
-                                                       const double* __restrict__ blockBox, Meta* meta) {
-    __shared__ long long ssum[2][1024];
-    __shared__ double sbox[6][16];
+__device__ __forceinline__ double box_side(const Batch& batch, int b) {
+    const double mnx = dec_d(batch.boxEnc[6 * b]), mny = dec_d(batch.boxEnc[6 * b + 1]);
+    const double mnz = dec_d(batch.boxEnc[6 * b + 2]), mxx = dec_d(batch.boxEnc[6 * b + 3]);
+    const double mxy = dec_d(batch.boxEnc[6 * b + 4]), mxz = dec_d(batch.boxEnc[6 * b + 5]);
+    double side = fmax(mxx - mnx, fmax(mxy - mny, mxz - mnz));
+    return fmax(side * (1 + 1e-9), 1e-12);  // build.go:19-20
+}
+
+// Exclusive scan of per-thread values v (1024 threads) in LDS; returns the
+// prefix before this thread and the total in *total.
+__device__ long long block_scan_1024(long long v, long long* s, long long* total) {
     const int tid = threadIdx.x;
-    const int per = (nb + 1023) / 1024;
-    int lo = tid * per, hi = min(nb, lo + per);
-    long long s0 = 0, s1 = 0;
-    double b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    for (int i = lo; i < hi; i++) {
-        s0 += blockStar[i];
-        s1 += blockHole[i];
-        for (int c = 0; c < 3; c++) {
-            b[c] = fmin(b[c], blockBox[i * 6 + c]);
-            b[3 + c] = fmax(b[3 + c], blockBox[i * 6 + 3 + c]);
+    s[tid] = v;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        long long a = tid >= d ? s[tid - d] : 0;
+        __syncthreads();
+        s[tid] += a;
+        __syncthreads();
+    }
+    const long long incl = s[tid];
+    *total = s[1023];
+    __syncthreads();
+    return incl - v;
+}
+
+// Single workgroup: exclusive scans of the per-block counts (the stable
+// compaction of collect) and of the per-instance star and group counts;
+// instance roots and root cells.
+__global__ __launch_bounds__(1024) void k_collect_scan(int nb, int* __restrict__ blockStar,
+                                                       int* __restrict__ blockHole, Meta* meta, Batch batch) {
+    __shared__ long long ss[1024];
+    const int tid = threadIdx.x;
+    long long total;
+    {
+        const int per = (nb + 1023) / 1024;
+        const int lo = tid * per, hi = min(nb, lo + per);
+        long long s0 = 0, s1 = 0;
+        for (int i = lo; i < hi; i++) {
+            s0 += blockStar[i];
+            s1 += blockHole[i];
+        }
+        long long run0 = block_scan_1024(s0, ss, &total);
+        if (tid == 0) meta->nstars = total;
+        long long run1 = block_scan_1024(s1, ss, &total);
+        if (tid == 0) meta->nholes = total;
+        for (int i = lo; i < hi; i++) {
+            const int c0 = blockStar[i], c1 = blockHole[i];
+            blockStar[i] = (int)run0;
+            blockHole[i] = (int)run1;
+            run0 += c0;
+            run1 += c1;
         }
     }
-    ssum[0][tid] = s0;
-    ssum[1][tid] = s1;
-    for (int c = 0; c < 3; c++) {
-        b[c] = wave_min_d(b[c]);
-        b[3 + c] = wave_max_d(b[3 + c]);
-    }
-    if ((tid & 63) == 0)
-        for (int c = 0; c < 6; c++) sbox[c][tid >> 6] = b[c];
-    __syncthreads();
-    // Hillis-Steele inclusive scan over the 1024 thread sums.
-    for (int d = 1; d < 1024; d <<= 1) {
-        long long a0 = tid >= d ? ssum[0][tid - d] : 0, a1 = tid >= d ? ssum[1][tid - d] : 0;
-        __syncthreads();
-        ssum[0][tid] += a0;
-        ssum[1][tid] += a1;
-        __syncthreads();
-    }
-    long long run0 = ssum[0][tid] - s0, run1 = ssum[1][tid] - s1;
-    for (int i = lo; i < hi; i++) {
-        int c0 = blockStar[i], c1 = blockHole[i];
-        blockStar[i] = (int)run0;
-        blockHole[i] = (int)run1;
-        run0 += c0;
-        run1 += c1;
-    }
-    if (tid == 0) {
-        double bb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-        for (int ww = 0; ww < 16; ww++)
-            for (int c = 0; c < 3; c++) {
-                bb[c] = fmin(bb[c], sbox[c][ww]);
-                bb[3 + c] = fmax(bb[3 + c], sbox[3 + c][ww]);
-            }
-        for (int c = 0; c < 6; c++) meta->box[c] = bb[c];
-        meta->nstars = ssum[0][1023];
-        meta->nholes = ssum[1][1023];
+    {
+        const int B = batch.B;
+        const int per = (B + 1023) / 1024;
+        const int lo = tid * per, hi = min(B, lo + per);
+        long long cs = 0, cg = 0, cr = 0;
+        for (int b = lo; b < hi; b++) {
+            const long long c = (long long)batch.starCount[b];
+            cs += c;
+            cg += (c + batch.gsize - 1) / batch.gsize;
+            cr += c > 0;
+        }
+        long long rs = block_scan_1024(cs, ss, &total);
+        if (tid == 0) batch.starOff[B] = total;
+        long long rg = block_scan_1024(cg, ss, &total);
+        if (tid == 0) batch.groupOff[B] = total;
+        long long rr = block_scan_1024(cr, ss, &total);
+        if (tid == 0) meta->levelCount[0] = total;
+        for (int b = lo; b < hi; b++) {
+            const long long c = (long long)batch.starCount[b];
+            batch.starOff[b] = rs;
+            batch.groupOff[b] = rg;
+            batch.root[b] = c > 0 ? (int)rr : -1;
+            batch.half[b] = c > 0 ? box_side(batch, b) / 2 : 0.0;
+            rs += c;
+            rg += (c + batch.gsize - 1) / batch.gsize;
+            rr += c > 0;
+        }
     }
 }
 
@@ -185,11 +262,6 @@ __global__ __launch_bounds__(256) void k_collect_scatter(const uint8_t* __restri
 // ---------------------------------------------------------------------------
 // keys: build.go:19-31
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double box_side(const Meta* meta) {
-    double side = fmax(meta->box[3] - meta->box[0], fmax(meta->box[4] - meta->box[1], meta->box[5] - meta->box[2]));
-    return fmax(side * (1 + 1e-9), 1e-12);
-}
-
 __device__ __forceinline__ uint64_t cell_of(double v, double lo, double scale) {
     double t = fmax(0.0, (v - lo) * scale);
     return (uint64_t)fmin(2097151.0, t);
@@ -197,23 +269,34 @@ __device__ __forceinline__ uint64_t cell_of(double v, double lo, double scale) {
 
 __global__ __launch_bounds__(256) void k_keys(const double* __restrict__ x, const double* __restrict__ y,
                                               const double* __restrict__ z, const int32_t* __restrict__ order,
-                                              const Meta* __restrict__ meta, uint64_t* __restrict__ keys) {
+                                              const Meta* __restrict__ meta, Batch batch,
+                                              uint64_t* __restrict__ keys) {
     long long k = (long long)blockIdx.x * 256 + threadIdx.x;
     if (k >= meta->nstars) return;
-    double side = box_side(meta);
-    double scale = (double)(1 << kMortonBits) / side;
     int32_t i = order[k];
-    uint64_t cx = cell_of(x[i], meta->box[0], scale);
-    uint64_t cy = cell_of(y[i], meta->box[1], scale);
-    uint64_t cz = cell_of(z[i], meta->box[2], scale);
+    const int b = upper_index(batch.bodyOff, batch.B, i);
+    double side = box_side(batch, b);
+    double scale = (double)(1 << kMortonBits) / side;
+    uint64_t cx = cell_of(x[i], dec_d(batch.boxEnc[6 * b]), scale);
+    uint64_t cy = cell_of(y[i], dec_d(batch.boxEnc[6 * b + 1]), scale);
+    uint64_t cz = cell_of(z[i], dec_d(batch.boxEnc[6 * b + 2]), scale);
     keys[k] = spread21(cx) << 2 | spread21(cy) << 1 | spread21(cz);
 }
 
 // ---------------------------------------------------------------------------
 // Stable LSD radix sort of (key, order) pairs, 8-bit digits.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, const Meta* __restrict__ meta,
-                                                   int shift, uint32_t* __restrict__ hist, int ntiles) {
+// Digit of a sort element: 8 bits of the Morton key, or (instance passes, the
+// last ones) 8 bits of the instance of the element's body.
+__device__ __forceinline__ uint32_t sort_digit(uint64_t key, int32_t val, int shift, bool instPass,
+                                               const Batch& batch) {
+    if (instPass) return ((uint32_t)upper_index(batch.bodyOff, batch.B, val) >> shift) & 255u;
+    return (uint32_t)(key >> shift) & 255u;
+}
+
+__global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, const int32_t* __restrict__ vals,
+                                                   const Meta* __restrict__ meta, int shift, int instPass,
+                                                   Batch batch, uint32_t* __restrict__ hist, int ntiles) {
     __shared__ uint32_t h[256];
     const int tid = threadIdx.x;
     h[tid] = 0;
@@ -222,7 +305,7 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
     long long base = (long long)blockIdx.x * kSortTile;
     for (int r = 0; r < kSortTile / 256; r++) {
         long long i = base + r * 256 + tid;
-        if (i < nst) atomicAdd(&h[(keys[i] >> shift) & 255], 1u);
+        if (i < nst) atomicAdd(&h[sort_digit(keys[i], vals[i], shift, instPass, batch)], 1u);
     }
     __syncthreads();
     hist[(size_t)tid * ntiles + blockIdx.x] = h[tid];
@@ -258,7 +341,8 @@ __global__ __launch_bounds__(256) void k_sort_scan(uint32_t* __restrict__ hist, 
 __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict__ kin,
                                                       const int32_t* __restrict__ vin, uint64_t* __restrict__ kout,
                                                       int32_t* __restrict__ vout, const Meta* __restrict__ meta,
-                                                      int shift, const uint32_t* __restrict__ hist,
+                                                      int shift, int instPass, Batch batch,
+                                                      const uint32_t* __restrict__ hist,
                                                       const uint32_t* __restrict__ digitTotal, int ntiles) {
     __shared__ uint32_t tileBase[256];
     __shared__ uint32_t running[256];
@@ -287,7 +371,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict
         bool valid = idx < nst;
         uint64_t key = valid ? kin[idx] : 0;
         int32_t val = valid ? vin[idx] : 0;
-        uint32_t dig = (uint32_t)(key >> shift) & 255u;
+        uint32_t dig = valid ? sort_digit(key, val, shift, instPass, batch) : 0u;
         uint64_t match = __ballot(valid);
         for (int b = 0; b < 8; b++) {
             bool bit = (dig >> b) & 1u;
@@ -334,41 +418,28 @@ __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ x, co
 // ---------------------------------------------------------------------------
 // topology
 // ---------------------------------------------------------------------------
-__global__ void k_tree_init(Meta* meta, TravNode* __restrict__ trav) {
-    if (threadIdx.x != 0) return;
-    for (int l = 0; l <= kMaxLevels; l++) {
-        meta->levelStart[l] = 0;
-        meta->levelCount[l] = 0;
-    }
-    meta->pp = 0;
-    meta->pc = 0;
-    meta->flags = 0;
-    meta->maxStack = 0;
-    meta->maxList[0] = meta->maxList[1] = meta->maxList[2] = 0;
-    meta->listEntries = 0;
-    meta->rounds = 0;
-    meta->mixed = 0;
-    meta->nnodes = 0;
-    long long nst = meta->nstars;
-    if (nst <= 0) return;
-    double side = box_side(meta);
-    double half = side / 2;
-    meta->rootHalf = half;
+// Root cell of every instance with stars: anchored at the instance's bounding
+// box min corner, centre = min + half (build.go:43-49).
+__global__ void k_tree_init(Meta* meta, Batch batch, TravNode* __restrict__ trav) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch.B) return;
+    const int r = batch.root[b];
+    if (r < 0) return;
+    const double half = batch.half[b];
     TravNode t;
-    t.x = meta->box[0] + half;
-    t.y = meta->box[1] + half;
-    t.z = meta->box[2] + half;
+    t.x = dec_d(batch.boxEnc[6 * b]) + half;
+    t.y = dec_d(batch.boxEnc[6 * b + 1]) + half;
+    t.z = dec_d(batch.boxEnc[6 * b + 2]) + half;
     t.crit2 = half;
     t.first = -1;
     t.next = -1;
-    t.lo = 0;
-    t.hi = (int32_t)nst;
+    t.lo = (int32_t)batch.starOff[b];
+    t.hi = (int32_t)batch.starOff[b + 1];
     t.nchild = 0;
     t.level = 0;
     t.parent = -1;
-    t.pad = 0;
-    trav[0] = t;
-    meta->levelCount[0] = 1;
+    t.inst = b;
+    trav[r] = t;
 }
 
 // Octant boundaries of the sorted range [lo, hi) at digit shift: b[o] is the
@@ -452,7 +523,7 @@ __device__ void topo_level(int L, const uint64_t* __restrict__ keys, TravNode* _
             continue;
         }
         const double cx = trav[ni].x, cy = trav[ni].y, cz = trav[ni].z, half = trav[ni].crit2;
-        const int pnext = trav[ni].next;
+        const int pnext = trav[ni].next, pinst = trav[ni].inst;
         if (nonempty) {
             int rank = __popcll(ne & ((1ull << lane) - 1));
             long long c = firstc + rank;
@@ -470,7 +541,7 @@ __device__ void topo_level(int L, const uint64_t* __restrict__ keys, TravNode* _
             t.nchild = 0;
             t.level = L + 1;
             t.parent = (int32_t)ni;
-            t.pad = 0;
+            t.inst = pinst;
             trav[c] = t;
         }
         if (lane == 0) {
@@ -503,8 +574,9 @@ __device__ __forceinline__ void quad_add(double q[6], double m, double dx, doubl
 // record holds the cell centre in x,y,z and the half side in crit2.
 __device__ void node_moments(long long ni, const double4* __restrict__ spos, TravNode* __restrict__ trav,
                              Mult64* __restrict__ m64, Mult32* __restrict__ m32, Mult16* __restrict__ m16,
-                             double theta) {
+                             const double* __restrict__ thetas) {
     TravNode nd = trav[ni];
+    const double theta = thetas[nd.inst];
     const double cx = nd.x, cy = nd.y, cz = nd.z, half = nd.crit2;
     double x = 0, y = 0, z = 0, m = 0;
     if (nd.first < 0) {
@@ -581,13 +653,13 @@ __device__ void node_moments(long long ni, const double4* __restrict__ spos, Tra
 __global__ __launch_bounds__(256) void k_moments_level(int L, const double4* __restrict__ spos,
                                                        TravNode* __restrict__ trav, Mult64* __restrict__ m64,
                                                        Mult32* __restrict__ m32, Mult16* __restrict__ m16,
-                                                       const Meta* __restrict__ meta, double theta) {
+                                                       const Meta* __restrict__ meta, Batch batch) {
     const long long start = meta->levelStart[L];
     const long long cnt = meta->levelCount[L];
     if (meta->flags & kFlagNodeOverflow) return;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
          t += (long long)gridDim.x * blockDim.x)
-        node_moments(start + t, spos, trav, m64, m32, m16, theta);
+        node_moments(start + t, spos, trav, m64, m32, m16, batch.theta);
 }
 
 __global__ void k_tree_finish(Meta* meta) {

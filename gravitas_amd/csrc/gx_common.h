@@ -24,7 +24,7 @@ struct alignas(64) TravNode {
     int32_t nchild;  // children first .. first+nchild-1 (octant order)
     int32_t level;
     int32_t parent;  // -1 for a root
-    int32_t pad;
+    int32_t inst;    // instance (system) the node belongs to
 };
 
 // FP64 multipole (m and the traceless quadrupole about the centre of mass).
@@ -45,10 +45,48 @@ struct alignas(16) Mult16 {
     uint16_t q[6];  // qxx, qyy, qzz, qxy, qxz, qyz as IEEE half
 };
 
+// Instances of one step: B independent systems packed one after the other in
+// the body arrays (a single system is B = 1).  Each gets its own bounding box,
+// root cell, tree, opening angle and softening; walk groups never mix them.
+struct Batch {
+    int B;
+    int gsize;                      // stars per walk group (32 or 64)
+    const long long* bodyOff;       // B+1 body offsets in the arena
+    const double* theta;            // B opening angles (DefaultTheta already applied)
+    const double* eps2;             // B squared softenings
+    unsigned long long* boxEnc;     // 6B: encoded min x,y,z, max x,y,z of the instance's stars
+    unsigned long long* starCount;  // B
+    long long* starOff;             // B+1 offsets of the instances in sorted order
+    long long* groupOff;            // B+1 offsets of the instances' walk groups
+    int* root;                      // B root node, -1 for an instance without stars
+    double* half;                   // B half side of the root cell
+};
+
+// Order-preserving map of doubles to unsigned integers, so that atomic
+// integer min/max give exact floating-point min/max (-0 orders below +0).
+__device__ __forceinline__ unsigned long long enc_d(double x) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dec_d(unsigned long long u) {
+    return __longlong_as_double((long long)((u >> 63) ? (u & 0x7fffffffffffffffull) : ~u));
+}
+
+// Largest i with off[i] <= v among off[0..count) (off ascending, off[0] <= v).
+__device__ __forceinline__ int upper_index(const long long* __restrict__ off, int count, long long v) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= v)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 // Device-side metadata of one build.
 struct Meta {
-    double box[6];          // minX, minY, minZ, maxX, maxY, maxZ of the tree stars
-    double rootHalf;        // half side of the root cell
     long long nstars;
     long long nholes;
     long long nnodes;
@@ -56,6 +94,7 @@ struct Meta {
     long long levelCount[kMaxLevels + 1];
     unsigned long long pp;  // particle-particle interactions (incl. self pairs)
     unsigned long long pc;  // particle-cell interactions
+    unsigned long long groupNext;    // work queue of the persistent walk
     unsigned long long listEntries;  // cell list entries over all groups (FP32 walk)
     unsigned long long rounds;       // FP32 walk traversal rounds over all groups
     unsigned long long mixed;        // nodes decided per star (box test inconclusive)

@@ -8,24 +8,26 @@ constexpr int kCollectTile = 4096;  // bodies per collect workgroup (256 threads
 constexpr int kSortTile = 4096;     // keys per sort tile (256 threads x 16)
 
 // gx_build.hip
+__global__ void k_step_init(Meta* meta, Batch batch);
 __global__ void k_collect_count(const double* x, const double* y, const double* z, const uint8_t* kind,
-                                const uint8_t* alive, long long n, int* blockStar, int* blockHole, double* blockBox);
-__global__ void k_collect_scan(int nb, int* blockStar, int* blockHole, const double* blockBox, Meta* meta);
+                                const uint8_t* alive, long long n, int* blockStar, int* blockHole, Batch batch);
+__global__ void k_collect_scan(int nb, int* blockStar, int* blockHole, Meta* meta, Batch batch);
 __global__ void k_collect_scatter(const uint8_t* kind, const uint8_t* alive, long long n, const int* starOff,
                                   const int* holeOff, int32_t* order, int32_t* holes);
 __global__ void k_keys(const double* x, const double* y, const double* z, const int32_t* order, const Meta* meta,
-                       uint64_t* keys);
-__global__ void k_sort_hist(const uint64_t* keys, const Meta* meta, int shift, uint32_t* hist, int ntiles);
+                       Batch batch, uint64_t* keys);
+__global__ void k_sort_hist(const uint64_t* keys, const int32_t* vals, const Meta* meta, int shift, int instPass,
+                            Batch batch, uint32_t* hist, int ntiles);
 __global__ void k_sort_scan(uint32_t* hist, int ntiles, uint32_t* digitTotal);
 __global__ void k_sort_scatter(const uint64_t* kin, const int32_t* vin, uint64_t* kout, int32_t* vout,
-                               const Meta* meta, int shift, const uint32_t* hist, const uint32_t* digitTotal,
-                               int ntiles);
+                               const Meta* meta, int shift, int instPass, Batch batch, const uint32_t* hist,
+                               const uint32_t* digitTotal, int ntiles);
 __global__ void k_gather(const double* x, const double* y, const double* z, const double* m, const uint8_t* kind,
                          const int32_t* order, const Meta* meta, int dmOff, double4* spos);
-__global__ void k_tree_init(Meta* meta, TravNode* trav);
+__global__ void k_tree_init(Meta* meta, Batch batch, TravNode* trav);
 __global__ void k_topo_level(int L, const uint64_t* keys, TravNode* trav, Meta* meta, long long cap);
 __global__ void k_moments_level(int L, const double4* spos, TravNode* trav, Mult64* m64, Mult32* m32,
-                                Mult16* m16, const Meta* meta, double theta);
+                                Mult16* m16, const Meta* meta, Batch batch);
 __global__ void k_tree_finish(Meta* meta);
 __global__ void k_zero_holes(const int32_t* holes, const Meta* meta, double* ax, double* ay, double* az);
 __global__ void k_kick(long long n, double* vx, double* vy, double* vz, const double* ax, const double* ay,
@@ -35,8 +37,8 @@ __global__ void k_drift(long long n, double* px, double* py, double* pz, const d
 
 // gx_walk64.hip
 __global__ void k_walk64(const double4* spos, const TravNode* trav, const Mult64* m64, const int32_t* order,
-                         Meta* meta, double eps2, int quad, double* ax, double* ay, double* az);
-__global__ void k_potential64(const double4* spos, const TravNode* trav, const Mult64* m64, Meta* meta, double eps2,
+                         Meta* meta, Batch batch, int quad, double* ax, double* ay, double* az);
+__global__ void k_potential64(const double4* spos, const TravNode* trav, const Mult64* m64, Meta* meta, Batch batch,
                               int quad, double* partial);
 __global__ void k_direct_sample(const double* x, const double* y, const double* z, const double* m,
                                 const uint8_t* kind, long long n, const long long* targets, int nt, double eps2,
@@ -56,9 +58,14 @@ struct Lists {
     int cf, cp, cl;  // capacities per group
 };
 
-// m16 non-null selects the FP16-stored quadrupoles.
+// m16 non-null selects the FP16-stored quadrupoles; eps0 marks a batch with an
+// unsoftened instance; persistentBlocks > 0 runs the fused walk as a work queue
+// over that many resident workgroups.
 void launch_walk32(const double4* spos, const TravNode* trav, const Mult32* m32, const Mult16* m16,
-                   const int32_t* order, Meta* meta, long long ngroupsUpper, float eps2, int quad, const Lists& lists,
-                   double* ax, double* ay, double* az, hipStream_t stream, hipEvent_t mid, bool split);
+                   const int32_t* order, Meta* meta, const Batch& batch, long long ngroupsUpper, bool eps0,
+                   int quad, const Lists& lists, double* ax, double* ay, double* az, hipStream_t stream,
+                   hipEvent_t mid, bool split, int persistentBlocks);
+// Resident workgroups per CU of the fused FP32 walk (for the persistent grid).
+int walk32_blocks_per_cu();
 
 }  // namespace gx

@@ -102,26 +102,33 @@ __device__ __forceinline__ void eval_parts(const float4* __restrict__ part, cons
 }
 
 
-// Group prologue shared by both kernels: the group's stars, FP64 bounding box
-// and centre c_g.
+// Group prologue shared by both kernels: the group's stars (up to gsize
+// consecutive stars of one instance in Morton order), FP64 bounding box and
+// centre c_g, the instance's root, softening and root half side.
 struct Group {
     long long g, k;
+    int inst, root;
     bool valid;
+    float eps2;
+    double half;
     double4 p;
     uint64_t gmask;
     double bminx, bminy, bminz, bmaxx, bmaxy, bmaxz, cgx, cgy, cgz;
 };
 
-__device__ __forceinline__ bool load_group(Group& G, const double4* __restrict__ spos, long long nst) {
-    const long long ngroups = (nst + 63) / 64;
-    // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so give
-    // each XCD a contiguous run of (spatially adjacent) groups for its L2.
-    const long long per = gridDim.x / 8;
-    G.g = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (G.g >= ngroups) return false;
-    G.k = G.g * 64 + lane_id();
-    G.valid = G.k < nst;
-    G.p = spos[G.valid ? G.k : G.g * 64];
+__device__ __forceinline__ bool load_group(Group& G, const double4* __restrict__ spos, const Batch& batch,
+                                           long long g) {
+    if (g >= batch.groupOff[batch.B]) return false;
+    G.g = g;
+    G.inst = upper_index(batch.groupOff, batch.B, g);
+    const long long base = batch.starOff[G.inst] + (g - batch.groupOff[G.inst]) * batch.gsize;
+    const long long count = min((long long)batch.gsize, batch.starOff[G.inst + 1] - base);
+    G.root = batch.root[G.inst];
+    G.eps2 = (float)batch.eps2[G.inst];
+    G.half = batch.half[G.inst];
+    G.k = base + lane_id();
+    G.valid = lane_id() < count;
+    G.p = spos[G.valid ? G.k : base];
     G.gmask = __ballot(G.valid);
     G.bminx = wave_min_d(G.valid ? G.p.x : INFINITY);
     G.bmaxx = wave_max_d(G.valid ? G.p.x : -INFINITY);
@@ -133,6 +140,13 @@ __device__ __forceinline__ bool load_group(Group& G, const double4* __restrict__
     G.cgy = 0.5 * (G.bminy + G.bmaxy);
     G.cgz = 0.5 * (G.bminz + G.bmaxz);
     return true;
+}
+
+// Group of this workgroup: workgroups are dealt round-robin to the 8 XCDs, so
+// each XCD gets a contiguous run of (spatially adjacent) groups for its L2.
+__device__ __forceinline__ long long xcd_group() {
+    const long long per = gridDim.x / 8;
+    return (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
 }
 
 // ---------------------------------------------------------------------------
@@ -173,7 +187,7 @@ __device__ bool trav_group(const Group& G, TravLDS& S, const TravNode* __restric
     int nf = 0, np = 0, nl = 0, sp = 1, maxsp = 1, nrounds = 0, nmixed = 0;
     bool overflow = false;
     if (lane == 0) {
-        stkRange[0] = 0u;  // the root, one node
+        stkRange[0] = (uint32_t)G.root;  // the instance's root, one node
         stkMask[0] = G.gmask;
     }
     __syncthreads();
@@ -324,9 +338,9 @@ __device__ bool trav_group(const Group& G, TravLDS& S, const TravNode* __restric
 template <bool QUAD, bool EPS0, bool Q16>
 __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict__ spos,
                            const TravNode* __restrict__ trav, const Mult32* __restrict__ m32,
-                           const Mult16* __restrict__ m16, double rootHalf, const int32_t* __restrict__ order,
-                           float eps2, const Lists& lists, double* __restrict__ oax, double* __restrict__ oay,
-                           double* __restrict__ oaz) {
+                           const Mult16* __restrict__ m16, const int32_t* __restrict__ order, const Lists& lists,
+                           double* __restrict__ oax, double* __restrict__ oay, double* __restrict__ oaz) {
+    const float eps2 = G.eps2;
     float4* cell = S.cell;
     uint64_t* cmask = S.cmask;
     float4* part = S.part;
@@ -351,7 +365,7 @@ __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict
         const TravNode& t = trav[node];
         if (Q16) {
             const Mult16 mh = m16[node];
-            const float h = (float)ldexp(rootHalf, -t.level);
+            const float h = (float)ldexp(G.half, -t.level);
             const float sc = mh.m * h * h;
             auto q = [&](int c) { return __half2float(__ushort_as_half(mh.q[c])) * sc; };
             d.a = make_float4((float)(t.x - G.cgx), (float)(t.y - G.cgy), (float)(t.z - G.cgz), mh.m);
@@ -464,68 +478,88 @@ __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict
 }
 
 __global__ __launch_bounds__(64) void k_trav32(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
-                                               Meta* meta, Lists lists) {
+                                               Meta* meta, Batch batch, Lists lists) {
     __shared__ TravLDS S;
     if (meta->flags & kFlagNodeOverflow) return;
     Group G;
-    if (!load_group(G, spos, meta->nstars)) return;
+    if (!load_group(G, spos, batch, xcd_group())) return;
     trav_group(G, S, trav, meta, lists);
 }
 
 template <bool QUAD, bool EPS0, bool Q16>
 __global__ __launch_bounds__(64) void k_eval32(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
                                                const Mult32* __restrict__ m32, const Mult16* __restrict__ m16,
-                                               const int32_t* __restrict__ order, Meta* meta, float eps2,
+                                               const int32_t* __restrict__ order, Meta* meta, Batch batch,
                                                Lists lists, double* __restrict__ oax, double* __restrict__ oay,
                                                double* __restrict__ oaz) {
     __shared__ EvalLDS S;
     if (meta->flags & (kFlagNodeOverflow | kFlagStackOverflow | kFlagListOverflow)) return;
     Group G;
-    if (!load_group(G, spos, meta->nstars)) return;
-    eval_group<QUAD, EPS0, Q16>(G, S, spos, trav, m32, m16, meta->rootHalf, order, eps2, lists, oax, oay, oaz);
+    if (!load_group(G, spos, batch, xcd_group())) return;
+    eval_group<QUAD, EPS0, Q16>(G, S, spos, trav, m32, m16, order, lists, oax, oay, oaz);
 }
 
 // Traversal and evaluation of one group in one wavefront: the latency-bound
 // traversal of some waves overlaps the VALU-bound evaluation of others on the
 // same SIMD.  The lists still go through HBM (they stay in L2/MALL between
-// the two phases), so LDS holds only one phase's buffers at a time.
+// the two phases), so LDS holds only one phase's buffers at a time.  With
+// persistent set, a grid of resident waves takes groups from a work queue
+// (one launch for any number of groups and instances).
 template <bool QUAD, bool EPS0, bool Q16>
 __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
                                                const Mult32* __restrict__ m32, const Mult16* __restrict__ m16,
-                                               const int32_t* __restrict__ order, Meta* meta, float eps2,
-                                               Lists lists, double* __restrict__ oax, double* __restrict__ oay,
-                                               double* __restrict__ oaz) {
+                                               const int32_t* __restrict__ order, Meta* meta, Batch batch,
+                                               Lists lists, int persistent, double* __restrict__ oax,
+                                               double* __restrict__ oay, double* __restrict__ oaz) {
     __shared__ WalkLDS S;
     if (meta->flags & kFlagNodeOverflow) return;
-    Group G;
-    if (!load_group(G, spos, meta->nstars)) return;
-    const bool ok = trav_group(G, S.t, trav, meta, lists);
-    __syncthreads();  // lists written by this wave's lanes are read back by others
-    if (ok)
-        eval_group<QUAD, EPS0, Q16>(G, S.e, spos, trav, m32, m16, meta->rootHalf, order, eps2, lists, oax, oay,
-                                    oaz);
+    long long g = persistent ? -1 : xcd_group();
+    for (;;) {
+        if (persistent) {
+            unsigned long long next = 0;
+            if (lane_id() == 0) next = atomicAdd(&meta->groupNext, 1ull);
+            g = (long long)__shfl((long long)next, 0, kWave);
+        }
+        Group G;
+        if (!load_group(G, spos, batch, g)) return;
+        const bool ok = trav_group(G, S.t, trav, meta, lists);
+        __syncthreads();  // lists written by this wave's lanes are read back by others
+        if (ok) eval_group<QUAD, EPS0, Q16>(G, S.e, spos, trav, m32, m16, order, lists, oax, oay, oaz);
+        if (!persistent) return;
+        __syncthreads();
+    }
+}
+
+int walk32_blocks_per_cu() {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_walk32<true, false, false>, 64, 0) !=
+        hipSuccess)
+        return 0;
+    return per;
 }
 
 void launch_walk32(const double4* spos, const TravNode* trav, const Mult32* m32, const Mult16* m16,
-                   const int32_t* order, Meta* meta, long long ngroupsUpper, float eps2, int quad, const Lists& lists,
-                   double* ax, double* ay, double* az, hipStream_t stream, hipEvent_t mid, bool split) {
+                   const int32_t* order, Meta* meta, const Batch& batch, long long ngroupsUpper, bool eps0,
+                   int quad, const Lists& lists, double* ax, double* ay, double* az, hipStream_t stream,
+                   hipEvent_t mid, bool split, int persistentBlocks) {
     if (ngroupsUpper <= 0) return;
     const long long per = (ngroupsUpper + 7) / 8;
-    dim3 grid((unsigned)(per * 8)), block(64);
-    const bool eps0 = !(eps2 > 0.0f);
+    const bool persistent = !split && persistentBlocks > 0;
+    dim3 grid((unsigned)(persistent ? persistentBlocks : per * 8)), block(64);
     const bool q16 = m16 != nullptr;
+    const int pers = persistent ? 1 : 0;
     if (split) {
-        hipLaunchKernelGGL(k_trav32, grid, block, 0, stream, spos, trav, meta, lists);
+        hipLaunchKernelGGL(k_trav32, grid, block, 0, stream, spos, trav, meta, batch, lists);
         if (mid) (void)hipEventRecord(mid, stream);
     }
 #define GX_LAUNCH(Q, E, H)                                                                                    \
     do {                                                                                                      \
         if (split)                                                                                            \
-            hipLaunchKernelGGL((k_eval32<Q, E, H>), grid, block, 0, stream, spos, trav, m32, m16, order, meta, eps2, \
+            hipLaunchKernelGGL((k_eval32<Q, E, H>), grid, block, 0, stream, spos, trav, m32, m16, order, meta, batch, \
                                lists, ax, ay, az);                                                            \
         else                                                                                                  \
-            hipLaunchKernelGGL((k_walk32<Q, E, H>), grid, block, 0, stream, spos, trav, m32, m16, order, meta, eps2, \
-                               lists, ax, ay, az);                                                            \
+            hipLaunchKernelGGL((k_walk32<Q, E, H>), grid, block, 0, stream, spos, trav, m32, m16, order, meta, batch, \
+                               lists, pers, ax, ay, az);                                                      \
     } while (0)
 #define GX_LAUNCH_E(Q, H) \
     if (eps0) GX_LAUNCH(Q, true, H); else GX_LAUNCH(Q, false, H)

@@ -12,17 +12,19 @@ namespace gx {
 
 __global__ __launch_bounds__(256) void k_walk64(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
                                                 const Mult64* __restrict__ m64, const int32_t* __restrict__ order,
-                                                Meta* meta, double eps2, int quad, double* __restrict__ ax,
+                                                Meta* meta, Batch batch, int quad, double* __restrict__ ax,
                                                 double* __restrict__ ay, double* __restrict__ az) {
     const long long nst = meta->nstars;
     if (meta->flags & kFlagNodeOverflow) return;
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long npp = 0, npc = 0;
     if (k < nst) {
+        const int b = upper_index(batch.starOff, batch.B, k);
+        const double eps2 = batch.eps2[b];
         const double4 p = spos[k];
         const double x = p.x, y = p.y, z = p.z;
         double sx = 0, sy = 0, sz = 0;
-        for (int32_t i = 0; i >= 0;) {
+        for (int32_t i = batch.root[b]; i >= 0;) {
             const TravNode nd = trav[i];
             double dx = nd.x - x, dy = nd.y - y, dz = nd.z - z;
             double d2 = dx * dx + dy * dy + dz * dz;
@@ -83,16 +85,18 @@ __global__ __launch_bounds__(256) void k_walk64(const double4* __restrict__ spos
 // sums them in a fixed order.
 __global__ __launch_bounds__(256) void k_potential64(const double4* __restrict__ spos,
                                                      const TravNode* __restrict__ trav,
-                                                     const Mult64* __restrict__ m64, Meta* meta, double eps2,
+                                                     const Mult64* __restrict__ m64, Meta* meta, Batch batch,
                                                      int quad, double* __restrict__ partial) {
     const long long nst = meta->nstars;
     if (meta->flags & kFlagNodeOverflow) return;
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nst) return;
+    const int b = upper_index(batch.starOff, batch.B, k);
+    const double eps2 = batch.eps2[b];
     const double4 p = spos[k];
     const double x = p.x, y = p.y, z = p.z;
     double psi = 0;
-    for (int32_t i = 0; i >= 0;) {
+    for (int32_t i = batch.root[b]; i >= 0;) {
         const TravNode nd = trav[i];
         double dx = nd.x - x, dy = nd.y - y, dz = nd.z - z;
         double d2 = dx * dx + dy * dy + dz * dz;

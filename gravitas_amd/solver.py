@@ -98,7 +98,8 @@ class Solver:
           to the reference's CPU walk).
     """
 
-    def __init__(self, theta: float = 0.0, monopole_only: bool = False, walk: str = "fp32", timing: bool = False):
+    def __init__(self, theta: float = 0.0, monopole_only: bool = False, walk: str = "fp32", timing: bool = False,
+                 group_size: int = 64, persistent: bool = False):
         reason = gpu_available()
         if reason is not None:
             raise RuntimeError(f"GPU backend unavailable: {reason}")
@@ -111,6 +112,8 @@ class Solver:
         self.monopole_only = monopole_only
         self.walk = walk
         self.timing = timing
+        self.group_size = group_size
+        self.persistent = persistent
 
     def __del__(self):
         try:
@@ -124,6 +127,7 @@ class Solver:
         w = {"fp32": _lib.GX_WALK_FP32, "fp64": _lib.GX_WALK_FP64, "fp32-q16": _lib.GX_WALK_FP32_Q16}[self.walk]
         _lib.check(self._L.gx_solver_config(self._h, float(self.theta), int(self.monopole_only), w,
                                             int(self.timing)))
+        _lib.check(self._L.gx_solver_tune(self._h, int(self.group_size), int(bool(self.persistent))))
 
     def set_stream(self, stream):
         """Launch on the given hipStream_t (a pointer-sized int or None)."""
@@ -161,6 +165,47 @@ class Solver:
         _lib.check(self._L.gx_accelerations(self._h, c(x), c(y), c(z), c(m), c(kind), c(alive), n, float(eps),
                                             int(dm_off), c(ax), c(ay), c(az)))
         return np.stack([ax, ay, az], axis=1)
+
+    # -- batches of independent systems -------------------------------------------------
+    @staticmethod
+    def _batch_args(offsets, theta, eps):
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        b = len(off) - 1
+        th = np.ascontiguousarray(np.broadcast_to(np.asarray(theta, dtype=np.float64), (b,)))
+        ep = np.ascontiguousarray(np.broadcast_to(np.asarray(eps, dtype=np.float64), (b,)))
+        keep = (off, th, ep)
+        c = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        return keep, c(off), b, c(th), c(ep)
+
+    def batch_accelerations(self, s: DeviceState, offsets, theta, eps, dark_matter_off=False):
+        """Accelerations of independent systems packed in s: system b owns
+        bodies [offsets[b], offsets[b+1]); theta and eps per system (or one value)."""
+        self._configure()
+        keep, off, b, th, ep = self._batch_args(offsets, theta, eps)
+        _lib.check(self._L.gx_batch_accelerations_dev(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.m), _ptr(s.kind), _ptr(s.alive), off, b, th, ep,
+            int(dark_matter_off), _ptr(s.ax), _ptr(s.ay), _ptr(s.az)))
+
+    def batch_step(self, s: DeviceState, offsets, theta, eps, dt, acc_valid=True, dark_matter_off=False):
+        self._configure()
+        keep, off, b, th, ep = self._batch_args(offsets, theta, eps)
+        _lib.check(self._L.gx_batch_step_dev(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.vx), _ptr(s.vy), _ptr(s.vz), _ptr(s.m),
+            _ptr(s.kind), _ptr(s.alive), off, b, th, ep, int(dark_matter_off), float(dt), int(bool(acc_valid)),
+            _ptr(s.ax), _ptr(s.ay), _ptr(s.az)))
+
+    def graph_capture_step(self, s: DeviceState, offsets, theta, eps, dt, dark_matter_off=False):
+        """Capture one batched KDK step as a hipGraph (after one eager force
+        evaluation); replay it with graph_launch."""
+        self._configure()
+        keep, off, b, th, ep = self._batch_args(offsets, theta, eps)
+        _lib.check(self._L.gx_graph_capture_step(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.vx), _ptr(s.vy), _ptr(s.vz), _ptr(s.m),
+            _ptr(s.kind), _ptr(s.alive), off, b, th, ep, int(dark_matter_off), float(dt),
+            _ptr(s.ax), _ptr(s.ay), _ptr(s.az)))
+
+    def graph_launch(self, steps: int = 1):
+        _lib.check(self._L.gx_graph_launch(self._h, int(steps)))
 
     # -- KDK step (sim.go:262-281, 302-343) -------------------------------------------
     def step(self, s: DeviceState, p: Params, acc_valid: bool = True):

@@ -71,6 +71,12 @@ void gx_solver_free(gx_solver *s);
  * per-phase GPU times (gx_phase_ms). */
 int gx_solver_config(gx_solver *s, double theta, int monopole_only, int walk, int timing);
 
+/* Walk tuning (no counterpart in the reference): stars per walk group (32 or
+ * 64; one wavefront per group) and whether the FP32 walk runs as a
+ * persistent work queue over resident workgroups (1) or one workgroup per
+ * group (0).  Used by the autotuner. */
+int gx_solver_tune(gx_solver *s, int group_size, int persistent);
+
 /* Uses the given hipStream_t (passed as void*) instead of the solver's own
  * stream; NULL restores the solver's stream. */
 int gx_solver_set_stream(gx_solver *s, void *stream);
@@ -112,6 +118,35 @@ int gx_kick_dev(gx_solver *s, double *vx, double *vy, double *vz, const double *
                 const double *az, int64_t n, double h);
 int gx_drift_dev(gx_solver *s, double *x, double *y, double *z, const double *vx, const double *vy,
                  const double *vz, int64_t n, double dt);
+
+/* Batched force step (no counterpart in the reference, which runs one
+ * Simulation per engine): count independent systems packed one after the
+ * other in the device arrays; system b owns bodies [offsets[b], offsets[b+1])
+ * (host array, offsets[0] = 0) and has its own theta[b] and eps[b] (host
+ * arrays).  Each system gets exactly the accelerations a separate
+ * gx_accelerations_dev call on its bodies would give; all systems are built
+ * and walked in one pass (one walk launch).  Synchronous. */
+int gx_batch_accelerations_dev(gx_solver *s, const double *x, const double *y, const double *z,
+                               const double *mass, const uint8_t *kind, const uint8_t *alive,
+                               const int64_t *offsets, int32_t count, const double *theta, const double *eps,
+                               int dark_matter_off, double *ax, double *ay, double *az);
+
+/* KDK leapfrog step of every system of a batch (as gx_step_dev). */
+int gx_batch_step_dev(gx_solver *s, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                      const double *mass, const uint8_t *kind, const uint8_t *alive, const int64_t *offsets,
+                      int32_t count, const double *theta, const double *eps, int dark_matter_off, double dt,
+                      int acc_valid, double *ax, double *ay, double *az);
+
+/* Captures one KDK step of the batch (kick, drift, force step, kick) into a
+ * hipGraph on the solver's stream, after one eager force evaluation that
+ * sizes every buffer; gx_graph_launch replays it `steps` times without host
+ * work between steps, and gx_sync waits for the replays.  The arrays, the
+ * instance layout and the parameters are fixed at capture. */
+int gx_graph_capture_step(gx_solver *s, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                          const double *mass, const uint8_t *kind, const uint8_t *alive, const int64_t *offsets,
+                          int32_t count, const double *theta, const double *eps, int dark_matter_off, double dt,
+                          double *ax, double *ay, double *az);
+int gx_graph_launch(gx_solver *s, int32_t steps);
 
 /* bh.Solver.PotentialEnergy (potential.go:12-42) on device arrays: theta
  * capped at 0.9; FP64 tree walk.  Star-star pairs only: returns
