@@ -34,7 +34,7 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector, datasheet
 HBM_PEAK_GBS = 8000.0
 FLOPS_PP = 18              # bh.go:145-150 counting rule (add/sub/mul/div/sqrt = 1, fma = 2)
 FLOPS_PC = 50              # bh.go:114-131
-CONFIGS = ("disk-halo-1m", "plummer-16k", "clumps-16m")
+CONFIGS = ("disk-halo-1m", "plummer-16k", "clumps-16m", "batched-256")
 
 
 def parse():
@@ -276,6 +276,78 @@ def bench_clumps(args):
     return out
 
 
+def bench_batched(args):
+    """256 independent Plummer/Hernquist systems, N log-uniform in [1024, 65536]
+    (seeded), theta 0.6, eps 0.01 N^(-1/3), one arena.  Autotunes the walk
+    (group size 32/64, persistent work queue or one workgroup per group) on
+    eager steps, captures the best as a hipGraph KDK step and replays it;
+    compared with the same step done as 256 separate per-system calls."""
+    from gravitas_amd import DeviceState, Solver, ics
+    from gravitas_amd.hip import synchronize
+    systems, eps = ics.batched_systems(256)
+    sizes = np.array([len(s["px"]) for s in systems], dtype=np.int64)
+    offsets = np.r_[0, np.cumsum(sizes)].astype(np.int64)
+    state = ics.concat(*systems)
+    n = int(offsets[-1])
+    theta, dt = 0.6, 1e-3
+    log(f"batched: {len(sizes)} systems, n={n}")
+    # autotune on eager batched force steps
+    tune = {}
+    for gsize in (64, 32):
+        for pers in (False, True):
+            s = Solver(walk="fp32", group_size=gsize, persistent=pers)
+            ds = DeviceState.from_host(state)
+            s.batch_accelerations(ds, offsets, theta, eps)
+            synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                s.batch_accelerations(ds, offsets, theta, eps)
+            synchronize()
+            tune[f"group{gsize}{'-persistent' if pers else ''}"] = (time.perf_counter() - t0) / 5 * 1e3
+    best = min(tune, key=tune.get)
+    gsize, pers = (32 if best.startswith("group32") else 64), best.endswith("persistent")
+    log(f"autotune {tune} -> {best}")
+    # graph-captured KDK step, replayed
+    solver = Solver(walk="fp32", group_size=gsize, persistent=pers, timing=True)
+    ds = DeviceState.from_host(state)
+    solver.graph_capture_step(ds, offsets, theta, eps, dt)
+    solver.graph_launch(max(1, args.warmup))
+    solver.sync()
+    synchronize()
+    t0 = time.perf_counter()
+    solver.graph_launch(args.steps)
+    solver.sync()
+    synchronize()
+    ms_graph = (time.perf_counter() - t0) * 1e3 / args.steps
+    st = solver.stats()
+    # the same KDK step as 256 separate per-system calls (one solver reused)
+    sep = Solver(theta=theta, walk="fp32")
+    ds2 = DeviceState.from_host(state)
+    views = [ds2.view(int(offsets[b]), int(sizes[b])) for b in range(len(sizes))]
+    from gravitas_amd import Params
+    prm = [Params(softening=eps[b], dt=dt) for b in range(len(sizes))]
+    for b in range(len(sizes)):
+        sep.step(views[b], prm[b], acc_valid=False)
+    nsep = max(2, args.steps // 4)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nsep):
+        for b in range(len(sizes)):
+            sep.step(views[b], prm[b], acc_valid=True)
+    synchronize()
+    ms_sep = (time.perf_counter() - t0) * 1e3 / nsep
+    out = base_line(args, n, ms_graph, "fp32", {
+        "workload": "batched-256", "systems": len(sizes), "n": n, "theta": theta,
+        "eps": "0.01 N^(-1/3) per system", "sizes": "log-uniform [1024, 65536], seed 4",
+        "step": "KDK (kick, drift, force step, kick), hipGraph replay", "walk_group": best})
+    out["steps_per_s"] = 1e3 / ms_graph
+    out["separate_launches"] = {"ms_per_step": ms_sep, "steps_per_s": 1e3 / ms_sep}
+    out["batched_over_separate"] = ms_sep / ms_graph
+    out["autotune_ms_per_force_step"] = tune
+    out["interactions_per_particle"] = {"pp": st["pp_interactions"] / n, "pc": st["pc_interactions"] / n}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -287,7 +359,8 @@ def main():
     if reason is not None:
         print(f"no GPU backend: {reason}", file=sys.stderr)
         sys.exit(1)
-    fn = {"disk-halo-1m": bench_disk_halo, "plummer-16k": bench_plummer, "clumps-16m": bench_clumps}[args.config]
+    fn = {"disk-halo-1m": bench_disk_halo, "plummer-16k": bench_plummer, "clumps-16m": bench_clumps,
+          "batched-256": bench_batched}[args.config]
     print(json.dumps(fn(args)), flush=True)
 
 

@@ -56,14 +56,25 @@ class DeviceArray:
     def ptr(self):
         return self._p
 
+    def view(self, start, count):
+        """Non-owning view of elements [start, start+count) of a 1-D array."""
+        v = DeviceArray.__new__(DeviceArray)
+        v.shape = (count,)
+        v.dtype = self.dtype
+        v.nbytes = count * self.dtype.itemsize
+        v._p = ctypes.c_void_p((self._p.value or 0) + start * self.dtype.itemsize)
+        v._owner = self
+        v._view = True
+        return v
+
     def __len__(self):
         return self.shape[0]
 
     def __del__(self):
         try:
-            if self._p:
+            if self._p and not getattr(self, "_view", False):
                 rt().hipFree(self._p)
-                self._p = None
+            self._p = None
         except Exception:
             pass
 

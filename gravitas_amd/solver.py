@@ -80,6 +80,10 @@ class DeviceState:
     def __len__(self):
         return len(self.px)
 
+    def view(self, start, count):
+        """The bodies [start, start+count) as a state of their own (no copy)."""
+        return DeviceState(**{f: getattr(self, f).view(start, count) for f in self.FIELDS + ("kind", "alive")})
+
     def acc_host(self):
         return np.stack([self.ax.numpy(), self.ay.numpy(), self.az.numpy()], axis=1)
 
