@@ -14,7 +14,10 @@ the MI355X vector peak (HIP events around the walk on the solver's stream);
 and the CPU oracle's throughput on the host cores.
 
 Other workloads (--config): plummer-16k (force + one KDK step, FP64 and FP32),
-clumps-16m (precision ladder FP64 / FP32 / FP32 with FP16-stored quadrupoles).
+clumps-16m (precision ladder FP64 / FP32 / FP32 with FP16-stored quadrupoles),
+batched-256 (256 systems in one arena, hipGraph KDK vs separate calls) and
+two-disk-4m (4,194,304-body encounter, block time steps on 4 rungs, FP16
+quadrupoles, FP64 energy drift over --steps global steps).
 """
 from __future__ import annotations
 
@@ -34,7 +37,8 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector, datasheet
 HBM_PEAK_GBS = 8000.0
 FLOPS_PP = 18              # bh.go:145-150 counting rule (add/sub/mul/div/sqrt = 1, fma = 2)
 FLOPS_PC = 50              # bh.go:114-131
-CONFIGS = ("disk-halo-1m", "plummer-16k", "clumps-16m", "batched-256")
+ENCOUNTER_T = 0.16          # global step of the two-disk encounter: about 3 % of the stars on the top rung
+CONFIGS = ("disk-halo-1m", "plummer-16k", "clumps-16m", "batched-256", "two-disk-4m")
 
 
 def parse():
@@ -348,6 +352,89 @@ def bench_batched(args):
     return out
 
 
+def bench_encounter(args):
+    """Two-disk encounter: two disk+halo galaxies of 2,097,152 bodies each on a
+    parabolic orbit (pericentre 5 R_d, 30 deg inclination), theta 0.5, eps
+    0.02, FP32 walk with FP16-stored quadrupoles.  Each global step T runs
+    block time steps on 4 power-of-two rungs (8 ticks): every tick drifts all
+    bodies and evaluates only those whose step ends there.  Energy (kinetic
+    + FP64 tree potential) and momentum are measured in FP64 on the GPU before
+    and after the timed steps."""
+    from gravitas_amd import DeviceState, Params, Solver, ics
+    from gravitas_amd.hip import synchronize
+    n = args.n or 4194304
+    theta = 0.5 if args.theta is None else args.theta
+    eps = 0.02 if args.eps is None else args.eps
+    levels, eta = 4, 0.025
+    log(f"building two-disk encounter n={n}")
+    state = ics.two_disk_encounter(n, seed=3)
+    ds = DeviceState.from_host(state)
+    params = Params(softening=eps)
+    solver = Solver(theta=theta, walk="fp32-q16")
+    solver.accelerations(ds, params)  # initial forces
+    acc0 = ds.acc_host()
+    a = np.linalg.norm(acc0, axis=1)
+    T = ENCOUNTER_T
+    dt_wanted = np.sqrt(2 * eta * eps / a[a > 0])
+    out_err = {}
+    if not args.no_errors:
+        log("FP64 walk and direct-sum sample of the initial forces")
+        d64 = DeviceState.from_host(state)
+        Solver(theta=theta, walk="fp64").accelerations(d64, params)
+        acc64 = d64.acc_host()
+        targets = ics.sample_targets(n, 4096, 7)
+        direct = solver.direct_sample(ds, params, targets).numpy()
+        out_err = {"err_fp32q16_vs_fp64_walk": errors(acc0, acc64),
+                   "err_fp32q16_vs_direct": errors(acc0[targets], direct)}
+        del d64
+    diag0 = solver.diagnostics(ds, params)
+    u0 = solver.potential_energy(ds, params)
+    e0 = diag0["kinetic"] + u0
+    for _ in range(args.warmup):  # warm-up steps advance the state; energy is taken after them too
+        solver.block_step(ds, params, T, levels, eta, acc_valid=True)
+    diag_w = solver.diagnostics(ds, params)
+    e_w = diag_w["kinetic"] + solver.potential_energy(ds, params)
+    solver.accelerations(ds, params)  # forces of the current positions (the potential call rebuilt the tree)
+    synchronize()
+    log(f"timing {args.steps} global steps of T={T} on {levels} rungs")
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        infos.append(solver.block_step(ds, params, T, levels, eta, acc_valid=True))
+    synchronize()
+    wall = time.perf_counter() - t0
+    ms = wall * 1e3 / args.steps
+    diag1 = solver.diagnostics(ds, params)
+    u1 = solver.potential_energy(ds, params)
+    e1 = diag1["kinetic"] + u1
+    ticks = 1 << (levels - 1)
+    act = np.zeros(ticks)
+    for inf in infos:
+        for t, c in zip(inf["tick"], inf["active"]):
+            act[t - 1] += c
+    act /= len(infos) * n
+    targets_total = sum(inf["targets"] for inf in infos)
+    rungs = np.mean([inf["rung_count"] for inf in infos], axis=0) / n
+    out = base_line(args, n, ms, "fp32", {
+        "workload": "two-disk-4m", "n": n, "theta": theta, "eps": eps, "levels": levels, "eta": eta, "T": T,
+        "model": "two disk+halo galaxies (2 x 2,097,152), parabolic orbit, pericentre 5 R_d, inclination 30 deg",
+        "walk": "fp32 group walk, FP16-stored quadrupoles, Kahan-compensated sums",
+        "step": "one global step T = 8 ticks of block steps; active stars evaluated per tick"})
+    out["unit"] = "particle-steps/s"
+    out["force_evaluations_per_s"] = targets_total / wall
+    out["evaluations_per_global_step"] = targets_total / len(infos) / n
+    out["active_fraction_per_tick"] = act.tolist()
+    out["rung_fraction"] = rungs.tolist()
+    out["wanted_step_quantiles"] = {q: float(np.quantile(dt_wanted, q)) for q in (0.01, 0.03, 0.1, 0.5)}
+    out["energy"] = {"E_start": e0, "E_after_warmup": e_w, "E_end": e1,
+                     "drift_over_timed_steps": (e1 - e_w) / abs(e_w), "drift_total": (e1 - e0) / abs(e0),
+                     "global_steps": args.steps + args.warmup, "time_span": (args.steps + args.warmup) * T}
+    p0, p1 = np.array(diag0["momentum"]), np.array(diag1["momentum"])
+    out["momentum_change"] = float(np.linalg.norm(p1 - p0))
+    out.update(out_err)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -360,7 +447,7 @@ def main():
         print(f"no GPU backend: {reason}", file=sys.stderr)
         sys.exit(1)
     fn = {"disk-halo-1m": bench_disk_halo, "plummer-16k": bench_plummer, "clumps-16m": bench_clumps,
-          "batched-256": bench_batched}[args.config]
+          "batched-256": bench_batched, "two-disk-4m": bench_encounter}[args.config]
     print(json.dumps(fn(args)), flush=True)
 
 

@@ -29,7 +29,12 @@ EXPORTS = (
     "gx_sync", "gx_potential_energy_dev", "gx_direct_sample_dev", "gx_last_seconds", "gx_get_stats",
     "gx_phase_ms", "gx_export", "gx_step_dev", "gx_kick_dev", "gx_drift_dev", "gx_solver_tune",
     "gx_batch_accelerations_dev", "gx_batch_step_dev", "gx_graph_capture_step", "gx_graph_launch",
+    "gx_accelerations_active_dev", "gx_block_step_dev", "gx_diagnostics_dev",
 )
+
+GX_BLOCK_MAX_LEVELS = 8
+GX_BLOCK_MAX_TICKS = 128
+GX_DIAG_VALUES = 11
 
 
 class GxStats(ctypes.Structure):
@@ -39,7 +44,15 @@ class GxStats(ctypes.Structure):
         ("max_stack", ctypes.c_int32), ("flags", ctypes.c_int32), ("depth", ctypes.c_int32),
         ("walk", ctypes.c_int32), ("fallbacks", ctypes.c_int64), ("list_entries", ctypes.c_int64),
         ("rounds", ctypes.c_int64), ("mixed_nodes", ctypes.c_int64),
-        ("max_list", ctypes.c_int32 * 4),
+        ("max_list", ctypes.c_int32 * 4), ("n_targets", ctypes.c_int64),
+    ]
+
+
+class GxBlockStats(ctypes.Structure):
+    _fields_ = [
+        ("levels", ctypes.c_int32), ("events", ctypes.c_int32), ("bodies", ctypes.c_int64),
+        ("targets", ctypes.c_int64), ("rung_count", ctypes.c_int64 * GX_BLOCK_MAX_LEVELS),
+        ("tick", ctypes.c_int32 * GX_BLOCK_MAX_TICKS), ("active", ctypes.c_int64 * GX_BLOCK_MAX_TICKS),
     ]
 
 
@@ -93,6 +106,10 @@ def load():
     L.gx_batch_step_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, P, ctypes.c_int32, P, P, I, D, I, P, P, P]
     L.gx_graph_capture_step.argtypes = [P, P, P, P, P, P, P, P, P, P, P, ctypes.c_int32, P, P, I, D, P, P, P]
     L.gx_graph_launch.argtypes = [P, ctypes.c_int32]
+    L.gx_accelerations_active_dev.argtypes = [P, P, P, P, P, P, P, P, I64, D, I, P, P, P]
+    L.gx_block_step_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, I64, D, I, D, ctypes.c_int32, D, I, P, P, P,
+                                    ctypes.POINTER(GxBlockStats)]
+    L.gx_diagnostics_dev.argtypes = [P, P, P, P, P, P, P, P, P, I64, I, ctypes.POINTER(ctypes.c_double)]
     L.gx_export.argtypes = [P, I, P, I64]
     L.gx_export.restype = I64
     _lib = L

@@ -70,6 +70,7 @@ struct StepArgs {
     long long n;
     int dmOff;
     double *ax, *ay, *az;
+    const uint8_t* active;  // walk only these bodies (state order); null: all
 };
 
 }  // namespace
@@ -91,6 +92,9 @@ struct gx_solver {
     DevBuf keys, keys2, order, order2, holes, hist, digitTotal, blockStar, blockHole, spos;
     DevBuf trav, m64, m32, m16, meta, partial;
     DevBuf lfull, lpart, lleaf, lcounts;
+    // active sets and block steps
+    DevBuf actS, blockAct, tOff, tlist, activeMask, rung, rungHist, diag;
+    unsigned long long* hhist = nullptr;  // pinned mirror of hist
     int cf = 2048, cp = 512, cl = 256;  // FP32-walk list capacities per group (grow on demand)
     long long capNodes = 0;
     // instances: device config (offsets | theta | eps2) and per-instance results
@@ -162,6 +166,7 @@ gx_solver* gx_solver_new(void) {
     gx_solver* s = new gx_solver();
     if (hipStreamCreateWithFlags(&s->own, hipStreamNonBlocking) != hipSuccess ||
         hipHostMalloc((void**)&s->hmeta, sizeof(Meta), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&s->hhist, 64 * 8, hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&s->instCopied, hipEventDisableTiming) != hipSuccess) {
         set_err(GX_ERR_HIP, "stream, event or pinned allocation failed");
         delete s;
@@ -189,7 +194,9 @@ void gx_solver_free(gx_solver* s) {
                       &s->order2,   &s->holes,   &s->hist,    &s->digitTotal, &s->blockStar, &s->blockHole,
                       &s->spos,     &s->trav,    &s->m64,     &s->m32,       &s->m16,       &s->meta,
                       &s->partial,  &s->lfull,   &s->lpart,   &s->lleaf,     &s->lcounts,   &s->instCfg,
-                      &s->boxEnc,   &s->starCount, &s->starOff, &s->groupOff, &s->root,      &s->half};
+                      &s->boxEnc,   &s->starCount, &s->starOff, &s->groupOff, &s->root,      &s->half,
+                      &s->actS,     &s->blockAct, &s->tOff,    &s->tlist,     &s->activeMask, &s->rung,
+                      &s->rungHist, &s->diag};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto& e : s->ev)
@@ -197,6 +204,7 @@ void gx_solver_free(gx_solver* s) {
     if (s->instCopied) (void)hipEventDestroy(s->instCopied);
     if (s->instHost) (void)hipHostFree(s->instHost);
     if (s->hmeta) (void)hipHostFree(s->hmeta);
+    if (s->hhist) (void)hipHostFree(s->hhist);
     if (s->own) (void)hipStreamDestroy(s->own);
     delete s;
 }
@@ -291,6 +299,8 @@ Batch make_batch(gx_solver* s) {
     b.groupOff = (long long*)s->groupOff.p;
     b.root = (int*)s->root.p;
     b.half = (double*)s->half.p;
+    b.tOff = b.starOff;
+    b.tlist = nullptr;
     return b;
 }
 
@@ -385,10 +395,38 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
     return GX_OK;
 }
 
-int enqueue_walk(gx_solver* s, int walk, const StepArgs& a) {
+// Target list of an active set (after the build): active sorted stars,
+// per-instance target offsets and walk groups.
+int enqueue_active(gx_solver* s, const StepArgs& a) {
+    const long long n = a.n;
+    const long long nb = std::max(1LL, (n + kCollectTile - 1) / kCollectTile);
+    int rc;
+    if ((rc = ensure(s->actS, std::max(n, 1LL))) || (rc = ensure(s->blockAct, nb * 4)) ||
+        (rc = ensure(s->tOff, (size_t)(s->B + 1) * 8)) || (rc = ensure(s->tlist, std::max(n, 1LL) * 4)))
+        return rc;
     hipStream_t st = s->stream;
     Meta* meta = (Meta*)s->meta.p;
     const Batch batch = make_batch(s);
+    hipLaunchKernelGGL(k_active_count, dim3((unsigned)nb), dim3(256), 0, st, a.active, (const int32_t*)s->order.p,
+                       meta, (uint8_t*)s->actS.p, (int*)s->blockAct.p);
+    hipLaunchKernelGGL(k_active_scan, dim3(1), dim3(1024), 0, st, (int)nb, (int*)s->blockAct.p, meta);
+    hipLaunchKernelGGL(k_active_inst, dim3(blocks((long long)(s->B + 1) * kWave, 256)), dim3(256), 0, st,
+                       (const uint8_t*)s->actS.p, (const int*)s->blockAct.p, meta, batch, (long long*)s->tOff.p);
+    hipLaunchKernelGGL(k_active_groups, dim3(1), dim3(1024), 0, st, batch, (const long long*)s->tOff.p);
+    hipLaunchKernelGGL(k_active_scatter, dim3((unsigned)nb), dim3(256), 0, st, (const uint8_t*)s->actS.p,
+                       (const int*)s->blockAct.p, meta, (int32_t*)s->tlist.p);
+    HIPCHK(hipGetLastError());
+    return GX_OK;
+}
+
+int enqueue_walk(gx_solver* s, int walk, const StepArgs& a) {
+    hipStream_t st = s->stream;
+    Meta* meta = (Meta*)s->meta.p;
+    Batch batch = make_batch(s);
+    if (a.active) {
+        batch.tOff = (const long long*)s->tOff.p;
+        batch.tlist = (const int32_t*)s->tlist.p;
+    }
     const long long n = a.n;
     if (walk == GX_WALK_FP32 || walk == GX_WALK_FP32_Q16) {
         const long long ng = (n + s->gsize - 1) / s->gsize + s->B;
@@ -423,6 +461,7 @@ int enqueue_step(gx_solver* s, const StepArgs& a) {
     if (a.n >= (1LL << 31) - 1) return set_err(GX_ERR_ARG, "n = %lld exceeds the int32 index range", a.n);
     int rc = enqueue_build(s, a);
     if (rc) return rc;
+    if (a.active && (rc = enqueue_active(s, a))) return rc;
     if ((rc = enqueue_walk(s, s->walk, a))) return rc;
     if (s->timing) HIPCHK(hipEventRecord(s->ev[kPhases], s->stream));
     HIPCHK(hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, s->stream));
@@ -460,7 +499,8 @@ int finish_step(gx_solver* s) {
     s->stats.n_nodes = h.nnodes;
     s->stats.pc_interactions = (int64_t)h.pc;
     // self pairs are counted by the kernels; the reference skips them
-    s->stats.pp_interactions = (int64_t)h.pp - (int64_t)h.nstars;
+    s->stats.pp_interactions = (int64_t)h.pp - (int64_t)h.ntargets;
+    s->stats.n_targets = h.ntargets;
     s->stats.max_stack = h.maxStack;
     s->stats.flags = h.flags;
     s->stats.depth = depth;
@@ -520,6 +560,7 @@ StepArgs step_args(const double* x, const double* y, const double* z, const doub
     a.n = n;
     a.dmOff = dmOff;
     a.ax = ax; a.ay = ay; a.az = az;
+    a.active = nullptr;
     return a;
 }
 
@@ -574,9 +615,126 @@ int kdk(gx_solver* s, const StepArgs& a, double* vx, double* vy, double* vz, dou
     return GX_OK;
 }
 
+// Block step of length T (blockstep.go's event loop, :130-198, for gravity):
+// rungs and opening kicks at tick 0; then, from event to event, every body
+// drifts, the bodies whose step ends are evaluated (tree of all stars, walk
+// of the active ones), kicked, and given a new step.  At the last tick every
+// body is active, so the step ends synchronised with valid forces.
+int block_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, double T, int levels, double eta,
+               double eps, int accValid, gx_block_stats* out) {
+    if (levels < 1 || levels > GX_BLOCK_MAX_LEVELS) return set_err(GX_ERR_ARG, "levels %d not in [1, %d]", levels, GX_BLOCK_MAX_LEVELS);
+    if (!(T > 0)) return set_err(GX_ERR_ARG, "bad step %g", T);
+    const long long n = a.n;
+    const int maxRung = levels - 1, full = 1 << maxRung;
+    const double unit = T / full;
+    const double crit = 2 * eta * eps;
+    double* x = (double*)a.x;
+    double* y = (double*)a.y;
+    double* z = (double*)a.z;
+    gx_block_stats st;
+    memset(&st, 0, sizeof st);
+    st.levels = levels;
+    st.bodies = n;
+    int rc;
+    if ((rc = ensure(s->rung, std::max(n, 1LL))) || (rc = ensure(s->activeMask, std::max(n, 1LL))) ||
+        (rc = ensure(s->rungHist, 64 * 8)))
+        return rc;
+    if (!accValid && (rc = run_dev(s, a))) return rc;
+    hipStream_t strm = s->stream;
+    unsigned long long* hist = (unsigned long long*)s->rungHist.p;
+    const unsigned grid = (unsigned)std::min<long long>(blocks(n, 256), 4096);
+    auto kick_at = [&](int now) -> int {
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, strm, hist, 64, 0ull);
+        if (n > 0)
+            hipLaunchKernelGGL(k_block_kick, dim3(grid), dim3(256), 0, strm, n, vx, vy, vz, (const double*)a.ax,
+                               (const double*)a.ay, (const double*)a.az, a.kind, a.alive, (int8_t*)s->rung.p, now,
+                               full, maxRung, T, crit, hist);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(s->hhist, hist, 64 * 8, hipMemcpyDeviceToHost, strm));
+        HIPCHK(hipStreamSynchronize(strm));
+        return GX_OK;
+    };
+    if ((rc = kick_at(0))) return rc;
+    for (int r = 0; r <= maxRung; r++) st.rung_count[r] = (int64_t)s->hhist[r];
+    StepArgs act = a;
+    act.active = (const uint8_t*)s->activeMask.p;
+    for (int now = 0; now < full;) {
+        int next = full;
+        for (int r = 0; r <= maxRung; r++)
+            if (s->hhist[r]) next = std::min(next, (now / (full >> r) + 1) * (full >> r));
+        if ((rc = drift(s, x, y, z, vx, vy, vz, n, (double)(next - now) * unit))) return rc;
+        now = next;
+        if (n > 0)
+            hipLaunchKernelGGL(k_block_active, dim3(grid), dim3(256), 0, strm, n, (const int8_t*)s->rung.p, now, full,
+                               (uint8_t*)s->activeMask.p);
+        HIPCHK(hipGetLastError());
+        if ((rc = run_dev(s, act))) return rc;
+        if (st.events < GX_BLOCK_MAX_TICKS) {
+            st.tick[st.events] = now;
+            st.active[st.events] = s->hmeta->ntargets;
+        }
+        st.events++;
+        st.targets += s->hmeta->ntargets;
+        if ((rc = kick_at(now))) return rc;
+    }
+    if (out) *out = st;
+    return GX_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int gx_accelerations_active_dev(gx_solver* s, const double* x, const double* y, const double* z,
+                                const double* mass, const uint8_t* kind, const uint8_t* alive,
+                                const uint8_t* active, int64_t n, double eps, int dark_matter_off, double* ax,
+                                double* ay, double* az) {
+    if (!s) return set_err(GX_ERR_ARG, "null solver");
+    if (n > 0 && !active) return set_err(GX_ERR_ARG, "null active mask");
+    StepArgs a = step_args(x, y, z, mass, kind, alive, n, dark_matter_off, ax, ay, az);
+    a.active = active;
+    int rc;
+    if ((rc = check_arrays(a)) || (rc = set_single(s, n, s->theta, eps))) return rc;
+    return run_dev(s, a);
+}
+
+int gx_block_step_dev(gx_solver* s, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                      const double* mass, const uint8_t* kind, const uint8_t* alive, int64_t n, double eps,
+                      int dark_matter_off, double T, int32_t levels, double eta, int acc_valid, double* ax,
+                      double* ay, double* az, gx_block_stats* out) {
+    if (!s) return set_err(GX_ERR_ARG, "null solver");
+    const StepArgs a = step_args(x, y, z, mass, kind, alive, n, dark_matter_off, ax, ay, az);
+    int rc;
+    if ((rc = check_arrays(a)) || (n > 0 && (!vx || !vy || !vz) && (rc = set_err(GX_ERR_ARG, "null array"))) ||
+        (rc = set_single(s, n, s->theta, eps)))
+        return rc;
+    return block_step(s, a, vx, vy, vz, T, levels, eta, eps, acc_valid, out);
+}
+
+int gx_diagnostics_dev(gx_solver* s, const double* x, const double* y, const double* z, const double* vx,
+                       const double* vy, const double* vz, const double* mass, const uint8_t* kind, int64_t n,
+                       int dark_matter_off, double* out) {
+    if (!s || !out) return set_err(GX_ERR_ARG, "null argument");
+    if (n < 0) return set_err(GX_ERR_ARG, "negative n");
+    if (n > 0 && (!x || !y || !z || !vx || !vy || !vz || !mass || !kind)) return set_err(GX_ERR_ARG, "null array");
+    if (s->pending) (void)finish_step(s);
+    for (int c = 0; c < GX_DIAG_VALUES; c++) out[c] = 0;
+    if (n == 0) return GX_OK;
+    const int nb = (int)std::min<long long>(blocks(n, 256), kDiagBlocks);
+    int rc;
+    if ((rc = ensure(s->diag, (size_t)nb * kDiagValues * 8))) return rc;
+    hipLaunchKernelGGL(k_diag, dim3(nb), dim3(256), 0, s->stream, (long long)n, x, y, z, vx, vy, vz, mass, kind,
+                       dark_matter_off, (double*)s->diag.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h((size_t)nb * kDiagValues);
+    HIPCHK(hipMemcpyAsync(h.data(), s->diag.p, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    double acc[kDiagValues] = {};
+    for (int b = 0; b < nb; b++)  // fixed order: the result does not depend on timing
+        for (int c = 0; c < kDiagValues; c++) acc[c] += h[(size_t)b * kDiagValues + c];
+    for (int c = 0; c < kDiagValues; c++) out[c] = acc[c];
+    return GX_OK;
+}
 
 int gx_accelerations_dev(gx_solver* s, const double* x, const double* y, const double* z, const double* mass,
                          const uint8_t* kind, const uint8_t* alive, int64_t n, double eps, int dark_matter_off,

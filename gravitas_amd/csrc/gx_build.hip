@@ -43,6 +43,7 @@ __global__ void k_step_init(Meta* meta, Batch batch) {
         meta->maxList[0] = meta->maxList[1] = meta->maxList[2] = 0;
         meta->listEntries = meta->rounds = meta->mixed = 0;
         meta->groupNext = 0;
+        meta->ntargets = 0;
     }
     if (b < batch.B) {
         for (int c = 0; c < 3; c++) {
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(1024) void k_collect_scan(int nb, int* __restrict__
             s1 += blockHole[i];
         }
         long long run0 = block_scan_1024(s0, ss, &total);
-        if (tid == 0) meta->nstars = total;
+        if (tid == 0) meta->nstars = meta->ntargets = total;
         long long run1 = block_scan_1024(s1, ss, &total);
         if (tid == 0) meta->nholes = total;
         for (int i = lo; i < hi; i++) {
@@ -685,6 +686,123 @@ __global__ void k_zero_holes(const int32_t* __restrict__ holes, const Meta* __re
 namespace gx {
 
 // ---------------------------------------------------------------------------
+// Active sets.  A force evaluation with an active mask (state order) builds
+// the tree from all stars as usual but walks only the active ones: their
+// sorted indices are compacted, in sorted order, into the target list, and
+// each instance's walk groups are cut from its own run of targets.  Forces of
+// inactive bodies are left as they are.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_active_count(const uint8_t* __restrict__ active,
+                                                      const int32_t* __restrict__ order, const Meta* __restrict__ meta,
+                                                      uint8_t* __restrict__ actS, int* __restrict__ blockAct) {
+    __shared__ int wsum[4];
+    const long long nst = meta->nstars;
+    const int tid = threadIdx.x;
+    constexpr int per = kCollectTile / 256;
+    const long long base = (long long)blockIdx.x * kCollectTile;
+    int cnt = 0;
+    for (int j = 0; j < per; j++) {
+        const long long k = base + (long long)j * 256 + tid;  // coalesced over the tile
+        if (k < nst) {
+            const uint8_t a = active[order[k]] != 0;
+            actS[k] = a;
+            cnt += a;
+        }
+    }
+    cnt = wave_sum_i(cnt);
+    if (lane_id() == 0) wsum[tid / kWave] = cnt;
+    __syncthreads();
+    if (tid == 0) blockAct[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// One workgroup: exclusive scan of the tile counts; the target count.
+__global__ __launch_bounds__(1024) void k_active_scan(int nb, int* __restrict__ blockAct, Meta* meta) {
+    __shared__ long long ss[1024];
+    const int tid = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    const int lo = tid * per, hi = min(nb, lo + per);
+    long long s0 = 0;
+    for (int i = lo; i < hi; i++) s0 += blockAct[i];
+    long long total;
+    long long run = block_scan_1024(s0, ss, &total);
+    if (tid == 0) meta->ntargets = total;
+    for (int i = lo; i < hi; i++) {
+        const int c = blockAct[i];
+        blockAct[i] = (int)run;
+        run += c;
+    }
+}
+
+// One wavefront per instance boundary b in [0, B]: tOff[b] = active stars
+// before sorted position starOff[b] (tile prefix + count inside the tile).
+__global__ __launch_bounds__(256) void k_active_inst(const uint8_t* __restrict__ actS,
+                                                     const int* __restrict__ blockAct, const Meta* __restrict__ meta,
+                                                     Batch batch, long long* __restrict__ tOff) {
+    const int b = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    if (b > batch.B) return;
+    const long long pos = batch.starOff[b];
+    if (pos >= meta->nstars) {
+        if (lane_id() == 0) tOff[b] = meta->ntargets;
+        return;
+    }
+    const long long tile = pos / kCollectTile;
+    int cnt = 0;
+    for (long long k = tile * kCollectTile + lane_id(); k < pos; k += kWave) cnt += actS[k];
+    cnt = wave_sum_i(cnt);
+    if (lane_id() == 0) tOff[b] = blockAct[tile] + cnt;
+}
+
+// One workgroup: walk groups per instance from its target count.
+__global__ __launch_bounds__(1024) void k_active_groups(Batch batch, const long long* __restrict__ tOff) {
+    __shared__ long long ss[1024];
+    const int tid = threadIdx.x;
+    const int B = batch.B;
+    const int per = (B + 1023) / 1024;
+    const int lo = tid * per, hi = min(B, lo + per);
+    long long cg = 0;
+    for (int b = lo; b < hi; b++) cg += (tOff[b + 1] - tOff[b] + batch.gsize - 1) / batch.gsize;
+    long long total;
+    long long rg = block_scan_1024(cg, ss, &total);
+    if (tid == 0) batch.groupOff[B] = total;
+    for (int b = lo; b < hi; b++) {
+        batch.groupOff[b] = rg;
+        rg += (tOff[b + 1] - tOff[b] + batch.gsize - 1) / batch.gsize;
+    }
+}
+
+// Stable compaction of the active sorted positions into the target list.
+__global__ __launch_bounds__(256) void k_active_scatter(const uint8_t* __restrict__ actS,
+                                                        const int* __restrict__ blockAct,
+                                                        const Meta* __restrict__ meta, int32_t* __restrict__ tlist) {
+    __shared__ int ssum[256];
+    const long long nst = meta->nstars;
+    const int tid = threadIdx.x;
+    constexpr int per = kCollectTile / 256;
+    const long long base = (long long)blockIdx.x * kCollectTile + (long long)tid * per;
+    uint32_t amask = 0;
+    for (int j = 0; j < per; j++) {
+        const long long k = base + j;
+        if (k < nst && actS[k]) amask |= 1u << j;
+    }
+    const int c = __popc(amask);
+    ssum[tid] = c;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const int a = tid >= d ? ssum[tid - d] : 0;
+        __syncthreads();
+        ssum[tid] += a;
+        __syncthreads();
+    }
+    int o = blockAct[blockIdx.x] + ssum[tid] - c;
+    for (int j = 0; j < per; j++)
+        if (amask >> j & 1) tlist[o++] = (int32_t)(base + j);
+}
+
+}  // namespace gx
+
+namespace gx {
+
+// ---------------------------------------------------------------------------
 // KDK leapfrog pieces: kick and drift, internal/sim/sim.go:302-343 (the
 // Newtonian branch, c <= 0), over every body as in the reference.
 // ---------------------------------------------------------------------------
@@ -707,6 +825,129 @@ __global__ __launch_bounds__(256) void k_drift(long long n, double* __restrict__
         py[i] += dt * vy[i];
         pz[i] += dt * vz[i];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Block time steps with power-of-two rungs for gravity.  The rung arithmetic
+// is the reference's for gas (internal/sim/blockstep.go:23-45, 81-90): rung r
+// means a step of T/2^r (maxRung = levels - 1 caps it), time is counted in
+// ticks of T/2^maxRung, and a step may start only at a multiple of its own
+// length.  A body is therefore active at tick t exactly when t is a multiple
+// of its step length.  The wanted step of a star is sqrt(2 eta eps / |a|)
+// (the acceleration criterion of GADGET-2, Springel 2005, eq. 34); the
+// reference has no gravity step criterion of its own.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int rung_for(double dt, double T, int maxRung) {  // blockstep.go:37-45
+    int r = 0;
+    double x = T;
+    while (x > dt && r < maxRung) {
+        x *= 0.5;
+        r++;
+    }
+    return r;
+}
+
+// active[i] = body i's step ends at tick now.
+__global__ __launch_bounds__(256) void k_block_active(long long n, const int8_t* __restrict__ rung, int now, int full,
+                                                      uint8_t* __restrict__ active) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        active[i] = now % (full >> rung[i]) == 0;
+}
+
+// At tick now: the closing kick of every body whose step ends there (now > 0),
+// then, before the end of the global step, its new rung (aligned to now, as
+// startStep, blockstep.go:81-90) and the opening kick of the new step.  With
+// now = 0 every body starts a step.  hist[r] counts the bodies on rung r
+// afterwards (all bodies, active or not).
+__global__ __launch_bounds__(256) void k_block_kick(long long n, double* __restrict__ vx, double* __restrict__ vy,
+                                                    double* __restrict__ vz, const double* __restrict__ ax,
+                                                    const double* __restrict__ ay, const double* __restrict__ az,
+                                                    const uint8_t* __restrict__ kind, const uint8_t* __restrict__ alive,
+                                                    int8_t* __restrict__ rung, int now, int full, int maxRung, double T,
+                                                    double crit, unsigned long long* __restrict__ hist) {
+    for (long long i0 = (long long)blockIdx.x * blockDim.x; i0 < n; i0 += (long long)gridDim.x * blockDim.x) {
+        const long long i = i0 + threadIdx.x;
+        int r = 0;
+        if (i < n) {
+            r = rung[i];
+            if (now == 0 || now % (full >> r) == 0) {
+                if (now > 0) {
+                    const double h = 0.5 * (T / (double)(1 << r));
+                    vx[i] += h * ax[i];
+                    vy[i] += h * ay[i];
+                    vz[i] += h * az[i];
+                }
+                if (now < full) {
+                    r = 0;
+                    if (alive[i] && kind[i] != kHole) {
+                        const double x = ax[i], y = ay[i], z = az[i];
+                        const double a = sqrt(x * x + y * y + z * z);
+                        r = rung_for(sqrt(crit / a), T, maxRung);
+                    }
+                    while (now % (full >> r) != 0) r++;
+                    rung[i] = (int8_t)r;
+                    const double h = 0.5 * (T / (double)(1 << r));
+                    vx[i] += h * ax[i];
+                    vy[i] += h * ay[i];
+                    vz[i] += h * az[i];
+                }
+            }
+        }
+        for (int q = 0; q <= maxRung; q++) {
+            const uint64_t m = __ballot(i < n && r == q);
+            if (lane_id() == 0 && m) atomicAdd(&hist[q], (unsigned long long)__popcll(m));
+        }
+    }
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int count, unsigned long long v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 diagnostics, internal/sim/diag.go: per workgroup partial sums, in a
+// fixed order, of
+//   0      kinetic energy of the sources, m u^2 / (1 + 1)  (sourceKinetic :68-77,
+//          kinetic relativity.go:24-26 with c <= 0)
+//   1..3   momentum m v                                  (Momentum :102-109)
+//   4..6   angular momentum x cross (m v)                (AngularMomentum :112-119)
+//   7      mass, 8..10 mass-weighted position            (CenterOfMass :122-134)
+// over every body, as the reference (Alive is not consulted there either).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_diag(long long n, const double* __restrict__ px,
+                                              const double* __restrict__ py, const double* __restrict__ pz,
+                                              const double* __restrict__ vx, const double* __restrict__ vy,
+                                              const double* __restrict__ vz, const double* __restrict__ m,
+                                              const uint8_t* __restrict__ kind, int dmOff,
+                                              double* __restrict__ partial) {
+    __shared__ double red[kDiagValues][256];
+    double acc[kDiagValues] = {};
+    const int tid = threadIdx.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + tid; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double mi = m[i], ux = vx[i], uy = vy[i], uz = vz[i];
+        const double x = px[i], y = py[i], z = pz[i];
+        if (!(dmOff && kind[i] == kDark)) acc[0] += mi * (ux * ux + uy * uy + uz * uz) / (1.0 + 1.0);
+        const double qx = ux * mi, qy = uy * mi, qz = uz * mi;
+        acc[1] += qx;
+        acc[2] += qy;
+        acc[3] += qz;
+        acc[4] += y * qz - z * qy;
+        acc[5] += z * qx - x * qz;
+        acc[6] += x * qy - y * qx;
+        acc[7] += mi;
+        acc[8] += x * mi;
+        acc[9] += y * mi;
+        acc[10] += z * mi;
+    }
+    for (int c = 0; c < kDiagValues; c++) red[c][tid] = acc[c];
+    __syncthreads();
+    for (int d = 128; d > 0; d >>= 1) {
+        if (tid < d)
+            for (int c = 0; c < kDiagValues; c++) red[c][tid] += red[c][tid + d];
+        __syncthreads();
+    }
+    if (tid < kDiagValues) partial[(size_t)blockIdx.x * kDiagValues + tid] = red[tid][0];
 }
 
 }  // namespace gx

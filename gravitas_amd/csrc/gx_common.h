@@ -60,7 +60,15 @@ struct Batch {
     long long* groupOff;            // B+1 offsets of the instances' walk groups
     int* root;                      // B root node, -1 for an instance without stars
     double* half;                   // B half side of the root cell
+    // Targets of the walk: positions [tOff[b], tOff[b+1]) of the target list
+    // belong to instance b, and target t is the sorted star tlist[t].  Without
+    // an active set tlist is null, tOff is starOff and target t is star t.
+    const long long* tOff;
+    const int32_t* tlist;
 };
+
+// Sorted star index of walk target t.
+__device__ __forceinline__ long long target_star(const Batch& b, long long t) { return b.tlist ? b.tlist[t] : t; }
 
 // Order-preserving map of doubles to unsigned integers, so that atomic
 // integer min/max give exact floating-point min/max (-0 orders below +0).
@@ -98,6 +106,7 @@ struct Meta {
     unsigned long long listEntries;  // cell list entries over all groups (FP32 walk)
     unsigned long long rounds;       // FP32 walk traversal rounds over all groups
     unsigned long long mixed;        // nodes decided per star (box test inconclusive)
+    long long ntargets;     // walk targets (the active stars of an active set)
     int flags;              // bit 0: node arena, bit 1: walk stack, bit 2: walk lists overflowed
     int maxStack;
     int maxList[3];         // longest full-cell, partial-cell and leaf list of a group

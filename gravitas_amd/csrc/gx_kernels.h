@@ -34,6 +34,22 @@ __global__ void k_kick(long long n, double* vx, double* vy, double* vz, const do
                        const double* az, double h);
 __global__ void k_drift(long long n, double* px, double* py, double* pz, const double* vx, const double* vy,
                         const double* vz, double dt);
+__global__ void k_active_count(const uint8_t* active, const int32_t* order, const Meta* meta, uint8_t* actS,
+                               int* blockAct);
+__global__ void k_active_scan(int nb, int* blockAct, Meta* meta);
+__global__ void k_active_inst(const uint8_t* actS, const int* blockAct, const Meta* meta, Batch batch, long long* tOff);
+__global__ void k_active_groups(Batch batch, const long long* tOff);
+__global__ void k_active_scatter(const uint8_t* actS, const int* blockAct, const Meta* meta, int32_t* tlist);
+__global__ void k_block_active(long long n, const int8_t* rung, int now, int full, uint8_t* active);
+__global__ void k_block_kick(long long n, double* vx, double* vy, double* vz, const double* ax, const double* ay,
+                             const double* az, const uint8_t* kind, const uint8_t* alive, int8_t* rung, int now,
+                             int full, int maxRung, double T, double crit, unsigned long long* hist);
+__global__ void k_fill_u64(unsigned long long* p, int count, unsigned long long v);
+constexpr int kDiagValues = 11;
+constexpr int kDiagBlocks = 1024;
+__global__ void k_diag(long long n, const double* px, const double* py, const double* pz, const double* vx,
+                       const double* vy, const double* vz, const double* m, const uint8_t* kind, int dmOff,
+                       double* partial);
 
 // gx_walk64.hip
 __global__ void k_walk64(const double4* spos, const TravNode* trav, const Mult64* m64, const int32_t* order,

@@ -103,8 +103,9 @@ __device__ __forceinline__ void eval_parts(const float4* __restrict__ part, cons
 
 
 // Group prologue shared by both kernels: the group's stars (up to gsize
-// consecutive stars of one instance in Morton order), FP64 bounding box and
-// centre c_g, the instance's root, softening and root half side.
+// consecutive targets of one instance in Morton order: all its stars, or the
+// active ones of an active set), FP64 bounding box and centre c_g, the
+// instance's root, softening and root half side.
 struct Group {
     long long g, k;
     int inst, root;
@@ -121,14 +122,14 @@ __device__ __forceinline__ bool load_group(Group& G, const double4* __restrict__
     if (g >= batch.groupOff[batch.B]) return false;
     G.g = g;
     G.inst = upper_index(batch.groupOff, batch.B, g);
-    const long long base = batch.starOff[G.inst] + (g - batch.groupOff[G.inst]) * batch.gsize;
-    const long long count = min((long long)batch.gsize, batch.starOff[G.inst + 1] - base);
+    const long long base = batch.tOff[G.inst] + (g - batch.groupOff[G.inst]) * batch.gsize;
+    const long long count = min((long long)batch.gsize, batch.tOff[G.inst + 1] - base);
     G.root = batch.root[G.inst];
     G.eps2 = (float)batch.eps2[G.inst];
     G.half = batch.half[G.inst];
-    G.k = base + lane_id();
     G.valid = lane_id() < count;
-    G.p = spos[G.valid ? G.k : base];
+    G.k = target_star(batch, G.valid ? base + lane_id() : base);
+    G.p = spos[G.k];
     G.gmask = __ballot(G.valid);
     G.bminx = wave_min_d(G.valid ? G.p.x : INFINITY);
     G.bmaxx = wave_max_d(G.valid ? G.p.x : -INFINITY);
@@ -144,9 +145,14 @@ __device__ __forceinline__ bool load_group(Group& G, const double4* __restrict__
 
 // Group of this workgroup: workgroups are dealt round-robin to the 8 XCDs, so
 // each XCD gets a contiguous run of (spatially adjacent) groups for its L2.
-__device__ __forceinline__ long long xcd_group() {
-    const long long per = gridDim.x / 8;
-    return (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+// The grid is an upper bound; the run length comes from the actual group
+// count, so a small active set still spreads over all XCDs.  Workgroups past
+// 8 runs get a group index past the end and exit.
+__device__ __forceinline__ long long xcd_group(const Batch& batch) {
+    const long long ng = batch.groupOff[batch.B];
+    const long long per = (ng + 7) / 8;
+    const long long j = blockIdx.x / 8;
+    return j < per ? (long long)(blockIdx.x % 8) * per + j : ng;
 }
 
 // ---------------------------------------------------------------------------
@@ -482,7 +488,7 @@ __global__ __launch_bounds__(64) void k_trav32(const double4* __restrict__ spos,
     __shared__ TravLDS S;
     if (meta->flags & kFlagNodeOverflow) return;
     Group G;
-    if (!load_group(G, spos, batch, xcd_group())) return;
+    if (!load_group(G, spos, batch, xcd_group(batch))) return;
     trav_group(G, S, trav, meta, lists);
 }
 
@@ -495,7 +501,7 @@ __global__ __launch_bounds__(64) void k_eval32(const double4* __restrict__ spos,
     __shared__ EvalLDS S;
     if (meta->flags & (kFlagNodeOverflow | kFlagStackOverflow | kFlagListOverflow)) return;
     Group G;
-    if (!load_group(G, spos, batch, xcd_group())) return;
+    if (!load_group(G, spos, batch, xcd_group(batch))) return;
     eval_group<QUAD, EPS0, Q16>(G, S, spos, trav, m32, m16, order, lists, oax, oay, oaz);
 }
 
@@ -513,7 +519,7 @@ __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos,
                                                double* __restrict__ oay, double* __restrict__ oaz) {
     __shared__ WalkLDS S;
     if (meta->flags & kFlagNodeOverflow) return;
-    long long g = persistent ? -1 : xcd_group();
+    long long g = persistent ? -1 : xcd_group(batch);
     for (;;) {
         if (persistent) {
             unsigned long long next = 0;

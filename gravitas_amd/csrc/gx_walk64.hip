@@ -14,12 +14,13 @@ __global__ __launch_bounds__(256) void k_walk64(const double4* __restrict__ spos
                                                 const Mult64* __restrict__ m64, const int32_t* __restrict__ order,
                                                 Meta* meta, Batch batch, int quad, double* __restrict__ ax,
                                                 double* __restrict__ ay, double* __restrict__ az) {
-    const long long nst = meta->nstars;
+    const long long nt = batch.tOff[batch.B];
     if (meta->flags & kFlagNodeOverflow) return;
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long npp = 0, npc = 0;
-    if (k < nst) {
-        const int b = upper_index(batch.starOff, batch.B, k);
+    if (t < nt) {
+        const long long k = target_star(batch, t);
+        const int b = upper_index(batch.tOff, batch.B, t);
         const double eps2 = batch.eps2[b];
         const double4 p = spos[k];
         const double x = p.x, y = p.y, z = p.z;

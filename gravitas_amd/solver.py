@@ -221,6 +221,44 @@ class Solver:
             _ptr(s.kind), _ptr(s.alive), len(s), float(p.softening), int(p.dark_matter_off), float(p.dt),
             int(bool(acc_valid)), _ptr(s.ax), _ptr(s.ay), _ptr(s.az)))
 
+    # -- active sets and block steps ----------------------------------------------------
+    def accelerations_active(self, s: DeviceState, p: Params, active):
+        """Accelerations of the bodies with active[i] != 0 only (active: uint8
+        DeviceArray or host array in state order); the others keep theirs."""
+        self._configure()
+        if not isinstance(active, DeviceArray):
+            active = DeviceArray.from_numpy(np.ascontiguousarray(active, dtype=np.uint8))
+        _lib.check(self._L.gx_accelerations_active_dev(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.m), _ptr(s.kind), _ptr(s.alive), _ptr(active),
+            len(s), float(p.softening), int(p.dark_matter_off), _ptr(s.ax), _ptr(s.ay), _ptr(s.az)))
+
+    def block_step(self, s: DeviceState, p: Params, T: float, levels: int = 4, eta: float = 0.025,
+                   acc_valid: bool = True) -> dict:
+        """One global step of length T with power-of-two block time steps
+        (rungs 0..levels-1); returns the rung populations and active counts."""
+        self._configure()
+        st = _lib.GxBlockStats()
+        _lib.check(self._L.gx_block_step_dev(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.vx), _ptr(s.vy), _ptr(s.vz), _ptr(s.m),
+            _ptr(s.kind), _ptr(s.alive), len(s), float(p.softening), int(p.dark_matter_off), float(T), int(levels),
+            float(eta), int(bool(acc_valid)), _ptr(s.ax), _ptr(s.ay), _ptr(s.az), ctypes.byref(st)))
+        ev = min(st.events, _lib.GX_BLOCK_MAX_TICKS)
+        return {"levels": st.levels, "events": st.events, "bodies": st.bodies, "targets": st.targets,
+                "rung_count": list(st.rung_count)[:st.levels], "tick": list(st.tick)[:ev],
+                "active": list(st.active)[:ev]}
+
+    def diagnostics(self, s: DeviceState, p: Params) -> dict:
+        """FP64 kinetic energy of the sources, momentum, angular momentum,
+        mass and centre of mass (internal/sim/diag.go) summed on the GPU."""
+        out = (ctypes.c_double * _lib.GX_DIAG_VALUES)()
+        _lib.check(self._L.gx_diagnostics_dev(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.vx), _ptr(s.vy), _ptr(s.vz), _ptr(s.m),
+            _ptr(s.kind), len(s), int(p.dark_matter_off), out))
+        v = list(out)
+        m = v[7]
+        com = [c * (1 / m) for c in v[8:11]] if m != 0 else [0.0, 0.0, 0.0]  # c.Scale(1 / m)
+        return {"kinetic": v[0], "momentum": v[1:4], "angular_momentum": v[4:7], "mass": m, "com": com}
+
     def host_state(self, s: DeviceState) -> dict:
         return {f: getattr(s, f).numpy() for f in DeviceState.FIELDS}
 

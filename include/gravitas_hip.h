@@ -148,6 +148,57 @@ int gx_graph_capture_step(gx_solver *s, double *x, double *y, double *z, double 
                           double *ax, double *ay, double *az);
 int gx_graph_launch(gx_solver *s, int32_t steps);
 
+/* Force step for an active set: as gx_accelerations_dev, but only bodies with
+ * active[i] != 0 (device array, state order) get new accelerations; the tree
+ * is still built from every alive star.  Each active star gets exactly the
+ * value a full call would give it; the others keep theirs.  The reference's
+ * counterpart is Hydro.Forces' active mask for gas (blockstep.go:166-172). */
+int gx_accelerations_active_dev(gx_solver *s, const double *x, const double *y, const double *z,
+                                const double *mass, const uint8_t *kind, const uint8_t *alive,
+                                const uint8_t *active, int64_t n, double eps, int dark_matter_off, double *ax,
+                                double *ay, double *az);
+
+#define GX_BLOCK_MAX_LEVELS 8
+#define GX_BLOCK_MAX_TICKS 128
+
+typedef struct {
+    int32_t levels;  /* rungs 0 .. levels-1 */
+    int32_t events;  /* force evaluations in the step */
+    int64_t bodies;  /* n */
+    int64_t targets; /* stars evaluated, summed over the events */
+    int64_t rung_count[GX_BLOCK_MAX_LEVELS]; /* bodies per rung at the start of the step */
+    int32_t tick[GX_BLOCK_MAX_TICKS];        /* tick of each event (a tick is T / 2^(levels-1)) */
+    int64_t active[GX_BLOCK_MAX_TICKS];      /* stars evaluated at each event */
+} gx_block_stats;
+
+/* One global step of length T with individual power-of-two time steps
+ * (block steps), on device arrays.  The rung rules are those of the
+ * reference's gas steps (internal/sim/blockstep.go:23-45, 81-90, event loop
+ * :130-198): rung r steps by T/2^r, r <= levels-1, steps start only at
+ * multiples of their length, every body drifts from event to event and the
+ * bodies whose step ends are evaluated (gx_accelerations_active_dev), kicked
+ * and given a new step.  A star's wanted step is sqrt(2 eta eps / |a|) (the
+ * reference has no gravity step criterion); holes and dead bodies stay on
+ * rung 0.  If acc_valid is 0 the forces are computed first.  The step ends
+ * with every body synchronised and valid accelerations.  out (optional)
+ * receives the rung populations and the active counts.  Synchronous. */
+int gx_block_step_dev(gx_solver *s, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                      const double *mass, const uint8_t *kind, const uint8_t *alive, int64_t n, double eps,
+                      int dark_matter_off, double T, int32_t levels, double eta, int acc_valid, double *ax,
+                      double *ay, double *az, gx_block_stats *out);
+
+#define GX_DIAG_VALUES 11
+
+/* FP64 diagnostics of internal/sim/diag.go on device arrays, summed on the
+ * GPU in a fixed order: out[0] kinetic energy of the sources (sourceKinetic,
+ * :68-77, c <= 0), out[1..3] momentum (Momentum :102-109), out[4..6] angular
+ * momentum about the origin (AngularMomentum :112-119), out[7] total mass and
+ * out[8..10] the mass-weighted position sum (CenterOfMass :122-134 before its
+ * division).  Synchronous. */
+int gx_diagnostics_dev(gx_solver *s, const double *x, const double *y, const double *z, const double *vx,
+                       const double *vy, const double *vz, const double *mass, const uint8_t *kind, int64_t n,
+                       int dark_matter_off, double *out);
+
 /* bh.Solver.PotentialEnergy (potential.go:12-42) on device arrays: theta
  * capped at 0.9; FP64 tree walk.  Star-star pairs only: returns
  * GX_ERR_UNSUPPORTED if alive holes are present. */
@@ -175,6 +226,7 @@ typedef struct {
     int64_t rounds;          /* FP32 walk: traversal rounds over all groups */
     int64_t mixed_nodes;     /* FP32 walk: nodes decided star by star */
     int32_t max_list[4];     /* FP32 walk: longest full-cell, partial-cell, leaf list; unused */
+    int64_t n_targets;       /* stars whose forces were computed (the active ones of an active set) */
 } gx_stats;
 
 /* Statistics of the last completed step. */

@@ -34,6 +34,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 _D = ctypes.POINTER(ctypes.c_double)
+_I8 = ctypes.POINTER(ctypes.c_int8)
 _U8 = ctypes.POINTER(ctypes.c_uint8)
 _I64 = ctypes.POINTER(ctypes.c_int64)
 
@@ -102,6 +103,12 @@ def lib():
         L.or_node_size.restype = ctypes.c_int
         L.or_kick.argtypes = [ctypes.c_int64, _D, _D, _D, _D, _D, _D, ctypes.c_double]
         L.or_drift.argtypes = [ctypes.c_int64, _D, _D, _D, _D, _D, _D, ctypes.c_double]
+        L.or_rung_for.restype = ctypes.c_int
+        L.or_rung_for.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.or_block_active.argtypes = [ctypes.c_int64, _I8, ctypes.c_int, ctypes.c_int, _U8]
+        L.or_block_kick.argtypes = [ctypes.c_int64, _D, _D, _D, _D, _D, _D, _U8, _U8, _I8, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.or_diagnostics.argtypes = [ctypes.c_int64, _D, _D, _D, _D, _D, _D, _D, _U8, ctypes.c_int, _D]
         assert L.or_node_size() == NODE_DTYPE.itemsize
         _lib = L
     return _lib
@@ -288,3 +295,72 @@ def kick_drift_kick(state, acc_fn, dt):
     L.or_kick(n, _dp(state["vx"]), _dp(state["vy"]), _dp(state["vz"]), _dp(ax), _dp(ay), _dp(az), 0.5 * dt)
     state["acc"] = a
     return state
+
+
+def rung_for(dt: float, T: float, max_rung: int) -> int:
+    """rungFor, internal/sim/blockstep.go:37-45."""
+    return lib().or_rung_for(float(dt), float(T), int(max_rung))
+
+
+def block_step(state, acc_fn, T, levels, eta, eps, kind, alive):
+    """One global step T with power-of-two block steps, the event loop of
+    internal/sim/blockstep.go:130-198 applied to gravity (or_block_kick holds
+    the per-body rules).  ``state`` holds float64 arrays px..vz and 'acc' (the
+    forces of the current positions); ``acc_fn(px, py, pz)`` returns the
+    (n, 3) forces of all bodies, of which only the active stars' rows are
+    taken (an active star's force does not depend on which others are active).
+    Returns the rung populations at the start and the events (tick, active
+    stars)."""
+    L = lib()
+    n = len(state["px"])
+    max_rung = levels - 1
+    full = 1 << max_rung
+    unit = T / full
+    crit = 2 * eta * eps
+    kind = np.ascontiguousarray(kind, np.uint8)
+    alive = np.ascontiguousarray(alive, np.uint8)
+    star = (alive == 1) & (kind != 1)
+    hole = (alive == 1) & (kind == 1)
+    rung = np.zeros(n, np.int8)
+    active = np.zeros(n, np.uint8)
+    i8 = lambda a: a.ctypes.data_as(_I8)
+
+    def kick(now):
+        acc = state["acc"]
+        ax, ay, az = (np.ascontiguousarray(acc[:, c]) for c in range(3))
+        L.or_block_kick(n, _dp(state["vx"]), _dp(state["vy"]), _dp(state["vz"]), _dp(ax), _dp(ay), _dp(az),
+                        _u8(kind), _u8(alive), i8(rung), now, full, max_rung, float(T), float(crit))
+
+    kick(0)
+    rungs = np.bincount(rung, minlength=levels)[:levels].tolist()
+    events = []
+    now = 0
+    while now < full:
+        nxt = full
+        for r in np.unique(rung):
+            ln = full >> int(r)
+            nxt = min(nxt, (now // ln + 1) * ln)
+        L.or_drift(n, _dp(state["px"]), _dp(state["py"]), _dp(state["pz"]), _dp(state["vx"]), _dp(state["vy"]),
+                   _dp(state["vz"]), float(nxt - now) * unit)
+        now = nxt
+        L.or_block_active(n, i8(rung), now, full, _u8(active))
+        sel = (active == 1) & star
+        a = acc_fn(state["px"], state["py"], state["pz"])
+        acc = state["acc"].copy()
+        acc[sel] = a[sel]
+        acc[hole] = 0.0
+        state["acc"] = acc
+        events.append((now, int(sel.sum())))
+        kick(now)
+    return {"rung_count": rungs, "events": events}
+
+
+def diagnostics(px, py, pz, vx, vy, vz, m, kind, dm_off=False):
+    """internal/sim/diag.go sums in state order: kinetic, momentum (3),
+    angular momentum (3), mass, sum m x (3)."""
+    out = np.zeros(11)
+    f = lambda a: np.ascontiguousarray(a, np.float64)
+    arrs = [f(a) for a in (px, py, pz, vx, vy, vz, m)]
+    k = np.ascontiguousarray(kind, np.uint8)
+    lib().or_diagnostics(len(arrs[0]), *[_dp(a) for a in arrs], _u8(k), int(dm_off), _dp(out))
+    return out

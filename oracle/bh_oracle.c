@@ -652,6 +652,86 @@ double or_potential_exact(const double *px, const double *py, const double *pz, 
 }
 
 /* ------------------------------------------------------------------ */
+/* Block time steps (gravity use of the gas rung scheme)                */
+/* ------------------------------------------------------------------ */
+/* rungFor, internal/sim/blockstep.go:37-45: the smallest r with T/2^r <= dt,
+ * capped at max_rung. */
+int or_rung_for(double dt, double T, int max_rung) {
+    int r = 0;
+    double x = T;
+    while (x > dt && r < max_rung) {
+        x *= 0.5;
+        r++;
+    }
+    return r;
+}
+
+/* Bodies whose step ends at tick `now` (a step of rung r spans full >> r
+ * ticks and starts at a multiple of that, blockstep.go:81-90). */
+void or_block_active(int64_t n, const int8_t *rung, int now, int full, uint8_t *active) {
+    for (int64_t i = 0; i < n; i++) active[i] = now % (full >> rung[i]) == 0;
+}
+
+/* Kicks at tick `now` of a global step T: the closing kick of each body whose
+ * step ends (now > 0; kickGas(i, 0.5*step), blockstep.go:187-189), then, if
+ * now < full, a new rung from the wanted step sqrt(crit / |a|) aligned to now
+ * (startStep, :81-90) and its opening kick.  now = 0 starts every body.
+ * Holes and dead bodies stay on rung 0.  crit = 2 eta eps (GADGET-2's
+ * criterion; the reference has no gravity step criterion). */
+void or_block_kick(int64_t n, double *vx, double *vy, double *vz, const double *ax, const double *ay,
+                   const double *az, const uint8_t *kind, const uint8_t *alive, int8_t *rung, int now, int full,
+                   int max_rung, double T, double crit) {
+    for (int64_t i = 0; i < n; i++) {
+        int r = rung[i];
+        if (now > 0 && now % (full >> r) != 0) continue;
+        if (now > 0) {
+            double h = 0.5 * (T / (double)(1 << r));
+            vx[i] += h * ax[i];
+            vy[i] += h * ay[i];
+            vz[i] += h * az[i];
+        }
+        if (now >= full) continue;
+        r = 0;
+        if (alive[i] && kind[i] != KIND_HOLE) {
+            double x = ax[i], y = ay[i], z = az[i];
+            double a = sqrt(x * x + y * y + z * z);
+            r = or_rung_for(sqrt(crit / a), T, max_rung);
+        }
+        while (now % (full >> r) != 0) r++;
+        rung[i] = (int8_t)r;
+        double h = 0.5 * (T / (double)(1 << r));
+        vx[i] += h * ax[i];
+        vy[i] += h * ay[i];
+        vz[i] += h * az[i];
+    }
+}
+
+/* Diagnostics, internal/sim/diag.go, summed in state order as there:
+ * out[0] sourceKinetic (:68-77, kinetic with c <= 0: m u^2 / (1 + 1)),
+ * out[1..3] Momentum (:102-109), out[4..6] AngularMomentum (:112-119, x cross
+ * (m v)), out[7] total mass and out[8..10] sum m x (CenterOfMass :122-134). */
+void or_diagnostics(int64_t n, const double *px, const double *py, const double *pz, const double *vx,
+                    const double *vy, const double *vz, const double *mass, const uint8_t *kind, int dm_off,
+                    double *out) {
+    for (int c = 0; c < 11; c++) out[c] = 0;
+    for (int64_t i = 0; i < n; i++) {
+        double m = mass[i], ux = vx[i], uy = vy[i], uz = vz[i], x = px[i], y = py[i], z = pz[i];
+        if (!(kind[i] == KIND_DARK && dm_off)) out[0] += m * (ux * ux + uy * uy + uz * uz) / (1.0 + 1.0);
+        double qx = ux * m, qy = uy * m, qz = uz * m;
+        out[1] += qx;
+        out[2] += qy;
+        out[3] += qz;
+        out[4] += y * qz - z * qy;
+        out[5] += z * qx - x * qz;
+        out[6] += x * qy - y * qx;
+        out[7] += m;
+        out[8] += x * m;
+        out[9] += y * m;
+        out[10] += z * m;
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* Accessors for the Python wrapper                                     */
 /* ------------------------------------------------------------------ */
 int64_t or_n_stars(const or_solver *s) { return s->n_stars; }

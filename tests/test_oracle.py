@@ -321,3 +321,119 @@ def test_kepler_orbit_conserves_energy():
             drift = max(drift, abs((energy() - e0) / e0))
     assert drift <= 1e-4
     assert abs(math.hypot(st["px"][1], st["py"][1]) - 0.7) <= 0.05
+
+
+# -- block time steps (gravity use of internal/sim/blockstep.go's rungs) ---------------
+
+def test_rung_for_known_answers():
+    """rungFor (blockstep.go:37-45): smallest r with T/2^r <= dt, capped."""
+    T = 0.25
+    assert [oracle.rung_for(dt, T, 3) for dt in (T, 0.5 * T, 0.3 * T, 0.25 * T, 0.2 * T, 1e-9)] == [0, 1, 2, 2, 3, 3]
+    assert oracle.rung_for(1e-9, T, 10) == 10
+    assert oracle.rung_for(math.inf, T, 3) == 0 and oracle.rung_for(math.nan, T, 3) == 0
+    assert oracle.rung_for(2 * T, T, 3) == 0
+
+
+def _block_setup(n=1500, seed=11):
+    s = ics.plummer(n, seed)
+    s["alive"] = np.ones(n, np.uint8)
+    solver = oracle.Solver(theta=0.5, workers=8)
+    eps = 0.01
+    acc = lambda x, y, z: solver.accelerations(x, y, z, s["m"], s["kind"], s["alive"], eps=eps)
+    st = {k: s[k].copy() for k in ("px", "py", "pz", "vx", "vy", "vz")}
+    st["acc"] = acc(st["px"], st["py"], st["pz"])
+    return s, st, acc, eps
+
+
+def test_block_step_single_rung_limits_are_kdk_steps():
+    """All bodies on rung 0 is one KDK step of T; all on the top rung (a tiny
+    eta) is 2^(levels-1) KDK steps of T/2^(levels-1), bit for bit."""
+    s, st0, acc, eps = _block_setup()
+    T = 0.01
+    a = {k: v.copy() for k, v in st0.items()}
+    info = oracle.block_step(a, acc, T, 4, 1e30, eps, s["kind"], s["alive"])  # huge eta: rung 0
+    assert info["rung_count"] == [len(s["px"]), 0, 0, 0] and info["events"] == [(8, len(s["px"]))]
+    b = {k: v.copy() for k, v in st0.items()}
+    oracle.kick_drift_kick(b, acc, T)
+    for k in ("px", "py", "vz", "acc"):
+        assert np.array_equal(a[k], b[k]), k
+    a = {k: v.copy() for k, v in st0.items()}
+    info = oracle.block_step(a, acc, T, 4, 1e-30, eps, s["kind"], s["alive"])  # tiny eta: rung 3
+    assert info["rung_count"] == [0, 0, 0, len(s["px"])] and [t for t, _ in info["events"]] == list(range(1, 9))
+    b = {k: v.copy() for k, v in st0.items()}
+    for _ in range(8):
+        oracle.kick_drift_kick(b, acc, T / 8)
+    for k in ("px", "py", "vz", "acc"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_block_step_mixed_rungs_events_and_alignment():
+    """Rungs from the acceleration criterion: events are exactly the ticks
+    where a populated rung's step ends, and the active count at tick t is the
+    number of stars whose rung divides t into whole steps."""
+    s, st, acc, eps = _block_setup(2000, 12)
+    a = np.linalg.norm(st["acc"], axis=1)
+    eta = 0.025
+    dt = np.sqrt(2 * eta * eps / a)
+    T = 4 * float(np.quantile(dt, 0.05))  # about 5 % of the stars on the top rung
+    info = oracle.block_step(st, acc, T, 4, eta, eps, s["kind"], s["alive"])
+    rc = info["rung_count"]
+    assert sum(rc) == len(s["px"]) and rc[0] > 0 and rc[1] > 0 and rc[3] > 0
+    want = [min(3, oracle.rung_for(d, T, 3)) for d in dt]
+    assert np.bincount(want, minlength=4).tolist() == rc
+    ticks = [t for t, _ in info["events"]]
+    assert ticks == list(range(1, 9))  # rung 3 is populated: every tick is an event
+    # tick 8 evaluates everything; tick 4 at least every body that started on rungs 1-3
+    assert info["events"][-1][1] == len(s["px"])
+    assert info["events"][3][1] >= rc[1] + rc[2] + rc[3]
+
+
+def test_block_steps_conserve_energy_better_than_one_level():
+    """Eccentric Kepler orbit (e = 0.9): with 4 rungs the light body takes
+    short steps through pericentre and the energy error at the same global
+    step T falls far below that of the single-level KDK."""
+    e = 0.9
+    m1, m2 = 0.999, 0.001
+    rp = 1 - e
+    vp = math.sqrt((1 + e) / rp)
+    m = np.array([m1, m2])
+    kind = np.zeros(2, np.uint8)
+    alive = np.ones(2, np.uint8)
+    acc = lambda x, y, z: oracle.direct(x, y, z, m, eps=0.0)
+    eps_crit = 1e-3  # only enters the step criterion here
+
+    def run(levels):
+        st = {"px": np.array([-rp * m2, rp * m1]), "py": np.zeros(2), "pz": np.zeros(2),
+              "vx": np.zeros(2), "vy": np.array([-vp * m2, vp * m1]), "vz": np.zeros(2)}
+        st["acc"] = acc(st["px"], st["py"], st["pz"])
+
+        def energy():
+            k = 0.5 * (m * (st["vx"] ** 2 + st["vy"] ** 2 + st["vz"] ** 2)).sum()
+            return k + oracle.potential_exact(st["px"], st["py"], st["pz"], m, eps=0.0)
+
+        e0, worst = energy(), 0.0
+        for _ in range(400):  # about 2 orbits at T = 2 pi / 200
+            oracle.block_step(st, acc, 2 * math.pi / 200, levels, 0.02, eps_crit, kind, alive)
+            worst = max(worst, abs((energy() - e0) / e0))
+        return worst
+
+    one, four = run(1), run(4)
+    assert four < one / 10, (one, four)
+
+
+def test_diagnostics_match_exact_sums():
+    """diag.go sums against math.fsum, and a two-body known answer."""
+    s = ics.plummer(3000, 13)
+    s["kind"][:100] = ics.KIND_DARK
+    d = oracle.diagnostics(s["px"], s["py"], s["pz"], s["vx"], s["vy"], s["vz"], s["m"], s["kind"], dm_off=True)
+    m, v2 = s["m"], s["vx"] ** 2 + s["vy"] ** 2 + s["vz"] ** 2
+    src = s["kind"] != ics.KIND_DARK
+    assert d[0] == pytest.approx(math.fsum(0.5 * m[src] * v2[src]), rel=1e-13)
+    L = np.cross(np.stack([s["px"], s["py"], s["pz"]], 1), m[:, None] * np.stack([s["vx"], s["vy"], s["vz"]], 1))
+    for c in range(3):
+        assert d[4 + c] == pytest.approx(math.fsum(L[:, c]), rel=1e-10, abs=1e-15)
+    assert d[7] == pytest.approx(math.fsum(m), rel=1e-14)
+    two = oracle.diagnostics([1.0, 0.0], [0.0, 2.0], [0.0, 0.0], [0.0, 3.0], [1.0, 0.0], [0.0, 0.0],
+                             [2.0, 1.0], [0, 0])
+    # K = (2*1 + 1*9)/2; p = (3, 2, 0); L = (1,0,0) x (0,2,0) + (0,2,0) x (3,0,0) = (0,0,2) + (0,0,-6)
+    assert two.tolist() == [5.5, 3.0, 2.0, 0.0, 0.0, 0.0, -4.0, 3.0, 2.0, 2.0, 0.0]
