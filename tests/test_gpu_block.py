@@ -140,16 +140,29 @@ def test_fp32_block_steps_follow_the_oracle(gpu, encounter):
         assert dx <= 1e-6 and dv <= 1e-3
 
 
-def test_diagnostics_match_oracle(gpu, encounter):
+def test_diagnostics_match_exact_sums(gpu, encounter):
+    """The GPU's fixed-order tree sums against math.fsum of the same terms
+    (the terms are those of the oracle's diag.go restatement, which sums them
+    sequentially in state order)."""
+    import math
     s = encounter
     ds = DeviceState.from_host(s)
     sv = Solver(theta=THETA)
+    m = s["m"]
+    pos = np.stack([s["px"], s["py"], s["pz"]], 1)
+    mv = np.stack([s["vx"] * m, s["vy"] * m, s["vz"] * m], 1)
+    L = np.stack([pos[:, 1] * mv[:, 2] - pos[:, 2] * mv[:, 1], pos[:, 2] * mv[:, 0] - pos[:, 0] * mv[:, 2],
+                  pos[:, 0] * mv[:, 1] - pos[:, 1] * mv[:, 0]], 1)
     for dm_off in (False, True):
         d = sv.diagnostics(ds, Params(softening=EPS, dark_matter_off=dm_off))
-        w = oracle.diagnostics(s["px"], s["py"], s["pz"], s["vx"], s["vy"], s["vz"], s["m"], s["kind"], dm_off)
-        assert d["kinetic"] == pytest.approx(w[0], rel=1e-12)
-        scale = np.abs(s["m"] * np.abs(s["vx"])).sum()  # momentum sums cancel: compare to the gross sum
-        for c in range(3):
-            assert abs(d["momentum"][c] - w[1 + c]) <= 1e-12 * scale
-        assert d["mass"] == pytest.approx(w[7], rel=1e-13)
-        assert np.allclose(d["angular_momentum"], w[4:7], rtol=1e-11, atol=1e-12 * scale * 50)
+        w = oracle.diagnostics(s["px"], s["py"], s["pz"], s["vx"], s["vy"], s["vz"], m, s["kind"], dm_off)
+        src = ~((s["kind"] == ics.KIND_DARK) & dm_off)
+        kin = m * (s["vx"] ** 2 + s["vy"] ** 2 + s["vz"] ** 2) / 2
+        terms = [kin[src]] + [mv[:, c] for c in range(3)] + [L[:, c] for c in range(3)] + [m] + \
+                [pos[:, c] * m for c in range(3)]
+        got = [d["kinetic"], *d["momentum"], *d["angular_momentum"], d["mass"]]
+        for c, t in enumerate(terms[:8]):
+            exact, gross = math.fsum(t), np.abs(t).sum()
+            assert abs(got[c] - exact) <= 1e-13 * gross, (c, got[c], exact)
+            assert abs(w[c] - exact) <= 1e-11 * gross, (c, w[c], exact)  # the oracle's sequential sum
+        assert d["com"] == pytest.approx([math.fsum(t) / math.fsum(m) for t in terms[8:]], rel=1e-12, abs=1e-12)
