@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-errors", action="store_true", help="skip the error measurements")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also launch 3 drifts with dt = 0 over the resident bodies (k_drift reads 48 and writes "
+                         "24 bytes per body): the known-size kernel that scales HBM counters")
     return ap.parse_args()
 
 
@@ -156,6 +159,11 @@ def bench_disk_halo(args):
     log(f"timing {args.steps} steps after {args.warmup} warm-up")
     ms, phases, st = timed_steps(solver, ds, params, args.steps, args.warmup)
     acc32 = ds.acc_host()
+    if args.calibrate:
+        from gravitas_amd.hip import synchronize
+        for _ in range(3):
+            solver.drift(ds, 0.0)
+        synchronize()
     out = base_line(args, n, ms, "fp32", {
         "workload": "disk-halo-1m", "n": n, "theta": theta, "eps": eps,
         "model": "70% exponential disk (R_d 1, z0 0.1, Q 1.5) + 30% Hernquist halo (a 5, 5x disk mass)",

@@ -865,38 +865,47 @@ __global__ __launch_bounds__(256) void k_block_kick(long long n, double* __restr
                                                     const uint8_t* __restrict__ kind, const uint8_t* __restrict__ alive,
                                                     int8_t* __restrict__ rung, int now, int full, int maxRung, double T,
                                                     double crit, unsigned long long* __restrict__ hist) {
-    for (long long i0 = (long long)blockIdx.x * blockDim.x; i0 < n; i0 += (long long)gridDim.x * blockDim.x) {
-        const long long i = i0 + threadIdx.x;
-        int r = 0;
-        if (i < n) {
-            r = rung[i];
-            if (now == 0 || now % (full >> r) == 0) {
-                if (now > 0) {
-                    const double h = 0.5 * (T / (double)(1 << r));
-                    vx[i] += h * ax[i];
-                    vy[i] += h * ay[i];
-                    vz[i] += h * az[i];
+    __shared__ unsigned int bh[4][8];
+    unsigned int cnt[8] = {};  // rung populations of this thread's bodies (maxRung <= 7)
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        int r = rung[i];
+        if (now == 0 || now % (full >> r) == 0) {
+            if (now > 0) {
+                const double h = 0.5 * (T / (double)(1 << r));
+                vx[i] += h * ax[i];
+                vy[i] += h * ay[i];
+                vz[i] += h * az[i];
+            }
+            if (now < full) {
+                r = 0;
+                if (alive[i] && kind[i] != kHole) {
+                    const double x = ax[i], y = ay[i], z = az[i];
+                    const double a = sqrt(x * x + y * y + z * z);
+                    r = rung_for(sqrt(crit / a), T, maxRung);
                 }
-                if (now < full) {
-                    r = 0;
-                    if (alive[i] && kind[i] != kHole) {
-                        const double x = ax[i], y = ay[i], z = az[i];
-                        const double a = sqrt(x * x + y * y + z * z);
-                        r = rung_for(sqrt(crit / a), T, maxRung);
-                    }
-                    while (now % (full >> r) != 0) r++;
-                    rung[i] = (int8_t)r;
-                    const double h = 0.5 * (T / (double)(1 << r));
-                    vx[i] += h * ax[i];
-                    vy[i] += h * ay[i];
-                    vz[i] += h * az[i];
-                }
+                while (now % (full >> r) != 0) r++;
+                rung[i] = (int8_t)r;
+                const double h = 0.5 * (T / (double)(1 << r));
+                vx[i] += h * ax[i];
+                vy[i] += h * ay[i];
+                vz[i] += h * az[i];
             }
         }
-        for (int q = 0; q <= maxRung; q++) {
-            const uint64_t m = __ballot(i < n && r == q);
-            if (lane_id() == 0 && m) atomicAdd(&hist[q], (unsigned long long)__popcll(m));
-        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) cnt[q] += r == q;
+    }
+    // block totals, then one atomic per populated rung and workgroup
+    const int w = threadIdx.x / kWave;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int c = wave_sum_i((int)cnt[q]);
+        if (lane_id() == 0) bh[w][q] = (unsigned int)c;
+    }
+    __syncthreads();
+    if (threadIdx.x <= maxRung) {
+        const int q = threadIdx.x;
+        const unsigned long long c = (unsigned long long)bh[0][q] + bh[1][q] + bh[2][q] + bh[3][q];
+        if (c) atomicAdd(&hist[q], c);
     }
 }
 

@@ -51,6 +51,8 @@ template <bool QUAD, bool MASKED>
 __device__ __forceinline__ void eval_cells(const float4* __restrict__ cell, const uint64_t* __restrict__ cmask,
                                            int ncell, float px, float py, float pz, float eps2, float& ax,
                                            float& ay, float& az) {
+    ncell = __builtin_amdgcn_readfirstlane(ncell);  // uniform: the loop counter lives in SGPRs
+#pragma unroll 2
     for (int e = 0; e < ncell; e++) {
         const float4 c0 = cell[3 * e];
         const float dx = c0.x - px, dy = c0.y - py, dz = c0.z - pz;
@@ -80,19 +82,24 @@ __device__ __forceinline__ void eval_cells(const float4* __restrict__ cell, cons
     }
 }
 
-template <bool EPS0>
+// Stars [first, first + npart) of the staging buffer; MASKED: each star
+// applies only to the lanes of its leaf's mask (otherwise to all lanes).
+template <bool EPS0, bool MASKED>
 __device__ __forceinline__ void eval_parts(const float4* __restrict__ part, const uint64_t* __restrict__ pmask,
-                                           int npart, float px, float py, float pz, float eps2, float& ax,
-                                           float& ay, float& az) {
-    for (int e = 0; e < npart; e++) {
-        const uint64_t em = uniform64(pmask[e]);
+                                           int first, int npart, float px, float py, float pz, float eps2,
+                                           float& ax, float& ay, float& az) {
+    first = __builtin_amdgcn_readfirstlane(first);
+    const int end = first + __builtin_amdgcn_readfirstlane(npart);
+#pragma unroll 2
+    for (int e = first; e < end; e++) {
         const float4 q = part[e];
         const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
         float ir = __builtin_amdgcn_rsqf(r2);
-        bool on = __builtin_amdgcn_inverse_ballot_w64(em);
+        bool on = true;
+        if (MASKED) on = __builtin_amdgcn_inverse_ballot_w64(uniform64(pmask[e]));
         if (EPS0) on = on && r2 > 0.0f;  // the star itself: d = 0 exactly
-        ir = on ? ir : 0.0f;
+        if (MASKED || EPS0) ir = on ? ir : 0.0f;
         const float mir = q.w * ir;
         const float f = mir * (ir * ir);
         ax = fmaf(f, dx, ax);
@@ -353,7 +360,10 @@ __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict
     uint64_t* pmask = S.pmask;
     const int lane = lane_id();
     const float px = (float)(G.p.x - G.cgx), py = (float)(G.p.y - G.cgy), pz = (float)(G.p.z - G.cgz);
-    const int4 cnts = lists.counts[G.g];
+    int4 cnts = lists.counts[G.g];
+    cnts.x = __builtin_amdgcn_readfirstlane(cnts.x);  // list lengths are uniform over the wave
+    cnts.y = __builtin_amdgcn_readfirstlane(cnts.y);
+    cnts.z = __builtin_amdgcn_readfirstlane(cnts.z);
     const uint32_t* __restrict__ fullIn = lists.full + (size_t)G.g * lists.cf;
     const uint4* __restrict__ partIn = lists.part + (size_t)G.g * lists.cp;
     const uint4* __restrict__ leafIn = lists.leaf + (size_t)G.g * lists.cl;
@@ -429,8 +439,22 @@ __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict
             cur = nxt;
         }
     }
-    // stars of opened leaves (large leaves are split over chunks)
-    int npart = 0;
+    // Stars of opened leaves (large leaves are split over chunks).  Stars of
+    // leaves opened by every lane fill the staging buffer from the front and
+    // are evaluated without masks; the others fill it from the back with
+    // their leaf's mask.
+    int nfull = 0, nmask = 0;
+    auto flush_parts = [&]() {
+        __syncthreads();
+        float ax = 0, ay = 0, az = 0;
+        eval_parts<EPS0, false>(part, pmask, 0, nfull, px, py, pz, eps2, ax, ay, az);
+        eval_parts<EPS0, true>(part, pmask, kParts - nmask, nmask, px, py, pz, eps2, ax, ay, az);
+        kahan_add(sx, cx, ax);
+        kahan_add(sy, cy, ay);
+        kahan_add(sz, cz, az);
+        nfull = nmask = 0;
+        __syncthreads();
+    };
     for (int c0 = 0; c0 < cnts.z; c0 += kWave) {
         const int cnt = min(kWave, cnts.z - c0);
         int lcur = 0, lcnt = 0;
@@ -441,40 +465,33 @@ __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict
             lcnt = trav[e.x].hi - lcur;
             lmask = (uint64_t)e.z | ((uint64_t)e.w << 32);
         }
+        const bool all = lmask == G.gmask;
         while (__ballot(lcnt > 0)) {
-            const int room = kParts - npart;
+            const int room = kParts - nfull - nmask;
             const int incl = wave_incl_scan(lcnt);
-            const int excl = incl - lcnt;
-            const int w = max(0, min(lcnt, room - excl));
+            const int w = max(0, min(lcnt, room - (incl - lcnt)));
+            const int wf = all ? w : 0, wm = all ? 0 : w;
+            const int ef = wave_incl_scan(wf) - wf, em = wave_incl_scan(wm) - wm;
             for (int i = 0; i < w; i++) {
                 const double4 q = spos[lcur + i];
-                const int slot = npart + excl + i;
-                part[slot] = make_float4((float)(q.x - G.cgx), (float)(q.y - G.cgy), (float)(q.z - G.cgz), (float)q.w);
-                pmask[slot] = lmask;
+                const float4 v = make_float4((float)(q.x - G.cgx), (float)(q.y - G.cgy), (float)(q.z - G.cgz),
+                                             (float)q.w);
+                if (all) {
+                    part[nfull + ef + i] = v;
+                } else {
+                    const int slot = kParts - 1 - (nmask + em + i);
+                    part[slot] = v;
+                    pmask[slot] = lmask;
+                }
             }
-            npart += wave_sum_i(w);
+            nfull += wave_sum_i(wf);
+            nmask += wave_sum_i(wm);
             lcur += w;
             lcnt -= w;
-            if (npart == kParts) {
-                __syncthreads();
-                float ax = 0, ay = 0, az = 0;
-                eval_parts<EPS0>(part, pmask, npart, px, py, pz, eps2, ax, ay, az);
-                kahan_add(sx, cx, ax);
-                kahan_add(sy, cy, ay);
-                kahan_add(sz, cz, az);
-                npart = 0;
-                __syncthreads();
-            }
+            if (nfull + nmask == kParts) flush_parts();
         }
     }
-    __syncthreads();
-    {
-        float ax = 0, ay = 0, az = 0;
-        eval_parts<EPS0>(part, pmask, npart, px, py, pz, eps2, ax, ay, az);
-        kahan_add(sx, cx, ax);
-        kahan_add(sy, cy, ay);
-        kahan_add(sz, cz, az);
-    }
+    flush_parts();
     if (G.valid) {
         const int32_t s = order[G.k];
         oax[s] = (double)sx - (double)cx;

@@ -259,6 +259,16 @@ class Solver:
         com = [c * (1 / m) for c in v[8:11]] if m != 0 else [0.0, 0.0, 0.0]  # c.Scale(1 / m)
         return {"kinetic": v[0], "momentum": v[1:4], "angular_momentum": v[4:7], "mass": m, "com": com}
 
+    def kick(self, s: DeviceState, h: float):
+        """v += h a over every body (enqueued on the solver's stream)."""
+        _lib.check(self._L.gx_kick_dev(self._h, _ptr(s.vx), _ptr(s.vy), _ptr(s.vz), _ptr(s.ax), _ptr(s.ay),
+                                       _ptr(s.az), len(s), float(h)))
+
+    def drift(self, s: DeviceState, dt: float):
+        """x += dt v over every body (enqueued on the solver's stream)."""
+        _lib.check(self._L.gx_drift_dev(self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.vx), _ptr(s.vy),
+                                        _ptr(s.vz), len(s), float(dt)))
+
     def host_state(self, s: DeviceState) -> dict:
         return {f: getattr(s, f).numpy() for f in DeviceState.FIELDS}
 
