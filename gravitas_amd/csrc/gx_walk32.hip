@@ -47,67 +47,123 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
     s = t;
 }
 
-template <bool QUAD, bool MASKED>
-__device__ __forceinline__ void eval_cells(const float4* __restrict__ cell, const uint64_t* __restrict__ cmask,
-                                           int ncell, float px, float py, float pz, float eps2, float& ax,
-                                           float& ay, float& az) {
+// One cell's pull on a star at (px, py, pz), group-relative: monopole and
+// quadrupole terms of bh.go:117-133 with (c0.xyz, c0.w) = (COM, m) and
+// (c1, c2.xy) = (qxx, qyy, qzz, qxy, qxz, qyz).
+template <bool QUAD>
+__device__ __forceinline__ void cell_force(const float4 c0, const float4 c1, const float4 c2, float px, float py,
+                                           float pz, float eps2, float& ax, float& ay, float& az) {
+    const float dx = c0.x - px, dy = c0.y - py, dz = c0.z - pz;
+    const float d2e = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+    const float ir = __builtin_amdgcn_rsqf(d2e);
+    const float inv = ir * ir;
+    const float inv3 = inv * ir;
+    float f = c0.w * inv3;
+    if (QUAD) {
+        const float qx = fmaf(c1.x, dx, fmaf(c1.w, dy, c2.x * dz));
+        const float qy = fmaf(c1.w, dx, fmaf(c1.y, dy, c2.y * dz));
+        const float qz = fmaf(c2.x, dx, fmaf(c2.y, dy, c1.z * dz));
+        const float inv5 = inv3 * inv;
+        const float dqd = fmaf(dx, qx, fmaf(dy, qy, dz * qz));
+        f = fmaf(2.5f * dqd, inv5 * inv, f);
+        ax = fmaf(-qx, inv5, ax);
+        ay = fmaf(-qy, inv5, ay);
+        az = fmaf(-qz, inv5, az);
+    }
+    ax = fmaf(f, dx, ax);
+    ay = fmaf(f, dy, ay);
+    az = fmaf(f, dz, az);
+}
+
+// Cells accepted by every lane of the group: all lanes evaluate every cell.
+template <bool QUAD>
+__device__ __forceinline__ void eval_cells(const float4* __restrict__ cell, int ncell, float px, float py, float pz,
+                                           float eps2, float& ax, float& ay, float& az) {
     ncell = __builtin_amdgcn_readfirstlane(ncell);  // uniform: the loop counter lives in SGPRs
 #pragma unroll 2
-    for (int e = 0; e < ncell; e++) {
-        const float4 c0 = cell[3 * e];
-        const float dx = c0.x - px, dy = c0.y - py, dz = c0.z - pz;
-        const float d2e = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-        float ir = __builtin_amdgcn_rsqf(d2e);
-        if (MASKED) ir = __builtin_amdgcn_inverse_ballot_w64(uniform64(cmask[e])) ? ir : 0.0f;
-        const float inv = ir * ir;
-        const float inv3 = inv * ir;
-        float f = c0.w * inv3;
-        if (QUAD) {
-            const float4 c1 = cell[3 * e + 1];
-            const float4 c2 = cell[3 * e + 2];
-            // Q = (qxx, qyy, qzz, qxy) in c1, (qxz, qyz) in c2.xy
-            const float qx = fmaf(c1.x, dx, fmaf(c1.w, dy, c2.x * dz));
-            const float qy = fmaf(c1.w, dx, fmaf(c1.y, dy, c2.y * dz));
-            const float qz = fmaf(c2.x, dx, fmaf(c2.y, dy, c1.z * dz));
-            const float inv5 = inv3 * inv;
-            const float dqd = fmaf(dx, qx, fmaf(dy, qy, dz * qz));
-            f = fmaf(2.5f * dqd, inv5 * inv, f);
-            ax = fmaf(-qx, inv5, ax);
-            ay = fmaf(-qy, inv5, ay);
-            az = fmaf(-qz, inv5, az);
-        }
-        ax = fmaf(f, dx, ax);
-        ay = fmaf(f, dy, ay);
-        az = fmaf(f, dz, az);
+    for (int e = 0; e < ncell; e++)
+        cell_force<QUAD>(cell[3 * e], cell[3 * e + 1], cell[3 * e + 2], px, py, pz, eps2, ax, ay, az);
+}
+
+// Transpose of the 64 x 64 bit matrix whose row l is held by lane l: returns
+// column l (bit e set iff bit l of lane e's row is set).  Six butterfly
+// stages, each swapping off-diagonal blocks with the lane l ^ s.
+__device__ __forceinline__ uint64_t transpose_bits(uint64_t x) {
+    const int lane = lane_id();
+    constexpr uint64_t keep[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                                  0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int sft = 32 >> k;
+        const uint64_t m = keep[k];
+        const uint64_t y = (uint64_t)__shfl_xor((long long)x, sft, kWave);
+        x = (lane & sft) ? ((x & ~m) | ((y & ~m) >> sft)) : ((x & m) | ((y & m) << sft));
+    }
+    return x;
+}
+
+// Cells accepted by some lanes only (mask of cell e held by lane e): each
+// lane loops over its own cells, so no lane slot is spent on a masked-off
+// star.  col = transpose of the masks.
+template <bool QUAD>
+__device__ __forceinline__ void eval_cells_own(const float4* __restrict__ cell, uint64_t col, float px, float py,
+                                               float pz, float eps2, float& ax, float& ay, float& az) {
+    while (col) {
+        const int e = __builtin_ctzll(col);
+        col &= col - 1;
+        cell_force<QUAD>(cell[3 * e], cell[3 * e + 1], cell[3 * e + 2], px, py, pz, eps2, ax, ay, az);
     }
 }
 
-// Stars [first, first + npart) of the staging buffer; MASKED: each star
-// applies only to the lanes of its leaf's mask (otherwise to all lanes).
-template <bool EPS0, bool MASKED>
-__device__ __forceinline__ void eval_parts(const float4* __restrict__ part, const uint64_t* __restrict__ pmask,
-                                           int first, int npart, float px, float py, float pz, float eps2,
-                                           float& ax, float& ay, float& az) {
+// One star's pull (bh.go:145-150, Plummer softening); with EPS0 the star
+// itself (d = 0 exactly) is skipped.
+template <bool EPS0>
+__device__ __forceinline__ void star_force(const float4 q, float px, float py, float pz, float eps2, float& ax,
+                                           float& ay, float& az) {
+    const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+    float ir = __builtin_amdgcn_rsqf(r2);
+    if (EPS0) ir = r2 > 0.0f ? ir : 0.0f;
+    const float mir = q.w * ir;
+    const float f = mir * (ir * ir);
+    ax = fmaf(f, dx, ax);
+    ay = fmaf(f, dy, ay);
+    az = fmaf(f, dz, az);
+}
+
+// Stars [first, first + npart) of the staging buffer, for every lane.
+template <bool EPS0>
+__device__ __forceinline__ void eval_parts(const float4* __restrict__ part, int first, int npart, float px, float py,
+                                           float pz, float eps2, float& ax, float& ay, float& az) {
     first = __builtin_amdgcn_readfirstlane(first);
     const int end = first + __builtin_amdgcn_readfirstlane(npart);
 #pragma unroll 2
-    for (int e = first; e < end; e++) {
-        const float4 q = part[e];
-        const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
-        const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
-        float ir = __builtin_amdgcn_rsqf(r2);
-        bool on = true;
-        if (MASKED) on = __builtin_amdgcn_inverse_ballot_w64(uniform64(pmask[e]));
-        if (EPS0) on = on && r2 > 0.0f;  // the star itself: d = 0 exactly
-        if (MASKED || EPS0) ir = on ? ir : 0.0f;
-        const float mir = q.w * ir;
-        const float f = mir * (ir * ir);
-        ax = fmaf(f, dx, ax);
-        ay = fmaf(f, dy, ay);
-        az = fmaf(f, dz, az);
-    }
+    for (int e = first; e < end; e++) star_force<EPS0>(part[e], px, py, pz, eps2, ax, ay, az);
 }
 
+// Stars [first, first + npart) of the staging buffer, each for the lanes of
+// its leaf's mask pmask[]: blocks of 64 stars, masks transposed, and each
+// lane loops over its own stars.
+template <bool EPS0>
+__device__ __forceinline__ void eval_parts_own(const float4* __restrict__ part, const uint64_t* __restrict__ pmask,
+                                               int first, int npart, float px, float py, float pz, float eps2,
+                                               float& ax, float& ay, float& az) {
+    first = __builtin_amdgcn_readfirstlane(first);
+    npart = __builtin_amdgcn_readfirstlane(npart);
+    // block b holds stars b, b + nb, b + 2 nb, ... (spread over the leaves)
+    const int nb = (npart + kWave - 1) / kWave;
+    const int lane = lane_id();
+    for (int b = 0; b < nb; b++) {
+        const int mine = b + nb * lane;
+        uint64_t col = transpose_bits(mine < npart ? pmask[first + mine] : 0ull);
+        const float4* blk = part + first + b;
+        while (col) {
+            const int e = __builtin_ctzll(col);
+            col &= col - 1;
+            star_force<EPS0>(blk[nb * e], px, py, pz, eps2, ax, ay, az);
+        }
+    }
+}
 
 // Group prologue shared by both kernels: the group's stars (up to gsize
 // consecutive targets of one instance in Morton order: all its stars, or the
@@ -174,7 +230,6 @@ struct TravLDS {
 
 struct EvalLDS {
     float4 cell[kChunk * 3];
-    uint64_t cmask[kChunk];
     float4 part[kParts];
     uint64_t pmask[kParts];
 };
@@ -355,7 +410,6 @@ __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict
                            double* __restrict__ oax, double* __restrict__ oay, double* __restrict__ oaz) {
     const float eps2 = G.eps2;
     float4* cell = S.cell;
-    uint64_t* cmask = S.cmask;
     float4* part = S.part;
     uint64_t* pmask = S.pmask;
     const int lane = lane_id();
@@ -405,33 +459,40 @@ __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict
         m = G.gmask;
         return (int)fullIn[i];
     };
+    // Full cells go in list order, chunk c holding entries [64c, 64c + 64).
+    // Partial cells are dealt to the chunks with a stride (chunk c holds
+    // entries c, c + nch, c + 2 nch, ...): consecutive entries have similar
+    // masks, and spreading them evens out how many cells each lane of a chunk
+    // evaluates in its own loop.
     for (int pass = 0; pass < 2; pass++) {
         const bool partial = pass == 1;
         const int n = partial ? cnts.y : cnts.x;
         if (n == 0) continue;
+        const int nch = (n + kChunk - 1) / kChunk;
+        auto entry = [&](int c, int j) { return partial ? c + nch * j : c * kChunk + j; };
         uint64_t m0 = 0, m1 = 0;
-        int idx0 = lane < n ? index_of(partial, lane, m0) : 0;
-        int idx1 = kChunk + lane < n ? index_of(partial, kChunk + lane, m1) : 0;
+        int idx0 = entry(0, lane) < n ? index_of(partial, entry(0, lane), m0) : 0;
+        int idx1 = nch > 1 && entry(1, lane) < n ? index_of(partial, entry(1, lane), m1) : 0;
         Staged cur;
-        if (lane < n) cur = fetch(idx0, m0);
-        for (int c0 = 0; c0 < n; c0 += kChunk) {
-            const int cnt = min(kChunk, n - c0);
-            if (lane < cnt) {
+        if (entry(0, lane) < n) cur = fetch(idx0, m0);
+        for (int c = 0; c < nch; c++) {
+            const int c0 = c * kChunk;
+            const bool mine = entry(c, lane) < n;
+            const int cnt = partial ? (n - c + nch - 1) / nch : min(kChunk, n - c0);
+            if (mine) {
                 cell[3 * lane] = cur.a;
                 cell[3 * lane + 1] = cur.b;
                 cell[3 * lane + 2] = make_float4(cur.c.x, cur.c.y, 0.0f, 0.0f);
-                cmask[lane] = cur.m;
             }
             __syncthreads();
             Staged nxt;
-            const int c1 = c0 + kChunk;
-            if (c1 + lane < n) nxt = fetch(idx1, m1);
-            if (c1 + kChunk + lane < n) idx1 = index_of(partial, c1 + kChunk + lane, m1);
+            if (c + 1 < nch && entry(c + 1, lane) < n) nxt = fetch(idx1, m1);
+            if (c + 2 < nch && entry(c + 2, lane) < n) idx1 = index_of(partial, entry(c + 2, lane), m1);
             float ax = 0, ay = 0, az = 0;
             if (partial)
-                eval_cells<QUAD, true>(cell, cmask, cnt, px, py, pz, eps2, ax, ay, az);
+                eval_cells_own<QUAD>(cell, transpose_bits(mine ? cur.m : 0ull), px, py, pz, eps2, ax, ay, az);
             else
-                eval_cells<QUAD, false>(cell, cmask, cnt, px, py, pz, eps2, ax, ay, az);
+                eval_cells<QUAD>(cell, cnt, px, py, pz, eps2, ax, ay, az);
             kahan_add(sx, cx, ax);
             kahan_add(sy, cy, ay);
             kahan_add(sz, cz, az);
@@ -447,8 +508,8 @@ __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict
     auto flush_parts = [&]() {
         __syncthreads();
         float ax = 0, ay = 0, az = 0;
-        eval_parts<EPS0, false>(part, pmask, 0, nfull, px, py, pz, eps2, ax, ay, az);
-        eval_parts<EPS0, true>(part, pmask, kParts - nmask, nmask, px, py, pz, eps2, ax, ay, az);
+        eval_parts<EPS0>(part, 0, nfull, px, py, pz, eps2, ax, ay, az);
+        eval_parts_own<EPS0>(part, pmask, kParts - nmask, nmask, px, py, pz, eps2, ax, ay, az);
         kahan_add(sx, cx, ax);
         kahan_add(sy, cy, ay);
         kahan_add(sz, cz, az);
