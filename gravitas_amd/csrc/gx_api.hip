@@ -97,6 +97,11 @@ struct gx_solver {
     unsigned long long* hhist = nullptr;  // pinned mirror of hist
     int cf = 2048, cp = 512, cl = 256;  // FP32-walk list capacities per group (grow on demand)
     long long capNodes = 0;
+    // Deepest tree level built: the last step's depth + 2 (a deeper tree
+    // raises kFlagDepth and the step is repeated with every level); all
+    // levels while a graph is captured.
+    int levels = kMortonBits;
+    bool allLevels = false;
     // instances: device config (offsets | theta | eps2) and per-instance results
     DevBuf instCfg, boxEnc, starCount, starOff, groupOff, root, half;
     char* instHost = nullptr;  // pinned staging of the config
@@ -381,11 +386,12 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
                        (const int32_t*)s->order.p, meta, a.dmOff, (double4*)s->spos.p);
     mark(s, 4);
     hipLaunchKernelGGL(k_tree_init, dim3(blocks(s->B, 256)), dim3(256), 0, st, meta, batch, (TravNode*)s->trav.p);
-    for (int L = 0; L <= kMortonBits; L++)
+    const int Lmax = s->allLevels ? kMortonBits : s->levels;
+    for (int L = 0; L <= Lmax; L++)
         hipLaunchKernelGGL(k_topo_level, dim3(kTreeGrid), dim3(256), 0, st, L, (const uint64_t*)s->keys.p,
-                           (TravNode*)s->trav.p, meta, s->capNodes);
+                           (TravNode*)s->trav.p, meta, s->capNodes, L == Lmax && Lmax < kMortonBits ? 1 : 0);
     mark(s, 5);
-    for (int L = kMortonBits; L >= 0; L--)
+    for (int L = Lmax; L >= 0; L--)
         hipLaunchKernelGGL(k_moments_level, dim3(kTreeGrid), dim3(256), 0, st, L, (const double4*)s->spos.p,
                            (TravNode*)s->trav.p, (Mult64*)s->m64.p, (Mult32*)s->m32.p, (Mult16*)s->m16.p, meta,
                            batch);
@@ -512,6 +518,8 @@ int finish_step(gx_solver* s) {
     s->stats.max_list[3] = 0;
     if (h.flags & kFlagNodeOverflow)
         return set_err(GX_ERR_CAPACITY, "node arena of %lld nodes too small", s->capNodes);
+    if (h.flags & kFlagDepth) return set_err(GX_ERR_CAPACITY, "tree deeper than the levels built");
+    s->levels = std::min(kMortonBits, depth + 2);
     return GX_OK;
 }
 
@@ -520,6 +528,10 @@ bool regrow(gx_solver* s) {
     const Meta& h = *s->hmeta;
     if (h.flags & kFlagNodeOverflow) {
         s->capNodes *= 2;
+        return true;
+    }
+    if (h.flags & kFlagDepth) {
+        s->levels = kMortonBits;
         return true;
     }
     if (h.flags & kFlagListOverflow) {
@@ -754,7 +766,10 @@ int gx_enqueue_accelerations_dev(gx_solver* s, const double* x, const double* y,
     int rc;
     if ((rc = check_arrays(a)) || (rc = set_single(s, n, s->theta, eps))) return rc;
     s->pending = false;  // a new step supersedes an unsynchronised one
-    return enqueue_step(s, a);
+    s->allLevels = true;  // no repeat is possible once the caller has moved on
+    rc = enqueue_step(s, a);
+    s->allLevels = false;
+    return rc;
 }
 
 int gx_sync(gx_solver* s) {
@@ -871,6 +886,7 @@ int gx_graph_capture_step(gx_solver* s, double* x, double* y, double* z, double*
     s->graph = nullptr;
     const int timing = s->timing;
     s->timing = 0;  // no event records inside the graph
+    s->allLevels = true;  // replays cannot repeat a step with more levels
     HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     rc = kick(s, vx, vy, vz, ax, ay, az, n, 0.5 * dt);
     if (!rc) rc = drift(s, x, y, z, vx, vy, vz, n, dt);
@@ -881,6 +897,7 @@ int gx_graph_capture_step(gx_solver* s, double* x, double* y, double* z, double*
     hipGraph_t g = nullptr;
     hipError_t ec = hipStreamEndCapture(s->stream, &g);
     s->timing = timing;
+    s->allLevels = false;
     s->pending = false;
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
@@ -923,6 +940,10 @@ int gx_potential_energy_dev(gx_solver* s, const double* x, const double* y, cons
         HIPCHK(hipStreamSynchronize(s->stream));
         if (s->hmeta->flags & kFlagNodeOverflow) {
             s->capNodes *= 2;
+            continue;
+        }
+        if (s->hmeta->flags & kFlagDepth) {
+            s->levels = kMortonBits;
             continue;
         }
         if (s->hmeta->nholes > 0)

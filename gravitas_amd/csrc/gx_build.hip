@@ -459,6 +459,21 @@ __device__ int octant_bounds(const uint64_t* __restrict__ keys, int lo, int hi, 
         }
         return result;
     }
+    if (R <= 8 * kWave) {
+        // one pass: every lane reads up to 8 keys; boundary o = lo + #(digit < o)
+        int c = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int k = lo + lane + kWave * j;
+            const int d = k < hi ? (int)((keys[k] >> shift) & 7) : 8;
+#pragma unroll
+            for (int o = 1; o < 8; o++) {
+                const int b = __popcll(__ballot(d < o));
+                if (lane == o) c += b;
+            }
+        }
+        return lane >= 1 && lane <= 7 ? lo + c : 0;
+    }
     // 7 groups of 9 lanes; group g searches octant o = g + 1 with a 10-ary search.
     const int g = lane / 9, s = lane % 9, o = g + 1;
     int a = lo, b = hi;  // answer in [a, b]
@@ -492,44 +507,78 @@ __device__ int octant_bounds(const uint64_t* __restrict__ keys, int lo, int hi, 
 }
 
 // Splits every node of level L (one wavefront per node, grid-stride over the
-// level) and appends the children to level L+1.
+// level) and appends the children to level L+1.  With checkOnly the level is
+// the deepest one built for this step: a node there that would be split
+// raises kFlagDepth (the step is then repeated with every level).
 __device__ void topo_level(int L, const uint64_t* __restrict__ keys, TravNode* __restrict__ trav, Meta* meta,
-                           long long cap) {
+                           long long cap, int checkOnly) {
     const long long start = meta->levelStart[L];
     const long long cnt = meta->levelCount[L];
     const long long nextBase = start + cnt;
+    if (checkOnly) {
+        if (L >= kMortonBits) return;
+        for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
+             t += (long long)gridDim.x * blockDim.x)
+            if (trav[start + t].hi - trav[start + t].lo > kLeafSize) atomicOr(&meta->flags, (int)kFlagDepth);
+        return;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) meta->levelStart[L + 1] = nextBase;
-    if (L >= kMortonBits || (meta->flags & kFlagNodeOverflow)) return;
+    __shared__ int wcnt[4];
+    __shared__ long long wbase;
+    __shared__ int stop;
+    if (threadIdx.x == 0) stop = L >= kMortonBits || (meta->flags & kFlagNodeOverflow);
+    __syncthreads();
+    if (stop) return;  // the same decision for every wave of the workgroup
     const int lane = lane_id();
-    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int w = threadIdx.x / kWave;  // 4 waves per workgroup
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    const long long iters = (cnt + nwaves - 1) / nwaves;
     const int shift = 3 * (kMortonBits - 1 - L);
-    for (long long wv = wave; wv < cnt; wv += nwaves) {
+    // Every wave of the workgroup runs the same number of iterations, so the
+    // children of the workgroup's (up to) 4 nodes get their slots from one
+    // atomic on the level counter.
+    for (long long it = 0; it < iters; it++) {
+        const long long wv = it * nwaves + ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
         const long long ni = start + wv;
-        const int lo = trav[ni].lo, hi = trav[ni].hi;
-        if (hi - lo <= kLeafSize) continue;  // leaf: first = -1 already
-        int b = octant_bounds(keys, lo, hi, shift);
-        // bounds[0] = lo, bounds[8] = hi
-        int bo = lane == 0 ? lo : (lane == 8 ? hi : b);
-        int bnext = __shfl_down(bo, 1, kWave);
-        bool nonempty = lane < 8 && bnext > bo;
-        uint64_t ne = __ballot(nonempty);
-        int nchild = __popcll(ne);
-        long long off = 0;
-        if (lane == 0) off = (long long)atomicAdd((unsigned long long*)&meta->levelCount[L + 1], (unsigned long long)nchild);
-        off = __shfl(off, 0, kWave);
-        long long firstc = nextBase + off;
+        int lo = 0, hi = 0;
+        if (wv < cnt) {
+            lo = trav[ni].lo;
+            hi = trav[ni].hi;
+        }
+        const bool split = wv < cnt && hi - lo > kLeafSize;  // otherwise a leaf: first = -1 already
+        int bo = 0, bnext = 0, nchild = 0;
+        uint64_t ne = 0;
+        if (split) {
+            const int b = octant_bounds(keys, lo, hi, shift);
+            // bounds[0] = lo, bounds[8] = hi
+            bo = lane == 0 ? lo : (lane == 8 ? hi : b);
+            bnext = __shfl_down(bo, 1, kWave);
+            ne = __ballot(lane < 8 && bnext > bo);
+            nchild = __popcll(ne);
+        }
+        if (lane == 0) wcnt[w] = nchild;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+            wbase = tot ? (long long)atomicAdd((unsigned long long*)&meta->levelCount[L + 1], (unsigned long long)tot)
+                        : 0;
+        }
+        __syncthreads();
+        long long firstc = nextBase + wbase;
+        for (int k = 0; k < w; k++) firstc += wcnt[k];
+        __syncthreads();  // wcnt and wbase are rewritten by the next iteration
+        if (!split) continue;
         if (firstc + nchild > cap) {
             if (lane == 0) atomicOr(&meta->flags, (int)kFlagNodeOverflow);
             continue;
         }
         const double cx = trav[ni].x, cy = trav[ni].y, cz = trav[ni].z, half = trav[ni].crit2;
         const int pnext = trav[ni].next, pinst = trav[ni].inst;
-        if (nonempty) {
-            int rank = __popcll(ne & ((1ull << lane) - 1));
-            long long c = firstc + rank;
-            double q = half / 2;
-            int oct = lane;
+        if (lane < 8 && bnext > bo) {
+            const int rank = __popcll(ne & ((1ull << lane) - 1));
+            const long long c = firstc + rank;
+            const double q = half / 2;
+            const int oct = lane;
             TravNode t;
             t.x = cx + (((oct >> 2) & 1) ? 1.0 : -1.0) * q;
             t.y = cy + (((oct >> 1) & 1) ? 1.0 : -1.0) * q;
@@ -553,8 +602,9 @@ __device__ void topo_level(int L, const uint64_t* __restrict__ keys, TravNode* _
 }
 
 __global__ __launch_bounds__(256) void k_topo_level(int L, const uint64_t* __restrict__ keys,
-                                                    TravNode* __restrict__ trav, Meta* meta, long long cap) {
-    topo_level(L, keys, trav, meta, cap);
+                                                    TravNode* __restrict__ trav, Meta* meta, long long cap,
+                                                    int checkOnly) {
+    topo_level(L, keys, trav, meta, cap, checkOnly);
 }
 
 // ---------------------------------------------------------------------------
@@ -580,7 +630,10 @@ __device__ void node_moments(long long ni, const double4* __restrict__ spos, Tra
     const double theta = thetas[nd.inst];
     const double cx = nd.x, cy = nd.y, cz = nd.z, half = nd.crit2;
     double x = 0, y = 0, z = 0, m = 0;
+    // the sums keep the reference's order; unrolling only lets the loads of
+    // several stars be in flight together
     if (nd.first < 0) {
+#pragma unroll 4
         for (int j = nd.lo; j < nd.hi; j++) {
             double4 p = spos[j];
             double mj = p.w;
@@ -609,6 +662,7 @@ __device__ void node_moments(long long ni, const double4* __restrict__ spos, Tra
     }
     double q[6] = {0, 0, 0, 0, 0, 0};
     if (nd.first < 0) {
+#pragma unroll 4
         for (int j = nd.lo; j < nd.hi; j++) {
             double4 p = spos[j];
             quad_add(q, p.w, p.x - x, p.y - y, p.z - z);
@@ -657,7 +711,7 @@ __global__ __launch_bounds__(256) void k_moments_level(int L, const double4* __r
                                                        const Meta* __restrict__ meta, Batch batch) {
     const long long start = meta->levelStart[L];
     const long long cnt = meta->levelCount[L];
-    if (meta->flags & kFlagNodeOverflow) return;
+    if (meta->flags & kFlagTreeIncomplete) return;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
          t += (long long)gridDim.x * blockDim.x)
         node_moments(start + t, spos, trav, m64, m32, m16, batch.theta);

@@ -107,12 +107,19 @@ struct Meta {
     unsigned long long rounds;       // FP32 walk traversal rounds over all groups
     unsigned long long mixed;        // nodes decided per star (box test inconclusive)
     long long ntargets;     // walk targets (the active stars of an active set)
-    int flags;              // bit 0: node arena, bit 1: walk stack, bit 2: walk lists overflowed
+    int flags;              // Flag bits
     int maxStack;
     int maxList[3];         // longest full-cell, partial-cell and leaf list of a group
 };
 
-enum Flag : int { kFlagNodeOverflow = 1, kFlagStackOverflow = 2, kFlagListOverflow = 4 };
+enum Flag : int {
+    kFlagNodeOverflow = 1,   // node arena too small
+    kFlagStackOverflow = 2,  // FP32 walk stack too small
+    kFlagListOverflow = 4,   // FP32 walk lists too small
+    kFlagDepth = 8,          // the tree needs more levels than were built
+};
+// The tree of this step is incomplete: later kernels skip their work.
+constexpr int kFlagTreeIncomplete = kFlagNodeOverflow | kFlagDepth;
 
 // ---------------------------------------------------------------------------
 // wavefront helpers

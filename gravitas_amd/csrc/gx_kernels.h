@@ -25,7 +25,7 @@ __global__ void k_sort_scatter(const uint64_t* kin, const int32_t* vin, uint64_t
 __global__ void k_gather(const double* x, const double* y, const double* z, const double* m, const uint8_t* kind,
                          const int32_t* order, const Meta* meta, int dmOff, double4* spos);
 __global__ void k_tree_init(Meta* meta, Batch batch, TravNode* trav);
-__global__ void k_topo_level(int L, const uint64_t* keys, TravNode* trav, Meta* meta, long long cap);
+__global__ void k_topo_level(int L, const uint64_t* keys, TravNode* trav, Meta* meta, long long cap, int checkOnly);
 __global__ void k_moments_level(int L, const double4* spos, TravNode* trav, Mult64* m64, Mult32* m32,
                                 Mult16* m16, const Meta* meta, Batch batch);
 __global__ void k_tree_finish(Meta* meta);

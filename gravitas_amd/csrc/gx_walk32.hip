@@ -564,7 +564,7 @@ __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict
 __global__ __launch_bounds__(64) void k_trav32(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
                                                Meta* meta, Batch batch, Lists lists) {
     __shared__ TravLDS S;
-    if (meta->flags & kFlagNodeOverflow) return;
+    if (meta->flags & kFlagTreeIncomplete) return;
     Group G;
     if (!load_group(G, spos, batch, xcd_group(batch))) return;
     trav_group(G, S, trav, meta, lists);
@@ -577,7 +577,7 @@ __global__ __launch_bounds__(64) void k_eval32(const double4* __restrict__ spos,
                                                Lists lists, double* __restrict__ oax, double* __restrict__ oay,
                                                double* __restrict__ oaz) {
     __shared__ EvalLDS S;
-    if (meta->flags & (kFlagNodeOverflow | kFlagStackOverflow | kFlagListOverflow)) return;
+    if (meta->flags & (kFlagTreeIncomplete | kFlagStackOverflow | kFlagListOverflow)) return;
     Group G;
     if (!load_group(G, spos, batch, xcd_group(batch))) return;
     eval_group<QUAD, EPS0, Q16>(G, S, spos, trav, m32, m16, order, lists, oax, oay, oaz);
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos,
                                                Lists lists, int persistent, double* __restrict__ oax,
                                                double* __restrict__ oay, double* __restrict__ oaz) {
     __shared__ WalkLDS S;
-    if (meta->flags & kFlagNodeOverflow) return;
+    if (meta->flags & kFlagTreeIncomplete) return;
     long long g = persistent ? -1 : xcd_group(batch);
     for (;;) {
         if (persistent) {

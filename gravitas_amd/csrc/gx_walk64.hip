@@ -15,7 +15,7 @@ __global__ __launch_bounds__(256) void k_walk64(const double4* __restrict__ spos
                                                 Meta* meta, Batch batch, int quad, double* __restrict__ ax,
                                                 double* __restrict__ ay, double* __restrict__ az) {
     const long long nt = batch.tOff[batch.B];
-    if (meta->flags & kFlagNodeOverflow) return;
+    if (meta->flags & kFlagTreeIncomplete) return;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long npp = 0, npc = 0;
     if (t < nt) {
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void k_potential64(const double4* __restrict__
                                                      const Mult64* __restrict__ m64, Meta* meta, Batch batch,
                                                      int quad, double* __restrict__ partial) {
     const long long nst = meta->nstars;
-    if (meta->flags & kFlagNodeOverflow) return;
+    if (meta->flags & kFlagTreeIncomplete) return;
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nst) return;
     const int b = upper_index(batch.starOff, batch.B, k);

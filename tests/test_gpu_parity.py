@@ -250,3 +250,22 @@ def test_repeat_is_deterministic(gpu):
     a = run_gpu(s, 0.5, 1e-3, "fp32")["acc"]
     b = run_gpu(s, 0.5, 1e-3, "fp32")["acc"]
     assert np.array_equal(a, b)
+
+
+def test_deeper_tree_after_shallow_one_is_rebuilt_in_full(gpu):
+    """The solver builds as many levels as the previous tree needed (+2); a
+    deeper tree in the next call must be detected and built in full."""
+    shallow = ics.plummer(3000, 31)
+    deep = ics.plummer(3000, 32)
+    # a tight pair of 40-star clusters forces splitting down to the deepest levels
+    deep["px"][:40] = 0.25 + 1e-9 * np.arange(40)
+    deep["py"][:40] = 0.25
+    deep["pz"][:40] = 0.25 + 1e-12 * np.arange(40)
+    solver = Solver(theta=0.6, walk="fp64")
+    for name, st in (("shallow", shallow), ("deep", deep), ("shallow again", shallow)):
+        say(f"[parity] depth change: {name}")
+        ds = DeviceState.from_host(st)
+        solver.accelerations(ds, Params(softening=0.01))
+        _, want = run_oracle(st, 0.6, 0.01)
+        assert np.array_equal(ds.acc_host(), want), name
+    assert solver.stats()["depth"] < 21
