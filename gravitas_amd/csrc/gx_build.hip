@@ -304,10 +304,16 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
     __syncthreads();
     long long nst = meta->nstars;
     long long base = (long long)blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortTile / 256; r++) {
-        long long i = base + r * 256 + tid;
-        if (i < nst) atomicAdd(&h[sort_digit(keys[i], vals[i], shift, instPass, batch)], 1u);
+    constexpr int R = kSortTile / 256;
+    uint32_t dig[R];  // all loads of the tile in flight before the first atomic
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const long long i = base + r * 256 + tid;
+        dig[r] = i < nst ? sort_digit(keys[i], instPass ? vals[i] : 0, shift, instPass, batch) : 256u;
     }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+        if (dig[r] < 256u) atomicAdd(&h[dig[r]], 1u);
     __syncthreads();
     hist[(size_t)tid * ntiles + blockIdx.x] = h[tid];
 }
@@ -367,11 +373,21 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict
     tileBase[tid] = scan[tid] - tot + hist[(size_t)tid * ntiles + blockIdx.x];
     __syncthreads();
     const long long base = (long long)blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortTile / 256; r++) {
+    constexpr int R = kSortTile / 256;
+    uint64_t keyr[R];  // the whole tile is loaded up front: one memory latency, not R
+    int32_t valr[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const long long idx = base + r * 256 + tid;
+        keyr[r] = idx < nst ? kin[idx] : 0;
+        valr[r] = idx < nst ? vin[idx] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
         long long idx = base + r * 256 + tid;
         bool valid = idx < nst;
-        uint64_t key = valid ? kin[idx] : 0;
-        int32_t val = valid ? vin[idx] : 0;
+        uint64_t key = keyr[r];
+        int32_t val = valr[r];
         uint32_t dig = valid ? sort_digit(key, val, shift, instPass, batch) : 0u;
         uint64_t match = __ballot(valid);
         for (int b = 0; b < 8; b++) {
