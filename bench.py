@@ -134,8 +134,24 @@ def walk_summary(n, st, phase_avg):
         "phases_ms": phase_avg,
         "roofline": {"bound": "valu", "kernel": "k_walk32", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "flops_per_launch": flops, "walk_ms": walk_s * 1e3},
+                     "flops_per_launch": flops, "walk_ms": walk_s * 1e3,
+                     "note": "FP32 vector ALU bound (no dense contraction, so no MFMA); peak = MI355X FP32 vector "
+                             "157.3 TFLOP/s (packed); with packed FP32 disabled the issue ceiling is 78.6"},
     }
+
+
+def measured_traffic(kernel, workload, n, theta):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known
+    size), or None if no such measurement exists for this configuration."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload.replace('-', '_')}_traffic.json")
+    try:
+        t = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if t.get("n") != n or t.get("theta") != theta:
+        return None
+    return t["traffic_bytes"]
 
 
 def base_line(args, n, ms_per_step, dtype, config):
@@ -171,6 +187,7 @@ def bench_disk_halo(args):
         "parallelism": "single GPU"})
     out["interactions_per_s"] = (st["pp_interactions"] + st["pc_interactions"]) / (ms * 1e-3)
     out.update(walk_summary(n, st, phases))
+    out["roofline"]["traffic"] = measured_traffic("k_walk32", "disk-halo-1m", n, theta)
     if not args.no_errors:
         log("FP64 walk and direct-sum sample")
         s64 = Solver(theta=theta, walk="fp64", timing=True)
@@ -381,6 +398,18 @@ def bench_encounter(args):
     solver = Solver(theta=theta, walk="fp32-q16")
     solver.accelerations(ds, params)  # initial forces
     acc0 = ds.acc_host()
+    # full force evaluations at 4M (the rate compared with the CPU path)
+    full = {}
+    for walk in ("fp32-q16", "fp32"):
+        sf = Solver(theta=theta, walk=walk)
+        sf.accelerations(ds, params)
+        synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            sf.accelerations(ds, params)
+        synchronize()
+        full[walk] = n / ((time.perf_counter() - t0) / 3)
+    solver.accelerations(ds, params)
     a = np.linalg.norm(acc0, axis=1)
     T = ENCOUNTER_T
     dt_wanted = np.sqrt(2 * eta * eps / a[a > 0])
@@ -439,7 +468,13 @@ def bench_encounter(args):
                      "global_steps": args.steps + args.warmup, "time_span": (args.steps + args.warmup) * T}
     p0, p1 = np.array(diag0["momentum"]), np.array(diag1["momentum"])
     out["momentum_change"] = float(np.linalg.norm(p1 - p0))
+    out["full_force_particles_per_s"] = full
     out.update(out_err)
+    if not args.no_cpu:
+        log("CPU baseline")
+        cb = cpu_baseline(state, theta, eps, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["full_force_gpu_over_cpu"] = full["fp32-q16"] / cb["value"]
     return out
 
 
