@@ -37,7 +37,7 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector, datasheet
 HBM_PEAK_GBS = 8000.0
 FLOPS_PP = 18              # bh.go:145-150 counting rule (add/sub/mul/div/sqrt = 1, fma = 2)
 FLOPS_PC = 50              # bh.go:114-131
-ENCOUNTER_T = 0.16          # global step of the two-disk encounter: about 3 % of the stars on the top rung
+ENCOUNTER_T = 0.12          # global step of the two-disk encounter: 1-5 % of the stars on the top rung
 CONFIGS = ("disk-halo-1m", "plummer-16k", "clumps-16m", "batched-256", "two-disk-4m")
 
 
@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-errors", action="store_true", help="skip the error measurements")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--T", type=float, default=None, help="two-disk-4m: override the global step")
     ap.add_argument("--calibrate", action="store_true",
                     help="also launch 3 drifts with dt = 0 over the resident bodies (k_drift reads 48 and writes "
                          "24 bytes per body): the known-size kernel that scales HBM counters")
@@ -411,7 +412,7 @@ def bench_encounter(args):
         full[walk] = n / ((time.perf_counter() - t0) / 3)
     solver.accelerations(ds, params)
     a = np.linalg.norm(acc0, axis=1)
-    T = ENCOUNTER_T
+    T = ENCOUNTER_T if args.T is None else args.T
     dt_wanted = np.sqrt(2 * eta * eps / a[a > 0])
     out_err = {}
     if not args.no_errors:

@@ -92,7 +92,7 @@ def load():
         getattr(L, f).argtypes = step
         getattr(L, f).restype = I
     L.gx_sync.argtypes = [P]
-    L.gx_potential_energy_dev.argtypes = [P, P, P, P, P, P, P, I64, D, I, P]
+    L.gx_potential_energy_dev.argtypes = [P, P, P, P, P, P, P, I64, D, I, D, P]
     L.gx_direct_sample_dev.argtypes = [P, P, P, P, P, P, I64, P, I64, D, I, P]
     L.gx_last_seconds.argtypes = [P]
     L.gx_last_seconds.restype = D

@@ -920,7 +920,7 @@ int gx_graph_launch(gx_solver* s, int32_t steps) {
 
 int gx_potential_energy_dev(gx_solver* s, const double* x, const double* y, const double* z, const double* mass,
                             const uint8_t* kind, const uint8_t* alive, int64_t n, double eps, int dark_matter_off,
-                            double* out) {
+                            double c, double* out) {
     if (!s || !out) return set_err(GX_ERR_ARG, "null argument");
     if (n < 0) return set_err(GX_ERR_ARG, "negative n");
     if (s->pending) (void)finish_step(s);
@@ -931,10 +931,13 @@ int gx_potential_energy_dev(gx_solver* s, const double* x, const double* y, cons
     if ((rc = set_single(s, n, std::min(theta_of(s->theta), 0.9), eps))) return rc;  // diagTheta, bh.go:20-23
     for (int attempt = 0; attempt < 6; attempt++) {
         if ((rc = enqueue_build(s, a))) return rc;
-        if ((rc = ensure(s->partial, (size_t)n * 8))) return rc;
+        // n star partials, then 2 per hole (the holes are at most n)
+        if ((rc = ensure(s->partial, (size_t)n * 8 * 3))) return rc;
+        double* part = (double*)s->partial.p;
         hipLaunchKernelGGL(k_potential64, dim3(blocks(n, 256)), dim3(256), 0, s->stream, (const double4*)s->spos.p,
                            (const TravNode*)s->trav.p, (const Mult64*)s->m64.p, (Meta*)s->meta.p, make_batch(s),
-                           !s->monopole, (double*)s->partial.p);
+                           !s->monopole, (const int32_t*)s->order.p, (const int32_t*)s->holes.p, x, y, z, mass, kind,
+                           dark_matter_off, c, part);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
@@ -946,17 +949,24 @@ int gx_potential_energy_dev(gx_solver* s, const double* x, const double* y, cons
             s->levels = kMortonBits;
             continue;
         }
-        if (s->hmeta->nholes > 0)
-            return set_err(GX_ERR_UNSUPPORTED, "potential energy with black holes is not implemented");
-        const long long ns = s->hmeta->nstars;
-        // fixed-order sum on the host (ns doubles)
-        double* h = nullptr;
-        HIPCHK(hipHostMalloc((void**)&h, std::max(ns, 1LL) * 8, hipHostMallocDefault));
-        hipError_t e = hipMemcpy(h, s->partial.p, (size_t)ns * 8, hipMemcpyDeviceToHost);
+        const long long ns = s->hmeta->nstars, nh = s->hmeta->nholes;
+        if (nh > 0) {
+            hipLaunchKernelGGL(k_hole_potential, dim3((unsigned)nh), dim3(256), 0, s->stream, (const double4*)s->spos.p,
+                               (const Meta*)s->meta.p, (const int32_t*)s->holes.p, x, y, z, mass, eps * eps, c,
+                               part + ns);
+            HIPCHK(hipGetLastError());
+        }
+        // fixed-order sums on the host, as potential.go:20-39: stars in sorted
+        // order, then per hole its star pairs and its hole pairs
+        std::vector<double> h((size_t)(ns + 2 * nh));
+        if (!h.empty()) HIPCHK(hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
         double sum = 0;
         for (long long k = 0; k < ns; k++) sum += h[k];
-        (void)hipHostFree(h);
-        if (e != hipSuccess) return set_err(GX_ERR_HIP, "copy of potential partials: %s", hipGetErrorString(e));
+        for (long long g = 0; g < nh; g++) {
+            sum -= h[ns + 2 * g];
+            sum -= h[ns + 2 * g + 1];
+        }
         *out = -sum / 2;
         return GX_OK;
     }

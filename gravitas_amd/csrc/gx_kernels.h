@@ -55,7 +55,12 @@ __global__ void k_diag(long long n, const double* px, const double* py, const do
 __global__ void k_walk64(const double4* spos, const TravNode* trav, const Mult64* m64, const int32_t* order,
                          Meta* meta, Batch batch, int quad, double* ax, double* ay, double* az);
 __global__ void k_potential64(const double4* spos, const TravNode* trav, const Mult64* m64, Meta* meta, Batch batch,
-                              int quad, double* partial);
+                              int quad, const int32_t* order, const int32_t* holes, const double* px, const double* py,
+                              const double* pz, const double* mass, const uint8_t* kind, int dmOff, double c,
+                              double* partial);
+__global__ void k_hole_potential(const double4* spos, const Meta* meta, const int32_t* holes, const double* px,
+                                 const double* py, const double* pz, const double* mass, double eps2, double c,
+                                 double* out);
 __global__ void k_direct_sample(const double* x, const double* y, const double* z, const double* m,
                                 const uint8_t* kind, long long n, const long long* targets, int nt, double eps2,
                                 int dmOff, int splits, double* partial);

@@ -81,13 +81,30 @@ __global__ __launch_bounds__(256) void k_walk64(const double4* __restrict__ spos
     }
 }
 
-// Tree potential: walkPotential, internal/sim/bh/potential.go:45-81.  Writes
-// sm_k * psi_k per star into partial[k] (sorted order); the host-side library
-// sums them in a fixed order.
+// PairPotential, internal/sim/pair.go:37-47 (rs = 0: Newtonian softened).
+__device__ __forceinline__ double pair_potential(double mi, double mj, double r2, double eps2, double rs) {
+    const double re = sqrt(r2 + eps2);
+    if (rs == 0) return -mi * mj / re;
+    if (re >= 1.5 * rs) return -mi * mj / (re - rs);
+    return mi * mj * (-2 / rs + 4 / (rs * rs) * (re - 1.5 * rs));
+}
+
+// SchwarzschildRadius, internal/sim/relativity.go:49-54 (0 when c <= 0).
+__device__ __forceinline__ double schwarzschild(double m, double c) { return c <= 0 ? 0 : 2 * m / (c * c); }
+
+// Tree potential: walkPotential, internal/sim/bh/potential.go:45-81, minus
+// the star's pairs with the holes (holePotential, :84-96).  Writes
+// sm_k * psi_k - holes_k per star into partial[k] (sorted order); the
+// host-side library sums them in a fixed order.
 __global__ __launch_bounds__(256) void k_potential64(const double4* __restrict__ spos,
                                                      const TravNode* __restrict__ trav,
                                                      const Mult64* __restrict__ m64, Meta* meta, Batch batch,
-                                                     int quad, double* __restrict__ partial) {
+                                                     int quad, const int32_t* __restrict__ order,
+                                                     const int32_t* __restrict__ holes,
+                                                     const double* __restrict__ px, const double* __restrict__ py,
+                                                     const double* __restrict__ pz, const double* __restrict__ mass,
+                                                     const uint8_t* __restrict__ kind, int dmOff, double c,
+                                                     double* __restrict__ partial) {
     const long long nst = meta->nstars;
     if (meta->flags & kFlagTreeIncomplete) return;
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -125,7 +142,61 @@ __global__ __launch_bounds__(256) void k_potential64(const double4* __restrict__
         }
         i = nd.next;
     }
-    partial[k] = p.w * psi;
+    // pairs with the holes: the star's source mass (SourceMass, sim.go:93-98)
+    // against each hole, PW radius of the hole (PairRS, pair.go:13-22)
+    double hp = 0;
+    const long long nh = meta->nholes;
+    if (nh > 0) {
+        const int32_t si = order[k];
+        const double xi = px[si], yi = py[si], zi = pz[si];
+        const double mi = (dmOff && kind[si] == kDark) ? 0.0 : mass[si];
+        for (long long h = 0; h < nh; h++) {
+            const int32_t hi = holes[h];
+            const double dx = px[hi] - xi, dy = py[hi] - yi, dz = pz[hi] - zi;
+            hp += pair_potential(mi, mass[hi], dx * dx + dy * dy + dz * dz, eps2, schwarzschild(mass[hi], c));
+        }
+    }
+    partial[k] = p.w * psi - hp;
+}
+
+// The holes' side of potential.go:29-38: per hole, its pairs with every star
+// (one workgroup per hole, tree sum in a fixed order) and with the other
+// holes (sequential).  out[2h] = sum over stars, out[2h+1] = over holes.
+__global__ __launch_bounds__(256) void k_hole_potential(const double4* __restrict__ spos, const Meta* __restrict__ meta,
+                                                        const int32_t* __restrict__ holes,
+                                                        const double* __restrict__ px, const double* __restrict__ py,
+                                                        const double* __restrict__ pz, const double* __restrict__ mass,
+                                                        double eps2, double c, double* __restrict__ out) {
+    __shared__ double red[256];
+    const long long h = blockIdx.x;
+    const long long nh = meta->nholes, nst = meta->nstars;
+    if (h >= nh) return;
+    const int32_t i = holes[h];
+    const double x = px[i], y = py[i], z = pz[i], m = mass[i];
+    const double rs = schwarzschild(m, c);
+    double acc = 0;
+    for (long long k = threadIdx.x; k < nst; k += blockDim.x) {
+        const double4 q = spos[k];
+        const double dx = q.x - x, dy = q.y - y, dz = q.z - z;
+        acc += pair_potential(m, q.w, dx * dx + dy * dy + dz * dz, eps2, rs);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int d = 128; d > 0; d >>= 1) {
+        if ((int)threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double hh = 0;  // two holes: PairRS = 0
+        for (long long g = 0; g < nh; g++) {
+            const int32_t j = holes[g];
+            if (j == i) continue;
+            const double dx = px[j] - x, dy = py[j] - y, dz = pz[j] - z;
+            hh += pair_potential(m, mass[j], dx * dx + dy * dy + dz * dz, eps2, 0);
+        }
+        out[2 * h] = red[0];
+        out[2 * h + 1] = hh;
+    }
 }
 
 // FP64 direct sum for a sample of targets (direct.go:25-48): every workgroup

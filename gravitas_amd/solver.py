@@ -27,6 +27,7 @@ class Params:
     softening: float = 0.0
     dark_matter_off: bool = False
     dt: float = 0.0
+    c: float = 0.0  # speed of light; <= 0 is Newtonian (only the potential's hole pairs use it)
 
 
 _avail = None
@@ -278,7 +279,7 @@ class Solver:
         out = ctypes.c_double()
         _lib.check(self._L.gx_potential_energy_dev(
             self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.m), _ptr(s.kind), _ptr(s.alive), len(s),
-            float(p.softening), int(p.dark_matter_off), ctypes.byref(out)))
+            float(p.softening), int(p.dark_matter_off), float(p.c), ctypes.byref(out)))
         return out.value
 
     # -- FP64 direct-sum oracle kernel (direct.go:25-48) ------------------------------

@@ -199,12 +199,14 @@ int gx_diagnostics_dev(gx_solver *s, const double *x, const double *y, const dou
                        const double *vy, const double *vz, const double *mass, const uint8_t *kind, int64_t n,
                        int dark_matter_off, double *out);
 
-/* bh.Solver.PotentialEnergy (potential.go:12-42) on device arrays: theta
- * capped at 0.9; FP64 tree walk.  Star-star pairs only: returns
- * GX_ERR_UNSUPPORTED if alive holes are present. */
+/* bh.Solver.PotentialEnergy (potential.go:12-42, holePotential :84-96) on
+ * device arrays: theta capped at 0.9; FP64 tree walk for star-star pairs and
+ * exact pair sums (PairPotential, pair.go:37-47) for the pairs with alive
+ * holes; c is Params.C (<= 0: Newtonian, else Paczynski-Wiita radii for
+ * star-hole pairs).  Synchronous. */
 int gx_potential_energy_dev(gx_solver *s, const double *x, const double *y, const double *z, const double *mass,
                             const uint8_t *kind, const uint8_t *alive, int64_t n, double eps, int dark_matter_off,
-                            double *out);
+                            double c, double *out);
 
 /* FP64 direct sum (internal/sim/direct/direct.go:25-48) for nt target
  * indices (device int64) against all n bodies (device arrays); out is a

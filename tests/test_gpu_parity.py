@@ -244,6 +244,23 @@ def test_potential_energy_matches_oracle(gpu):
     assert abs(u - exact) <= 1e-3 * abs(exact)
 
 
+@pytest.mark.parametrize("c", [0.0, 3.0])
+def test_potential_energy_with_holes_matches_oracle(gpu, c):
+    """potential.go:20-39 with holes: star-hole pairs (Paczynski-Wiita for
+    c > 0) and hole-hole pairs beside the tree potential of the stars."""
+    s = ics.plummer(6000, 62)
+    s["kind"][[5, 2500, 5999]] = ics.KIND_HOLE
+    s["m"][[5, 2500, 5999]] = [0.05, 0.02, 0.03]
+    s["alive"] = np.ones(len(s["px"]), np.uint8)
+    s["alive"][[10, 4000]] = 0
+    say(f"[parity] potential energy with holes, c={c}")
+    ds = DeviceState.from_host(s)
+    u = Solver(theta=0.5).potential_energy(ds, Params(softening=0.01, c=c))
+    want = oracle.Solver(theta=0.5).potential_energy(s["px"], s["py"], s["pz"], s["m"], s["kind"], s["alive"],
+                                                     eps=0.01, c=c)
+    assert abs(u - want) <= 1e-12 * abs(want)
+
+
 def test_repeat_is_deterministic(gpu):
     s = ics.clumps(20000, 8, 71, scale=0.01)
     say("[parity] determinism")

@@ -296,6 +296,43 @@ def test_potential_energy_matches_exact():
     assert u2 == -10.0 / 5.0
 
 
+def _pair_model_potential(px, py, pz, m, kind, eps, c):
+    """diag.go:44-64 in numpy: the sum over all pairs of PairPotential with
+    PairRS (pair.go:13-47): Newtonian for star-star and hole-hole pairs,
+    Paczynski-Wiita with the hole's radius 2m/c^2 for star-hole pairs."""
+    n = len(px)
+    i, j = np.triu_indices(n, 1)
+    re = np.sqrt((px[j] - px[i]) ** 2 + (py[j] - py[i]) ** 2 + (pz[j] - pz[i]) ** 2 + eps * eps)
+    hi, hj = kind[i] == ics.KIND_HOLE, kind[j] == ics.KIND_HOLE
+    rs = np.where(hi == hj, 0.0, np.where(hi, 2 * m[i], 2 * m[j]) / (c * c) if c > 0 else 0.0)
+    mm = m[i] * m[j]
+    with np.errstate(all="ignore"):  # both branches are evaluated; only the selected one is finite
+        u = _pw_select(rs, re, mm)
+    return math.fsum(u)
+
+
+def _pw_select(rs, re, mm):
+    return np.where(rs == 0, -mm / re,
+                 np.where(re >= 1.5 * np.maximum(rs, 1e-300), -mm / np.maximum(re - rs, 1e-300),
+                          mm * (-2 / np.maximum(rs, 1e-300) + 4 / np.maximum(rs, 1e-300) ** 2 * (re - 1.5 * rs))))
+
+
+@pytest.mark.parametrize("c", [0.0, 3.0])
+def test_potential_energy_with_holes_matches_pair_model(c):
+    """potential.go:20-39, 84-96: at tiny theta the tree potential with holes
+    equals the exact pair model, star-hole pairs Paczynski-Wiita for c > 0."""
+    s = ics.plummer(1500, 14)
+    s["kind"][[3, 500, 1200]] = ics.KIND_HOLE
+    s["m"][[3, 500, 1200]] = [0.05, 0.02, 0.03]
+    want = _pair_model_potential(s["px"], s["py"], s["pz"], s["m"], s["kind"], 0.02, c)
+    got = oracle.Solver(theta=1e-9).potential_energy(s["px"], s["py"], s["pz"], s["m"], s["kind"], eps=0.02, c=c)
+    assert abs(got - want) <= 1e-10 * abs(want)
+    # the PW terms matter at this c: the Newtonian value differs
+    if c > 0:
+        newt = _pair_model_potential(s["px"], s["py"], s["pz"], s["m"], s["kind"], 0.02, 0.0)
+        assert abs(newt - want) > 1e-6 * abs(want)
+
+
 def test_kepler_orbit_conserves_energy():
     """direct_test.go:26-63 (shortened to 20 orbits): KDK with the direct sum."""
     e = 0.3
