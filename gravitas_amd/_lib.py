@@ -29,7 +29,7 @@ EXPORTS = (
     "gx_sync", "gx_potential_energy_dev", "gx_direct_sample_dev", "gx_last_seconds", "gx_get_stats",
     "gx_phase_ms", "gx_export", "gx_step_dev", "gx_kick_dev", "gx_drift_dev", "gx_solver_tune",
     "gx_batch_accelerations_dev", "gx_batch_step_dev", "gx_graph_capture_step", "gx_graph_launch",
-    "gx_accelerations_active_dev", "gx_block_step_dev", "gx_diagnostics_dev",
+    "gx_accelerations_active_dev", "gx_block_step_dev", "gx_diagnostics_dev", "gx_split_step_dev",
 )
 
 GX_BLOCK_MAX_LEVELS = 8
@@ -109,6 +109,8 @@ def load():
     L.gx_accelerations_active_dev.argtypes = [P, P, P, P, P, P, P, P, I64, D, I, P, P, P]
     L.gx_block_step_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, I64, D, I, D, ctypes.c_int32, D, I, P, P, P,
                                     ctypes.POINTER(GxBlockStats)]
+    L.gx_split_step_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, I64, D, I, D, D, I, P, P, P,
+                                    ctypes.POINTER(ctypes.c_int32)]
     L.gx_diagnostics_dev.argtypes = [P, P, P, P, P, P, P, P, P, I64, I, ctypes.POINTER(ctypes.c_double)]
     L.gx_export.argtypes = [P, I, P, I64]
     L.gx_export.restype = I64

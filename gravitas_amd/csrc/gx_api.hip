@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <vector>
 
@@ -94,6 +95,8 @@ struct gx_solver {
     DevBuf lfull, lpart, lleaf, lcounts;
     // active sets and block steps
     DevBuf actS, blockAct, tOff, tlist, activeMask, rung, rungHist, diag;
+    // split steps near black holes
+    DevBuf nearFlag, nearList, tileCnt, splitScalars, farF, fastF;
     unsigned long long* hhist = nullptr;  // pinned mirror of hist
     int cf = 2048, cp = 512, cl = 256;  // FP32-walk list capacities per group (grow on demand)
     long long capNodes = 0;
@@ -201,7 +204,8 @@ void gx_solver_free(gx_solver* s) {
                       &s->partial,  &s->lfull,   &s->lpart,   &s->lleaf,     &s->lcounts,   &s->instCfg,
                       &s->boxEnc,   &s->starCount, &s->starOff, &s->groupOff, &s->root,      &s->half,
                       &s->actS,     &s->blockAct, &s->tOff,    &s->tlist,     &s->activeMask, &s->rung,
-                      &s->rungHist, &s->diag};
+                      &s->rungHist, &s->diag,     &s->nearFlag, &s->nearList, &s->tileCnt,  &s->splitScalars,
+                      &s->farF,     &s->fastF};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto& e : s->ev)
@@ -338,6 +342,25 @@ void mark(gx_solver* s, int i) {
     if (s->timing) (void)hipEventRecord(s->ev[i], s->stream);
 }
 
+// collect (build.go:62-113): stars in state order into order[], alive holes
+// into holes[], counts and boxes into Meta and the instance arrays.  Needs
+// reserve() first.
+void enqueue_collect(gx_solver* s, const StepArgs& a) {
+    const long long n = a.n;
+    const long long nb = (n + kCollectTile - 1) / kCollectTile;
+    hipStream_t st = s->stream;
+    Meta* meta = (Meta*)s->meta.p;
+    const Batch batch = make_batch(s);
+    hipLaunchKernelGGL(k_step_init, dim3(blocks(s->B, 256)), dim3(256), 0, st, meta, batch);
+    hipLaunchKernelGGL(k_collect_count, dim3(blocks(n, kCollectTile)), dim3(256), 0, st, a.x, a.y, a.z, a.kind,
+                       a.alive, n, (int*)s->blockStar.p, (int*)s->blockHole.p, batch);
+    hipLaunchKernelGGL(k_collect_scan, dim3(1), dim3(1024), 0, st, (int)std::max(nb, 1LL), (int*)s->blockStar.p,
+                       (int*)s->blockHole.p, meta, batch);
+    hipLaunchKernelGGL(k_collect_scatter, dim3(blocks(n, kCollectTile)), dim3(256), 0, st, a.kind, a.alive, n,
+                       (const int*)s->blockStar.p, (const int*)s->blockHole.p, (int32_t*)s->order.p,
+                       (int32_t*)s->holes.p);
+}
+
 // Build the trees of all instances (collect .. moments).  theta is baked into
 // crit2 (build.go:209-211).
 int enqueue_build(gx_solver* s, const StepArgs& a) {
@@ -347,17 +370,9 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
     hipStream_t st = s->stream;
     Meta* meta = (Meta*)s->meta.p;
     const Batch batch = make_batch(s);
-    const long long nb = (n + kCollectTile - 1) / kCollectTile;
     const int ntiles = std::max(1, (int)((n + kSortTile - 1) / kSortTile));
     mark(s, 0);
-    hipLaunchKernelGGL(k_step_init, dim3(blocks(s->B, 256)), dim3(256), 0, st, meta, batch);
-    hipLaunchKernelGGL(k_collect_count, dim3(blocks(n, kCollectTile)), dim3(256), 0, st, a.x, a.y, a.z, a.kind,
-                       a.alive, n, (int*)s->blockStar.p, (int*)s->blockHole.p, batch);
-    hipLaunchKernelGGL(k_collect_scan, dim3(1), dim3(1024), 0, st, (int)std::max(nb, 1LL), (int*)s->blockStar.p,
-                       (int*)s->blockHole.p, meta, batch);
-    hipLaunchKernelGGL(k_collect_scatter, dim3(blocks(n, kCollectTile)), dim3(256), 0, st, a.kind, a.alive, n,
-                       (const int*)s->blockStar.p, (const int*)s->blockHole.p, (int32_t*)s->order.p,
-                       (int32_t*)s->holes.p);
+    enqueue_collect(s, a);
     mark(s, 1);
     hipLaunchKernelGGL(k_keys, dim3(blocks(n, 256)), dim3(256), 0, st, a.x, a.y, a.z, (const int32_t*)s->order.p,
                        meta, batch, (uint64_t*)s->keys.p);
@@ -693,9 +708,150 @@ int block_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, dou
     return GX_OK;
 }
 
+// One step with black holes, stepSplit (internal/sim/split.go:164-199) for a
+// state without gas, with non-spinning holes and instant gravity; without
+// alive holes it is step()'s plain KDK (sim.go:266-277).  With c > 0 the pairs
+// with holes use Paczynski-Wiita radii and drifts use the proper velocity.
+int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, double dt, double c, double eps,
+               int accValid, int32_t* substepsOut) {
+    const long long n = a.n;
+    int sub = 1;
+    if (substepsOut) *substepsOut = 1;
+    if (n == 0) return GX_OK;
+    double* x = (double*)a.x;
+    double* y = (double*)a.y;
+    double* z = (double*)a.z;
+    const long long nb = std::max(1LL, (n + kCollectTile - 1) / kCollectTile);
+    int rc;
+    if ((rc = reserve(s, n)) || (rc = ensure(s->nearFlag, (size_t)n)) || (rc = ensure(s->nearList, (size_t)n * 4)) ||
+        (rc = ensure(s->tileCnt, (size_t)nb * 4)) || (rc = ensure(s->splitScalars, 16)) ||
+        (rc = ensure(s->farF, (size_t)n * 24)) || (rc = ensure(s->fastF, (size_t)n * 24)))
+        return rc;
+    if (!accValid && (rc = run_dev(s, a))) return rc;  // initForces
+    hipStream_t st = s->stream;
+    const unsigned grid = (unsigned)std::min<long long>(blocks(n, 256), 4096);
+    const double eps2 = eps * eps;
+    // the alive holes in state order (collectHoles, sim.go:291-299)
+    enqueue_collect(s, a);
+    HIPCHK(hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const long long nh = s->hmeta->nholes;
+    if (nh == 0) {
+        // kick, drift (sim.go:316-343: dt / sqrt(1 + u^2 / c^2) with c > 0), forces, kick
+        if ((rc = kick(s, vx, vy, vz, a.ax, a.ay, a.az, n, 0.5 * dt))) return rc;
+        hipLaunchKernelGGL(k_split_drift_far, dim3(grid), dim3(256), 0, st, n, a.kind, (const uint8_t*)nullptr, x, y, z,
+                           (const double*)vx, (const double*)vy, (const double*)vz, dt, c);
+        HIPCHK(hipGetLastError());
+        if ((rc = run_dev(s, a)) || (rc = kick(s, vx, vy, vz, a.ax, a.ay, a.az, n, 0.5 * dt))) return rc;
+        HIPCHK(hipStreamSynchronize(st));
+        return GX_OK;
+    }
+    const Meta* meta = (const Meta*)s->meta.p;
+    const int32_t* holes = (const int32_t*)s->holes.p;
+    uint8_t* nearFlag = (uint8_t*)s->nearFlag.p;
+    int32_t* nearList = (int32_t*)s->nearList.p;
+    long long* nnDev = (long long*)s->splitScalars.p;
+    unsigned long long* tauEnc = (unsigned long long*)s->splitScalars.p + 1;
+    double* farX = (double*)s->farF.p;
+    double* farY = farX + n;
+    double* farZ = farY + n;
+    double* fastX = (double*)s->fastF.p;
+    double* fastY = fastX + n;
+    double* fastZ = fastY + n;
+    // classify (split.go:39-80)
+    const double inf = INFINITY;
+    unsigned long long infBits;
+    memcpy(&infBits, &inf, 8);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, tauEnc, 1, infBits | 0x8000000000000000ull);
+    hipLaunchKernelGGL(k_split_classify, dim3(grid), dim3(256), 0, st, n, (const double*)x, (const double*)y,
+                       (const double*)z, a.m, a.kind, holes, meta, eps2, c, 20.0 * dt, nearFlag, tauEnc);
+    hipLaunchKernelGGL(k_flag_count, dim3((unsigned)nb), dim3(256), 0, st, n, (const uint8_t*)nearFlag,
+                       (int*)s->tileCnt.p);
+    hipLaunchKernelGGL(k_flag_scan, dim3(1), dim3(1024), 0, st, (int)nb, (int*)s->tileCnt.p, nnDev);
+    hipLaunchKernelGGL(k_flag_scatter, dim3((unsigned)nb), dim3(256), 0, st, n, (const uint8_t*)nearFlag,
+                       (const int*)s->tileCnt.p, nearList);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(s->hhist, s->splitScalars.p, 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const unsigned long long te = s->hhist[1];
+    const unsigned long long tb = (te >> 63) ? (te & 0x7fffffffffffffffull) : ~te;
+    double tauMin;
+    memcpy(&tauMin, &tb, 8);
+    if (!std::isinf(tauMin)) {
+        while ((double)sub * tauMin < 20.0 * dt && sub < 256) sub *= 2;
+    }
+    if (substepsOut) *substepsOut = sub;
+    auto outer = [&]() {  // outerHoleForces, split.go:82-117
+        hipLaunchKernelGGL(k_split_far, dim3(grid), dim3(256), 0, st, n, (const double*)x, (const double*)y,
+                           (const double*)z, a.m, a.kind, (const uint8_t*)nearFlag, holes, meta, eps2, c, farX, farY,
+                           farZ);
+        hipLaunchKernelGGL(k_split_hole_sum, dim3((unsigned)nh), dim3(256), 0, st, n, (const double*)x,
+                           (const double*)y, (const double*)z, a.m, a.kind, a.dmOff, (const uint8_t*)nearFlag,
+                           (const int32_t*)nearList, (const long long*)nnDev, holes, meta, eps2, c, 0, farX, farY,
+                           farZ);
+    };
+    auto fast = [&]() {  // fastForces, split.go:141-160
+        hipLaunchKernelGGL(k_split_fast, dim3(grid), dim3(256), 0, st, (const int32_t*)nearList,
+                           (const long long*)nnDev, (const double*)x, (const double*)y, (const double*)z, a.m, a.kind,
+                           holes, meta, eps2, c, fastX, fastY, fastZ);
+        hipLaunchKernelGGL(k_split_hole_sum, dim3((unsigned)nh), dim3(256), 0, st, n, (const double*)x,
+                           (const double*)y, (const double*)z, a.m, a.kind, a.dmOff, (const uint8_t*)nearFlag,
+                           (const int32_t*)nearList, (const long long*)nnDev, holes, meta, eps2, c, 1, fastX, fastY,
+                           fastZ);
+    };
+    auto kick_outer = [&](double h) {  // kickOuter, split.go:201-208
+        hipLaunchKernelGGL(k_split_kick_outer, dim3(grid), dim3(256), 0, st, n, vx, vy, vz, (const double*)a.ax,
+                           (const double*)a.ay, (const double*)a.az, (const double*)farX, (const double*)farY,
+                           (const double*)farZ, h);
+    };
+    auto kick_fast = [&](double h) {  // kickFast, split.go:210-219: near stars, then holes
+        hipLaunchKernelGGL(k_split_kick_list, dim3(grid), dim3(256), 0, st, (const int32_t*)nearList, 0LL,
+                           (const long long*)nnDev, vx, vy, vz, (const double*)fastX, (const double*)fastY,
+                           (const double*)fastZ, h);
+        hipLaunchKernelGGL(k_split_kick_list, dim3(1), dim3(256), 0, st, holes, nh, (const long long*)nullptr, vx, vy,
+                           vz, (const double*)fastX, (const double*)fastY, (const double*)fastZ, h);
+    };
+    outer();
+    kick_outer(0.5 * dt);
+    const double h = dt / (double)sub;
+    fast();
+    for (int k = 0; k < sub; k++) {
+        kick_fast(0.5 * h);
+        hipLaunchKernelGGL(k_split_drift_list, dim3(grid), dim3(256), 0, st, (const int32_t*)nearList, 0LL,
+                           (const long long*)nnDev, x, y, z, (const double*)vx, (const double*)vy, (const double*)vz,
+                           h, c);
+        hipLaunchKernelGGL(k_split_drift_list, dim3(1), dim3(256), 0, st, holes, nh, (const long long*)nullptr, x, y, z,
+                           (const double*)vx, (const double*)vy, (const double*)vz, h, c);
+        fast();
+        kick_fast(0.5 * h);
+    }
+    hipLaunchKernelGGL(k_split_drift_far, dim3(grid), dim3(256), 0, st, n, a.kind, (const uint8_t*)nearFlag, x, y, z,
+                       (const double*)vx, (const double*)vy, (const double*)vz, dt, c);
+    HIPCHK(hipGetLastError());
+    if ((rc = run_dev(s, a))) return rc;  // the slow forces at the new positions
+    outer();
+    kick_outer(0.5 * dt);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return GX_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int gx_split_step_dev(gx_solver* s, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                      const double* mass, const uint8_t* kind, const uint8_t* alive, int64_t n, double eps,
+                      int dark_matter_off, double dt, double c, int acc_valid, double* ax, double* ay, double* az,
+                      int32_t* substeps) {
+    if (!s) return set_err(GX_ERR_ARG, "null solver");
+    const StepArgs a = step_args(x, y, z, mass, kind, alive, n, dark_matter_off, ax, ay, az);
+    int rc;
+    if ((rc = check_arrays(a)) || (n > 0 && (!vx || !vy || !vz) && (rc = set_err(GX_ERR_ARG, "null array"))) ||
+        (rc = set_single(s, n, s->theta, eps)))
+        return rc;
+    return split_step(s, a, vx, vy, vz, dt, c, eps, acc_valid, substeps);
+}
 
 int gx_accelerations_active_dev(gx_solver* s, const double* x, const double* y, const double* z,
                                 const double* mass, const uint8_t* kind, const uint8_t* alive,

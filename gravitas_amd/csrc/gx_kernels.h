@@ -66,6 +66,34 @@ __global__ void k_direct_sample(const double* x, const double* y, const double* 
                                 int dmOff, int splits, double* partial);
 __global__ void k_direct_reduce(const double* partial, int nt, int splits, double* out);
 
+// gx_split.hip
+__global__ void k_split_classify(long long n, const double* x, const double* y, const double* z, const double* mass,
+                                 const uint8_t* kind, const int32_t* holes, const Meta* meta, double eps2, double c,
+                                 double limit, uint8_t* nearFlag, unsigned long long* tauEnc);
+__global__ void k_flag_count(long long n, const uint8_t* flag, int* tileCnt);
+__global__ void k_flag_scan(int nb, int* tileCnt, long long* total);
+__global__ void k_flag_scatter(long long n, const uint8_t* flag, const int* tileOff, int32_t* list);
+__global__ void k_split_far(long long n, const double* x, const double* y, const double* z, const double* mass,
+                            const uint8_t* kind, const uint8_t* nearFlag, const int32_t* holes, const Meta* meta,
+                            double eps2, double c, double* fx, double* fy, double* fz);
+__global__ void k_split_fast(const int32_t* nearList, const long long* nn, const double* x, const double* y,
+                             const double* z, const double* mass, const uint8_t* kind, const int32_t* holes,
+                             const Meta* meta, double eps2, double c, double* fx, double* fy, double* fz);
+__global__ void k_split_hole_sum(long long n, const double* x, const double* y, const double* z, const double* mass,
+                                 const uint8_t* kind, int dmOff, const uint8_t* nearFlag, const int32_t* nearList,
+                                 const long long* nn, const int32_t* holes, const Meta* meta, double eps2, double c,
+                                 int nearMode, double* fx, double* fy, double* fz);
+__global__ void k_split_kick_outer(long long n, double* vx, double* vy, double* vz, const double* ax, const double* ay,
+                                   const double* az, const double* fx, const double* fy, const double* fz, double h);
+__global__ void k_split_kick_list(const int32_t* list, long long cntHost, const long long* cntDev, double* vx,
+                                  double* vy, double* vz, const double* fx, const double* fy, const double* fz,
+                                  double h);
+__global__ void k_split_drift_list(const int32_t* list, long long cntHost, const long long* cntDev, double* x,
+                                   double* y, double* z, const double* vx, const double* vy, const double* vz,
+                                   double dt, double c);
+__global__ void k_split_drift_far(long long n, const uint8_t* kind, const uint8_t* nearFlag, double* x, double* y,
+                                  double* z, const double* vx, const double* vy, const double* vz, double dt, double c);
+
 // gx_walk32.hip
 // Per-group interaction lists written by the FP32 walk's traversal and read by
 // its evaluation: full-mask cells as bare node indices, partial-mask cells and

@@ -7,6 +7,7 @@
 // star in Morton order, stackless skip-pointer traversal.
 #include "gx_common.h"
 #include "gx_kernels.h"
+#include "gx_pair.h"
 
 namespace gx {
 
@@ -80,17 +81,6 @@ __global__ __launch_bounds__(256) void k_walk64(const double4* __restrict__ spos
         atomicAdd(&meta->pc, npc);
     }
 }
-
-// PairPotential, internal/sim/pair.go:37-47 (rs = 0: Newtonian softened).
-__device__ __forceinline__ double pair_potential(double mi, double mj, double r2, double eps2, double rs) {
-    const double re = sqrt(r2 + eps2);
-    if (rs == 0) return -mi * mj / re;
-    if (re >= 1.5 * rs) return -mi * mj / (re - rs);
-    return mi * mj * (-2 / rs + 4 / (rs * rs) * (re - 1.5 * rs));
-}
-
-// SchwarzschildRadius, internal/sim/relativity.go:49-54 (0 when c <= 0).
-__device__ __forceinline__ double schwarzschild(double m, double c) { return c <= 0 ? 0 : 2 * m / (c * c); }
 
 // Tree potential: walkPotential, internal/sim/bh/potential.go:45-81, minus
 // the star's pairs with the holes (holePotential, :84-96).  Writes

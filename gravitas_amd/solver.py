@@ -27,7 +27,7 @@ class Params:
     softening: float = 0.0
     dark_matter_off: bool = False
     dt: float = 0.0
-    c: float = 0.0  # speed of light; <= 0 is Newtonian (only the potential's hole pairs use it)
+    c: float = 0.0  # speed of light; <= 0 is Newtonian (used by the pairs with holes and their drifts)
 
 
 _avail = None
@@ -247,6 +247,20 @@ class Solver:
         return {"levels": st.levels, "events": st.events, "bodies": st.bodies, "targets": st.targets,
                 "rung_count": list(st.rung_count)[:st.levels], "tick": list(st.tick)[:ev],
                 "active": list(st.active)[:ev]}
+
+    def split_step(self, s: DeviceState, p: Params, acc_valid: bool = True) -> int:
+        """One step of p.dt with black holes (internal/sim/split.go:164-199):
+        tree forces between stars, direct sums for every pair with a hole,
+        power-of-two substeps for the holes and the stars near them; p.c > 0
+        switches on the Paczynski-Wiita pairs and relativistic drifts.
+        Returns the number of substeps."""
+        self._configure()
+        sub = ctypes.c_int32(0)
+        _lib.check(self._L.gx_split_step_dev(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.vx), _ptr(s.vy), _ptr(s.vz), _ptr(s.m),
+            _ptr(s.kind), _ptr(s.alive), len(s), float(p.softening), int(p.dark_matter_off), float(p.dt),
+            float(p.c), int(bool(acc_valid)), _ptr(s.ax), _ptr(s.ay), _ptr(s.az), ctypes.byref(sub)))
+        return sub.value
 
     def diagnostics(self, s: DeviceState, p: Params) -> dict:
         """FP64 kinetic energy of the sources, momentum, angular momentum,

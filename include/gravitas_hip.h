@@ -187,6 +187,25 @@ int gx_block_step_dev(gx_solver *s, double *x, double *y, double *z, double *vx,
                       int dark_matter_off, double T, int32_t levels, double eta, int acc_valid, double *ax,
                       double *ay, double *az, gx_block_stats *out);
 
+/* One step with black holes, on device arrays: stepSplit
+ * (internal/sim/split.go:164-199), the impulse/RESPA scheme.  Star-star forces
+ * come from the tree once per step (the solver's walk); every force involving
+ * an alive hole is a direct sum.  Stars whose timescale at a hole,
+ * sqrt(r_eps / a), is below 20 steps dt, and all holes, take power-of-two
+ * substeps (at most 256) from the shortest such timescale; far stars get the
+ * holes' pull at the outer kicks.  Without alive holes it is step()'s KDK
+ * (sim.go:266-277).  c is Params.C: <= 0 Newtonian; > 0 Paczynski-Wiita
+ * radii for the pairs with holes and drifts with the proper velocity u/gamma
+ * (relativity.go).  Restated for states without gas, with non-spinning holes
+ * and instant gravity (the reference's frame dragging and retarded sources
+ * are not part of this path).  If acc_valid is 0 the forces are computed
+ * first.  *substeps (optional) receives the substep count (Simulation.Substeps).
+ * Synchronous. */
+int gx_split_step_dev(gx_solver *s, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                      const double *mass, const uint8_t *kind, const uint8_t *alive, int64_t n, double eps,
+                      int dark_matter_off, double dt, double c, int acc_valid, double *ax, double *ay, double *az,
+                      int32_t *substeps);
+
 #define GX_DIAG_VALUES 11
 
 /* FP64 diagnostics of internal/sim/diag.go on device arrays, summed on the

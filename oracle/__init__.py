@@ -109,6 +109,10 @@ def lib():
         L.or_block_kick.argtypes = [ctypes.c_int64, _D, _D, _D, _D, _D, _D, _U8, _U8, _I8, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.or_diagnostics.argtypes = [ctypes.c_int64, _D, _D, _D, _D, _D, _D, _D, _U8, ctypes.c_int, _D]
+        L.or_split_step.restype = ctypes.c_int
+        L.or_split_step.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _D, _D, _D, _U8, _U8, ctypes.c_int64,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_int, _D, _D, _D]
         assert L.or_node_size() == NODE_DTYPE.itemsize
         _lib = L
     return _lib
@@ -364,3 +368,28 @@ def diagnostics(px, py, pz, vx, vy, vz, m, kind, dm_off=False):
     k = np.ascontiguousarray(kind, np.uint8)
     lib().or_diagnostics(len(arrs[0]), *[_dp(a) for a in arrs], _u8(k), int(dm_off), _dp(out))
     return out
+
+
+def split_step(solver, state, m, kind, alive, eps, dt, c=0.0, dm_off=False, acc_valid=True):
+    """One step with black holes (internal/sim/split.go:164-199: RESPA
+    substeps for stars near holes, no gas, non-spinning holes, instant
+    gravity) on the float64 arrays px..vz and 'acc' of ``state``, in place;
+    the slow forces come from ``solver`` (an oracle Solver).  Returns the
+    number of substeps."""
+    L = lib()
+    n = len(state["px"])
+    for k in ("px", "py", "pz", "vx", "vy", "vz"):
+        state[k] = np.ascontiguousarray(state[k], np.float64)
+    if "acc" not in state:
+        state["acc"] = np.zeros((n, 3))
+    ax, ay, az = (np.ascontiguousarray(state["acc"][:, c_]) for c_ in range(3))
+    m = np.ascontiguousarray(m, np.float64)
+    kind = np.ascontiguousarray(kind, np.uint8)
+    alive = np.ascontiguousarray(alive, np.uint8)
+    L.or_set_monopole(solver._h, int(solver.monopole_only))
+    L.or_set_workers(solver._h, int(solver.workers))
+    sub = L.or_split_step(solver._h, _dp(state["px"]), _dp(state["py"]), _dp(state["pz"]), _dp(state["vx"]),
+                          _dp(state["vy"]), _dp(state["vz"]), _dp(m), _u8(kind), _u8(alive), n, float(solver.theta),
+                          float(eps), int(dm_off), float(dt), float(c), int(bool(acc_valid)), _dp(ax), _dp(ay), _dp(az))
+    state["acc"] = np.stack([ax, ay, az], axis=1)
+    return sub

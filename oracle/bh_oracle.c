@@ -732,6 +732,256 @@ void or_diagnostics(int64_t n, const double *px, const double *py, const double 
 }
 
 /* ------------------------------------------------------------------ */
+/* Multiple time stepping near black holes: internal/sim/split.go       */
+/* (no gas, non-spinning holes, instant gravity)                         */
+/* ------------------------------------------------------------------ */
+/* PairForce, pair.go:26-34 */
+static double pair_force(double m, double r2, double eps2, double rs) {
+    double re2 = r2 + eps2;
+    double re = sqrt(re2);
+    if (rs == 0) return m / (re2 * re);
+    double den = go_max(re - rs, rs / 2);
+    return m / (den * den * re);
+}
+/* gamma, relativity.go:14-19 */
+static double gamma_of(double ux, double uy, double uz, double c) {
+    if (c <= 0) return 1;
+    return sqrt(1 + (ux * ux + uy * uy + uz * uz) / (c * c));
+}
+/* PairRS, pair.go:13-22 */
+static double pair_rs(int ki, double mi, int kj, double mj, double c) {
+    if (c <= 0 || (ki == KIND_HOLE) == (kj == KIND_HOLE)) return 0;
+    return ki == KIND_HOLE ? schwarzschild(mi, c) : schwarzschild(mj, c);
+}
+
+typedef struct {
+    double *px, *py, *pz, *vx, *vy, *vz;
+    const double *mass;
+    const uint8_t *kind;
+    int64_t n;
+    double eps2, c;
+    int dm_off;
+    const int32_t *holes;
+    int64_t nh;
+    uint8_t *near;
+    int32_t *nearList;
+    int64_t nn;
+} split_ctx;
+
+static double source_mass(const split_ctx *q, int64_t i) {
+    return (q->kind[i] == KIND_DARK && q->dm_off) ? 0.0 : q->mass[i];
+}
+
+/* holeTimescale, split.go:27-36 */
+static double hole_timescale(const split_ctx *q, int64_t i, int64_t h) {
+    double dx = q->px[i] - q->px[h], dy = q->py[i] - q->py[h], dz = q->pz[i] - q->pz[h];
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double rs = pair_rs(q->kind[i], q->mass[i], KIND_HOLE, q->mass[h], q->c);
+    double a = pair_force(q->mass[h], r2, q->eps2, rs) * sqrt(r2);
+    if (a == 0) return INFINITY;
+    return sqrt(sqrt(r2 + q->eps2) / a);
+}
+
+/* holePull, split.go:120-137 (instant gravity: the hole's current position) */
+static void hole_pull(const split_ctx *q, int64_t i, double out[3]) {
+    double ax = 0, ay = 0, az = 0;
+    for (int64_t k = 0; k < q->nh; k++) {
+        int64_t h = q->holes[k];
+        if (h == i) continue;
+        double x = q->px[i], y = q->py[i], z = q->pz[i];
+        double dx = q->px[h] - x, dy = q->py[h] - y, dz = q->pz[h] - z;
+        double rs = pair_rs(q->kind[i], q->mass[i], KIND_HOLE, q->mass[h], q->c);
+        double f = pair_force(q->mass[h], dx * dx + dy * dy + dz * dz, q->eps2, rs);
+        ax = ax + f * dx;
+        ay = ay + f * dy;
+        az = az + f * dz;
+    }
+    out[0] = ax; out[1] = ay; out[2] = az;
+}
+
+/* classify, split.go:39-80: near flags, near list (index order), substeps */
+static int classify(split_ctx *q, double dt) {
+    double limit = 20.0 * dt, tau_min = INFINITY;
+    q->nn = 0;
+    for (int64_t i = 0; i < q->n; i++) {
+        int near = 0;
+        if (q->kind[i] != KIND_HOLE) {
+            for (int64_t k = 0; k < q->nh; k++) {
+                double tau = hole_timescale(q, i, q->holes[k]);
+                if (tau < limit) {
+                    near = 1;
+                    tau_min = go_min(tau_min, tau);
+                }
+            }
+        }
+        q->near[i] = (uint8_t)near;
+        if (near) q->nearList[q->nn++] = (int32_t)i;
+    }
+    for (int64_t a = 0; a < q->nh; a++)
+        for (int64_t b = a + 1; b < q->nh; b++) {
+            double tau = hole_timescale(q, q->holes[a], q->holes[b]);
+            if (tau < limit) tau_min = go_min(tau_min, tau);
+        }
+    if (isinf(tau_min)) return 1;
+    int k = 1;
+    while ((double)k * tau_min < 20.0 * dt && k < 256) k *= 2;
+    return k;
+}
+
+/* outerHoleForces, split.go:82-117 (hole sums in index order) */
+static void outer_hole_forces(const split_ctx *q, double *fx, double *fy, double *fz) {
+    for (int64_t i = 0; i < q->n; i++) {
+        fx[i] = fy[i] = fz[i] = 0;
+        if (q->kind[i] == KIND_HOLE || q->near[i]) continue;
+        double a[3];
+        hole_pull(q, i, a);
+        fx[i] = a[0]; fy[i] = a[1]; fz[i] = a[2];
+    }
+    for (int64_t k = 0; k < q->nh; k++) {
+        int64_t h = q->holes[k];
+        double x = q->px[h], y = q->py[h], z = q->pz[h], rs = schwarzschild(q->mass[h], q->c);
+        double ax = 0, ay = 0, az = 0;
+        for (int64_t i = 0; i < q->n; i++) {
+            if (q->kind[i] == KIND_HOLE || q->near[i]) continue;
+            double dx = q->px[i] - x, dy = q->py[i] - y, dz = q->pz[i] - z;
+            double f = pair_force(source_mass(q, i), dx * dx + dy * dy + dz * dz, q->eps2, rs);
+            ax = ax + f * dx;
+            ay = ay + f * dy;
+            az = az + f * dz;
+        }
+        fx[h] = fx[h] + ax;
+        fy[h] = fy[h] + ay;
+        fz[h] = fz[h] + az;
+    }
+}
+
+/* fastForces, split.go:141-160 */
+static void fast_forces(const split_ctx *q, double *fx, double *fy, double *fz) {
+    for (int64_t j = 0; j < q->nn; j++) {
+        int64_t i = q->nearList[j];
+        double a[3];
+        hole_pull(q, i, a);
+        fx[i] = a[0]; fy[i] = a[1]; fz[i] = a[2];
+    }
+    for (int64_t k = 0; k < q->nh; k++) {
+        int64_t h = q->holes[k];
+        double x = q->px[h], y = q->py[h], z = q->pz[h], rs = schwarzschild(q->mass[h], q->c);
+        double a[3];
+        hole_pull(q, h, a);
+        double ax = a[0], ay = a[1], az = a[2];
+        for (int64_t j = 0; j < q->nn; j++) {
+            int64_t i = q->nearList[j];
+            double dx = q->px[i] - x, dy = q->py[i] - y, dz = q->pz[i] - z;
+            double f = pair_force(source_mass(q, i), dx * dx + dy * dy + dz * dz, q->eps2, rs);
+            ax = ax + f * dx;
+            ay = ay + f * dy;
+            az = az + f * dz;
+        }
+        fx[h] = ax; fy[h] = ay; fz[h] = az;
+    }
+}
+
+static void kick_idx(const split_ctx *q, const int32_t *idx, int64_t cnt, const double *fx, const double *fy,
+                     const double *fz, double h) {
+    for (int64_t j = 0; j < cnt; j++) {
+        int64_t i = idx[j];
+        q->vx[i] += h * fx[i];
+        q->vy[i] += h * fy[i];
+        q->vz[i] += h * fz[i];
+    }
+}
+
+/* driftIdx, split.go:214-222 */
+static void drift_idx(const split_ctx *q, const int32_t *idx, int64_t cnt, double dt) {
+    for (int64_t j = 0; j < cnt; j++) {
+        int64_t i = idx[j];
+        double f = dt / gamma_of(q->vx[i], q->vy[i], q->vz[i], q->c);
+        q->px[i] += f * q->vx[i];
+        q->py[i] += f * q->vy[i];
+        q->pz[i] += f * q->vz[i];
+    }
+}
+
+/* stepSplit, split.go:164-199, for a state with alive holes and no gas; the
+ * slow forces are the tree's (theta, eps, dm_off).  Returns the substeps. */
+int or_split_step(or_solver *s, double *px, double *py, double *pz, double *vx, double *vy, double *vz,
+                  const double *mass, const uint8_t *kind, const uint8_t *alive, int64_t n, double theta, double eps,
+                  int dm_off, double dt, double c, int acc_valid, double *ax, double *ay, double *az) {
+    split_ctx q = {px, py, pz, vx, vy, vz, mass, kind, n, eps * eps, c, dm_off, NULL, 0, NULL, NULL, 0};
+    int32_t *holes = malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    q.near = malloc((size_t)(n > 0 ? n : 1));
+    q.nearList = malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    double *farx = malloc((size_t)(n > 0 ? n : 1) * 8 * 6);
+    double *fary = farx + n, *farz = fary + n, *fsx = farz + n, *fsy = fsx + n, *fsz = fsy + n;
+    for (int64_t i = 0; i < n; i++)
+        if (alive[i] && kind[i] == KIND_HOLE) holes[q.nh++] = (int32_t)i; /* collectHoles, sim.go:291-299 */
+    q.holes = holes;
+    theta = theta == 0 ? DEFAULT_THETA : theta;
+    if (!acc_valid) or_accelerations(s, px, py, pz, mass, kind, alive, n, theta, eps, dm_off, s->monopole, s->workers,
+                                     ax, ay, az);
+    if (q.nh == 0) {
+        /* step() without holes (sim.go:266-277): kick, drift (:316-343), forces, kick */
+        or_kick(n, vx, vy, vz, ax, ay, az, 0.5 * dt);
+        if (c <= 0) {
+            or_drift(n, px, py, pz, vx, vy, vz, dt);
+        } else {
+            double invC2 = 1 / (c * c);
+            for (int64_t i = 0; i < n; i++) {
+                double ux = vx[i], uy = vy[i], uz = vz[i];
+                double f = dt / sqrt(1 + (ux * ux + uy * uy + uz * uz) * invC2);
+                px[i] += f * ux;
+                py[i] += f * uy;
+                pz[i] += f * uz;
+            }
+        }
+        or_accelerations(s, px, py, pz, mass, kind, alive, n, theta, eps, dm_off, s->monopole, s->workers, ax, ay, az);
+        or_kick(n, vx, vy, vz, ax, ay, az, 0.5 * dt);
+        free(holes);
+        free(q.near);
+        free(q.nearList);
+        free(farx);
+        return 1;
+    }
+    int sub = classify(&q, dt);
+    outer_hole_forces(&q, farx, fary, farz);
+    for (int64_t i = 0; i < n; i++) { /* kickOuter, split.go:201-208 */
+        vx[i] += 0.5 * dt * (ax[i] + farx[i]);
+        vy[i] += 0.5 * dt * (ay[i] + fary[i]);
+        vz[i] += 0.5 * dt * (az[i] + farz[i]);
+    }
+    double h = dt / (double)sub;
+    fast_forces(&q, fsx, fsy, fsz);
+    for (int k = 0; k < sub; k++) {
+        kick_idx(&q, q.nearList, q.nn, fsx, fsy, fsz, 0.5 * h);
+        kick_idx(&q, holes, q.nh, fsx, fsy, fsz, 0.5 * h);
+        drift_idx(&q, q.nearList, q.nn, h);
+        drift_idx(&q, holes, q.nh, h);
+        fast_forces(&q, fsx, fsy, fsz);
+        kick_idx(&q, q.nearList, q.nn, fsx, fsy, fsz, 0.5 * h);
+        kick_idx(&q, holes, q.nh, fsx, fsy, fsz, 0.5 * h);
+    }
+    for (int64_t i = 0; i < n; i++) { /* driftFar, split.go:225-238 */
+        if (kind[i] == KIND_HOLE || q.near[i]) continue;
+        double f = dt / gamma_of(vx[i], vy[i], vz[i], c);
+        px[i] += f * vx[i];
+        py[i] += f * vy[i];
+        pz[i] += f * vz[i];
+    }
+    or_accelerations(s, px, py, pz, mass, kind, alive, n, theta, eps, dm_off, s->monopole, s->workers, ax, ay, az);
+    outer_hole_forces(&q, farx, fary, farz);
+    for (int64_t i = 0; i < n; i++) {
+        vx[i] += 0.5 * dt * (ax[i] + farx[i]);
+        vy[i] += 0.5 * dt * (ay[i] + fary[i]);
+        vz[i] += 0.5 * dt * (az[i] + farz[i]);
+    }
+    free(holes);
+    free(q.near);
+    free(q.nearList);
+    free(farx);
+    return sub;
+}
+
+/* ------------------------------------------------------------------ */
 /* Accessors for the Python wrapper                                     */
 /* ------------------------------------------------------------------ */
 int64_t or_n_stars(const or_solver *s) { return s->n_stars; }

@@ -474,3 +474,66 @@ def test_diagnostics_match_exact_sums():
                              [2.0, 1.0], [0, 0])
     # K = (2*1 + 1*9)/2; p = (3, 2, 0); L = (1,0,0) x (0,2,0) + (0,2,0) x (3,0,0) = (0,0,2) + (0,0,-6)
     assert two.tolist() == [5.5, 3.0, 2.0, 0.0, 0.0, 0.0, -4.0, 3.0, 2.0, 2.0, 0.0]
+
+
+# -- black holes: multiple time stepping (internal/sim/split.go) -----------------------
+
+def _hole_system(n=2000, seed=15, dt=1e-2):
+    s = ics.plummer(n, seed)
+    kind = s["kind"].copy()
+    m = s["m"].copy()
+    kind[[0, 1]] = ics.KIND_HOLE
+    m[[0, 1]] = [0.05, 0.03]
+    st = {k: s[k].copy() for k in ("px", "py", "pz", "vx", "vy", "vz")}
+    return st, m, kind, np.ones(n, np.uint8)
+
+
+def test_split_step_conserves_momentum_and_substeps_near_holes():
+    """split.go:8-15: both partners of every pair are kicked at the same times
+    with the same force, so with an exact slow force (tiny theta) the total
+    momentum is conserved to round-off; stars within 20 outer steps of a
+    hole's timescale get power-of-two substeps."""
+    st, m, kind, alive = _hole_system()
+    o = oracle.Solver(theta=1e-9, workers=8)
+    p0 = np.array([(m * st[v]).sum() for v in ("vx", "vy", "vz")])
+    gross = sum(np.abs(m * st[v]).sum() for v in ("vx", "vy", "vz"))
+    subs = [oracle.split_step(o, st, m, kind, alive, 0.01, 1e-2, acc_valid=i > 0) for i in range(3)]
+    assert all(sb > 1 and sb & (sb - 1) == 0 for sb in subs), subs
+    p1 = np.array([(m * st[v]).sum() for v in ("vx", "vy", "vz")])
+    assert np.abs(p1 - p0).max() <= 1e-13 * gross
+
+
+def test_split_step_substeps_resolve_a_close_orbit():
+    """A light star on an eccentric orbit (e = 0.6) around a hole of mass 1,
+    outer step 1/20 of the period: with substeps the energy error over 3
+    orbits stays below 1 % (measured 0.36 %), more than 100x below the plain
+    KDK at the same step (measured 150 %)."""
+    e, a = 0.6, 1.0
+    rp = a * (1 - e)
+    vp = math.sqrt((1 + e) / rp)
+    m = np.array([1.0, 1e-6])
+    kind = np.array([ics.KIND_HOLE, ics.KIND_STAR], np.uint8)
+    alive = np.ones(2, np.uint8)
+    T = 2 * math.pi
+    dt = T / 20
+
+    def run(split):
+        st = {"px": np.array([0.0, rp]), "py": np.zeros(2), "pz": np.zeros(2),
+              "vx": np.zeros(2), "vy": np.array([0.0, vp]), "vz": np.zeros(2)}
+        o = oracle.Solver(theta=0.5)
+        energy = lambda: 0.5 * m[1] * (st["vx"][1] ** 2 + st["vy"][1] ** 2) - m[0] * m[1] / math.hypot(
+            st["px"][1] - st["px"][0], st["py"][1] - st["py"][0])
+        e0, worst = energy(), 0.0
+        for i in range(60):
+            if split:
+                oracle.split_step(o, st, m, kind, alive, 0.0, dt, acc_valid=i > 0)
+            else:
+                acc = lambda x, y, z: oracle.direct(x, y, z, m, np.zeros(2, np.uint8), eps=0.0)
+                if i == 0:
+                    st["acc"] = acc(st["px"], st["py"], st["pz"])
+                oracle.kick_drift_kick(st, acc, dt)
+            worst = max(worst, abs((energy() - e0) / e0))
+        return worst
+
+    split, plain = run(True), run(False)
+    assert split <= 1e-2 and split * 100 <= plain, (split, plain)
