@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 FLOPS_PP = 18              # bh.go:145-150 counting rule (add/sub/mul/div/sqrt = 1, fma = 2)
 FLOPS_PC = 50              # bh.go:114-131
 ENCOUNTER_T = 0.12          # global step of the two-disk encounter: 1-5 % of the stars on the top rung
-CONFIGS = ("disk-halo-1m", "plummer-16k", "clumps-16m", "batched-256", "two-disk-4m")
+CONFIGS = ("disk-halo-1m", "plummer-16k", "clumps-16m", "batched-256", "two-disk-4m", "disk-halo-1m-holes")
 
 
 def parse():
@@ -479,6 +479,56 @@ def bench_encounter(args):
     return out
 
 
+def bench_holes(args):
+    """The disk+halo model (1,048,576 bodies) with a central black hole of
+    0.1 disk masses and two of 0.02 in the disk plane at R = 2 and 4 on
+    circular orbits; steps of dt 2e-3 with the split scheme of
+    internal/sim/split.go (tree forces between stars, direct sums for every
+    pair with a hole, power-of-two substeps for the holes and the stars near
+    them).  Reports steps/s, the substep counts and the FP64 momentum drift."""
+    from gravitas_amd import DeviceState, Params, Solver, ics
+    from gravitas_amd.hip import synchronize
+    n = args.n or 1048576
+    theta = 0.5 if args.theta is None else args.theta
+    eps = 0.02 if args.eps is None else args.eps
+    dt = 2e-3
+    log(f"building disk+halo with holes n={n}")
+    state = ics.disk_halo(n, seed=2, eps=eps)
+    state["alive"] = np.ones(n, np.uint8)
+    holes = [0, 1, 2]
+    state["kind"][holes] = ics.KIND_HOLE
+    state["m"][holes] = [0.1, 0.02, 0.02]
+    for h, R in zip(holes, (0.0, 2.0, 4.0)):
+        state["px"][h], state["py"][h], state["pz"][h] = R, 0.0, 0.0
+        state["vx"][h] = state["vz"][h] = 0.0
+        state["vy"][h] = 0.0
+    r = np.sqrt(state["px"] ** 2 + state["py"] ** 2 + state["pz"] ** 2)
+    for hh, R in zip(holes[1:], (2.0, 4.0)):  # circular speed from the mass inside R (spherical estimate)
+        state["vy"][hh] = float(np.sqrt(state["m"][r < R].sum() / R))
+    ds = DeviceState.from_host(state)
+    solver = Solver(theta=theta, walk="fp32")
+    params = Params(softening=eps, dt=dt)
+    d0 = solver.diagnostics(ds, params)
+    subs = [solver.split_step(ds, params, acc_valid=False)]
+    for _ in range(max(0, args.warmup - 1)):
+        subs.append(solver.split_step(ds, params))
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        subs.append(solver.split_step(ds, params))
+    synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    d1 = solver.diagnostics(ds, params)
+    gross = float(np.abs(state["m"] * np.abs(state["vx"])).sum())
+    out = base_line(args, n, ms, "fp32", {
+        "workload": "disk-halo-1m-holes", "n": n, "theta": theta, "eps": eps, "dt": dt,
+        "holes": "0.1 at the centre, 0.02 at R = 2 and R = 4 (circular)",
+        "step": "split step: tree forces + hole direct sums + substeps near holes"})
+    out["substeps"] = subs[-args.steps:]
+    out["momentum_drift_over_gross"] = float(np.abs(np.array(d1["momentum"]) - np.array(d0["momentum"])).max() / gross)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -491,7 +541,8 @@ def main():
         print(f"no GPU backend: {reason}", file=sys.stderr)
         sys.exit(1)
     fn = {"disk-halo-1m": bench_disk_halo, "plummer-16k": bench_plummer, "clumps-16m": bench_clumps,
-          "batched-256": bench_batched, "two-disk-4m": bench_encounter}[args.config]
+          "batched-256": bench_batched, "two-disk-4m": bench_encounter,
+          "disk-halo-1m-holes": bench_holes}[args.config]
     print(json.dumps(fn(args)), flush=True)
 
 

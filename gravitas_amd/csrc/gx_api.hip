@@ -727,6 +727,8 @@ int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, dou
         (rc = ensure(s->tileCnt, (size_t)nb * 4)) || (rc = ensure(s->splitScalars, 16)) ||
         (rc = ensure(s->farF, (size_t)n * 24)) || (rc = ensure(s->fastF, (size_t)n * 24)))
         return rc;
+    // slices of the holes' pair sums: ~4096 bodies per workgroup over the far bodies
+    const int farSlices = (int)std::min<long long>(256, std::max(1LL, n / 4096));
     if (!accValid && (rc = run_dev(s, a))) return rc;  // initForces
     hipStream_t st = s->stream;
     const unsigned grid = (unsigned)std::min<long long>(blocks(n, 256), 4096);
@@ -736,6 +738,8 @@ int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, dou
     HIPCHK(hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     const long long nh = s->hmeta->nholes;
+    if ((rc = ensure(s->partial, (size_t)std::max<long long>(nh, 1) * farSlices * 24))) return rc;
+    double* partial = (double*)s->partial.p;
     if (nh == 0) {
         // kick, drift (sim.go:316-343: dt / sqrt(1 + u^2 / c^2) with c > 0), forces, kick
         if ((rc = kick(s, vx, vy, vz, a.ax, a.ay, a.az, n, 0.5 * dt))) return rc;
@@ -785,19 +789,24 @@ int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, dou
         hipLaunchKernelGGL(k_split_far, dim3(grid), dim3(256), 0, st, n, (const double*)x, (const double*)y,
                            (const double*)z, a.m, a.kind, (const uint8_t*)nearFlag, holes, meta, eps2, c, farX, farY,
                            farZ);
-        hipLaunchKernelGGL(k_split_hole_sum, dim3((unsigned)nh), dim3(256), 0, st, n, (const double*)x,
-                           (const double*)y, (const double*)z, a.m, a.kind, a.dmOff, (const uint8_t*)nearFlag,
-                           (const int32_t*)nearList, (const long long*)nnDev, holes, meta, eps2, c, 0, farX, farY,
-                           farZ);
+        hipLaunchKernelGGL(k_split_hole_sum, dim3((unsigned)nh, (unsigned)farSlices), dim3(256), 0, st, n,
+                           (const double*)x, (const double*)y, (const double*)z, a.m, a.kind, a.dmOff,
+                           (const uint8_t*)nearFlag, (const int32_t*)nearList, (const long long*)nnDev, holes, meta,
+                           eps2, c, 0, partial);
+        hipLaunchKernelGGL(k_split_hole_finish, dim3(1), dim3(64), 0, st, (const double*)x, (const double*)y,
+                           (const double*)z, a.m, a.kind, holes, meta, eps2, c, 0, farSlices,
+                           (const double*)partial, farX, farY, farZ);
     };
     auto fast = [&]() {  // fastForces, split.go:141-160
         hipLaunchKernelGGL(k_split_fast, dim3(grid), dim3(256), 0, st, (const int32_t*)nearList,
                            (const long long*)nnDev, (const double*)x, (const double*)y, (const double*)z, a.m, a.kind,
                            holes, meta, eps2, c, fastX, fastY, fastZ);
-        hipLaunchKernelGGL(k_split_hole_sum, dim3((unsigned)nh), dim3(256), 0, st, n, (const double*)x,
+        hipLaunchKernelGGL(k_split_hole_sum, dim3((unsigned)nh, 1), dim3(256), 0, st, n, (const double*)x,
                            (const double*)y, (const double*)z, a.m, a.kind, a.dmOff, (const uint8_t*)nearFlag,
-                           (const int32_t*)nearList, (const long long*)nnDev, holes, meta, eps2, c, 1, fastX, fastY,
-                           fastZ);
+                           (const int32_t*)nearList, (const long long*)nnDev, holes, meta, eps2, c, 1, partial);
+        hipLaunchKernelGGL(k_split_hole_finish, dim3(1), dim3(64), 0, st, (const double*)x, (const double*)y,
+                           (const double*)z, a.m, a.kind, holes, meta, eps2, c, 1, 1, (const double*)partial, fastX,
+                           fastY, fastZ);
     };
     auto kick_outer = [&](double h) {  // kickOuter, split.go:201-208
         hipLaunchKernelGGL(k_split_kick_outer, dim3(grid), dim3(256), 0, st, n, vx, vy, vz, (const double*)a.ax,

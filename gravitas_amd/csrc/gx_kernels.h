@@ -82,7 +82,11 @@ __global__ void k_split_fast(const int32_t* nearList, const long long* nn, const
 __global__ void k_split_hole_sum(long long n, const double* x, const double* y, const double* z, const double* mass,
                                  const uint8_t* kind, int dmOff, const uint8_t* nearFlag, const int32_t* nearList,
                                  const long long* nn, const int32_t* holes, const Meta* meta, double eps2, double c,
-                                 int nearMode, double* fx, double* fy, double* fz);
+                                 int nearMode, double* partial);
+__global__ void k_split_hole_finish(const double* x, const double* y, const double* z, const double* mass,
+                                    const uint8_t* kind, const int32_t* holes, const Meta* meta, double eps2, double c,
+                                    int nearMode, int slices, const double* partial, double* fx, double* fy,
+                                    double* fz);
 __global__ void k_split_kick_outer(long long n, double* vx, double* vy, double* vz, const double* ax, const double* ay,
                                    const double* az, const double* fx, const double* fy, const double* fz, double h);
 __global__ void k_split_kick_list(const int32_t* list, long long cntHost, const long long* cntDev, double* vx,

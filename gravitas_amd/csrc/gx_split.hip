@@ -194,7 +194,9 @@ __global__ __launch_bounds__(256) void k_split_fast(const int32_t* __restrict__ 
 
 // The holes' side of outerHoleForces (far: every non-hole body that is not
 // near) and of fastForces (near: the near list, after the pull of the other
-// holes).  One workgroup per hole; the pair sum is a tree reduction.
+// holes).  Workgroup (k, j) sums slice j of the bodies for hole k into
+// partial[(k * slices + j) * 3 ..]; k_split_hole_finish adds the slices in
+// order.  Both sums are fixed-order tree reductions.
 __global__ __launch_bounds__(256) void k_split_hole_sum(long long n, const double* __restrict__ x,
                                                         const double* __restrict__ y, const double* __restrict__ z,
                                                         const double* __restrict__ mass,
@@ -204,17 +206,18 @@ __global__ __launch_bounds__(256) void k_split_hole_sum(long long n, const doubl
                                                         const long long* __restrict__ nn,
                                                         const int32_t* __restrict__ holes,
                                                         const Meta* __restrict__ meta, double eps2, double c,
-                                                        int nearMode, double* __restrict__ fx, double* __restrict__ fy,
-                                                        double* __restrict__ fz) {
+                                                        int nearMode, double* __restrict__ partial) {
     __shared__ double red[3][256];
     const long long k = blockIdx.x;
     const long long nh = meta->nholes;
     if (k >= nh) return;
     const int32_t h = holes[k];
     const double hx = x[h], hy = y[h], hz = z[h], rs = schwarzschild(mass[h], c);
-    double ax = 0, ay = 0, az = 0;
     const long long cnt = nearMode ? *nn : n;
-    for (long long j = threadIdx.x; j < cnt; j += blockDim.x) {
+    const long long per = (cnt + gridDim.y - 1) / gridDim.y;
+    const long long lo = (long long)blockIdx.y * per, hi = min(cnt, lo + per);
+    double ax = 0, ay = 0, az = 0;
+    for (long long j = lo + threadIdx.x; j < hi; j += blockDim.x) {
         long long i = j;
         if (nearMode)
             i = nearList[j];
@@ -236,12 +239,29 @@ __global__ __launch_bounds__(256) void k_split_hole_sum(long long n, const doubl
             for (int q = 0; q < 3; q++) red[q][threadIdx.x] += red[q][threadIdx.x + d];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        double px = 0, py = 0, pz = 0;
-        if (nearMode) hole_pull(h, x, y, z, mass, kind, holes, nh, eps2, c, px, py, pz);
-        fx[h] = px + red[0][0];
-        fy[h] = py + red[1][0];
-        fz[h] = pz + red[2][0];
+    if (threadIdx.x < 3) partial[(k * gridDim.y + blockIdx.y) * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_split_hole_finish(const double* __restrict__ x, const double* __restrict__ y,
+                                    const double* __restrict__ z, const double* __restrict__ mass,
+                                    const uint8_t* __restrict__ kind, const int32_t* __restrict__ holes,
+                                    const Meta* __restrict__ meta, double eps2, double c, int nearMode, int slices,
+                                    const double* __restrict__ partial, double* __restrict__ fx,
+                                    double* __restrict__ fy, double* __restrict__ fz) {
+    const long long nh = meta->nholes;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nh; k += (long long)gridDim.x * blockDim.x) {
+        const int32_t h = holes[k];
+        double ax = 0, ay = 0, az = 0;
+        if (nearMode) hole_pull(h, x, y, z, mass, kind, holes, nh, eps2, c, ax, ay, az);
+        double sx = 0, sy = 0, sz = 0;
+        for (int j = 0; j < slices; j++) {
+            sx += partial[(k * slices + j) * 3];
+            sy += partial[(k * slices + j) * 3 + 1];
+            sz += partial[(k * slices + j) * 3 + 2];
+        }
+        fx[h] = ax + sx;
+        fy[h] = ay + sy;
+        fz[h] = az + sz;
     }
 }
 

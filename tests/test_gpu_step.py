@@ -81,3 +81,30 @@ def test_fp32_step_close_and_errors_vs_direct(gpu, plummer):
     ed = np.sort(ics.rel_errors(a0, direct))
     say(f"[step] theta 0.5 tree vs direct: p50 {ed[N // 2]:.2e} p99 {ed[int(0.99 * N)]:.2e} max {ed[-1]:.2e}")
     assert ed[int(0.99 * N)] <= 1e-2
+
+
+def test_resume_from_saved_state_and_forces_is_exact(gpu, plummer):
+    """sim.go:185-226 (LastForces / RestoreForces): a run resumed from saved
+    positions, velocities and the last forces (acc_valid, no initForces)
+    continues bit-identically, here across a host round trip and a fresh
+    solver, for the KDK step and the FP32 walk."""
+    s = plummer
+    p = Params(softening=EPS, dt=DT)
+    say("[step] resume from a saved state")
+    a = Solver(theta=THETA, walk="fp32")
+    ds = DeviceState.from_host(s)
+    a.step(ds, p, acc_valid=False)
+    saved = a.host_state(ds)  # the checkpoint: state plus the last force evaluation
+    a.step(ds, p, acc_valid=True)
+    straight = a.host_state(ds)
+    state = dict(s)
+    for k in ("px", "py", "pz", "vx", "vy", "vz"):
+        state[k] = saved[k]
+    resumed = DeviceState.from_host(state)
+    for k in ("ax", "ay", "az"):
+        getattr(resumed, k).upload(saved[k])
+    b = Solver(theta=THETA, walk="fp32")
+    b.step(resumed, p, acc_valid=True)
+    again = b.host_state(resumed)
+    for k in ("px", "py", "pz", "vx", "vy", "vz", "ax", "ay", "az"):
+        assert np.array_equal(again[k], straight[k]), k
