@@ -322,8 +322,11 @@ __device__ bool trav_group(const Group& G, TravLDS& S, const TravNode* __restric
         while (mixed) {
             const int src = __builtin_ctzll(mixed);
             mixed &= mixed - 1;
-            const double nx = readlane_d(tx, src), ny = readlane_d(ty, src), nz = readlane_d(tz, src);
-            const double ncrit2 = readlane_d(tc, src);
+            // the node is uniform across the wave: its record comes in through a
+            // scalar load rather than eight lane reads
+            const int nid = __builtin_amdgcn_readlane(node, src);
+            const double nx = trav[nid].x, ny = trav[nid].y, nz = trav[nid].z;
+            const double ncrit2 = trav[nid].crit2;
             const uint64_t smask = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(mask >> 32), src) << 32) |
                                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mask, src);
             // the reference's per-star opening test, bh.go:114-116 (FP64, unfused)
