@@ -101,20 +101,28 @@ def cpu_baseline(state, theta, eps, budget):
                       f"C restatement of bh.Solver, -O2 -ffp-contract=off, {workers} threads"}
 
 
-def timed_steps(solver, ds, params, steps, warmup):
-    """Warm-up, then `steps` synchronous force steps; per-phase GPU times from
-    HIP events recorded on the solver's stream."""
+def timed_steps(solver, ds, params, steps, warmup, phase_steps=5):
+    """Warm-up, then `steps` synchronous force steps timed on the host clock
+    (the step replays as one captured graph once its buffers are sized); then
+    `phase_steps` more steps with HIP events recorded on the solver's stream
+    between the phases (per-phase GPU times; events keep those steps eager)."""
     from gravitas_amd.hip import synchronize
-    for _ in range(max(1, warmup)):
+    timing = solver.timing
+    solver.timing = False
+    for _ in range(max(3, warmup)):
         solver.accelerations(ds, params)
     synchronize()
-    phases = []
     t0 = time.perf_counter()
     for _ in range(steps):
         solver.accelerations(ds, params)
-        phases.append(solver.phase_ms())
     synchronize()
     wall = time.perf_counter() - t0
+    solver.timing = True
+    phases = []
+    for _ in range(max(1, phase_steps)):
+        solver.accelerations(ds, params)
+        phases.append(solver.phase_ms())
+    solver.timing = timing
     phase_avg = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
     return wall * 1e3 / steps, phase_avg, solver.stats()
 
