@@ -478,8 +478,8 @@ __device__ int octant_bounds(const uint64_t* __restrict__ keys, int lo, int hi, 
     if (R <= 8 * kWave) {
         // one pass: every lane reads up to 8 keys; boundary o = lo + #(digit < o)
         int c = 0;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
+        const int jn = (R + kWave - 1) / kWave;  // rows of 64 keys, uniform over the wave
+        for (int j = 0; j < jn; j++) {
             const int k = lo + lane + kWave * j;
             const int d = k < hi ? (int)((keys[k] >> shift) & 7) : 8;
 #pragma unroll
