@@ -1,27 +1,31 @@
 // gx_walk32.hip -- FP32 group walk for gfx950 (wave64).
 //
-// One wavefront walks the tree for a group of 64 consecutive stars in Morton
-// order (one star per lane).  The wave shares one traversal stack and two
-// interaction lists in LDS:
-//   * cells     : accepted nodes, staged as (COM - c_g, m, Q) in FP32;
-//   * particles : stars of opened leaves, staged as (x - c_g, m) in FP32,
-// where c_g is the group's bounding-box centre.  Coordinates are made
-// group-relative in FP64 before rounding, so FP32 only carries offsets that
-// are small for near interactions.
+// One wavefront walks the tree for a group of 64 consecutive targets in
+// Morton order (one star per lane), in two phases per group:
 //
-// Every stack and list entry carries a 64-bit lane mask.  A node is tested
-// against the group's bounding box in FP64; when the box does not decide it
-// (some stars would accept the node and others open it), the wave evaluates
-// the reference's own per-star test d2 >= crit2 in FP64 (bh.go:114-116) with a
-// ballot, and the node is accepted for the lanes that pass and opened for the
-// others.  The interaction set of every star is therefore exactly the one of
-// the reference's per-particle walk (bh.go:106-155); only the arithmetic of
-// the interactions is FP32.  Masks are applied with inverse ballots (the mask
-// is used as the lane-select operand directly).
+// Traversal (trav_group): a shared stack of child ranges in LDS, each entry
+// with a 64-bit lane mask.  Up to 64 nodes are popped per round and tested
+// against the group's bounding box in FP64; when the box does not decide a
+// node (some stars would accept it and others open it), the reference's own
+// per-star test d2 >= crit2 (bh.go:114-116) is evaluated in FP64 by every
+// lane and collected with a ballot.  The interaction set of every star is
+// therefore exactly the one of the reference's per-particle walk
+// (bh.go:106-155); only the arithmetic of the interactions is FP32.  The
+// group's lists go to HBM: cells accepted by all lanes, cells accepted by
+// some lanes and opened leaves (with their lane masks).
+//
+// Evaluation (eval_group): cells are staged 64 at a time in LDS as
+// (COM - c_g, m, Q) in FP32, stars of opened leaves 256 at a time as
+// (x - c_g, m), where c_g is the group's bounding-box centre; coordinates
+// are made group-relative in FP64 before rounding, so FP32 only carries
+// offsets that are small for near interactions.  Entries that apply to every
+// lane are evaluated by all lanes in a uniform loop; for the others the 64
+// lane masks of a chunk are transposed, and each lane loops over its own
+// entries only.
 //
 // Interactions follow bh.go:117-133 (cells, monopole + quadrupole) and
-// bh.go:145-150 (stars, Plummer softening).  Each list flush is summed in
-// FP32 and added to the per-star total with Kahan compensation.
+// bh.go:145-150 (stars, Plummer softening).  Each chunk is summed in FP32 and
+// added to the per-star total with Kahan compensation.
 #include <hip/hip_fp16.h>
 
 #include "gx_common.h"
@@ -33,12 +37,6 @@ constexpr int kStack = 768;   // traversal stack entries: (first child, child co
 constexpr int kChunk = 64;    // cells staged per evaluation chunk
 constexpr int kParts = 256;   // stars staged per evaluation chunk
 constexpr double kMargin = 1e-12;  // >> FP64 rounding of d2; box tests stay conservative
-
-__device__ __forceinline__ double readlane_d(double v, int l) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
-    return __hiloint2double(hi, lo);
-}
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
     float y = v - c;
