@@ -1,0 +1,105 @@
+/*
+ * c_abi_smoke.c -- the backend's C ABI used from plain C, the way the Go
+ * binding would (cgo calls the same functions with the same pointers).
+ *
+ * Without a usable GPU: gx_init must fail cleanly with a status code and a
+ * message, and gx_solver_new must return NULL (exit 0, "no device").
+ * With a GPU:
+ *   - two stars, m = 1 at x = 1 and m = 2 at x = 2, eps = 0: the
+ *     accelerations are (2, 0, 0) and (-1, 0, 0) (bh_test.go:137-147);
+ *   - a hole gets 0 and a dead body keeps its value (bh.go:87-101);
+ *   - a KDK step on device arrays gives the same positions as one computed
+ *     here from the step's own formulas (sim.go:302-343), using the forces
+ *     returned by the host entry point;
+ *   - argument errors come back as status codes, not crashes.
+ * Prints "ok ..." and exits 0 on success.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "gravitas_hip.h"
+
+#include <hip/hip_runtime_api.h>
+
+static int fail(const char *what) {
+    fprintf(stderr, "FAIL: %s (%s)\n", what, gx_error());
+    return 1;
+}
+
+int main(void) {
+    char msg[256];
+    int rc = gx_init(msg, sizeof msg);
+    if (rc != GX_OK) {
+        if (rc != GX_ERR_NO_DEVICE && rc != GX_ERR_CODE_OBJECT) return fail("unexpected gx_init status");
+        if (gx_solver_new() != NULL) return fail("solver without a device");
+        printf("no device: %s\n", msg);
+        return 0;
+    }
+    char name[256];
+    gx_name(name, sizeof name);
+    gx_solver *s = gx_solver_new();
+    if (!s) return fail("gx_solver_new");
+    if (gx_solver_config(s, 0.5, 0, GX_WALK_FP64, 0) != GX_OK) return fail("config");
+
+    /* two stars, a hole and a dead body */
+    double x[4] = {1, 2, 5, 7}, y[4] = {0, 0, 0, 0}, z[4] = {0, 0, 0, 0}, m[4] = {1, 2, 3, 4};
+    uint8_t kind[4] = {GX_KIND_STAR, GX_KIND_STAR, GX_KIND_HOLE, GX_KIND_STAR};
+    uint8_t alive[4] = {1, 1, 1, 0};
+    double ax[4] = {9, 9, 9, 42}, ay[4] = {9, 9, 9, 42}, az[4] = {9, 9, 9, 42};
+    if (gx_accelerations(s, x, y, z, m, kind, alive, 4, 0.0, 0, ax, ay, az) != GX_OK) return fail("accelerations");
+    if (ax[0] != 2 || ax[1] != -1 || ay[0] != 0 || az[1] != 0) return fail("two-star known answer");
+    if (ax[2] != 0 || ay[2] != 0 || az[2] != 0) return fail("hole must get 0");
+    if (ax[3] != 42 || ay[3] != 42 || az[3] != 42) return fail("dead body must be untouched");
+
+    /* errors are status codes */
+    if (gx_accelerations(s, NULL, y, z, m, kind, alive, 4, 0.0, 0, ax, ay, az) != GX_ERR_ARG)
+        return fail("null array must be GX_ERR_ARG");
+    if (gx_solver_config(s, -1.0, 0, GX_WALK_FP64, 0) != GX_ERR_ARG) return fail("negative theta must be GX_ERR_ARG");
+    if (gx_solver_config(s, 0.5, 0, GX_WALK_FP64, 0) != GX_OK) return fail("config");
+
+    /* a KDK step on device arrays (two stars only) */
+    const int n = 2;
+    const double dt = 0.01;
+    double hx[2] = {1, 2}, hv[2] = {0.1, -0.2}, zero[2] = {0, 0}, mm[2] = {1, 2};
+    uint8_t hk[2] = {0, 0}, ha[2] = {1, 1};
+    double *d[10];
+    for (int i = 0; i < 10; i++)
+        if (hipMalloc((void **)&d[i], n * sizeof(double)) != hipSuccess) return fail("hipMalloc");
+    uint8_t *dk, *da;
+    if (hipMalloc((void **)&dk, n) != hipSuccess || hipMalloc((void **)&da, n) != hipSuccess) return fail("hipMalloc");
+    /* d: x y z vx vy vz m ax ay az */
+    hipMemcpy(d[0], hx, sizeof hx, hipMemcpyHostToDevice);
+    hipMemcpy(d[1], zero, sizeof zero, hipMemcpyHostToDevice);
+    hipMemcpy(d[2], zero, sizeof zero, hipMemcpyHostToDevice);
+    hipMemcpy(d[3], hv, sizeof hv, hipMemcpyHostToDevice);
+    hipMemcpy(d[4], zero, sizeof zero, hipMemcpyHostToDevice);
+    hipMemcpy(d[5], zero, sizeof zero, hipMemcpyHostToDevice);
+    hipMemcpy(d[6], mm, sizeof mm, hipMemcpyHostToDevice);
+    hipMemcpy(dk, hk, sizeof hk, hipMemcpyHostToDevice);
+    hipMemcpy(da, ha, sizeof ha, hipMemcpyHostToDevice);
+    if (gx_step_dev(s, d[0], d[1], d[2], d[3], d[4], d[5], d[6], dk, da, n, 0.0, 0, dt, 0, d[7], d[8], d[9]) != GX_OK)
+        return fail("gx_step_dev");
+    double gx[2], gv[2];
+    hipMemcpy(gx, d[0], sizeof gx, hipMemcpyDeviceToHost);
+    hipMemcpy(gv, d[3], sizeof gv, hipMemcpyDeviceToHost);
+    /* the same step on the host with forces from the host entry point */
+    double px[2] = {1, 2}, vx[2] = {0.1, -0.2}, a[2], a0[2], a1[2];
+    double py[2] = {0, 0}, pz[2] = {0, 0};
+    if (gx_accelerations(s, px, py, pz, mm, hk, ha, 2, 0.0, 0, a, a0, a1) != GX_OK) return fail("accelerations");
+    for (int i = 0; i < 2; i++) vx[i] += 0.5 * dt * a[i];
+    for (int i = 0; i < 2; i++) px[i] += dt * vx[i];
+    if (gx_accelerations(s, px, py, pz, mm, hk, ha, 2, 0.0, 0, a, a0, a1) != GX_OK) return fail("accelerations");
+    for (int i = 0; i < 2; i++) vx[i] += 0.5 * dt * a[i];
+    if (gx[0] != px[0] || gx[1] != px[1] || gv[0] != vx[0] || gv[1] != vx[1]) return fail("KDK step");
+    for (int i = 0; i < 10; i++) hipFree(d[i]);
+    hipFree(dk);
+    hipFree(da);
+
+    gx_stats st;
+    if (gx_get_stats(s, &st) != GX_OK || st.n_stars != 2) return fail("stats");
+    gx_solver_free(s);
+    printf("ok: %s; two stars (%g, %g), KDK x = (%.15g, %.15g)\n", name, ax[0], ax[1], gx[0], gx[1]);
+    return 0;
+}

@@ -1,0 +1,164 @@
+"""Writes the golden fixtures of the force step: small seeded inputs and the
+oracle's outputs on them, as .npz files (no pickles) next to this script.
+
+The reference holds no golden vectors for this path (its bh tests are
+invariants on self-generated inputs, bh_test.go), and its inputs come from Go's
+math/rand, which cannot be reproduced here; these files pin both the inputs and
+the expected outputs, so the oracle and the HIP path are compared on the same
+bytes on every machine, whatever numpy's generators do later.
+
+    python tests/golden/make_fixtures.py        # from the repository root
+
+Files:
+  plummer_512.npz  equal-mass Plummer sphere (scene.go:389-405 distribution):
+                   Morton keys and permutation (build.go:19-32), accelerations at
+                   theta 0.5, at theta 0 (= 0.9, bh.go:69-74) and monopole-only,
+                   the potential energy (potential.go:12-42) and one KDK step
+                   (sim.go:262-281).
+  mixed_768.npz    Plummer stars + Hernquist dark halo + 2 holes + 3 dead
+                   bodies: accelerations with dark matter on and off
+                   (sim.go:93-98), potential energy with c = 0 and c = 40
+                   (pair terms of the holes), one split step with holes
+                   (split.go:164-199).
+  encounter_768.npz two-disk encounter on 4 rungs: one global block step
+                   (blockstep.go:130-198): rung populations, events, state.
+  edges.npz        two stars and a hole (bh_test.go:137-147), 100 coincident
+                   stars and one more (bh_test.go:149-161), a leaf-size
+                   boundary of 24 and 25 stars (build.go:134-160), one star.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from gravitas_amd import ics  # noqa: E402
+
+STATE = ("px", "py", "pz", "vx", "vy", "vz", "m", "kind", "alive")
+
+
+def inputs(s):
+    n = len(s["px"])
+    out = {k: np.ascontiguousarray(s[k]) for k in STATE if k in s}
+    out.setdefault("alive", np.ones(n, np.uint8))
+    return out
+
+
+def acc(o, s, eps, dm_off=False):
+    n = len(s["px"])
+    return o.accelerations(s["px"], s["py"], s["pz"], s["m"], s["kind"], s["alive"], eps=eps, dm_off=dm_off,
+                           out=tuple(np.zeros(n) for _ in range(3)))
+
+
+def plummer_512():
+    s = inputs(ics.plummer(512, 101))
+    eps, dt = 0.01, 1e-3
+    o = oracle.Solver(theta=0.5)
+    f = {"theta": 0.5, "eps": eps, "dt": dt}
+    f["acc"] = acc(o, s, eps)
+    f["keys"], f["order"], f["n_nodes"] = o.keys(), o.order(), o.n_nodes
+    f["acc_theta0"] = acc(oracle.Solver(theta=0.0), s, eps)
+    f["acc_mono"] = acc(oracle.Solver(theta=0.5, monopole_only=True), s, eps)
+    f["potential"] = o.potential_energy(s["px"], s["py"], s["pz"], s["m"], s["kind"], s["alive"], eps=eps)
+    st = {k: s[k].copy() for k in ("px", "py", "pz", "vx", "vy", "vz")}
+    st["acc"] = f["acc"].copy()
+    oracle.kick_drift_kick(st, lambda x, y, z: acc(o, dict(s, px=x, py=y, pz=z), eps), dt)
+    for k in ("px", "py", "pz", "vx", "vy", "vz"):
+        f["kdk_" + k] = st[k]
+    f["kdk_acc"] = st["acc"]
+    return s, f
+
+
+def mixed_768():
+    s = ics.concat(ics.plummer(384, 102), ics.hernquist_halo(384, 103, mass=2.0, a=2.0, rmax=20.0))
+    s["kind"][[3, 300]] = ics.KIND_HOLE
+    s["m"][[3, 300]] = [0.2, 0.1]
+    s["px"][3], s["py"][3], s["pz"][3] = 0.02, -0.01, 0.03  # a hole in the core: stars near it substep
+    s["alive"] = np.ones(768, np.uint8)
+    s["alive"][[5, 99, 500]] = 0
+    s = inputs(s)
+    eps, theta, dt = 0.01, 0.6, 2e-2
+    o = oracle.Solver(theta=theta)
+    f = {"theta": theta, "eps": eps, "dt": dt}
+    f["acc"] = acc(o, s, eps)
+    f["acc_dmoff"] = acc(o, s, eps, dm_off=True)
+    for c in (0.0, 40.0):
+        f[f"potential_c{int(c)}"] = o.potential_energy(s["px"], s["py"], s["pz"], s["m"], s["kind"], s["alive"],
+                                                       eps=eps, c=c)
+    st = {k: s[k].copy() for k in ("px", "py", "pz", "vx", "vy", "vz")}
+    f["split_substeps"] = oracle.split_step(o, st, s["m"], s["kind"], s["alive"], eps, dt, acc_valid=False)
+    for k in ("px", "py", "pz", "vx", "vy", "vz"):
+        f["split_" + k] = st[k]
+    return s, f
+
+
+def encounter_768():
+    s = ics.two_disk_encounter(768, seed=104)
+    s["alive"] = np.ones(768, np.uint8)
+    s["alive"][[3, 600]] = 0
+    s["kind"][7] = ics.KIND_HOLE
+    s = inputs(s)
+    eps, theta, eta, levels = 0.02, 0.5, 0.025, 4
+    o = oracle.Solver(theta=theta)
+    fn = lambda x, y, z: acc(o, dict(s, px=x, py=y, pz=z), eps)
+    st = {k: s[k].copy() for k in ("px", "py", "pz", "vx", "vy", "vz")}
+    st["acc"] = fn(st["px"], st["py"], st["pz"])
+    a = np.linalg.norm(st["acc"], axis=1)
+    a = a[a > 0]
+    # global step that puts about 5 % of the stars on the top rung
+    T = 4 * float(np.quantile(np.sqrt(2 * eta * eps / a), 0.05))
+    r = oracle.block_step(st, fn, T, levels, eta, eps, s["kind"], s["alive"])
+    f = {"theta": theta, "eps": eps, "eta": eta, "levels": levels, "T": T,
+         "rung_count": np.array(r["rung_count"], np.int64), "events": np.array(r["events"], np.int64).reshape(-1, 2)}
+    for k in ("px", "py", "pz", "vx", "vy", "vz"):
+        f["block_" + k] = st[k]
+    f["block_acc"] = st["acc"]
+    return s, f
+
+
+def edges():
+    """Small cases at the default theta (0 = 0.9), each with its own eps."""
+    f = {}
+    cases = {}
+    # bh_test.go:137-147: stars 1@x=1, 2@x=2 and a hole of mass 4 at the origin, eps 0
+    cases["two"] = {"px": np.array([1.0, 2.0, 0.0]), "m": np.array([1.0, 2.0, 4.0]),
+                    "kind": np.array([0, 0, ics.KIND_HOLE], np.uint8), "eps": 0.0}
+    # bh_test.go:149-161: 100 stars at (1, 2, 3) and one at the origin, eps 0.01
+    cases["coincident"] = {"px": np.r_[np.full(100, 1.0), 0.0], "py": np.r_[np.full(100, 2.0), 0.0],
+                           "pz": np.r_[np.full(100, 3.0), 0.0], "m": np.ones(101), "eps": 0.01}
+    rng = np.random.default_rng(105)
+    for n in (24, 25):  # one leaf, and the first split (leaf size 24, build.go:134-160)
+        cases[f"leaf{n}"] = {"px": rng.random(n), "py": rng.random(n), "pz": rng.random(n),
+                             "m": rng.random(n) + 0.5, "eps": 0.01}
+    cases["single"] = {"px": np.array([0.3]), "m": np.array([2.0]), "eps": 0.01}
+    for name, c in cases.items():
+        n = len(c["px"])
+        s = {"px": c["px"], "py": c.get("py", np.zeros(n)), "pz": c.get("pz", np.zeros(n)), "m": c["m"],
+             "kind": c.get("kind", np.zeros(n, np.uint8)), "alive": np.ones(n, np.uint8)}
+        for k in ("px", "py", "pz", "m", "kind", "alive"):
+            f[f"{name}__{k}"] = np.ascontiguousarray(s[k])
+        f[f"{name}__eps"] = c["eps"]
+        f[f"{name}__acc"] = acc(oracle.Solver(theta=0.0), s, c["eps"])
+    return f
+
+
+def write(name, arrays):
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, **{k: np.asarray(v) for k, v in arrays.items()})
+    print(f"{name}: {os.path.getsize(path)} bytes, {len(arrays)} arrays")
+
+
+def main():
+    for name, make in (("plummer_512.npz", plummer_512), ("mixed_768.npz", mixed_768),
+                       ("encounter_768.npz", encounter_768)):
+        s, f = make()
+        write(name, {**{"in_" + k: v for k, v in s.items()}, **f})
+    write("edges.npz", edges())
+
+
+if __name__ == "__main__":
+    main()

@@ -55,3 +55,40 @@ def test_product_does_not_import_the_oracle():
             if f.endswith((".py", ".hip", ".h", ".cpp")):
                 src = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in re.sub(r"#.*|//.*", "", src).replace("the oracle", ""), f
+
+
+def build_c_smoke(tmp_path):
+    """gcc builds tests/c_abi_smoke.c against the library, the way a cgo
+    binding links it (plain C, no Python in the loop)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not on PATH")
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    lib = os.path.dirname(_lib.LIB_PATH)
+    exe = str(tmp_path / "c_abi_smoke")
+    cmd = ["gcc", "-std=c11", "-O2", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(rocm, "include"),
+           os.path.join(ROOT, "tests", "c_abi_smoke.c"), "-o", exe,
+           "-L", lib, "-lgravitas_hip", "-L", os.path.join(rocm, "lib"), "-lamdhip64",
+           "-Wl,-rpath," + lib, "-Wl,-rpath," + os.path.join(rocm, "lib")]
+    subprocess.run(cmd, check=True)
+    return exe
+
+
+def run_c_smoke(exe):
+    import subprocess
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_c_harness_builds_and_reports_status(tmp_path):
+    out = run_c_smoke(build_c_smoke(tmp_path))
+    assert out.startswith(("no device:", "ok:")), out
+
+
+@pytest.mark.gpu
+def test_c_harness_known_answers_on_gpu(gpu, tmp_path):
+    out = run_c_smoke(build_c_smoke(tmp_path))
+    assert out.startswith("ok:"), out
