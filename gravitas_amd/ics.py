@@ -289,3 +289,46 @@ def error_stats(got, want):
         return dict(p50=0.0, p99=0.0, max=0.0)
     return dict(p50=float(e[len(e) // 2]), p99=float(e[min(len(e) - 1, int(0.99 * len(e)))]),
                 max=float(e[-1]))
+
+
+# -- gas (hydro workloads) -------------------------------------------------------------
+
+
+def _gas(s, u):
+    n = len(s["px"])
+    s["kind"] = np.full(n, KIND_GAS, np.uint8)
+    s["alive"] = np.ones(n, np.uint8)
+    s["u"] = np.ascontiguousarray(u, np.float64)
+    s["h"] = np.zeros(n)  # 0: the hydro makes a first guess (internal/sph/sph.go:277-302)
+    s["rho"] = np.zeros(n)
+    return s
+
+
+def gas_cloud(n, seed):
+    """Gaussian gas cloud of the hydro tests (internal/sph/sph_test.go:32-45):
+    positions N(0, 1) with z halved, velocities N(0, 0.3), masses
+    (0.5 + U) / n, internal energy 0.1 + 0.2 U."""
+    rng = np.random.default_rng([seed, 0x6A5])
+    pos = rng.standard_normal((n, 3)) * np.array([1.0, 1.0, 0.5])
+    vel = rng.standard_normal((n, 3)) * 0.3
+    m = (0.5 + rng.random(n)) / n
+    u = 0.1 + 0.2 * rng.random(n)
+    return _gas(_state(pos, vel, m, KIND_GAS), u)
+
+
+def gas_sphere(n, seed=9):
+    """Plummer sphere of gas made hot enough to be roughly pressure supported
+    (internal/sph/system_test.go:14-23): u = 0.2, velocities scaled by 0.3."""
+    s = plummer(n, seed)
+    for k in ("vx", "vy", "vz"):
+        s[k] = s[k] * 0.3
+    return _gas(s, np.full(n, 0.2))
+
+
+def gas_lattice(k=16, spacing=0.1):
+    """k^3 bodies of unit mass on a cubic lattice, u = 1 (internal/sph/sph_test.go:133-143)."""
+    a = np.arange(k) * spacing
+    X, Y, Z = np.meshgrid(a, a, a, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    n = len(pos)
+    return _gas(_state(pos, np.zeros((n, 3)), np.ones(n), KIND_GAS), np.ones(n))

@@ -37,6 +37,7 @@ _D = ctypes.POINTER(ctypes.c_double)
 _I8 = ctypes.POINTER(ctypes.c_int8)
 _U8 = ctypes.POINTER(ctypes.c_uint8)
 _I64 = ctypes.POINTER(ctypes.c_int64)
+_I32 = ctypes.POINTER(ctypes.c_int32)
 
 NODE_DTYPE = np.dtype(
     [("x", "f8"), ("y", "f8"), ("z", "f8"), ("m", "f8"),
@@ -114,6 +115,35 @@ def lib():
                                     ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_int, _D, _D, _D]
         assert L.or_node_size() == NODE_DTYPE.itemsize
+        # SPH neighbour tree and hydro (sph_oracle.c)
+        for f in ("or_cbrt",):
+            getattr(L, f).restype = ctypes.c_double
+            getattr(L, f).argtypes = [ctypes.c_double]
+        for f in ("or_sph_w", "or_sph_dw"):
+            getattr(L, f).restype = ctypes.c_double
+            getattr(L, f).argtypes = [ctypes.c_double, ctypes.c_double]
+        L.or_sph_tree_new.restype = ctypes.c_void_p
+        L.or_sph_tree_free.argtypes = [ctypes.c_void_p]
+        L.or_sph_tree_build.restype = ctypes.c_int32
+        L.or_sph_tree_build.argtypes = [ctypes.c_void_p, _I32, ctypes.c_int32, _D, _D, _D, _D]
+        L.or_sph_tree_refit_h.argtypes = [ctypes.c_void_p, _D]
+        L.or_sph_tree_sort_leaves.argtypes = [ctypes.c_void_p]
+        for f in ("or_sph_tree_nnodes", "or_sph_tree_nleaves"):
+            getattr(L, f).restype = ctypes.c_int32
+            getattr(L, f).argtypes = [ctypes.c_void_p]
+        for f in ("or_sph_tree_nodes", "or_sph_tree_keys", "or_sph_tree_order", "or_sph_tree_leaves"):
+            getattr(L, f).restype = ctypes.c_void_p
+            getattr(L, f).argtypes = [ctypes.c_void_p]
+        L.or_sph_each.restype = ctypes.c_int32
+        L.or_sph_each.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_int, _I32, _D, ctypes.c_int32]
+        L.or_sph_group_scan.restype = ctypes.c_int32
+        L.or_sph_group_scan.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, _I32, _D, ctypes.c_int32]
+        L.or_sph_forces.restype = ctypes.c_int64
+        L.or_sph_forces.argtypes = [ctypes.c_int64, _D, _D, _D, _D, _D, _D, _D, _U8, _U8, _D, _D, _D,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _U8, _D,
+                                    _D, _D, _D, _D, _D, _D, ctypes.POINTER(ctypes.c_uint64)]
         _lib = L
     return _lib
 
@@ -393,3 +423,120 @@ def split_step(solver, state, m, kind, alive, eps, dt, c=0.0, dm_off=False, acc_
                           float(eps), int(dm_off), float(dt), float(c), int(bool(acc_valid)), _dp(ax), _dp(ay), _dp(az))
     state["acc"] = np.stack([ax, ay, az], axis=1)
     return sub
+
+
+# -- SPH neighbour tree and hydro (sph_oracle.c; internal/sph/*.go) ----------------------------
+
+SPH_NODE_DTYPE = np.dtype(
+    [("minx", "f8"), ("miny", "f8"), ("minz", "f8"), ("maxx", "f8"), ("maxy", "f8"), ("maxz", "f8"),
+     ("hmax", "f8"), ("lo", "i4"), ("hi", "i4"), ("left", "i4"), ("right", "i4")])
+
+
+def cbrt(x: float) -> float:
+    """The cube root the hydro uses for smoothing-length targets (or_cbrt)."""
+    return lib().or_cbrt(float(x))
+
+
+def sph_w(r: float, h: float) -> float:
+    """Cubic spline kernel W(r, h), internal/sph/kernel.go:9-22."""
+    return lib().or_sph_w(float(r), float(h))
+
+
+def sph_dw(r: float, h: float) -> float:
+    """dW/dr, internal/sph/kernel.go:24-36."""
+    return lib().or_sph_dw(float(r), float(h))
+
+
+def _i32(a):
+    return a.ctypes.data_as(_I32)
+
+
+class SphTree:
+    """The neighbour BVH of internal/sph/tree.go over a list of gas bodies."""
+
+    def __init__(self):
+        self._L = lib()
+        self._h = ctypes.c_void_p(self._L.or_sph_tree_new())
+        self._keep = None
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._L.or_sph_tree_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def build(self, gas, x, y, z, h):
+        gas = np.ascontiguousarray(gas, np.int32)
+        x, y, z, h = (np.ascontiguousarray(a, np.float64) for a in (x, y, z, h))
+        self._keep = (gas, x, y, z, h)
+        return int(self._L.or_sph_tree_build(self._h, _i32(gas), len(gas), _dp(x), _dp(y), _dp(z), _dp(h)))
+
+    def refit_h(self, h):
+        h = np.ascontiguousarray(h, np.float64)
+        self._L.or_sph_tree_refit_h(self._h, _dp(h))
+
+    def sort_leaves(self):
+        self._L.or_sph_tree_sort_leaves(self._h)
+
+    def _array(self, fn, count, dtype):
+        if count == 0:
+            return np.zeros(0, dtype)
+        buf = (ctypes.c_char * (count * np.dtype(dtype).itemsize)).from_address(fn(self._h))
+        return np.frombuffer(buf, dtype=dtype).copy()
+
+    def nodes(self):
+        return self._array(self._L.or_sph_tree_nodes, self._L.or_sph_tree_nnodes(self._h), SPH_NODE_DTYPE)
+
+    def leaves(self):
+        return self._array(self._L.or_sph_tree_leaves, self._L.or_sph_tree_nleaves(self._h), np.int32)
+
+    def keys(self):
+        return self._array(self._L.or_sph_tree_keys, len(self._keep[0]), np.uint64)
+
+    def order(self):
+        return self._array(self._L.or_sph_tree_order, len(self._keep[0]), np.int32)
+
+    def each(self, px, py, pz, radius, symmetric):
+        """Bodies each() reports, in its visiting order, with their r^2."""
+        cap = len(self._keep[0]) + 1
+        out, r2 = np.zeros(cap, np.int32), np.zeros(cap)
+        n = self._L.or_sph_each(self._h, px, py, pz, radius, int(symmetric), _i32(out), _dp(r2), cap)
+        return out[:n], r2[:n]
+
+    def group_scan(self, leaf, symmetric, px, py, pz, radius):
+        """candidates(leaf) + scan for one body, in visiting order."""
+        cap = len(self._keep[0]) + 1
+        out, r2 = np.zeros(cap, np.int32), np.zeros(cap)
+        n = self._L.or_sph_group_scan(self._h, int(leaf), int(symmetric), px, py, pz, radius, _i32(out), _dp(r2),
+                                      cap)
+        return out[:n], r2[:n]
+
+
+def sph_forces(s, softening=0.0, gamma=0.0, ngb=0.0, alpha=0.0, active=None, step=None, wake=None, out=None):
+    """One hydro evaluation, sph.Solver.Forces (internal/sph/sph.go:83-263),
+    on the float64 arrays px..vz, m, u, h, rho and uint8 kind, alive of the
+    dict ``s``; h and rho are updated in place.  Returns (out, evals): out is
+    a dict ax, ay, az, dudt, divv, dt (float64) and wake (uint64 bits of
+    float64; +Inf where not lowered), evals the number of density sums."""
+    L = lib()
+    n = len(s["px"])
+    for k in ("px", "py", "pz", "vx", "vy", "vz", "m", "u", "h", "rho"):
+        s[k] = np.ascontiguousarray(s[k], np.float64)
+    kind = np.ascontiguousarray(s["kind"], np.uint8)
+    alive = np.ascontiguousarray(s.get("alive", np.ones(n, np.uint8)), np.uint8)
+    if out is None:
+        out = {k: np.zeros(n) for k in ("ax", "ay", "az", "dudt", "divv", "dt")}
+    if wake is None:
+        wake = np.full(n, np.inf).view(np.uint64).copy()
+    out["wake"] = wake
+    act = None if active is None else np.ascontiguousarray(active, np.uint8)
+    st = None if step is None else np.ascontiguousarray(step, np.float64)
+    evals = L.or_sph_forces(n, *[_dp(s[k]) for k in ("px", "py", "pz", "vx", "vy", "vz", "m")], _u8(kind),
+                            _u8(alive), _dp(s["u"]), _dp(s["h"]), _dp(s["rho"]), float(gamma), float(ngb),
+                            float(alpha), float(softening), None if act is None else _u8(act),
+                            None if st is None else _dp(st), *[_dp(out[k]) for k in
+                                                               ("ax", "ay", "az", "dudt", "divv", "dt")],
+                            wake.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    return out, int(evals)

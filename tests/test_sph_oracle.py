@@ -1,0 +1,309 @@
+"""The SPH oracle (oracle/sph_oracle.c) pinned against what the reference's
+tests and the mathematics fix (internal/sph/*_test.go):
+
+  * cube root: correctly rounded, by exact rational comparison;
+  * kernel: normalisation and derivative (sph_test.go:12-30);
+  * tree: keys against an independent bit-spreading, the permutation against
+    numpy's stable argsort, the split rule, boxes and hmax against direct
+    min/max over each node's range;
+  * neighbour search: each() and the group scan against brute force
+    (sph_test.go:47-129), and the visiting order the GPU relies on (leaves
+    by decreasing range start, bodies in leaf order);
+  * hydro: density of a lattice and no interior force (sph_test.go:131-157);
+    momentum and energy conservation (sph_test.go:159-181); densities equal
+    the exact sums; viscous heating and div v of a uniform compression;
+    the Courant step of a static gas; the step limiter's wake; an active
+    subset equals the full evaluation.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from gravitas_amd import ics
+
+
+# -- cube root ------------------------------------------------------------------------------
+
+
+def correctly_rounded_cbrt(x, y):
+    """True if y is the double nearest to x^(1/3) (exact rational test)."""
+    lo = (Fraction(math.nextafter(y, 0.0)) + Fraction(y)) / 2
+    hi = (Fraction(y) + Fraction(math.nextafter(y, math.inf))) / 2
+    X = Fraction(x)
+    return lo ** 3 <= X <= hi ** 3
+
+
+def test_cbrt_is_correctly_rounded():
+    rng = np.random.default_rng(1)
+    xs = list(np.exp(rng.uniform(-700, 700, 3000)))
+    xs += [float(k) ** 3 for k in range(1, 200)] + [2.0 ** e for e in range(-1060, 1020, 7)]
+    xs += [1.0, 2.0, 3.0, 0.5, 4.0 - 2 ** -50, 8.0, 5e-324, 1.7976931348623157e308]
+    bad = [x for x in xs if not correctly_rounded_cbrt(x, oracle.cbrt(x))]
+    assert not bad, bad[:5]
+    for k in range(1, 200):
+        assert oracle.cbrt(float(k) ** 3) == k
+    assert oracle.cbrt(0.0) == 0.0 and math.isinf(oracle.cbrt(math.inf)) and math.isnan(oracle.cbrt(math.nan))
+    assert oracle.cbrt(-27.0) == -3.0
+
+
+def test_cbrt_within_one_ulp_of_libm():
+    x = np.exp(np.random.default_rng(2).uniform(-50, 50, 2000))
+    got = np.array([oracle.cbrt(v) for v in x])
+    assert (np.abs(got - np.cbrt(x)) <= np.spacing(np.cbrt(x))).all()
+
+
+# -- kernel --------------------------------------------------------------------------------------
+
+
+def test_kernel_normalisation_and_derivative():
+    h, n = 0.7, 20000
+    dr = 2 * h / n
+    r = (np.arange(n) + 0.5) * dr
+    integral = sum(4 * math.pi * ri * ri * oracle.sph_w(ri, h) * dr for ri in r)
+    assert abs(integral - 1) <= 1e-6
+    for r in (0.1, 0.5, 0.69, 0.71, 1.2, 1.39):
+        num = (oracle.sph_w(r + 1e-7, h) - oracle.sph_w(r - 1e-7, h)) / 2e-7
+        assert abs(num - oracle.sph_dw(r, h)) <= 1e-5
+    assert oracle.sph_w(2 * h, h) == 0 and oracle.sph_dw(2 * h, h) == 0
+    assert oracle.sph_w(0, 1.0) == 1 / math.pi
+
+
+# -- tree ----------------------------------------------------------------------------------------
+
+
+def spread(v):
+    out = np.zeros_like(v)
+    for b in range(21):
+        out |= ((v >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b)
+    return out
+
+
+def cloud_with_h(n, seed, wide=0.0):
+    s = ics.gas_cloud(n, seed)
+    rng = np.random.default_rng(seed + 100)
+    s["h"] = 0.03 + 0.1 * rng.random(n)
+    if wide:
+        w = rng.random(n) < wide
+        s["h"][w] = 1 + 2 * rng.random(int(w.sum()))
+    return s
+
+
+def test_tree_keys_order_and_nodes():
+    s = cloud_with_h(3000, 3)
+    gas = np.arange(3000, dtype=np.int32)[::-1].copy()  # a gas list out of body order
+    s["px"][[10, 11, 12]] = s["px"][9]  # a few equal keys
+    s["py"][[10, 11, 12]] = s["py"][9]
+    s["pz"][[10, 11, 12]] = s["pz"][9]
+    t = oracle.SphTree()
+    t.build(gas, s["px"], s["py"], s["pz"], s["h"])
+    x, y, z = s["px"][gas], s["py"][gas], s["pz"][gas]
+    mn = [c.min() for c in (x, y, z)]
+    side = max(max(x.max() - mn[0], y.max() - mn[1]), max(z.max() - mn[2], 1e-12)) * (1 + 1e-9)
+    scale = 2097151.0 / side
+    q = [((c - m0) * scale).astype(np.uint64) for c, m0 in zip((x, y, z), mn)]
+    keys = spread(q[0]) << np.uint64(2) | spread(q[1]) << np.uint64(1) | spread(q[2])
+    perm = np.argsort(keys, kind="stable")
+    assert np.array_equal(t.keys(), keys[perm])
+    assert np.array_equal(t.order(), gas[perm])
+    nodes, k = t.nodes(), t.keys()
+    order = t.order()
+    leaves = set(t.leaves().tolist())
+    assert nodes["lo"][0] == 0 and nodes["hi"][0] == 3000
+    for ni, nd in enumerate(nodes):
+        lo, hi = int(nd["lo"]), int(nd["hi"])
+        sl = order[lo:hi]
+        assert nd["minx"] == s["px"][sl].min() and nd["maxz"] == s["pz"][sl].max()
+        assert nd["hmax"] == s["h"][sl].max()
+        if nd["left"] < 0:
+            assert hi - lo <= 8 and ni in leaves
+            continue
+        assert hi - lo > 8 and nd["left"] == ni + 1  # pre-order, left first
+        L, R = nodes[nd["left"]], nodes[nd["right"]]
+        assert L["lo"] == lo and L["hi"] == R["lo"] and R["hi"] == hi
+        mid = int(L["hi"])
+        if k[lo] == k[hi - 1]:
+            assert mid == (lo + hi) // 2
+        else:
+            bit = np.uint64(1) << np.uint64(int(k[lo] ^ k[hi - 1]).bit_length() - 1)
+            assert ((k[lo:mid] & bit) == 0).all() and ((k[mid:hi] & bit) != 0).all()
+    # leaves partition the range in order
+    lv = sorted((int(nodes["lo"][l]), int(nodes["hi"][l])) for l in leaves)
+    assert lv[0][0] == 0 and lv[-1][1] == 3000 and all(a[1] == b[0] for a, b in zip(lv, lv[1:]))
+
+
+def brute(s, i, sym):
+    d = np.sqrt((s["px"] - s["px"][i]) ** 2 + (s["py"] - s["py"][i]) ** 2 + (s["pz"] - s["pz"][i]) ** 2)
+    lim = np.full(len(d), 2 * s["h"][i])
+    if sym:
+        lim = np.maximum(lim, 2 * s["h"])
+    return set(np.nonzero(d < lim)[0].tolist())
+
+
+def visit_key(t):
+    """Sort key of the visiting order: leaf by decreasing lo, then position."""
+    nodes, order = t.nodes(), t.order()
+    pos = np.empty(len(order), np.int64)
+    pos[order] = np.arange(len(order))
+    leaf_lo = np.empty(len(order), np.int64)
+    for l in t.leaves():
+        leaf_lo[nodes["lo"][l]:nodes["hi"][l]] = nodes["lo"][l]
+    return lambda j: (-leaf_lo[pos[j]], pos[j])
+
+
+def test_each_matches_brute_force_in_leaf_order():
+    s = cloud_with_h(2000, 1)
+    s["h"] = 0.05 + 0.3 * np.random.default_rng(5).random(2000)
+    t = oracle.SphTree()
+    t.build(np.arange(2000), s["px"], s["py"], s["pz"], s["h"])
+    key = visit_key(t)
+    for i in (0, 17, 999, 1500):
+        for sym in (False, True):
+            got, _ = t.each(s["px"][i], s["py"][i], s["pz"][i], 2 * s["h"][i], sym)
+            assert set(got.tolist()) == brute(s, i, sym) and len(set(got.tolist())) == len(got)
+            assert got.tolist() == sorted(got.tolist(), key=key)
+
+
+def test_group_scan_matches_brute_force_with_wide_bodies():
+    s = cloud_with_h(2000, 3, wide=0.05)
+    t = oracle.SphTree()
+    t.build(np.arange(2000), s["px"], s["py"], s["pz"], s["h"])
+    t.sort_leaves()
+    nodes, order = t.nodes(), t.order()
+    key = visit_key(t)
+    for sym in (False, True):
+        for leaf in t.leaves()[::7]:
+            for k in range(nodes["lo"][leaf], nodes["hi"][leaf]):
+                i = int(order[k])
+                got, _ = t.group_scan(leaf, sym, s["px"][i], s["py"][i], s["pz"][i], 2 * s["h"][i])
+                assert set(got.tolist()) == brute(s, i, sym) and len(set(got.tolist())) == len(got)
+                assert got.tolist() == sorted(got.tolist(), key=key)
+                each, _ = t.each(s["px"][i], s["py"][i], s["pz"][i], 2 * s["h"][i], sym)
+                assert got.tolist() == each.tolist()
+
+
+def test_sort_leaves_orders_by_decreasing_h_stably():
+    s = cloud_with_h(1000, 4)
+    s["h"][::3] = 0.05  # ties
+    t = oracle.SphTree()
+    t.build(np.arange(1000), s["px"], s["py"], s["pz"], s["h"])
+    before = t.order()
+    t.sort_leaves()
+    after, nodes = t.order(), t.nodes()
+    for l in t.leaves():
+        lo, hi = nodes["lo"][l], nodes["hi"][l]
+        b = before[lo:hi]
+        want = b[np.argsort(-s["h"][b], kind="stable")]
+        assert np.array_equal(after[lo:hi], want)
+
+
+# -- hydro ----------------------------------------------------------------------------------------
+
+
+def test_density_of_lattice_and_no_interior_force():
+    s = ics.gas_lattice(16, 0.1)
+    out, _ = oracle.sph_forces(s)
+    c = (8 * 16 + 8) * 16 + 8
+    want = 1 / 0.1 ** 3
+    assert abs(s["rho"][c] / want - 1) <= 0.03
+    assert math.hypot(out["ax"][c], out["ay"][c]) <= 1e-6 * want
+
+
+def test_hydro_conserves_momentum_and_energy():
+    s = ics.gas_cloud(3000, 2)
+    out, _ = oracle.sph_forces(s)
+    m = s["m"]
+    scale = (m * np.abs(out["dudt"])).sum()
+    p = np.linalg.norm([(m * out[k]).sum() for k in ("ax", "ay", "az")])
+    dE = (m * (s["vx"] * out["ax"] + s["vy"] * out["ay"] + s["vz"] * out["az"] + out["dudt"])).sum()
+    assert p <= 1e-11 * scale
+    assert abs(dE) <= 1e-10 * scale
+
+
+def test_densities_are_the_exact_sums_and_h_converges():
+    s = ics.gas_cloud(1500, 6)
+    _, evals = oracle.sph_forces(s, softening=0.01)
+    assert evals > 1500  # h was guessed, so some bodies iterated
+    x, y, z, h, m = s["px"], s["py"], s["pz"], s["h"], s["m"]
+    ok = 0
+    for i in range(0, 1500, 5):
+        r = np.sqrt((x - x[i]) ** 2 + (y - y[i]) ** 2 + (z - z[i]) ** 2)
+        sel = r < 2 * h[i]
+        exact = math.fsum(m[j] * oracle.sph_w(r[j], h[i]) for j in np.nonzero(sel)[0])
+        assert abs(s["rho"][i] - exact) <= 1e-13 * exact
+        target = max(0.25 * 0.01, 0.5 * (3 * 40 * m[i] / (4 * math.pi * s["rho"][i])) ** (1 / 3))
+        ok += abs(target - h[i]) <= 0.05 * h[i]
+    assert ok >= 0.95 * len(range(0, 1500, 5))
+
+
+def test_viscosity_heats_a_uniform_compression():
+    s = ics.gas_lattice(12, 0.1)
+    c = s["px"].mean(), s["py"].mean(), s["pz"].mean()
+    s["vx"], s["vy"], s["vz"] = -(s["px"] - c[0]), -(s["py"] - c[1]), -(s["pz"] - c[2])
+    s["u"][:] = 0.0  # no pressure: viscosity only
+    out, _ = oracle.sph_forces(s)
+    m = s["m"]
+    assert (out["dudt"] >= 0).all() and out["dudt"].sum() > 0
+    assert (m * (s["vx"] * out["ax"] + s["vy"] * out["ay"] + s["vz"] * out["az"])).sum() < 0
+    inner = (np.abs(s["px"] - c[0]) < 0.25) & (np.abs(s["py"] - c[1]) < 0.25) & (np.abs(s["pz"] - c[2]) < 0.25)
+    assert np.abs(out["divv"][inner] + 3).max() <= 0.05
+
+
+def test_courant_step_of_static_gas():
+    s = ics.gas_lattice(8, 0.1)
+    s["u"][:] = 0.5
+    out, _ = oracle.sph_forces(s)
+    c = math.sqrt(5 / 3 * 2 / 3 * 0.5)
+    assert np.allclose(out["dt"], 0.3 * s["h"] / (2 * c), rtol=1e-15)
+
+
+def test_step_limiter_wakes_long_steps():
+    s = ics.gas_cloud(800, 7)
+    oracle.sph_forces(s)  # converge h
+    step = np.full(800, 1e-3)
+    step[::10] = 10.0  # very long steps
+    out, _ = oracle.sph_forces(s, step=step)
+    wake = out["wake"].view(np.float64)
+    assert (out["dt"] <= 4 * 1e-3 + 0).all()
+    woken = np.isfinite(wake)
+    assert woken[::10].any() and not woken[1::10].any()
+    # each woken body got the smallest dt of a neighbour that found it too long
+    assert (wake[woken] < 10.0 / 4).all()
+
+
+def test_active_subset_equals_full_evaluation():
+    s = ics.gas_cloud(2000, 8)
+    s["kind"][::17] = ics.KIND_STAR  # not gas: ignored
+    s["alive"][5] = 0
+    # Evaluate until every h is within tolerance, so that an evaluation changes
+    # nothing; a body whose damped update keeps cycling (its neighbour count
+    # jumps across the target) is taken out after a few passes.
+    for it in range(20):
+        h, rho = s["h"].copy(), s["rho"].copy()
+        full, _ = oracle.sph_forces(s)
+        moved = (s["h"] != h) | (s["rho"] != rho)
+        if not moved.any():
+            break
+        if it >= 3:
+            s["alive"][moved] = 0
+    assert not moved.any()
+    h, rho = s["h"].copy(), s["rho"].copy()
+    active = (np.random.default_rng(3).random(2000) < 0.1).astype(np.uint8)
+    sub = {k: np.full(2000, 7.0) for k in ("ax", "ay", "az", "dudt", "divv", "dt")}
+    out, _ = oracle.sph_forces(s, active=active, out=sub)
+    ev = (active == 1) & (s["kind"] == ics.KIND_GAS) & (s["alive"] == 1)
+    for k in ("ax", "ay", "az", "dudt", "divv", "dt"):
+        assert np.array_equal(out[k][ev], full[k][ev]), k
+        assert (out[k][~ev] == 7.0).all()
+    assert np.array_equal(s["h"], h) and np.array_equal(s["rho"], rho)
+
+
+def test_fewer_than_two_gas_bodies():
+    s = ics.gas_cloud(5, 9)
+    s["kind"][1:] = ics.KIND_STAR
+    out = {k: np.full(5, 3.0) for k in ("ax", "ay", "az", "dudt", "divv", "dt")}
+    oracle.sph_forces(s, out=out)
+    assert out["ax"][0] == 0 and out["dt"][0] == math.inf and (out["ax"][1:] == 3.0).all()
+    assert s["h"][0] == 0  # no smoothing length guess below two gas bodies
