@@ -30,6 +30,8 @@ EXPORTS = (
     "gx_phase_ms", "gx_export", "gx_step_dev", "gx_kick_dev", "gx_drift_dev", "gx_solver_tune",
     "gx_batch_accelerations_dev", "gx_batch_step_dev", "gx_graph_capture_step", "gx_graph_launch",
     "gx_accelerations_active_dev", "gx_block_step_dev", "gx_diagnostics_dev", "gx_split_step_dev",
+    "gx_hydro_new", "gx_hydro_free", "gx_hydro_config", "gx_hydro_set_stream", "gx_hydro_forces_dev",
+    "gx_hydro_forces", "gx_hydro_get_stats", "gx_hydro_phase_ms", "gx_hydro_export",
 )
 
 GX_BLOCK_MAX_LEVELS = 8
@@ -53,6 +55,18 @@ class GxBlockStats(ctypes.Structure):
         ("levels", ctypes.c_int32), ("events", ctypes.c_int32), ("bodies", ctypes.c_int64),
         ("targets", ctypes.c_int64), ("rung_count", ctypes.c_int64 * GX_BLOCK_MAX_LEVELS),
         ("tick", ctypes.c_int32 * GX_BLOCK_MAX_TICKS), ("active", ctypes.c_int64 * GX_BLOCK_MAX_TICKS),
+    ]
+
+
+class GxGasParams(ctypes.Structure):
+    _fields_ = [("gamma", ctypes.c_double), ("neighbors", ctypes.c_double), ("alpha", ctypes.c_double)]
+
+
+class GxHydroStats(ctypes.Structure):
+    _fields_ = [
+        ("n_gas", ctypes.c_int64), ("n_nodes", ctypes.c_int64), ("n_leaves", ctypes.c_int64),
+        ("depth", ctypes.c_int32), ("max_iterations", ctypes.c_int32), ("density_sums", ctypes.c_int64),
+        ("pairs", ctypes.c_int64), ("leaf_visits", ctypes.c_int64),
     ]
 
 
@@ -114,6 +128,18 @@ def load():
     L.gx_diagnostics_dev.argtypes = [P, P, P, P, P, P, P, P, P, I64, I, ctypes.POINTER(ctypes.c_double)]
     L.gx_export.argtypes = [P, I, P, I64]
     L.gx_export.restype = I64
+    L.gx_hydro_new.restype = P
+    L.gx_hydro_free.argtypes = [P]
+    L.gx_hydro_free.restype = None
+    L.gx_hydro_config.argtypes = [P, I]
+    L.gx_hydro_set_stream.argtypes = [P, P]
+    hydro = [P] + [P] * 12 + [I64, ctypes.POINTER(GxGasParams), D, P, P] + [P] * 7
+    L.gx_hydro_forces_dev.argtypes = hydro
+    L.gx_hydro_forces.argtypes = hydro
+    L.gx_hydro_get_stats.argtypes = [P, ctypes.POINTER(GxHydroStats)]
+    L.gx_hydro_phase_ms.argtypes = [P, ctypes.POINTER(ctypes.c_float), I]
+    L.gx_hydro_export.argtypes = [P, I, P, I64]
+    L.gx_hydro_export.restype = I64
     _lib = L
     return L
 

@@ -166,6 +166,25 @@ void gx_name(char* out, int len) {
 
 const char* gx_error(void) { return g_err; }
 
+}  // extern "C"
+
+// Error reporting and device check for the other translation units (the
+// hydro in gx_sph.hip).
+int gx::set_error(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int gx::device_ready() {
+    if (gx_init(nullptr, 0) != GX_OK) return set_err(g_init_code, "%s", g_init_msg);
+    return GX_OK;
+}
+
+extern "C" {
+
 gx_solver* gx_solver_new(void) {
     if (gx_init(nullptr, 0) != GX_OK) {
         set_err(g_init_code, "%s", g_init_msg);

@@ -4,6 +4,11 @@
 
 namespace gx {
 
+// gx_api.hip: the thread's error message (gx_error) and the device check of
+// gx_init, for entry points defined in other files.
+int set_error(int code, const char* fmt, ...);
+int device_ready();
+
 constexpr int kCollectTile = 4096;  // bodies per collect workgroup (256 threads x 16)
 constexpr int kSortTile = 4096;     // keys per sort tile (256 threads x 16)
 

@@ -267,6 +267,74 @@ int gx_phase_ms(const gx_solver *s, float *ms, int n);
  * qyy, qzz, qxy, qxz, qyz, pad).  Returns the number of items copied. */
 int64_t gx_export(gx_solver *s, int what, void *host, int64_t count);
 
+/* ---------------------------------------------------------------------- */
+/* SPH hydro: neighbour BVH and forces (sim.Hydro, internal/sph)          */
+
+/* sph.Solver (internal/sph/sph.go:11-19): owns the neighbour tree and the
+ * scratch of the hydro on the GPU, and its own stream.  NULL without a
+ * device (gx_error tells why). */
+typedef struct gx_hydro gx_hydro;
+gx_hydro *gx_hydro_new(void);
+void gx_hydro_free(gx_hydro *h);
+/* timing != 0 records per-phase GPU times (gx_hydro_phase_ms). */
+int gx_hydro_config(gx_hydro *h, int timing);
+int gx_hydro_set_stream(gx_hydro *h, void *stream);
+
+/* The fields of sim.GasParams the hydro reads (internal/sim/sim.go:26-29);
+ * 0 takes the defaults 5/3, 40, 1 (sph.go:64-76). */
+typedef struct {
+    double gamma, neighbors, alpha;
+} gx_gas_params;
+
+/* sph.Solver.Forces (internal/sph/sph.go:83-263) on device arrays of n
+ * bodies in state order.  Alive bodies of kind GX_KIND_GAS take part.  h
+ * (smoothing lengths; <= 0: first guess, sph.go:277-302) and rho (densities)
+ * are read and updated in place; u is the internal energy.  active (may be
+ * NULL: all) selects the evaluated bodies, step (may be NULL) their current
+ * time steps for the limiter.  For evaluated bodies ax, ay, az (pressure and
+ * viscosity acceleration), dudt, divv and dt are written; wake holds float64
+ * bits and is lowered atomically for neighbours whose step is too long
+ * (sim.HydroOut, internal/sim/sim.go:60-68).  FP64, bit-identical to the
+ * reference's order of operations.  Synchronous. */
+int gx_hydro_forces_dev(gx_hydro *h, const double *x, const double *y, const double *z, const double *vx,
+                        const double *vy, const double *vz, const double *mass, const uint8_t *kind,
+                        const uint8_t *alive, const double *u, double *hsml, double *rho, int64_t n,
+                        const gx_gas_params *gas, double softening, const uint8_t *active, const double *step,
+                        double *ax, double *ay, double *az, double *dudt, double *divv, double *dt,
+                        uint64_t *wake);
+
+/* Same on host arrays (copies in and out), the drop-in for the Go engine's
+ * Hydro.Forces call (internal/sim/gas.go:14-26). */
+int gx_hydro_forces(gx_hydro *h, const double *x, const double *y, const double *z, const double *vx,
+                    const double *vy, const double *vz, const double *mass, const uint8_t *kind,
+                    const uint8_t *alive, const double *u, double *hsml, double *rho, int64_t n,
+                    const gx_gas_params *gas, double softening, const uint8_t *active, const double *step,
+                    double *ax, double *ay, double *az, double *dudt, double *divv, double *dt, uint64_t *wake);
+
+typedef struct {
+    int64_t n_gas, n_nodes, n_leaves;
+    int32_t depth, max_iterations;  /* tree levels; most density passes of one body */
+    int64_t density_sums;           /* density evaluations over all bodies and passes */
+    int64_t pairs;                  /* force-pass neighbour pairs (j != i) */
+    int64_t leaf_visits;            /* leaves visited by the force pass's wavefronts */
+} gx_hydro_stats;
+
+int gx_hydro_get_stats(const gx_hydro *h, gx_hydro_stats *out);
+
+/* Per-phase GPU milliseconds of the last evaluation when timing is on:
+ * 0 gas list, keys and sort, 1 tree build and refit, 2 density passes,
+ * 3 leaf sort and per-body terms, 4 force pass, 5 total.  Returns the
+ * number written. */
+int gx_hydro_phase_ms(const gx_hydro *h, float *ms, int n);
+
+/* Test hook with no counterpart in the reference: copies count items of the
+ * last evaluation's tree to host memory.  what: 0 keys (uint64, sorted), 1
+ * order (int32, tree position -> state index, leaves sorted by decreasing
+ * h), 2 nodes (80-byte records: double minx, miny, minz, maxx, maxy, maxz,
+ * hmax; int32 lo, hi, first (right child), second (left child), escape,
+ * pad).  Returns the number of items copied. */
+int64_t gx_hydro_export(gx_hydro *h, int what, void *host, int64_t count);
+
 #ifdef __cplusplus
 }
 #endif
