@@ -38,7 +38,15 @@ HBM_PEAK_GBS = 8000.0
 FLOPS_PP = 18              # bh.go:145-150 counting rule (add/sub/mul/div/sqrt = 1, fma = 2)
 FLOPS_PC = 50              # bh.go:114-131
 ENCOUNTER_T = 0.12          # global step of the two-disk encounter: 1-5 % of the stars on the top rung
-CONFIGS = ("disk-halo-1m", "plummer-16k", "clumps-16m", "batched-256", "two-disk-4m", "disk-halo-1m-holes")
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector, datasheet
+# SPH counting rule (add/sub/mul/div/sqrt = 1; comparisons, min/max not counted):
+# a force pair 62 (sph.go:219-246 without the viscosity branch), + 19 when it
+# approaches (:231-235); a density term 20 (r^2, sqrt, W, m W, the sum).
+FLOPS_SPH_PAIR = 62
+FLOPS_SPH_VISC = 19
+FLOPS_SPH_DENSITY = 20
+CONFIGS = ("disk-halo-1m", "plummer-16k", "clumps-16m", "batched-256", "two-disk-4m", "disk-halo-1m-holes",
+           "sph-sphere-1m")
 
 
 def parse():
@@ -537,11 +545,86 @@ def bench_holes(args):
     return out
 
 
+def bench_sph(args):
+    """SPH hydro (sph.Solver.Forces): a Plummer sphere of 1,048,576 gas bodies
+    (u 0.2, velocities x 0.3, internal/sph/system_test.go:14-23), softening
+    0.01, default gas parameters (gamma 5/3, 40 neighbours, alpha 1).  h starts
+    at the target of the sphere's analytic density and converges in the
+    warm-up; one step = one full evaluation (gas list, keys, sort, tree,
+    density passes, refit, leaf sort, forces), FP64, bit-identical to the
+    oracle."""
+    from gravitas_amd import GasState, Hydro, ics
+    from gravitas_amd.hip import synchronize
+    from gravitas_amd.hydro import HydroOut
+    n = args.n or 1048576
+    eps = 0.01 if args.eps is None else args.eps
+    log(f"building gas sphere n={n}")
+    state = ics.gas_sphere(n, 9, h_from_density=True)
+    g = GasState.from_host(state)
+    hy = Hydro(timing=False)
+    out = HydroOut(n)
+    for _ in range(max(3, args.warmup)):
+        hy.forces(g, eps, out=out)
+    synchronize()
+    log(f"timing {args.steps} evaluations")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        hy.forces(g, eps, out=out)
+    synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    st = hy.stats()
+    hy_t = Hydro(timing=True)
+    phases = []
+    for _ in range(5):
+        hy_t.forces(g, eps, out=out)
+        phases.append(hy_t.phase_ms())
+    ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+    res = base_line(args, n, ms, "fp64", {
+        "workload": "sph-sphere-1m", "n": n, "eps": eps,
+        "model": "Plummer sphere of gas (M = a = 1), u 0.2, velocities x 0.3; gamma 5/3, 40 neighbours, alpha 1",
+        "parallelism": "single GPU"})
+    res["metric"] = "gas particles/s (SPH hydro forces, FP64, bit-identical to the reference order)"
+    force_flops = FLOPS_SPH_PAIR * st["pairs"] + FLOPS_SPH_VISC * st["visc_pairs"]
+    achieved = force_flops / (ph["force"] * 1e-3) / 1e12
+    res.update({
+        "neighbours_per_body": st["pairs"] / n, "density_passes_per_body": st["density_sums"] / n,
+        "pairs_per_s": st["pairs"] / (ms * 1e-3),
+        "tree": {"nodes": st["n_nodes"], "leaves": st["n_leaves"], "depth": st["depth"],
+                 "max_density_passes": st["max_iterations"], "leaf_visits_per_wave": st["leaf_visits"] / ((n + 63) // 64)},
+        "phases_ms": ph,
+        "roofline": {"bound": "valu", "kernel": "k_hy_force", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": force_flops, "force_ms": ph["force"],
+                     "density_tflops": FLOPS_SPH_DENSITY * st["density_pairs"] / (ph["density"] * 1e-3) / 1e12,
+                     "note": "FP64 vector ALU bound (divisions and square roots per pair); peak = MI355X FP64 "
+                             "vector 78.6 TFLOP/s"},
+    })
+    if not args.no_cpu:
+        import oracle
+        log("CPU baseline: one oracle evaluation from the converged state")
+        hs = g.host()  # the input of one more GPU evaluation, then of the oracle's
+        hy.forces(g, eps, out=out)
+        go, gs = out.host(), g.host()
+        s = {k: hs[k].copy() for k in ("px", "py", "pz", "vx", "vy", "vz", "m", "u", "h", "rho", "kind", "alive")}
+        t0 = time.perf_counter()
+        oo, _ = oracle.sph_forces(s, softening=eps)
+        tc = time.perf_counter() - t0
+        res["cpu_baseline"] = {
+            "value": n / tc, "unit": "gas particles/s", "cores": 1, "kind": "port",
+            "sample": f"the whole {n}-body workload, one Forces evaluation from the GPU-converged h and rho; "
+                      "C restatement of sph.Solver.Forces, -O2 -ffp-contract=off, 1 thread"}
+        res["gpu_over_cpu"] = res["value"] / res["cpu_baseline"]["value"]
+        res["bit_identical_to_oracle"] = bool(all(np.array_equal(go[k], oo[k]) for k in
+                                                  ("ax", "ay", "az", "dudt", "divv", "dt")) and
+                                              np.array_equal(gs["h"], s["h"]) and np.array_equal(gs["rho"], s["rho"]))
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
-              "the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU)",
+              file=sys.stderr)
         sys.exit(2)
     from gravitas_amd import gpu_available
     reason = gpu_available()
@@ -550,7 +633,7 @@ def main():
         sys.exit(1)
     fn = {"disk-halo-1m": bench_disk_halo, "plummer-16k": bench_plummer, "clumps-16m": bench_clumps,
           "batched-256": bench_batched, "two-disk-4m": bench_encounter,
-          "disk-halo-1m-holes": bench_holes}[args.config]
+          "disk-halo-1m-holes": bench_holes, "sph-sphere-1m": bench_sph}[args.config]
     print(json.dumps(fn(args)), flush=True)
 
 

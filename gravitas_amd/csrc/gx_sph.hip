@@ -11,7 +11,7 @@
 //   right child (walked first) and its escape (the next node after its
 //   subtree), so leaves are visited by decreasing range start, the order of
 //   the reference's stack walks (tree.go:205-267)
-//   refit per level (boxes, hmax), density passes, refit, leaf sort by h
+//   refit bottom up (boxes, hmax), density passes, refit, leaf sort by h
 //   (tree.go:314-331), per-body terms, force pass.
 //
 // Walks: one wavefront takes 64 consecutive tree positions.  It walks the
@@ -50,9 +50,20 @@ struct BvhNode {
     int first;   // right child, walked first; -1 for a leaf
     int second;  // left child; -1 for a leaf
     int escape;  // next node after this subtree in the walk; -1 ends it
-    int pad;
+    int parent;  // -1 for the root
 };
 static_assert(sizeof(BvhNode) == 80, "node record");
+
+// What a walk reads per node, 64 bytes (one scalar load): the box, hmax and
+// two link words.  Inside: a = right child (walked first); at a leaf:
+// a = ~lo.  b = (escape + 1) << 4 | (a leaf's body count; 0 inside).
+struct WalkNode {
+    double minx, miny, minz, maxx, maxy, maxz, hmax;
+    int a, b;
+};
+static_assert(sizeof(WalkNode) == 64, "walk record");
+
+__device__ inline int walk_escape(const WalkNode& w) { return (w.b >> 4) - 1; }
 
 struct HydroMeta {
     unsigned long long box[6];  // encoded min x, y, z and max x, y, z of the gas
@@ -63,6 +74,8 @@ struct HydroMeta {
     int leaves;
     int flags;
     unsigned long long evals, pairs, leafVisits;
+    unsigned long long densityPairs;  // neighbour terms summed by the density passes
+    unsigned long long viscPairs;     // force pairs that approach (viscosity branch)
     int maxPasses;
     int pad;
 };
@@ -132,11 +145,15 @@ __device__ inline double kernel_dw(double r, double h, double sig4) {
     return 0;
 }
 
-// boxDist2, tree.go:349-355.
-__device__ inline double box_dist2(const BvhNode& nd, double px, double py, double pz) {
-    const double dx = go_max(go_max(nd.minx - px, px - nd.maxx), 0.0);
-    const double dy = go_max(go_max(nd.miny - py, py - nd.maxy), 0.0);
-    const double dz = go_max(go_max(nd.minz - pz, pz - nd.maxz), 0.0);
+// boxDist2, tree.go:349-355, for pruning only.  A node is skipped when this
+// distance reaches the lane's reach; every body j in the node's box has
+// r2(j) >= box distance^2 in floating point too (rounding is monotone), so a
+// skipped node holds no neighbour and the sums do not depend on how exactly
+// this is rounded or on NaN/-0 handling: plain fmax, branch-free.
+__device__ inline double box_dist2(const WalkNode& nd, double px, double py, double pz) {
+    const double dx = fmax(fmax(nd.minx - px, px - nd.maxx), 0.0);
+    const double dy = fmax(fmax(nd.miny - py, py - nd.maxy), 0.0);
+    const double dz = fmax(fmax(nd.minz - pz, pz - nd.maxz), 0.0);
     return dx * dx + dy * dy + dz * dz;
 }
 
@@ -182,6 +199,7 @@ __global__ void k_hy_meta_init(HydroMeta* meta) {
         meta->leaves = 0;
         meta->flags = 0;
         meta->evals = meta->pairs = meta->leafVisits = 0;
+        meta->densityPairs = meta->viscPairs = 0;
         meta->maxPasses = 0;
     }
     for (int l = t; l <= kBvhLevels; l += blockDim.x) {
@@ -285,6 +303,7 @@ __global__ void k_hy_root(HydroMeta* meta, BvhNode* nodes) {
     r.hi = (int)meta->ngas;
     r.first = r.second = -1;
     r.escape = -1;
+    r.parent = -1;
     nodes[0] = r;
     meta->lvlStart[0] = 0;
     meta->lvlCount[0] = 1;
@@ -307,8 +326,8 @@ __device__ inline int split_range(const uint64_t* __restrict__ keys, int lo, int
 // Tree level L: leaves (<= 8 bodies) are marked, the other nodes split into
 // a left and a right child on level L + 1 (buildNode, tree.go:126-146).
 __global__ __launch_bounds__(256) void k_hy_level(int L, const uint64_t* __restrict__ keys,
-                                                  BvhNode* __restrict__ nodes, HydroMeta* meta,
-                                                  int2* __restrict__ leafRange) {
+                                                  BvhNode* __restrict__ nodes, WalkNode* __restrict__ wn,
+                                                  HydroMeta* meta, int2* __restrict__ leafRange) {
     const int start = meta->lvlStart[L], cnt = meta->lvlCount[L];
     const int next = start + cnt;
     if (blockIdx.x == 0 && threadIdx.x == 0) meta->lvlStart[L + 1] = next;
@@ -318,6 +337,8 @@ __global__ __launch_bounds__(256) void k_hy_level(int L, const uint64_t* __restr
         const int lo = nd->lo, hi = nd->hi;
         if (hi - lo <= kBvhLeaf) {
             nd->first = nd->second = -1;
+            wn[start + idx].a = ~lo;
+            wn[start + idx].b = (nd->escape + 1) << 4 | (hi - lo);
             for (int k = lo; k < hi; k++) leafRange[k] = make_int2(lo, hi);
             leaves++;
             continue;
@@ -326,6 +347,7 @@ __global__ __launch_bounds__(256) void k_hy_level(int L, const uint64_t* __restr
         const int left = next + atomicAdd(&meta->lvlCount[L + 1], 2), right = left + 1;
         BvhNode c = {};
         c.first = c.second = -1;
+        c.parent = start + idx;
         c.lo = lo;
         c.hi = mid;
         c.escape = nd->escape;
@@ -336,89 +358,134 @@ __global__ __launch_bounds__(256) void k_hy_level(int L, const uint64_t* __restr
         nodes[right] = c;
         nd->first = right;
         nd->second = left;
+        wn[start + idx].a = right;
+        wn[start + idx].b = (nd->escape + 1) << 4;
     }
     leaves = wave_sum_i(leaves);
     if (lane_id() == 0 && leaves) atomicAdd(&meta->leaves, leaves);
 }
 
-// Boxes and hmax of level L from the sorted copies or the children
+// Boxes and hmax of the nodes of one level from the sorted copies (leaves;
+// hmax from h through the order when h is given) or from the children
 // (refit, tree.go:164-180; refitH, :184-201).
-__global__ __launch_bounds__(256) void k_hy_refit(int L, const double4* __restrict__ sp,
-                                                  BvhNode* __restrict__ nodes, const HydroMeta* meta) {
-    const int start = meta->lvlStart[L], cnt = meta->lvlCount[L];
-    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < cnt; idx += gridDim.x * blockDim.x) {
-        BvhNode* nd = &nodes[start + idx];
-        double mnx = INFINITY, mny = INFINITY, mnz = INFINITY, mxx = -INFINITY, mxy = -INFINITY, mxz = -INFINITY;
-        double hm = 0;
-        if (nd->first < 0) {
-            for (int k = nd->lo; k < nd->hi; k++) {
-                const double4 p = sp[k];
-                mnx = go_min(mnx, p.x); mxx = go_max(mxx, p.x);
-                mny = go_min(mny, p.y); mxy = go_max(mxy, p.y);
-                mnz = go_min(mnz, p.z); mxz = go_max(mxz, p.z);
-                hm = go_max(hm, p.w);
-            }
-        } else {
-            const BvhNode* c[2] = {&nodes[nd->second], &nodes[nd->first]};  // left, right
-            for (int s = 0; s < 2; s++) {
-                mnx = go_min(mnx, c[s]->minx); mxx = go_max(mxx, c[s]->maxx);
-                mny = go_min(mny, c[s]->miny); mxy = go_max(mxy, c[s]->maxy);
-                mnz = go_min(mnz, c[s]->minz); mxz = go_max(mxz, c[s]->maxz);
-                hm = go_max(hm, c[s]->hmax);
-            }
+__device__ inline void refit_node(BvhNode* __restrict__ nodes, WalkNode* __restrict__ wn, int ni,
+                                  const double4* __restrict__ sp, const int32_t* __restrict__ order,
+                                  const double* __restrict__ h) {
+    BvhNode* nd = &nodes[ni];
+    double mnx = INFINITY, mny = INFINITY, mnz = INFINITY, mxx = -INFINITY, mxy = -INFINITY, mxz = -INFINITY;
+    double hm = 0;
+    if (nd->first < 0) {
+        for (int k = nd->lo; k < nd->hi; k++) {
+            const double4 p = sp[k];
+            mnx = go_min(mnx, p.x); mxx = go_max(mxx, p.x);
+            mny = go_min(mny, p.y); mxy = go_max(mxy, p.y);
+            mnz = go_min(mnz, p.z); mxz = go_max(mxz, p.z);
+            hm = go_max(hm, h ? h[order[k]] : p.w);
         }
-        nd->minx = mnx; nd->miny = mny; nd->minz = mnz;
-        nd->maxx = mxx; nd->maxy = mxy; nd->maxz = mxz;
-        nd->hmax = hm;
+    } else {
+        const BvhNode* c[2] = {&nodes[nd->second], &nodes[nd->first]};  // left, right
+        for (int s = 0; s < 2; s++) {
+            mnx = go_min(mnx, c[s]->minx); mxx = go_max(mxx, c[s]->maxx);
+            mny = go_min(mny, c[s]->miny); mxy = go_max(mxy, c[s]->maxy);
+            mnz = go_min(mnz, c[s]->minz); mxz = go_max(mxz, c[s]->maxz);
+            hm = go_max(hm, c[s]->hmax);
+        }
+    }
+    nd->minx = mnx; nd->miny = mny; nd->minz = mnz;
+    nd->maxx = mxx; nd->maxy = mxy; nd->maxz = mxz;
+    nd->hmax = hm;
+    WalkNode* w = &wn[ni];
+    w->minx = mnx; w->miny = mny; w->minz = mnz;
+    w->maxx = mxx; w->maxy = mxy; w->maxz = mxz;
+    w->hmax = hm;
+}
+
+// One wide level, a grid over its nodes.
+__global__ __launch_bounds__(256) void k_hy_refit(int L, const double4* __restrict__ sp,
+                                                  const int32_t* __restrict__ order, const double* __restrict__ h,
+                                                  BvhNode* __restrict__ nodes, WalkNode* __restrict__ wn,
+                                                  const HydroMeta* meta) {
+    const int start = meta->lvlStart[L], cnt = meta->lvlCount[L];
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < cnt; idx += gridDim.x * blockDim.x)
+        refit_node(nodes, wn, start + idx, sp, order, h);
+}
+
+// Levels Lhi down to Llo, all small, in one workgroup: the waves of a
+// workgroup share one CU and its L1, so a barrier between levels is enough
+// for the children to be seen.
+__global__ __launch_bounds__(1024) void k_hy_refit_small(int Lhi, int Llo, const double4* __restrict__ sp,
+                                                         const int32_t* __restrict__ order,
+                                                         const double* __restrict__ h, BvhNode* __restrict__ nodes,
+                                                         WalkNode* __restrict__ wn, const HydroMeta* meta) {
+    for (int L = Lhi; L >= Llo; L--) {
+        const int start = meta->lvlStart[L], cnt = meta->lvlCount[L];
+        for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) refit_node(nodes, wn, start + idx, sp, order, h);
+        __syncthreads();
     }
 }
 
-// sortLeaves, tree.go:314-331: in every leaf the bodies by decreasing h,
-// stable; out of place, one thread per position.
+// sortLeaves, tree.go:314-331: in every leaf the bodies by decreasing h
+// (the final h, read through the order), stable; out of place, one thread per
+// position; the sorted copies carry the final h.
 __global__ void k_hy_sort_leaves(const HydroMeta* meta, const int2* __restrict__ leafRange,
                                  const int32_t* __restrict__ order, const double4* __restrict__ sp,
-                                 int32_t* __restrict__ order2, double4* __restrict__ sp2) {
+                                 const double* __restrict__ h, int32_t* __restrict__ order2,
+                                 double4* __restrict__ sp2) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= meta->ngas) return;
     const int2 lr = leafRange[k];
-    const double hk = sp[k].w;
+    const int32_t i = order[k];
+    const double hk = h[i];
     int rank = 0;
     for (int f = lr.x; f < lr.y; f++) {
-        const double hf = sp[f].w;
+        const double hf = h[order[f]];
         rank += hf > hk || (hf == hk && f < k);
     }
-    order2[lr.x + rank] = order[k];
-    sp2[lr.x + rank] = sp[k];
+    order2[lr.x + rank] = i;
+    const double4 p = sp[k];
+    sp2[lr.x + rank] = make_double4(p.x, p.y, p.z, hk);
 }
 
 // ---------------------------------------------------------------------------
 // walks
 
-// The walk over the tree for the 64 bodies of a wavefront.  Visit(nd) is the
-// per-lane node test; Leaf(nd) handles a leaf every lane has entered.
+// The walk over the tree for the 64 bodies of a wavefront.  visit(nd) is the
+// per-lane node test; leaf(lo, count) handles a leaf some lane entered and
+// returns true when a lane's list is (nearly) full: the walk then returns
+// with node at the next node, to be resumed after the lists are summed.
 template <class Visit, class Leaf>
-__device__ inline void walk_tree(const BvhNode* __restrict__ nodes, int& node, Visit visit, Leaf leaf,
+__device__ inline void walk_tree(const WalkNode* __restrict__ wn, int& node, Visit visit, Leaf leaf,
                                  unsigned long long& leafVisits) {
     while (node >= 0) {
         node = __builtin_amdgcn_readfirstlane(node);
-        const BvhNode& nd = nodes[node];
-        if (__ballot(visit(nd)) == 0) {
-            node = nd.escape;
+        const WalkNode nd = wn[node];
+        const bool in = visit(nd);
+        const int escape = walk_escape(nd);
+        if (__ballot(in) == 0) {
+            node = escape;
             continue;
         }
-        if (nd.first >= 0) {
-            node = nd.first;
+        if (nd.a >= 0) {
+            node = nd.a;
             continue;
         }
         leafVisits++;
-        node = nd.escape;
-        if (leaf(nd)) return;  // a list is (nearly) full: sum it, then resume at node
+        node = escape;
+        if (leaf(~nd.a, nd.b & 15)) return;
     }
+}
+
+// The slots of a leaf, loaded together (indices clamped into the leaf; the
+// caller skips j >= hi - lo).
+__device__ inline void load_leaf(const double4* __restrict__ sp, int lo, int hi, double4 (&q)[kBvhLeaf]) {
+#pragma unroll
+    for (int j = 0; j < kBvhLeaf; j++) q[j] = sp[min(lo + j, hi - 1)];
 }
 
 // Density passes and smoothing lengths (sph.go:115-182).  Lane = tree
 // position; each evaluated body iterates on its own h, at most 6 passes.
-__global__ __launch_bounds__(64) void k_hy_density(const BvhNode* __restrict__ nodes, double4* __restrict__ sp,
+__global__ __launch_bounds__(64) void k_hy_density(const WalkNode* __restrict__ wn,
+                                                   const double4* __restrict__ sp,
                                                    const double* __restrict__ sm, const int32_t* __restrict__ order,
                                                    const uint8_t* __restrict__ active, HydroMeta* meta, double hMin,
                                                    double ngb, double* __restrict__ h, double* __restrict__ rho) {
@@ -437,7 +504,7 @@ __global__ __launch_bounds__(64) void k_hy_density(const BvhNode* __restrict__ n
     double hi = p.w, rhoi = 0;
     bool need = act;
     int passes = 0;
-    unsigned long long leafVisits = 0;
+    unsigned long long leafVisits = 0, dpairs = 0;
     for (int it = 0; it < kDensityPasses; it++) {
         if (__ballot(need) == 0) break;
         const double radius = 2 * hi;
@@ -454,17 +521,21 @@ __global__ __launch_bounds__(64) void k_hy_density(const BvhNode* __restrict__ n
                 const double r2 = ex * ex + ey * ey + ez * ez;
                 sum += sm[kk] * kernel_w(sqrt(r2), hi, sigma);
             }
+            dpairs += cnt;
             cnt = 0;
         };
         while (node >= 0) {
             walk_tree(
-                nodes, node, [&](const BvhNode& nd) { return need && !(box_dist2(nd, p.x, p.y, p.z) >= rr); },
-                [&](const BvhNode& nd) {
-                    for (int kk = nd.lo; kk < nd.hi; kk++) {
-                        const double4 q = sp[kk];
-                        const double ex = q.x - p.x, ey = q.y - p.y, ez = q.z - p.z;
+                wn, node, [&](const WalkNode& nd) { return !(box_dist2(nd, p.x, p.y, p.z) >= rr) & need; },
+                [&](int lo, int count) {
+                    double4 q[kBvhLeaf];
+                    load_leaf(sp, lo, lo + count, q);
+#pragma unroll
+                    for (int j = 0; j < kBvhLeaf; j++) {
+                        if (j >= count) break;
+                        const double ex = q[j].x - p.x, ey = q[j].y - p.y, ez = q[j].z - p.z;
                         const double r2 = ex * ex + ey * ey + ez * ez;
-                        if (need && r2 < rr) list[cnt++ * kWave + lane] = kk;
+                        if (need && r2 < rr) list[cnt++ * kWave + lane] = lo + j;
                     }
                     return __ballot(cnt > kHyList - kBvhLeaf) != 0;
                 },
@@ -484,13 +555,14 @@ __global__ __launch_bounds__(64) void k_hy_density(const BvhNode* __restrict__ n
     if (act) {
         rho[i] = rhoi;
         h[i] = hi;
-        sp[k].w = hi;
     }
     const unsigned long long ev = wave_sum_u64((unsigned long long)passes);
+    const unsigned long long dp = wave_sum_u64(dpairs);
     int mp = passes;
     for (int m = 32; m >= 1; m >>= 1) mp = max(mp, __shfl_xor(mp, m, kWave));
     if (lane == 0) {
         if (ev) atomicAdd(&meta->evals, ev);
+        if (dp) atomicAdd(&meta->densityPairs, dp);
         atomicMax(&meta->maxPasses, mp);
     }
 }
@@ -534,7 +606,7 @@ __device__ inline void lower_wake(unsigned long long* w, double dt) {
 
 // Forces, energy equation and velocity divergence of the evaluated bodies
 // (sph.go:196-261), and the wake of neighbours with too long steps.
-__global__ __launch_bounds__(64) void k_hy_force(const BvhNode* __restrict__ nodes, const double4* __restrict__ sp,
+__global__ __launch_bounds__(64) void k_hy_force(const WalkNode* __restrict__ wn, const double4* __restrict__ sp,
                                                  const double4* __restrict__ sv, const HyTerms* __restrict__ st,
                                                  const double* __restrict__ sstep, const int32_t* __restrict__ order,
                                                  const uint8_t* __restrict__ active, HydroMeta* meta, double alpha,
@@ -560,7 +632,7 @@ __global__ __launch_bounds__(64) void k_hy_force(const BvhNode* __restrict__ nod
     const double r2max = radius * radius;
     double fx = 0, fy = 0, fz = 0, du = 0, dv = 0;
     double vsig = 2 * ci, minStep = INFINITY, maxStep = 0.0;
-    unsigned long long leafVisits = 0, pairs = 0;
+    unsigned long long leafVisits = 0, pairs = 0, viscPairs = 0;
     int cnt = 0;
     int node = 0;
     // scan's test of a body slot (tree.go:295-307)
@@ -573,10 +645,10 @@ __global__ __launch_bounds__(64) void k_hy_force(const BvhNode* __restrict__ nod
         return r2 < lim2;
     };
     // each's node test with symmetric set (tree.go:214-221)
-    auto enters = [&](const BvhNode& nd, bool on) {
+    auto enters = [&](const WalkNode& nd, bool on) {
         double reach = radius;
         if (2 * nd.hmax > reach) reach = 2 * nd.hmax;
-        return on && !(box_dist2(nd, p.x, p.y, p.z) >= reach * reach);
+        return !(box_dist2(nd, p.x, p.y, p.z) >= reach * reach) & on;
     };
     auto flush = [&]() {
         for (int e = 0; e < cnt; e++) {
@@ -603,6 +675,7 @@ __global__ __launch_bounds__(64) void k_hy_force(const BvhNode* __restrict__ nod
             const double fj = tj.f;
             double visc = 0;
             if (vr < 0) {
+                viscPairs++;
                 const double hm = 0.5 * (hi + q.w);
                 const double mu = hm * vr * r / (r2 + 0.01 * hm * hm);
                 visc = (-alpha * 0.5 * (ci + tj.c) * mu + 2 * alpha * mu * mu) / (0.5 * (rhoi + tj.rho));
@@ -619,12 +692,15 @@ __global__ __launch_bounds__(64) void k_hy_force(const BvhNode* __restrict__ nod
     };
     while (node >= 0) {
         walk_tree(
-            nodes, node, [&](const BvhNode& nd) { return enters(nd, act); },
-            [&](const BvhNode& nd) {
-                for (int kk = nd.lo; kk < nd.hi; kk++) {
-                    const double4 q = sp[kk];
+            wn, node, [&](const WalkNode& nd) { return enters(nd, act); },
+            [&](int lo, int count) {
+                double4 q[kBvhLeaf];
+                load_leaf(sp, lo, lo + count, q);
+#pragma unroll
+                for (int j = 0; j < kBvhLeaf; j++) {
+                    if (j >= count) break;
                     double r2;
-                    if (act && reaches(q, r2) && kk != k && r2 != 0) list[cnt++ * kWave + lane] = kk;
+                    if (act && reaches(q[j], r2) && lo + j != k && r2 != 0) list[cnt++ * kWave + lane] = lo + j;
                 }
                 return __ballot(cnt > kHyList - kBvhLeaf) != 0;
             },
@@ -646,20 +722,25 @@ __global__ __launch_bounds__(64) void k_hy_force(const BvhNode* __restrict__ nod
     if (__ballot(wakeNeeded) != 0) {
         node = 0;
         walk_tree(
-            nodes, node, [&](const BvhNode& nd) { return enters(nd, wakeNeeded); },
-            [&](const BvhNode& nd) {
-                for (int kk = nd.lo; kk < nd.hi; kk++) {
-                    const double4 q = sp[kk];
+            wn, node, [&](const WalkNode& nd) { return enters(nd, wakeNeeded); },
+            [&](int lo, int count) {
+                double4 q[kBvhLeaf];
+                load_leaf(sp, lo, lo + count, q);
+#pragma unroll
+                for (int j = 0; j < kBvhLeaf; j++) {
+                    if (j >= count) break;
+                    const int kk = lo + j;
                     double r2;
-                    if (wakeNeeded && reaches(q, r2) && sstep[kk] > 4 * dt) lower_wake(&wake[order[kk]], dt);
+                    if (wakeNeeded && reaches(q[j], r2) && sstep[kk] > 4 * dt) lower_wake(&wake[order[kk]], dt);
                 }
                 return false;
             },
             leafVisits);
     }
-    const unsigned long long pw = wave_sum_u64(pairs);
+    const unsigned long long pw = wave_sum_u64(pairs), vw = wave_sum_u64(viscPairs);
     if (lane == 0) {
         if (pw) atomicAdd(&meta->pairs, pw);
+        if (vw) atomicAdd(&meta->viscPairs, vw);
         atomicAdd(&meta->leafVisits, leafVisits);
     }
 }
@@ -710,7 +791,7 @@ struct gx_hydro {
     hipStream_t own = nullptr, stream = nullptr;
     int timing = 0;
     Buf meta, sortMeta, flag, tileCnt, gas, keys, keys2, vals, vals2, hist, digitTotal;
-    Buf sp, sp2, sm, order2, nodes, leafRange, sv, terms, sstep;
+    Buf sp, sp2, sm, order2, nodes, walk, leafRange, sv, terms, sstep;
     Buf stage[21];  // host-array entry point
     HydroMeta* hmeta = nullptr;  // pinned mirror
     int depth = 0;               // levels of the last tree, for the next build's first batch
@@ -786,6 +867,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
         (rc = grow(H->digitTotal, sizeof(uint32_t) * 256)) || (rc = grow(H->sp, sizeof(double4) * g1)) ||
         (rc = grow(H->sp2, sizeof(double4) * g1)) || (rc = grow(H->sm, 8 * g1)) ||
         (rc = grow(H->order2, 4 * g1)) || (rc = grow(H->nodes, sizeof(BvhNode) * maxNodes)) ||
+        (rc = grow(H->walk, sizeof(WalkNode) * maxNodes)) ||
         (rc = grow(H->leafRange, sizeof(int2) * g1)) || (rc = grow(H->sv, sizeof(double4) * g1)) ||
         (rc = grow(H->terms, sizeof(HyTerms) * g1)) || (step && (rc = grow(H->sstep, 8 * g1))))
         return rc;
@@ -817,6 +899,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
     mark(1);
     // tree levels; the first batch covers the last tree's depth + 4
     BvhNode* nodes = (BvhNode*)H->nodes.p;
+    WalkNode* wn = (WalkNode*)H->walk.p;
     int2* leafRange = (int2*)H->leafRange.p;
     hipLaunchKernelGGL(k_hy_root, dim3(1), dim3(1), 0, st, meta, nodes);
     int built = 0;
@@ -825,7 +908,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
         for (int L = built; L < want; L++) {
             const long long upper = std::min<long long>(L < 40 ? (1ll << L) : ngas, ngas);
             hipLaunchKernelGGL(k_hy_level, dim3((unsigned)std::min<long long>(1024, blocks_of(upper, 256))),
-                               dim3(256), 0, st, L, (const uint64_t*)H->keys.p, nodes, meta, leafRange);
+                               dim3(256), 0, st, L, (const uint64_t*)H->keys.p, nodes, wn, meta, leafRange);
         }
         built = want;
         HYCHK(hipMemcpyAsync(H->hmeta, meta, sizeof(HydroMeta), hipMemcpyDeviceToHost, st));
@@ -844,23 +927,37 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
     }
     H->depth = depth;
     H->nnodes = nn;
-    auto refit = [&]() {
-        for (int L = depth - 1; L >= 0; L--) {
+    // Bottom-up refit: runs of levels with at most kSmallLevel nodes go to one
+    // workgroup each, wider levels get a grid launch of their own.
+    constexpr int kSmallLevel = 2048;
+    auto refit = [&](const double* hNew) {
+        int L = depth - 1;
+        while (L >= 0) {
             const long long cntL = H->hmeta->lvlCount[L];
-            hipLaunchKernelGGL(k_hy_refit, dim3((unsigned)std::min<long long>(1024, blocks_of(cntL, 256))), dim3(256),
-                               0, st, L, (const double4*)sp, nodes, (const HydroMeta*)meta);
+            if (cntL > kSmallLevel) {
+                hipLaunchKernelGGL(k_hy_refit, dim3((unsigned)std::min<long long>(1024, blocks_of(cntL, 256))),
+                                   dim3(256), 0, st, L, (const double4*)sp, order, hNew, nodes, wn,
+                                   (const HydroMeta*)meta);
+                L--;
+                continue;
+            }
+            int lo = L;
+            while (lo - 1 >= 0 && H->hmeta->lvlCount[lo - 1] <= kSmallLevel) lo--;
+            hipLaunchKernelGGL(k_hy_refit_small, dim3(1), dim3(1024), 0, st, L, lo, (const double4*)sp, order, hNew,
+                               nodes, wn, (const HydroMeta*)meta);
+            L = lo - 1;
         }
     };
-    refit();
+    refit(nullptr);
     mark(2);
     const long long nw = (ngas + kWave - 1) / kWave;
     const unsigned waveGrid = (unsigned)(8 * ((nw + 7) / 8));
-    hipLaunchKernelGGL(k_hy_density, dim3(waveGrid), dim3(kWave), 0, st, (const BvhNode*)nodes, sp,
+    hipLaunchKernelGGL(k_hy_density, dim3(waveGrid), dim3(kWave), 0, st, (const WalkNode*)wn, (const double4*)sp,
                        (const double*)H->sm.p, order, active, meta, 0.25 * softening, ngb, h, rho);
     mark(3);
-    refit();  // hmax of the final smoothing lengths (refitH)
+    refit(h);  // hmax of the final smoothing lengths (refitH)
     hipLaunchKernelGGL(k_hy_sort_leaves, dim3(blocks_of(ngas, 256)), dim3(256), 0, st, (const HydroMeta*)meta,
-                       (const int2*)leafRange, order, (const double4*)sp, (int32_t*)H->order2.p,
+                       (const int2*)leafRange, order, (const double4*)sp, (const double*)h, (int32_t*)H->order2.p,
                        (double4*)H->sp2.p);
     const int32_t* order2 = (const int32_t*)H->order2.p;
     const double4* sp2 = (const double4*)H->sp2.p;
@@ -868,7 +965,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
                        vx, vy, vz, m, (const double*)rho, u, step, gamma, (double4*)H->sv.p, (HyTerms*)H->terms.p,
                        step ? (double*)H->sstep.p : nullptr);
     mark(4);
-    hipLaunchKernelGGL(k_hy_force, dim3(waveGrid), dim3(kWave), 0, st, (const BvhNode*)nodes, sp2,
+    hipLaunchKernelGGL(k_hy_force, dim3(waveGrid), dim3(kWave), 0, st, (const WalkNode*)wn, sp2,
                        (const double4*)H->sv.p, (const HyTerms*)H->terms.p,
                        step ? (const double*)H->sstep.p : nullptr, order2, active, meta, alpha, ax, ay, az, dudt,
                        divv, dtOut, (unsigned long long*)wake);
@@ -883,6 +980,8 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
     H->stats.density_sums = (int64_t)H->hmeta->evals;
     H->stats.pairs = (int64_t)H->hmeta->pairs;
     H->stats.leaf_visits = (int64_t)H->hmeta->leafVisits;
+    H->stats.density_pairs = (int64_t)H->hmeta->densityPairs;
+    H->stats.visc_pairs = (int64_t)H->hmeta->viscPairs;
     if (H->timing) {
         for (int p = 0; p < kHyPhases; p++) (void)hipEventElapsedTime(&H->phase[p], H->ev[p], H->ev[p + 1]);
         (void)hipEventElapsedTime(&H->phase[kHyPhases], H->ev[0], H->ev[kHyPhases]);
@@ -914,7 +1013,7 @@ void gx_hydro_free(gx_hydro* H) {
     if (H->stream) (void)hipStreamSynchronize(H->stream);
     Buf* bufs[] = {&H->meta, &H->sortMeta, &H->flag, &H->tileCnt, &H->gas, &H->keys, &H->keys2, &H->vals,
                    &H->vals2, &H->hist, &H->digitTotal, &H->sp, &H->sp2, &H->sm, &H->order2, &H->nodes,
-                   &H->leafRange, &H->sv, &H->terms, &H->sstep};
+                   &H->leafRange, &H->sv, &H->terms, &H->sstep, &H->walk};
     for (Buf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (Buf& b : H->stage)
@@ -946,7 +1045,8 @@ int gx_hydro_forces_dev(gx_hydro* H, const double* x, const double* y, const dou
                         uint64_t* wake) {
     if (!H) return set_error(GX_ERR_ARG, "null hydro");
     if (n < 0) return set_error(GX_ERR_ARG, "negative n");
-    if (n >= (1LL << 30)) return set_error(GX_ERR_ARG, "n = %lld exceeds the tree's int32 node range", (long long)n);
+    if (n >= (1LL << 26))  // walk links hold (node + 1) << 4 in an int32
+        return set_error(GX_ERR_ARG, "n = %lld exceeds the hydro's limit of 2^26 bodies", (long long)n);
     if (n == 0) return GX_OK;
     if (!x || !y || !z || !vx || !vy || !vz || !mass || !kind || !alive || !u || !hsml || !rho || !ax || !ay ||
         !az || !dudt || !divv || !dt || !wake)
@@ -962,6 +1062,8 @@ int gx_hydro_forces(gx_hydro* H, const double* x, const double* y, const double*
                     double* ax, double* ay, double* az, double* dudt, double* divv, double* dt, uint64_t* wake) {
     if (!H) return set_error(GX_ERR_ARG, "null hydro");
     if (n < 0) return set_error(GX_ERR_ARG, "negative n");
+    if (n >= (1LL << 26))
+        return set_error(GX_ERR_ARG, "n = %lld exceeds the hydro's limit of 2^26 bodies", (long long)n);
     if (n == 0) return GX_OK;
     if (!x || !y || !z || !vx || !vy || !vz || !mass || !kind || !alive || !u || !hsml || !rho || !ax || !ay ||
         !az || !dudt || !divv || !dt || !wake)

@@ -316,13 +316,21 @@ def gas_cloud(n, seed):
     return _gas(_state(pos, vel, m, KIND_GAS), u)
 
 
-def gas_sphere(n, seed=9):
+def gas_sphere(n, seed=9, neighbors=40.0, h_from_density=False):
     """Plummer sphere of gas made hot enough to be roughly pressure supported
-    (internal/sph/system_test.go:14-23): u = 0.2, velocities scaled by 0.3."""
+    (internal/sph/system_test.go:14-23): u = 0.2, velocities scaled by 0.3.
+    With h_from_density the smoothing lengths start at the hydro's target for
+    the sphere's analytic density, 0.5 (3 N m / 4 pi rho(r))^(1/3) (sph.go:117-
+    118), instead of 0 (the hydro's box-volume guess)."""
     s = plummer(n, seed)
     for k in ("vx", "vy", "vz"):
         s[k] = s[k] * 0.3
-    return _gas(s, np.full(n, 0.2))
+    s = _gas(s, np.full(n, 0.2))
+    if h_from_density:
+        r2 = s["px"] ** 2 + s["py"] ** 2 + s["pz"] ** 2
+        rho = 3.0 / (4 * math.pi) * (1 + r2) ** -2.5  # M = a = 1
+        s["h"] = 0.5 * np.cbrt(3 * neighbors * s["m"] / (4 * math.pi * rho))
+    return s
 
 
 def gas_lattice(k=16, spacing=0.1):

@@ -317,6 +317,8 @@ typedef struct {
     int64_t density_sums;           /* density evaluations over all bodies and passes */
     int64_t pairs;                  /* force-pass neighbour pairs (j != i) */
     int64_t leaf_visits;            /* leaves visited by the force pass's wavefronts */
+    int64_t density_pairs;          /* neighbour terms summed by the density passes (self included) */
+    int64_t visc_pairs;             /* force pairs that approach (the viscosity branch) */
 } gx_hydro_stats;
 
 int gx_hydro_get_stats(const gx_hydro *h, gx_hydro_stats *out);
