@@ -7,19 +7,18 @@
 //   stable radix sort (the gravity sort kernels) -> sorted copies (x, y, z, h)
 //   tree levels: one launch per level, each node split where its keys first
 //   differ (tree.go:126-161); children are numbered per level by atomics,
-//   but the walk order does not depend on numbering: every node links its
-//   right child (walked first) and its escape (the next node after its
-//   subtree), so leaves are visited by decreasing range start, the order of
+//   but the walk order does not depend on numbering: the walk visits right
+//   children first, so leaves come by decreasing range start, the order of
 //   the reference's stack walks (tree.go:205-267)
 //   refit bottom up (boxes, hmax), density passes, refit, leaf sort by h
 //   (tree.go:314-331), per-body terms, force pass.
 //
 // Walks: one wavefront takes 64 consecutive tree positions.  It walks the
-// tree once for all of them (a node is entered if any lane reaches it, the
-// reference's own box test per lane), and every lane appends the slots of a
-// visited leaf that pass its own test to its list in LDS; the lanes then sum
-// their lists, each at full width.  A list that fills up is summed and the
-// walk resumes where it stopped (the walk state is one node index).  So each
+// tree once for all of them, four nodes per step (a node is entered if any
+// lane reaches it, the reference's own box test per lane); every lane keeps
+// the leaves it accepts, tests their slots and appends its neighbours to its
+// list in LDS; the lanes then sum their lists, each at full width.  Lists
+// that fill up are worked off and the walk resumes from its stack.  So each
 // body adds exactly the reference's neighbours, in its order: leaves by
 // decreasing range start, bodies in leaf order.  Everything is FP64 with
 // -ffp-contract=off; W, DW and the force terms keep the reference's
@@ -40,7 +39,8 @@ namespace {
 
 constexpr int kBvhLeaf = 8;        // leafSize, tree.go:17
 constexpr int kBvhLevels = 256;    // tree levels the meta record holds
-constexpr int kHyList = 48;        // neighbour slots per lane before a flush
+constexpr int kHyList = 24;        // neighbour slots per lane before they are summed
+constexpr int kLeafList = 12;      // accepted leaves per lane before their slots are tested
 constexpr int kDensityPasses = 6;  // maxDensityIterations, sph.go:41
 constexpr double kPi = 3.14159265358979323846;
 
@@ -54,16 +54,23 @@ struct BvhNode {
 };
 static_assert(sizeof(BvhNode) == 80, "node record");
 
-// What a walk reads per node, 64 bytes (one scalar load): the box, hmax and
-// two link words.  Inside: a = right child (walked first); at a leaf:
-// a = ~lo.  b = (escape + 1) << 4 | (a leaf's body count; 0 inside).
-struct WalkNode {
-    double minx, miny, minz, maxx, maxy, maxz, hmax;
-    int a, b;
+// What a walk reads per step, 256 bytes (one batch of scalar loads): the
+// boxes and hmax of up to four nodes, the grandchildren of an inner node in
+// the reference's walk order (right child's right, right child's left, left
+// child's right, left child's left; a child that is a leaf stands for
+// itself).  ref >= 0: an inner node, whose own record is quad[ref]; ref < 0:
+// a leaf, ~(lo << 4 | count).
+struct QuadNode {
+    double minx[4], miny[4], minz[4], maxx[4], maxy[4], maxz[4], hmax[4];
+    int ref[4];
+    int count;
+    int pad[3];
 };
-static_assert(sizeof(WalkNode) == 64, "walk record");
+static_assert(sizeof(QuadNode) == 256, "walk record");
 
-__device__ inline int walk_escape(const WalkNode& w) { return (w.b >> 4) - 1; }
+struct NodeBox {
+    double minx, miny, minz, maxx, maxy, maxz, hmax;
+};
 
 struct HydroMeta {
     unsigned long long box[6];  // encoded min x, y, z and max x, y, z of the gas
@@ -150,7 +157,7 @@ __device__ inline double kernel_dw(double r, double h, double sig4) {
 // r2(j) >= box distance^2 in floating point too (rounding is monotone), so a
 // skipped node holds no neighbour and the sums do not depend on how exactly
 // this is rounded or on NaN/-0 handling: plain fmax, branch-free.
-__device__ inline double box_dist2(const WalkNode& nd, double px, double py, double pz) {
+__device__ inline double box_dist2(const NodeBox& nd, double px, double py, double pz) {
     const double dx = fmax(fmax(nd.minx - px, px - nd.maxx), 0.0);
     const double dy = fmax(fmax(nd.miny - py, py - nd.maxy), 0.0);
     const double dz = fmax(fmax(nd.minz - pz, pz - nd.maxz), 0.0);
@@ -326,8 +333,8 @@ __device__ inline int split_range(const uint64_t* __restrict__ keys, int lo, int
 // Tree level L: leaves (<= 8 bodies) are marked, the other nodes split into
 // a left and a right child on level L + 1 (buildNode, tree.go:126-146).
 __global__ __launch_bounds__(256) void k_hy_level(int L, const uint64_t* __restrict__ keys,
-                                                  BvhNode* __restrict__ nodes, WalkNode* __restrict__ wn,
-                                                  HydroMeta* meta, int2* __restrict__ leafRange) {
+                                                  BvhNode* __restrict__ nodes, HydroMeta* meta,
+                                                  int2* __restrict__ leafRange) {
     const int start = meta->lvlStart[L], cnt = meta->lvlCount[L];
     const int next = start + cnt;
     if (blockIdx.x == 0 && threadIdx.x == 0) meta->lvlStart[L + 1] = next;
@@ -337,8 +344,6 @@ __global__ __launch_bounds__(256) void k_hy_level(int L, const uint64_t* __restr
         const int lo = nd->lo, hi = nd->hi;
         if (hi - lo <= kBvhLeaf) {
             nd->first = nd->second = -1;
-            wn[start + idx].a = ~lo;
-            wn[start + idx].b = (nd->escape + 1) << 4 | (hi - lo);
             for (int k = lo; k < hi; k++) leafRange[k] = make_int2(lo, hi);
             leaves++;
             continue;
@@ -358,8 +363,6 @@ __global__ __launch_bounds__(256) void k_hy_level(int L, const uint64_t* __restr
         nodes[right] = c;
         nd->first = right;
         nd->second = left;
-        wn[start + idx].a = right;
-        wn[start + idx].b = (nd->escape + 1) << 4;
     }
     leaves = wave_sum_i(leaves);
     if (lane_id() == 0 && leaves) atomicAdd(&meta->leaves, leaves);
@@ -368,7 +371,7 @@ __global__ __launch_bounds__(256) void k_hy_level(int L, const uint64_t* __restr
 // Boxes and hmax of the nodes of one level from the sorted copies (leaves;
 // hmax from h through the order when h is given) or from the children
 // (refit, tree.go:164-180; refitH, :184-201).
-__device__ inline void refit_node(BvhNode* __restrict__ nodes, WalkNode* __restrict__ wn, int ni,
+__device__ inline void refit_node(BvhNode* __restrict__ nodes, int ni,
                                   const double4* __restrict__ sp, const int32_t* __restrict__ order,
                                   const double* __restrict__ h) {
     BvhNode* nd = &nodes[ni];
@@ -394,20 +397,15 @@ __device__ inline void refit_node(BvhNode* __restrict__ nodes, WalkNode* __restr
     nd->minx = mnx; nd->miny = mny; nd->minz = mnz;
     nd->maxx = mxx; nd->maxy = mxy; nd->maxz = mxz;
     nd->hmax = hm;
-    WalkNode* w = &wn[ni];
-    w->minx = mnx; w->miny = mny; w->minz = mnz;
-    w->maxx = mxx; w->maxy = mxy; w->maxz = mxz;
-    w->hmax = hm;
 }
 
 // One wide level, a grid over its nodes.
 __global__ __launch_bounds__(256) void k_hy_refit(int L, const double4* __restrict__ sp,
                                                   const int32_t* __restrict__ order, const double* __restrict__ h,
-                                                  BvhNode* __restrict__ nodes, WalkNode* __restrict__ wn,
-                                                  const HydroMeta* meta) {
+                                                  BvhNode* __restrict__ nodes, const HydroMeta* meta) {
     const int start = meta->lvlStart[L], cnt = meta->lvlCount[L];
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < cnt; idx += gridDim.x * blockDim.x)
-        refit_node(nodes, wn, start + idx, sp, order, h);
+        refit_node(nodes, start + idx, sp, order, h);
 }
 
 // Levels Lhi down to Llo, all small, in one workgroup: the waves of a
@@ -416,12 +414,55 @@ __global__ __launch_bounds__(256) void k_hy_refit(int L, const double4* __restri
 __global__ __launch_bounds__(1024) void k_hy_refit_small(int Lhi, int Llo, const double4* __restrict__ sp,
                                                          const int32_t* __restrict__ order,
                                                          const double* __restrict__ h, BvhNode* __restrict__ nodes,
-                                                         WalkNode* __restrict__ wn, const HydroMeta* meta) {
+                                                         const HydroMeta* meta) {
     for (int L = Lhi; L >= Llo; L--) {
         const int start = meta->lvlStart[L], cnt = meta->lvlCount[L];
-        for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) refit_node(nodes, wn, start + idx, sp, order, h);
+        for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) refit_node(nodes, start + idx, sp, order, h);
         __syncthreads();
     }
+}
+
+// The walk record of every inner node (and of the root when it is a leaf):
+// its grandchildren in walk order, with their boxes and hmax.
+__global__ __launch_bounds__(256) void k_hy_quad(const BvhNode* __restrict__ nodes, long long nnodes,
+                                                 QuadNode* __restrict__ quad) {
+    const long long ni = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ni >= nnodes) return;
+    const BvhNode& nd = nodes[ni];
+    int f[4], n = 0;
+    if (nd.first < 0) {
+        if (ni != 0) return;
+        f[n++] = 0;  // a root leaf: a walk of one leaf
+    } else {
+        const int c[2] = {nd.first, nd.second};  // right, left
+        for (int s = 0; s < 2; s++) {
+            const BvhNode& cn = nodes[c[s]];
+            if (cn.first >= 0) {
+                f[n++] = cn.first;
+                f[n++] = cn.second;
+            } else {
+                f[n++] = c[s];
+            }
+        }
+    }
+    QuadNode q;
+    for (int s = 0; s < 4; s++) {
+        if (s < n) {
+            const BvhNode& e = nodes[f[s]];
+            q.minx[s] = e.minx; q.miny[s] = e.miny; q.minz[s] = e.minz;
+            q.maxx[s] = e.maxx; q.maxy[s] = e.maxy; q.maxz[s] = e.maxz;
+            q.hmax[s] = e.hmax;
+            q.ref[s] = e.first >= 0 ? f[s] : ~(e.lo << 4 | (e.hi - e.lo));
+        } else {
+            q.minx[s] = q.miny[s] = q.minz[s] = INFINITY;
+            q.maxx[s] = q.maxy[s] = q.maxz[s] = -INFINITY;
+            q.hmax[s] = 0;
+            q.ref[s] = 0;
+        }
+    }
+    q.count = n;
+    q.pad[0] = q.pad[1] = q.pad[2] = 0;
+    quad[ni] = q;
 }
 
 // sortLeaves, tree.go:314-331: in every leaf the bodies by decreasing h
@@ -449,47 +490,150 @@ __global__ void k_hy_sort_leaves(const HydroMeta* meta, const int2* __restrict__
 // ---------------------------------------------------------------------------
 // walks
 
-// The walk over the tree for the 64 bodies of a wavefront.  visit(nd) is the
-// per-lane node test; leaf(lo, count) handles a leaf some lane entered and
-// returns true when a lane's list is (nearly) full: the walk then returns
-// with node at the next node, to be resumed after the lists are summed.
+// The walk over the tree for the 64 bodies of a wavefront, resumable: the
+// stack (LDS, wave-uniform entries: ref, and for a leaf the mask of lanes
+// that accept it) holds what is still to visit, top first.  Popping an inner
+// node loads its record and tests its (up to four) grandchildren per lane
+// (visit); those some lane accepts are pushed in reverse, so they pop in walk
+// order.  Popping a leaf calls leaf(lo, count, in) (in: this lane accepts
+// it); leaf returns true when a lane's list is full, and the walk returns
+// with the stack as it stands.  A descendant is accepted only where its
+// ancestors are (smaller box, smaller hmax), so skipping the tests of the
+// children in between loses no leaf.
+struct WalkStack {
+    int* ref;
+    unsigned long long* mask;
+    int cap;
+    int top;
+};
+
 template <class Visit, class Leaf>
-__device__ inline void walk_tree(const WalkNode* __restrict__ wn, int& node, Visit visit, Leaf leaf,
-                                 unsigned long long& leafVisits) {
-    while (node >= 0) {
-        node = __builtin_amdgcn_readfirstlane(node);
-        const WalkNode nd = wn[node];
-        const bool in = visit(nd);
-        const int escape = walk_escape(nd);
-        if (__ballot(in) == 0) {
-            node = escape;
+__device__ inline void walk_tree(const QuadNode* __restrict__ quad, WalkStack& s, Visit visit, Leaf leaf,
+                                 unsigned long long& steps, int* overflow) {
+    const int lane = lane_id();
+    while (s.top > 0) {
+        s.top--;
+        const int ref = __builtin_amdgcn_readfirstlane(s.ref[s.top]);
+        if (ref < 0) {
+            const unsigned long long m = s.mask[s.top];
+            const int e = ~ref;
+            if (leaf(e >> 4, e & 15, ((m >> lane) & 1) != 0)) return;
             continue;
         }
-        if (nd.a >= 0) {
-            node = nd.a;
-            continue;
+        steps++;
+        const QuadNode& q = quad[ref];
+        const int n = q.count;
+        unsigned long long m[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const NodeBox b = {q.minx[c], q.miny[c], q.minz[c], q.maxx[c], q.maxy[c], q.maxz[c], q.hmax[c]};
+            m[c] = __ballot(visit(b)) & (c < n ? ~0ull : 0ull);
         }
-        leafVisits++;
-        node = escape;
-        if (leaf(~nd.a, nd.b & 15)) return;
+#pragma unroll
+        for (int c = 3; c >= 0; c--) {
+            if (m[c] == 0) continue;
+            if (s.top >= s.cap) {  // cannot happen with the host's capacity; abandon the walk, report it
+                if (lane == 0) atomicOr(overflow, 1);
+                s.top = 0;
+                return;
+            }
+            if (lane == 0) {
+                s.ref[s.top] = q.ref[c];
+                s.mask[s.top] = m[c];
+            }
+            s.top++;
+        }
     }
 }
 
-// The slots of a leaf, loaded together (indices clamped into the leaf; the
-// caller skips j >= hi - lo).
-__device__ inline void load_leaf(const double4* __restrict__ sp, int lo, int hi, double4 (&q)[kBvhLeaf]) {
+__device__ inline int wave_max_i(int v) {
+    for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, kWave));
+    return v;
+}
+
+// Neighbours of the wavefront's bodies in the reference's order.  The walk is
+// shared; every lane keeps the leaves it accepts (leaf list, LDS, kLeafList
+// per lane), then tests the slots of its own leaves (slot(q, k) on the sorted
+// copy q of tree position k) and appends the hits to its neighbour list (LDS,
+// kHyList per lane); sum(list, cnt) works off the neighbour lists, in order.
+// Leaves come in walk order (decreasing range start), slots in leaf order, so
+// each body sees its neighbours in the order of the reference's scans.
+template <class Visit, class Slot, class Sum>
+__device__ inline void neighbour_walk(const QuadNode* __restrict__ quad, const double4* __restrict__ sp,
+                                      int* __restrict__ llist, int* __restrict__ nlist, WalkStack& stk, int lane,
+                                      Visit visit, Slot slot, Sum sum, unsigned long long& steps, int* overflow) {
+    int nl = 0, cnt = 0;
+    stk.top = 1;
+    if (lane == 0) {
+        stk.ref[0] = 0;
+        stk.mask[0] = ~0ull;
+    }
+    while (stk.top > 0) {
+        walk_tree(
+            quad, stk, visit,
+            [&](int lo, int count, bool in) {
+                if (in) llist[nl++ * kWave + lane] = lo << 4 | count;
+                return __ballot(nl >= kLeafList) != 0;
+            },
+            steps, overflow);
+        const int maxl = wave_max_i(nl);
+        for (int e = 0; e < maxl; e++) {
+            const int ent = e < nl ? llist[e * kWave + lane] : 0;
+            const int lo = ent >> 4, count = ent & 15;
+            const int last = max(count, 1) - 1;
 #pragma unroll
-    for (int j = 0; j < kBvhLeaf; j++) q[j] = sp[min(lo + j, hi - 1)];
+            for (int j0 = 0; j0 < kBvhLeaf; j0 += 4) {  // the slots in two batches of gathers
+                double4 q[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) q[j] = sp[lo + min(j0 + j, last)];
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (j0 + j < count && slot(q[j], lo + j0 + j)) nlist[cnt++ * kWave + lane] = lo + j0 + j;
+            }
+            if (__ballot(cnt > kHyList - kBvhLeaf) != 0) {
+                sum(nlist, cnt);
+                cnt = 0;
+            }
+        }
+        nl = 0;
+    }
+    sum(nlist, cnt);
+}
+
+// LDS of a walk kernel: leaf lists, neighbour lists, then the stack.
+__host__ __device__ inline size_t walk_lds_bytes(int stackCap) {
+    return sizeof(int) * (kLeafList + kHyList) * kWave + (sizeof(int) + sizeof(unsigned long long)) * stackCap + 16;
 }
 
 // Density passes and smoothing lengths (sph.go:115-182).  Lane = tree
 // position; each evaluated body iterates on its own h, at most 6 passes.
-__global__ __launch_bounds__(64) void k_hy_density(const WalkNode* __restrict__ wn,
+extern __shared__ __align__(16) unsigned char hy_lds[];
+
+// The LDS of a walk kernel carved into leaf lists, neighbour lists and stack.
+struct WalkLds {
+    int* llist;
+    int* nlist;
+    WalkStack stk;
+};
+
+__device__ inline WalkLds walk_lds(int stackCap) {
+    WalkLds l;
+    l.llist = (int*)hy_lds;
+    l.nlist = l.llist + kLeafList * kWave;
+    unsigned long long* mask = (unsigned long long*)(l.nlist + kHyList * kWave);  // 8-byte aligned
+    l.stk.mask = mask;
+    l.stk.ref = (int*)(mask + stackCap);
+    l.stk.cap = stackCap;
+    l.stk.top = 0;
+    return l;
+}
+
+__global__ __launch_bounds__(64) void k_hy_density(const QuadNode* __restrict__ quad, int stackCap,
                                                    const double4* __restrict__ sp,
                                                    const double* __restrict__ sm, const int32_t* __restrict__ order,
                                                    const uint8_t* __restrict__ active, HydroMeta* meta, double hMin,
                                                    double ngb, double* __restrict__ h, double* __restrict__ rho) {
-    __shared__ int list[kHyList * kWave];
+    WalkLds L = walk_lds(stackCap);
     const long long ngas = meta->ngas;
     const long long nw = (ngas + kWave - 1) / kWave;
     const long long w = xcd_wave(nw);
@@ -510,42 +654,29 @@ __global__ __launch_bounds__(64) void k_hy_density(const WalkNode* __restrict__ 
         const double radius = 2 * hi;
         const double rr = radius * radius;
         const double sigma = 1 / (kPi * hi * hi * hi);
-        double sum = 0;
-        int cnt = 0;
-        int node = 0;
-        auto flush = [&]() {
-            for (int e = 0; e < cnt; e++) {
-                const int kk = list[e * kWave + lane];
-                const double4 q = sp[kk];
+        double rsum = 0;
+        neighbour_walk(
+            quad, sp, L.llist, L.nlist, L.stk, lane,
+            [&](const NodeBox& nd) { return !(box_dist2(nd, p.x, p.y, p.z) >= rr) & need; },
+            [&](const double4& q, int) {
                 const double ex = q.x - p.x, ey = q.y - p.y, ez = q.z - p.z;
-                const double r2 = ex * ex + ey * ey + ez * ez;
-                sum += sm[kk] * kernel_w(sqrt(r2), hi, sigma);
-            }
-            dpairs += cnt;
-            cnt = 0;
-        };
-        while (node >= 0) {
-            walk_tree(
-                wn, node, [&](const WalkNode& nd) { return !(box_dist2(nd, p.x, p.y, p.z) >= rr) & need; },
-                [&](int lo, int count) {
-                    double4 q[kBvhLeaf];
-                    load_leaf(sp, lo, lo + count, q);
-#pragma unroll
-                    for (int j = 0; j < kBvhLeaf; j++) {
-                        if (j >= count) break;
-                        const double ex = q[j].x - p.x, ey = q[j].y - p.y, ez = q[j].z - p.z;
-                        const double r2 = ex * ex + ey * ey + ez * ez;
-                        if (need && r2 < rr) list[cnt++ * kWave + lane] = lo + j;
-                    }
-                    return __ballot(cnt > kHyList - kBvhLeaf) != 0;
-                },
-                leafVisits);
-            flush();
-        }
+                return need && ex * ex + ey * ey + ez * ez < rr;
+            },
+            [&](const int* list, int cnt) {
+                for (int e = 0; e < cnt; e++) {
+                    const int kk = list[e * kWave + lane];
+                    const double4 q = sp[kk];
+                    const double ex = q.x - p.x, ey = q.y - p.y, ez = q.z - p.z;
+                    const double r2 = ex * ex + ey * ey + ez * ez;
+                    rsum += sm[kk] * kernel_w(sqrt(r2), hi, sigma);
+                }
+                dpairs += cnt;
+            },
+            leafVisits, &meta->flags);
         if (need) {  // setH, sph.go:121-131
             passes++;
-            rhoi = sum;
-            const double target = go_max(hMin, 0.5 * hy_cbrt(3 * ngb * mi / (4 * kPi * sum)));
+            rhoi = rsum;
+            const double target = go_max(hMin, 0.5 * hy_cbrt(3 * ngb * mi / (4 * kPi * rsum)));
             const bool last = it == kDensityPasses - 1;
             const bool redo = !last && fabs(target - hi) > 0.05 * hi;
             if (redo) hi = go_max(hMin, go_max(0.5 * hi, go_min(2 * hi, target)));
@@ -558,8 +689,7 @@ __global__ __launch_bounds__(64) void k_hy_density(const WalkNode* __restrict__ 
     }
     const unsigned long long ev = wave_sum_u64((unsigned long long)passes);
     const unsigned long long dp = wave_sum_u64(dpairs);
-    int mp = passes;
-    for (int m = 32; m >= 1; m >>= 1) mp = max(mp, __shfl_xor(mp, m, kWave));
+    const int mp = wave_max_i(passes);
     if (lane == 0) {
         if (ev) atomicAdd(&meta->evals, ev);
         if (dp) atomicAdd(&meta->densityPairs, dp);
@@ -606,7 +736,8 @@ __device__ inline void lower_wake(unsigned long long* w, double dt) {
 
 // Forces, energy equation and velocity divergence of the evaluated bodies
 // (sph.go:196-261), and the wake of neighbours with too long steps.
-__global__ __launch_bounds__(64) void k_hy_force(const WalkNode* __restrict__ wn, const double4* __restrict__ sp,
+__global__ __launch_bounds__(64) void k_hy_force(const QuadNode* __restrict__ quad, int stackCap,
+                                                 const double4* __restrict__ sp,
                                                  const double4* __restrict__ sv, const HyTerms* __restrict__ st,
                                                  const double* __restrict__ sstep, const int32_t* __restrict__ order,
                                                  const uint8_t* __restrict__ active, HydroMeta* meta, double alpha,
@@ -614,7 +745,7 @@ __global__ __launch_bounds__(64) void k_hy_force(const WalkNode* __restrict__ wn
                                                  double* __restrict__ az, double* __restrict__ dudt,
                                                  double* __restrict__ divv, double* __restrict__ dtOut,
                                                  unsigned long long* __restrict__ wake) {
-    __shared__ int list[kHyList * kWave];
+    WalkLds L = walk_lds(stackCap);
     const long long ngas = meta->ngas;
     const long long nw = (ngas + kWave - 1) / kWave;
     const long long w = xcd_wave(nw);
@@ -633,8 +764,6 @@ __global__ __launch_bounds__(64) void k_hy_force(const WalkNode* __restrict__ wn
     double fx = 0, fy = 0, fz = 0, du = 0, dv = 0;
     double vsig = 2 * ci, minStep = INFINITY, maxStep = 0.0;
     unsigned long long leafVisits = 0, pairs = 0, viscPairs = 0;
-    int cnt = 0;
-    int node = 0;
     // scan's test of a body slot (tree.go:295-307)
     auto reaches = [&](const double4& q, double& r2) {
         const double ex = q.x - p.x, ey = q.y - p.y, ez = q.z - p.z;
@@ -645,68 +774,57 @@ __global__ __launch_bounds__(64) void k_hy_force(const WalkNode* __restrict__ wn
         return r2 < lim2;
     };
     // each's node test with symmetric set (tree.go:214-221)
-    auto enters = [&](const WalkNode& nd, bool on) {
+    auto enters = [&](const NodeBox& nd, bool on) {
         double reach = radius;
         if (2 * nd.hmax > reach) reach = 2 * nd.hmax;
         return !(box_dist2(nd, p.x, p.y, p.z) >= reach * reach) & on;
     };
-    auto flush = [&]() {
-        for (int e = 0; e < cnt; e++) {
-            const int kk = list[e * kWave + lane];
-            const double4 q = sp[kk];
-            const double4 sj = sv[kk];
-            const HyTerms tj = st[kk];
-            const double ex0 = q.x - p.x, ey0 = q.y - p.y, ez0 = q.z - p.z;
-            const double r2 = ex0 * ex0 + ey0 * ey0 + ez0 * ez0;
-            if (sstep) {
-                const double sj0 = sstep[kk];
-                if (sj0 > 0) {
-                    minStep = go_min(minStep, sj0);
-                    maxStep = go_max(maxStep, sj0);
+    neighbour_walk(
+        quad, sp, L.llist, L.nlist, L.stk, lane, [&](const NodeBox& nd) { return enters(nd, act); },
+        [&](const double4& q, int kk) {
+            double r2;
+            return act && reaches(q, r2) && kk != k && r2 != 0;
+        },
+        [&](const int* list, int cnt) {
+            for (int e = 0; e < cnt; e++) {
+                const int kk = list[e * kWave + lane];
+                const double4 q = sp[kk];
+                const double4 sj = sv[kk];
+                const HyTerms tj = st[kk];
+                const double ex0 = q.x - p.x, ey0 = q.y - p.y, ez0 = q.z - p.z;
+                const double r2 = ex0 * ex0 + ey0 * ey0 + ez0 * ez0;
+                if (sstep) {
+                    const double sj0 = sstep[kk];
+                    if (sj0 > 0) {
+                        minStep = go_min(minStep, sj0);
+                        maxStep = go_max(maxStep, sj0);
+                    }
                 }
-            }
-            const double r = sqrt(r2);
-            const double ex = (p.x - q.x) / r, ey = (p.y - q.y) / r, ez = (p.z - q.z) / r;
-            const double vx = v.x - sj.x, vy = v.y - sj.y, vz = v.z - sj.z;
-            const double vr = vx * ex + vy * ey + vz * ez;
-            vsig = go_max(vsig, ci + tj.c - 3 * go_min(0.0, vr));
-            const double dwi = kernel_dw(r, hi, ti.sig4), dwj = kernel_dw(r, q.w, tj.sig4);
-            const double dwm = 0.5 * (dwi + dwj);
-            const double fj = tj.f;
-            double visc = 0;
-            if (vr < 0) {
-                viscPairs++;
-                const double hm = 0.5 * (hi + q.w);
-                const double mu = hm * vr * r / (r2 + 0.01 * hm * hm);
-                visc = (-alpha * 0.5 * (ci + tj.c) * mu + 2 * alpha * mu * mu) / (0.5 * (rhoi + tj.rho));
-            }
-            const double f = -sj.w * (fi * dwi + fj * dwj + visc * dwm);
-            fx = fx + f * ex;
-            fy = fy + f * ey;
-            fz = fz + f * ez;
-            du += sj.w * (fi * dwi + 0.5 * visc * dwm) * vr;
-            dv -= sj.w * dwi * vr;
-        }
-        pairs += cnt;
-        cnt = 0;
-    };
-    while (node >= 0) {
-        walk_tree(
-            wn, node, [&](const WalkNode& nd) { return enters(nd, act); },
-            [&](int lo, int count) {
-                double4 q[kBvhLeaf];
-                load_leaf(sp, lo, lo + count, q);
-#pragma unroll
-                for (int j = 0; j < kBvhLeaf; j++) {
-                    if (j >= count) break;
-                    double r2;
-                    if (act && reaches(q[j], r2) && lo + j != k && r2 != 0) list[cnt++ * kWave + lane] = lo + j;
+                const double r = sqrt(r2);
+                const double ex = (p.x - q.x) / r, ey = (p.y - q.y) / r, ez = (p.z - q.z) / r;
+                const double vx = v.x - sj.x, vy = v.y - sj.y, vz = v.z - sj.z;
+                const double vr = vx * ex + vy * ey + vz * ez;
+                vsig = go_max(vsig, ci + tj.c - 3 * go_min(0.0, vr));
+                const double dwi = kernel_dw(r, hi, ti.sig4), dwj = kernel_dw(r, q.w, tj.sig4);
+                const double dwm = 0.5 * (dwi + dwj);
+                const double fj = tj.f;
+                double visc = 0;
+                if (vr < 0) {
+                    viscPairs++;
+                    const double hm = 0.5 * (hi + q.w);
+                    const double mu = hm * vr * r / (r2 + 0.01 * hm * hm);
+                    visc = (-alpha * 0.5 * (ci + tj.c) * mu + 2 * alpha * mu * mu) / (0.5 * (rhoi + tj.rho));
                 }
-                return __ballot(cnt > kHyList - kBvhLeaf) != 0;
-            },
-            leafVisits);
-        flush();
-    }
+                const double f = -sj.w * (fi * dwi + fj * dwj + visc * dwm);
+                fx = fx + f * ex;
+                fy = fy + f * ey;
+                fz = fz + f * ez;
+                du += sj.w * (fi * dwi + 0.5 * visc * dwm) * vr;
+                dv -= sj.w * dwi * vr;
+            }
+            pairs += cnt;
+        },
+        leafVisits, &meta->flags);
     double dt = 4 * minStep;  // LimiterRatio, internal/sim/sim.go:90
     if (vsig > 0) dt = go_min(dt, 0.3 * hi / vsig);
     if (act) {
@@ -717,15 +835,21 @@ __global__ __launch_bounds__(64) void k_hy_force(const WalkNode* __restrict__ wn
         divv[i] = dv / rhoi;
         dtOut[i] = dt;
     }
-    // Neighbours with much longer steps must be woken up (sph.go:250-258).
+    // Neighbours with much longer steps must be woken up (sph.go:250-258);
+    // j = i and r2 = 0 count here, as in the reference's second scan.
     const bool wakeNeeded = act && sstep != nullptr && maxStep > 4 * dt;
     if (__ballot(wakeNeeded) != 0) {
-        node = 0;
+        L.stk.top = 1;
+        if (lane == 0) {
+            L.stk.ref[0] = 0;
+            L.stk.mask[0] = ~0ull;
+        }
         walk_tree(
-            wn, node, [&](const WalkNode& nd) { return enters(nd, wakeNeeded); },
-            [&](int lo, int count) {
+            quad, L.stk, [&](const NodeBox& nd) { return enters(nd, wakeNeeded); },
+            [&](int lo, int count, bool) {
                 double4 q[kBvhLeaf];
-                load_leaf(sp, lo, lo + count, q);
+#pragma unroll
+                for (int j = 0; j < kBvhLeaf; j++) q[j] = sp[lo + min(j, count - 1)];
 #pragma unroll
                 for (int j = 0; j < kBvhLeaf; j++) {
                     if (j >= count) break;
@@ -735,7 +859,7 @@ __global__ __launch_bounds__(64) void k_hy_force(const WalkNode* __restrict__ wn
                 }
                 return false;
             },
-            leafVisits);
+            leafVisits, &meta->flags);
     }
     const unsigned long long pw = wave_sum_u64(pairs), vw = wave_sum_u64(viscPairs);
     if (lane == 0) {
@@ -791,7 +915,7 @@ struct gx_hydro {
     hipStream_t own = nullptr, stream = nullptr;
     int timing = 0;
     Buf meta, sortMeta, flag, tileCnt, gas, keys, keys2, vals, vals2, hist, digitTotal;
-    Buf sp, sp2, sm, order2, nodes, walk, leafRange, sv, terms, sstep;
+    Buf sp, sp2, sm, order2, nodes, quad, leafRange, sv, terms, sstep;
     Buf stage[21];  // host-array entry point
     HydroMeta* hmeta = nullptr;  // pinned mirror
     int depth = 0;               // levels of the last tree, for the next build's first batch
@@ -867,7 +991,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
         (rc = grow(H->digitTotal, sizeof(uint32_t) * 256)) || (rc = grow(H->sp, sizeof(double4) * g1)) ||
         (rc = grow(H->sp2, sizeof(double4) * g1)) || (rc = grow(H->sm, 8 * g1)) ||
         (rc = grow(H->order2, 4 * g1)) || (rc = grow(H->nodes, sizeof(BvhNode) * maxNodes)) ||
-        (rc = grow(H->walk, sizeof(WalkNode) * maxNodes)) ||
+        (rc = grow(H->quad, sizeof(QuadNode) * maxNodes)) ||
         (rc = grow(H->leafRange, sizeof(int2) * g1)) || (rc = grow(H->sv, sizeof(double4) * g1)) ||
         (rc = grow(H->terms, sizeof(HyTerms) * g1)) || (step && (rc = grow(H->sstep, 8 * g1))))
         return rc;
@@ -899,7 +1023,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
     mark(1);
     // tree levels; the first batch covers the last tree's depth + 4
     BvhNode* nodes = (BvhNode*)H->nodes.p;
-    WalkNode* wn = (WalkNode*)H->walk.p;
+    QuadNode* quad = (QuadNode*)H->quad.p;
     int2* leafRange = (int2*)H->leafRange.p;
     hipLaunchKernelGGL(k_hy_root, dim3(1), dim3(1), 0, st, meta, nodes);
     int built = 0;
@@ -908,7 +1032,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
         for (int L = built; L < want; L++) {
             const long long upper = std::min<long long>(L < 40 ? (1ll << L) : ngas, ngas);
             hipLaunchKernelGGL(k_hy_level, dim3((unsigned)std::min<long long>(1024, blocks_of(upper, 256))),
-                               dim3(256), 0, st, L, (const uint64_t*)H->keys.p, nodes, wn, meta, leafRange);
+                               dim3(256), 0, st, L, (const uint64_t*)H->keys.p, nodes, meta, leafRange);
         }
         built = want;
         HYCHK(hipMemcpyAsync(H->hmeta, meta, sizeof(HydroMeta), hipMemcpyDeviceToHost, st));
@@ -936,7 +1060,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
             const long long cntL = H->hmeta->lvlCount[L];
             if (cntL > kSmallLevel) {
                 hipLaunchKernelGGL(k_hy_refit, dim3((unsigned)std::min<long long>(1024, blocks_of(cntL, 256))),
-                                   dim3(256), 0, st, L, (const double4*)sp, order, hNew, nodes, wn,
+                                   dim3(256), 0, st, L, (const double4*)sp, order, hNew, nodes,
                                    (const HydroMeta*)meta);
                 L--;
                 continue;
@@ -944,15 +1068,22 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
             int lo = L;
             while (lo - 1 >= 0 && H->hmeta->lvlCount[lo - 1] <= kSmallLevel) lo--;
             hipLaunchKernelGGL(k_hy_refit_small, dim3(1), dim3(1024), 0, st, L, lo, (const double4*)sp, order, hNew,
-                               nodes, wn, (const HydroMeta*)meta);
+                               nodes, (const HydroMeta*)meta);
             L = lo - 1;
         }
+        hipLaunchKernelGGL(k_hy_quad, dim3(blocks_of(nn, 256)), dim3(256), 0, st, (const BvhNode*)nodes, nn, quad);
     };
+    // The walk stack holds at most three siblings per level of the 4-wide walk
+    // plus the top.
+    const int stackCap = 3 * ((depth + 1) / 2) + 4;
+    const size_t walkLds = walk_lds_bytes(stackCap);
+    if (walkLds > 64 * 1024) return set_error(GX_ERR_CAPACITY, "neighbour tree too deep (%d levels)", depth);
     refit(nullptr);
     mark(2);
     const long long nw = (ngas + kWave - 1) / kWave;
     const unsigned waveGrid = (unsigned)(8 * ((nw + 7) / 8));
-    hipLaunchKernelGGL(k_hy_density, dim3(waveGrid), dim3(kWave), 0, st, (const WalkNode*)wn, (const double4*)sp,
+    hipLaunchKernelGGL(k_hy_density, dim3(waveGrid), dim3(kWave), walkLds, st, (const QuadNode*)quad, stackCap,
+                       (const double4*)sp,
                        (const double*)H->sm.p, order, active, meta, 0.25 * softening, ngb, h, rho);
     mark(3);
     refit(h);  // hmax of the final smoothing lengths (refitH)
@@ -965,7 +1096,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
                        vx, vy, vz, m, (const double*)rho, u, step, gamma, (double4*)H->sv.p, (HyTerms*)H->terms.p,
                        step ? (double*)H->sstep.p : nullptr);
     mark(4);
-    hipLaunchKernelGGL(k_hy_force, dim3(waveGrid), dim3(kWave), 0, st, (const WalkNode*)wn, sp2,
+    hipLaunchKernelGGL(k_hy_force, dim3(waveGrid), dim3(kWave), walkLds, st, (const QuadNode*)quad, stackCap, sp2,
                        (const double4*)H->sv.p, (const HyTerms*)H->terms.p,
                        step ? (const double*)H->sstep.p : nullptr, order2, active, meta, alpha, ax, ay, az, dudt,
                        divv, dtOut, (unsigned long long*)wake);
@@ -973,6 +1104,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
     HYCHK(hipGetLastError());
     HYCHK(hipMemcpyAsync(H->hmeta, meta, sizeof(HydroMeta), hipMemcpyDeviceToHost, st));
     HYCHK(hipStreamSynchronize(st));
+    if (H->hmeta->flags) return set_error(GX_ERR_CAPACITY, "walk stack overflow (flags %d)", H->hmeta->flags);
     H->stats.n_nodes = nn;
     H->stats.n_leaves = H->hmeta->leaves;
     H->stats.depth = depth;
@@ -1013,7 +1145,7 @@ void gx_hydro_free(gx_hydro* H) {
     if (H->stream) (void)hipStreamSynchronize(H->stream);
     Buf* bufs[] = {&H->meta, &H->sortMeta, &H->flag, &H->tileCnt, &H->gas, &H->keys, &H->keys2, &H->vals,
                    &H->vals2, &H->hist, &H->digitTotal, &H->sp, &H->sp2, &H->sm, &H->order2, &H->nodes,
-                   &H->leafRange, &H->sv, &H->terms, &H->sstep, &H->walk};
+                   &H->leafRange, &H->sv, &H->terms, &H->sstep, &H->quad};
     for (Buf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (Buf& b : H->stage)
