@@ -590,7 +590,7 @@ def bench_sph(args):
         "neighbours_per_body": st["pairs"] / n, "density_passes_per_body": st["density_sums"] / n,
         "pairs_per_s": st["pairs"] / (ms * 1e-3),
         "tree": {"nodes": st["n_nodes"], "leaves": st["n_leaves"], "depth": st["depth"],
-                 "max_density_passes": st["max_iterations"], "leaf_visits_per_wave": st["leaf_visits"] / ((n + 63) // 64)},
+                 "max_density_passes": st["max_iterations"], "walk_steps_per_wave": st["walk_steps"] / ((n + 63) // 64)},
         "phases_ms": ph,
         "roofline": {"bound": "valu", "kernel": "k_hy_force", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,

@@ -66,7 +66,7 @@ class GxHydroStats(ctypes.Structure):
     _fields_ = [
         ("n_gas", ctypes.c_int64), ("n_nodes", ctypes.c_int64), ("n_leaves", ctypes.c_int64),
         ("depth", ctypes.c_int32), ("max_iterations", ctypes.c_int32), ("density_sums", ctypes.c_int64),
-        ("pairs", ctypes.c_int64), ("leaf_visits", ctypes.c_int64), ("density_pairs", ctypes.c_int64),
+        ("pairs", ctypes.c_int64), ("walk_steps", ctypes.c_int64), ("density_pairs", ctypes.c_int64),
         ("visc_pairs", ctypes.c_int64),
     ]
 

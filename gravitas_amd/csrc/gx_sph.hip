@@ -80,11 +80,11 @@ struct HydroMeta {
     int lvlCount[kBvhLevels + 1];
     int leaves;
     int flags;
-    unsigned long long evals, pairs, leafVisits;
+    unsigned long long evals, pairs, walkSteps;
     unsigned long long densityPairs;  // neighbour terms summed by the density passes
     unsigned long long viscPairs;     // force pairs that approach (viscosity branch)
     int maxPasses;
-    int pad;
+    int hChanged;  // some density pass moved a smoothing length (refitH is needed)
 };
 
 // Per tree position: the bodies' terms of the force pass.
@@ -205,9 +205,10 @@ __global__ void k_hy_meta_init(HydroMeta* meta) {
         meta->ngas = 0;
         meta->leaves = 0;
         meta->flags = 0;
-        meta->evals = meta->pairs = meta->leafVisits = 0;
+        meta->evals = meta->pairs = meta->walkSteps = 0;
         meta->densityPairs = meta->viscPairs = 0;
         meta->maxPasses = 0;
+        meta->hChanged = 0;
     }
     for (int l = t; l <= kBvhLevels; l += blockDim.x) {
         meta->lvlStart[l] = 0;
@@ -403,6 +404,7 @@ __device__ inline void refit_node(BvhNode* __restrict__ nodes, int ni,
 __global__ __launch_bounds__(256) void k_hy_refit(int L, const double4* __restrict__ sp,
                                                   const int32_t* __restrict__ order, const double* __restrict__ h,
                                                   BvhNode* __restrict__ nodes, const HydroMeta* meta) {
+    if (h && !meta->hChanged) return;  // refitH only when a smoothing length moved (sph.go:178-180)
     const int start = meta->lvlStart[L], cnt = meta->lvlCount[L];
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < cnt; idx += gridDim.x * blockDim.x)
         refit_node(nodes, start + idx, sp, order, h);
@@ -415,6 +417,7 @@ __global__ __launch_bounds__(1024) void k_hy_refit_small(int Lhi, int Llo, const
                                                          const int32_t* __restrict__ order,
                                                          const double* __restrict__ h, BvhNode* __restrict__ nodes,
                                                          const HydroMeta* meta) {
+    if (h && !meta->hChanged) return;
     for (int L = Lhi; L >= Llo; L--) {
         const int start = meta->lvlStart[L], cnt = meta->lvlCount[L];
         for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) refit_node(nodes, start + idx, sp, order, h);
@@ -425,9 +428,9 @@ __global__ __launch_bounds__(1024) void k_hy_refit_small(int Lhi, int Llo, const
 // The walk record of every inner node (and of the root when it is a leaf):
 // its grandchildren in walk order, with their boxes and hmax.
 __global__ __launch_bounds__(256) void k_hy_quad(const BvhNode* __restrict__ nodes, long long nnodes,
-                                                 QuadNode* __restrict__ quad) {
+                                                 QuadNode* __restrict__ quad, const int* __restrict__ onlyIf) {
     const long long ni = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (ni >= nnodes) return;
+    if (ni >= nnodes || (onlyIf && !*onlyIf)) return;
     const BvhNode& nd = nodes[ni];
     int f[4], n = 0;
     if (nd.first < 0) {
@@ -648,7 +651,7 @@ __global__ __launch_bounds__(64) void k_hy_density(const QuadNode* __restrict__ 
     double hi = p.w, rhoi = 0;
     bool need = act;
     int passes = 0;
-    unsigned long long leafVisits = 0, dpairs = 0;
+    unsigned long long walkSteps = 0, dpairs = 0;
     for (int it = 0; it < kDensityPasses; it++) {
         if (__ballot(need) == 0) break;
         const double radius = 2 * hi;
@@ -672,7 +675,7 @@ __global__ __launch_bounds__(64) void k_hy_density(const QuadNode* __restrict__ 
                 }
                 dpairs += cnt;
             },
-            leafVisits, &meta->flags);
+            walkSteps, &meta->flags);
         if (need) {  // setH, sph.go:121-131
             passes++;
             rhoi = rsum;
@@ -687,6 +690,7 @@ __global__ __launch_bounds__(64) void k_hy_density(const QuadNode* __restrict__ 
         rho[i] = rhoi;
         h[i] = hi;
     }
+    if (__ballot(act && hi != p.w) != 0 && lane == 0) atomicOr(&meta->hChanged, 1);
     const unsigned long long ev = wave_sum_u64((unsigned long long)passes);
     const unsigned long long dp = wave_sum_u64(dpairs);
     const int mp = wave_max_i(passes);
@@ -763,7 +767,7 @@ __global__ __launch_bounds__(64) void k_hy_force(const QuadNode* __restrict__ qu
     const double r2max = radius * radius;
     double fx = 0, fy = 0, fz = 0, du = 0, dv = 0;
     double vsig = 2 * ci, minStep = INFINITY, maxStep = 0.0;
-    unsigned long long leafVisits = 0, pairs = 0, viscPairs = 0;
+    unsigned long long walkSteps = 0, pairs = 0, viscPairs = 0;
     // scan's test of a body slot (tree.go:295-307)
     auto reaches = [&](const double4& q, double& r2) {
         const double ex = q.x - p.x, ey = q.y - p.y, ez = q.z - p.z;
@@ -824,7 +828,7 @@ __global__ __launch_bounds__(64) void k_hy_force(const QuadNode* __restrict__ qu
             }
             pairs += cnt;
         },
-        leafVisits, &meta->flags);
+        walkSteps, &meta->flags);
     double dt = 4 * minStep;  // LimiterRatio, internal/sim/sim.go:90
     if (vsig > 0) dt = go_min(dt, 0.3 * hi / vsig);
     if (act) {
@@ -859,13 +863,13 @@ __global__ __launch_bounds__(64) void k_hy_force(const QuadNode* __restrict__ qu
                 }
                 return false;
             },
-            leafVisits, &meta->flags);
+            walkSteps, &meta->flags);
     }
     const unsigned long long pw = wave_sum_u64(pairs), vw = wave_sum_u64(viscPairs);
     if (lane == 0) {
         if (pw) atomicAdd(&meta->pairs, pw);
         if (vw) atomicAdd(&meta->viscPairs, vw);
-        atomicAdd(&meta->leafVisits, leafVisits);
+        atomicAdd(&meta->walkSteps, walkSteps);
     }
 }
 
@@ -1053,7 +1057,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
     H->nnodes = nn;
     // Bottom-up refit: runs of levels with at most kSmallLevel nodes go to one
     // workgroup each, wider levels get a grid launch of their own.
-    constexpr int kSmallLevel = 2048;
+    constexpr int kSmallLevel = 512;
     auto refit = [&](const double* hNew) {
         int L = depth - 1;
         while (L >= 0) {
@@ -1071,7 +1075,8 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
                                nodes, (const HydroMeta*)meta);
             L = lo - 1;
         }
-        hipLaunchKernelGGL(k_hy_quad, dim3(blocks_of(nn, 256)), dim3(256), 0, st, (const BvhNode*)nodes, nn, quad);
+        hipLaunchKernelGGL(k_hy_quad, dim3(blocks_of(nn, 256)), dim3(256), 0, st, (const BvhNode*)nodes, nn, quad,
+                           hNew ? (const int*)&meta->hChanged : nullptr);
     };
     // The walk stack holds at most three siblings per level of the 4-wide walk
     // plus the top.
@@ -1086,7 +1091,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
                        (const double4*)sp,
                        (const double*)H->sm.p, order, active, meta, 0.25 * softening, ngb, h, rho);
     mark(3);
-    refit(h);  // hmax of the final smoothing lengths (refitH)
+    refit(h);  // hmax of the final smoothing lengths (refitH; the kernels return at once if no h moved)
     hipLaunchKernelGGL(k_hy_sort_leaves, dim3(blocks_of(ngas, 256)), dim3(256), 0, st, (const HydroMeta*)meta,
                        (const int2*)leafRange, order, (const double4*)sp, (const double*)h, (int32_t*)H->order2.p,
                        (double4*)H->sp2.p);
@@ -1111,7 +1116,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
     H->stats.max_iterations = H->hmeta->maxPasses;
     H->stats.density_sums = (int64_t)H->hmeta->evals;
     H->stats.pairs = (int64_t)H->hmeta->pairs;
-    H->stats.leaf_visits = (int64_t)H->hmeta->leafVisits;
+    H->stats.walk_steps = (int64_t)H->hmeta->walkSteps;
     H->stats.density_pairs = (int64_t)H->hmeta->densityPairs;
     H->stats.visc_pairs = (int64_t)H->hmeta->viscPairs;
     if (H->timing) {

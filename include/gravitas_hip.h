@@ -316,7 +316,7 @@ typedef struct {
     int32_t depth, max_iterations;  /* tree levels; most density passes of one body */
     int64_t density_sums;           /* density evaluations over all bodies and passes */
     int64_t pairs;                  /* force-pass neighbour pairs (j != i) */
-    int64_t leaf_visits;            /* leaves visited by the force pass's wavefronts */
+    int64_t walk_steps;             /* walk records the force pass's wavefronts loaded */
     int64_t density_pairs;          /* neighbour terms summed by the density passes (self included) */
     int64_t visc_pairs;             /* force pairs that approach (the viscosity branch) */
 } gx_hydro_stats;
