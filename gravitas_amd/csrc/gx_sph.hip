@@ -49,7 +49,7 @@ struct BvhNode {
     int lo, hi;  // range of tree positions
     int first;   // right child, walked first; -1 for a leaf
     int second;  // left child; -1 for a leaf
-    int escape;  // next node after this subtree in the walk; -1 ends it
+    int escape;  // next node after this subtree in a right-first walk; -1 ends it (exported)
     int parent;  // -1 for the root
 };
 static_assert(sizeof(BvhNode) == 80, "node record");

@@ -81,7 +81,7 @@ class Hydro:
     _EXPORT = {"keys": (0, np.uint64), "order": (1, np.int32), "nodes": (2, None)}
     NODE_DTYPE = np.dtype([("minx", "f8"), ("miny", "f8"), ("minz", "f8"), ("maxx", "f8"), ("maxy", "f8"),
                            ("maxz", "f8"), ("hmax", "f8"), ("lo", "i4"), ("hi", "i4"), ("first", "i4"),
-                           ("second", "i4"), ("escape", "i4"), ("pad", "i4")])
+                           ("second", "i4"), ("escape", "i4"), ("parent", "i4")])
 
     def __init__(self, timing: bool = False):
         self._L = _lib.load()

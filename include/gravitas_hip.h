@@ -333,8 +333,9 @@ int gx_hydro_phase_ms(const gx_hydro *h, float *ms, int n);
  * last evaluation's tree to host memory.  what: 0 keys (uint64, sorted), 1
  * order (int32, tree position -> state index, leaves sorted by decreasing
  * h), 2 nodes (80-byte records: double minx, miny, minz, maxx, maxy, maxz,
- * hmax; int32 lo, hi, first (right child), second (left child), escape,
- * pad).  Returns the number of items copied. */
+ * hmax; int32 lo, hi, first (right child), second (left child), escape (the
+ * next node after the subtree in a right-first walk, -1 at the end), parent
+ * (-1 at the root)).  Returns the number of items copied. */
 int64_t gx_hydro_export(gx_hydro *h, int what, void *host, int64_t count);
 
 #ifdef __cplusplus
