@@ -141,3 +141,23 @@ def test_host_entry_point_equals_device(gpu):
     for k in OUT:
         assert np.array_equal(ho[k], go[k]), k
     assert np.array_equal(hs["h"], gs["h"]) and np.array_equal(hs["rho"], gs["rho"])
+
+
+def test_degenerate_inputs(gpu):
+    """All gas at one point (equal keys, r2 = 0 everywhere), exactly two gas
+    bodies, dead gas, and an empty active set."""
+    s = ics.gas_cloud(40, 11)
+    for k in ("px", "py", "pz"):
+        s[k][:] = 0.5
+    say("[hydro] 40 bodies at one point")
+    assert_same(*run_both(s, softening=0.01))
+    s = ics.gas_cloud(50, 12)
+    s["kind"][2:] = ics.KIND_STAR
+    say("[hydro] two gas bodies")
+    assert_same(*run_both(s, softening=0.01))
+    s = ics.gas_cloud(3000, 13)
+    s["alive"][::5] = 0
+    say("[hydro] dead gas and an empty active set")
+    r = run_both(s, softening=0.01, active=np.zeros(3000, np.uint8))
+    assert_same(*r)
+    assert (r[1]["ax"] == 7.0).all() and r[5].stats()["pairs"] == 0

@@ -307,3 +307,13 @@ def test_fewer_than_two_gas_bodies():
     oracle.sph_forces(s, out=out)
     assert out["ax"][0] == 0 and out["dt"][0] == math.inf and (out["ax"][1:] == 3.0).all()
     assert s["h"][0] == 0  # no smoothing length guess below two gas bodies
+
+
+def test_all_gas_at_one_point():
+    s = ics.gas_cloud(30, 14)
+    for k in ("px", "py", "pz"):
+        s[k][:] = -1.25
+    out, _ = oracle.sph_forces(s, softening=0.01)
+    assert np.isfinite(s["rho"]).all() and (s["rho"] > 0).all()
+    for k in ("ax", "ay", "az", "dudt", "divv"):
+        assert (out[k] == 0).all(), k  # every pair has r = 0: skipped
