@@ -963,6 +963,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
     const long long ngas = H->hmeta->ngas;
     H->ngas = ngas;
     memset(&H->stats, 0, sizeof H->stats);
+    memset(H->phase, 0, sizeof H->phase);  // an early return leaves no stale phase times
     H->stats.n_gas = ngas;
     const int32_t* gas = (const int32_t*)H->gas.p;
     if (ngas < 2) {
