@@ -272,7 +272,8 @@ int64_t gx_export(gx_solver *s, int what, void *host, int64_t count);
 
 /* sph.Solver (internal/sph/sph.go:11-19): owns the neighbour tree and the
  * scratch of the hydro on the GPU, and its own stream.  NULL without a
- * device (gx_error tells why). */
+ * device (gx_error tells why).  One gx_hydro is used by one thread at a
+ * time, as one sph.Solver is; a hydro and a solver may run side by side. */
 typedef struct gx_hydro gx_hydro;
 gx_hydro *gx_hydro_new(void);
 void gx_hydro_free(gx_hydro *h);
