@@ -22,6 +22,9 @@ Files:
                    (split.go:164-199).
   encounter_768.npz two-disk encounter on 4 rungs: one global block step
                    (blockstep.go:130-198): rung populations, events, state.
+  gas_768.npz      SPH hydro (sph.go:83-263): a gas cloud with stars and dead
+                   bodies, h guessed from the box, a step array; h, rho,
+                   forces, du/dt, div v, dt, wake and the density-sum count.
   edges.npz        two stars and a hole (bh_test.go:137-147), 100 coincident
                    stars and one more (bh_test.go:149-161), a leaf-size
                    boundary of 24 and 25 stars (build.go:134-160), one star.
@@ -146,6 +149,23 @@ def edges():
     return f
 
 
+def gas_768():
+    """A gas cloud with stars and dead bodies mixed in, h from the box guess
+    (several density passes), a step array for the limiter: one
+    sph.Solver.Forces evaluation (sph.go:83-263)."""
+    s = ics.gas_cloud(768, 106)
+    s["kind"][::13] = ics.KIND_STAR
+    s["alive"][[7, 300]] = 0
+    rng = np.random.default_rng(107)
+    step = np.where(rng.random(768) < 0.1, 5.0, 1e-3)
+    so = {k: v.copy() for k, v in s.items()}
+    out, evals = oracle.sph_forces(so, softening=0.02, step=step)
+    f = {"eps": 0.02, "step": step, "evals": evals, "h_out": so["h"], "rho_out": so["rho"]}
+    for k in ("ax", "ay", "az", "dudt", "divv", "dt", "wake"):
+        f[k] = out[k]
+    return {k: s[k] for k in ("px", "py", "pz", "vx", "vy", "vz", "m", "kind", "alive", "u", "h", "rho")}, f
+
+
 def write(name, arrays):
     path = os.path.join(HERE, name)
     np.savez_compressed(path, **{k: np.asarray(v) for k, v in arrays.items()})
@@ -154,7 +174,7 @@ def write(name, arrays):
 
 def main():
     for name, make in (("plummer_512.npz", plummer_512), ("mixed_768.npz", mixed_768),
-                       ("encounter_768.npz", encounter_768)):
+                       ("encounter_768.npz", encounter_768), ("gas_768.npz", gas_768)):
         s, f = make()
         write(name, {**{"in_" + k: v for k, v in s.items()}, **f})
     write("edges.npz", edges())

@@ -120,3 +120,18 @@ def test_edge_fixtures(gpu, case):
     got = gpu_acc(s, 0.0, eps, walk="fp32")[1]
     assert np.isfinite(got).all()
     assert np.abs(got - want).max() <= 1e-5 * max(1.0, np.abs(want).max())
+
+
+def test_hydro_fixture(gpu):
+    from gravitas_amd import GasState, Hydro
+    f = load("gas_768.npz")
+    s = state(f)
+    say("[golden] gas_768 hydro")
+    g = GasState.from_host(s)
+    hy = Hydro()
+    out = hy.forces(g, softening=float(f["eps"]), step=f["step"]).host()
+    h = g.host()
+    assert np.array_equal(h["h"], f["h_out"]) and np.array_equal(h["rho"], f["rho_out"])
+    for k in ("ax", "ay", "az", "dudt", "divv", "dt", "wake"):
+        assert np.array_equal(out[k], f[k]), k
+    assert hy.stats()["density_sums"] == int(f["evals"])
