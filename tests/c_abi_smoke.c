@@ -11,7 +11,9 @@
  *   - a KDK step on device arrays gives the same positions as one computed
  *     here from the step's own formulas (sim.go:302-343), using the forces
  *     returned by the host entry point;
- *   - argument errors come back as status codes, not crashes.
+ *   - argument errors come back as status codes, not crashes;
+ *   - the SPH hydro from C: a 3x3x3 gas lattice, no net force on the centre
+ *     body, densities and smoothing lengths positive (sph.go:83-263).
  * Prints "ok ..." and exits 0 on success.
  */
 #include <math.h>
@@ -100,6 +102,40 @@ int main(void) {
     gx_stats st;
     if (gx_get_stats(s, &st) != GX_OK || st.n_stars != 2) return fail("stats");
     gx_solver_free(s);
+
+    /* the hydro (sph.Solver.Forces) on host arrays: a 3x3x3 gas lattice of
+     * unit masses, spacing 0.1, u = 1, h guessed; by symmetry the centre body
+     * feels no net force (to round-off), and every body gets rho > 0, h > 0 */
+    enum { NL = 27 };
+    double lx[NL], ly[NL], lz[NL], lv[NL] = {0}, lm[NL], lu[NL], lh[NL] = {0}, lr[NL] = {0};
+    double hax[NL], hay[NL], haz[NL], hdu[NL], hdv[NL], hdt[NL];
+    uint64_t wake[NL];
+    uint8_t lk[NL], la[NL];
+    const double inf = 1.0 / 0.0;
+    for (int i = 0; i < NL; i++) {
+        lx[i] = 0.1 * (i % 3);
+        ly[i] = 0.1 * (i / 3 % 3);
+        lz[i] = 0.1 * (i / 9);
+        lm[i] = 1;
+        lu[i] = 1;
+        lk[i] = GX_KIND_GAS;
+        la[i] = 1;
+        memcpy(&wake[i], &inf, sizeof inf);
+    }
+    gx_hydro *hy = gx_hydro_new();
+    if (!hy) return fail("gx_hydro_new");
+    gx_gas_params gas = {0, 0, 0};
+    if (gx_hydro_forces(hy, lx, ly, lz, lv, lv, lv, lm, lk, la, lu, lh, lr, NL, &gas, 0.0, NULL, NULL, hax, hay,
+                        haz, hdu, hdv, hdt, wake) != GX_OK)
+        return fail("gx_hydro_forces");
+    for (int i = 0; i < NL; i++)
+        if (!(lr[i] > 0) || !(lh[i] > 0) || !(hdt[i] > 0)) return fail("hydro density, h or dt");
+    const double c = fabs(hax[13]) + fabs(hay[13]) + fabs(haz[13]), edge = fabs(hax[12]);
+    if (!(c <= 1e-9 * edge) || !(edge > 0)) return fail("hydro lattice symmetry");
+    if (gx_hydro_forces(hy, NULL, ly, lz, lv, lv, lv, lm, lk, la, lu, lh, lr, NL, &gas, 0.0, NULL, NULL, hax, hay,
+                        haz, hdu, hdv, hdt, wake) != GX_ERR_ARG)
+        return fail("hydro null array must be GX_ERR_ARG");
+    gx_hydro_free(hy);
     printf("ok: %s; two stars (%g, %g), KDK x = (%.15g, %.15g)\n", name, ax[0], ax[1], gx[0], gx[1]);
     return 0;
 }
