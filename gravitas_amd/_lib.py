@@ -32,6 +32,7 @@ EXPORTS = (
     "gx_accelerations_active_dev", "gx_block_step_dev", "gx_diagnostics_dev", "gx_split_step_dev",
     "gx_hydro_new", "gx_hydro_free", "gx_hydro_config", "gx_hydro_set_stream", "gx_hydro_forces_dev",
     "gx_hydro_forces", "gx_hydro_get_stats", "gx_hydro_phase_ms", "gx_hydro_export",
+    "gx_split_step_spin_dev", "gx_drag_outer_dev", "gx_drag_fast_dev", "gx_drag_get_stats",
 )
 
 GX_BLOCK_MAX_LEVELS = 8
@@ -48,6 +49,11 @@ class GxStats(ctypes.Structure):
         ("rounds", ctypes.c_int64), ("mixed_nodes", ctypes.c_int64),
         ("max_list", ctypes.c_int32 * 4), ("n_targets", ctypes.c_int64),
     ]
+
+
+class GxDragStats(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int64) for f in ("passes", "iterations", "max_iterations", "sequential_passes",
+                                              "segments", "slow_segments")]
 
 
 class GxBlockStats(ctypes.Structure):
@@ -126,6 +132,11 @@ def load():
                                     ctypes.POINTER(GxBlockStats)]
     L.gx_split_step_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, I64, D, I, D, D, I, P, P, P,
                                     ctypes.POINTER(ctypes.c_int32)]
+    L.gx_split_step_spin_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, P, P, P, I64, D, I, D, D, I, P, P, P,
+                                         ctypes.POINTER(ctypes.c_int32)]
+    L.gx_drag_outer_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, P, P, P, P, I64, D, D, D]
+    L.gx_drag_fast_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, P, I64, P, P, P, I64, D, D, D]
+    L.gx_drag_get_stats.argtypes = [P, ctypes.POINTER(GxDragStats)]
     L.gx_diagnostics_dev.argtypes = [P, P, P, P, P, P, P, P, P, I64, I, ctypes.POINTER(ctypes.c_double)]
     L.gx_export.argtypes = [P, I, P, I64]
     L.gx_export.restype = I64

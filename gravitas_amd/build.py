@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libgravitas_hip.so")
-SOURCES = ["gx_api.hip", "gx_build.hip", "gx_walk64.hip", "gx_walk32.hip", "gx_split.hip", "gx_sph.hip"]
+SOURCES = ["gx_api.hip", "gx_build.hip", "gx_walk64.hip", "gx_walk32.hip", "gx_split.hip", "gx_drag.hip", "gx_sph.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 FLAGS = [

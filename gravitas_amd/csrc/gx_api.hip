@@ -97,6 +97,11 @@ struct gx_solver {
     DevBuf actS, blockAct, tOff, tlist, activeMask, rung, rungHist, diag;
     // split steps near black holes
     DevBuf nearFlag, nearList, tileCnt, splitScalars, farF, fastF;
+    // frame dragging: list, chains (increments, two guesses), segment summaries, scalars
+    DevBuf dragFlag, dragList, dragD, dragS0, dragS1, dragSig, dragGuess, dragSeg, dragSegStart, dragSegMode,
+        dragScal;
+    int dragIters = 16;  // GX_DRAG_ITERATIONS: fixed-point iterations before the sequential pass (0: always)
+    gx_drag_stats dragStats = {};
     unsigned long long* hhist = nullptr;  // pinned mirror of hist
     int cf = 2048, cp = 512, cl = 256;  // FP32-walk list capacities per group (grow on demand)
     long long capNodes = 0;
@@ -204,6 +209,8 @@ gx_solver* gx_solver_new(void) {
     for (auto& e : s->ev) (void)hipEventCreate(&e);
     const char* sw = getenv("GX_SPLIT_WALK");
     s->splitWalk = sw && sw[0] == '1';
+    const char* di = getenv("GX_DRAG_ITERATIONS");
+    if (di && di[0]) s->dragIters = std::max(0, atoi(di));
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
@@ -224,7 +231,9 @@ void gx_solver_free(gx_solver* s) {
                       &s->boxEnc,   &s->starCount, &s->starOff, &s->groupOff, &s->root,      &s->half,
                       &s->actS,     &s->blockAct, &s->tOff,    &s->tlist,     &s->activeMask, &s->rung,
                       &s->rungHist, &s->diag,     &s->nearFlag, &s->nearList, &s->tileCnt,  &s->splitScalars,
-                      &s->farF,     &s->fastF};
+                      &s->farF,     &s->fastF,    &s->dragFlag, &s->dragList, &s->dragD,    &s->dragS0,
+                      &s->dragS1,   &s->dragSig,  &s->dragGuess, &s->dragSeg, &s->dragSegStart, &s->dragSegMode,
+                      &s->dragScal};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto& e : s->ev)
@@ -727,12 +736,177 @@ int block_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, dou
     return GX_OK;
 }
 
+// Frame dragging (internal/sim/framedrag.go).  Scalars in dragScal: the
+// frame-dragging flag (int in 8 bytes), the changed flag and the
+// element-by-element segment counter (8 bytes each, cleared together), one
+// spare slot, then the chain starts and ends startJ | endJ | startV | endV
+// (C = 3 nh doubles each).
+struct DragBufs {
+    double *startJ, *endJ, *startV, *endV;
+    int* on;
+    unsigned long long *changed, *slow;
+};
+
+int drag_bufs(gx_solver* s, long long nh, DragBufs& b) {
+    const long long C = 3 * std::max(nh, 1LL);
+    int rc = ensure(s->dragScal, (size_t)C * 32 + 32);
+    if (rc) return rc;
+    unsigned long long* u = (unsigned long long*)s->dragScal.p;
+    b.on = (int*)u;
+    b.changed = u + 1;
+    b.slow = u + 2;
+    double* d = (double*)(u + 4);
+    b.startJ = d;
+    b.endJ = d + C;
+    b.startV = d + 2 * C;
+    b.endV = d + 3 * C;
+    return GX_OK;
+}
+
+// Exact ordered scan of the C chains of D (chain-major, ne per chain) from
+// start (gx_drag.hip): end receives the chain ends; with S given also the
+// value every position sees, and *changed is set if any differs from Sold
+// (from the start values if Sold is null).
+int drag_scan(gx_solver* s, const double* D, long long ne, int C, const double* start, const double* Sold, double* S,
+              double* end, const DragBufs& b) {
+    const int nseg = (int)((ne + kDragSeg - 1) / kDragSeg);
+    hipStream_t st = s->stream;
+    const unsigned wgrid = blocks((long long)C * nseg * 64, 256);
+    hipLaunchKernelGGL(k_scan_segsum, dim3(wgrid), dim3(256), 0, st, D, ne, C, nseg, (double*)s->dragSig.p);
+    hipLaunchKernelGGL(k_scan_guess, dim3((unsigned)C), dim3(1024), 0, st, (const double*)s->dragSig.p, nseg, start,
+                       (double*)s->dragGuess.p);
+    hipLaunchKernelGGL(k_scan_summary, dim3(wgrid), dim3(256), 0, st, D, ne, C, nseg, (const double*)s->dragGuess.p,
+                       (DragSeg*)s->dragSeg.p);
+    hipLaunchKernelGGL(k_scan_chain, dim3((unsigned)C), dim3(256), 0, st, D, ne, nseg, (const DragSeg*)s->dragSeg.p,
+                       start, S, (double*)s->dragSegStart.p, (int*)s->dragSegMode.p, end, b.slow);
+    if (S)
+        hipLaunchKernelGGL(k_scan_fill, dim3(wgrid), dim3(256), 0, st, D, ne, C, nseg, (const DragSeg*)s->dragSeg.p,
+                           (const double*)s->dragSegStart.p, (const int*)s->dragSegMode.p, start, Sold, S,
+                           (int*)b.changed);
+    HIPCHK(hipGetLastError());
+    return GX_OK;
+}
+
+// One frame-dragging pass over an ordered list of bodies (dragOuter, or the
+// near part of dragFast): the fixed point described in gx_drag.hip, or one
+// sequential thread when it has not settled within dragIters iterations (or
+// the chains would not fit).  *b.on must hold spinning() for the pass.
+int drag_pass(gx_solver* s, const DragArgs& a, const int32_t* list, long long ne, double* vx, double* vy, double* vz,
+              double* jx, double* jy, double* jz, const DragBufs& b) {
+    if (ne <= 0 || a.nh <= 0) return GX_OK;
+    hipStream_t st = s->stream;
+    s->dragStats.passes++;
+    const long long C = 3LL * a.nh;
+    const long long nseg = (ne + kDragSeg - 1) / kDragSeg;
+    const size_t chain = (size_t)C * (size_t)ne * 8;
+    int rc;
+    if (s->dragIters > 0 && chain <= ((size_t)8 << 30) && C * nseg < (1LL << 30)) {
+        if ((rc = ensure(s->dragD, chain)) || (rc = ensure(s->dragS0, chain)) || (rc = ensure(s->dragS1, chain)) ||
+            (rc = ensure(s->dragSig, (size_t)C * nseg * 8)) || (rc = ensure(s->dragGuess, (size_t)C * nseg * 8)) ||
+            (rc = ensure(s->dragSeg, (size_t)C * nseg * sizeof(DragSeg))) ||
+            (rc = ensure(s->dragSegStart, (size_t)C * nseg * 8)) || (rc = ensure(s->dragSegMode, (size_t)C * nseg * 4)))
+            return rc;
+        const unsigned grid = (unsigned)std::min<long long>(blocks(ne, 256), 4096);
+        double* D = (double*)s->dragD.p;
+        hipLaunchKernelGGL(k_drag_chain_io, dim3(1), dim3(64), 0, st, a.holes, a.nh, jx, jy, jz, b.startJ, 0);
+        const double* Sold = nullptr;  // the guess: null is the start spins
+        double* Snew = (double*)s->dragS0.p;
+        bool settled = false;
+        int it = 0;
+        while (it < s->dragIters && !settled) {
+            it++;
+            hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, b.changed, 2, 0ULL);  // changed, slow
+            hipLaunchKernelGGL(k_drag_eval, dim3(grid), dim3(256), 0, st, a, list, ne, vx, vy, vz, (const double*)jx,
+                               (const double*)jy, (const double*)jz, Sold, 0, (const int*)b.on, D);
+            if ((rc = drag_scan(s, D, ne, (int)C, b.startJ, Sold, Snew, b.endJ, b))) return rc;
+            HIPCHK(hipMemcpyAsync(&s->hhist[8], b.changed, 16, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            settled = s->hhist[8] == 0;
+            s->dragStats.segments += C * nseg;
+            s->dragStats.slow_segments += (int64_t)s->hhist[9];
+            if (!settled) {
+                Sold = Snew;
+                Snew = Snew == (double*)s->dragS0.p ? (double*)s->dragS1.p : (double*)s->dragS0.p;
+            }
+        }
+        s->dragStats.iterations += it;
+        s->dragStats.max_iterations = std::max<int64_t>(s->dragStats.max_iterations, it);
+        if (settled) {
+            // Sold is the reference's chain: the bodies' velocities and the
+            // holes' velocity increments from it, then the holes' velocities
+            // and spins (the last scan's chain ends).
+            hipLaunchKernelGGL(k_drag_eval, dim3(grid), dim3(256), 0, st, a, list, ne, vx, vy, vz, (const double*)jx,
+                               (const double*)jy, (const double*)jz, Sold, 1, (const int*)b.on, D);
+            hipLaunchKernelGGL(k_drag_chain_io, dim3(1), dim3(64), 0, st, a.holes, a.nh, vx, vy, vz, b.startV, 0);
+            if ((rc = drag_scan(s, D, ne, (int)C, b.startV, nullptr, nullptr, b.endV, b))) return rc;
+            hipLaunchKernelGGL(k_drag_chain_io, dim3(1), dim3(64), 0, st, a.holes, a.nh, vx, vy, vz, b.endV, 1);
+            hipLaunchKernelGGL(k_drag_chain_io, dim3(1), dim3(64), 0, st, a.holes, a.nh, jx, jy, jz, b.endJ, 1);
+            HIPCHK(hipGetLastError());
+            return GX_OK;
+        }
+    }
+    s->dragStats.sequential_passes++;
+    hipLaunchKernelGGL(k_drag_seq, dim3(1), dim3(64), 0, st, a, list, ne, vx, vy, vz, jx, jy, jz, (const int*)b.on);
+    HIPCHK(hipGetLastError());
+    return GX_OK;
+}
+
+// dragFast (framedrag.go:129-143): the near list as a pass, then the holes one
+// by one (they read each other's velocities and spins).
+int drag_fast(gx_solver* s, const DragArgs& a, const int32_t* nearList, long long nn, double* vx, double* vy,
+              double* vz, double* jx, double* jy, double* jz, const DragBufs& b, const Meta* meta) {
+    hipLaunchKernelGGL(k_drag_spinning, dim3(1), dim3(64), 0, s->stream, a.holes, meta, (const double*)jx,
+                       (const double*)jy, (const double*)jz, a.c, b.on);
+    int rc = drag_pass(s, a, nearList, nn, vx, vy, vz, jx, jy, jz, b);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_drag_seq, dim3(1), dim3(64), 0, s->stream, a, a.holes, (long long)a.nh, vx, vy, vz, jx, jy,
+                       jz, (const int*)b.on);
+    HIPCHK(hipGetLastError());
+    return GX_OK;
+}
+
+// The bodies of dragOuter (alive, not holes, not near; state order) into
+// dragList; returns their count (synchronises).
+int drag_outer_list(gx_solver* s, long long n, const uint8_t* kind, const uint8_t* alive, const uint8_t* nearFlag,
+                    long long* count) {
+    const long long nb = std::max(1LL, (n + kCollectTile - 1) / kCollectTile);
+    int rc;
+    if ((rc = ensure(s->dragFlag, (size_t)std::max(n, 1LL))) || (rc = ensure(s->dragList, (size_t)std::max(n, 1LL) * 4)) ||
+        (rc = ensure(s->tileCnt, (size_t)nb * 4)) || (rc = ensure(s->splitScalars, 32)))
+        return rc;
+    hipStream_t st = s->stream;
+    const unsigned grid = (unsigned)std::min<long long>(blocks(n, 256), 4096);
+    long long* cnt = (long long*)s->splitScalars.p + 2;
+    hipLaunchKernelGGL(k_drag_outer_flag, dim3(grid), dim3(256), 0, st, n, kind, alive, nearFlag,
+                       (uint8_t*)s->dragFlag.p);
+    hipLaunchKernelGGL(k_flag_count, dim3((unsigned)nb), dim3(256), 0, st, n, (const uint8_t*)s->dragFlag.p,
+                       (int*)s->tileCnt.p);
+    hipLaunchKernelGGL(k_flag_scan, dim3(1), dim3(1024), 0, st, (int)nb, (int*)s->tileCnt.p, cnt);
+    hipLaunchKernelGGL(k_flag_scatter, dim3((unsigned)nb), dim3(256), 0, st, n, (const uint8_t*)s->dragFlag.p,
+                       (const int*)s->tileCnt.p, (int32_t*)s->dragList.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&s->hhist[10], cnt, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *count = (long long)s->hhist[10];
+    return GX_OK;
+}
+
+// dragOuter (framedrag.go:115-127) over the list built by drag_outer_list.
+int drag_outer(gx_solver* s, const DragArgs& a, long long count, double* vx, double* vy, double* vz, double* jx,
+               double* jy, double* jz, const DragBufs& b, const Meta* meta) {
+    hipLaunchKernelGGL(k_drag_spinning, dim3(1), dim3(64), 0, s->stream, a.holes, meta, (const double*)jx,
+                       (const double*)jy, (const double*)jz, a.c, b.on);
+    return drag_pass(s, a, (const int32_t*)s->dragList.p, count, vx, vy, vz, jx, jy, jz, b);
+}
+
 // One step with black holes, stepSplit (internal/sim/split.go:164-199) for a
-// state without gas, with non-spinning holes and instant gravity; without
-// alive holes it is step()'s plain KDK (sim.go:266-277).  With c > 0 the pairs
-// with holes use Paczynski-Wiita radii and drifts use the proper velocity.
-int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, double dt, double c, double eps,
-               int accValid, int32_t* substepsOut) {
+// state without gas and with instant gravity; without alive holes it is
+// step()'s plain KDK (sim.go:266-277).  With c > 0 the pairs with holes use
+// Paczynski-Wiita radii and drifts use the proper velocity, and with spins
+// (jx non-null) the holes drag frames (framedrag.go) around the outer kicks
+// and the substeps' drifts.
+int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, double* jx, double* jy, double* jz,
+               double dt, double c, double eps, int accValid, int32_t* substepsOut) {
     const long long n = a.n;
     int sub = 1;
     if (substepsOut) *substepsOut = 1;
@@ -743,7 +917,7 @@ int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, dou
     const long long nb = std::max(1LL, (n + kCollectTile - 1) / kCollectTile);
     int rc;
     if ((rc = reserve(s, n)) || (rc = ensure(s->nearFlag, (size_t)n)) || (rc = ensure(s->nearList, (size_t)n * 4)) ||
-        (rc = ensure(s->tileCnt, (size_t)nb * 4)) || (rc = ensure(s->splitScalars, 16)) ||
+        (rc = ensure(s->tileCnt, (size_t)nb * 4)) || (rc = ensure(s->splitScalars, 32)) ||
         (rc = ensure(s->farF, (size_t)n * 24)) || (rc = ensure(s->fastF, (size_t)n * 24)))
         return rc;
     // slices of the holes' pair sums: ~4096 bodies per workgroup over the far bodies
@@ -804,6 +978,26 @@ int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, dou
         while ((double)sub * tauMin < 20.0 * dt && sub < 256) sub *= 2;
     }
     if (substepsOut) *substepsOut = sub;
+    // frame dragging (framedrag.go): acts only with c > 0 and a spinning hole
+    const bool dragOn = jx && c > 0;
+    const long long nnHost = (long long)s->hhist[0];
+    DragArgs da{x, y, z, a.m, a.alive, holes, (int)nh, eps2, c, 0.0};
+    DragBufs db{};
+    long long nOuter = 0;
+    if (dragOn && ((rc = drag_bufs(s, nh, db)) || (rc = drag_outer_list(s, n, a.kind, a.alive, nearFlag, &nOuter))))
+        return rc;
+    auto drag_o = [&](double h) -> int {  // dragOuter, framedrag.go:115-127
+        if (!dragOn) return GX_OK;
+        DragArgs d = da;
+        d.dt = h;
+        return drag_outer(s, d, nOuter, vx, vy, vz, jx, jy, jz, db, meta);
+    };
+    auto drag_f = [&](double h) -> int {  // dragFast, framedrag.go:129-143
+        if (!dragOn) return GX_OK;
+        DragArgs d = da;
+        d.dt = h;
+        return drag_fast(s, d, (const int32_t*)nearList, nnHost, vx, vy, vz, jx, jy, jz, db, meta);
+    };
     auto outer = [&]() {  // outerHoleForces, split.go:82-117
         hipLaunchKernelGGL(k_split_far, dim3(grid), dim3(256), 0, st, n, (const double*)x, (const double*)y,
                            (const double*)z, a.m, a.kind, (const uint8_t*)nearFlag, holes, meta, eps2, c, farX, farY,
@@ -841,16 +1035,19 @@ int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, dou
     };
     outer();
     kick_outer(0.5 * dt);
+    if ((rc = drag_o(0.5 * dt))) return rc;
     const double h = dt / (double)sub;
     fast();
     for (int k = 0; k < sub; k++) {
         kick_fast(0.5 * h);
+        if ((rc = drag_f(0.5 * h))) return rc;
         hipLaunchKernelGGL(k_split_drift_list, dim3(grid), dim3(256), 0, st, (const int32_t*)nearList, 0LL,
                            (const long long*)nnDev, x, y, z, (const double*)vx, (const double*)vy, (const double*)vz,
                            h, c);
         hipLaunchKernelGGL(k_split_drift_list, dim3(1), dim3(256), 0, st, holes, nh, (const long long*)nullptr, x, y, z,
                            (const double*)vx, (const double*)vy, (const double*)vz, h, c);
         fast();
+        if ((rc = drag_f(0.5 * h))) return rc;
         kick_fast(0.5 * h);
     }
     hipLaunchKernelGGL(k_split_drift_far, dim3(grid), dim3(256), 0, st, n, a.kind, (const uint8_t*)nearFlag, x, y, z,
@@ -858,6 +1055,7 @@ int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, dou
     HIPCHK(hipGetLastError());
     if ((rc = run_dev(s, a))) return rc;  // the slow forces at the new positions
     outer();
+    if ((rc = drag_o(0.5 * dt))) return rc;
     kick_outer(0.5 * dt);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
@@ -878,7 +1076,80 @@ int gx_split_step_dev(gx_solver* s, double* x, double* y, double* z, double* vx,
     if ((rc = check_arrays(a)) || (n > 0 && (!vx || !vy || !vz) && (rc = set_err(GX_ERR_ARG, "null array"))) ||
         (rc = set_single(s, n, s->theta, eps)))
         return rc;
-    return split_step(s, a, vx, vy, vz, dt, c, eps, acc_valid, substeps);
+    return split_step(s, a, vx, vy, vz, nullptr, nullptr, nullptr, dt, c, eps, acc_valid, substeps);
+}
+
+int gx_split_step_spin_dev(gx_solver* s, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                           const double* mass, const uint8_t* kind, const uint8_t* alive, double* spin_x,
+                           double* spin_y, double* spin_z, int64_t n, double eps, int dark_matter_off, double dt,
+                           double c, int acc_valid, double* ax, double* ay, double* az, int32_t* substeps) {
+    if (!s) return set_err(GX_ERR_ARG, "null solver");
+    const StepArgs a = step_args(x, y, z, mass, kind, alive, n, dark_matter_off, ax, ay, az);
+    int rc;
+    if ((rc = check_arrays(a)) ||
+        (n > 0 && (!vx || !vy || !vz || !spin_x || !spin_y || !spin_z) && (rc = set_err(GX_ERR_ARG, "null array"))) ||
+        (rc = set_single(s, n, s->theta, eps)))
+        return rc;
+    return split_step(s, a, vx, vy, vz, n > 0 ? spin_x : nullptr, spin_y, spin_z, dt, c, eps, acc_valid, substeps);
+}
+
+// Shared set-up of the frame-dragging entry points: the alive holes of the
+// state (collect), their count, the chain scalars.
+static int drag_setup(gx_solver* s, const double* x, const double* y, const double* z, const double* mass,
+                      const uint8_t* kind, const uint8_t* alive, double* vx, double* vy, double* vz, double* jx,
+                      double* jy, double* jz, int64_t n, double eps, DragBufs& b, long long* nh) {
+    if (!s) return set_err(GX_ERR_ARG, "null solver");
+    if (n < 0) return set_err(GX_ERR_ARG, "negative n");
+    if (n > 0 && (!x || !y || !z || !mass || !kind || !alive || !vx || !vy || !vz || !jx || !jy || !jz))
+        return set_err(GX_ERR_ARG, "null array");
+    if (n > ((1LL << 31) - 1)) return set_err(GX_ERR_ARG, "n must be below 2^31");
+    *nh = 0;
+    if (n == 0) return GX_OK;
+    int rc;
+    if ((rc = set_single(s, n, s->theta, eps)) || (rc = reserve(s, n))) return rc;
+    const StepArgs a = step_args(x, y, z, mass, kind, alive, n, 0, nullptr, nullptr, nullptr);
+    enqueue_collect(s, a);
+    HIPCHK(hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    *nh = s->hmeta->nholes;
+    return drag_bufs(s, *nh, b);
+}
+
+int gx_drag_outer_dev(gx_solver* s, const double* x, const double* y, const double* z, double* vx, double* vy,
+                      double* vz, const double* mass, const uint8_t* kind, const uint8_t* alive, const uint8_t* near,
+                      double* spin_x, double* spin_y, double* spin_z, int64_t n, double eps, double dt, double c) {
+    DragBufs b{};
+    long long nh = 0;
+    int rc = drag_setup(s, x, y, z, mass, kind, alive, vx, vy, vz, spin_x, spin_y, spin_z, n, eps, b, &nh);
+    if (rc || nh == 0) return rc;
+    long long count = 0;
+    if ((rc = drag_outer_list(s, n, kind, alive, near, &count))) return rc;
+    const DragArgs a{x, y, z, mass, alive, (const int32_t*)s->holes.p, (int)nh, eps * eps, c, dt};
+    if ((rc = drag_outer(s, a, count, vx, vy, vz, spin_x, spin_y, spin_z, b, (const Meta*)s->meta.p))) return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return GX_OK;
+}
+
+int gx_drag_fast_dev(gx_solver* s, const double* x, const double* y, const double* z, double* vx, double* vy,
+                     double* vz, const double* mass, const uint8_t* kind, const uint8_t* alive,
+                     const int32_t* near_list, int64_t near_count, double* spin_x, double* spin_y, double* spin_z,
+                     int64_t n, double eps, double dt, double c) {
+    if (near_count < 0 || (near_count > 0 && !near_list)) return set_err(GX_ERR_ARG, "bad near list");
+    DragBufs b{};
+    long long nh = 0;
+    int rc = drag_setup(s, x, y, z, mass, kind, alive, vx, vy, vz, spin_x, spin_y, spin_z, n, eps, b, &nh);
+    if (rc || nh == 0) return rc;
+    const DragArgs a{x, y, z, mass, alive, (const int32_t*)s->holes.p, (int)nh, eps * eps, c, dt};
+    if ((rc = drag_fast(s, a, near_list, near_count, vx, vy, vz, spin_x, spin_y, spin_z, b, (const Meta*)s->meta.p)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return GX_OK;
+}
+
+int gx_drag_get_stats(const gx_solver* s, gx_drag_stats* out) {
+    if (!s || !out) return set_err(GX_ERR_ARG, "null argument");
+    *out = s->dragStats;
+    return GX_OK;
 }
 
 int gx_accelerations_active_dev(gx_solver* s, const double* x, const double* y, const double* z,

@@ -1,0 +1,539 @@
+// gx_drag.hip -- frame dragging of spinning black holes on the GPU,
+// internal/sim/framedrag.go.
+//
+// The reference kicks bodies one after another in a fixed order (dragOuter:
+// state order; dragFast: the near list, then the holes).  Each kick rotates
+// the body's velocity in the gravitomagnetic field of the holes' spins and
+// books the opposite momentum and angular momentum on the holes, so every kick
+// reads the spins that all earlier kicks left.  The pass over a list of
+// bodies is evaluated here as a fixed point that is bit-identical to that
+// sequential loop:
+//   1. every body j of the list is kicked in parallel with a guess of the
+//      spins it sees (first the spins at the start of the pass, then the
+//      previous iterate), giving its spin increments dJ_j, one per hole and
+//      component (-0.0 where the body gives nothing: x + -0.0 == x for every
+//      x, signed zeros included);
+//   2. the spins the bodies see are the exclusive prefix sums of the dJ in
+//      list order, rounded as the sequential loop rounds them (the exact
+//      ordered scan below);
+//   3. if these equal the guess bit for bit, the guess is a fixed point of the
+//      sequential recursion S_{j+1} = S_j + dJ(S_j), S_0 = the start spins,
+//      whose solution is unique, so it is the reference's chain; otherwise
+//      the prefix sums become the next guess.  Iterate k is exact at least on
+//      positions 0..k; a kick changes a spin by a tiny relative amount, so in
+//      practice the second or third iterate is the fixed point.  Past an
+//      iteration limit the pass runs as one sequential thread.
+// The holes' velocities are sums of the same kind: once the spins are exact,
+// one more ordered scan of their increments.  A body's own velocity depends
+// only on the spins it sees.
+//
+// Exact ordered scan.  A chain s_{t+1} = fl(s_t + d_t) is cut into segments of
+// kDragSeg elements.  While s_t lies strictly inside one binade, |s_t| in
+// (2^E, 2^(E+1)), a step is s_{t+1} = s_t + u rint(d_t / u), u = 2^(E-52),
+// whenever d_t / u is not a half-integer: the exact sum lies within u/2 of
+// that grid point, which is inside the binade, and the grid there is u.  So
+// a segment whose partial sums all stay inside the binade of its start value
+// is an integer sum, which is associative: its total and its partial minimum
+// and maximum are computed in parallel (one wavefront per segment), and one
+// thread per chain walks the segments in order from the exact start value,
+// checking the binade condition from the segment's partial extremes.  A
+// segment that fails it (a binade crossing, a zero or subnormal value, a
+// tie, a wrong binade guess) is added element by element, as the reference
+// adds.  The binade of a segment is guessed from an approximate prefix sum.
+//
+// Compiled with -ffp-contract=off: the kick repeats framedrag.go's
+// operations in its order.  One deliberate difference: dragBack's weights
+// |J| / r^3 use r2 * sqrt(r2) for math.Pow(r2, 1.5), which Go evaluates
+// through Exp and Log (a few ulp apart); the weights only enter as
+// w / sum(w), which is exactly 1 with one spinning hole.
+#include "gx_common.h"
+#include "gx_kernels.h"
+#include "gx_pair.h"
+
+namespace gx {
+
+namespace {
+
+__device__ __forceinline__ double pow15(double x) { return x * sqrt(x); }
+
+__device__ __forceinline__ unsigned long long dbits(double v) { return (unsigned long long)__double_as_longlong(v); }
+
+constexpr unsigned long long kNegZero = 0x8000000000000000ull;
+
+// Spins seen by the body at list position j: the guess S (chain-major,
+// S[(3k + comp) * ne + j]) or, with S null, the holes' spins.
+struct SpinView {
+    const double* S;
+    long long ne, j;
+    const double *jx, *jy, *jz;
+    const int32_t* holes;
+    __device__ __forceinline__ void operator()(int k, double& x, double& y, double& z) const {
+        if (S) {
+            x = S[(3LL * k) * ne + j];
+            y = S[(3LL * k + 1) * ne + j];
+            z = S[(3LL * k + 2) * ne + j];
+        } else {
+            const int32_t h = holes[k];
+            x = jx[h];
+            y = jy[h];
+            z = jz[h];
+        }
+    }
+};
+
+// The spins in memory (the sequential kernels).
+struct SpinLive {
+    const double *jx, *jy, *jz;
+    const int32_t* holes;
+    __device__ __forceinline__ void operator()(int k, double& x, double& y, double& z) const {
+        const int32_t h = holes[k];
+        x = jx[h];
+        y = jy[h];
+        z = jz[h];
+    }
+};
+
+// gravitomagnetic (framedrag.go:34-54) and the Boris rotation of dragKick
+// (:56-72) of body i with proper velocity u; false if B = 0 (no kick).
+template <class Spin>
+__device__ __forceinline__ bool drag_rotate(long long i, const DragArgs& a, const Spin& spin, double ux, double uy,
+                                            double uz, double& unx, double& uny, double& unz) {
+    const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
+    double bx = 0, by = 0, bz = 0;
+    for (int k = 0; k < a.nh; k++) {
+        const int32_t h = a.holes[k];
+        if (h == i || !a.alive[h]) continue;
+        double jx, jy, jz;
+        spin(k, jx, jy, jz);
+        if (jx * jx + jy * jy + jz * jz == 0) continue;
+        const double rx = xi - a.x[h], ry = yi - a.y[h], rz = zi - a.z[h];
+        const double r2 = (rx * rx + ry * ry + rz * rz) + a.eps2;
+        const double inv = 1 / sqrt(r2);
+        const double k3 = 2 / (a.c * a.c) * inv * inv * inv;
+        const double sc = 3 * (jx * rx + jy * ry + jz * rz) / r2;
+        bx = bx + (jx - rx * sc) * k3;
+        by = by + (jy - ry * sc) * k3;
+        bz = bz + (jz - rz * sc) * k3;
+    }
+    if (bx * bx + by * by + bz * bz == 0) return false;
+    const double g = gamma_of(ux, uy, uz, a.c);
+    const double hs = 0.5 * a.dt / g;
+    const double tx = bx * hs, ty = by * hs, tz = bz * hs;
+    const double upx = ux + (uy * tz - uz * ty), upy = uy + (uz * tx - ux * tz), upz = uz + (ux * ty - uy * tx);
+    const double s2 = 2 / (1 + (tx * tx + ty * ty + tz * tz));
+    const double sx = tx * s2, sy = ty * s2, sz = tz * s2;
+    unx = ux + (upy * sz - upz * sy);
+    uny = uy + (upz * sx - upx * sz);
+    unz = uz + (upx * sy - upy * sx);
+    return true;
+}
+
+// dragBack's weight of hole slot k for body i (framedrag.go:82-93).
+template <class Spin>
+__device__ __forceinline__ double drag_weight(long long i, int k, const DragArgs& a, const Spin& spin) {
+    const int32_t h = a.holes[k];
+    if (h == i || !a.alive[h]) return 0;
+    const double rx = a.x[i] - a.x[h], ry = a.y[i] - a.y[h], rz = a.z[i] - a.z[h];
+    double jx, jy, jz;
+    spin(k, jx, jy, jz);
+    const double j = sqrt(jx * jx + jy * jy + jz * jz);
+    if (j > 0) return j / pow15(rx * rx + ry * ry + rz * rz);
+    return 0;
+}
+
+template <class Spin>
+__device__ __forceinline__ double drag_total(long long i, const DragArgs& a, const Spin& spin) {
+    double total = 0;
+    for (int k = 0; k < a.nh; k++) total += drag_weight(i, k, a, spin);
+    return total;
+}
+
+// One dragKick (framedrag.go:56-113) on the arrays in memory, for the
+// sequential kernels.
+__device__ void drag_kick_live(long long i, const DragArgs& a, double* vx, double* vy, double* vz, double* jx,
+                               double* jy, double* jz) {
+    const SpinLive spin{jx, jy, jz, a.holes};
+    const double ux = vx[i], uy = vy[i], uz = vz[i];
+    double unx, uny, unz;
+    if (!drag_rotate(i, a, spin, ux, uy, uz, unx, uny, unz)) return;
+    vx[i] = unx;
+    vy[i] = uny;
+    vz[i] = unz;
+    const double mi = a.mass[i];
+    const double pxm = (unx - ux) * mi, pym = (uny - uy) * mi, pzm = (unz - uz) * mi;
+    const double total = drag_total(i, a, spin);
+    if (total == 0) return;
+    for (int k = 0; k < a.nh; k++) {
+        const double w = drag_weight(i, k, a, spin);
+        if (w == 0) continue;
+        const int32_t h = a.holes[k];
+        const double f = w / total, mh = a.mass[h];
+        vx[h] += (pxm * -f) / mh;
+        vy[h] += (pym * -f) / mh;
+        vz[h] += (pzm * -f) / mh;
+        const double rx = a.x[i] - a.x[h], ry = a.y[i] - a.y[h], rz = a.z[i] - a.z[h];
+        const double gx = pxm * f, gy = pym * f, gz = pzm * f;
+        jx[h] += (ry * gz - rz * gy) * -1;
+        jy[h] += (rz * gx - rx * gz) * -1;
+        jz[h] += (rx * gy - ry * gx) * -1;
+    }
+}
+
+__device__ __forceinline__ long long wave_incl_scan(long long v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const long long o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// The integer step of element d in binade e (see the header): n = rint(d / u)
+// with u = 2^(e-52); big if |d / u| >= 2^53 (or not finite), tie if d / u is
+// a half-integer.  ldexp by a power of two is exact unless the result is
+// subnormal, and then |d / u| < 2^-1022 and n = 0 with no tie either way.
+__device__ __forceinline__ long long seg_step(double d, int e, bool& big, bool& tie) {
+    const double q = ldexp(d, 52 - e);
+    big = !(fabs(q) < 9007199254740992.0);
+    if (big) {
+        tie = false;
+        return 0;
+    }
+    tie = fabs(q - trunc(q)) == 0.5;
+    return (long long)rint(q);
+}
+
+__device__ __forceinline__ int dexp(double v) { return (int)((dbits(v) >> 52) & 0x7ff); }
+
+}  // namespace
+
+__global__ void k_drag_spinning(const int32_t* __restrict__ holes, const Meta* __restrict__ meta,
+                                const double* __restrict__ jx, const double* __restrict__ jy,
+                                const double* __restrict__ jz, double c, int* __restrict__ on) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int v = 0;
+    if (c > 0)
+        for (long long k = 0; k < meta->nholes; k++) {
+            const int32_t h = holes[k];
+            if (jx[h] != 0 || jy[h] != 0 || jz[h] != 0) v = 1;
+        }
+    *on = v;
+}
+
+// dragOuter's bodies (framedrag.go:115-127): alive, not a hole, not near.
+__global__ __launch_bounds__(256) void k_drag_outer_flag(long long n, const uint8_t* __restrict__ kind,
+                                                         const uint8_t* __restrict__ alive,
+                                                         const uint8_t* __restrict__ nearFlag,
+                                                         uint8_t* __restrict__ flag) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        flag[i] = kind[i] != kHole && alive[i] && !(nearFlag && nearFlag[i]);
+}
+
+// The chain starts: spins (mode 0) or velocities (mode 1) of the holes,
+// start[3k + comp]; with mode 2 the chain ends are written back instead.
+__global__ void k_drag_chain_io(const int32_t* __restrict__ holes, int nh, double* __restrict__ ax,
+                                double* __restrict__ ay, double* __restrict__ az, double* __restrict__ vals,
+                                int writeBack) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nh; k += gridDim.x * blockDim.x) {
+        const int32_t h = holes[k];
+        if (writeBack) {
+            ax[h] = vals[3 * k];
+            ay[h] = vals[3 * k + 1];
+            az[h] = vals[3 * k + 2];
+        } else {
+            vals[3 * k] = ax[h];
+            vals[3 * k + 1] = ay[h];
+            vals[3 * k + 2] = az[h];
+        }
+    }
+}
+
+// Step 1 (apply = 0): the spin increments of every body of the list with the
+// spins it sees (S, or the holes' spins with S null).  apply = 1: the body's
+// new velocity is written and D receives the holes' velocity increments.
+// Without frame dragging (*on == 0) every increment is -0.0.
+__global__ __launch_bounds__(256) void k_drag_eval(DragArgs a, const int32_t* __restrict__ list, long long ne,
+                                                   double* __restrict__ vx, double* __restrict__ vy,
+                                                   double* __restrict__ vz, const double* __restrict__ jx,
+                                                   const double* __restrict__ jy, const double* __restrict__ jz,
+                                                   const double* __restrict__ S, int apply,
+                                                   const int* __restrict__ on, double* __restrict__ D) {
+    const bool active = *on != 0;
+    const double nz = __longlong_as_double((long long)kNegZero);
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < ne; j += (long long)gridDim.x * blockDim.x) {
+        const long long i = list[j];
+        const SpinView spin{S, ne, j, jx, jy, jz, a.holes};
+        const double ux = vx[i], uy = vy[i], uz = vz[i];
+        double unx, uny, unz;
+        const bool kicked = active && drag_rotate(i, a, spin, ux, uy, uz, unx, uny, unz);
+        if (!kicked) {
+            for (int k = 0; k < 3 * a.nh; k++) D[k * ne + j] = nz;
+            continue;
+        }
+        if (apply) {
+            vx[i] = unx;
+            vy[i] = uny;
+            vz[i] = unz;
+        }
+        const double mi = a.mass[i];
+        const double pxm = (unx - ux) * mi, pym = (uny - uy) * mi, pzm = (unz - uz) * mi;
+        const double total = drag_total(i, a, spin);
+        for (int k = 0; k < a.nh; k++) {
+            const double w = total == 0 ? 0.0 : drag_weight(i, k, a, spin);
+            double o0 = nz, o1 = nz, o2 = nz;
+            if (w != 0) {
+                const int32_t h = a.holes[k];
+                const double f = w / total;
+                if (apply) {
+                    const double mh = a.mass[h];
+                    o0 = (pxm * -f) / mh;
+                    o1 = (pym * -f) / mh;
+                    o2 = (pzm * -f) / mh;
+                } else {
+                    const double rx = a.x[i] - a.x[h], ry = a.y[i] - a.y[h], rz = a.z[i] - a.z[h];
+                    const double gx = pxm * f, gy = pym * f, gz = pzm * f;
+                    o0 = (ry * gz - rz * gy) * -1;
+                    o1 = (rz * gx - rx * gz) * -1;
+                    o2 = (rx * gy - ry * gx) * -1;
+                }
+            }
+            D[(3LL * k) * ne + j] = o0;
+            D[(3LL * k + 1) * ne + j] = o1;
+            D[(3LL * k + 2) * ne + j] = o2;
+        }
+    }
+}
+
+// The pass as one thread, in the reference's order: the fallback of the fixed
+// point, and dragFast's holes (framedrag.go:138-142).
+__global__ void k_drag_seq(DragArgs a, const int32_t* __restrict__ list, long long ne, double* vx, double* vy,
+                           double* vz, double* jx, double* jy, double* jz, const int* __restrict__ on) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || !*on) return;
+    for (long long j = 0; j < ne; j++) drag_kick_live(list[j], a, vx, vy, vz, jx, jy, jz);
+}
+
+// Exact ordered scan, step A: an approximate sum of every segment (one
+// wavefront per segment; chain c, segment g).
+__global__ __launch_bounds__(256) void k_scan_segsum(const double* __restrict__ D, long long ne, int chains, int nseg,
+                                                     double* __restrict__ sig) {
+    const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (w >= (long long)chains * nseg) return;
+    const int c = (int)(w / nseg), g = (int)(w % nseg);
+    const long long lo = (long long)g * kDragSeg, hi = min(ne, lo + kDragSeg);
+    double v = 0;
+    for (long long t = lo + lane; t < hi; t += 64) v += D[c * ne + t];
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (lane == 0) sig[w] = v;
+}
+
+// Step B: approximate start values of the segments, guess[g] = start +
+// sum of the segments before g (one workgroup of 1024 threads per chain).
+__global__ __launch_bounds__(1024) void k_scan_guess(const double* __restrict__ sig, int nseg,
+                                                     const double* __restrict__ start, double* __restrict__ guess) {
+    __shared__ double buf[1024];
+    const int c = blockIdx.x, tid = threadIdx.x;
+    double carry = start[c];
+    for (int base = 0; base < nseg; base += 1024) {
+        const int g = base + tid;
+        const double v = g < nseg ? sig[(long long)c * nseg + g] : 0.0;
+        buf[tid] = v;
+        __syncthreads();
+        for (int d = 1; d < 1024; d <<= 1) {
+            const double o = tid >= d ? buf[tid - d] : 0.0;
+            __syncthreads();
+            buf[tid] += o;
+            __syncthreads();
+        }
+        if (g < nseg) guess[(long long)c * nseg + g] = carry + (buf[tid] - v);
+        carry += buf[1023];
+        __syncthreads();
+    }
+}
+
+// Step C: the integer summary of every segment in the binade of its guess.
+__global__ __launch_bounds__(256) void k_scan_summary(const double* __restrict__ D, long long ne, int chains, int nseg,
+                                                      const double* __restrict__ guess, DragSeg* __restrict__ seg) {
+    const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (w >= (long long)chains * nseg) return;
+    const int c = (int)(w / nseg), g = (int)(w % nseg);
+    const long long lo = (long long)g * kDragSeg, hi = min(ne, lo + kDragSeg);
+    const double A = guess[w];
+    const int ea = dexp(A);
+    int flags = 0;
+    if (ea == 0 || ea == 0x7ff) flags |= kSegNoGuess;
+    const int e = ea - 1023;
+    long long carry = 0, pmin = 0, pmax = 0;
+    bool big = false, tie = false, allNegZero = true;
+    for (long long r = lo; r < hi; r += 64) {
+        const long long t = r + lane;
+        const double d = t < hi ? D[c * ne + t] : __longlong_as_double((long long)kNegZero);
+        allNegZero = allNegZero && dbits(d) == kNegZero;
+        bool b = false, ti = false;
+        long long n = 0;
+        if (!(flags & kSegNoGuess)) n = seg_step(d, e, b, ti);
+        big = big || b;
+        tie = tie || ti;
+        const long long incl = wave_incl_scan(n);
+        const long long p = carry + incl;
+        pmin = min(pmin, p);
+        pmax = max(pmax, p);
+        carry += __shfl(incl, 63, 64);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        pmin = min(pmin, __shfl_xor(pmin, d, 64));
+        pmax = max(pmax, __shfl_xor(pmax, d, 64));
+    }
+    if (__any(big)) flags |= kSegBig;
+    if (__any(tie)) flags |= kSegTie;
+    if (__all(allNegZero)) flags |= kSegAllNegZero;
+    if (lane == 0) {
+        DragSeg s;
+        s.total = carry;
+        s.pmin = pmin;
+        s.pmax = pmax;
+        s.e = e;
+        s.flags = flags;
+        seg[w] = s;
+    }
+}
+
+// Step D: the exact chain over the segments, one workgroup per chain.
+// Thread 0 carries the exact value; a segment whose integer summary is not
+// valid from that value is added element by element (its elements staged in
+// LDS by the workgroup), writing the values the bodies see when S is given.
+// segStart/segMode receive each segment's exact start value and how it was
+// done (0 integer, 1 unchanged, 2 element by element); end[c] the chain's
+// final value; counters[0] += segments done element by element.
+__global__ __launch_bounds__(256) void k_scan_chain(const double* __restrict__ D, long long ne, int nseg,
+                                                    const DragSeg* __restrict__ seg, const double* __restrict__ start,
+                                                    double* __restrict__ S, double* __restrict__ segStart,
+                                                    int* __restrict__ segMode, double* __restrict__ end,
+                                                    unsigned long long* __restrict__ counters) {
+    __shared__ DragSeg tile[256];
+    __shared__ double buf[kDragSeg];
+    __shared__ int slowAt;
+    __shared__ double carried;
+    const int c = blockIdx.x, tid = threadIdx.x;
+    const long long two52 = 1LL << 52, two53 = 1LL << 53, two54 = 1LL << 54;
+    double s = start[c];
+    unsigned long long slowCount = 0;
+    for (int base = 0; base < nseg; base += 256) {
+        const int cnt = min(256, nseg - base);
+        if (tid < cnt) tile[tid] = seg[(long long)c * nseg + base + tid];
+        __syncthreads();
+        int pos = 0;
+        while (pos < cnt) {
+            if (tid == 0) {
+                int q = pos;
+                for (; q < cnt; q++) {
+                    const DragSeg ss = tile[q];
+                    const long long gi = (long long)c * nseg + base + q;
+                    segStart[gi] = s;
+                    if (ss.flags & kSegAllNegZero) {
+                        segMode[gi] = 1;
+                        continue;
+                    }
+                    const int es = dexp(s);
+                    bool ok = !(ss.flags & (kSegNoGuess | kSegBig | kSegTie)) && es != 0 && es != 0x7ff &&
+                              es - 1023 == ss.e && ss.pmin >= -two54 && ss.pmax <= two54;
+                    if (ok) {
+                        const long long M = (long long)ldexp(s, 52 - ss.e);  // exact: |M| in [2^52, 2^53)
+                        const long long lo = M + ss.pmin, hi = M + ss.pmax;  // every partial value
+                        ok = M > 0 ? (lo > two52 && hi < two53) : (lo > -two53 && hi < -two52);
+                        if (ok) {
+                            segMode[gi] = 0;
+                            s = ldexp((double)(M + ss.total), ss.e - 52);
+                        }
+                    }
+                    if (!ok) {
+                        segMode[gi] = 2;
+                        break;
+                    }
+                }
+                slowAt = q;
+                carried = s;
+            }
+            __syncthreads();
+            const int q = slowAt;
+            if (q < cnt) {
+                const long long lo = (long long)(base + q) * kDragSeg;
+                const int len = (int)min((long long)kDragSeg, ne - lo);
+                for (int t = tid; t < len; t += blockDim.x) buf[t] = D[c * ne + lo + t];
+                __syncthreads();
+                if (tid == 0) {
+                    double v = carried;
+                    if (S)
+                        for (int t = 0; t < len; t++) {
+                            S[c * ne + lo + t] = v;
+                            v = v + buf[t];
+                        }
+                    else
+                        for (int t = 0; t < len; t++) v = v + buf[t];
+                    s = v;
+                    slowCount++;
+                }
+                __syncthreads();
+                pos = q + 1;
+            } else {
+                pos = cnt;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        end[c] = s;
+        if (slowCount) atomicAdd(counters, slowCount);
+    }
+}
+
+// Step E: the values the bodies see, S[c][t] (exclusive prefix), for the
+// integer and unchanged segments (the element-by-element ones were written by
+// step D), and the comparison with the previous guess Sold (null: the start
+// values): *changed = 1 if any differs bit for bit.
+__global__ __launch_bounds__(256) void k_scan_fill(const double* __restrict__ D, long long ne, int chains, int nseg,
+                                                   const DragSeg* __restrict__ seg,
+                                                   const double* __restrict__ segStart,
+                                                   const int* __restrict__ segMode, const double* __restrict__ start,
+                                                   const double* __restrict__ Sold, double* __restrict__ S,
+                                                   int* __restrict__ changed) {
+    const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (w >= (long long)chains * nseg) return;
+    const int c = (int)(w / nseg), g = (int)(w % nseg);
+    const long long lo = (long long)g * kDragSeg, hi = min(ne, lo + kDragSeg);
+    const int mode = segMode[w];
+    const double s0 = segStart[w];
+    const DragSeg ss = seg[w];
+    const long long M = mode == 0 ? (long long)ldexp(s0, 52 - ss.e) : 0;
+    long long carry = 0;
+    bool diff = false;
+    for (long long r = lo; r < hi; r += 64) {
+        const long long t = r + lane;
+        double v;
+        if (mode == 0) {
+            const double d = t < hi ? D[c * ne + t] : 0.0;
+            bool b, ti;
+            const long long n = t < hi ? seg_step(d, ss.e, b, ti) : 0;
+            const long long incl = wave_incl_scan(n);
+            v = ldexp((double)(M + carry + incl - n), ss.e - 52);
+            carry += __shfl(incl, 63, 64);
+        } else if (mode == 1) {
+            v = s0;
+        } else {
+            v = t < hi ? S[c * ne + t] : 0.0;
+        }
+        if (t < hi) {
+            if (mode != 2) S[c * ne + t] = v;
+            const double o = Sold ? Sold[c * ne + t] : start[c];
+            diff = diff || dbits(o) != dbits(v);
+        }
+    }
+    if (__any(diff) && lane == 0) atomicOr(changed, 1);
+}
+
+}  // namespace gx

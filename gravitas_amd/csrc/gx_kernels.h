@@ -71,6 +71,42 @@ __global__ void k_direct_sample(const double* x, const double* y, const double* 
                                 int dmOff, int splits, double* partial);
 __global__ void k_direct_reduce(const double* partial, int nt, int splits, double* out);
 
+// gx_drag.hip: frame dragging (internal/sim/framedrag.go)
+struct DragArgs {
+    const double *x, *y, *z, *mass;
+    const uint8_t* alive;
+    const int32_t* holes;  // the alive holes in state order
+    int nh;
+    double eps2, c, dt;
+};
+// Integer summary of one segment of an exact ordered scan.
+struct DragSeg {
+    long long total, pmin, pmax;  // sum of the integer steps; extremes of the partial sums (0 included)
+    int e, flags;                 // binade exponent guessed for the segment; kSeg* flags
+};
+constexpr int kDragSeg = 1024;  // elements per segment
+constexpr int kSegNoGuess = 1, kSegBig = 2, kSegTie = 4, kSegAllNegZero = 8;
+__global__ void k_drag_spinning(const int32_t* holes, const Meta* meta, const double* jx, const double* jy,
+                                const double* jz, double c, int* on);
+__global__ void k_drag_outer_flag(long long n, const uint8_t* kind, const uint8_t* alive, const uint8_t* nearFlag,
+                                  uint8_t* flag);
+__global__ void k_drag_chain_io(const int32_t* holes, int nh, double* ax, double* ay, double* az, double* vals,
+                                int writeBack);
+__global__ void k_drag_eval(DragArgs a, const int32_t* list, long long ne, double* vx, double* vy, double* vz,
+                            const double* jx, const double* jy, const double* jz, const double* S, int apply,
+                            const int* on, double* D);
+__global__ void k_drag_seq(DragArgs a, const int32_t* list, long long ne, double* vx, double* vy, double* vz,
+                           double* jx, double* jy, double* jz, const int* on);
+__global__ void k_scan_segsum(const double* D, long long ne, int chains, int nseg, double* sig);
+__global__ void k_scan_guess(const double* sig, int nseg, const double* start, double* guess);
+__global__ void k_scan_summary(const double* D, long long ne, int chains, int nseg, const double* guess,
+                               DragSeg* seg);
+__global__ void k_scan_chain(const double* D, long long ne, int nseg, const DragSeg* seg, const double* start,
+                             double* S, double* segStart, int* segMode, double* end, unsigned long long* counters);
+__global__ void k_scan_fill(const double* D, long long ne, int chains, int nseg, const DragSeg* seg,
+                            const double* segStart, const int* segMode, const double* start, const double* Sold,
+                            double* S, int* changed);
+
 // gx_split.hip
 __global__ void k_split_classify(long long n, const double* x, const double* y, const double* z, const double* mass,
                                  const uint8_t* kind, const int32_t* holes, const Meta* meta, double eps2, double c,

@@ -58,9 +58,11 @@ def gpu_name() -> str:
 
 class DeviceState:
     """Struct-of-arrays state resident in HBM (sim.State, internal/sim/state.go:33-52):
-    float64 px, py, pz, vx, vy, vz, ax, ay, az, m; uint8 kind, alive."""
+    float64 px, py, pz, vx, vy, vz, ax, ay, az, m; uint8 kind, alive; and,
+    when the host state has "jx" (spins, state.go:48-49), float64 jx, jy, jz."""
 
     FIELDS = ("px", "py", "pz", "vx", "vy", "vz", "ax", "ay", "az", "m")
+    SPIN = ("jx", "jy", "jz")
 
     def __init__(self, **arrays):
         for k, v in arrays.items():
@@ -76,14 +78,22 @@ class DeviceState:
         kind, alive = s.get("kind"), s.get("alive")
         out["kind"] = DeviceArray.from_numpy(np.zeros(n, np.uint8) if kind is None else np.asarray(kind, np.uint8))
         out["alive"] = DeviceArray.from_numpy(np.ones(n, np.uint8) if alive is None else np.asarray(alive, np.uint8))
+        if "jx" in s:
+            for f in cls.SPIN:
+                out[f] = DeviceArray.from_numpy(np.asarray(s[f], dtype=np.float64))
         return cls(**out)
+
+    @property
+    def has_spin(self):
+        return hasattr(self, "jx")
 
     def __len__(self):
         return len(self.px)
 
     def view(self, start, count):
         """The bodies [start, start+count) as a state of their own (no copy)."""
-        return DeviceState(**{f: getattr(self, f).view(start, count) for f in self.FIELDS + ("kind", "alive")})
+        extra = self.SPIN if self.has_spin else ()
+        return DeviceState(**{f: getattr(self, f).view(start, count) for f in self.FIELDS + ("kind", "alive") + extra})
 
     def acc_host(self):
         return np.stack([self.ax.numpy(), self.ay.numpy(), self.az.numpy()], axis=1)
@@ -252,15 +262,46 @@ class Solver:
         """One step of p.dt with black holes (internal/sim/split.go:164-199):
         tree forces between stars, direct sums for every pair with a hole,
         power-of-two substeps for the holes and the stars near them; p.c > 0
-        switches on the Paczynski-Wiita pairs and relativistic drifts.
-        Returns the number of substeps."""
+        switches on the Paczynski-Wiita pairs and relativistic drifts, and, if
+        the state carries spins, frame dragging (framedrag.go).  Returns the
+        number of substeps."""
         self._configure()
         sub = ctypes.c_int32(0)
-        _lib.check(self._L.gx_split_step_dev(
-            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.vx), _ptr(s.vy), _ptr(s.vz), _ptr(s.m),
-            _ptr(s.kind), _ptr(s.alive), len(s), float(p.softening), int(p.dark_matter_off), float(p.dt),
-            float(p.c), int(bool(acc_valid)), _ptr(s.ax), _ptr(s.ay), _ptr(s.az), ctypes.byref(sub)))
+        head = (self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.vx), _ptr(s.vy), _ptr(s.vz), _ptr(s.m),
+                _ptr(s.kind), _ptr(s.alive))
+        tail = (len(s), float(p.softening), int(p.dark_matter_off), float(p.dt), float(p.c), int(bool(acc_valid)),
+                _ptr(s.ax), _ptr(s.ay), _ptr(s.az), ctypes.byref(sub))
+        if s.has_spin:
+            _lib.check(self._L.gx_split_step_spin_dev(*head, _ptr(s.jx), _ptr(s.jy), _ptr(s.jz), *tail))
+        else:
+            _lib.check(self._L.gx_split_step_dev(*head, *tail))
         return sub.value
+
+    def drag_outer(self, s: DeviceState, p: Params, dt: float, near=None):
+        """Simulation.dragOuter (framedrag.go:115-127) over time dt: every
+        alive body that is neither a hole nor flagged in ``near`` (a uint8
+        DeviceArray, or None) is dragged in state order; velocities and spins
+        change in place."""
+        self._configure()
+        _lib.check(self._L.gx_drag_outer_dev(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.vx), _ptr(s.vy), _ptr(s.vz), _ptr(s.m),
+            _ptr(s.kind), _ptr(s.alive), None if near is None else _ptr(near), _ptr(s.jx), _ptr(s.jy), _ptr(s.jz),
+            len(s), float(p.softening), float(dt), float(p.c)))
+
+    def drag_fast(self, s: DeviceState, p: Params, dt: float, near_list):
+        """Simulation.dragFast (framedrag.go:129-143): the bodies of
+        ``near_list`` (an int32 DeviceArray of state indices) in order, then
+        the holes."""
+        self._configure()
+        _lib.check(self._L.gx_drag_fast_dev(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.vx), _ptr(s.vy), _ptr(s.vz), _ptr(s.m),
+            _ptr(s.kind), _ptr(s.alive), _ptr(near_list) if len(near_list) else None, len(near_list), _ptr(s.jx),
+            _ptr(s.jy), _ptr(s.jz), len(s), float(p.softening), float(dt), float(p.c)))
+
+    def drag_stats(self) -> dict:
+        st = _lib.GxDragStats()
+        _lib.check(self._L.gx_drag_get_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_}
 
     def diagnostics(self, s: DeviceState, p: Params) -> dict:
         """FP64 kinetic energy of the sources, momentum, angular momentum,
@@ -285,7 +326,8 @@ class Solver:
                                         _ptr(s.vz), len(s), float(dt)))
 
     def host_state(self, s: DeviceState) -> dict:
-        return {f: getattr(s, f).numpy() for f in DeviceState.FIELDS}
+        extra = DeviceState.SPIN if s.has_spin else ()
+        return {f: getattr(s, f).numpy() for f in DeviceState.FIELDS + extra}
 
     # -- PotentialEnergy (potential.go:12-42) ----------------------------------------
     def potential_energy(self, s: DeviceState, p: Params) -> float:

@@ -196,15 +196,61 @@ int gx_block_step_dev(gx_solver *s, double *x, double *y, double *z, double *vx,
  * holes' pull at the outer kicks.  Without alive holes it is step()'s KDK
  * (sim.go:266-277).  c is Params.C: <= 0 Newtonian; > 0 Paczynski-Wiita
  * radii for the pairs with holes and drifts with the proper velocity u/gamma
- * (relativity.go).  Restated for states without gas, with non-spinning holes
- * and instant gravity (the reference's frame dragging and retarded sources
- * are not part of this path).  If acc_valid is 0 the forces are computed
+ * (relativity.go).  Restated for states without gas, with instant gravity
+ * and non-spinning holes (gx_split_step_spin_dev adds the spins; the
+ * reference's retarded sources are not part of this path).  If acc_valid is
+ * 0 the forces are computed
  * first.  *substeps (optional) receives the substep count (Simulation.Substeps).
  * Synchronous. */
 int gx_split_step_dev(gx_solver *s, double *x, double *y, double *z, double *vx, double *vy, double *vz,
                       const double *mass, const uint8_t *kind, const uint8_t *alive, int64_t n, double eps,
                       int dark_matter_off, double dt, double c, int acc_valid, double *ax, double *ay, double *az,
                       int32_t *substeps);
+
+/* stepSplit with spinning holes: as gx_split_step_dev, and spin_x/y/z (the
+ * angular momentum J of every body, nonzero only for holes; sim.State.SpinX/Y/Z,
+ * internal/sim/state.go:48-49) switch on frame dragging when c > 0
+ * (internal/sim/framedrag.go): dragOuter after the first outer kick and
+ * before the last one, dragFast around the drifts of every substep
+ * (split.go:176-198).  The spins are updated in place. */
+int gx_split_step_spin_dev(gx_solver *s, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                           const double *mass, const uint8_t *kind, const uint8_t *alive, double *spin_x,
+                           double *spin_y, double *spin_z, int64_t n, double eps, int dark_matter_off, double dt,
+                           double c, int acc_valid, double *ax, double *ay, double *az, int32_t *substeps);
+
+/* Simulation.dragOuter (internal/sim/framedrag.go:115-127) on device arrays:
+ * if c > 0 and an alive hole spins, every alive body that is not a hole and
+ * not near (near may be NULL: none) is kicked in state order by the
+ * gravitomagnetic field of the holes' spins over time dt (a Boris rotation of
+ * its proper velocity), and the holes take the opposite momentum and angular
+ * momentum (velocities and spins updated in place).  Bit-identical to the
+ * reference's sequential loop (see the notes in gx_drag.hip); with two or more
+ * spinning holes the weights use r^2 sqrt(r^2) for math.Pow(r^2, 1.5).
+ * eps is the softening.  Synchronous. */
+int gx_drag_outer_dev(gx_solver *s, const double *x, const double *y, const double *z, double *vx, double *vy,
+                      double *vz, const double *mass, const uint8_t *kind, const uint8_t *alive, const uint8_t *near,
+                      double *spin_x, double *spin_y, double *spin_z, int64_t n, double eps, double dt, double c);
+
+/* Simulation.dragFast (framedrag.go:129-143): the bodies near_list[0 ..
+ * near_count) in that order (state indices; the near list of a split step),
+ * then every alive hole in state order.  Synchronous. */
+int gx_drag_fast_dev(gx_solver *s, const double *x, const double *y, const double *z, double *vx, double *vy,
+                     double *vz, const double *mass, const uint8_t *kind, const uint8_t *alive,
+                     const int32_t *near_list, int64_t near_count, double *spin_x, double *spin_y, double *spin_z,
+                     int64_t n, double eps, double dt, double c);
+
+typedef struct {
+    int64_t passes;            /* passes over a list of bodies (dragOuter, dragFast's near part) */
+    int64_t iterations;        /* fixed-point iterations over all passes */
+    int64_t max_iterations;    /* most iterations of one pass */
+    int64_t sequential_passes; /* passes done by one sequential thread instead */
+    int64_t segments;          /* spin-chain segments scanned (1024 bodies each), over all iterations */
+    int64_t slow_segments;     /* of these, added body by body */
+} gx_drag_stats;
+
+/* Frame-dragging counters since the solver was created (no counterpart in
+ * the reference). */
+int gx_drag_get_stats(const gx_solver *s, gx_drag_stats *out);
 
 #define GX_DIAG_VALUES 11
 

@@ -111,9 +111,14 @@ def lib():
                                     ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.or_diagnostics.argtypes = [ctypes.c_int64, _D, _D, _D, _D, _D, _D, _D, _U8, ctypes.c_int, _D]
         L.or_split_step.restype = ctypes.c_int
-        L.or_split_step.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _D, _D, _D, _U8, _U8, ctypes.c_int64,
-                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                    ctypes.c_int, _D, _D, _D]
+        L.or_split_step.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _D, _D, _D, _U8, _U8, _D, _D, _D,
+                                    ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int, _D, _D, _D]
+        L.or_drag_kick.argtypes = [_D, _D, _D, _D, _D, _D, _D, _U8, _U8, _D, _D, _D, ctypes.c_int64, ctypes.c_int64,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.or_drag_pass.argtypes = [_D, _D, _D, _D, _D, _D, _D, _U8, _U8, _D, _D, _D, ctypes.c_int64, _U8,
+                                   ctypes.POINTER(ctypes.c_int32), ctypes.c_int64, ctypes.c_int, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double]
         assert L.or_node_size() == NODE_DTYPE.itemsize
         # SPH neighbour tree and hydro (sph_oracle.c)
         for f in ("or_cbrt",):
@@ -402,13 +407,16 @@ def diagnostics(px, py, pz, vx, vy, vz, m, kind, dm_off=False):
 
 def split_step(solver, state, m, kind, alive, eps, dt, c=0.0, dm_off=False, acc_valid=True):
     """One step with black holes (internal/sim/split.go:164-199: RESPA
-    substeps for stars near holes, no gas, non-spinning holes, instant
-    gravity) on the float64 arrays px..vz and 'acc' of ``state``, in place;
-    the slow forces come from ``solver`` (an oracle Solver).  Returns the
-    number of substeps."""
+    substeps for stars near holes, no gas, instant gravity) on the float64
+    arrays px..vz and 'acc' of ``state``, in place; the slow forces come from
+    ``solver`` (an oracle Solver).  If ``state`` holds the hole spins "jx",
+    "jy", "jz" (float64, per body), frame dragging (framedrag.go) acts with
+    c > 0 and the spins are updated in place.  Returns the number of
+    substeps."""
     L = lib()
     n = len(state["px"])
-    for k in ("px", "py", "pz", "vx", "vy", "vz"):
+    spin = "jx" in state
+    for k in ("px", "py", "pz", "vx", "vy", "vz") + (("jx", "jy", "jz") if spin else ()):
         state[k] = np.ascontiguousarray(state[k], np.float64)
     if "acc" not in state:
         state["acc"] = np.zeros((n, 3))
@@ -418,11 +426,55 @@ def split_step(solver, state, m, kind, alive, eps, dt, c=0.0, dm_off=False, acc_
     alive = np.ascontiguousarray(alive, np.uint8)
     L.or_set_monopole(solver._h, int(solver.monopole_only))
     L.or_set_workers(solver._h, int(solver.workers))
+    j = tuple(_dp(state[k]) if spin else None for k in ("jx", "jy", "jz"))
     sub = L.or_split_step(solver._h, _dp(state["px"]), _dp(state["py"]), _dp(state["pz"]), _dp(state["vx"]),
-                          _dp(state["vy"]), _dp(state["vz"]), _dp(m), _u8(kind), _u8(alive), n, float(solver.theta),
-                          float(eps), int(dm_off), float(dt), float(c), int(bool(acc_valid)), _dp(ax), _dp(ay), _dp(az))
+                          _dp(state["vy"]), _dp(state["vz"]), _dp(m), _u8(kind), _u8(alive), *j, n,
+                          float(solver.theta), float(eps), int(dm_off), float(dt), float(c), int(bool(acc_valid)),
+                          _dp(ax), _dp(ay), _dp(az))
     state["acc"] = np.stack([ax, ay, az], axis=1)
     return sub
+
+
+def drag_kick(state, m, kind, alive, i, dt, eps2, c):
+    """One frame-dragging kick of body i (dragKick, framedrag.go:56-72) on
+    the arrays px..vz, jx..jz of ``state``, in place."""
+    L = lib()
+    n = len(state["px"])
+    for k in ("px", "py", "pz", "vx", "vy", "vz", "jx", "jy", "jz"):
+        state[k] = np.ascontiguousarray(state[k], np.float64)
+    m = np.ascontiguousarray(m, np.float64)
+    kind = np.ascontiguousarray(kind, np.uint8)
+    alive = np.ascontiguousarray(alive, np.uint8)
+    L.or_drag_kick(*(_dp(state[k]) for k in ("px", "py", "pz", "vx", "vy", "vz")), _dp(m), _u8(kind), _u8(alive),
+                   *(_dp(state[k]) for k in ("jx", "jy", "jz")), n, int(i), float(dt), float(eps2), float(c))
+
+
+def _drag(state, m, kind, alive, near, near_list, fast, dt, eps2, c):
+    L = lib()
+    n = len(state["px"])
+    for k in ("px", "py", "pz", "vx", "vy", "vz", "jx", "jy", "jz"):
+        state[k] = np.ascontiguousarray(state[k], np.float64)
+    m = np.ascontiguousarray(m, np.float64)
+    kind = np.ascontiguousarray(kind, np.uint8)
+    alive = np.ascontiguousarray(alive, np.uint8)
+    near = None if near is None else np.ascontiguousarray(near, np.uint8)
+    lst = np.ascontiguousarray(np.zeros(0) if near_list is None else near_list, np.int32)
+    L.or_drag_pass(*(_dp(state[k]) for k in ("px", "py", "pz", "vx", "vy", "vz")), _dp(m), _u8(kind), _u8(alive),
+                   *(_dp(state[k]) for k in ("jx", "jy", "jz")), n, None if near is None else _u8(near),
+                   lst.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(lst), int(fast), float(dt), float(eps2),
+                   float(c))
+
+
+def drag_outer(state, m, kind, alive, dt, eps2, c, near=None):
+    """dragOuter (framedrag.go:115-127): every alive body of ``state`` that
+    is neither a hole nor flagged in ``near``, in state order, in place."""
+    _drag(state, m, kind, alive, near, None, False, dt, eps2, c)
+
+
+def drag_fast(state, m, kind, alive, near_list, dt, eps2, c):
+    """dragFast (framedrag.go:129-143): the bodies of ``near_list`` in order,
+    then the alive holes, in place."""
+    _drag(state, m, kind, alive, None, near_list, True, dt, eps2, c)
 
 
 # -- SPH neighbour tree and hydro (sph_oracle.c; internal/sph/*.go) ----------------------------

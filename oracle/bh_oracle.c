@@ -732,8 +732,9 @@ void or_diagnostics(int64_t n, const double *px, const double *py, const double 
 }
 
 /* ------------------------------------------------------------------ */
-/* Multiple time stepping near black holes: internal/sim/split.go       */
-/* (no gas, non-spinning holes, instant gravity)                         */
+/* Multiple time stepping near black holes: internal/sim/split.go,      */
+/* with frame dragging of spinning holes (framedrag.go); no gas,         */
+/* instant gravity                                                       */
 /* ------------------------------------------------------------------ */
 /* PairForce, pair.go:26-34 */
 static double pair_force(double m, double r2, double eps2, double rs) {
@@ -766,6 +767,9 @@ typedef struct {
     uint8_t *near;
     int32_t *nearList;
     int64_t nn;
+    const uint8_t *alive;
+    double *jx, *jy, *jz; /* hole spins (angular momentum), NULL: none */
+    double *weights;      /* dragBack's per-hole weights, nh entries */
 } split_ctx;
 
 static double source_mass(const split_ctx *q, int64_t i) {
@@ -902,13 +906,169 @@ static void drift_idx(const split_ctx *q, const int32_t *idx, int64_t cnt, doubl
     }
 }
 
-/* stepSplit, split.go:164-199, for a state with alive holes and no gas; the
- * slow forces are the tree's (theta, eps, dm_off).  Returns the substeps. */
-int or_split_step(or_solver *s, double *px, double *py, double *pz, double *vx, double *vy, double *vz,
-                  const double *mass, const uint8_t *kind, const uint8_t *alive, int64_t n, double theta, double eps,
-                  int dm_off, double dt, double c, int acc_valid, double *ax, double *ay, double *az) {
-    split_ctx q = {px, py, pz, vx, vy, vz, mass, kind, n, eps * eps, c, dm_off, NULL, 0, NULL, NULL, 0};
+/* spinning, framedrag.go:20-32: frame dragging acts if c > 0 and an alive
+ * hole has a spin */
+static int spinning(const split_ctx *q) {
+    if (q->c <= 0 || !q->jx) return 0;
+    for (int64_t k = 0; k < q->nh; k++) {
+        int64_t h = q->holes[k];
+        if (q->jx[h] != 0 || q->jy[h] != 0 || q->jz[h] != 0) return 1;
+    }
+    return 0;
+}
+
+/* x^1.5 for dragBack's weights.  The reference calls math.Pow(x, 1.5), which
+ * Go evaluates as Exp(0.5 Log x) times the binary fraction of x (a few ulp
+ * from x^1.5); restated here as x * sqrt(x).  Only the ratios w_h / sum(w)
+ * are used, so with one spinning hole the share is w / w = 1 either way. */
+static double pow15(double x) { return x * sqrt(x); }
+
+/* gravitomagnetic, framedrag.go:34-54: B at body i from every spinning alive
+ * hole except i, softened like the Newtonian pull */
+static void gravitomagnetic(const split_ctx *q, int64_t i, double b[3]) {
+    double bx = 0, by = 0, bz = 0;
+    for (int64_t k = 0; k < q->nh; k++) {
+        int64_t h = q->holes[k];
+        if (h == i || !q->alive[h]) continue;
+        double jx = q->jx[h], jy = q->jy[h], jz = q->jz[h];
+        if (jx * jx + jy * jy + jz * jz == 0) continue;
+        double rx = q->px[i] - q->px[h], ry = q->py[i] - q->py[h], rz = q->pz[i] - q->pz[h];
+        double r2 = (rx * rx + ry * ry + rz * rz) + q->eps2;
+        double inv = 1 / sqrt(r2);
+        double k3 = 2 / (q->c * q->c) * inv * inv * inv;
+        double sc = 3 * (jx * rx + jy * ry + jz * rz) / r2;
+        bx = bx + (jx - rx * sc) * k3;
+        by = by + (jy - ry * sc) * k3;
+        bz = bz + (jz - rz * sc) * k3;
+    }
+    b[0] = bx; b[1] = by; b[2] = bz;
+}
+
+/* dragBack, framedrag.go:74-113: the holes take -m du in proportion to their
+ * weights |J| / r^3, as momentum and as spin (-(r x m du f)) */
+static void drag_back(const split_ctx *q, int64_t i, double dux, double duy, double duz) {
+    double mi = q->mass[i];
+    double pxm = dux * mi, pym = duy * mi, pzm = duz * mi;
+    double total = 0;
+    for (int64_t k = 0; k < q->nh; k++) {
+        int64_t h = q->holes[k];
+        double w = 0;
+        if (h != i && q->alive[h]) {
+            double rx = q->px[i] - q->px[h], ry = q->py[i] - q->py[h], rz = q->pz[i] - q->pz[h];
+            double j = sqrt(q->jx[h] * q->jx[h] + q->jy[h] * q->jy[h] + q->jz[h] * q->jz[h]);
+            if (j > 0) w = j / pow15(rx * rx + ry * ry + rz * rz);
+        }
+        q->weights[k] = w;
+        total += w;
+    }
+    if (total == 0) return;
+    for (int64_t k = 0; k < q->nh; k++) {
+        if (q->weights[k] == 0) continue;
+        int64_t h = q->holes[k];
+        double f = q->weights[k] / total, mh = q->mass[h];
+        double phx = pxm * -f, phy = pym * -f, phz = pzm * -f;
+        q->vx[h] += phx / mh;
+        q->vy[h] += phy / mh;
+        q->vz[h] += phz / mh;
+        double rx = q->px[i] - q->px[h], ry = q->py[i] - q->py[h], rz = q->pz[i] - q->pz[h];
+        double gx = pxm * f, gy = pym * f, gz = pzm * f;
+        q->jx[h] += (ry * gz - rz * gy) * -1;
+        q->jy[h] += (rz * gx - rx * gz) * -1;
+        q->jz[h] += (rx * gy - ry * gx) * -1;
+    }
+}
+
+/* dragKick, framedrag.go:56-72: Boris rotation of u by B over time dt, then
+ * dragBack */
+static void drag_kick(const split_ctx *q, int64_t i, double dt) {
+    double b[3];
+    gravitomagnetic(q, i, b);
+    if (b[0] * b[0] + b[1] * b[1] + b[2] * b[2] == 0) return;
+    double ux = q->vx[i], uy = q->vy[i], uz = q->vz[i];
+    double g = gamma_of(ux, uy, uz, q->c);
+    double hs = 0.5 * dt / g;
+    double tx = b[0] * hs, ty = b[1] * hs, tz = b[2] * hs;
+    double upx = ux + (uy * tz - uz * ty), upy = uy + (uz * tx - ux * tz), upz = uz + (ux * ty - uy * tx);
+    double s2 = 2 / (1 + (tx * tx + ty * ty + tz * tz));
+    double sx = tx * s2, sy = ty * s2, sz = tz * s2;
+    double unx = ux + (upy * sz - upz * sy), uny = uy + (upz * sx - upx * sz), unz = uz + (upx * sy - upy * sx);
+    q->vx[i] = unx;
+    q->vy[i] = uny;
+    q->vz[i] = unz;
+    drag_back(q, i, unx - ux, uny - uy, unz - uz);
+}
+
+/* dragOuter, framedrag.go:115-127: every alive body that is neither a hole
+ * nor near, in state order */
+static void drag_outer(const split_ctx *q, double dt) {
+    if (!spinning(q)) return;
+    for (int64_t i = 0; i < q->n; i++)
+        if (q->kind[i] != KIND_HOLE && !q->near[i] && q->alive[i]) drag_kick(q, i, dt);
+}
+
+/* dragFast, framedrag.go:129-143: the near list, then the holes */
+static void drag_fast(const split_ctx *q, double dt) {
+    if (!spinning(q)) return;
+    for (int64_t j = 0; j < q->nn; j++) drag_kick(q, q->nearList[j], dt);
+    for (int64_t k = 0; k < q->nh; k++) drag_kick(q, q->holes[k], dt);
+}
+
+/* Test hook: one dragKick of body i against the alive holes of the state
+ * (framedrag_test.go:82-89 calls dragKick directly). */
+void or_drag_kick(double *px, double *py, double *pz, double *vx, double *vy, double *vz, const double *mass,
+                  const uint8_t *kind, const uint8_t *alive, double *jx, double *jy, double *jz, int64_t n, int64_t i,
+                  double dt, double eps2, double c) {
+    split_ctx q = {px, py, pz, vx, vy, vz, mass, kind, n, eps2, c, 0, NULL, 0, NULL, NULL, 0, alive, jx, jy, jz,
+                   NULL};
     int32_t *holes = malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    q.weights = malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    for (int64_t k = 0; k < n; k++)
+        if (alive[k] && kind[k] == KIND_HOLE) holes[q.nh++] = (int32_t)k;
+    q.holes = holes;
+    drag_kick(&q, i, dt);
+    free(holes);
+    free(q.weights);
+}
+
+/* Test hooks: dragOuter over time dt with the near flags near (NULL: none),
+ * and dragFast over the near list list[0 .. count). */
+void or_drag_pass(double *px, double *py, double *pz, double *vx, double *vy, double *vz, const double *mass,
+                  const uint8_t *kind, const uint8_t *alive, double *jx, double *jy, double *jz, int64_t n,
+                  const uint8_t *near, const int32_t *list, int64_t count, int fast, double dt, double eps2,
+                  double c) {
+    split_ctx q = {px, py, pz, vx, vy, vz, mass, kind, n, eps2, c, 0, NULL, 0, NULL, NULL, 0, alive, jx, jy, jz,
+                   NULL};
+    int32_t *holes = malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    q.weights = malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    q.near = calloc((size_t)(n > 0 ? n : 1), 1);
+    if (near) memcpy(q.near, near, (size_t)n);
+    q.nearList = (int32_t *)list;
+    q.nn = count;
+    for (int64_t k = 0; k < n; k++)
+        if (alive[k] && kind[k] == KIND_HOLE) holes[q.nh++] = (int32_t)k;
+    q.holes = holes;
+    if (fast)
+        drag_fast(&q, dt);
+    else
+        drag_outer(&q, dt);
+    free(holes);
+    free(q.weights);
+    free(q.near);
+}
+
+/* stepSplit, split.go:164-199, for a state with alive holes and no gas; the
+ * slow forces are the tree's (theta, eps, dm_off).  jx, jy, jz (the hole
+ * spins, may be NULL) switch on frame dragging (framedrag.go) when c > 0:
+ * after the first outer kick and before the last one, and around the drifts
+ * of every substep.  Returns the substeps. */
+int or_split_step(or_solver *s, double *px, double *py, double *pz, double *vx, double *vy, double *vz,
+                  const double *mass, const uint8_t *kind, const uint8_t *alive, double *jx, double *jy, double *jz,
+                  int64_t n, double theta, double eps, int dm_off, double dt, double c, int acc_valid, double *ax,
+                  double *ay, double *az) {
+    split_ctx q = {px, py, pz, vx, vy, vz, mass, kind, n, eps * eps, c, dm_off, NULL, 0, NULL, NULL, 0, alive,
+                   jx, jy, jz, NULL};
+    int32_t *holes = malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    q.weights = malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
     q.near = malloc((size_t)(n > 0 ? n : 1));
     q.nearList = malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
     double *farx = malloc((size_t)(n > 0 ? n : 1) * 8 * 6);
@@ -937,6 +1097,7 @@ int or_split_step(or_solver *s, double *px, double *py, double *pz, double *vx, 
         or_accelerations(s, px, py, pz, mass, kind, alive, n, theta, eps, dm_off, s->monopole, s->workers, ax, ay, az);
         or_kick(n, vx, vy, vz, ax, ay, az, 0.5 * dt);
         free(holes);
+        free(q.weights);
         free(q.near);
         free(q.nearList);
         free(farx);
@@ -949,14 +1110,17 @@ int or_split_step(or_solver *s, double *px, double *py, double *pz, double *vx, 
         vy[i] += 0.5 * dt * (ay[i] + fary[i]);
         vz[i] += 0.5 * dt * (az[i] + farz[i]);
     }
+    drag_outer(&q, 0.5 * dt);
     double h = dt / (double)sub;
     fast_forces(&q, fsx, fsy, fsz);
     for (int k = 0; k < sub; k++) {
         kick_idx(&q, q.nearList, q.nn, fsx, fsy, fsz, 0.5 * h);
         kick_idx(&q, holes, q.nh, fsx, fsy, fsz, 0.5 * h);
+        drag_fast(&q, 0.5 * h);
         drift_idx(&q, q.nearList, q.nn, h);
         drift_idx(&q, holes, q.nh, h);
         fast_forces(&q, fsx, fsy, fsz);
+        drag_fast(&q, 0.5 * h);
         kick_idx(&q, q.nearList, q.nn, fsx, fsy, fsz, 0.5 * h);
         kick_idx(&q, holes, q.nh, fsx, fsy, fsz, 0.5 * h);
     }
@@ -969,12 +1133,14 @@ int or_split_step(or_solver *s, double *px, double *py, double *pz, double *vx, 
     }
     or_accelerations(s, px, py, pz, mass, kind, alive, n, theta, eps, dm_off, s->monopole, s->workers, ax, ay, az);
     outer_hole_forces(&q, farx, fary, farz);
+    drag_outer(&q, 0.5 * dt);
     for (int64_t i = 0; i < n; i++) {
         vx[i] += 0.5 * dt * (ax[i] + farx[i]);
         vy[i] += 0.5 * dt * (ay[i] + fary[i]);
         vz[i] += 0.5 * dt * (az[i] + farz[i]);
     }
     free(holes);
+    free(q.weights);
     free(q.near);
     free(q.nearList);
     free(farx);
