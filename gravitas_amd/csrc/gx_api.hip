@@ -99,7 +99,7 @@ struct gx_solver {
     DevBuf nearFlag, nearList, tileCnt, splitScalars, farF, fastF;
     // frame dragging: list, chains (increments, two guesses), segment summaries, scalars
     DevBuf dragFlag, dragList, dragD, dragS0, dragS1, dragSig, dragGuess, dragSeg, dragSegStart, dragSegMode,
-        dragScal;
+        dragSpec, dragScal;
     int dragIters = 16;  // GX_DRAG_ITERATIONS: fixed-point iterations before the sequential pass (0: always)
     gx_drag_stats dragStats = {};
     unsigned long long* hhist = nullptr;  // pinned mirror of hist
@@ -233,7 +233,7 @@ void gx_solver_free(gx_solver* s) {
                       &s->rungHist, &s->diag,     &s->nearFlag, &s->nearList, &s->tileCnt,  &s->splitScalars,
                       &s->farF,     &s->fastF,    &s->dragFlag, &s->dragList, &s->dragD,    &s->dragS0,
                       &s->dragS1,   &s->dragSig,  &s->dragGuess, &s->dragSeg, &s->dragSegStart, &s->dragSegMode,
-                      &s->dragScal};
+                      &s->dragSpec, &s->dragScal};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto& e : s->ev)
@@ -763,22 +763,41 @@ int drag_bufs(gx_solver* s, long long nh, DragBufs& b) {
     return GX_OK;
 }
 
-// Exact ordered scan of the C chains of D (chain-major, ne per chain) from
-// start (gx_drag.hip): end receives the chain ends; with S given also the
-// value every position sees, and *changed is set if any differs from Sold
-// (from the start values if Sold is null).
+// Ordered scan of the C chains of D (chain-major, ne per chain) from start
+// (gx_drag.hip).  Exact: end receives the chain ends; with S given also the
+// value every position sees (segments speculated from Sold's values), and
+// *changed is set if any differs from Sold (from the start values if Sold is
+// null).  Not exact (S given): S receives an approximation (segments added
+// from approximate starts), the guess of the first fixed-point iterations.
 int drag_scan(gx_solver* s, const double* D, long long ne, int C, const double* start, const double* Sold, double* S,
-              double* end, const DragBufs& b) {
+              double* end, const DragBufs& b, bool exact) {
     const int nseg = (int)((ne + kDragSeg - 1) / kDragSeg);
     hipStream_t st = s->stream;
     const unsigned wgrid = blocks((long long)C * nseg * 64, 256);
-    hipLaunchKernelGGL(k_scan_segsum, dim3(wgrid), dim3(256), 0, st, D, ne, C, nseg, (double*)s->dragSig.p);
+    const unsigned rgrid = blocks((long long)C * nseg * kDragRows, 256);
+    double* specStart = (double*)s->dragSpec.p;
+    double* specEnd = specStart + (long long)C * nseg;
+    int* specOk = (int*)(specEnd + (long long)C * nseg);
+    double* rowSig = (double*)s->dragSig.p + (long long)C * nseg;
+    hipLaunchKernelGGL(k_scan_segsum, dim3(wgrid), dim3(256), 0, st, D, ne, C, nseg, (double*)s->dragSig.p, rowSig);
     hipLaunchKernelGGL(k_scan_guess, dim3((unsigned)C), dim3(1024), 0, st, (const double*)s->dragSig.p, nseg, start,
                        (double*)s->dragGuess.p);
+    if (!exact) {
+        hipLaunchKernelGGL(k_scan_spec, dim3(rgrid), dim3(256), 0, st, D, ne, C, nseg, start,
+                           (const double*)s->dragGuess.p, (const double*)rowSig, (const double*)nullptr, S,
+                           (double*)nullptr, (double*)nullptr, (int*)nullptr);
+        HIPCHK(hipGetLastError());
+        return GX_OK;
+    }
     hipLaunchKernelGGL(k_scan_summary, dim3(wgrid), dim3(256), 0, st, D, ne, C, nseg, (const double*)s->dragGuess.p,
                        (DragSeg*)s->dragSeg.p);
+    const bool spec = S && Sold;  // speculate from the previous guess
+    if (spec)
+        hipLaunchKernelGGL(k_scan_spec, dim3(rgrid), dim3(256), 0, st, D, ne, C, nseg, start,
+                           (const double*)s->dragGuess.p, (const double*)rowSig, Sold, S, specStart, specEnd, specOk);
     hipLaunchKernelGGL(k_scan_chain, dim3((unsigned)C), dim3(256), 0, st, D, ne, nseg, (const DragSeg*)s->dragSeg.p,
-                       start, S, (double*)s->dragSegStart.p, (int*)s->dragSegMode.p, end, b.slow);
+                       start, spec ? (const double*)specStart : nullptr, (const double*)specEnd,
+                       (const int*)specOk, S, (double*)s->dragSegStart.p, (int*)s->dragSegMode.p, end, b.slow);
     if (S)
         hipLaunchKernelGGL(k_scan_fill, dim3(wgrid), dim3(256), 0, st, D, ne, C, nseg, (const DragSeg*)s->dragSeg.p,
                            (const double*)s->dragSegStart.p, (const int*)s->dragSegMode.p, start, Sold, S,
@@ -802,9 +821,11 @@ int drag_pass(gx_solver* s, const DragArgs& a, const int32_t* list, long long ne
     int rc;
     if (s->dragIters > 0 && chain <= ((size_t)8 << 30) && C * nseg < (1LL << 30)) {
         if ((rc = ensure(s->dragD, chain)) || (rc = ensure(s->dragS0, chain)) || (rc = ensure(s->dragS1, chain)) ||
-            (rc = ensure(s->dragSig, (size_t)C * nseg * 8)) || (rc = ensure(s->dragGuess, (size_t)C * nseg * 8)) ||
+            (rc = ensure(s->dragSig, (size_t)C * nseg * 8 * (1 + kDragRows))) ||
+            (rc = ensure(s->dragGuess, (size_t)C * nseg * 8)) ||
             (rc = ensure(s->dragSeg, (size_t)C * nseg * sizeof(DragSeg))) ||
-            (rc = ensure(s->dragSegStart, (size_t)C * nseg * 8)) || (rc = ensure(s->dragSegMode, (size_t)C * nseg * 4)))
+            (rc = ensure(s->dragSegStart, (size_t)C * nseg * 8)) || (rc = ensure(s->dragSegMode, (size_t)C * nseg * 4)) ||
+            (rc = ensure(s->dragSpec, (size_t)C * nseg * 20)))
             return rc;
         const unsigned grid = (unsigned)std::min<long long>(blocks(ne, 256), 4096);
         double* D = (double*)s->dragD.p;
@@ -813,17 +834,23 @@ int drag_pass(gx_solver* s, const DragArgs& a, const int32_t* list, long long ne
         double* Snew = (double*)s->dragS0.p;
         bool settled = false;
         int it = 0;
+        // the first kDragApprox iterations only improve the guess (approximate
+        // scans); the exact ones check for the fixed point
+        constexpr int kDragApprox = 2;
         while (it < s->dragIters && !settled) {
             it++;
-            hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, b.changed, 2, 0ULL);  // changed, slow
+            const bool exact = it > kDragApprox;
+            if (exact) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, b.changed, 2, 0ULL);  // changed, slow
             hipLaunchKernelGGL(k_drag_eval, dim3(grid), dim3(256), 0, st, a, list, ne, vx, vy, vz, (const double*)jx,
                                (const double*)jy, (const double*)jz, Sold, 0, (const int*)b.on, D);
-            if ((rc = drag_scan(s, D, ne, (int)C, b.startJ, Sold, Snew, b.endJ, b))) return rc;
-            HIPCHK(hipMemcpyAsync(&s->hhist[8], b.changed, 16, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            settled = s->hhist[8] == 0;
-            s->dragStats.segments += C * nseg;
-            s->dragStats.slow_segments += (int64_t)s->hhist[9];
+            if ((rc = drag_scan(s, D, ne, (int)C, b.startJ, Sold, Snew, b.endJ, b, exact))) return rc;
+            if (exact) {
+                HIPCHK(hipMemcpyAsync(&s->hhist[8], b.changed, 16, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                settled = s->hhist[8] == 0;
+                s->dragStats.segments += C * nseg;
+                s->dragStats.slow_segments += (int64_t)s->hhist[9];
+            }
             if (!settled) {
                 Sold = Snew;
                 Snew = Snew == (double*)s->dragS0.p ? (double*)s->dragS1.p : (double*)s->dragS0.p;
@@ -838,7 +865,7 @@ int drag_pass(gx_solver* s, const DragArgs& a, const int32_t* list, long long ne
             hipLaunchKernelGGL(k_drag_eval, dim3(grid), dim3(256), 0, st, a, list, ne, vx, vy, vz, (const double*)jx,
                                (const double*)jy, (const double*)jz, Sold, 1, (const int*)b.on, D);
             hipLaunchKernelGGL(k_drag_chain_io, dim3(1), dim3(64), 0, st, a.holes, a.nh, vx, vy, vz, b.startV, 0);
-            if ((rc = drag_scan(s, D, ne, (int)C, b.startV, nullptr, nullptr, b.endV, b))) return rc;
+            if ((rc = drag_scan(s, D, ne, (int)C, b.startV, nullptr, nullptr, b.endV, b, true))) return rc;
             hipLaunchKernelGGL(k_drag_chain_io, dim3(1), dim3(64), 0, st, a.holes, a.nh, vx, vy, vz, b.endV, 1);
             hipLaunchKernelGGL(k_drag_chain_io, dim3(1), dim3(64), 0, st, a.holes, a.nh, jx, jy, jz, b.endJ, 1);
             HIPCHK(hipGetLastError());

@@ -13,33 +13,40 @@
 //      previous iterate), giving its spin increments dJ_j, one per hole and
 //      component (-0.0 where the body gives nothing: x + -0.0 == x for every
 //      x, signed zeros included);
-//   2. the spins the bodies see are the exclusive prefix sums of the dJ in
-//      list order, rounded as the sequential loop rounds them (the exact
-//      ordered scan below);
-//   3. if these equal the guess bit for bit, the guess is a fixed point of the
-//      sequential recursion S_{j+1} = S_j + dJ(S_j), S_0 = the start spins,
-//      whose solution is unique, so it is the reference's chain; otherwise
-//      the prefix sums become the next guess.  Iterate k is exact at least on
-//      positions 0..k; a kick changes a spin by a tiny relative amount, so in
-//      practice the second or third iterate is the fixed point.  Past an
-//      iteration limit the pass runs as one sequential thread.
+//   2. the next guess is the exclusive prefix sums of the dJ in list order:
+//      approximate in the first two iterations, then rounded exactly as the
+//      sequential loop rounds them (the exact ordered scan below);
+//   3. if an exact scan equals the guess bit for bit, the guess is a fixed
+//      point of the sequential recursion S_{j+1} = S_j + dJ(S_j), S_0 = the
+//      start spins, whose solution is unique, so it is the reference's chain.
+//      Each exact iteration is exact at least one position further than the
+//      one before; a kick changes a spin by a tiny relative amount, so in
+//      practice the first or second exact iteration is the fixed point.  Past
+//      an iteration limit the pass runs as one sequential thread.
 // The holes' velocities are sums of the same kind: once the spins are exact,
 // one more ordered scan of their increments.  A body's own velocity depends
 // only on the spins it sees.
 //
 // Exact ordered scan.  A chain s_{t+1} = fl(s_t + d_t) is cut into segments of
-// kDragSeg elements.  While s_t lies strictly inside one binade, |s_t| in
-// (2^E, 2^(E+1)), a step is s_{t+1} = s_t + u rint(d_t / u), u = 2^(E-52),
-// whenever d_t / u is not a half-integer: the exact sum lies within u/2 of
-// that grid point, which is inside the binade, and the grid there is u.  So
-// a segment whose partial sums all stay inside the binade of its start value
-// is an integer sum, which is associative: its total and its partial minimum
-// and maximum are computed in parallel (one wavefront per segment), and one
-// thread per chain walks the segments in order from the exact start value,
-// checking the binade condition from the segment's partial extremes.  A
+// kDragSeg elements.  Every segment is first added element by element, all at
+// once, from a guess of its start (the previous iterate's value there); a
+// walk over the segments in order takes a segment from there when the guess
+// is its exact start bit for bit, which in the iteration that confirms the
+// fixed point is every segment.  Otherwise: while s_t lies strictly inside
+// one binade, |s_t| in (2^E, 2^(E+1)), a step is s_{t+1} = s_t + u rint(d_t /
+// u), u = 2^(E-52), whenever d_t / u is not a half-integer: the exact sum
+// lies within u/2 of that grid point, which is inside the binade, and the
+// grid there is u.  So a segment whose partial sums all stay inside the
+// binade of its start value is an integer sum, which is associative: its
+// total and its partial minimum and maximum are computed in parallel (one
+// wavefront per segment), and the walk (one thread per chain, from the exact
+// start value) checks the binade condition from those extremes.  A
 // segment that fails it (a binade crossing, a zero or subnormal value, a
-// tie, a wrong binade guess) is added element by element, as the reference
-// adds.  The binade of a segment is guessed from an approximate prefix sum.
+// tie, a wrong binade guess) is redone by one wavefront in blocks of 64 from
+// its exact start value, with the same integer rule and the binade read from
+// that value; only a block that leaves its binade (or meets a tie) is added
+// element by element, as the reference adds.  The binade of a segment is
+// guessed from an approximate prefix sum.
 //
 // Compiled with -ffp-contract=off: the kick repeats framedrag.go's
 // operations in its order.  One deliberate difference: dragBack's weights
@@ -179,13 +186,27 @@ __device__ void drag_kick_live(long long i, const DragArgs& a, double* vx, doubl
     }
 }
 
+// A 64-bit value moved across lanes by one DPP control (both halves); lanes
+// with no source, or in rows outside RowMask, receive 0.
+template <int Ctrl, int RowMask, bool ZeroOutside>
+__device__ __forceinline__ long long dpp64(long long v) {
+    const int lo = (int)(unsigned)(unsigned long long)v, hi = (int)((unsigned long long)v >> 32);
+    const unsigned l = (unsigned)__builtin_amdgcn_update_dpp(0, lo, Ctrl, RowMask, 0xf, ZeroOutside);
+    const unsigned h = (unsigned)__builtin_amdgcn_update_dpp(0, hi, Ctrl, RowMask, 0xf, ZeroOutside);
+    return (long long)(((unsigned long long)h << 32) | l);
+}
+
+// Inclusive prefix sum over the 64 lanes of a wavefront (all lanes active):
+// row_shr 1, 2, 4, 8 within rows of 16, then the last lane of row 0 into row
+// 1 and of row 2 into row 3 (row_bcast:15), and of row 1 into rows 2 and 3
+// (row_bcast:31).  VALU only, no LDS.
 __device__ __forceinline__ long long wave_incl_scan(long long v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const long long o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
+    v += dpp64<0x111, 0xf, true>(v);
+    v += dpp64<0x112, 0xf, true>(v);
+    v += dpp64<0x114, 0xf, true>(v);
+    v += dpp64<0x118, 0xf, true>(v);
+    v += dpp64<0x142, 0xa, false>(v);
+    v += dpp64<0x143, 0xc, false>(v);
     return v;
 }
 
@@ -205,6 +226,38 @@ __device__ __forceinline__ long long seg_step(double d, int e, bool& big, bool& 
 }
 
 __device__ __forceinline__ int dexp(double v) { return (int)((dbits(v) >> 52) & 0x7ff); }
+
+// One block of up to 64 elements in[0 .. len) of a chain, by one wavefront,
+// from the exact value v (wave-uniform; lane l holds d = in[l], -0.0 past
+// len): as an integer prefix sum in the binade of v when every partial value
+// stays strictly inside it (the exactness argument of the header, with the
+// binade read from v itself), else element by element in lane 0.  out[l]
+// receives the value before element l; returns the value after the block.
+__device__ double chain_block(double v, double d, int len, const double* in, double* out) {
+    const int lane = threadIdx.x & 63;
+    const long long two52 = 1LL << 52, two53 = 1LL << 53;
+    const int ev = dexp(v);
+    if (ev != 0 && ev != 0x7ff) {
+        const int e = ev - 1023;
+        const long long M = (long long)ldexp(v, 52 - e);  // exact: |M| in [2^52, 2^53)
+        bool big, tie;
+        const long long n = seg_step(d, e, big, tie);
+        const long long P = M + wave_incl_scan(n);  // |P| < 2^53 + 64 * 2^53: no overflow
+        const bool in = M > 0 ? (P > two52 && P < two53) : (P > -two53 && P < -two52);
+        const bool mok = M > 0 ? M > two52 : M < -two52;
+        if (mok && !__any(big || tie || !in)) {
+            if (lane < len) out[lane] = ldexp((double)(P - n), e - 52);
+            return ldexp((double)__shfl(P, 63, 64), e - 52);
+        }
+    }
+    double r = v;
+    if (lane == 0)
+        for (int t = 0; t < len; t++) {
+            out[t] = r;
+            r = r + in[t];
+        }
+    return __shfl(r, 0, 64);
+}
 
 }  // namespace
 
@@ -313,19 +366,24 @@ __global__ void k_drag_seq(DragArgs a, const int32_t* __restrict__ list, long lo
     for (long long j = 0; j < ne; j++) drag_kick_live(list[j], a, vx, vy, vz, jx, jy, jz);
 }
 
-// Exact ordered scan, step A: an approximate sum of every segment (one
-// wavefront per segment; chain c, segment g).
+// Exact ordered scan, step A: approximate sums of every segment and of its
+// rows of kDragRow elements (one wavefront per segment; chain c, segment g).
 __global__ __launch_bounds__(256) void k_scan_segsum(const double* __restrict__ D, long long ne, int chains, int nseg,
-                                                     double* __restrict__ sig) {
+                                                     double* __restrict__ sig, double* __restrict__ rowSig) {
     const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (w >= (long long)chains * nseg) return;
     const int c = (int)(w / nseg), g = (int)(w % nseg);
     const long long lo = (long long)g * kDragSeg, hi = min(ne, lo + kDragSeg);
     double v = 0;
-    for (long long t = lo + lane; t < hi; t += 64) v += D[c * ne + t];
+    for (int r = 0; r < kDragRows; r++) {
+        const long long t = lo + (long long)r * kDragRow + lane;
+        double x = t < hi ? D[c * ne + t] : 0.0;
 #pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+        for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+        if (lane == 0) rowSig[w * kDragRows + r] = x;
+        v += x;
+    }
     if (lane == 0) sig[w] = v;
 }
 
@@ -402,20 +460,97 @@ __global__ __launch_bounds__(256) void k_scan_summary(const double* __restrict__
     }
 }
 
+// Step S: every row of kDragRow elements added element by element from a
+// guessed start value, all rows at once (one thread per row): the chain's
+// start for the first row; else, with Sold given, Sold's value at the row's
+// first position, or, without, the segment's approximate start guess[] plus
+// the approximate sums of the rows before it.  S receives the values.  With
+// Sold given, specStart/specEnd receive each segment's start and end and
+// specOk whether its rows chain (every row ends on the next row's guessed
+// start bit for bit): then the segment is exact whenever its start is, which
+// step D checks.  Without Sold this is the approximate scan of the first
+// fixed-point iterations.
+__global__ __launch_bounds__(256) void k_scan_spec(const double* __restrict__ D, long long ne, int chains, int nseg,
+                                                   const double* __restrict__ start, const double* __restrict__ guess,
+                                                   const double* __restrict__ rowSig, const double* __restrict__ Sold,
+                                                   double* __restrict__ S, double* __restrict__ specStart,
+                                                   double* __restrict__ specEnd, int* __restrict__ specOk) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const long long rowsPerChain = (long long)nseg * kDragRows;
+    bool ok = true;
+    if (t < chains * rowsPerChain) {
+        const int c = (int)(t / rowsPerChain);
+        const long long rem = t % rowsPerChain;
+        const int g = (int)(rem / kDragRows), r = (int)(rem % kDragRows);
+        const long long w = (long long)c * nseg + g;
+        const long long segLo = (long long)g * kDragSeg, segHi = min(ne, segLo + kDragSeg);
+        const long long lo = segLo + (long long)r * kDragRow, hi = min(segHi, lo + kDragRow);
+        const int lastRow = (int)((segHi - segLo - 1) / kDragRow);
+        if (r <= lastRow) {
+            double v;
+            if (g == 0 && r == 0) {
+                v = start[c];
+            } else if (Sold) {
+                v = Sold[c * ne + lo];
+            } else {
+                v = guess[w];
+                for (int q = 0; q < r; q++) v += rowSig[w * kDragRows + q];
+            }
+            if (Sold && r == 0) specStart[w] = v;
+            const double* __restrict__ d = D + c * ne;
+            double* __restrict__ out = S + c * ne;
+            constexpr int kB = 16;  // loads issued ahead of the dependent adds
+            long long e0 = lo;
+            for (; e0 + kB <= hi; e0 += kB) {
+                double x[kB];
+#pragma unroll
+                for (int k = 0; k < kB; k++) x[k] = d[e0 + k];
+#pragma unroll
+                for (int k = 0; k < kB; k++) {
+                    out[e0 + k] = v;
+                    v = v + x[k];
+                }
+            }
+            for (long long e = e0; e < hi; e++) {
+                out[e] = v;
+                v = v + d[e];
+            }
+            if (Sold) {
+                if (r < lastRow)
+                    ok = dbits(v) == dbits(Sold[c * ne + hi]);
+                else
+                    specEnd[w] = v;
+            }
+        }
+    }
+    // the rows of a segment are 16 consecutive lanes, aligned to 16
+    const unsigned long long bad = __ballot(!ok);
+    if (Sold && (lane & (kDragRows - 1)) == 0 && t < chains * rowsPerChain)
+        specOk[t / kDragRows] = ((bad >> (lane & ~(kDragRows - 1))) & ((1ULL << kDragRows) - 1)) == 0;
+}
+
 // Step D: the exact chain over the segments, one workgroup per chain.
-// Thread 0 carries the exact value; a segment whose integer summary is not
-// valid from that value is added element by element (its elements staged in
-// LDS by the workgroup), writing the values the bodies see when S is given.
-// segStart/segMode receive each segment's exact start value and how it was
-// done (0 integer, 1 unchanged, 2 element by element); end[c] the chain's
-// final value; counters[0] += segments done element by element.
+// Thread 0 carries the exact value.  A segment is taken from step S when its
+// rows chain and its guessed start equals that value bit for bit, else from
+// its integer summary
+// when that is valid from the value, else it is staged in LDS and done by
+// wavefront 0 in blocks of 64 (chain_block: most blocks stay in one binade
+// even where the segment does not), writing the values the bodies see when S
+// is given.  segStart/segMode receive each segment's exact start value and
+// how it was done (0 integer, 1 unchanged, 2 in blocks, 3 from step S);
+// end[c] the chain's final value; counters[0] += segments done in blocks.
 __global__ __launch_bounds__(256) void k_scan_chain(const double* __restrict__ D, long long ne, int nseg,
                                                     const DragSeg* __restrict__ seg, const double* __restrict__ start,
-                                                    double* __restrict__ S, double* __restrict__ segStart,
-                                                    int* __restrict__ segMode, double* __restrict__ end,
-                                                    unsigned long long* __restrict__ counters) {
+                                                    const double* __restrict__ specStart,
+                                                    const double* __restrict__ specEnd, const int* __restrict__ specOk,
+                                                    double* __restrict__ S,
+                                                    double* __restrict__ segStart, int* __restrict__ segMode,
+                                                    double* __restrict__ end, unsigned long long* __restrict__ counters) {
     __shared__ DragSeg tile[256];
-    __shared__ double buf[kDragSeg];
+    __shared__ double tss[256], tse[256];
+    __shared__ int tok[256];
+    __shared__ double buf[kDragSeg], sbuf[kDragSeg];
     __shared__ int slowAt;
     __shared__ double carried;
     const int c = blockIdx.x, tid = threadIdx.x;
@@ -424,7 +559,13 @@ __global__ __launch_bounds__(256) void k_scan_chain(const double* __restrict__ D
     unsigned long long slowCount = 0;
     for (int base = 0; base < nseg; base += 256) {
         const int cnt = min(256, nseg - base);
-        if (tid < cnt) tile[tid] = seg[(long long)c * nseg + base + tid];
+        if (tid < cnt) {
+            const long long gi = (long long)c * nseg + base + tid;
+            tile[tid] = seg[gi];
+            tss[tid] = specStart ? specStart[gi] : 0.0;
+            tse[tid] = specStart ? specEnd[gi] : 0.0;
+            tok[tid] = specStart ? specOk[gi] : 0;
+        }
         __syncthreads();
         int pos = 0;
         while (pos < cnt) {
@@ -434,6 +575,11 @@ __global__ __launch_bounds__(256) void k_scan_chain(const double* __restrict__ D
                     const DragSeg ss = tile[q];
                     const long long gi = (long long)c * nseg + base + q;
                     segStart[gi] = s;
+                    if (tok[q] && dbits(tss[q]) == dbits(s)) {
+                        segMode[gi] = 3;
+                        s = tse[q];
+                        continue;
+                    }
                     if (ss.flags & kSegAllNegZero) {
                         segMode[gi] = 1;
                         continue;
@@ -465,19 +611,21 @@ __global__ __launch_bounds__(256) void k_scan_chain(const double* __restrict__ D
                 const int len = (int)min((long long)kDragSeg, ne - lo);
                 for (int t = tid; t < len; t += blockDim.x) buf[t] = D[c * ne + lo + t];
                 __syncthreads();
-                if (tid == 0) {
+                if (tid < 64) {
                     double v = carried;
-                    if (S)
-                        for (int t = 0; t < len; t++) {
-                            S[c * ne + lo + t] = v;
-                            v = v + buf[t];
-                        }
-                    else
-                        for (int t = 0; t < len; t++) v = v + buf[t];
-                    s = v;
-                    slowCount++;
+                    for (int b0 = 0; b0 < len; b0 += 64) {
+                        const int bl = min(64, len - b0);
+                        const double d = tid < bl ? buf[b0 + tid] : __longlong_as_double((long long)kNegZero);
+                        v = chain_block(v, d, bl, buf + b0, sbuf + b0);
+                    }
+                    if (tid == 0) {
+                        s = v;
+                        slowCount++;
+                    }
                 }
                 __syncthreads();
+                if (S)
+                    for (int t = tid; t < len; t += blockDim.x) S[c * ne + lo + t] = sbuf[t];
                 pos = q + 1;
             } else {
                 pos = cnt;
@@ -492,9 +640,9 @@ __global__ __launch_bounds__(256) void k_scan_chain(const double* __restrict__ D
 }
 
 // Step E: the values the bodies see, S[c][t] (exclusive prefix), for the
-// integer and unchanged segments (the element-by-element ones were written by
-// step D), and the comparison with the previous guess Sold (null: the start
-// values): *changed = 1 if any differs bit for bit.
+// integer and unchanged segments (the others were written by steps S and D),
+// and the comparison with the previous guess Sold (null: the start values):
+// *changed = 1 if any differs bit for bit.
 __global__ __launch_bounds__(256) void k_scan_fill(const double* __restrict__ D, long long ne, int chains, int nseg,
                                                    const DragSeg* __restrict__ seg,
                                                    const double* __restrict__ segStart,
@@ -528,7 +676,7 @@ __global__ __launch_bounds__(256) void k_scan_fill(const double* __restrict__ D,
             v = t < hi ? S[c * ne + t] : 0.0;
         }
         if (t < hi) {
-            if (mode != 2) S[c * ne + t] = v;
+            if (mode < 2) S[c * ne + t] = v;
             const double o = Sold ? Sold[c * ne + t] : start[c];
             diff = diff || dbits(o) != dbits(v);
         }

@@ -85,6 +85,8 @@ struct DragSeg {
     int e, flags;                 // binade exponent guessed for the segment; kSeg* flags
 };
 constexpr int kDragSeg = 1024;  // elements per segment
+constexpr int kDragRow = 64;    // elements per speculated row
+constexpr int kDragRows = kDragSeg / kDragRow;
 constexpr int kSegNoGuess = 1, kSegBig = 2, kSegTie = 4, kSegAllNegZero = 8;
 __global__ void k_drag_spinning(const int32_t* holes, const Meta* meta, const double* jx, const double* jy,
                                 const double* jz, double c, int* on);
@@ -97,12 +99,16 @@ __global__ void k_drag_eval(DragArgs a, const int32_t* list, long long ne, doubl
                             const int* on, double* D);
 __global__ void k_drag_seq(DragArgs a, const int32_t* list, long long ne, double* vx, double* vy, double* vz,
                            double* jx, double* jy, double* jz, const int* on);
-__global__ void k_scan_segsum(const double* D, long long ne, int chains, int nseg, double* sig);
+__global__ void k_scan_segsum(const double* D, long long ne, int chains, int nseg, double* sig, double* rowSig);
 __global__ void k_scan_guess(const double* sig, int nseg, const double* start, double* guess);
 __global__ void k_scan_summary(const double* D, long long ne, int chains, int nseg, const double* guess,
                                DragSeg* seg);
+__global__ void k_scan_spec(const double* D, long long ne, int chains, int nseg, const double* start,
+                            const double* guess, const double* rowSig, const double* Sold, double* S,
+                            double* specStart, double* specEnd, int* specOk);
 __global__ void k_scan_chain(const double* D, long long ne, int nseg, const DragSeg* seg, const double* start,
-                             double* S, double* segStart, int* segMode, double* end, unsigned long long* counters);
+                             const double* specStart, const double* specEnd, const int* specOk, double* S,
+                             double* segStart, int* segMode, double* end, unsigned long long* counters);
 __global__ void k_scan_fill(const double* D, long long ne, int chains, int nseg, const DragSeg* seg,
                             const double* segStart, const int* segMode, const double* start, const double* Sold,
                             double* S, int* changed);
