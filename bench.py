@@ -17,7 +17,9 @@ Other workloads (--config): plummer-16k (force + one KDK step, FP64 and FP32),
 clumps-16m (precision ladder FP64 / FP32 / FP32 with FP16-stored quadrupoles),
 batched-256 (256 systems in one arena, hipGraph KDK vs separate calls) and
 two-disk-4m (4,194,304-body encounter, block time steps on 4 rungs, FP16
-quadrupoles, FP64 energy drift over --steps global steps).
+quadrupoles, FP64 energy drift over --steps global steps), disk-halo-1m-holes
+(split steps with three black holes), sph-sphere-1m (SPH hydro forces) and
+spin-cluster-1m (split steps with a spinning hole and frame dragging).
 """
 from __future__ import annotations
 
@@ -46,7 +48,7 @@ FLOPS_SPH_PAIR = 62
 FLOPS_SPH_VISC = 19
 FLOPS_SPH_DENSITY = 20
 CONFIGS = ("disk-halo-1m", "plummer-16k", "clumps-16m", "batched-256", "two-disk-4m", "disk-halo-1m-holes",
-           "sph-sphere-1m")
+           "sph-sphere-1m", "spin-cluster-1m")
 
 
 def parse():
@@ -620,6 +622,105 @@ def bench_sph(args):
     return res
 
 
+def bench_spin(args):
+    """The star cluster and spinning hole of the reference's accretion scene
+    (internal/scene/scenes/11-accretion.json, without its gas cloud): a
+    Plummer cluster of 1,048,576 stars (mass 1, scale 1.5), a central hole of
+    mass 0.1 spinning at a* = 0.5 about z (J = a* M^2 / c, scene/spec.go:
+    241-251), c = 3, softening 0.03, dt 0.002.  One step = the split step
+    (split.go:164-199) with frame dragging (framedrag.go): dragOuter over the
+    far stars after the first and before the last outer kick, dragFast around
+    the drifts of every substep; FP32 tree walk between stars, FP64 for every
+    pair with the hole and for the dragging.  Also timed: the same steps
+    without the spin, one dragOuter pass over all stars (bit-identical to the
+    sequential loop) and, on the CPU, the oracle's pass."""
+    from gravitas_amd import DeviceState, Params, Solver, ics
+    from gravitas_amd.hip import synchronize
+    n = args.n or 1048576
+    theta = 0.5 if args.theta is None else args.theta
+    eps = 0.03 if args.eps is None else args.eps
+    dt, c, hole_m, a_star = 0.002, 3.0, 0.1, 0.5
+    log(f"building spinning-hole cluster n={n}")
+    state = ics.plummer(n, 7, mass=1.0, a=1.5)
+    state["alive"] = np.ones(n, np.uint8)
+    state["kind"][0] = ics.KIND_HOLE
+    state["m"][0] = hole_m
+    for k in ("px", "py", "pz", "vx", "vy", "vz"):
+        state[k][0] = 0.0
+    for k in ("jx", "jy", "jz"):
+        state[k] = np.zeros(n)
+    state["jz"][0] = a_star * hole_m * hole_m / c
+
+    def run(st, steps, warmup):
+        ds = DeviceState.from_host(st)
+        solver = Solver(theta=theta, walk="fp32")
+        params = Params(softening=eps, dt=dt, c=c)
+        subs = [solver.split_step(ds, params, acc_valid=False)]
+        for _ in range(max(0, warmup - 1)):
+            subs.append(solver.split_step(ds, params))
+        synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            subs.append(solver.split_step(ds, params))
+        synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps, subs[-steps:], solver, ds
+
+    log(f"timing {args.steps} split steps with frame dragging")
+    ms, subs, solver, ds = run(state, args.steps, args.warmup)
+    drag = solver.drag_stats()
+    nospin = {k: v for k, v in state.items() if k not in ("jx", "jy", "jz")}
+    ms0, subs0, _, _ = run(nospin, args.steps, args.warmup)
+    out = base_line(args, n, ms, "fp64+fp32", {
+        "workload": "spin-cluster-1m", "n": n, "theta": theta, "eps": eps, "dt": dt, "c": c,
+        "model": "Plummer cluster (M 1, a 1.5) with a central hole of 0.1 spinning at a* = 0.5; "
+                 "the accretion scene's stars and hole, without the gas",
+        "step": "split step with frame dragging: tree forces (FP32 walk) + hole direct sums + substeps + "
+                "dragOuter/dragFast (FP64)"})
+    out["metric"] = "particles/s (split step with a spinning hole and frame dragging, c = 3)"
+    out["substeps"] = subs
+    out["ms_per_step_without_spin"] = ms0
+    out["substeps_without_spin"] = subs0
+    steps_run = args.steps + max(1, args.warmup)
+    out["drag_per_step"] = {"passes": drag["passes"] / steps_run, "iterations": drag["iterations"] / steps_run,
+                            "sequential_passes": drag["sequential_passes"],
+                            "slow_segments": drag["slow_segments"] / steps_run}
+    # one dragOuter pass over every star, from the state the steps left
+    hs = solver.host_state(ds)
+    for k in ("m", "kind", "alive"):
+        hs[k] = state[k]
+    p = Params(softening=eps, c=c)
+    sv = Solver(walk="fp64")
+    dsp = DeviceState.from_host(hs)
+    reps = 10
+    sv.drag_outer(dsp, p, 0.5 * dt)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sv.drag_outer(dsp, p, 0.5 * dt)
+    synchronize()
+    pass_ms = (time.perf_counter() - t0) * 1e3 / reps
+    out["drag_outer_pass"] = {"bodies": n - 1, "ms": pass_ms, "passes_per_s": 1e3 / pass_ms,
+                              "stats": sv.drag_stats()}
+    if not args.no_cpu:
+        import oracle
+        keys = ("px", "py", "pz", "vx", "vy", "vz", "jx", "jy", "jz")
+        one = DeviceState.from_host(hs)
+        sv.drag_outer(one, p, 0.5 * dt)
+        g = sv.host_state(one)
+        w = {k: hs[k].copy() for k in keys}
+        log("CPU baseline: one oracle dragOuter pass")
+        t0 = time.perf_counter()
+        oracle.drag_outer(w, hs["m"], hs["kind"], hs["alive"], 0.5 * dt, eps * eps, c)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": 1.0 / tc, "unit": "dragOuter passes/s", "cores": 1, "kind": "port",
+            "sample": f"one dragOuter pass over the {n - 1} stars (the whole workload), C restatement of "
+                      "framedrag.go's sequential loop, -O2 -ffp-contract=off, 1 thread"}
+        out["drag_gpu_over_cpu"] = (1e3 / pass_ms) * tc
+        out["drag_bit_identical_to_oracle"] = bool(all(np.array_equal(g[k], w[k]) for k in keys[3:]))
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -633,7 +734,7 @@ def main():
         sys.exit(1)
     fn = {"disk-halo-1m": bench_disk_halo, "plummer-16k": bench_plummer, "clumps-16m": bench_clumps,
           "batched-256": bench_batched, "two-disk-4m": bench_encounter,
-          "disk-halo-1m-holes": bench_holes, "sph-sphere-1m": bench_sph}[args.config]
+          "disk-halo-1m-holes": bench_holes, "sph-sphere-1m": bench_sph, "spin-cluster-1m": bench_spin}[args.config]
     print(json.dumps(fn(args)), flush=True)
 
 

@@ -99,7 +99,7 @@ struct gx_solver {
     DevBuf nearFlag, nearList, tileCnt, splitScalars, farF, fastF;
     // frame dragging: list, chains (increments, two guesses), segment summaries, scalars
     DevBuf dragFlag, dragList, dragD, dragS0, dragS1, dragSig, dragGuess, dragSeg, dragSegStart, dragSegMode,
-        dragSpec, dragScal;
+        dragSpec, dragScal, dragCount;
     int dragIters = 16;  // GX_DRAG_ITERATIONS: fixed-point iterations before the sequential pass (0: always)
     gx_drag_stats dragStats = {};
     unsigned long long* hhist = nullptr;  // pinned mirror of hist
@@ -233,7 +233,7 @@ void gx_solver_free(gx_solver* s) {
                       &s->rungHist, &s->diag,     &s->nearFlag, &s->nearList, &s->tileCnt,  &s->splitScalars,
                       &s->farF,     &s->fastF,    &s->dragFlag, &s->dragList, &s->dragD,    &s->dragS0,
                       &s->dragS1,   &s->dragSig,  &s->dragGuess, &s->dragSeg, &s->dragSegStart, &s->dragSegMode,
-                      &s->dragSpec, &s->dragScal};
+                      &s->dragSpec, &s->dragScal, &s->dragCount};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto& e : s->ev)
@@ -751,6 +751,10 @@ int drag_bufs(gx_solver* s, long long nh, DragBufs& b) {
     const long long C = 3 * std::max(nh, 1LL);
     int rc = ensure(s->dragScal, (size_t)C * 32 + 32);
     if (rc) return rc;
+    if (!s->dragCount.p) {  // iterations and sequential passes of k_drag_small
+        if ((rc = ensure(s->dragCount, 16))) return rc;
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, (unsigned long long*)s->dragCount.p, 2, 0ULL);
+    }
     unsigned long long* u = (unsigned long long*)s->dragScal.p;
     b.on = (int*)u;
     b.changed = u + 1;
@@ -811,14 +815,27 @@ int drag_scan(gx_solver* s, const double* D, long long ne, int C, const double* 
 // sequential thread when it has not settled within dragIters iterations (or
 // the chains would not fit).  *b.on must hold spinning() for the pass.
 int drag_pass(gx_solver* s, const DragArgs& a, const int32_t* list, long long ne, double* vx, double* vy, double* vz,
-              double* jx, double* jy, double* jz, const DragBufs& b) {
-    if (ne <= 0 || a.nh <= 0) return GX_OK;
+              double* jx, double* jy, double* jz, const DragBufs& b, bool holesToo = false) {
+    if (a.nh <= 0) return GX_OK;
     hipStream_t st = s->stream;
-    s->dragStats.passes++;
     const long long C = 3LL * a.nh;
     const long long nseg = (ne + kDragSeg - 1) / kDragSeg;
     const size_t chain = (size_t)C * (size_t)ne * 8;
     int rc;
+    if (ne <= kDragSmall && C <= 1024) {  // one workgroup, no host synchronisation
+        if (ne > 0) s->dragStats.passes++;
+        if ((rc = ensure(s->dragD, std::max<size_t>(chain, 8))) || (rc = ensure(s->dragS0, std::max<size_t>(chain, 8))) ||
+            (rc = ensure(s->dragS1, std::max<size_t>(chain, 8))) || (rc = ensure(s->dragSpec, (size_t)C * 16)))
+            return rc;
+        hipLaunchKernelGGL(k_drag_small, dim3(1), dim3(1024), 0, st, a, list, ne, vx, vy, vz, jx, jy, jz,
+                           (const int*)b.on, (double*)s->dragD.p, (double*)s->dragS0.p, (double*)s->dragS1.p,
+                           (double*)s->dragSpec.p, s->dragIters, holesToo ? 1 : 0,
+                           (unsigned long long*)s->dragCount.p);
+        HIPCHK(hipGetLastError());
+        return GX_OK;
+    }
+    if (ne <= 0) return GX_OK;
+    s->dragStats.passes++;
     if (s->dragIters > 0 && chain <= ((size_t)8 << 30) && C * nseg < (1LL << 30)) {
         if ((rc = ensure(s->dragD, chain)) || (rc = ensure(s->dragS0, chain)) || (rc = ensure(s->dragS1, chain)) ||
             (rc = ensure(s->dragSig, (size_t)C * nseg * 8 * (1 + kDragRows))) ||
@@ -868,12 +885,18 @@ int drag_pass(gx_solver* s, const DragArgs& a, const int32_t* list, long long ne
             if ((rc = drag_scan(s, D, ne, (int)C, b.startV, nullptr, nullptr, b.endV, b, true))) return rc;
             hipLaunchKernelGGL(k_drag_chain_io, dim3(1), dim3(64), 0, st, a.holes, a.nh, vx, vy, vz, b.endV, 1);
             hipLaunchKernelGGL(k_drag_chain_io, dim3(1), dim3(64), 0, st, a.holes, a.nh, jx, jy, jz, b.endJ, 1);
+            if (holesToo)
+                hipLaunchKernelGGL(k_drag_seq, dim3(1), dim3(64), 0, st, a, a.holes, (long long)a.nh, vx, vy, vz, jx,
+                                   jy, jz, (const int*)b.on);
             HIPCHK(hipGetLastError());
             return GX_OK;
         }
     }
     s->dragStats.sequential_passes++;
     hipLaunchKernelGGL(k_drag_seq, dim3(1), dim3(64), 0, st, a, list, ne, vx, vy, vz, jx, jy, jz, (const int*)b.on);
+    if (holesToo)
+        hipLaunchKernelGGL(k_drag_seq, dim3(1), dim3(64), 0, st, a, a.holes, (long long)a.nh, vx, vy, vz, jx, jy, jz,
+                           (const int*)b.on);
     HIPCHK(hipGetLastError());
     return GX_OK;
 }
@@ -884,12 +907,7 @@ int drag_fast(gx_solver* s, const DragArgs& a, const int32_t* nearList, long lon
               double* vz, double* jx, double* jy, double* jz, const DragBufs& b, const Meta* meta) {
     hipLaunchKernelGGL(k_drag_spinning, dim3(1), dim3(64), 0, s->stream, a.holes, meta, (const double*)jx,
                        (const double*)jy, (const double*)jz, a.c, b.on);
-    int rc = drag_pass(s, a, nearList, nn, vx, vy, vz, jx, jy, jz, b);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_drag_seq, dim3(1), dim3(64), 0, s->stream, a, a.holes, (long long)a.nh, vx, vy, vz, jx, jy,
-                       jz, (const int*)b.on);
-    HIPCHK(hipGetLastError());
-    return GX_OK;
+    return drag_pass(s, a, nearList, nn, vx, vy, vz, jx, jy, jz, b, true);
 }
 
 // The bodies of dragOuter (alive, not holes, not near; state order) into
@@ -1176,6 +1194,13 @@ int gx_drag_fast_dev(gx_solver* s, const double* x, const double* y, const doubl
 int gx_drag_get_stats(const gx_solver* s, gx_drag_stats* out) {
     if (!s || !out) return set_err(GX_ERR_ARG, "null argument");
     *out = s->dragStats;
+    if (s->dragCount.p) {  // the one-workgroup passes count on the device
+        unsigned long long c[2];
+        HIPCHK(hipStreamSynchronize(s->stream));
+        HIPCHK(hipMemcpy(c, s->dragCount.p, 16, hipMemcpyDeviceToHost));
+        out->iterations += (int64_t)c[0];
+        out->sequential_passes += (int64_t)c[1];
+    }
     return GX_OK;
 }
 

@@ -36,17 +36,20 @@
 // one binade, |s_t| in (2^E, 2^(E+1)), a step is s_{t+1} = s_t + u rint(d_t /
 // u), u = 2^(E-52), whenever d_t / u is not a half-integer: the exact sum
 // lies within u/2 of that grid point, which is inside the binade, and the
-// grid there is u.  So a segment whose partial sums all stay inside the
-// binade of its start value is an integer sum, which is associative: its
+// grid there is u.  A half-integer (a tie) rounds to the even grid point,
+// which depends on the running value only through its parity; after a tie
+// the value is even, so only the first tie of a segment depends on its start
+// (one bit, carried symbolically).  So a segment whose partial sums all stay
+// inside the binade of its start value is an integer sum, associative: its
 // total and its partial minimum and maximum are computed in parallel (one
 // wavefront per segment), and the walk (one thread per chain, from the exact
 // start value) checks the binade condition from those extremes.  A
-// segment that fails it (a binade crossing, a zero or subnormal value, a
-// tie, a wrong binade guess) is redone by one wavefront in blocks of 64 from
-// its exact start value, with the same integer rule and the binade read from
-// that value; only a block that leaves its binade (or meets a tie) is added
-// element by element, as the reference adds.  The binade of a segment is
-// guessed from an approximate prefix sum.
+// segment that fails it (a binade crossing, a zero or subnormal value, an
+// element as large as the sum, a wrong binade guess) is redone by one
+// wavefront in blocks of 64 from its exact start value, with the same
+// integer rule and the binade read from that value; only a block that leaves
+// its binade is added element by element, as the reference adds.  The
+// binade of a segment is guessed from an approximate prefix sum.
 //
 // Compiled with -ffp-contract=off: the kick repeats framedrag.go's
 // operations in its order.  One deliberate difference: dragBack's weights
@@ -210,10 +213,12 @@ __device__ __forceinline__ long long wave_incl_scan(long long v) {
     return v;
 }
 
-// The integer step of element d in binade e (see the header): n = rint(d / u)
-// with u = 2^(e-52); big if |d / u| >= 2^53 (or not finite), tie if d / u is
-// a half-integer.  ldexp by a power of two is exact unless the result is
-// subnormal, and then |d / u| < 2^-1022 and n = 0 with no tie either way.
+// The integer step of element d in binade e (see the header), q = d / u with
+// u = 2^(e-52): big if |q| >= 2^53 (or not finite); else a tie if q is a
+// half-integer (the exact sum lies halfway between two grid points and rounds
+// to the one with the even integer), with base = floor(q), or base = rint(q).
+// ldexp by a power of two is exact unless the result is subnormal, and then
+// |q| < 2^-1022: base 0, no tie either way.
 __device__ __forceinline__ long long seg_step(double d, int e, bool& big, bool& tie) {
     const double q = ldexp(d, 52 - e);
     big = !(fabs(q) < 9007199254740992.0);
@@ -222,18 +227,69 @@ __device__ __forceinline__ long long seg_step(double d, int e, bool& big, bool& 
         return 0;
     }
     tie = fabs(q - trunc(q)) == 0.5;
-    return (long long)rint(q);
+    return (long long)(tie ? floor(q) : rint(q));
 }
 
 __device__ __forceinline__ int dexp(double v) { return (int)((dbits(v) >> 52) & 0x7ff); }
 
-// One block of up to 64 elements in[0 .. len) of a chain, by one wavefront,
-// from the exact value v (wave-uniform; lane l holds d = in[l], -0.0 past
-// len): as an integer prefix sum in the binade of v when every partial value
-// stays strictly inside it (the exactness argument of the header, with the
-// binade read from v itself), else element by element in lane 0.  out[l]
-// receives the value before element l; returns the value after the block.
-__device__ double chain_block(double v, double d, int len, const double* in, double* out) {
+// Integer scan along a chain with ties, one wavefront, row after row of 64
+// elements.  A tie at element j adds base_j + a_j, a_j the parity of the
+// integer just below the halfway point (round half to even).  Every tie
+// leaves an even value, so a_j of each tie after the first is the parity of
+// the bases added since the previous tie; the first tie's a_f is the parity
+// of M + Q_f (M the start, Q_f the bases up to and including that tie) --
+// used when M is known, else taken as 0 here and added by the caller (the
+// segment summaries, computed before M is).
+struct TieState {
+    long long Q = 0, A = 0;            // bases and adjustments of the rows so far
+    long long lastQ = 0, firstQ = 0;   // Q at the last and at the first tie
+    bool seen = false;                 // a tie so far
+};
+
+// One row: lane l's base and tie flag (0 and false past the data).  Returns
+// the inclusive partial sum (bases + adjustments, relative to M); own = this
+// lane's base + adjustment; after = a tie lies at or before this lane.
+__device__ __forceinline__ long long tie_row(long long base, bool tie, bool knownM, long long M, TieState& st,
+                                             long long& own, bool& after) {
+    const int lane = threadIdx.x & 63;
+    const long long Q = st.Q + wave_incl_scan(base);
+    const unsigned long long tm = __ballot(tie);
+    const unsigned long long below = tm & ((1ULL << lane) - 1);
+    const long long Qp = __shfl(Q, below ? 63 - __clzll((long long)below) : lane, 64);
+    long long a = 0;
+    if (tie) {
+        if (below)
+            a = (Q - Qp) & 1;
+        else if (st.seen)
+            a = (Q - st.lastQ) & 1;
+        else
+            a = knownM ? ((M + Q) & 1) : 0;
+    }
+    const long long A = st.A + wave_incl_scan(a);
+    own = base + a;
+    after = st.seen || (tm & ((2ULL << lane) - 1)) != 0;
+    const long long qEnd = __shfl(Q, 63, 64), aEnd = __shfl(A, 63, 64);
+    if (tm) {
+        const long long ql = __shfl(Q, 63 - __clzll((long long)tm), 64);
+        const long long qf = __shfl(Q, __ffsll((long long)tm) - 1, 64);
+        if (!st.seen) st.firstQ = qf;
+        st.lastQ = ql;
+        st.seen = true;
+    }
+    st.Q = qEnd;
+    st.A = aEnd;
+    return Q + A;
+}
+
+// One block of up to 64 elements of a chain, by one wavefront, from the exact
+// value v (wave-uniform; lane l holds element l in d, -0.0 past len): as an
+// integer prefix sum in the binade of v when every partial value stays
+// strictly inside it (the exactness argument of the header, with the binade
+// read from v itself and ties rounded to even), else element by element,
+// every lane carrying the same running value and reading element t from
+// lane t.  out[l] (may be null) receives the value before element l; returns
+// the value after the block.
+__device__ double chain_block(double v, double d, int len, double* out) {
     const int lane = threadIdx.x & 63;
     const long long two52 = 1LL << 52, two53 = 1LL << 53;
     const int ev = dexp(v);
@@ -242,21 +298,29 @@ __device__ double chain_block(double v, double d, int len, const double* in, dou
         const long long M = (long long)ldexp(v, 52 - e);  // exact: |M| in [2^52, 2^53)
         bool big, tie;
         const long long n = seg_step(d, e, big, tie);
-        const long long P = M + wave_incl_scan(n);  // |P| < 2^53 + 64 * 2^53: no overflow
+        TieState st;
+        long long own;
+        bool after;
+        const long long P = M + tie_row(n, tie, true, M, st, own, after);  // |P| < 2^53 + 65 * 2^53
         const bool in = M > 0 ? (P > two52 && P < two53) : (P > -two53 && P < -two52);
         const bool mok = M > 0 ? M > two52 : M < -two52;
-        if (mok && !__any(big || tie || !in)) {
-            if (lane < len) out[lane] = ldexp((double)(P - n), e - 52);
+        if (mok && !__any(big || !in)) {
+            if (out && lane < len) out[lane] = ldexp((double)(P - own), e - 52);
             return ldexp((double)__shfl(P, 63, 64), e - 52);
         }
     }
-    double r = v;
-    if (lane == 0)
-        for (int t = 0; t < len; t++) {
-            out[t] = r;
-            r = r + in[t];
-        }
-    return __shfl(r, 0, 64);
+    const unsigned long long db = dbits(d);
+    const int dlo = (int)(unsigned)db, dhi = (int)(db >> 32);
+    double r = v, mine = 0;
+    for (int t = 0; t < len; t++) {
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane(dlo, t);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane(dhi, t);
+        const double x = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+        mine = lane == t ? r : mine;
+        r = r + x;
+    }
+    if (out && lane < len) out[lane] = mine;
+    return r;
 }
 
 }  // namespace
@@ -366,6 +430,151 @@ __global__ void k_drag_seq(DragArgs a, const int32_t* __restrict__ list, long lo
     for (long long j = 0; j < ne; j++) drag_kick_live(list[j], a, vx, vy, vz, jx, jy, jz);
 }
 
+// A whole pass over a short list in one workgroup, without the host (the
+// near lists of dragFast, at most kDragSmall bodies, two passes per
+// substep): the fixed point of the header with the chains added in order by
+// one thread each (C = 3 nh threads; a short chain costs less than the scan's
+// launches), the apply step, then, with holesToo, the holes one by one
+// (framedrag.go:138-142); the sequential pass by thread 0 if it has not
+// settled after maxIt iterations.  D, S0, S1: C * ne doubles each; E: 2 C
+// doubles (chain ends).  counters[0] += iterations, counters[1] += 1 if
+// sequential.
+__global__ __launch_bounds__(1024) void k_drag_small(DragArgs a, const int32_t* __restrict__ list, long long ne,
+                                                     double* vx, double* vy, double* vz, double* jx, double* jy,
+                                                     double* jz, const int* __restrict__ on, double* D, double* S0,
+                                                     double* S1, double* E, int maxIt, int holesToo,
+                                                     unsigned long long* counters) {
+    __shared__ int changed;
+    const int tid = threadIdx.x, C = 3 * a.nh;
+    if (!*on) return;
+    const double nz = __longlong_as_double((long long)kNegZero);
+    double* Sold = nullptr;
+    double* Snew = S0;
+    bool settled = false;
+    int it = 0;
+    auto eval = [&](const double* S, int apply) {
+        for (long long j = tid; j < ne; j += blockDim.x) {
+            const long long i = list[j];
+            const SpinView spin{S, ne, j, jx, jy, jz, a.holes};
+            const double ux = vx[i], uy = vy[i], uz = vz[i];
+            double unx, uny, unz;
+            if (!drag_rotate(i, a, spin, ux, uy, uz, unx, uny, unz)) {
+                for (int k = 0; k < C; k++) D[k * ne + j] = nz;
+                continue;
+            }
+            if (apply) {
+                vx[i] = unx;
+                vy[i] = uny;
+                vz[i] = unz;
+            }
+            const double mi = a.mass[i];
+            const double pxm = (unx - ux) * mi, pym = (uny - uy) * mi, pzm = (unz - uz) * mi;
+            const double total = drag_total(i, a, spin);
+            for (int k = 0; k < a.nh; k++) {
+                const double w = total == 0 ? 0.0 : drag_weight(i, k, a, spin);
+                double o0 = nz, o1 = nz, o2 = nz;
+                if (w != 0) {
+                    const int32_t h = a.holes[k];
+                    const double f = w / total;
+                    if (apply) {
+                        const double mh = a.mass[h];
+                        o0 = (pxm * -f) / mh;
+                        o1 = (pym * -f) / mh;
+                        o2 = (pzm * -f) / mh;
+                    } else {
+                        const double rx = a.x[i] - a.x[h], ry = a.y[i] - a.y[h], rz = a.z[i] - a.z[h];
+                        const double gx = pxm * f, gy = pym * f, gz = pzm * f;
+                        o0 = (ry * gz - rz * gy) * -1;
+                        o1 = (rz * gx - rx * gz) * -1;
+                        o2 = (rx * gy - ry * gx) * -1;
+                    }
+                }
+                D[(3LL * k) * ne + j] = o0;
+                D[(3LL * k + 1) * ne + j] = o1;
+                D[(3LL * k + 2) * ne + j] = o2;
+            }
+        }
+    };
+    auto chain_start = [&](int c, const double* px, const double* py, const double* pz) {
+        const int32_t h = a.holes[c / 3];
+        return c % 3 == 0 ? px[h] : (c % 3 == 1 ? py[h] : pz[h]);
+    };
+    while (it < maxIt && !settled) {
+        it++;
+        eval(Sold, 0);
+        if (tid == 0) changed = 0;
+        __syncthreads();
+        if (tid < C) {
+            const double st = chain_start(tid, jx, jy, jz);
+            double v = st;
+            bool diff = false;
+            const double* __restrict__ d = D + tid * ne;
+            const double* __restrict__ so = Sold ? Sold + tid * ne : nullptr;
+            double* __restrict__ sn = Snew + tid * ne;
+            constexpr int kB = 16;  // loads issued ahead of the dependent adds
+            for (long long j0 = 0; j0 < ne; j0 += kB) {
+                double x[kB], o[kB];
+#pragma unroll
+                for (int k = 0; k < kB; k++) {
+                    x[k] = j0 + k < ne ? d[j0 + k] : 0.0;
+                    o[k] = so && j0 + k < ne ? so[j0 + k] : st;
+                }
+#pragma unroll
+                for (int k = 0; k < kB; k++)
+                    if (j0 + k < ne) {
+                        diff = diff || dbits(o[k]) != dbits(v);
+                        sn[j0 + k] = v;
+                        v = v + x[k];
+                    }
+            }
+            E[tid] = v;
+            if (diff) changed = 1;
+        }
+        __syncthreads();
+        settled = changed == 0;
+        if (!settled) {
+            Sold = Snew;
+            Snew = Snew == S0 ? S1 : S0;
+        }
+        __syncthreads();
+    }
+    if (settled) {
+        eval(Sold, 1);  // the bodies' velocities and the holes' velocity increments
+        __syncthreads();
+        if (tid < C) {
+            double v = chain_start(tid, vx, vy, vz);
+            const double* __restrict__ d = D + tid * ne;
+            constexpr int kB = 16;
+            for (long long j0 = 0; j0 < ne; j0 += kB) {
+                double x[kB];
+#pragma unroll
+                for (int k = 0; k < kB; k++) x[k] = j0 + k < ne ? d[j0 + k] : 0.0;
+#pragma unroll
+                for (int k = 0; k < kB; k++)
+                    if (j0 + k < ne) v = v + x[k];
+            }
+            E[C + tid] = v;
+        }
+        __syncthreads();
+        if (tid < C) {
+            const int32_t h = a.holes[tid / 3];
+            double* pv = tid % 3 == 0 ? vx : (tid % 3 == 1 ? vy : vz);
+            double* pj = tid % 3 == 0 ? jx : (tid % 3 == 1 ? jy : jz);
+            pv[h] = E[C + tid];
+            pj[h] = E[tid];
+        }
+        __syncthreads();
+        if (tid == 0) atomicAdd(counters, (unsigned long long)it);
+    } else if (tid == 0) {
+        for (long long j = 0; j < ne; j++) drag_kick_live(list[j], a, vx, vy, vz, jx, jy, jz);
+        atomicAdd(counters, (unsigned long long)it);
+        atomicAdd(counters + 1, 1ULL);
+    }
+    __syncthreads();
+    if (holesToo && tid == 0)
+        for (int k = 0; k < a.nh; k++) drag_kick_live(a.holes[k], a, vx, vy, vz, jx, jy, jz);
+}
+
 // Exact ordered scan, step A: approximate sums of every segment and of its
 // rows of kDragRow elements (one wavefront per segment; chain c, segment g).
 __global__ __launch_bounds__(256) void k_scan_segsum(const double* __restrict__ D, long long ne, int chains, int nseg,
@@ -424,8 +633,9 @@ __global__ __launch_bounds__(256) void k_scan_summary(const double* __restrict__
     int flags = 0;
     if (ea == 0 || ea == 0x7ff) flags |= kSegNoGuess;
     const int e = ea - 1023;
-    long long carry = 0, pmin = 0, pmax = 0;
-    bool big = false, tie = false, allNegZero = true;
+    long long pmin = 0, pmax = 0;
+    bool big = false, allNegZero = true;
+    TieState st;
     for (long long r = lo; r < hi; r += 64) {
         const long long t = r + lane;
         const double d = t < hi ? D[c * ne + t] : __longlong_as_double((long long)kNegZero);
@@ -434,12 +644,11 @@ __global__ __launch_bounds__(256) void k_scan_summary(const double* __restrict__
         long long n = 0;
         if (!(flags & kSegNoGuess)) n = seg_step(d, e, b, ti);
         big = big || b;
-        tie = tie || ti;
-        const long long incl = wave_incl_scan(n);
-        const long long p = carry + incl;
+        long long own;
+        bool after;
+        const long long p = tie_row(n, ti, false, 0, st, own, after);
         pmin = min(pmin, p);
-        pmax = max(pmax, p);
-        carry += __shfl(incl, 63, 64);
+        pmax = max(pmax, p + (after ? 1 : 0));  // the first tie may add 1 (its a_f)
     }
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
@@ -447,16 +656,17 @@ __global__ __launch_bounds__(256) void k_scan_summary(const double* __restrict__
         pmax = max(pmax, __shfl_xor(pmax, d, 64));
     }
     if (__any(big)) flags |= kSegBig;
-    if (__any(tie)) flags |= kSegTie;
     if (__all(allNegZero)) flags |= kSegAllNegZero;
+    if (st.seen) flags |= kSegHasTie;
     if (lane == 0) {
-        DragSeg s;
-        s.total = carry;
-        s.pmin = pmin;
-        s.pmax = pmax;
-        s.e = e;
-        s.flags = flags;
-        seg[w] = s;
+        DragSeg sg;
+        sg.total = st.Q + st.A;
+        sg.pmin = pmin;
+        sg.pmax = pmax;
+        sg.qf = st.firstQ;
+        sg.e = e;
+        sg.flags = flags;
+        seg[w] = sg;
     }
 }
 
@@ -585,15 +795,16 @@ __global__ __launch_bounds__(256) void k_scan_chain(const double* __restrict__ D
                         continue;
                     }
                     const int es = dexp(s);
-                    bool ok = !(ss.flags & (kSegNoGuess | kSegBig | kSegTie)) && es != 0 && es != 0x7ff &&
+                    bool ok = !(ss.flags & (kSegNoGuess | kSegBig)) && es != 0 && es != 0x7ff &&
                               es - 1023 == ss.e && ss.pmin >= -two54 && ss.pmax <= two54;
                     if (ok) {
                         const long long M = (long long)ldexp(s, 52 - ss.e);  // exact: |M| in [2^52, 2^53)
-                        const long long lo = M + ss.pmin, hi = M + ss.pmax;  // every partial value
+                        const long long lo = M + ss.pmin, hi = M + ss.pmax;  // bounds of every partial value
                         ok = M > 0 ? (lo > two52 && hi < two53) : (lo > -two53 && hi < -two52);
                         if (ok) {
+                            const long long af = (ss.flags & kSegHasTie) ? ((M + ss.qf) & 1) : 0;
                             segMode[gi] = 0;
-                            s = ldexp((double)(M + ss.total), ss.e - 52);
+                            s = ldexp((double)(M + ss.total + af), ss.e - 52);
                         }
                     }
                     if (!ok) {
@@ -616,7 +827,7 @@ __global__ __launch_bounds__(256) void k_scan_chain(const double* __restrict__ D
                     for (int b0 = 0; b0 < len; b0 += 64) {
                         const int bl = min(64, len - b0);
                         const double d = tid < bl ? buf[b0 + tid] : __longlong_as_double((long long)kNegZero);
-                        v = chain_block(v, d, bl, buf + b0, sbuf + b0);
+                        v = chain_block(v, d, bl, S ? sbuf + b0 : nullptr);
                     }
                     if (tid == 0) {
                         s = v;
@@ -658,18 +869,19 @@ __global__ __launch_bounds__(256) void k_scan_fill(const double* __restrict__ D,
     const double s0 = segStart[w];
     const DragSeg ss = seg[w];
     const long long M = mode == 0 ? (long long)ldexp(s0, 52 - ss.e) : 0;
-    long long carry = 0;
+    TieState st;
     bool diff = false;
     for (long long r = lo; r < hi; r += 64) {
         const long long t = r + lane;
         double v;
         if (mode == 0) {
             const double d = t < hi ? D[c * ne + t] : 0.0;
-            bool b, ti;
+            bool b = false, ti = false;
             const long long n = t < hi ? seg_step(d, ss.e, b, ti) : 0;
-            const long long incl = wave_incl_scan(n);
-            v = ldexp((double)(M + carry + incl - n), ss.e - 52);
-            carry += __shfl(incl, 63, 64);
+            long long own;
+            bool after;
+            const long long p = tie_row(n, ti && t < hi, true, M, st, own, after);
+            v = ldexp((double)(M + p - own), ss.e - 52);
         } else if (mode == 1) {
             v = s0;
         } else {

@@ -81,13 +81,14 @@ struct DragArgs {
 };
 // Integer summary of one segment of an exact ordered scan.
 struct DragSeg {
-    long long total, pmin, pmax;  // sum of the integer steps; extremes of the partial sums (0 included)
+    long long total, pmin, pmax;  // sum of the integer steps; bounds of the partial sums (0 included)
+    long long qf;                 // sum of the integer steps up to the first tie (kSegHasTie)
     int e, flags;                 // binade exponent guessed for the segment; kSeg* flags
 };
 constexpr int kDragSeg = 1024;  // elements per segment
 constexpr int kDragRow = 64;    // elements per speculated row
 constexpr int kDragRows = kDragSeg / kDragRow;
-constexpr int kSegNoGuess = 1, kSegBig = 2, kSegTie = 4, kSegAllNegZero = 8;
+constexpr int kSegNoGuess = 1, kSegBig = 2, kSegHasTie = 4, kSegAllNegZero = 8;
 __global__ void k_drag_spinning(const int32_t* holes, const Meta* meta, const double* jx, const double* jy,
                                 const double* jz, double c, int* on);
 __global__ void k_drag_outer_flag(long long n, const uint8_t* kind, const uint8_t* alive, const uint8_t* nearFlag,
@@ -99,6 +100,10 @@ __global__ void k_drag_eval(DragArgs a, const int32_t* list, long long ne, doubl
                             const int* on, double* D);
 __global__ void k_drag_seq(DragArgs a, const int32_t* list, long long ne, double* vx, double* vy, double* vz,
                            double* jx, double* jy, double* jz, const int* on);
+constexpr int kDragSmall = 4096;  // lists up to this length run in one workgroup (k_drag_small)
+__global__ void k_drag_small(DragArgs a, const int32_t* list, long long ne, double* vx, double* vy, double* vz,
+                             double* jx, double* jy, double* jz, const int* on, double* D, double* S0, double* S1,
+                             double* E, int maxIt, int holesToo, unsigned long long* counters);
 __global__ void k_scan_segsum(const double* D, long long ne, int chains, int nseg, double* sig, double* rowSig);
 __global__ void k_scan_guess(const double* sig, int nseg, const double* start, double* guess);
 __global__ void k_scan_summary(const double* D, long long ne, int chains, int nseg, const double* guess,

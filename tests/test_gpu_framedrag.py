@@ -7,7 +7,8 @@ against the oracle's sequential restatement.
     holes, dead bodies, near flags and an unsorted near list;
   * the same through the sequential fallback (no fixed-point iteration, and a
     fixed point stopped after one iteration), which must leave the state
-    untouched before it runs;
+    untouched before it runs, for a list short enough for the one-workgroup
+    pass and for a longer one;
   * whole split steps with spinning holes: as the other split-step tests,
     1e-12 of the oracle (the holes' sums over bodies are tree reductions on
     the GPU), spins included, and the same substep counts.
@@ -108,12 +109,13 @@ def test_drag_fast_bit_identical(gpu, n, dt, two, strong):
     assert_same(got, want, s)
 
 
+@pytest.mark.parametrize("n", [2000, 20000])  # one workgroup / host-driven passes
 @pytest.mark.parametrize("iterations", ["0", "1"])
-def test_sequential_fallback_bit_identical(gpu, monkeypatch, iterations):
+def test_sequential_fallback_bit_identical(gpu, monkeypatch, iterations, n):
     monkeypatch.setenv("GX_DRAG_ITERATIONS", iterations)
-    s = spin_state(2000, 63, two=True)
-    near = np.zeros(2000, np.uint8)
-    say(f"[drag] sequential fallback after {iterations} iterations")
+    s = spin_state(n, 63, two=True)
+    near = np.zeros(n, np.uint8)
+    say(f"[drag] n={n} sequential fallback after {iterations} iterations")
     want = oracle_pass(s, 0.01, near=near)
     sv = Solver(walk="fp64")
     got = gpu_pass(sv, s, 0.01, near=near)
