@@ -111,9 +111,21 @@ def lib():
                                     ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.or_diagnostics.argtypes = [ctypes.c_int64, _D, _D, _D, _D, _D, _D, _D, _U8, ctypes.c_int, _D]
         L.or_split_step.restype = ctypes.c_int
+        _U64 = ctypes.POINTER(ctypes.c_uint64)
         L.or_split_step.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _D, _D, _D, _U8, _U8, _D, _D, _D,
                                     ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
-                                    ctypes.c_double, ctypes.c_int, _D, _D, _D]
+                                    ctypes.c_double, ctypes.c_int, _D, _D, _D, ctypes.c_void_p, _U64,
+                                    ctypes.c_double, ctypes.c_int]
+        L.or_retard_new.restype = ctypes.c_void_p
+        L.or_retard_free.argtypes = [ctypes.c_void_p]
+        L.or_retard_count.restype = ctypes.c_int
+        L.or_retard_count.argtypes = [ctypes.c_void_p]
+        L.or_hole_appeared.argtypes = [ctypes.c_void_p, _D, _D, _D, _D, _D, _D, _U64, ctypes.c_int64,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.or_source_pos.restype = ctypes.c_int
+        L.or_source_pos.argtypes = [ctypes.c_void_p, _D, _D, _D, _U8, _U8, _U64, ctypes.c_int64, ctypes.c_int64,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, _D]
         L.or_drag_kick.argtypes = [_D, _D, _D, _D, _D, _D, _D, _U8, _U8, _D, _D, _D, ctypes.c_int64, ctypes.c_int64,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.or_drag_pass.argtypes = [_D, _D, _D, _D, _D, _D, _D, _U8, _U8, _D, _D, _D, ctypes.c_int64, _U8,
@@ -405,13 +417,59 @@ def diagnostics(px, py, pz, vx, vy, vz, m, kind, dm_off=False):
     return out
 
 
-def split_step(solver, state, m, kind, alive, eps, dt, c=0.0, dm_off=False, acc_valid=True):
+GRAVITY = {"instant": 0, "retarded": 1, "retarded+v": 2}  # sim.GravityMode, retarded.go:21-28
+
+
+def _u64(a):
+    return None if a is None else np.ascontiguousarray(a, np.uint64).ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+class Retard:
+    """The holes' histories for retarded gravity (holeHistory by body ID,
+    internal/sim/retarded.go:62-209), kept across steps."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p(lib().or_retard_new())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().or_retard_free(self._h)
+            self._h = None
+
+    def __len__(self):
+        return lib().or_retard_count(self._h)
+
+    def hole_appeared(self, state, i, time, dt, c, ids=None):
+        """holeAppeared (retarded.go:157-161): hole i was added at ``time``."""
+        keep = None if ids is None else np.ascontiguousarray(ids, np.uint64)
+        lib().or_hole_appeared(self._h, *(_dp(np.ascontiguousarray(state[k], np.float64))
+                                          for k in ("px", "py", "pz", "vx", "vy", "vz")),
+                               _u64(keep), int(i), float(time), float(dt), float(c))
+
+    def source_pos(self, state, kind, alive, k, obs, time, dt, c, gravity, ids=None):
+        """sourcePos (retarded.go:118-155) of alive hole number k for an
+        observer at obs; None if its gravity has not arrived there."""
+        out = np.zeros(3)
+        keep = None if ids is None else np.ascontiguousarray(ids, np.uint64)
+        pos = [np.ascontiguousarray(state[a], np.float64) for a in ("px", "py", "pz")]
+        vis = lib().or_source_pos(self._h, *(_dp(p) for p in pos), _u8(np.ascontiguousarray(kind, np.uint8)),
+                                  _u8(np.ascontiguousarray(alive, np.uint8)), _u64(keep), len(pos[0]), int(k),
+                                  float(obs[0]), float(obs[1]), float(obs[2]), float(time), float(dt), float(c),
+                                  GRAVITY[gravity], _dp(out))
+        return out if vis else None
+
+
+def split_step(solver, state, m, kind, alive, eps, dt, c=0.0, dm_off=False, acc_valid=True, retard=None, time=0.0,
+               gravity="instant", ids=None):
     """One step with black holes (internal/sim/split.go:164-199: RESPA
-    substeps for stars near holes, no gas, instant gravity) on the float64
-    arrays px..vz and 'acc' of ``state``, in place; the slow forces come from
-    ``solver`` (an oracle Solver).  If ``state`` holds the hole spins "jx",
-    "jy", "jz" (float64, per body), frame dragging (framedrag.go) acts with
-    c > 0 and the spins are updated in place.  Returns the number of
+    substeps for stars near holes, no gas) on the float64 arrays px..vz and
+    'acc' of ``state``, in place; the slow forces come from ``solver`` (an
+    oracle Solver).  If ``state`` holds the hole spins "jx", "jy", "jz"
+    (float64, per body), frame dragging (framedrag.go) acts with c > 0 and the
+    spins are updated in place.  With ``retard`` (a Retard) the holes' pull
+    follows ``gravity`` (retarded.go) at simulation time ``time``, and the
+    holes are recorded at time + dt after the step (sim.go:285-287); ``ids``
+    are the bodies' IDs (default: the state index).  Returns the number of
     substeps."""
     L = lib()
     n = len(state["px"])
@@ -427,10 +485,12 @@ def split_step(solver, state, m, kind, alive, eps, dt, c=0.0, dm_off=False, acc_
     L.or_set_monopole(solver._h, int(solver.monopole_only))
     L.or_set_workers(solver._h, int(solver.workers))
     j = tuple(_dp(state[k]) if spin else None for k in ("jx", "jy", "jz"))
+    keep = None if ids is None else np.ascontiguousarray(ids, np.uint64)
     sub = L.or_split_step(solver._h, _dp(state["px"]), _dp(state["py"]), _dp(state["pz"]), _dp(state["vx"]),
                           _dp(state["vy"]), _dp(state["vz"]), _dp(m), _u8(kind), _u8(alive), *j, n,
                           float(solver.theta), float(eps), int(dm_off), float(dt), float(c), int(bool(acc_valid)),
-                          _dp(ax), _dp(ay), _dp(az))
+                          _dp(ax), _dp(ay), _dp(az), None if retard is None else retard._h, _u64(keep),
+                          float(time), GRAVITY[gravity])
     state["acc"] = np.stack([ax, ay, az], axis=1)
     return sub
 

@@ -755,6 +755,162 @@ static double pair_rs(int ki, double mi, int kj, double mj, double c) {
     return ki == KIND_HOLE ? schwarzschild(mi, c) : schwarzschild(mj, c);
 }
 
+/* Retarded gravity of black holes, internal/sim/retarded.go: a ring buffer
+ * of each hole's positions and coordinate velocities at outer steps, keyed
+ * by body ID; a body sees a hole where it was at the retarded time. */
+#define HIST_LEN 8192     /* historyLen, retarded.go:50-52 */
+#define RETARD_TOL 1e-7   /* retardTolerance, :54-59 */
+#define RETARD_MAX_IT 64  /* retardMaxIterations */
+enum { GRAVITY_INSTANT = 0, GRAVITY_RETARDED = 1, GRAVITY_RETARDED_V = 2 };
+
+typedef struct {
+    uint64_t id;
+    double t0, appeared, dt;
+    int n, head;
+    double *x, *y, *z, *vx, *vy, *vz;
+} or_hist;
+
+typedef struct {
+    or_hist **h;
+    int count, cap;
+} or_retard;
+
+or_retard *or_retard_new(void) { return calloc(1, sizeof(or_retard)); }
+static void hist_free(or_hist *h) {
+    free(h->x);
+    free(h);
+}
+void or_retard_free(or_retard *r) {
+    if (!r) return;
+    for (int i = 0; i < r->count; i++) hist_free(r->h[i]);
+    free(r->h);
+    free(r);
+}
+int or_retard_count(const or_retard *r) { return r->count; }
+
+static or_hist *hist_find(const or_retard *r, uint64_t id) {
+    for (int i = 0; i < r->count; i++)
+        if (r->h[i]->id == id) return r->h[i];
+    return NULL;
+}
+
+/* push, retarded.go:75-83 */
+static void hist_push(or_hist *h, double t, double x, double y, double z, double vx, double vy, double vz) {
+    h->head = (h->head + 1) % HIST_LEN;
+    h->x[h->head] = x;
+    h->y[h->head] = y;
+    h->z[h->head] = z;
+    h->vx[h->head] = vx;
+    h->vy[h->head] = vy;
+    h->vz[h->head] = vz;
+    if (h->n < HIST_LEN) h->n++;
+    h->t0 = t - (double)(h->n - 1) * h->dt;
+}
+
+/* at, retarded.go:85-103: the state interpolated at t (extrapolated before
+ * the oldest entry); 0 if the hole had not appeared yet */
+static int hist_at(const or_hist *h, double t, double o[6]) {
+    if (t < h->appeared) {
+        for (int i = 0; i < 6; i++) o[i] = 0;
+        return 0;
+    }
+    int oldest = (h->head - h->n + 1 + HIST_LEN) % HIST_LEN;
+    if (t <= h->t0 || h->n == 1) {
+        double d = t - h->t0;
+        o[0] = h->x[oldest] + d * h->vx[oldest];
+        o[1] = h->y[oldest] + d * h->vy[oldest];
+        o[2] = h->z[oldest] + d * h->vz[oldest];
+        o[3] = h->vx[oldest];
+        o[4] = h->vy[oldest];
+        o[5] = h->vz[oldest];
+        return 1;
+    }
+    double f = (t - h->t0) / h->dt;
+    int k = (int)f;
+    if (k > h->n - 2) k = h->n - 2;
+    double w = f - (double)k;
+    int a = (oldest + k) % HIST_LEN, b = (oldest + k + 1) % HIST_LEN;
+    const double *arr[6] = {h->x, h->y, h->z, h->vx, h->vy, h->vz};
+    for (int i = 0; i < 6; i++) o[i] = arr[i][a] + w * (arr[i][b] - arr[i][a]);
+    return 1;
+}
+
+/* history, retarded.go:163-179: create with the current state as first entry */
+static or_hist *hist_create(or_retard *r, uint64_t id, double t, double dt, double x, double y, double z,
+                            double ux, double uy, double uz, double c) {
+    or_hist *h = calloc(1, sizeof(or_hist));
+    h->id = id;
+    h->appeared = -INFINITY;
+    h->dt = dt;
+    h->x = malloc(6 * HIST_LEN * sizeof(double));
+    h->y = h->x + HIST_LEN;
+    h->z = h->y + HIST_LEN;
+    h->vx = h->z + HIST_LEN;
+    h->vy = h->vx + HIST_LEN;
+    h->vz = h->vy + HIST_LEN;
+    double g = c <= 0 ? 1 : sqrt(1 + (ux * ux + uy * uy + uz * uz) / (c * c)); /* CoordinateVelocity */
+    double ig = 1 / g;
+    hist_push(h, t, x, y, z, ux * ig, uy * ig, uz * ig);
+    if (r->count == r->cap) {
+        r->cap = r->cap ? 2 * r->cap : 8;
+        r->h = realloc(r->h, (size_t)r->cap * sizeof(or_hist *));
+    }
+    r->h[r->count++] = h;
+    return h;
+}
+
+static uint64_t body_id(const uint64_t *ids, int64_t i) { return ids ? ids[i] : (uint64_t)i; }
+
+/* holeAppeared, retarded.go:157-161 (AddBody of a hole, sim.go:236-243) */
+void or_hole_appeared(or_retard *r, const double *px, const double *py, const double *pz, const double *vx,
+                      const double *vy, const double *vz, const uint64_t *ids, int64_t i, double time, double dt,
+                      double c) {
+    uint64_t id = body_id(ids, i);
+    or_hist *h = hist_find(r, id);
+    if (!h) h = hist_create(r, id, time, dt, px[i], py[i], pz[i], vx[i], vy[i], vz[i], c);
+    h->appeared = time;
+}
+
+/* recordHoles, retarded.go:181-209, at the time after the step */
+static void record_holes(or_retard *r, const double *px, const double *py, const double *pz, const double *vx,
+                         const double *vy, const double *vz, const uint8_t *kind, const uint8_t *alive,
+                         const uint64_t *ids, int64_t n, double time, double dt, double c, int mode) {
+    if (!(mode != GRAVITY_INSTANT && c > 0)) {
+        for (int i = 0; i < r->count; i++) hist_free(r->h[i]);
+        r->count = 0;
+        return;
+    }
+    int64_t nh = 0;
+    for (int64_t i = 0; i < n; i++)
+        if (alive[i] && kind[i] == KIND_HOLE) nh++;
+    int seen = r->count != nh;
+    for (int64_t i = 0; i < n; i++) {
+        if (!(alive[i] && kind[i] == KIND_HOLE)) continue;
+        uint64_t id = body_id(ids, i);
+        or_hist *h = hist_find(r, id);
+        if (h) {
+            double g = c <= 0 ? 1 : sqrt(1 + (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]) / (c * c));
+            double ig = 1 / g;
+            hist_push(h, time, px[i], py[i], pz[i], vx[i] * ig, vy[i] * ig, vz[i] * ig);
+        } else {
+            hist_create(r, id, time, dt, px[i], py[i], pz[i], vx[i], vy[i], vz[i], c);
+        }
+    }
+    if (seen) { /* drop histories of holes that merged away */
+        int keep = 0;
+        for (int j = 0; j < r->count; j++) {
+            int alive_hole = 0;
+            for (int64_t i = 0; i < n && !alive_hole; i++)
+                alive_hole = alive[i] && kind[i] == KIND_HOLE && body_id(ids, i) == r->h[j]->id;
+            if (alive_hole)
+                r->h[keep++] = r->h[j];
+            else
+                hist_free(r->h[j]);
+        }
+        r->count = keep;
+    }
+}
+
 typedef struct {
     double *px, *py, *pz, *vx, *vy, *vz;
     const double *mass;
@@ -770,6 +926,9 @@ typedef struct {
     const uint8_t *alive;
     double *jx, *jy, *jz; /* hole spins (angular momentum), NULL: none */
     double *weights;      /* dragBack's per-hole weights, nh entries */
+    or_hist **hists;      /* history of each hole slot (bindHistories), NULL: none */
+    int mode;             /* Params.Gravity */
+    double time, dt;      /* Simulation.Time during the step, Params.Dt */
 } split_ctx;
 
 static double source_mass(const split_ctx *q, int64_t i) {
@@ -786,14 +945,59 @@ static double hole_timescale(const split_ctx *q, int64_t i, int64_t h) {
     return sqrt(sqrt(r2 + q->eps2) / a);
 }
 
-/* holePull, split.go:120-137 (instant gravity: the hole's current position) */
+/* sourcePos, retarded.go:118-155: where a body at (x, y, z) sees hole slot
+ * k now; 0 if its gravity has not arrived there yet */
+static int source_pos(const split_ctx *q, int64_t k, double x, double y, double z, double o[3]) {
+    int64_t h = q->holes[k];
+    double hx = q->px[h], hy = q->py[h], hz = q->pz[h];
+    o[0] = hx;
+    o[1] = hy;
+    o[2] = hz;
+    if (!(q->mode != GRAVITY_INSTANT && q->c > 0)) return 1;
+    const or_hist *hist = q->hists ? q->hists[k] : NULL;
+    double c = q->c;
+    double delay = sqrt((x - hx) * (x - hx) + (y - hy) * (y - hy) + (z - hz) * (z - hz)) / c;
+    if (!hist || delay < q->dt) return 1;
+    double st[6];
+    for (int it = 0; it < RETARD_MAX_IT; it++) {
+        if (!hist_at(hist, q->time - delay, st)) {
+            o[0] = o[1] = o[2] = 0;
+            return 0;
+        }
+        hx = st[0];
+        hy = st[1];
+        hz = st[2];
+        double next = sqrt((x - hx) * (x - hx) + (y - hy) * (y - hy) + (z - hz) * (z - hz)) / c;
+        int converged = fabs(next - delay) < RETARD_TOL;
+        delay = next;
+        if (converged) break;
+    }
+    hist_at(hist, q->time - delay, st);
+    hx = st[0];
+    hy = st[1];
+    hz = st[2];
+    if (q->mode == GRAVITY_RETARDED_V) {
+        hx = hx + delay * st[3];
+        hy = hy + delay * st[4];
+        hz = hz + delay * st[5];
+    }
+    o[0] = hx;
+    o[1] = hy;
+    o[2] = hz;
+    return 1;
+}
+
+/* holePull, split.go:120-137: the pull of every hole but i on body i, from
+ * where the body sees it (sourcePos) */
 static void hole_pull(const split_ctx *q, int64_t i, double out[3]) {
     double ax = 0, ay = 0, az = 0;
     for (int64_t k = 0; k < q->nh; k++) {
         int64_t h = q->holes[k];
         if (h == i) continue;
         double x = q->px[i], y = q->py[i], z = q->pz[i];
-        double dx = q->px[h] - x, dy = q->py[h] - y, dz = q->pz[h] - z;
+        double src[3];
+        if (!source_pos(q, k, x, y, z, src)) continue;
+        double dx = src[0] - x, dy = src[1] - y, dz = src[2] - z;
         double rs = pair_rs(q->kind[i], q->mass[i], KIND_HOLE, q->mass[h], q->c);
         double f = pair_force(q->mass[h], dx * dx + dy * dy + dz * dz, q->eps2, rs);
         ax = ax + f * dx;
@@ -1019,7 +1223,7 @@ void or_drag_kick(double *px, double *py, double *pz, double *vx, double *vy, do
                   const uint8_t *kind, const uint8_t *alive, double *jx, double *jy, double *jz, int64_t n, int64_t i,
                   double dt, double eps2, double c) {
     split_ctx q = {px, py, pz, vx, vy, vz, mass, kind, n, eps2, c, 0, NULL, 0, NULL, NULL, 0, alive, jx, jy, jz,
-                   NULL};
+                   NULL, NULL, GRAVITY_INSTANT, 0, 0};
     int32_t *holes = malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
     q.weights = malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
     for (int64_t k = 0; k < n; k++)
@@ -1037,7 +1241,7 @@ void or_drag_pass(double *px, double *py, double *pz, double *vx, double *vy, do
                   const uint8_t *near, const int32_t *list, int64_t count, int fast, double dt, double eps2,
                   double c) {
     split_ctx q = {px, py, pz, vx, vy, vz, mass, kind, n, eps2, c, 0, NULL, 0, NULL, NULL, 0, alive, jx, jy, jz,
-                   NULL};
+                   NULL, NULL, GRAVITY_INSTANT, 0, 0};
     int32_t *holes = malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
     q.weights = malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
     q.near = calloc((size_t)(n > 0 ? n : 1), 1);
@@ -1064,9 +1268,9 @@ void or_drag_pass(double *px, double *py, double *pz, double *vx, double *vy, do
 int or_split_step(or_solver *s, double *px, double *py, double *pz, double *vx, double *vy, double *vz,
                   const double *mass, const uint8_t *kind, const uint8_t *alive, double *jx, double *jy, double *jz,
                   int64_t n, double theta, double eps, int dm_off, double dt, double c, int acc_valid, double *ax,
-                  double *ay, double *az) {
+                  double *ay, double *az, or_retard *r, const uint64_t *ids, double time, int mode) {
     split_ctx q = {px, py, pz, vx, vy, vz, mass, kind, n, eps * eps, c, dm_off, NULL, 0, NULL, NULL, 0, alive,
-                   jx, jy, jz, NULL};
+                   jx, jy, jz, NULL, NULL, r ? mode : GRAVITY_INSTANT, time, dt};
     int32_t *holes = malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
     q.weights = malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
     q.near = malloc((size_t)(n > 0 ? n : 1));
@@ -1076,6 +1280,9 @@ int or_split_step(or_solver *s, double *px, double *py, double *pz, double *vx, 
     for (int64_t i = 0; i < n; i++)
         if (alive[i] && kind[i] == KIND_HOLE) holes[q.nh++] = (int32_t)i; /* collectHoles, sim.go:291-299 */
     q.holes = holes;
+    q.hists = calloc((size_t)(q.nh > 0 ? q.nh : 1), sizeof(or_hist *));
+    if (r && mode != GRAVITY_INSTANT && c > 0) /* bindHistories, retarded.go:106-116 */
+        for (int64_t k = 0; k < q.nh; k++) q.hists[k] = hist_find(r, body_id(ids, holes[k]));
     theta = theta == 0 ? DEFAULT_THETA : theta;
     if (!acc_valid) or_accelerations(s, px, py, pz, mass, kind, alive, n, theta, eps, dm_off, s->monopole, s->workers,
                                      ax, ay, az);
@@ -1096,7 +1303,9 @@ int or_split_step(or_solver *s, double *px, double *py, double *pz, double *vx, 
         }
         or_accelerations(s, px, py, pz, mass, kind, alive, n, theta, eps, dm_off, s->monopole, s->workers, ax, ay, az);
         or_kick(n, vx, vy, vz, ax, ay, az, 0.5 * dt);
+        if (r) record_holes(r, px, py, pz, vx, vy, vz, kind, alive, ids, n, time + dt, dt, c, mode);
         free(holes);
+        free(q.hists);
         free(q.weights);
         free(q.near);
         free(q.nearList);
@@ -1139,12 +1348,34 @@ int or_split_step(or_solver *s, double *px, double *py, double *pz, double *vx, 
         vy[i] += 0.5 * dt * (ay[i] + fary[i]);
         vz[i] += 0.5 * dt * (az[i] + farz[i]);
     }
+    if (r) record_holes(r, px, py, pz, vx, vy, vz, kind, alive, ids, n, time + dt, dt, c, mode); /* sim.go:285-287 */
     free(holes);
+    free(q.hists);
     free(q.weights);
     free(q.near);
     free(q.nearList);
     free(farx);
     return sub;
+}
+
+/* Test hook: sourcePos (retarded.go:118-155) of alive hole number k (in
+ * state order) for an observer at (x, y, z), with the histories of r bound
+ * as in a step at the given time. */
+int or_source_pos(or_retard *r, double *px, double *py, double *pz, const uint8_t *kind, const uint8_t *alive,
+                  const uint64_t *ids, int64_t n, int64_t k, double x, double y, double z, double time, double dt,
+                  double c, int mode, double out[3]) {
+    split_ctx q = {px, py, pz, NULL, NULL, NULL, NULL, kind, n, 0, c, 0, NULL, 0, NULL, NULL, 0, alive, NULL, NULL,
+                   NULL, NULL, NULL, mode, time, dt};
+    int32_t *holes = malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    for (int64_t i = 0; i < n; i++)
+        if (alive[i] && kind[i] == KIND_HOLE) holes[q.nh++] = (int32_t)i;
+    q.holes = holes;
+    q.hists = calloc((size_t)(q.nh > 0 ? q.nh : 1), sizeof(or_hist *));
+    for (int64_t j = 0; j < q.nh; j++) q.hists[j] = hist_find(r, body_id(ids, holes[j]));
+    int vis = k < q.nh ? source_pos(&q, k, x, y, z, out) : 0;
+    free(holes);
+    free(q.hists);
+    return vis;
 }
 
 /* ------------------------------------------------------------------ */

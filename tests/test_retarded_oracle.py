@@ -1,0 +1,192 @@
+"""Retarded gravity of black holes in the oracle (internal/sim/retarded.go,
+restated in oracle/bh_oracle.c), pinned by the reference's own tests
+(retarded_test.go), restated on the oracle's split step:
+
+  * a uniformly moving hole is seen on the observer's past light cone, behind
+    its current position; in the extrapolated mode it is seen at its
+    current position (uniform motion);
+  * the gravity of a hole placed during the run arrives at a star 2 length
+    units away after 2/c;
+  * naive retardation widens a relativistic binary's orbit by > 20 % within
+    one orbit, the extrapolated mode by a tenth of that, instant gravity
+    keeps it;
+  * a disk around a hole stays bound in the extrapolated mode;
+
+and by the ring buffer's wrap-around: after more steps than it holds, the
+retarded time lies before the oldest entry and the linear extrapolation from
+it still puts a uniformly moving hole on the light cone.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from gravitas_amd import ics
+
+KEYS = ("px", "py", "pz", "vx", "vy", "vz")
+
+
+def proper_velocity(v, c):
+    """ProperVelocity, internal/sim/relativity.go:38-46."""
+    v = np.asarray(v, float)
+    ln = math.sqrt(v @ v)
+    if ln > 0.99 * c:
+        v = v * (0.99 * c / ln)
+    return v * (1 / math.sqrt(1 - (v @ v) / (c * c)))
+
+
+def bodies(spec):
+    n = len(spec)
+    st = {k: np.zeros(n) for k in KEYS}
+    m = np.zeros(n)
+    kind = np.zeros(n, np.uint8)
+    for i, b in enumerate(spec):
+        st["px"][i], st["py"][i], st["pz"][i] = b.get("pos", (0, 0, 0))
+        st["vx"][i], st["vy"][i], st["vz"][i] = b.get("vel", (0, 0, 0))
+        m[i] = b["m"]
+        kind[i] = b.get("kind", ics.KIND_STAR)
+    return st, m, kind, np.ones(n, np.uint8)
+
+
+class Run:
+    """A simulation of the split step with retarded gravity: Simulation.Time
+    advances by dt after every step, as in sim.go:285-287."""
+
+    def __init__(self, st, m, kind, alive, dt, eps, c, gravity, theta=0.5):
+        self.st, self.m, self.kind, self.alive = st, m, kind, alive
+        self.dt, self.eps, self.c, self.gravity = dt, eps, c, gravity
+        self.o = oracle.Solver(theta=theta)
+        self.r = oracle.Retard()
+        self.time = 0.0
+        self.steps = 0
+
+    def step(self):
+        oracle.split_step(self.o, self.st, self.m, self.kind, self.alive, self.eps, self.dt, c=self.c,
+                          acc_valid=self.steps > 0, retard=self.r, time=self.time, gravity=self.gravity)
+        self.time += self.dt
+        self.steps += 1
+
+    def source(self, k, obs):
+        return self.r.source_pos(self.st, self.kind, self.alive, k, obs, self.time, self.dt, self.c, self.gravity)
+
+
+def moving_hole(gravity, steps=500):
+    """retarded_test.go:10-20: a light hole moving along x at 0.3 c."""
+    st, m, kind, alive = bodies([
+        {"vel": proper_velocity((3, 0, 0), 10), "m": 1e-12, "kind": ics.KIND_HOLE},
+        {"pos": (50, 0, 0), "m": 1e-12}])
+    run = Run(st, m, kind, alive, 0.004, 0.01, 10.0, gravity)
+    for _ in range(steps):
+        run.step()
+    return run
+
+
+@pytest.mark.parametrize("steps", [500, 9000])  # 9,000 > the 8,192 entries of the ring buffer
+def test_retarded_position_lies_on_the_light_cone(steps):
+    """retarded_test.go:22-38."""
+    run = moving_hole("retarded", steps)
+    obs = np.array([1.0, 4.0, -2.0]) if steps == 500 else np.array([-400.0, 40.0, -20.0])
+    src = run.source(0, obs)
+    assert src is not None
+    tr = src[0] / 3  # uniform motion from the origin at v = 3
+    d = np.linalg.norm(obs - src)
+    assert abs(d - 10 * (run.time - tr)) <= 1e-6, (d, 10 * (run.time - tr))
+    assert src[0] < run.st["px"][0]  # it lags behind the current position
+    if steps == 9000:
+        assert run.time - d / 10 < run.time - (8192 - 1) * run.dt  # before the oldest entry: extrapolated
+
+
+def test_extrapolated_retardation_points_at_the_current_position():
+    """retarded_test.go:40-48."""
+    run = moving_hole("retarded+v")
+    src = run.source(0, (1, 4, -2))
+    cur = np.array([run.st["px"][0], run.st["py"][0], run.st["pz"][0]])
+    assert np.linalg.norm(src - cur) <= 1e-6
+
+
+def test_gravity_of_a_new_hole_spreads_at_the_speed_of_light():
+    """retarded_test.go:50-73: a hole placed during the run (AddBody) pulls a
+    star 2 away only after 2/c = 0.2."""
+    st, m, kind, alive = bodies([{"pos": (2, 0, 0), "m": 1e-9}, {"m": 1.0, "kind": ics.KIND_HOLE}])
+    alive[1] = 0  # not there yet
+    run = Run(st, m, kind, alive, 0.004, 0.01, 10.0, "retarded+v")
+    for _ in range(10):
+        run.step()
+    alive[1] = 1
+    run.r.hole_appeared(st, 1, run.time, run.dt, run.c)
+    placed = run.time
+    arrived = None
+    for _ in range(100):
+        if arrived is None and run.source(0, (st["px"][0], st["py"][0], st["pz"][0])) is not None:
+            arrived = run.time
+        run.step()
+    assert arrived is not None and abs(arrived - placed - 0.2) <= 2 * run.dt, (arrived, placed)
+
+
+def binary_drift(gravity, orbits):
+    """retarded_test.go:75-97: an equal-mass hole binary on a circular orbit
+    at v ~ 0.1 c; the relative change of its separation averaged over the
+    last orbit."""
+    mh, d, c = 0.5, 1.0, 5.0
+    v = math.sqrt(mh / (2 * d))
+    st, m, kind, alive = bodies([
+        {"pos": (-d / 2, 0, 0), "vel": proper_velocity((0, -v, 0), c), "m": mh, "kind": ics.KIND_HOLE},
+        {"pos": (d / 2, 0, 0), "vel": proper_velocity((0, v, 0), c), "m": mh, "kind": ics.KIND_HOLE}])
+    period = 2 * math.pi * (d / 2) / v
+    run = Run(st, m, kind, alive, period / 1000, 0.001, c, gravity)
+    total = 0.0
+    for k in range(orbits * 1000):
+        run.step()
+        if k >= (orbits - 1) * 1000:
+            total += math.dist((st["px"][0], st["py"][0], st["pz"][0]), (st["px"][1], st["py"][1], st["pz"][1]))
+    return total / 1000 / d - 1
+
+
+def test_naive_retardation_destabilises_orbits_and_extrapolation_suppresses_it():
+    """retarded_test.go:99-121 (the reference measured +0.5 %, +57 %, +4.3 %
+    after one orbit)."""
+    instant1, instant20 = binary_drift("instant", 1), binary_drift("instant", 20)
+    naive1 = binary_drift("retarded", 1)
+    extrap1 = binary_drift("retarded+v", 1)
+    assert abs(instant20 - instant1) <= 1e-3, (instant1, instant20)
+    assert naive1 >= 0.2, naive1
+    assert abs(extrap1) <= naive1 / 10, (extrap1, naive1)
+
+
+def test_retarded_galaxy_stays_bound():
+    """retarded_test.go:133-150: 200 stars on circular orbits around a hole,
+    extrapolated retardation, 2,000 steps."""
+    spec = [{"m": 1.0, "kind": ics.KIND_HOLE}]
+    for i in range(200):
+        r = 0.3 + 0.01 * i
+        phi = i * 2.4
+        v = math.sqrt(1 / r)
+        spec.append({"pos": (r * math.cos(phi), r * math.sin(phi), 0), "vel": (-v * math.sin(phi), v * math.cos(phi), 0),
+                     "m": 1e-6})
+    st, m, kind, alive = bodies(spec)
+    run = Run(st, m, kind, alive, 0.002, 0.01, 10.0, "retarded+v", theta=1e-6)
+    for _ in range(2000):
+        run.step()
+    r = np.sqrt((st["px"][1:] - st["px"][0]) ** 2 + (st["py"][1:] - st["py"][0]) ** 2 + (st["pz"][1:] - st["pz"][0]) ** 2)
+    assert r.min() >= 0.15 and r.max() <= 5, (r.min(), r.max())
+    assert len(run.r) == 1
+
+
+def test_instant_mode_and_newtonian_keep_no_history_and_match_the_plain_step():
+    s0 = ics.plummer(300, 5)
+    m, kind, alive = s0["m"].copy(), s0["kind"].copy(), np.ones(300, np.uint8)
+    kind[[0, 1]] = ics.KIND_HOLE
+    m[[0, 1]] = [0.05, 0.03]
+    for gravity, c in (("instant", 10.0), ("retarded", 0.0)):
+        a = {k: s0[k].copy() for k in KEYS}
+        b = {k: s0[k].copy() for k in KEYS}
+        o = oracle.Solver(theta=0.5)
+        r = oracle.Retard()
+        for k in range(3):
+            oracle.split_step(o, a, m, kind, alive, 0.01, 1e-2, c=c, acc_valid=k > 0)
+            oracle.split_step(o, b, m, kind, alive, 0.01, 1e-2, c=c, acc_valid=k > 0, retard=r, time=k * 1e-2,
+                              gravity=gravity)
+        for k in KEYS:
+            assert np.array_equal(a[k], b[k]), (gravity, k)
+        assert len(r) == 0
