@@ -33,7 +33,11 @@ EXPORTS = (
     "gx_hydro_new", "gx_hydro_free", "gx_hydro_config", "gx_hydro_set_stream", "gx_hydro_forces_dev",
     "gx_hydro_forces", "gx_hydro_get_stats", "gx_hydro_phase_ms", "gx_hydro_export",
     "gx_split_step_spin_dev", "gx_drag_outer_dev", "gx_drag_fast_dev", "gx_drag_get_stats",
+    "gx_retard_new", "gx_retard_free", "gx_retard_count", "gx_retard_hole_appeared_dev", "gx_retard_source_pos_dev",
+    "gx_split_step_retarded_dev",
 )
+
+GX_GRAVITY = {"instant": 0, "retarded": 1, "retarded+v": 2}  # sim.GravityMode, internal/sim/retarded.go:21-28
 
 GX_BLOCK_MAX_LEVELS = 8
 GX_BLOCK_MAX_TICKS = 128
@@ -134,6 +138,15 @@ def load():
                                     ctypes.POINTER(ctypes.c_int32)]
     L.gx_split_step_spin_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, P, P, P, I64, D, I, D, D, I, P, P, P,
                                          ctypes.POINTER(ctypes.c_int32)]
+    L.gx_split_step_retarded_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, P, P, P, P, I64, D, I, D, D, I, P, P, P,
+                                             P, D, I, ctypes.POINTER(ctypes.c_int32)]
+    L.gx_retard_new.restype = P
+    L.gx_retard_new.argtypes = []
+    L.gx_retard_free.argtypes = [P]
+    L.gx_retard_free.restype = None
+    L.gx_retard_count.argtypes = [P]
+    L.gx_retard_hole_appeared_dev.argtypes = [P, P, P, P, P, P, P, P, P, I64, I64, D, D, D]
+    L.gx_retard_source_pos_dev.argtypes = [P, P, P, P, P, P, P, P, I64, I64, P, P, P, I64, D, D, D, I, P, P, P, P]
     L.gx_drag_outer_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, P, P, P, P, I64, D, D, D]
     L.gx_drag_fast_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, P, I64, P, P, P, I64, D, D, D]
     L.gx_drag_get_stats.argtypes = [P, ctypes.POINTER(GxDragStats)]

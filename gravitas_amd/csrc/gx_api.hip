@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <new>
 #include <vector>
 
 #include "gravitas_hip.h"
@@ -129,6 +130,25 @@ struct gx_solver {
     // captured KDK step
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
+};
+
+// The holes' histories for retarded gravity (Simulation.histories,
+// internal/sim/retarded.go:62-73, 163-209), keyed by body ID: each a ring of
+// kHistLen outer steps (x, y, z, vx, vy, vz; 384 KiB) in device memory, its
+// header (head, n, t0, appeared, dt) on the host.
+struct gx_retard {
+    struct Hist {
+        uint64_t id;
+        double* ring;
+        double t0, appeared, dt;
+        int n, head;
+    };
+    std::vector<Hist> hists;
+    DevBuf bound;   // HistDev per hole slot of the current step (bindHistories)
+    DevBuf pushes;  // HistPush records of one push kernel
+    DevBuf ids;     // IDs of the alive holes, slot order
+    std::vector<uint64_t> holeIds;
+    std::vector<int32_t> holeIdx;
 };
 
 extern "C" {
@@ -944,18 +964,147 @@ int drag_outer(gx_solver* s, const DragArgs& a, long long count, double* vx, dou
     return drag_pass(s, a, (const int32_t*)s->dragList.p, count, vx, vy, vz, jx, jy, jz, b);
 }
 
+// holeHistory.push, retarded.go:75-83, the header part: the ring slot that the
+// new entry goes to.
+int hist_advance(gx_retard::Hist& h, double t) {
+    h.head = (h.head + 1) % kHistLen;
+    if (h.n < kHistLen) h.n++;
+    h.t0 = t - (double)(h.n - 1) * h.dt;
+    return h.head;
+}
+
+gx_retard::Hist* hist_find(gx_retard* r, uint64_t id) {
+    for (auto& h : r->hists)
+        if (h.id == id) return &h;
+    return nullptr;
+}
+
+// history, retarded.go:163-179: a new, empty history (the caller pushes the
+// current state as its first entry).
+int hist_create(gx_retard* r, uint64_t id, double dt, gx_retard::Hist** out) {
+    gx_retard::Hist h{id, nullptr, 0.0, -INFINITY, dt, 0, 0};
+    HIPCHK(hipMalloc(&h.ring, (size_t)6 * kHistLen * sizeof(double)));
+    r->hists.push_back(h);
+    *out = &r->hists.back();
+    return GX_OK;
+}
+
+void hists_clear(gx_retard* r) {
+    for (auto& h : r->hists) hipFree(h.ring);
+    r->hists.clear();
+}
+
+// Writes the listed pushes into the rings (k_retard_push) and waits.
+int run_pushes(gx_solver* s, gx_retard* r, const std::vector<HistPush>& p, const double* x, const double* y,
+               const double* z, const double* vx, const double* vy, const double* vz, double c) {
+    if (p.empty()) return GX_OK;
+    int rc;
+    if ((rc = ensure(r->pushes, p.size() * sizeof(HistPush)))) return rc;
+    HIPCHK(hipMemcpyAsync(r->pushes.p, p.data(), p.size() * sizeof(HistPush), hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_retard_push, dim3((unsigned)std::min<size_t>((p.size() + 63) / 64, 1024)), dim3(64), 0,
+                       s->stream, (const HistPush*)r->pushes.p, (int)p.size(), x, y, z, vx, vy, vz, c);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return GX_OK;
+}
+
+// The alive holes of the last collect (slot order) and their IDs, to the host.
+int fetch_holes(gx_solver* s, gx_retard* r, long long nh, const uint64_t* ids) {
+    r->holeIdx.resize((size_t)nh);
+    r->holeIds.resize((size_t)nh);
+    if (nh == 0) return GX_OK;
+    int rc;
+    if ((rc = ensure(r->ids, (size_t)nh * 8))) return rc;
+    hipLaunchKernelGGL(k_gather_hole_ids, dim3((unsigned)std::min<long long>(blocks(nh, 256), 1024)), dim3(256), 0,
+                       s->stream, (const int32_t*)s->holes.p, nh, ids, (uint64_t*)r->ids.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(r->holeIdx.data(), s->holes.p, (size_t)nh * 4, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(r->holeIds.data(), r->ids.p, (size_t)nh * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return GX_OK;
+}
+
+bool retarded(const gx_retard* r, int mode, double c) { return r && mode != GX_GRAVITY_INSTANT && c > 0; }
+
+// bindHistories, retarded.go:106-116: the history of every hole slot (after
+// fetch_holes) for the kernels.  Instant gravity: rd.hist stays null.
+int bind_histories(gx_solver* s, gx_retard* r, long long nh, int mode, double time, double dt, double c,
+                   RetardDev* rd) {
+    rd->hist = nullptr;
+    rd->mode = mode;
+    rd->time = time;
+    rd->dt = dt;
+    if (!retarded(r, mode, c) || nh == 0) return GX_OK;
+    std::vector<HistDev> hd((size_t)nh);
+    for (long long k = 0; k < nh; k++) {
+        const gx_retard::Hist* h = hist_find(r, r->holeIds[(size_t)k]);
+        hd[(size_t)k] = h ? HistDev{h->ring, h->t0, h->appeared, h->dt, h->n, h->head}
+                          : HistDev{nullptr, 0.0, 0.0, 0.0, 0, 0};
+    }
+    int rc;
+    if ((rc = ensure(r->bound, hd.size() * sizeof(HistDev)))) return rc;
+    HIPCHK(hipMemcpyAsync(r->bound.p, hd.data(), hd.size() * sizeof(HistDev), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    rd->hist = (const HistDev*)r->bound.p;
+    return GX_OK;
+}
+
+// recordHoles, retarded.go:181-209, at the time after the step: every alive
+// hole (the slots of fetch_holes) appends its state to its history, a hole
+// without one starts it; histories of holes gone are dropped.  Outside the
+// retarded modes the histories are cleared.
+int record_holes(gx_solver* s, gx_retard* r, const double* x, const double* y, const double* z, const double* vx,
+                 const double* vy, const double* vz, double time, double dt, double c, int mode) {
+    if (!r) return GX_OK;
+    if (!retarded(r, mode, c)) {
+        hists_clear(r);
+        return GX_OK;
+    }
+    const size_t nh = r->holeIds.size();
+    const bool seen = r->hists.size() != nh;
+    std::vector<HistPush> p;
+    int rc;
+    for (size_t k = 0; k < nh; k++) {
+        gx_retard::Hist* h = hist_find(r, r->holeIds[k]);
+        if (!h && (rc = hist_create(r, r->holeIds[k], dt, &h))) return rc;
+        p.push_back(HistPush{h->ring, r->holeIdx[k], hist_advance(*h, time)});
+    }
+    if ((rc = run_pushes(s, r, p, x, y, z, vx, vy, vz, c))) return rc;
+    if (seen) {  // drop histories of holes that merged away
+        size_t keep = 0;
+        for (size_t j = 0; j < r->hists.size(); j++) {
+            const bool alive = std::find(r->holeIds.begin(), r->holeIds.end(), r->hists[j].id) != r->holeIds.end();
+            if (alive)
+                r->hists[keep++] = r->hists[j];
+            else
+                hipFree(r->hists[j].ring);
+        }
+        r->hists.resize(keep);
+    }
+    return GX_OK;
+}
+
 // One step with black holes, stepSplit (internal/sim/split.go:164-199) for a
-// state without gas and with instant gravity; without alive holes it is
-// step()'s plain KDK (sim.go:266-277).  With c > 0 the pairs with holes use
+// state without gas; without alive holes it is step()'s plain KDK
+// (sim.go:266-277).  With a history set r in a retarded mode the holes pull
+// from their sources on the light cone (retarded.go), and the holes are
+// recorded at time + dt after the step (sim.go:285-287).  With c > 0 the pairs with holes use
 // Paczynski-Wiita radii and drifts use the proper velocity, and with spins
 // (jx non-null) the holes drag frames (framedrag.go) around the outer kicks
 // and the substeps' drifts.
 int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, double* jx, double* jy, double* jz,
-               double dt, double c, double eps, int accValid, int32_t* substepsOut) {
+               double dt, double c, double eps, int accValid, int32_t* substepsOut, gx_retard* retard = nullptr,
+               const uint64_t* ids = nullptr, double time = 0, int mode = GX_GRAVITY_INSTANT) {
     const long long n = a.n;
     int sub = 1;
     if (substepsOut) *substepsOut = 1;
-    if (n == 0) return GX_OK;
+    if (n == 0) {
+        if (retard) {
+            retard->holeIds.clear();
+            retard->holeIdx.clear();
+        }
+        return record_holes(s, retard, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, time + dt, dt, c, mode);
+    }
     double* x = (double*)a.x;
     double* y = (double*)a.y;
     double* z = (double*)a.z;
@@ -978,6 +1127,10 @@ int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, dou
     const long long nh = s->hmeta->nholes;
     if ((rc = ensure(s->partial, (size_t)std::max<long long>(nh, 1) * farSlices * 24))) return rc;
     double* partial = (double*)s->partial.p;
+    RetardDev rd{nullptr, mode, time, dt};
+    if (retard && ((rc = fetch_holes(s, retard, retarded(retard, mode, c) ? nh : 0, ids)) ||
+                   (rc = bind_histories(s, retard, nh, mode, time, dt, c, &rd))))
+        return rc;
     if (nh == 0) {
         // kick, drift (sim.go:316-343: dt / sqrt(1 + u^2 / c^2) with c > 0), forces, kick
         if ((rc = kick(s, vx, vy, vz, a.ax, a.ay, a.az, n, 0.5 * dt))) return rc;
@@ -986,7 +1139,7 @@ int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, dou
         HIPCHK(hipGetLastError());
         if ((rc = run_dev(s, a)) || (rc = kick(s, vx, vy, vz, a.ax, a.ay, a.az, n, 0.5 * dt))) return rc;
         HIPCHK(hipStreamSynchronize(st));
-        return GX_OK;
+        return record_holes(s, retard, x, y, z, vx, vy, vz, time + dt, dt, c, mode);
     }
     const Meta* meta = (const Meta*)s->meta.p;
     const int32_t* holes = (const int32_t*)s->holes.p;
@@ -1045,26 +1198,26 @@ int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, dou
     };
     auto outer = [&]() {  // outerHoleForces, split.go:82-117
         hipLaunchKernelGGL(k_split_far, dim3(grid), dim3(256), 0, st, n, (const double*)x, (const double*)y,
-                           (const double*)z, a.m, a.kind, (const uint8_t*)nearFlag, holes, meta, eps2, c, farX, farY,
-                           farZ);
+                           (const double*)z, a.m, a.kind, (const uint8_t*)nearFlag, holes, meta, eps2, c, rd, farX,
+                           farY, farZ);
         hipLaunchKernelGGL(k_split_hole_sum, dim3((unsigned)nh, (unsigned)farSlices), dim3(256), 0, st, n,
                            (const double*)x, (const double*)y, (const double*)z, a.m, a.kind, a.dmOff,
                            (const uint8_t*)nearFlag, (const int32_t*)nearList, (const long long*)nnDev, holes, meta,
                            eps2, c, 0, partial);
         hipLaunchKernelGGL(k_split_hole_finish, dim3(1), dim3(64), 0, st, (const double*)x, (const double*)y,
-                           (const double*)z, a.m, a.kind, holes, meta, eps2, c, 0, farSlices,
+                           (const double*)z, a.m, a.kind, holes, meta, eps2, c, rd, 0, farSlices,
                            (const double*)partial, farX, farY, farZ);
     };
     auto fast = [&]() {  // fastForces, split.go:141-160
         hipLaunchKernelGGL(k_split_fast, dim3(grid), dim3(256), 0, st, (const int32_t*)nearList,
                            (const long long*)nnDev, (const double*)x, (const double*)y, (const double*)z, a.m, a.kind,
-                           holes, meta, eps2, c, fastX, fastY, fastZ);
+                           holes, meta, eps2, c, rd, fastX, fastY, fastZ);
         hipLaunchKernelGGL(k_split_hole_sum, dim3((unsigned)nh, 1), dim3(256), 0, st, n, (const double*)x,
                            (const double*)y, (const double*)z, a.m, a.kind, a.dmOff, (const uint8_t*)nearFlag,
                            (const int32_t*)nearList, (const long long*)nnDev, holes, meta, eps2, c, 1, partial);
         hipLaunchKernelGGL(k_split_hole_finish, dim3(1), dim3(64), 0, st, (const double*)x, (const double*)y,
-                           (const double*)z, a.m, a.kind, holes, meta, eps2, c, 1, 1, (const double*)partial, fastX,
-                           fastY, fastZ);
+                           (const double*)z, a.m, a.kind, holes, meta, eps2, c, rd, 1, 1, (const double*)partial,
+                           fastX, fastY, fastZ);
     };
     auto kick_outer = [&](double h) {  // kickOuter, split.go:201-208
         hipLaunchKernelGGL(k_split_kick_outer, dim3(grid), dim3(256), 0, st, n, vx, vy, vz, (const double*)a.ax,
@@ -1104,7 +1257,7 @@ int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, dou
     kick_outer(0.5 * dt);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
-    return GX_OK;
+    return record_holes(s, retard, x, y, z, vx, vy, vz, time + dt, dt, c, mode);  // sim.go:285-287
 }
 
 }  // namespace
@@ -1136,6 +1289,88 @@ int gx_split_step_spin_dev(gx_solver* s, double* x, double* y, double* z, double
         (rc = set_single(s, n, s->theta, eps)))
         return rc;
     return split_step(s, a, vx, vy, vz, n > 0 ? spin_x : nullptr, spin_y, spin_z, dt, c, eps, acc_valid, substeps);
+}
+
+int gx_split_step_retarded_dev(gx_solver* s, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                               const double* mass, const uint8_t* kind, const uint8_t* alive, double* spin_x,
+                               double* spin_y, double* spin_z, const uint64_t* ids, int64_t n, double eps,
+                               int dark_matter_off, double dt, double c, int acc_valid, double* ax, double* ay,
+                               double* az, gx_retard* r, double time, int gravity, int32_t* substeps) {
+    if (!s) return set_err(GX_ERR_ARG, "null solver");
+    if (gravity < GX_GRAVITY_INSTANT || gravity > GX_GRAVITY_RETARDED_V)
+        return set_err(GX_ERR_ARG, "unknown gravity mode %d (want instant, retarded, retarded+v)", gravity);
+    const StepArgs a = step_args(x, y, z, mass, kind, alive, n, dark_matter_off, ax, ay, az);
+    int rc;
+    if ((rc = check_arrays(a)) || (n > 0 && (!vx || !vy || !vz) && (rc = set_err(GX_ERR_ARG, "null array"))) ||
+        (n > 0 && spin_x && (!spin_y || !spin_z) && (rc = set_err(GX_ERR_ARG, "null spin array"))) ||
+        (rc = set_single(s, n, s->theta, eps)))
+        return rc;
+    return split_step(s, a, vx, vy, vz, n > 0 ? spin_x : nullptr, spin_y, spin_z, dt, c, eps, acc_valid, substeps, r,
+                      ids, time, gravity);
+}
+
+gx_retard* gx_retard_new(void) { return new (std::nothrow) gx_retard(); }
+
+void gx_retard_free(gx_retard* r) {
+    if (!r) return;
+    hists_clear(r);
+    for (DevBuf* b : {&r->bound, &r->pushes, &r->ids})
+        if (b->p) hipFree(b->p);
+    delete r;
+}
+
+int gx_retard_count(const gx_retard* r) { return r ? (int)r->hists.size() : 0; }
+
+int gx_retard_hole_appeared_dev(gx_solver* s, gx_retard* r, const double* x, const double* y, const double* z,
+                                const double* vx, const double* vy, const double* vz, const uint64_t* ids, int64_t n,
+                                int64_t i, double time, double dt, double c) {
+    if (!s || !r) return set_err(GX_ERR_ARG, "null solver or history set");
+    if (i < 0 || i >= n) return set_err(GX_ERR_ARG, "body %lld out of range [0, %lld)", (long long)i, (long long)n);
+    if (!x || !y || !z || !vx || !vy || !vz) return set_err(GX_ERR_ARG, "null array");
+    uint64_t id = (uint64_t)i;
+    if (ids) {
+        HIPCHK(hipMemcpyAsync(&id, ids + i, 8, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    gx_retard::Hist* h = hist_find(r, id);
+    int rc;
+    if (!h) {
+        if ((rc = hist_create(r, id, dt, &h))) return rc;
+        const std::vector<HistPush> p{HistPush{h->ring, (int32_t)i, hist_advance(*h, time)}};
+        if ((rc = run_pushes(s, r, p, x, y, z, vx, vy, vz, c))) return rc;
+    }
+    h->appeared = time;
+    return GX_OK;
+}
+
+int gx_retard_source_pos_dev(gx_solver* s, gx_retard* r, const double* x, const double* y, const double* z,
+                             const uint8_t* kind, const uint8_t* alive, const uint64_t* ids, int64_t n, int64_t k,
+                             const double* obs_x, const double* obs_y, const double* obs_z, int64_t m, double time,
+                             double dt, double c, int gravity, double* src_x, double* src_y, double* src_z,
+                             uint8_t* visible) {
+    if (!s || !r) return set_err(GX_ERR_ARG, "null solver or history set");
+    if (gravity < GX_GRAVITY_INSTANT || gravity > GX_GRAVITY_RETARDED_V)
+        return set_err(GX_ERR_ARG, "unknown gravity mode %d", gravity);
+    if (n <= 0 || !x || !y || !z || !kind || !alive) return set_err(GX_ERR_ARG, "no bodies");
+    if (m < 0 || (m > 0 && (!obs_x || !obs_y || !obs_z || !src_x || !src_y || !src_z || !visible)))
+        return set_err(GX_ERR_ARG, "null observer array");
+    StepArgs a = step_args(x, y, z, nullptr, kind, alive, n, 0, nullptr, nullptr, nullptr);
+    int rc;
+    if ((rc = set_single(s, n, s->theta, 0.0)) || (rc = reserve(s, n))) return rc;
+    enqueue_collect(s, a);
+    HIPCHK(hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const long long nh = s->hmeta->nholes;
+    if (k < 0 || k >= nh) return set_err(GX_ERR_ARG, "hole %lld out of range [0, %lld)", (long long)k, nh);
+    RetardDev rd{nullptr, gravity, time, dt};
+    if ((rc = fetch_holes(s, r, nh, ids)) || (rc = bind_histories(s, r, nh, gravity, time, dt, c, &rd))) return rc;
+    if (m == 0) return GX_OK;
+    hipLaunchKernelGGL(k_retard_source, dim3((unsigned)std::min<long long>(blocks(m, 256), 4096)), dim3(256), 0,
+                       s->stream, rd, (long long)k, (const int32_t*)s->holes.p, x, y, z, obs_x, obs_y, obs_z,
+                       (long long)m, c, src_x, src_y, src_z, visible);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return GX_OK;
 }
 
 // Shared set-up of the frame-dragging entry points: the alive holes of the

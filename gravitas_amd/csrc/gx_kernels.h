@@ -119,6 +119,36 @@ __global__ void k_scan_fill(const double* D, long long ne, int chains, int nseg,
                             double* S, int* changed);
 
 // gx_split.hip
+// Retarded gravity of black holes (internal/sim/retarded.go).  holeHistory
+// (:62-73) as the kernels read it: the header, kept by the host, and a ring of
+// kHistLen entries per coordinate in device memory (x, y, z, vx, vy, vz back
+// to back; the velocities are coordinate velocities u / gamma).
+constexpr int kHistLen = 8192;      // historyLen, retarded.go:50-52
+constexpr double kRetardTol = 1e-7;  // retardTolerance, :54-59
+constexpr int kRetardMaxIt = 64;    // retardMaxIterations
+struct HistDev {
+    const double* ring;  // nullptr: the hole has no history (bindHistories found none)
+    double t0, appeared, dt;
+    int n, head;
+};
+// The holes' sources during one step (sourcePos, retarded.go:118-155).
+struct RetardDev {
+    const HistDev* hist;  // per hole slot (m.holes order); nullptr: instant gravity
+    int mode;             // 1 GravityRetarded, 2 GravityRetardedExtrapolated
+    double time, dt;      // Simulation.Time during the step, Params.Dt
+};
+// One push (holeHistory.push, retarded.go:75-83) of body `body` into slot
+// `pos` of a ring.
+struct HistPush {
+    double* ring;
+    int32_t body, pos;
+};
+__global__ void k_retard_push(const HistPush* p, int cnt, const double* x, const double* y, const double* z,
+                              const double* vx, const double* vy, const double* vz, double c);
+__global__ void k_retard_source(RetardDev r, long long k, const int32_t* holes, const double* x, const double* y,
+                                const double* z, const double* ox, const double* oy, const double* oz, long long m,
+                                double c, double* sx, double* sy, double* sz, uint8_t* visible);
+__global__ void k_gather_hole_ids(const int32_t* holes, long long nh, const uint64_t* ids, uint64_t* out);
 __global__ void k_split_classify(long long n, const double* x, const double* y, const double* z, const double* mass,
                                  const uint8_t* kind, const int32_t* holes, const Meta* meta, double eps2, double c,
                                  double limit, uint8_t* nearFlag, unsigned long long* tauEnc);
@@ -127,18 +157,19 @@ __global__ void k_flag_scan(int nb, int* tileCnt, long long* total);
 __global__ void k_flag_scatter(long long n, const uint8_t* flag, const int* tileOff, int32_t* list);
 __global__ void k_split_far(long long n, const double* x, const double* y, const double* z, const double* mass,
                             const uint8_t* kind, const uint8_t* nearFlag, const int32_t* holes, const Meta* meta,
-                            double eps2, double c, double* fx, double* fy, double* fz);
+                            double eps2, double c, RetardDev r, double* fx, double* fy, double* fz);
 __global__ void k_split_fast(const int32_t* nearList, const long long* nn, const double* x, const double* y,
                              const double* z, const double* mass, const uint8_t* kind, const int32_t* holes,
-                             const Meta* meta, double eps2, double c, double* fx, double* fy, double* fz);
+                             const Meta* meta, double eps2, double c, RetardDev r, double* fx, double* fy,
+                             double* fz);
 __global__ void k_split_hole_sum(long long n, const double* x, const double* y, const double* z, const double* mass,
                                  const uint8_t* kind, int dmOff, const uint8_t* nearFlag, const int32_t* nearList,
                                  const long long* nn, const int32_t* holes, const Meta* meta, double eps2, double c,
                                  int nearMode, double* partial);
 __global__ void k_split_hole_finish(const double* x, const double* y, const double* z, const double* mass,
                                     const uint8_t* kind, const int32_t* holes, const Meta* meta, double eps2, double c,
-                                    int nearMode, int slices, const double* partial, double* fx, double* fy,
-                                    double* fz);
+                                    RetardDev r, int nearMode, int slices, const double* partial, double* fx,
+                                    double* fy, double* fz);
 __global__ void k_split_kick_outer(long long n, double* vx, double* vy, double* vz, const double* ax, const double* ay,
                                    const double* az, const double* fx, const double* fy, const double* fz, double h);
 __global__ void k_split_kick_list(const int32_t* list, long long cntHost, const long long* cntDev, double* vx,

@@ -3,9 +3,11 @@
 // the tree once per step; every force involving a hole is a direct sum here.
 // Stars whose timescale at a hole is below 20 outer steps ("near") and all
 // holes take power-of-two substeps; far stars receive hole forces at the
-// outer kicks.  Restated for states without gas, with non-spinning holes and
-// instant gravity; with c > 0 the pairs with holes use the Paczynski-Wiita
-// radius and drifts use u / gamma (relativity.go).
+// outer kicks.  Restated for states without gas; with c > 0 the pairs with
+// holes use the Paczynski-Wiita radius and drifts use u / gamma
+// (relativity.go), and in the retarded gravity modes (retarded.go) a body
+// feels each hole where it was on the body's past light cone, read from the
+// hole's history of outer steps.
 //
 // Compiled with -ffp-contract=off: per-body sums over the holes run in the
 // hole order of the reference and are bit-identical to it; sums over bodies
@@ -17,11 +19,89 @@
 
 namespace gx {
 
-// holePull, split.go:120-137: pull of every hole except i itself on body i.
+// holeHistory.at, retarded.go:85-103: the hole's position at time t,
+// interpolated linearly between the two outer steps around t, or extrapolated
+// from the oldest entry before it; the velocity too if vel is non-null.
+// False if the hole had not appeared yet at t.
+__device__ __forceinline__ bool hist_at(const HistDev& h, double t, double* pos, double* vel) {
+    if (t < h.appeared) return false;
+    const double* rx = h.ring;
+    const double* ry = rx + kHistLen;
+    const double* rz = ry + kHistLen;
+    const double* rvx = rz + kHistLen;
+    const double* rvy = rvx + kHistLen;
+    const double* rvz = rvy + kHistLen;
+    const int oldest = (h.head - h.n + 1 + kHistLen) % kHistLen;
+    if (t <= h.t0 || h.n == 1) {
+        const double d = t - h.t0;
+        const double ux = rvx[oldest], uy = rvy[oldest], uz = rvz[oldest];
+        pos[0] = rx[oldest] + d * ux;
+        pos[1] = ry[oldest] + d * uy;
+        pos[2] = rz[oldest] + d * uz;
+        if (vel) {
+            vel[0] = ux;
+            vel[1] = uy;
+            vel[2] = uz;
+        }
+        return true;
+    }
+    const double f = (t - h.t0) / h.dt;
+    int k = (int)f;
+    if (k > h.n - 2) k = h.n - 2;
+    const double w = f - (double)k;
+    const int a = (oldest + k) % kHistLen, b = (oldest + k + 1) % kHistLen;
+    pos[0] = rx[a] + w * (rx[b] - rx[a]);
+    pos[1] = ry[a] + w * (ry[b] - ry[a]);
+    pos[2] = rz[a] + w * (rz[b] - rz[a]);
+    if (vel) {
+        vel[0] = rvx[a] + w * (rvx[b] - rvx[a]);
+        vel[1] = rvy[a] + w * (rvy[b] - rvy[a]);
+        vel[2] = rvz[a] + w * (rvz[b] - rvz[a]);
+    }
+    return true;
+}
+
+// sourcePos, retarded.go:118-155, in a retarded mode (r.hist non-null): where
+// a body at (x, y, z) sees hole slot k, whose current position (hx, hy, hz)
+// is replaced by the source; false if its gravity has not arrived yet.  The
+// light-cone delay is a fixed point iterated to 1e-7 time units (at most 64
+// times); the extrapolated mode adds delay * v.
+__device__ __forceinline__ bool source_pos(const RetardDev& r, long long k, double x, double y, double z, double c,
+                                        double& hx, double& hy, double& hz) {
+    const HistDev h = r.hist[k];
+    double delay = sqrt((x - hx) * (x - hx) + (y - hy) * (y - hy) + (z - hz) * (z - hz)) / c;
+    if (!h.ring || delay < r.dt) return true;  // within one step: below the step resolution
+    double p[3], v[3];
+    for (int it = 0; it < kRetardMaxIt; it++) {
+        if (!hist_at(h, r.time - delay, p, nullptr)) {
+            hx = hy = hz = 0;
+            return false;
+        }
+        const double next = sqrt((x - p[0]) * (x - p[0]) + (y - p[1]) * (y - p[1]) + (z - p[2]) * (z - p[2])) / c;
+        const bool converged = fabs(next - delay) < kRetardTol;
+        delay = next;
+        if (converged) break;
+    }
+    hist_at(h, r.time - delay, p, v);
+    if (r.mode == 2) {
+        p[0] = p[0] + delay * v[0];
+        p[1] = p[1] + delay * v[1];
+        p[2] = p[2] + delay * v[2];
+    }
+    hx = p[0];
+    hy = p[1];
+    hz = p[2];
+    return true;
+}
+
+// holePull, split.go:120-137: pull of every hole except i itself on body i,
+// from where the body sees it (sourcePos; the hole itself with instant
+// gravity).
 __device__ __forceinline__ void hole_pull(long long i, const double* __restrict__ x, const double* __restrict__ y,
                                           const double* __restrict__ z, const double* __restrict__ mass,
                                           const uint8_t* __restrict__ kind, const int32_t* __restrict__ holes,
-                                          long long nh, double eps2, double c, double& ax, double& ay, double& az) {
+                                          long long nh, double eps2, double c, const RetardDev& r, double& ax,
+                                          double& ay, double& az) {
     ax = ay = az = 0;
     const double xi = x[i], yi = y[i], zi = z[i];
     const int ki = kind[i];
@@ -29,7 +109,9 @@ __device__ __forceinline__ void hole_pull(long long i, const double* __restrict_
     for (long long k = 0; k < nh; k++) {
         const int32_t h = holes[k];
         if (h == i) continue;
-        const double dx = x[h] - xi, dy = y[h] - yi, dz = z[h] - zi;
+        double hx = x[h], hy = y[h], hz = z[h];
+        if (r.hist && !source_pos(r, k, xi, yi, zi, c, hx, hy, hz)) continue;  // not arrived yet
+        const double dx = hx - xi, dy = hy - yi, dz = hz - zi;
         const double rs = pair_rs(ki, mi, kHole, mass[h], c);
         const double f = pair_force(mass[h], dx * dx + dy * dy + dz * dz, eps2, rs);
         ax = ax + f * dx;
@@ -161,12 +243,12 @@ __global__ __launch_bounds__(256) void k_split_far(long long n, const double* __
                                                    const double* __restrict__ mass, const uint8_t* __restrict__ kind,
                                                    const uint8_t* __restrict__ nearFlag,
                                                    const int32_t* __restrict__ holes, const Meta* __restrict__ meta,
-                                                   double eps2, double c, double* __restrict__ fx,
+                                                   double eps2, double c, RetardDev r, double* __restrict__ fx,
                                                    double* __restrict__ fy, double* __restrict__ fz) {
     const long long nh = meta->nholes;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         double ax = 0, ay = 0, az = 0;
-        if (kind[i] != kHole && !nearFlag[i]) hole_pull(i, x, y, z, mass, kind, holes, nh, eps2, c, ax, ay, az);
+        if (kind[i] != kHole && !nearFlag[i]) hole_pull(i, x, y, z, mass, kind, holes, nh, eps2, c, r, ax, ay, az);
         fx[i] = ax;
         fy[i] = ay;
         fz[i] = az;
@@ -179,13 +261,13 @@ __global__ __launch_bounds__(256) void k_split_fast(const int32_t* __restrict__ 
                                                     const double* __restrict__ z, const double* __restrict__ mass,
                                                     const uint8_t* __restrict__ kind, const int32_t* __restrict__ holes,
                                                     const Meta* __restrict__ meta, double eps2, double c,
-                                                    double* __restrict__ fx, double* __restrict__ fy,
+                                                    RetardDev r, double* __restrict__ fx, double* __restrict__ fy,
                                                     double* __restrict__ fz) {
     const long long nh = meta->nholes, cnt = *nn;
     for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += (long long)gridDim.x * blockDim.x) {
         const int32_t i = nearList[j];
         double ax, ay, az;
-        hole_pull(i, x, y, z, mass, kind, holes, nh, eps2, c, ax, ay, az);
+        hole_pull(i, x, y, z, mass, kind, holes, nh, eps2, c, r, ax, ay, az);
         fx[i] = ax;
         fy[i] = ay;
         fz[i] = az;
@@ -245,14 +327,14 @@ __global__ __launch_bounds__(256) void k_split_hole_sum(long long n, const doubl
 __global__ void k_split_hole_finish(const double* __restrict__ x, const double* __restrict__ y,
                                     const double* __restrict__ z, const double* __restrict__ mass,
                                     const uint8_t* __restrict__ kind, const int32_t* __restrict__ holes,
-                                    const Meta* __restrict__ meta, double eps2, double c, int nearMode, int slices,
-                                    const double* __restrict__ partial, double* __restrict__ fx,
+                                    const Meta* __restrict__ meta, double eps2, double c, RetardDev r, int nearMode,
+                                    int slices, const double* __restrict__ partial, double* __restrict__ fx,
                                     double* __restrict__ fy, double* __restrict__ fz) {
     const long long nh = meta->nholes;
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nh; k += (long long)gridDim.x * blockDim.x) {
         const int32_t h = holes[k];
         double ax = 0, ay = 0, az = 0;
-        if (nearMode) hole_pull(h, x, y, z, mass, kind, holes, nh, eps2, c, ax, ay, az);
+        if (nearMode) hole_pull(h, x, y, z, mass, kind, holes, nh, eps2, c, r, ax, ay, az);
         double sx = 0, sy = 0, sz = 0;
         for (int j = 0; j < slices; j++) {
             sx += partial[(k * slices + j) * 3];
@@ -331,6 +413,57 @@ __global__ __launch_bounds__(256) void k_split_drift_far(long long n, const uint
         y[i] += f * uy;
         z[i] += f * uz;
     }
+}
+
+// holeHistory.push, retarded.go:75-83 (the ring part; the host keeps head, n
+// and t0): the position and coordinate velocity u / gamma
+// (CoordinateVelocity, relativity.go:29-32) of each listed body into its slot.
+__global__ __launch_bounds__(64) void k_retard_push(const HistPush* __restrict__ p, int cnt,
+                                                    const double* __restrict__ x, const double* __restrict__ y,
+                                                    const double* __restrict__ z, const double* __restrict__ vx,
+                                                    const double* __restrict__ vy, const double* __restrict__ vz,
+                                                    double c) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
+        const HistPush q = p[j];
+        const int32_t b = q.body;
+        const double ux = vx[b], uy = vy[b], uz = vz[b];
+        const double ig = 1 / gamma_of(ux, uy, uz, c);
+        double* ring = q.ring + q.pos;
+        ring[0] = x[b];
+        ring[kHistLen] = y[b];
+        ring[2 * kHistLen] = z[b];
+        ring[3 * kHistLen] = ux * ig;
+        ring[4 * kHistLen] = uy * ig;
+        ring[5 * kHistLen] = uz * ig;
+    }
+}
+
+// sourcePos of hole slot k for m observers (the test hook of the retarded
+// modes): (sx, sy, sz) and visible per observer.
+__global__ __launch_bounds__(256) void k_retard_source(RetardDev r, long long k, const int32_t* __restrict__ holes,
+                                                       const double* __restrict__ x, const double* __restrict__ y,
+                                                       const double* __restrict__ z, const double* __restrict__ ox,
+                                                       const double* __restrict__ oy, const double* __restrict__ oz,
+                                                       long long m, double c, double* __restrict__ sx,
+                                                       double* __restrict__ sy, double* __restrict__ sz,
+                                                       uint8_t* __restrict__ visible) {
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (long long)gridDim.x * blockDim.x) {
+        const int32_t h = holes[k];
+        double hx = x[h], hy = y[h], hz = z[h];
+        bool vis = true;
+        if (r.hist) vis = source_pos(r, k, ox[j], oy[j], oz[j], c, hx, hy, hz);
+        sx[j] = hx;
+        sy[j] = hy;
+        sz[j] = hz;
+        visible[j] = vis;
+    }
+}
+
+// The IDs of the alive holes (ids[holes[k]], or the state index without ids).
+__global__ void k_gather_hole_ids(const int32_t* __restrict__ holes, long long nh, const uint64_t* __restrict__ ids,
+                                  uint64_t* __restrict__ out) {
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nh; k += (long long)gridDim.x * blockDim.x)
+        out[k] = ids ? ids[holes[k]] : (uint64_t)holes[k];
 }
 
 }  // namespace gx

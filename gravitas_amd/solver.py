@@ -103,6 +103,37 @@ def _ptr(t):
     return t.ptr
 
 
+class Retard:
+    """The holes' histories of one simulation for retarded gravity
+    (Simulation.histories, internal/sim/retarded.go:62-209) in device memory,
+    kept across split steps."""
+
+    def __init__(self):
+        self._L = _lib.load()
+        h = self._L.gx_retard_new()
+        if not h:
+            raise MemoryError("gx_retard_new")
+        self._h = ctypes.c_void_p(h)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                self._L.gx_retard_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def __len__(self):
+        return self._L.gx_retard_count(self._h)
+
+    def hole_appeared(self, solver: "Solver", s: DeviceState, p: Params, i: int, time: float, ids=None):
+        """holeAppeared (retarded.go:157-161; AddBody of a hole, sim.go:236-243):
+        hole i, added at ``time``, starts its history with its current state."""
+        _lib.check(self._L.gx_retard_hole_appeared_dev(
+            solver._h, self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.vx), _ptr(s.vy), _ptr(s.vz),
+            None if ids is None else _ptr(ids), len(s), int(i), float(time), float(p.dt), float(p.c)))
+
+
 class Solver:
     """bh.Solver on the GPU.
 
@@ -258,24 +289,52 @@ class Solver:
                 "rung_count": list(st.rung_count)[:st.levels], "tick": list(st.tick)[:ev],
                 "active": list(st.active)[:ev]}
 
-    def split_step(self, s: DeviceState, p: Params, acc_valid: bool = True) -> int:
+    def split_step(self, s: DeviceState, p: Params, acc_valid: bool = True, retard: "Retard | None" = None,
+                   time: float = 0.0, gravity: str = "instant", ids=None) -> int:
         """One step of p.dt with black holes (internal/sim/split.go:164-199):
         tree forces between stars, direct sums for every pair with a hole,
         power-of-two substeps for the holes and the stars near them; p.c > 0
         switches on the Paczynski-Wiita pairs and relativistic drifts, and, if
-        the state carries spins, frame dragging (framedrag.go).  Returns the
-        number of substeps."""
+        the state carries spins, frame dragging (framedrag.go).  With
+        ``retard`` (a Retard) the holes pull in the ``gravity`` mode
+        (retarded.go: "instant", "retarded", "retarded+v") at simulation time
+        ``time``, and the holes are recorded at time + p.dt after the step
+        (sim.go:285-287); ``ids`` (a uint64 DeviceArray, or None: the state
+        index) are the bodies' IDs.  Returns the number of substeps."""
         self._configure()
         sub = ctypes.c_int32(0)
         head = (self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.vx), _ptr(s.vy), _ptr(s.vz), _ptr(s.m),
                 _ptr(s.kind), _ptr(s.alive))
         tail = (len(s), float(p.softening), int(p.dark_matter_off), float(p.dt), float(p.c), int(bool(acc_valid)),
                 _ptr(s.ax), _ptr(s.ay), _ptr(s.az), ctypes.byref(sub))
-        if s.has_spin:
+        if retard is not None:
+            spins = (_ptr(s.jx), _ptr(s.jy), _ptr(s.jz)) if s.has_spin else (None, None, None)
+            _lib.check(self._L.gx_split_step_retarded_dev(
+                *head, *spins, None if ids is None else _ptr(ids), *tail[:-1], retard._h, float(time),
+                _lib.GX_GRAVITY[gravity], tail[-1]))
+        elif s.has_spin:
             _lib.check(self._L.gx_split_step_spin_dev(*head, _ptr(s.jx), _ptr(s.jy), _ptr(s.jz), *tail))
         else:
             _lib.check(self._L.gx_split_step_dev(*head, *tail))
         return sub.value
+
+    def source_pos(self, retard: "Retard", s: DeviceState, p: Params, k: int, obs, time: float, gravity: str,
+                   ids=None):
+        """sourcePos (retarded.go:118-155) of alive hole number k for the
+        observers ``obs`` (host array, shape (m, 3)) at simulation time
+        ``time``: an (m, 3) array of source positions and a bool array, false
+        where the hole's gravity has not arrived."""
+        obs = np.ascontiguousarray(np.atleast_2d(np.asarray(obs, np.float64)))
+        m = len(obs)
+        o = [DeviceArray.from_numpy(np.ascontiguousarray(obs[:, j])) for j in range(3)]
+        out = [DeviceArray(m, np.float64) for _ in range(3)]
+        vis = DeviceArray(m, np.uint8)
+        self._configure()
+        _lib.check(self._L.gx_retard_source_pos_dev(
+            self._h, retard._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.kind), _ptr(s.alive),
+            None if ids is None else _ptr(ids), len(s), int(k), *(_ptr(a) for a in o), m, float(time), float(p.dt),
+            float(p.c), _lib.GX_GRAVITY[gravity], *(_ptr(a) for a in out), _ptr(vis)))
+        return np.stack([a.numpy() for a in out], axis=1), vis.numpy().astype(bool)
 
     def drag_outer(self, s: DeviceState, p: Params, dt: float, near=None):
         """Simulation.dragOuter (framedrag.go:115-127) over time dt: every

@@ -197,10 +197,10 @@ int gx_block_step_dev(gx_solver *s, double *x, double *y, double *z, double *vx,
  * (sim.go:266-277).  c is Params.C: <= 0 Newtonian; > 0 Paczynski-Wiita
  * radii for the pairs with holes and drifts with the proper velocity u/gamma
  * (relativity.go).  Restated for states without gas, with instant gravity
- * and non-spinning holes (gx_split_step_spin_dev adds the spins; the
- * reference's retarded sources are not part of this path).  If acc_valid is
- * 0 the forces are computed
- * first.  *substeps (optional) receives the substep count (Simulation.Substeps).
+ * and non-spinning holes (gx_split_step_spin_dev adds the spins,
+ * gx_split_step_retarded_dev retarded gravity).  If acc_valid is 0 the forces
+ * are computed first.  *substeps (optional) receives the substep count
+ * (Simulation.Substeps).
  * Synchronous. */
 int gx_split_step_dev(gx_solver *s, double *x, double *y, double *z, double *vx, double *vy, double *vz,
                       const double *mass, const uint8_t *kind, const uint8_t *alive, int64_t n, double eps,
@@ -217,6 +217,64 @@ int gx_split_step_spin_dev(gx_solver *s, double *x, double *y, double *z, double
                            const double *mass, const uint8_t *kind, const uint8_t *alive, double *spin_x,
                            double *spin_y, double *spin_z, int64_t n, double eps, int dark_matter_off, double dt,
                            double c, int acc_valid, double *ax, double *ay, double *az, int32_t *substeps);
+
+/* Retarded gravity of black holes (internal/sim/retarded.go).  The modes are
+ * sim.GravityMode (retarded.go:21-28): instant (Newtonian, infinite speed),
+ * retarded (a body feels each hole where it was at the retarded time on the
+ * body's past light cone) and retarded+v (that position extrapolated by the
+ * hole's velocity times the delay).  They act only with c > 0, and only on
+ * forces from holes; forces on holes stay instantaneous. */
+#define GX_GRAVITY_INSTANT 0
+#define GX_GRAVITY_RETARDED 1
+#define GX_GRAVITY_RETARDED_V 2
+
+/* The holes' histories of one simulation (Simulation.histories,
+ * retarded.go:62-73): per alive hole, keyed by body ID, a ring buffer of its
+ * position and coordinate velocity at the last 8,192 outer steps (384 KiB of
+ * device memory each).  Used from one thread, with one solver at a time. */
+typedef struct gx_retard gx_retard;
+
+/* An empty history set (NULL if out of memory). */
+gx_retard *gx_retard_new(void);
+/* Frees the set and its device buffers; NULL is ignored. */
+void gx_retard_free(gx_retard *r);
+/* Number of histories held (len(Simulation.histories)). */
+int gx_retard_count(const gx_retard *r);
+
+/* holeAppeared (retarded.go:157-161; Simulation.AddBody of a hole,
+ * sim.go:236-243): body i, a hole added at time, starts its history now with
+ * its current state (device arrays) as the first entry, unless it has one; its
+ * gravity spreads from there at speed c.  ids (device, may be NULL: the state
+ * index) are the bodies' IDs; dt is Params.Dt.  Synchronous. */
+int gx_retard_hole_appeared_dev(gx_solver *s, gx_retard *r, const double *x, const double *y, const double *z,
+                                const double *vx, const double *vy, const double *vz, const uint64_t *ids, int64_t n,
+                                int64_t i, double time, double dt, double c);
+
+/* sourcePos (retarded.go:118-155) of alive hole number k (state order) for
+ * the observers obs_x/y/z[0 .. m), all device arrays: where each sees the hole
+ * at simulation time `time`, in src_x/y/z, and visible[j] = 0 where its gravity
+ * has not arrived yet (src then 0).  With the histories of r bound as in a
+ * step.  Synchronous. */
+int gx_retard_source_pos_dev(gx_solver *s, gx_retard *r, const double *x, const double *y, const double *z,
+                             const uint8_t *kind, const uint8_t *alive, const uint64_t *ids, int64_t n, int64_t k,
+                             const double *obs_x, const double *obs_y, const double *obs_z, int64_t m, double time,
+                             double dt, double c, int gravity, double *src_x, double *src_y, double *src_z,
+                             uint8_t *visible);
+
+/* stepSplit with retarded hole gravity: as gx_split_step_spin_dev (spin_x/y/z
+ * may be NULL: no spins), and the pull of every hole on a body or on another
+ * hole (holePull, split.go:120-137) comes from where the body sees it
+ * (sourcePos, retarded.go:118-155) at simulation time `time`, from the
+ * histories of r (r NULL: instant gravity).  After the step the alive holes
+ * are recorded at time + dt (recordHoles, retarded.go:181-209, as Step does,
+ * sim.go:285-287); outside the retarded modes (gravity instant or c <= 0) the
+ * histories are dropped.  ids: device array of body IDs, NULL: the state
+ * index.  Synchronous. */
+int gx_split_step_retarded_dev(gx_solver *s, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                               const double *mass, const uint8_t *kind, const uint8_t *alive, double *spin_x,
+                               double *spin_y, double *spin_z, const uint64_t *ids, int64_t n, double eps,
+                               int dark_matter_off, double dt, double c, int acc_valid, double *ax, double *ay,
+                               double *az, gx_retard *r, double time, int gravity, int32_t *substeps);
 
 /* Simulation.dragOuter (internal/sim/framedrag.go:115-127) on device arrays:
  * if c > 0 and an alive hole spins, every alive body that is not a hole and
