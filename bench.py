@@ -18,7 +18,8 @@ clumps-16m (precision ladder FP64 / FP32 / FP32 with FP16-stored quadrupoles),
 batched-256 (256 systems in one arena, hipGraph KDK vs separate calls) and
 two-disk-4m (4,194,304-body encounter, block time steps on 4 rungs, FP16
 quadrupoles, FP64 energy drift over --steps global steps), disk-halo-1m-holes
-(split steps with three black holes), sph-sphere-1m (SPH hydro forces) and
+(split steps with three black holes), disk-halo-1m-holes-retarded (the same
+with retarded hole gravity at c = 10), sph-sphere-1m (SPH hydro forces) and
 spin-cluster-1m (split steps with a spinning hole and frame dragging).
 """
 from __future__ import annotations
@@ -48,7 +49,7 @@ FLOPS_SPH_PAIR = 62
 FLOPS_SPH_VISC = 19
 FLOPS_SPH_DENSITY = 20
 CONFIGS = ("disk-halo-1m", "plummer-16k", "clumps-16m", "batched-256", "two-disk-4m", "disk-halo-1m-holes",
-           "sph-sphere-1m", "spin-cluster-1m")
+           "disk-halo-1m-holes-retarded", "sph-sphere-1m", "spin-cluster-1m")
 
 
 def parse():
@@ -503,13 +504,29 @@ def bench_holes(args):
     circular orbits; steps of dt 2e-3 with the split scheme of
     internal/sim/split.go (tree forces between stars, direct sums for every
     pair with a hole, power-of-two substeps for the holes and the stars near
-    them).  Reports steps/s, the substep counts and the FP64 momentum drift."""
-    from gravitas_amd import DeviceState, Params, Solver, ics
+    them).  Reports steps/s, the substep counts and the FP64 momentum drift.
+
+    disk-halo-1m-holes-retarded: the same with c = 10 and retarded gravity of
+    the holes extrapolated by their velocity (retarded.go, "retarded+v"):
+    every hole pull comes from the hole's position on the body's past light
+    cone, read from its history of outer steps.  The warm-up steps fill the
+    histories (the timed steps read up to --warmup + --steps entries; older
+    retarded times extrapolate from the oldest one)."""
+    from gravitas_amd import DeviceState, Params, Retard, Solver, ics
     from gravitas_amd.hip import synchronize
     n = args.n or 1048576
     theta = 0.5 if args.theta is None else args.theta
     eps = 0.02 if args.eps is None else args.eps
     dt = 2e-3
+    retarded = args.config == "disk-halo-1m-holes-retarded"
+    c, gravity = (10.0, "retarded+v") if retarded else (0.0, "instant")
+    hist = Retard() if retarded else None
+    clock = [0.0]
+
+    def split(**kw):
+        sub = solver.split_step(ds, params, retard=hist, time=clock[0], gravity=gravity, **kw)
+        clock[0] += dt
+        return sub
     log(f"building disk+halo with holes n={n}")
     state = ics.disk_halo(n, seed=2, eps=eps)
     state["alive"] = np.ones(n, np.uint8)
@@ -525,23 +542,25 @@ def bench_holes(args):
         state["vy"][hh] = float(np.sqrt(state["m"][r < R].sum() / R))
     ds = DeviceState.from_host(state)
     solver = Solver(theta=theta, walk="fp32")
-    params = Params(softening=eps, dt=dt)
+    params = Params(softening=eps, dt=dt, c=c)
     d0 = solver.diagnostics(ds, params)
-    subs = [solver.split_step(ds, params, acc_valid=False)]
+    subs = [split(acc_valid=False)]
     for _ in range(max(0, args.warmup - 1)):
-        subs.append(solver.split_step(ds, params))
+        subs.append(split())
     synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        subs.append(solver.split_step(ds, params))
+        subs.append(split())
     synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     d1 = solver.diagnostics(ds, params)
     gross = float(np.abs(state["m"] * np.abs(state["vx"])).sum())
-    out = base_line(args, n, ms, "fp32", {
-        "workload": "disk-halo-1m-holes", "n": n, "theta": theta, "eps": eps, "dt": dt,
-        "holes": "0.1 at the centre, 0.02 at R = 2 and R = 4 (circular)",
-        "step": "split step: tree forces + hole direct sums + substeps near holes"})
+    cfg = {"workload": args.config, "n": n, "theta": theta, "eps": eps, "dt": dt,
+           "holes": "0.1 at the centre, 0.02 at R = 2 and R = 4 (circular)",
+           "step": "split step: tree forces + hole direct sums + substeps near holes"}
+    if retarded:
+        cfg.update({"c": c, "gravity": gravity, "history_entries": len(subs)})
+    out = base_line(args, n, ms, "fp32", cfg)
     out["substeps"] = subs[-args.steps:]
     out["momentum_drift_over_gross"] = float(np.abs(np.array(d1["momentum"]) - np.array(d0["momentum"])).max() / gross)
     return out
@@ -734,7 +753,8 @@ def main():
         sys.exit(1)
     fn = {"disk-halo-1m": bench_disk_halo, "plummer-16k": bench_plummer, "clumps-16m": bench_clumps,
           "batched-256": bench_batched, "two-disk-4m": bench_encounter,
-          "disk-halo-1m-holes": bench_holes, "sph-sphere-1m": bench_sph, "spin-cluster-1m": bench_spin}[args.config]
+          "disk-halo-1m-holes": bench_holes, "disk-halo-1m-holes-retarded": bench_holes, "sph-sphere-1m": bench_sph,
+          "spin-cluster-1m": bench_spin}[args.config]
     print(json.dumps(fn(args)), flush=True)
 
 

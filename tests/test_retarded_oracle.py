@@ -190,3 +190,48 @@ def test_instant_mode_and_newtonian_keep_no_history_and_match_the_plain_step():
         for k in KEYS:
             assert np.array_equal(a[k], b[k]), (gravity, k)
         assert len(r) == 0
+
+
+@pytest.mark.parametrize("gravity", ["retarded", "retarded+v"])
+def test_source_of_a_curving_hole_is_the_light_cone_point_of_its_interpolated_history(gravity):
+    """A hole of a binary moves on a circle, so the history's piecewise-linear
+    interpolation matters (for uniform motion any consistent interpolation is
+    exact).  The source must be the point where the light cone meets the
+    polygon through the recorded positions (holeHistory.at, retarded.go:85-103),
+    found here independently by bisection on |x - P(t)| - c (T - t) with numpy's
+    interp; the extrapolated mode adds delay times the interpolated coordinate
+    velocity u / gamma."""
+    mh, d, c = 0.5, 1.0, 5.0
+    v = math.sqrt(mh / (2 * d))
+    st, m, kind, alive = bodies([
+        {"pos": (-d / 2, 0, 0), "vel": proper_velocity((0, -v, 0), c), "m": mh, "kind": ics.KIND_HOLE},
+        {"pos": (d / 2, 0, 0), "vel": proper_velocity((0, v, 0), c), "m": mh, "kind": ics.KIND_HOLE}])
+    period = 2 * math.pi * (d / 2) / v
+    run = Run(st, m, kind, alive, period / 200, 0.001, c, gravity)
+    times, pos, vel = [], [], []
+    for _ in range(300):
+        run.step()
+        times.append(run.time)  # recordHoles after each step, at the new time
+        pos.append([st["px"][0], st["py"][0], st["pz"][0]])
+        u = np.array([st["vx"][0], st["vy"][0], st["vz"][0]])
+        vel.append(u / math.sqrt(1 + (u @ u) / (c * c)))
+    times, pos, vel = np.array(times), np.array(pos), np.array(vel)
+
+    def P(t, arr):
+        return np.array([np.interp(t, times, arr[:, j]) for j in range(3)])
+
+    for obs in ([2.5, 1.0, 0.5], [-1.5, -2.0, 1.0], [0.3, 3.2, -0.7]):
+        obs = np.array(obs)
+        lo, hi = times[0], run.time  # g(lo) > 0 > g(hi) is checked below
+        g = lambda t: np.linalg.norm(obs - P(t, pos)) - c * (run.time - t)  # noqa: E731
+        assert g(lo) < 0 < g(hi)
+        for _ in range(200):
+            mid = 0.5 * (lo + hi)
+            lo, hi = (mid, hi) if g(mid) < 0 else (lo, mid)
+        tr = 0.5 * (lo + hi)
+        want = P(tr, pos)
+        if gravity == "retarded+v":
+            want = want + (run.time - tr) * P(tr, vel)
+        got = run.source(0, obs)
+        assert got is not None
+        assert np.abs(got - want).max() <= 1e-6, (obs, got, want)
