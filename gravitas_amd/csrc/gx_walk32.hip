@@ -103,13 +103,21 @@ __device__ __forceinline__ uint64_t transpose_bits(uint64_t x) {
 // Cells accepted by some lanes only (mask of cell e held by lane e): each
 // lane loops over its own cells, so no lane slot is spent on a masked-off
 // star.  col = transpose of the masks.
+// The column is walked as two 32-bit words (cells 0-31, then 32-63): a 32-bit
+// find-first-bit, clear and test cost four VALU instructions per cell against
+// nine for the 64-bit word.
 template <bool QUAD>
 __device__ __forceinline__ void eval_cells_own(const float4* __restrict__ cell, uint64_t col, float px, float py,
                                                float pz, float eps2, float& ax, float& ay, float& az) {
-    while (col) {
-        const int e = __builtin_ctzll(col);
-        col &= col - 1;
-        cell_force<QUAD>(cell[3 * e], cell[3 * e + 1], cell[3 * e + 2], px, py, pz, eps2, ax, ay, az);
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+        uint32_t w = (uint32_t)(col >> (32 * half));
+        const float4* __restrict__ c = cell + 96 * half;
+        while (w) {
+            const int e = __builtin_ctz(w);
+            w &= w - 1;
+            cell_force<QUAD>(c[3 * e], c[3 * e + 1], c[3 * e + 2], px, py, pz, eps2, ax, ay, az);
+        }
     }
 }
 
@@ -140,25 +148,30 @@ __device__ __forceinline__ void eval_parts(const float4* __restrict__ part, int 
 }
 
 // Stars [first, first + npart) of the staging buffer, each for the lanes of
-// its leaf's mask pmask[]: blocks of 64 stars, masks transposed, and each
-// lane loops over its own stars.
+// its leaf's mask pmask[]: blocks of 32 stars, masks transposed (lanes 0-31
+// hold the rows, so every column fits in 32 bits), and each lane loops over
+// its own stars.  A 32-bit column costs four VALU instructions of loop
+// control per star against nine for a 64-bit one; the lane that does the most
+// work is mostly the same in every block, so smaller blocks add little
+// imbalance.
 template <bool EPS0>
 __device__ __forceinline__ void eval_parts_own(const float4* __restrict__ part, const uint64_t* __restrict__ pmask,
                                                int first, int npart, float px, float py, float pz, float eps2,
                                                float& ax, float& ay, float& az) {
+    constexpr int kBlk = 32;
     first = __builtin_amdgcn_readfirstlane(first);
     npart = __builtin_amdgcn_readfirstlane(npart);
     // block b holds stars b, b + nb, b + 2 nb, ... (spread over the leaves)
-    const int nb = (npart + kWave - 1) / kWave;
+    const int nb = (npart + kBlk - 1) / kBlk;
     const int lane = lane_id();
     for (int b = 0; b < nb; b++) {
         const int mine = b + nb * lane;
-        uint64_t col = transpose_bits(mine < npart ? pmask[first + mine] : 0ull);
+        uint32_t col = (uint32_t)transpose_bits(lane < kBlk && mine < npart ? pmask[first + mine] : 0ull);
         const float4* blk = part + first + b;
         while (col) {
-            const int e = __builtin_ctzll(col);
+            const int e = __builtin_ctz(col);
             col &= col - 1;
-            star_force<EPS0>(blk[nb * e], px, py, pz, eps2, ax, ay, az);
+            star_force<EPS0>(blk[__umul24(nb, e)], px, py, pz, eps2, ax, ay, az);
         }
     }
 }
