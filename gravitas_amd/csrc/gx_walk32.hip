@@ -56,21 +56,24 @@ __device__ __forceinline__ void cell_force(const float4 c0, const float4 c1, con
     const float ir = __builtin_amdgcn_rsqf(d2e);
     const float inv = ir * ir;
     const float inv3 = inv * ir;
-    float f = c0.w * inv3;
     if (QUAD) {
-        const float qx = fmaf(c1.x, dx, fmaf(c1.w, dy, c2.x * dz));
-        const float qy = fmaf(c1.w, dx, fmaf(c1.y, dy, c2.y * dz));
-        const float qz = fmaf(c2.x, dx, fmaf(c2.y, dy, c1.z * dz));
-        const float inv5 = inv3 * inv;
-        const float dqd = fmaf(dx, qx, fmaf(dy, qy, dz * qz));
-        f = fmaf(2.5f * dqd, inv5 * inv, f);
-        ax = fmaf(-qx, inv5, ax);
-        ay = fmaf(-qy, inv5, ay);
-        az = fmaf(-qz, inv5, az);
+        // With e = d inv and g = Q e = inv Qd:  f d - Qd inv5
+        //   = inv3 ((m + 2.5 (e.g)) d - g),  since e.g = inv^2 d.Qd;
+        // 31 instructions against 32 for the term-by-term form.
+        const float ex = dx * inv, ey = dy * inv, ez = dz * inv;
+        const float gx = fmaf(c1.x, ex, fmaf(c1.w, ey, c2.x * ez));
+        const float gy = fmaf(c1.w, ex, fmaf(c1.y, ey, c2.y * ez));
+        const float gz = fmaf(c2.x, ex, fmaf(c2.y, ey, c1.z * ez));
+        const float s = fmaf(2.5f, fmaf(ex, gx, fmaf(ey, gy, ez * gz)), c0.w);
+        ax = fmaf(inv3, fmaf(s, dx, -gx), ax);
+        ay = fmaf(inv3, fmaf(s, dy, -gy), ay);
+        az = fmaf(inv3, fmaf(s, dz, -gz), az);
+    } else {
+        const float f = c0.w * inv3;
+        ax = fmaf(f, dx, ax);
+        ay = fmaf(f, dy, ay);
+        az = fmaf(f, dz, az);
     }
-    ax = fmaf(f, dx, ax);
-    ay = fmaf(f, dy, ay);
-    az = fmaf(f, dz, az);
 }
 
 // Cells accepted by every lane of the group: all lanes evaluate every cell.
@@ -98,6 +101,27 @@ __device__ __forceinline__ uint64_t transpose_bits(uint64_t x) {
         x = (lane & sft) ? ((x & ~m) | ((y & ~m) >> sft)) : ((x & m) | ((y & m) << sft));
     }
     return x;
+}
+
+// The same for a 32 x 64 matrix whose rows sit on lanes 0-31 (lanes 32-63
+// pass 0): column l has 32 bits.  The first stage only moves the high words
+// of lanes 0-31 to lanes 32-63; the other five run on 32-bit words, each lane
+// sending the block its partner keeps (no divergent branches).
+__device__ __forceinline__ uint32_t transpose_bits32(uint64_t x) {
+    const int lane = lane_id();
+    const uint32_t hi = __shfl_xor((int)(uint32_t)(x >> 32), 32, kWave);
+    uint32_t w = lane < 32 ? (uint32_t)x : hi;
+    constexpr uint32_t keep[5] = {0x0000FFFFu, 0x00FF00FFu, 0x0F0F0F0Fu, 0x33333333u, 0x55555555u};
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const int sft = 16 >> k;
+        const uint32_t m = keep[k];
+        const bool up = (lane & sft) != 0;
+        const uint32_t send = up ? (w & m) << sft : (w & ~m) >> sft;
+        const uint32_t kept = up ? (w & ~m) : (w & m);
+        w = kept | (uint32_t)__shfl_xor((int)send, sft, kWave);
+    }
+    return w;
 }
 
 // Cells accepted by some lanes only (mask of cell e held by lane e): each
@@ -166,7 +190,7 @@ __device__ __forceinline__ void eval_parts_own(const float4* __restrict__ part, 
     const int lane = lane_id();
     for (int b = 0; b < nb; b++) {
         const int mine = b + nb * lane;
-        uint32_t col = (uint32_t)transpose_bits(lane < kBlk && mine < npart ? pmask[first + mine] : 0ull);
+        uint32_t col = transpose_bits32(lane < kBlk && mine < npart ? pmask[first + mine] : 0ull);
         const float4* blk = part + first + b;
         while (col) {
             const int e = __builtin_ctz(col);
