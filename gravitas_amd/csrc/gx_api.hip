@@ -453,12 +453,15 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
     for (int L = 0; L <= Lmax; L++)
         hipLaunchKernelGGL(k_topo_level, dim3(kTreeGrid), dim3(256), 0, st, L, (const uint64_t*)s->keys.p,
                            (TravNode*)s->trav.p, meta, s->capNodes, L == Lmax && Lmax < kMortonBits ? 1 : 0);
+    hipLaunchKernelGGL(k_tree_finish, dim3(1), dim3(64), 0, st, meta);
     mark(s, 5);
+    // all leaves first, then the internal nodes level by level, bottom up
+    hipLaunchKernelGGL(k_moments_leaves, dim3(kTreeGrid), dim3(256), 0, st, (const double4*)s->spos.p,
+                       (TravNode*)s->trav.p, (Mult64*)s->m64.p, (Mult32*)s->m32.p, (Mult16*)s->m16.p, meta, batch);
     for (int L = Lmax; L >= 0; L--)
         hipLaunchKernelGGL(k_moments_level, dim3(kTreeGrid), dim3(256), 0, st, L, (const double4*)s->spos.p,
                            (TravNode*)s->trav.p, (Mult64*)s->m64.p, (Mult32*)s->m32.p, (Mult16*)s->m16.p, meta,
                            batch);
-    hipLaunchKernelGGL(k_tree_finish, dim3(1), dim3(64), 0, st, meta);
     mark(s, 6);
     HIPCHK(hipGetLastError());
     return GX_OK;

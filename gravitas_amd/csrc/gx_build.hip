@@ -721,6 +721,7 @@ __device__ void node_moments(long long ni, const double4* __restrict__ spos, Tra
     trav[ni] = nd;
 }
 
+// Internal nodes of level L (their children, one level down, are done).
 __global__ __launch_bounds__(256) void k_moments_level(int L, const double4* __restrict__ spos,
                                                        TravNode* __restrict__ trav, Mult64* __restrict__ m64,
                                                        Mult32* __restrict__ m32, Mult16* __restrict__ m16,
@@ -730,7 +731,24 @@ __global__ __launch_bounds__(256) void k_moments_level(int L, const double4* __r
     if (meta->flags & kFlagTreeIncomplete) return;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
          t += (long long)gridDim.x * blockDim.x)
-        node_moments(start + t, spos, trav, m64, m32, m16, batch.theta);
+        if (trav[start + t].first >= 0) node_moments(start + t, spos, trav, m64, m32, m16, batch.theta);
+}
+
+// Every leaf of the tree, whatever its level, in one launch: a leaf's moments
+// depend only on its stars, so the level-by-level launches that follow only
+// carry the internal nodes (short sums over at most eight children) and no
+// longer wait, level after level, on the leaves' sums over up to 24 stars.
+// The nodes of all levels are numbered contiguously from 0 (k_tree_finish
+// has counted them).
+__global__ __launch_bounds__(256) void k_moments_leaves(const double4* __restrict__ spos, TravNode* __restrict__ trav,
+                                                        Mult64* __restrict__ m64, Mult32* __restrict__ m32,
+                                                        Mult16* __restrict__ m16, const Meta* __restrict__ meta,
+                                                        Batch batch) {
+    if (meta->flags & kFlagTreeIncomplete) return;
+    const long long cnt = meta->nnodes;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
+         t += (long long)gridDim.x * blockDim.x)
+        if (trav[t].first < 0) node_moments(t, spos, trav, m64, m32, m16, batch.theta);
 }
 
 __global__ void k_tree_finish(Meta* meta) {
