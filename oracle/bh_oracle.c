@@ -24,6 +24,13 @@
  *                        internal/sim/diag.go:44-64, relativity.go:24-26, diag.go:100-109
  *   PairPotential        internal/sim/pair.go:37-47, SchwarzschildRadius
  *                        internal/sim/relativity.go:49-54
+ *   stepSplit            internal/sim/split.go:27-199 (classify, holePull,
+ *                        outerHoleForces, fastForces, kicks and drifts),
+ *                        PairForce/PairRS internal/sim/pair.go:13-34
+ *   frame dragging       internal/sim/framedrag.go:20-143
+ *   retarded gravity     internal/sim/retarded.go:62-209 (holeHistory,
+ *                        sourcePos, holeAppeared, recordHoles)
+ *   rungs, block steps   internal/sim/blockstep.go:23-90
  *
  * Build with -O2 -ffp-contract=off (oracle/Makefile): Go's gc compiler on
  * amd64 does not fuse a*b+c, so the same operations in the same order give
