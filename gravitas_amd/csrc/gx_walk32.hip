@@ -352,26 +352,29 @@ __device__ bool trav_group(const Group& G, TravLDS& S, const TravNode* __restric
         }
         uint64_t amask = cls == 1 ? mask : 0;
         uint64_t omask = cls == 2 ? mask : 0;
-        uint64_t mixed = __ballot(cls == 3);
+        const uint64_t mixed = __ballot(cls == 3);
         nmixed += __popcll(mixed);
-        while (mixed) {
-            const int src = __builtin_ctzll(mixed);
-            mixed &= mixed - 1;
+        uint64_t accMine = 0;  // lane src: the ballot of its mixed node
+        for (uint64_t todo = mixed; todo;) {
+            const int src = __builtin_ctzll(todo);
+            todo &= todo - 1;
             // the node is uniform across the wave: its record comes in through a
             // scalar load rather than eight lane reads
             const int nid = __builtin_amdgcn_readlane(node, src);
             const double nx = trav[nid].x, ny = trav[nid].y, nz = trav[nid].z;
             const double ncrit2 = trav[nid].crit2;
-            const uint64_t smask = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(mask >> 32), src) << 32) |
-                                   (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mask, src);
             // the reference's per-star opening test, bh.go:114-116 (FP64, unfused)
             const double dx = nx - p.x, dy = ny - p.y, dz = nz - p.z;
             const double d2 = dx * dx + dy * dy + dz * dz;
-            const uint64_t acc = __ballot(d2 >= ncrit2) & smask;
-            if (lane == src) {
-                amask = acc;
-                omask = smask & ~acc;
-            }
+            const uint64_t acc = __ballot(d2 >= ncrit2);
+            // lane src keeps the ballot and applies its node's lane mask after
+            // the loop (no lane reads of the mask, one select instead of two)
+            if (lane == src) accMine = acc;
+        }
+        if (cls == 3) {
+            const uint64_t acc = accMine & mask;
+            amask = acc;
+            omask = mask & ~acc;
         }
 
         // accepted cells -> full or partial list
