@@ -12,6 +12,8 @@
  *     here from the step's own formulas (sim.go:302-343), using the forces
  *     returned by the host entry point;
  *   - argument errors come back as status codes, not crashes;
+ *   - retarded hole gravity: after 200 split steps a hole moving at 0.3 c is
+ *     seen on the observer's past light cone (retarded_test.go:22-38);
  *   - the SPH hydro from C: a 3x3x3 gas lattice, no net force on the centre
  *     body, densities and smoothing lengths positive (sph.go:83-263).
  * Prints "ok ..." and exits 0 on success.
@@ -101,6 +103,66 @@ int main(void) {
 
     gx_stats st;
     if (gx_get_stats(s, &st) != GX_OK || st.n_stars != 2) return fail("stats");
+
+    /* retarded hole gravity (retarded.go): a light hole moving along x at
+     * 0.3 c (c = 10) and a star at x = 50, 200 split steps of dt 0.004 with
+     * the holes' histories; the hole seen from (1, 4, -2) lies on that
+     * point's past light cone, behind the hole's current position */
+    {
+        const double c = 10.0, dtr = 0.004;
+        const double u = 3.0 / sqrt(1.0 - 0.09); /* ProperVelocity of v = 3 */
+        double rx[2] = {0, 50}, rv[2] = {u, 0}, rz[2] = {0, 0}, rm[2] = {1e-12, 1e-12};
+        uint8_t rk[2] = {GX_KIND_HOLE, GX_KIND_STAR}, ra[2] = {1, 1};
+        double *e[10];
+        uint8_t *ek, *ea;
+        for (int i = 0; i < 10; i++)
+            if (hipMalloc((void **)&e[i], 2 * sizeof(double)) != hipSuccess) return fail("hipMalloc");
+        if (hipMalloc((void **)&ek, 2) != hipSuccess || hipMalloc((void **)&ea, 2) != hipSuccess) return fail("hipMalloc");
+        hipMemcpy(e[0], rx, sizeof rx, hipMemcpyHostToDevice);
+        hipMemcpy(e[3], rv, sizeof rv, hipMemcpyHostToDevice);
+        hipMemcpy(e[6], rm, sizeof rm, hipMemcpyHostToDevice);
+        for (int i = 1; i < 6; i++)
+            if (i != 3) hipMemcpy(e[i], rz, sizeof rz, hipMemcpyHostToDevice);
+        hipMemcpy(ek, rk, sizeof rk, hipMemcpyHostToDevice);
+        hipMemcpy(ea, ra, sizeof ra, hipMemcpyHostToDevice);
+        gx_retard *r = gx_retard_new();
+        if (!r) return fail("gx_retard_new");
+        double T = 0;
+        for (int k = 0; k < 200; k++, T += dtr) {
+            int32_t sub = 0;
+            if (gx_split_step_retarded_dev(s, e[0], e[1], e[2], e[3], e[4], e[5], e[6], ek, ea, NULL, NULL, NULL, NULL, 2,
+                                           0.01, 0, dtr, c, k > 0, e[7], e[8], e[9], r, T, GX_GRAVITY_RETARDED,
+                                           &sub) != GX_OK)
+                return fail("gx_split_step_retarded_dev");
+        }
+        if (gx_retard_count(r) != 1) return fail("one hole, one history");
+        double ob[3] = {1, 4, -2}, *o, *sxyz;
+        uint8_t *vis, hvis = 0;
+        if (hipMalloc((void **)&o, 3 * sizeof(double)) != hipSuccess ||
+            hipMalloc((void **)&sxyz, 3 * sizeof(double)) != hipSuccess || hipMalloc((void **)&vis, 1) != hipSuccess)
+            return fail("hipMalloc");
+        hipMemcpy(o, ob, sizeof ob, hipMemcpyHostToDevice);
+        if (gx_retard_source_pos_dev(s, r, e[0], e[1], e[2], ek, ea, NULL, 2, 0, o, o + 1, o + 2, 1, T, dtr, c,
+                                     GX_GRAVITY_RETARDED, sxyz, sxyz + 1, sxyz + 2, vis) != GX_OK)
+            return fail("gx_retard_source_pos_dev");
+        double src[3], cur;
+        hipMemcpy(src, sxyz, sizeof src, hipMemcpyDeviceToHost);
+        hipMemcpy(&hvis, vis, 1, hipMemcpyDeviceToHost);
+        hipMemcpy(&cur, e[0], sizeof cur, hipMemcpyDeviceToHost);
+        const double dist = sqrt((ob[0] - src[0]) * (ob[0] - src[0]) + (ob[1] - src[1]) * (ob[1] - src[1]) +
+                                 (ob[2] - src[2]) * (ob[2] - src[2]));
+        if (!hvis || fabs(dist - c * (T - src[0] / 3.0)) > 1e-6 || !(src[0] < cur)) return fail("light cone");
+        if (gx_split_step_retarded_dev(s, e[0], e[1], e[2], e[3], e[4], e[5], e[6], ek, ea, NULL, NULL, NULL, NULL, 2,
+                                       0.01, 0, dtr, c, 1, e[7], e[8], e[9], r, T, 7, NULL) != GX_ERR_ARG)
+            return fail("unknown gravity mode must be GX_ERR_ARG");
+        gx_retard_free(r);
+        for (int i = 0; i < 10; i++) hipFree(e[i]);
+        hipFree(ek);
+        hipFree(ea);
+        hipFree(o);
+        hipFree(sxyz);
+        hipFree(vis);
+    }
     gx_solver_free(s);
 
     /* the hydro (sph.Solver.Forces) on host arrays: a 3x3x3 gas lattice of

@@ -71,7 +71,7 @@ def build_c_smoke(tmp_path):
            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(rocm, "include"),
            os.path.join(ROOT, "tests", "c_abi_smoke.c"), "-o", exe,
            "-L", lib, "-lgravitas_hip", "-L", os.path.join(rocm, "lib"), "-lamdhip64",
-           "-Wl,-rpath," + lib, "-Wl,-rpath," + os.path.join(rocm, "lib")]
+           "-Wl,-rpath," + lib, "-Wl,-rpath," + os.path.join(rocm, "lib"), "-lm"]
     subprocess.run(cmd, check=True)
     return exe
 
