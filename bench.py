@@ -202,7 +202,7 @@ def bench_disk_halo(args):
         synchronize()
     out = base_line(args, n, ms, "fp32", {
         "workload": "disk-halo-1m", "n": n, "theta": theta, "eps": eps,
-        "model": "70% exponential disk (R_d 1, z0 0.1, Q 1.5) + 30% Hernquist halo (a 5, 5x disk mass)",
+        "bodies": "70% exponential disk (R_d 1, z0 0.1, Q 1.5) + 30% Hernquist halo (a 5, 5x disk mass)",
         "walk": "fp32 group walk (64 stars per wavefront), Kahan-compensated per-star sums",
         "parallelism": "single GPU"})
     out["interactions_per_s"] = (st["pp_interactions"] + st["pc_interactions"]) / (ms * 1e-3)
@@ -319,7 +319,7 @@ def bench_clumps(args):
             ladder[walk].update(walk_summary(n, st, ph))
     ms = ladder["fp32"]["ms_per_step"]
     out = base_line(args, n, ms, "fp32", {"workload": "clumps-16m", "n": n, "theta": theta, "eps": eps,
-                                          "model": "1000 Plummer clumps (scale 0.005), log-normal masses, unit cube"})
+                                          "bodies": "1000 Plummer clumps (scale 0.005), log-normal masses, unit cube"})
     out["ladder"] = ladder
     out["roofline"] = ladder["fp32"]["roofline"]
     return out
@@ -474,7 +474,7 @@ def bench_encounter(args):
     rungs = np.mean([inf["rung_count"] for inf in infos], axis=0) / n
     out = base_line(args, n, ms, "fp32", {
         "workload": "two-disk-4m", "n": n, "theta": theta, "eps": eps, "levels": levels, "eta": eta, "T": T,
-        "model": "two disk+halo galaxies (2 x 2,097,152), parabolic orbit, pericentre 5 R_d, inclination 30 deg",
+        "bodies": "two disk+halo galaxies (2 x 2,097,152), parabolic orbit, pericentre 5 R_d, inclination 30 deg",
         "walk": "fp32 group walk, FP16-stored quadrupoles, Kahan-compensated sums",
         "step": "one global step T = 8 ticks of block steps; active stars evaluated per tick"})
     out["unit"] = "particle-steps/s"
@@ -602,7 +602,7 @@ def bench_sph(args):
     ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
     res = base_line(args, n, ms, "fp64", {
         "workload": "sph-sphere-1m", "n": n, "eps": eps,
-        "model": "Plummer sphere of gas (M = a = 1), u 0.2, velocities x 0.3; gamma 5/3, 40 neighbours, alpha 1",
+        "bodies": "Plummer sphere of gas (M = a = 1), u 0.2, velocities x 0.3; gamma 5/3, 40 neighbours, alpha 1",
         "parallelism": "single GPU"})
     res["metric"] = "gas particles/s (SPH hydro forces, FP64, bit-identical to the reference order)"
     force_flops = FLOPS_SPH_PAIR * st["pairs"] + FLOPS_SPH_VISC * st["visc_pairs"]
@@ -691,7 +691,7 @@ def bench_spin(args):
     ms0, subs0, _, _ = run(nospin, args.steps, args.warmup)
     out = base_line(args, n, ms, "fp64+fp32", {
         "workload": "spin-cluster-1m", "n": n, "theta": theta, "eps": eps, "dt": dt, "c": c,
-        "model": "Plummer cluster (M 1, a 1.5) with a central hole of 0.1 spinning at a* = 0.5; "
+        "bodies": "Plummer cluster (M 1, a 1.5) with a central hole of 0.1 spinning at a* = 0.5; "
                  "the accretion scene's stars and hole, without the gas",
         "step": "split step with frame dragging: tree forces (FP32 walk) + hole direct sums + substeps + "
                 "dragOuter/dragFast (FP64)"})
