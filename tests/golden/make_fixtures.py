@@ -25,6 +25,10 @@ Files:
   gas_768.npz      SPH hydro (sph.go:83-263): a gas cloud with stars and dead
                    bodies, h guessed from the box, a step array; h, rho,
                    forces, du/dt, div v, dt, wake and the density-sum count.
+  holes_512.npz    Plummer cluster with two moving holes, one spinning, c = 4:
+                   25 split steps with frame dragging and retarded+v hole
+                   gravity (split.go, framedrag.go, retarded.go): substeps,
+                   final state and spins, the holes' sources for 16 observers.
   edges.npz        two stars and a hole (bh_test.go:137-147), 100 coincident
                    stars and one more (bh_test.go:149-161), a leaf-size
                    boundary of 24 and 25 stars (build.go:134-160), one star.
@@ -166,6 +170,41 @@ def gas_768():
     return {k: s[k] for k in ("px", "py", "pz", "vx", "vy", "vz", "m", "kind", "alive", "u", "h", "rho")}, f
 
 
+def holes_512():
+    """A Plummer cluster with two moving holes, the first spinning, c = 4:
+    25 split steps (split.go:164-199) with frame dragging (framedrag.go) and
+    extrapolated retarded hole gravity (retarded.go, "retarded+v"), the time
+    advancing by dt after each; the substeps of every step, the final state
+    and spins, and where each hole is seen from 16 observers at the end."""
+    s = ics.plummer(512, 108)
+    s["kind"][[0, 1]] = ics.KIND_HOLE
+    s["m"][[0, 1]] = [0.05, 0.03]
+    s["px"][[0, 1]], s["py"][[0, 1]], s["pz"][[0, 1]] = [0.05, -0.1], [0.0, 0.05], [0.0, 0.0]  # in the core
+    s["vx"][[0, 1]] = [0.8, -0.6]
+    s["alive"] = np.ones(512, np.uint8)
+    s["alive"][[9, 200]] = 0
+    c, eps, theta, dt = 4.0, 0.01, 0.5, 1e-2
+    for k in ("jx", "jy", "jz"):
+        s[k] = np.zeros(512)
+    s["jz"][0] = 0.5 * 0.05 ** 2 / c  # a* = 0.5: J = a* M^2 / c
+    s = {**inputs(s), "jx": s["jx"], "jy": s["jy"], "jz": s["jz"]}
+    o, r = oracle.Solver(theta=theta), oracle.Retard()
+    st = {k: s[k].copy() for k in ("px", "py", "pz", "vx", "vy", "vz", "jx", "jy", "jz")}
+    subs = []
+    for k in range(25):
+        subs.append(oracle.split_step(o, st, s["m"], s["kind"], s["alive"], eps, dt, c=c, acc_valid=k > 0, retard=r,
+                                      time=k * dt, gravity="retarded+v"))
+    obs = np.random.default_rng(109).uniform(-2, 2, (16, 3))
+    f = {"theta": theta, "eps": eps, "dt": dt, "c": c, "steps": 25, "substeps": np.array(subs, np.int64),
+         "obs": obs}
+    for h in range(2):
+        src = [r.source_pos(st, s["kind"], s["alive"], h, q, 25 * dt, dt, c, "retarded+v") for q in obs]
+        f[f"source{h}"] = np.array([q if q is not None else [np.nan] * 3 for q in src])
+    for k in st:
+        f["out_" + k] = st[k]
+    return s, f
+
+
 def write(name, arrays):
     path = os.path.join(HERE, name)
     np.savez_compressed(path, **{k: np.asarray(v) for k, v in arrays.items()})
@@ -174,7 +213,7 @@ def write(name, arrays):
 
 def main():
     for name, make in (("plummer_512.npz", plummer_512), ("mixed_768.npz", mixed_768),
-                       ("encounter_768.npz", encounter_768), ("gas_768.npz", gas_768)):
+                       ("encounter_768.npz", encounter_768), ("gas_768.npz", gas_768), ("holes_512.npz", holes_512)):
         s, f = make()
         write(name, {**{"in_" + k: v for k, v in s.items()}, **f})
     write("edges.npz", edges())

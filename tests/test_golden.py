@@ -30,7 +30,8 @@ def acc(o, s, eps, dm_off=False):
 
 
 def test_fixture_files_hold_no_objects():
-    for name in ("plummer_512.npz", "mixed_768.npz", "encounter_768.npz", "gas_768.npz", "edges.npz"):
+    for name in ("plummer_512.npz", "mixed_768.npz", "encounter_768.npz", "gas_768.npz", "holes_512.npz",
+                 "edges.npz"):
         for k, v in load(name).items():
             assert v.dtype != object, (name, k)
 
@@ -119,3 +120,23 @@ def test_oracle_reproduces_hydro_fixture():
     for k in ("ax", "ay", "az", "dudt", "divv", "dt", "wake"):
         assert np.array_equal(out[k], f[k]), k
     assert np.isfinite(f["wake"].view(np.float64)).any()  # the limiter woke some bodies
+
+
+def test_oracle_reproduces_holes_fixture():
+    """25 split steps with a spinning hole, frame dragging and retarded+v
+    hole gravity (split.go, framedrag.go, retarded.go)."""
+    f = load("holes_512.npz")
+    s = state(f)
+    c, eps, dt = float(f["c"]), float(f["eps"]), float(f["dt"])
+    o, r = oracle.Solver(theta=float(f["theta"])), oracle.Retard()
+    st = {k: s[k].copy() for k in KEYS + ("jx", "jy", "jz")}
+    subs = [oracle.split_step(o, st, s["m"], s["kind"], s["alive"], eps, dt, c=c, acc_valid=k > 0, retard=r,
+                              time=k * dt, gravity="retarded+v") for k in range(int(f["steps"]))]
+    assert subs == list(f["substeps"]) and max(subs) > 1
+    for k in st:
+        assert np.array_equal(st[k], f["out_" + k]), k
+    assert not np.array_equal(st["jz"], s["jz"])  # the spin was dragged
+    for h in range(2):
+        for q, want in zip(f["obs"], f[f"source{h}"]):
+            got = r.source_pos(st, s["kind"], s["alive"], h, q, int(f["steps"]) * dt, dt, c, "retarded+v")
+            assert got is not None and np.array_equal(got, want), (h, q)

@@ -3,8 +3,9 @@ the same stored inputs on every machine, the oracle's stored outputs as the
 answer.
 
   * FP64 walk, keys, permutation, KDK step, block step: bit-identical;
-  * potential energy and split step: 1e-12 relative (their sums over many
-    bodies are tree reductions on the GPU and sequential in the oracle);
+  * potential energy and split steps (also with a spinning hole and
+    retarded hole gravity): 1e-12 relative (their sums over many bodies are
+    tree reductions on the GPU and sequential in the oracle);
   * FP32 group walk: the reference's GPU bound (gpu_test.go:18: median
     <= 1e-5, max <= 1e-3 relative to the FP64 forces).
 """
@@ -13,7 +14,7 @@ import sys
 import numpy as np
 import pytest
 
-from gravitas_amd import DeviceState, Params, Solver, ics
+from gravitas_amd import DeviceState, Params, Retard, Solver, ics
 from test_golden import EDGES, KEYS, load, state
 
 pytestmark = pytest.mark.gpu
@@ -135,3 +136,28 @@ def test_hydro_fixture(gpu):
     for k in ("ax", "ay", "az", "dudt", "divv", "dt", "wake"):
         assert np.array_equal(out[k], f[k]), k
     assert hy.stats()["density_sums"] == int(f["evals"])
+
+
+def test_holes_fixture(gpu):
+    """25 split steps with a spinning hole, frame dragging and retarded+v hole
+    gravity: substeps equal, state and spins within 1e-12, the holes' sources
+    within 1e-11."""
+    f = load("holes_512.npz")
+    s = state(f)
+    c, eps, dt = float(f["c"]), float(f["eps"]), float(f["dt"])
+    ds = DeviceState.from_host(s)
+    sv, r = Solver(theta=float(f["theta"]), walk="fp64"), Retard()
+    p = Params(softening=eps, dt=dt, c=c)
+    say("[golden] holes_512: 25 split steps with spin and retarded gravity")
+    subs = [sv.split_step(ds, p, acc_valid=k > 0, retard=r, time=k * dt, gravity="retarded+v")
+            for k in range(int(f["steps"]))]
+    assert subs == list(f["substeps"])
+    h = sv.host_state(ds)
+    for k in KEYS:
+        want = f["out_" + k]
+        assert np.abs(h[k] - want).max() <= 1e-12 * max(1.0, np.abs(want).max()), k
+    for k in ("jx", "jy", "jz"):
+        assert np.abs(h[k] - f["out_" + k]).max() <= 1e-12 * np.abs(f["out_jz"]).max(), k
+    for hole in range(2):
+        got, vis = sv.source_pos(r, ds, p, hole, f["obs"], int(f["steps"]) * dt, "retarded+v")
+        assert vis.all() and np.abs(got - f[f"source{hole}"]).max() <= 1e-11, hole
