@@ -91,7 +91,7 @@ struct gx_solver {
     // host-API staging
     DevBuf x, y, z, m, kind, alive, ax, ay, az;
     // arena
-    DevBuf keys, keys2, order, order2, holes, hist, digitTotal, blockStar, blockHole, spos;
+    DevBuf keys, keys2, order, order2, rank, holes, hist, digitTotal, blockStar, blockHole, spos;
     DevBuf trav, m64, m32, m16, meta, partial;
     DevBuf lfull, lpart, lleaf, lcounts;
     // active sets and block steps
@@ -245,7 +245,8 @@ void gx_solver_free(gx_solver* s) {
     if (s->graph) (void)hipGraphDestroy(s->graph);
     DevBuf* bufs[] = {&s->x,        &s->y,       &s->z,       &s->m,         &s->kind,      &s->alive,
                       &s->ax,       &s->ay,      &s->az,      &s->keys,      &s->keys2,     &s->order,
-                      &s->order2,   &s->holes,   &s->hist,    &s->digitTotal, &s->blockStar, &s->blockHole,
+                      &s->order2,   &s->rank,    &s->holes,   &s->hist,    &s->digitTotal, &s->blockStar,
+                      &s->blockHole,
                       &s->spos,     &s->trav,    &s->m64,     &s->m32,       &s->m16,       &s->meta,
                       &s->partial,  &s->lfull,   &s->lpart,   &s->lleaf,     &s->lcounts,   &s->instCfg,
                       &s->boxEnc,   &s->starCount, &s->starOff, &s->groupOff, &s->root,      &s->half,
@@ -366,7 +367,8 @@ int reserve(gx_solver* s, long long n) {
     const long long nt = (n + kSortTile - 1) / kSortTile;
     int rc;
     if ((rc = ensure(s->keys, n * 8)) || (rc = ensure(s->keys2, n * 8)) || (rc = ensure(s->order, n * 4)) ||
-        (rc = ensure(s->order2, n * 4)) || (rc = ensure(s->holes, n * 4)) ||
+        (rc = ensure(s->order2, n * 4)) || (rc = ensure(s->rank, std::max(n, 1LL) * 4)) ||
+        (rc = ensure(s->holes, n * 4)) ||
         (rc = ensure(s->hist, (size_t)256 * std::max(nt, 1LL) * 4)) || (rc = ensure(s->digitTotal, 256 * 4)) ||
         (rc = ensure(s->blockStar, std::max(nb, 1LL) * 4)) || (rc = ensure(s->blockHole, std::max(nb, 1LL) * 4)) ||
         (rc = ensure(s->spos, std::max(n, 1LL) * 32)) || (rc = ensure(s->meta, sizeof(Meta))))
@@ -440,13 +442,15 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
         hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(256), 0, st, (uint32_t*)s->hist.p, ntiles,
                            (uint32_t*)s->digitTotal.p);
         hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(256), 0, st, kin, vin, kout, vout, meta, shift, inst,
-                           batch, (const uint32_t*)s->hist.p, (const uint32_t*)s->digitTotal.p, ntiles);
+                           batch, (const uint32_t*)s->hist.p, (const uint32_t*)s->digitTotal.p, ntiles,
+                           pass == passes - 1 ? (int32_t*)s->rank.p : (int32_t*)nullptr);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
     mark(s, 3);
-    hipLaunchKernelGGL(k_gather, dim3(blocks(n, 256)), dim3(256), 0, st, a.x, a.y, a.z, a.m, a.kind,
-                       (const int32_t*)s->order.p, meta, a.dmOff, (double4*)s->spos.p);
+    hipLaunchKernelGGL(k_scatter_spos, dim3((unsigned)std::min<long long>(blocks(n, 256), 8192)), dim3(256), 0, st,
+                       a.x, a.y, a.z, a.m, a.kind, a.alive, n, (const int32_t*)s->rank.p, a.dmOff,
+                       (double4*)s->spos.p);
     mark(s, 4);
     hipLaunchKernelGGL(k_tree_init, dim3(blocks(s->B, 256)), dim3(256), 0, st, meta, batch, (TravNode*)s->trav.p);
     const int Lmax = s->allLevels ? kMortonBits : s->levels;

@@ -350,7 +350,8 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict
                                                       int32_t* __restrict__ vout, const Meta* __restrict__ meta,
                                                       int shift, int instPass, Batch batch,
                                                       const uint32_t* __restrict__ hist,
-                                                      const uint32_t* __restrict__ digitTotal, int ntiles) {
+                                                      const uint32_t* __restrict__ digitTotal, int ntiles,
+                                                      int32_t* __restrict__ rankOut) {
     __shared__ uint32_t tileBase[256];
     __shared__ uint32_t running[256];
     __shared__ uint32_t waveCnt[4][256];
@@ -413,6 +414,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict
             uint32_t dst = tileBase[dig] + waveOff[r & 1][w][dig] + (uint32_t)rank;
             kout[dst] = key;
             vout[dst] = val;
+            if (rankOut) rankOut[val] = (int32_t)dst;  // last pass: state index -> sorted position
         }
     }
 }
@@ -420,16 +422,22 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict
 // ---------------------------------------------------------------------------
 // sorted copies: build.go:34-41 with SourceMass (sim.go:93-98)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_gather(const double* __restrict__ x, const double* __restrict__ y,
-                                                const double* __restrict__ z, const double* __restrict__ m,
-                                                const uint8_t* __restrict__ kind, const int32_t* __restrict__ order,
-                                                const Meta* __restrict__ meta, int dmOff,
-                                                double4* __restrict__ spos) {
-    long long k = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (k >= meta->nstars) return;
-    int32_t i = order[k];
-    double sm = (dmOff && kind[i] == kDark) ? 0.0 : m[i];
-    spos[k] = make_double4(x[i], y[i], z[i], sm);
+// The sorted copies are written from the state side: body i (read in state
+// order, coalesced) goes to its sorted position rank[i], written by the last
+// sort pass.  Reading the state through the sorted order instead fetches a
+// whole cache line for every 8-byte read (7x the bytes at 16M bodies).
+__global__ __launch_bounds__(256) void k_scatter_spos(const double* __restrict__ x, const double* __restrict__ y,
+                                                      const double* __restrict__ z, const double* __restrict__ m,
+                                                      const uint8_t* __restrict__ kind,
+                                                      const uint8_t* __restrict__ alive, long long n,
+                                                      const int32_t* __restrict__ rank, int dmOff,
+                                                      double4* __restrict__ spos) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const uint8_t k = kind[i];
+        if (!alive[i] || k == kHole) continue;  // the stars of collect (build.go:62-113)
+        const double sm = (dmOff && k == kDark) ? 0.0 : m[i];
+        spos[rank[i]] = make_double4(x[i], y[i], z[i], sm);
+    }
 }
 
 // ---------------------------------------------------------------------------

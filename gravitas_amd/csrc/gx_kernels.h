@@ -26,9 +26,10 @@ __global__ void k_sort_hist(const uint64_t* keys, const int32_t* vals, const Met
 __global__ void k_sort_scan(uint32_t* hist, int ntiles, uint32_t* digitTotal);
 __global__ void k_sort_scatter(const uint64_t* kin, const int32_t* vin, uint64_t* kout, int32_t* vout,
                                const Meta* meta, int shift, int instPass, Batch batch, const uint32_t* hist,
-                               const uint32_t* digitTotal, int ntiles);
-__global__ void k_gather(const double* x, const double* y, const double* z, const double* m, const uint8_t* kind,
-                         const int32_t* order, const Meta* meta, int dmOff, double4* spos);
+                               const uint32_t* digitTotal, int ntiles, int32_t* rank);
+__global__ void k_scatter_spos(const double* x, const double* y, const double* z, const double* m,
+                               const uint8_t* kind, const uint8_t* alive, long long n, const int32_t* rank,
+                               int dmOff, double4* spos);
 __global__ void k_tree_init(Meta* meta, Batch batch, TravNode* trav);
 __global__ void k_topo_level(int L, const uint64_t* keys, TravNode* trav, Meta* meta, long long cap, int checkOnly);
 __global__ void k_moments_level(int L, const double4* spos, TravNode* trav, Mult64* m64, Mult32* m32,
