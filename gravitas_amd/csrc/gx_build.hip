@@ -345,6 +345,11 @@ __global__ __launch_bounds__(256) void k_sort_scan(uint32_t* __restrict__ hist, 
     if (tid == 255) digitTotal[blockIdx.x] = s[255];
 }
 
+// Scatter of one tile: the tile's elements are first placed in LDS in their
+// output order (by digit, stable within a digit), then written out so that
+// consecutive threads write consecutive positions of each digit's run (the
+// stores coalesce instead of scattering over 256 runs).  With rankOut (the
+// last pass) each body's sorted position is recorded by its state index.
 __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict__ kin,
                                                       const int32_t* __restrict__ vin, uint64_t* __restrict__ kout,
                                                       int32_t* __restrict__ vout, const Meta* __restrict__ meta,
@@ -352,26 +357,36 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict
                                                       const uint32_t* __restrict__ hist,
                                                       const uint32_t* __restrict__ digitTotal, int ntiles,
                                                       int32_t* __restrict__ rankOut) {
-    __shared__ uint32_t tileBase[256];
+    __shared__ uint32_t globalBase[256];  // output index of digit d's first element of this tile
+    __shared__ uint32_t localStart[256];  // staging slot of digit d's first element
     __shared__ uint32_t running[256];
     __shared__ uint32_t waveCnt[4][256];
     __shared__ uint32_t waveOff[2][4][256];
-    __shared__ uint32_t scan[256];
+    __shared__ uint32_t scan[2][256];
+    __shared__ uint64_t skey[kSortTile];
+    __shared__ int32_t sval[kSortTile];
+    __shared__ uint8_t sdig[kSortTile];
     const int tid = threadIdx.x, w = tid >> 6;
     const long long nst = meta->nstars;
-    // digit bases: exclusive scan of the digit totals
-    uint32_t tot = digitTotal[tid];
-    scan[tid] = tot;
+    // digit bases (exclusive scan of the digit totals) and this tile's digit
+    // counts (differences of the per-tile prefixes k_sort_scan left in hist)
+    const uint32_t tot = digitTotal[tid];
+    const uint32_t pre = hist[(size_t)tid * ntiles + blockIdx.x];
+    const uint32_t cnt = ((int)blockIdx.x + 1 < ntiles ? hist[(size_t)tid * ntiles + blockIdx.x + 1] : tot) - pre;
+    scan[0][tid] = tot;
+    scan[1][tid] = cnt;
     for (int ww = 0; ww < 4; ww++) waveCnt[ww][tid] = 0;
     running[tid] = 0;
     __syncthreads();
     for (int d = 1; d < 256; d <<= 1) {
-        uint32_t a = tid >= d ? scan[tid - d] : 0;
+        const uint32_t a0 = tid >= d ? scan[0][tid - d] : 0, a1 = tid >= d ? scan[1][tid - d] : 0;
         __syncthreads();
-        scan[tid] += a;
+        scan[0][tid] += a0;
+        scan[1][tid] += a1;
         __syncthreads();
     }
-    tileBase[tid] = scan[tid] - tot + hist[(size_t)tid * ntiles + blockIdx.x];
+    globalBase[tid] = scan[0][tid] - tot + pre;
+    localStart[tid] = scan[1][tid] - cnt;
     __syncthreads();
     const long long base = (long long)blockIdx.x * kSortTile;
     constexpr int R = kSortTile / 256;
@@ -411,11 +426,21 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict
         }
         __syncthreads();
         if (valid) {
-            uint32_t dst = tileBase[dig] + waveOff[r & 1][w][dig] + (uint32_t)rank;
-            kout[dst] = key;
-            vout[dst] = val;
-            if (rankOut) rankOut[val] = (int32_t)dst;  // last pass: state index -> sorted position
+            const uint32_t slot = localStart[dig] + waveOff[r & 1][w][dig] + (uint32_t)rank;
+            skey[slot] = key;
+            sval[slot] = val;
+            sdig[slot] = (uint8_t)dig;
         }
+    }
+    __syncthreads();
+    const int count = (int)min((long long)kSortTile, nst - base);
+    for (int j = tid; j < count; j += 256) {
+        const uint32_t d = sdig[j];
+        const uint32_t dst = globalBase[d] + ((uint32_t)j - localStart[d]);
+        const int32_t val = sval[j];
+        kout[dst] = skey[j];
+        vout[dst] = val;
+        if (rankOut) rankOut[val] = (int32_t)dst;  // last pass: state index -> sorted position
     }
 }
 

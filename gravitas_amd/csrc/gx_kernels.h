@@ -10,7 +10,7 @@ int set_error(int code, const char* fmt, ...);
 int device_ready();
 
 constexpr int kCollectTile = 4096;  // bodies per collect workgroup (256 threads x 16)
-constexpr int kSortTile = 4096;     // keys per sort tile (256 threads x 16)
+constexpr int kSortTile = 2048;     // keys per sort tile (256 threads x 8; the scatter stages a tile in LDS)
 
 // gx_build.hip
 __global__ void k_step_init(Meta* meta, Batch batch);
