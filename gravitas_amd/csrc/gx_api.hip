@@ -455,7 +455,8 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
     hipLaunchKernelGGL(k_tree_init, dim3(blocks(s->B, 256)), dim3(256), 0, st, meta, batch, (TravNode*)s->trav.p);
     const int Lmax = s->allLevels ? kMortonBits : s->levels;
     for (int L = 0; L <= Lmax; L++)
-        hipLaunchKernelGGL(k_topo_level, dim3(kTreeGrid), dim3(256), 0, st, L, (const uint64_t*)s->keys.p,
+        hipLaunchKernelGGL(k_topo_level, dim3(kTreeGrid * 256 / kTopoThreads), dim3(kTopoThreads), 0, st, L,
+                           (const uint64_t*)s->keys.p,
                            (TravNode*)s->trav.p, meta, s->capNodes, L == Lmax && Lmax < kMortonBits ? 1 : 0);
     hipLaunchKernelGGL(k_tree_finish, dim3(1), dim3(64), 0, st, meta);
     mark(s, 5);

@@ -572,20 +572,21 @@ __device__ void topo_level(int L, const uint64_t* __restrict__ keys, TravNode* _
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) meta->levelStart[L + 1] = nextBase;
-    __shared__ int wcnt[4];
+    __shared__ int wcnt[kTopoThreads / kWave];
     __shared__ long long wbase;
     __shared__ int stop;
     if (threadIdx.x == 0) stop = L >= kMortonBits || (meta->flags & kFlagNodeOverflow);
     __syncthreads();
     if (stop) return;  // the same decision for every wave of the workgroup
     const int lane = lane_id();
-    const int w = threadIdx.x / kWave;  // 4 waves per workgroup
+    const int w = threadIdx.x / kWave;  // kTopoThreads / 64 waves per workgroup
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     const long long iters = (cnt + nwaves - 1) / nwaves;
     const int shift = 3 * (kMortonBits - 1 - L);
     // Every wave of the workgroup runs the same number of iterations, so the
-    // children of the workgroup's (up to) 4 nodes get their slots from one
-    // atomic on the level counter.
+    // children of the workgroup's (up to) 16 nodes get their slots from one
+    // atomic on the level counter (the one address every workgroup adds to:
+    // the fewer atomics, the less they queue behind each other).
     for (long long it = 0; it < iters; it++) {
         const long long wv = it * nwaves + ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
         const long long ni = start + wv;
@@ -608,7 +609,8 @@ __device__ void topo_level(int L, const uint64_t* __restrict__ keys, TravNode* _
         if (lane == 0) wcnt[w] = nchild;
         __syncthreads();
         if (threadIdx.x == 0) {
-            const int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+            int tot = 0;
+            for (int k = 0; k < kTopoThreads / kWave; k++) tot += wcnt[k];
             wbase = tot ? (long long)atomicAdd((unsigned long long*)&meta->levelCount[L + 1], (unsigned long long)tot)
                         : 0;
         }
@@ -650,7 +652,7 @@ __device__ void topo_level(int L, const uint64_t* __restrict__ keys, TravNode* _
     }
 }
 
-__global__ __launch_bounds__(256) void k_topo_level(int L, const uint64_t* __restrict__ keys,
+__global__ __launch_bounds__(kTopoThreads) void k_topo_level(int L, const uint64_t* __restrict__ keys,
                                                     TravNode* __restrict__ trav, Meta* meta, long long cap,
                                                     int checkOnly) {
     topo_level(L, keys, trav, meta, cap, checkOnly);

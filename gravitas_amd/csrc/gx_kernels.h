@@ -31,6 +31,7 @@ __global__ void k_scatter_spos(const double* x, const double* y, const double* z
                                const uint8_t* kind, const uint8_t* alive, long long n, const int32_t* rank,
                                int dmOff, double4* spos);
 __global__ void k_tree_init(Meta* meta, Batch batch, TravNode* trav);
+constexpr int kTopoThreads = 1024;  // threads per topology workgroup (16 wavefronts share one slot atomic)
 __global__ void k_topo_level(int L, const uint64_t* keys, TravNode* trav, Meta* meta, long long cap, int checkOnly);
 __global__ void k_moments_level(int L, const double4* spos, TravNode* trav, Mult64* m64, Mult32* m32,
                                 Mult16* m16, const Meta* meta, Batch batch);
