@@ -83,7 +83,7 @@ struct gx_solver {
     int walk = GX_WALK_FP32;
     int timing = 0;
     int gsize = 64;           // stars per walk group
-    int persistent = 0;       // FP32 walk as a work queue over resident workgroups
+    int persistent = 1;       // FP32 walk as a work queue over resident workgroups (default)
     int persistentBlocks = 0;
     bool splitWalk = false;   // GX_SPLIT_WALK=1: traversal and evaluation as two kernels
     hipStream_t own = nullptr;
@@ -522,7 +522,10 @@ int enqueue_walk(gx_solver* s, int walk, const StepArgs& a) {
         launch_walk32((const double4*)s->spos.p, (const TravNode*)s->trav.p, (const Mult32*)s->m32.p,
                       walk == GX_WALK_FP32_Q16 ? (const Mult16*)s->m16.p : nullptr, (const int32_t*)s->order.p, meta,
                       batch, ng, s->eps0, !s->monopole, lists, a.ax, a.ay, a.az, st,
-                      s->timing ? s->ev[7] : nullptr, s->splitWalk, s->persistent ? s->persistentBlocks : 0);
+                      s->timing ? s->ev[7] : nullptr, s->splitWalk,
+                      // the work queue pays once there are more groups than resident
+                      // workgroups; below that, one workgroup per group
+                      s->persistent && ng > s->persistentBlocks ? s->persistentBlocks : 0);
     } else {
         hipLaunchKernelGGL(k_walk64, dim3(blocks(n, 256)), dim3(256), 0, st, (const double4*)s->spos.p,
                            (const TravNode*)s->trav.p, (const Mult64*)s->m64.p, (const int32_t*)s->order.p, meta,

@@ -145,7 +145,7 @@ class Solver:
     """
 
     def __init__(self, theta: float = 0.0, monopole_only: bool = False, walk: str = "fp32", timing: bool = False,
-                 group_size: int = 64, persistent: bool = False):
+                 group_size: int = 64, persistent: bool = True):
         reason = gpu_available()
         if reason is not None:
             raise RuntimeError(f"GPU backend unavailable: {reason}")

@@ -73,8 +73,9 @@ int gx_solver_config(gx_solver *s, double theta, int monopole_only, int walk, in
 
 /* Walk tuning (no counterpart in the reference): stars per walk group (32 or
  * 64; one wavefront per group) and whether the FP32 walk runs as a
- * persistent work queue over resident workgroups (1) or one workgroup per
- * group (0).  Used by the autotuner. */
+ * persistent work queue over resident workgroups (1, the default: used when
+ * there are more groups than resident workgroups) or one workgroup per group
+ * (0).  Used by the autotuner. */
 int gx_solver_tune(gx_solver *s, int group_size, int persistent);
 
 /* Uses the given hipStream_t (passed as void*) instead of the solver's own
