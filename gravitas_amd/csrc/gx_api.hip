@@ -439,7 +439,7 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
         const int shift = inst ? 8 * (pass - 8) : 8 * pass;
         hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(256), 0, st, kin, vin, meta, shift, inst, batch,
                            (uint32_t*)s->hist.p, ntiles);
-        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(256), 0, st, (uint32_t*)s->hist.p, ntiles,
+        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(kScanThreads), 0, st, (uint32_t*)s->hist.p, ntiles,
                            (uint32_t*)s->digitTotal.p);
         hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(256), 0, st, kin, vin, kout, vout, meta, shift, inst,
                            batch, (const uint32_t*)s->hist.p, (const uint32_t*)s->digitTotal.p, ntiles,

@@ -319,30 +319,31 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
 }
 
 // One workgroup per digit: exclusive scan over the tiles, total per digit.
-__global__ __launch_bounds__(256) void k_sort_scan(uint32_t* __restrict__ hist, int ntiles,
-                                                   uint32_t* __restrict__ digitTotal) {
-    __shared__ uint32_t s[256];
+// One workgroup per digit: exclusive scan of the digit's counts over the
+// tiles, 1,024 consecutive tiles at a time (coalesced loads, a block scan and
+// a running carry); the digit's total.
+__global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict__ hist, int ntiles,
+                                                            uint32_t* __restrict__ digitTotal) {
+    __shared__ uint32_t s[kScanThreads];
     const int tid = threadIdx.x;
     uint32_t* row = hist + (size_t)blockIdx.x * ntiles;
-    const int per = (ntiles + 255) / 256;
-    int lo = tid * per, hi = min(ntiles, lo + per);
-    uint32_t sum = 0;
-    for (int i = lo; i < hi; i++) sum += row[i];
-    s[tid] = sum;
-    __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {
-        uint32_t a = tid >= d ? s[tid - d] : 0;
+    uint32_t carry = 0;
+    for (int base = 0; base < ntiles; base += kScanThreads) {
+        const int i = base + tid;
+        const uint32_t c = i < ntiles ? row[i] : 0u;
+        s[tid] = c;
         __syncthreads();
-        s[tid] += a;
-        __syncthreads();
+        for (int d = 1; d < kScanThreads; d <<= 1) {
+            const uint32_t v = tid >= d ? s[tid - d] : 0u;
+            __syncthreads();
+            s[tid] += v;
+            __syncthreads();
+        }
+        if (i < ntiles) row[i] = carry + s[tid] - c;
+        carry += s[kScanThreads - 1];
+        __syncthreads();  // s is rewritten by the next chunk
     }
-    uint32_t run = s[tid] - sum;
-    for (int i = lo; i < hi; i++) {
-        uint32_t c = row[i];
-        row[i] = run;
-        run += c;
-    }
-    if (tid == 255) digitTotal[blockIdx.x] = s[255];
+    if (tid == 0) digitTotal[blockIdx.x] = carry;
 }
 
 // Scatter of one tile: the tile's elements are first placed in LDS in their

@@ -23,6 +23,7 @@ __global__ void k_keys(const double* x, const double* y, const double* z, const 
                        Batch batch, uint64_t* keys);
 __global__ void k_sort_hist(const uint64_t* keys, const int32_t* vals, const Meta* meta, int shift, int instPass,
                             Batch batch, uint32_t* hist, int ntiles);
+constexpr int kScanThreads = 1024;  // threads of the per-digit scan over the tiles
 __global__ void k_sort_scan(uint32_t* hist, int ntiles, uint32_t* digitTotal);
 __global__ void k_sort_scatter(const uint64_t* kin, const int32_t* vin, uint64_t* kout, int32_t* vout,
                                const Meta* meta, int shift, int instPass, Batch batch, const uint32_t* hist,
