@@ -132,6 +132,12 @@ def lib():
                                    ctypes.POINTER(ctypes.c_int32), ctypes.c_int64, ctypes.c_int, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double]
         assert L.or_node_size() == NODE_DTYPE.itemsize
+        L.or_pair_force.restype = ctypes.c_double
+        L.or_pair_force.argtypes = [ctypes.c_double] * 4
+        L.or_pair_potential.restype = ctypes.c_double
+        L.or_pair_potential.argtypes = [ctypes.c_double] * 5
+        L.or_pair_rs.restype = ctypes.c_double
+        L.or_pair_rs.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         # SPH neighbour tree and hydro (sph_oracle.c)
         for f in ("or_cbrt",):
             getattr(L, f).restype = ctypes.c_double
@@ -310,6 +316,23 @@ def potential_exact(x, y, z, m, kind=None, eps=0.0, dm_off=False):
     """Exact softened star-star potential energy (diag.go:44-64, no holes)."""
     x, y, z, m, kind, _ = _state_arrays(x, y, z, m, kind, None)
     return float(lib().or_potential_exact(_dp(x), _dp(y), _dp(z), _dp(m), _u8(kind), len(x), eps, int(dm_off)))
+
+
+def pair_force(m, r2, eps2, rs):
+    """PairForce, pair.go:26-34: f with acceleration f*d from mass m at |d|^2 = r2."""
+    return float(lib().or_pair_force(m, r2, eps2, rs))
+
+
+def pair_potential(mi, mj, r2, eps2, rs):
+    """PairPotential, pair.go:37-47 (Newtonian, or Paczynski-Wiita with a
+    linear continuation inside 1.5 rs)."""
+    return float(lib().or_pair_potential(mi, mj, r2, eps2, rs))
+
+
+def pair_rs(ki, mi, kj, mj, c):
+    """PairRS, pair.go:13-22: the hole's Schwarzschild radius for a star-hole
+    pair, 0 otherwise."""
+    return float(lib().or_pair_rs(ki, mi, kj, mj, c))
 
 
 def morton_key(x: int, y: int, z: int) -> int:

@@ -1399,3 +1399,11 @@ const double *or_sorted(const or_solver *s, int which) {
 void or_set_monopole(or_solver *s, int monopole) { s->monopole = monopole; }
 void or_set_workers(or_solver *s, int workers) { s->workers = workers; }
 int or_node_size(void) { return (int)sizeof(or_node); }
+
+/* The pair model (pair.go:13-47, relativity.go:49-54) for the tests that
+ * restate internal/sim/pair_test.go. */
+double or_pair_force(double m, double r2, double eps2, double rs) { return pair_force(m, r2, eps2, rs); }
+double or_pair_potential(double mi, double mj, double r2, double eps2, double rs) {
+    return pair_potential(mi, mj, r2, eps2, rs);
+}
+double or_pair_rs(int ki, double mi, int kj, double mj, double c) { return pair_rs(ki, mi, kj, mj, c); }

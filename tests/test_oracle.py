@@ -296,6 +296,71 @@ def test_potential_energy_matches_exact():
     assert u2 == -10.0 / 5.0
 
 
+# ---------------------------------------------------------------------------
+# the pair model (pair.go), restating internal/sim/pair_test.go
+# ---------------------------------------------------------------------------
+
+
+def test_pair_force_is_gradient_of_potential():
+    """pair_test.go:8-19: with m 0.7, eps 0.01 and a central difference of
+    step 1e-7, PairForce * r equals dU/dr of PairPotential to 1e-6, below and
+    above 1.5 rs, Newtonian (rs 0) and Paczynski-Wiita (rs 0.05)."""
+    m, eps2, h = 0.7, 0.01 * 0.01, 1e-7
+    for rs in (0.0, 0.05):
+        for r in (0.06, 0.074, 0.08, 0.15, 0.4, 2.0):
+            du = (oracle.pair_potential(1, m, (r + h) * (r + h), eps2, rs)
+                  - oracle.pair_potential(1, m, (r - h) * (r - h), eps2, rs)) / (2 * h)
+            f = oracle.pair_force(m, r * r, eps2, rs) * r
+            assert abs(f - du) <= 1e-6 * abs(f), (rs, r, f, du)
+
+
+def test_pair_potential_limits_and_pair_rs():
+    """pair_test.go:21-44: the Newtonian and Paczynski-Wiita known answers,
+    and PairRS only for star-hole pairs (dark matter counts as a star), with
+    the hole's radius 2 m / c^2 (relativity.go:49-54)."""
+    assert oracle.pair_potential(2, 3, 4, 0, 0) == -3.0
+    assert abs(oracle.pair_potential(1, 1, 1, 0, 0.1) - (-1 / 0.9)) <= 1e-15
+    S, H, D = ics.KIND_STAR, ics.KIND_HOLE, ics.KIND_DARK
+    assert oracle.pair_rs(S, 1, S, 1, 10) == 0
+    assert oracle.pair_rs(H, 1, H, 1, 10) == 0
+    assert oracle.pair_rs(S, 1, H, 5, 10) == 0.1
+    assert oracle.pair_rs(H, 5, S, 1, 10) == 0.1
+    assert oracle.pair_rs(D, 1, S, 1, 10) == 0 and oracle.pair_rs(S, 1, D, 1, 10) == 0
+    assert oracle.pair_rs(D, 1, D, 1, 10) == 0
+    assert oracle.pair_rs(D, 1, H, 5, 10) == 0.1
+    # c <= 0: Newtonian for every pair
+    assert oracle.pair_rs(S, 1, H, 5, 0) == 0 and oracle.pair_rs(S, 1, H, 5, -1) == 0
+
+
+def test_pair_model_closed_forms():
+    """What the model's definition fixes (docs in pair.go:5-10): rs 0 is the
+    softened Newtonian pair; outside 1.5 rs the PW potential -m/(r - rs) has
+    the force m/((r - rs)^2 r) per unit separation; inside, the force is
+    capped at its photon-sphere value 4 m / rs^2 and the potential continues
+    linearly, continuous at 1.5 rs; the pair is symmetric in the masses."""
+    m, rs = 0.3, 0.02
+    for r in (0.001, 0.01, 0.029, 0.031, 0.5, 7.0):
+        for eps2 in (0.0, 1e-4):
+            re = math.sqrt(r * r + eps2)
+            assert oracle.pair_force(m, r * r, eps2, 0.0) == m / ((r * r + eps2) * re)
+            assert oracle.pair_potential(m, 2.0, r * r, eps2, 0.0) == -m * 2.0 / re
+            f = oracle.pair_force(m, r * r, eps2, rs)
+            if re >= 1.5 * rs:
+                assert abs(f * re - m / (re - rs) ** 2) <= 1e-14 * f * re
+                assert abs(oracle.pair_potential(1, m, r * r, eps2, rs) + m / (re - rs)) <= 1e-14 * m / (re - rs)
+            else:
+                assert abs(f * re - 4 * m / (rs * rs)) <= 1e-14 * f * re  # the capped magnitude
+            assert oracle.pair_potential(m, 2.0, r * r, eps2, rs) == oracle.pair_potential(2.0, m, r * r, eps2, rs)
+    # continuity at the photon sphere and at the cap's onset
+    r0 = 1.5 * rs
+    lo = oracle.pair_potential(1, m, (r0 * (1 - 1e-12)) ** 2, 0, rs)
+    hi = oracle.pair_potential(1, m, (r0 * (1 + 1e-12)) ** 2, 0, rs)
+    assert abs(lo - hi) <= 1e-9 * abs(hi)
+    flo = oracle.pair_force(m, (r0 * (1 - 1e-12)) ** 2, 0, rs)
+    fhi = oracle.pair_force(m, (r0 * (1 + 1e-12)) ** 2, 0, rs)
+    assert abs(flo - fhi) <= 1e-9 * fhi
+
+
 def _pair_model_potential(px, py, pz, m, kind, eps, c):
     """diag.go:44-64 in numpy: the sum over all pairs of PairPotential with
     PairRS (pair.go:13-47): Newtonian for star-star and hole-hole pairs,
