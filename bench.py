@@ -11,7 +11,10 @@ Printed (one JSON line): particles/s of the FP32 step; interactions/s; the
 FP32-vs-FP64-walk and FP32-vs-direct-sum relative force errors (p50/p99/max);
 the FP64 GPU walk's throughput; the walk kernel's achieved FP32 FLOP/s against
 the MI355X vector peak (HIP events around the walk on the solver's stream);
-and the CPU oracle's throughput on the host cores.
+the CPU oracle's throughput on the host (16 threads and 1 thread, with the
+CPU model); and, as "batched", the metric's second half: steps/s of the
+256-system batched workload (hipGraph KDK replays) against 256 separate
+per-system steps, with the autotuner's table.
 
 Other workloads (--config): plummer-16k (force + one KDK step, FP64 and FP32),
 clumps-16m (precision ladder FP64 / FP32 / FP32 with FP16-stored quadrupoles),
@@ -63,7 +66,9 @@ def parse():
     ap.add_argument("--eps", type=float, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-errors", action="store_true", help="skip the error measurements")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-batched", action="store_true", help="disk-halo-1m: skip the batched-256 leg")
+    ap.add_argument("--cpu-seconds", type=float, default=16.0,
+                    help="budget of the CPU baseline samples (half on all host threads, half on one)")
     ap.add_argument("--T", type=float, default=None, help="two-disk-4m: override the global step")
     ap.add_argument("--calibrate", action="store_true",
                     help="also launch 3 drifts with dt = 0 over the resident bodies (k_drift reads 48 and writes "
@@ -80,13 +85,33 @@ def errors(got, want):
     return error_stats(got, want)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(state, theta, eps, budget):
     """The CPU oracle (C restatement of bh.Solver: serial build, ParallelFor
-    walk) on the host cores: one warm-up call, then timed calls of the full
-    Accelerations until the budget is spent, on an evenly strided sample of
-    the workload sized so that one call takes about a quarter of the budget."""
+    walk) on the host: min(16, cores) threads (the host share of one GPU)
+    and 1 thread, each with half the budget (see cpu_sample)."""
+    out = cpu_sample(state, theta, eps, budget / 2, min(16, os.cpu_count() or 1))
+    one = cpu_sample(state, theta, eps, budget / 2, 1)
+    out["cpu_model"] = cpu_model()
+    out["host_cpus"] = os.cpu_count()
+    out["one_core"] = {k: one[k] for k in ("value", "unit", "cores", "sample")}
+    return out
+
+
+def cpu_sample(state, theta, eps, budget, workers):
+    """One warm-up call, then timed calls of the full Accelerations until the
+    budget is spent, on an evenly strided sample of the workload sized so
+    that one call takes about a quarter of the budget."""
     import oracle
-    workers = min(16, os.cpu_count() or 1)
     n = len(state["px"])
     o = oracle.Solver(theta=theta, workers=workers)
     probe = min(n, 65536)
@@ -113,10 +138,10 @@ def cpu_baseline(state, theta, eps, budget):
 
 
 def timed_steps(solver, ds, params, steps, warmup, phase_steps=5):
-    """Warm-up, then `steps` synchronous force steps timed on the host clock
-    (the step replays as one captured graph once its buffers are sized); then
-    `phase_steps` more steps with HIP events recorded on the solver's stream
-    between the phases (per-phase GPU times; events keep those steps eager)."""
+    """Warm-up, then `steps` synchronous force steps (each an eager sequence
+    of launches on the solver's stream, ending in one synchronisation) timed
+    on the host clock; then `phase_steps` more steps with HIP events recorded
+    on the solver's stream between the phases (per-phase GPU times)."""
     from gravitas_amd.hip import synchronize
     timing = solver.timing
     solver.timing = False
@@ -239,6 +264,11 @@ def bench_disk_halo(args):
         cb = cpu_baseline(state, theta, eps, args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = out["value"] / cb["value"]
+    if not args.no_batched:
+        # the metric's second half: batched steps/s of the 256-system workload
+        del ds
+        log("batched leg")
+        _, _, out["batched"] = batched_measure(max(4, args.steps // 2), 2)
     return out
 
 
@@ -325,13 +355,14 @@ def bench_clumps(args):
     return out
 
 
-def bench_batched(args):
+def batched_measure(steps, warmup):
     """256 independent Plummer/Hernquist systems, N log-uniform in [1024, 65536]
     (seeded), theta 0.6, eps 0.01 N^(-1/3), one arena.  Autotunes the walk
     (group size 32/64, persistent work queue or one workgroup per group) on
     eager steps, captures the best as a hipGraph KDK step and replays it;
-    compared with the same step done as 256 separate per-system calls."""
-    from gravitas_amd import DeviceState, Solver, ics
+    compared with the same step done as 256 separate per-system calls.
+    Returns the body count, the graph's ms per step and the summary."""
+    from gravitas_amd import DeviceState, Params, Solver, ics
     from gravitas_amd.hip import synchronize
     systems, eps = ics.batched_systems(256)
     sizes = np.array([len(s["px"]) for s in systems], dtype=np.int64)
@@ -343,7 +374,7 @@ def bench_batched(args):
     # autotune on eager batched force steps
     tune = {}
     for gsize in (64, 32):
-        for pers in (False, True):
+        for pers in (0, 2):
             s = Solver(walk="fp32", group_size=gsize, persistent=pers)
             ds = DeviceState.from_host(state)
             s.batch_accelerations(ds, offsets, theta, eps)
@@ -354,30 +385,29 @@ def bench_batched(args):
             synchronize()
             tune[f"group{gsize}{'-persistent' if pers else ''}"] = (time.perf_counter() - t0) / 5 * 1e3
     best = min(tune, key=tune.get)
-    gsize, pers = (32 if best.startswith("group32") else 64), best.endswith("persistent")
+    gsize, pers = (32 if best.startswith("group32") else 64), (2 if best.endswith("persistent") else 0)
     log(f"autotune {tune} -> {best}")
     # graph-captured KDK step, replayed
     solver = Solver(walk="fp32", group_size=gsize, persistent=pers, timing=True)
     ds = DeviceState.from_host(state)
     solver.graph_capture_step(ds, offsets, theta, eps, dt)
-    solver.graph_launch(max(1, args.warmup))
+    solver.graph_launch(max(1, warmup))
     solver.sync()
     synchronize()
     t0 = time.perf_counter()
-    solver.graph_launch(args.steps)
+    solver.graph_launch(steps)
     solver.sync()
     synchronize()
-    ms_graph = (time.perf_counter() - t0) * 1e3 / args.steps
+    ms_graph = (time.perf_counter() - t0) * 1e3 / steps
     st = solver.stats()
     # the same KDK step as 256 separate per-system calls (one solver reused)
     sep = Solver(theta=theta, walk="fp32")
     ds2 = DeviceState.from_host(state)
     views = [ds2.view(int(offsets[b]), int(sizes[b])) for b in range(len(sizes))]
-    from gravitas_amd import Params
     prm = [Params(softening=eps[b], dt=dt) for b in range(len(sizes))]
     for b in range(len(sizes)):
         sep.step(views[b], prm[b], acc_valid=False)
-    nsep = max(2, args.steps // 4)
+    nsep = max(2, steps // 4)
     synchronize()
     t0 = time.perf_counter()
     for _ in range(nsep):
@@ -385,15 +415,30 @@ def bench_batched(args):
             sep.step(views[b], prm[b], acc_valid=True)
     synchronize()
     ms_sep = (time.perf_counter() - t0) * 1e3 / nsep
+    summary = {
+        "systems": len(sizes), "bodies": n, "theta": theta, "eps": "0.01 N^(-1/3) per system",
+        "sizes": "log-uniform [1024, 65536], seed 4",
+        "step": "KDK (kick, drift, force step, kick), hipGraph replay", "walk_group": best,
+        "ms_per_step": ms_graph, "steps_per_s": 1e3 / ms_graph, "steps_timed": steps,
+        "separate_launches": {"ms_per_step": ms_sep, "steps_per_s": 1e3 / ms_sep, "steps_timed": nsep},
+        "batched_over_separate": ms_sep / ms_graph,
+        "autotune_ms_per_force_step": tune,
+        "interactions_per_particle": {"pp": st["pp_interactions"] / n, "pc": st["pc_interactions"] / n}}
+    return n, ms_graph, summary
+
+
+def bench_batched(args):
+    """The batched workload on its own (see batched_measure)."""
+    n, ms_graph, summary = batched_measure(args.steps, args.warmup)
     out = base_line(args, n, ms_graph, "fp32", {
-        "workload": "batched-256", "systems": len(sizes), "n": n, "theta": theta,
-        "eps": "0.01 N^(-1/3) per system", "sizes": "log-uniform [1024, 65536], seed 4",
-        "step": "KDK (kick, drift, force step, kick), hipGraph replay", "walk_group": best})
-    out["steps_per_s"] = 1e3 / ms_graph
-    out["separate_launches"] = {"ms_per_step": ms_sep, "steps_per_s": 1e3 / ms_sep}
-    out["batched_over_separate"] = ms_sep / ms_graph
-    out["autotune_ms_per_force_step"] = tune
-    out["interactions_per_particle"] = {"pp": st["pp_interactions"] / n, "pc": st["pc_interactions"] / n}
+        "workload": "batched-256", "systems": summary["systems"], "n": n, "theta": summary["theta"],
+        "eps": summary["eps"], "sizes": summary["sizes"], "step": summary["step"],
+        "walk_group": summary["walk_group"]})
+    out["steps_per_s"] = summary["steps_per_s"]
+    out["separate_launches"] = summary["separate_launches"]
+    out["batched_over_separate"] = summary["batched_over_separate"]
+    out["autotune_ms_per_force_step"] = summary["autotune_ms_per_force_step"]
+    out["interactions_per_particle"] = summary["interactions_per_particle"]
     return out
 
 

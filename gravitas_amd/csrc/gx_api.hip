@@ -64,6 +64,7 @@ int ensure(DevBuf& b, size_t bytes) {
 
 constexpr int kPhases = 9;
 constexpr int kTreeGrid = 1024;
+enum { kNoEvents = 0, kStepEvents = 1, kGraphEvents = 2 };
 
 // The arrays of one step.
 struct StepArgs {
@@ -83,7 +84,7 @@ struct gx_solver {
     int walk = GX_WALK_FP32;
     int timing = 0;
     int gsize = 64;           // stars per walk group
-    int persistent = 1;       // FP32 walk as a work queue over resident workgroups (default)
+    int persistent = 1;       // FP32 walk work queue: 0 never, 1 when groups outnumber resident workgroups, 2 always
     int persistentBlocks = 0;
     bool splitWalk = false;   // GX_SPLIT_WALK=1: traversal and evaluation as two kernels
     hipStream_t own = nullptr;
@@ -121,6 +122,10 @@ struct gx_solver {
     bool eps0 = false;
     Meta* hmeta = nullptr;  // pinned mirror of the device Meta
     hipEvent_t ev[kPhases + 1] = {};
+    hipEvent_t gev[2] = {};  // around a run of graph replays
+    int lastEvents = kNoEvents;  // which events time the pending step
+    int graphSteps = 0;
+    bool capturing = false;  // no event records inside a captured graph
     float phase[kPhases] = {};
     double lastSeconds = 0;
     gx_stats stats = {};
@@ -130,6 +135,25 @@ struct gx_solver {
     // captured KDK step
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
+    std::vector<void*> graphBufs;  // the arena's buffers when the step was captured
+    bool nodesSet = false;  // gx_solver_reserve set the node capacity: no automatic floor
+
+    // Every device buffer the solver owns.
+    std::vector<DevBuf*> buffers() {
+        return {&x,        &y,       &z,        &m,          &kind,         &alive,        &ax,        &ay,
+                &az,       &keys,    &keys2,    &order,      &order2,       &rank,         &holes,     &hist,
+                &digitTotal, &blockStar, &blockHole, &spos,   &trav,         &m64,          &m32,       &m16,
+                &meta,     &partial, &lfull,    &lpart,      &lleaf,        &lcounts,      &instCfg,   &boxEnc,
+                &starCount, &starOff, &groupOff, &root,      &half,         &actS,         &blockAct,  &tOff,
+                &tlist,    &activeMask, &rung,  &rungHist,   &diag,         &nearFlag,     &nearList,  &tileCnt,
+                &splitScalars, &farF, &fastF,   &dragFlag,   &dragList,     &dragD,        &dragS0,    &dragS1,
+                &dragSig,  &dragGuess, &dragSeg, &dragSegStart, &dragSegMode, &dragSpec,   &dragScal,  &dragCount};
+    }
+    std::vector<void*> buffer_ptrs() {
+        std::vector<void*> p;
+        for (DevBuf* b : buffers()) p.push_back(b->p);
+        return p;
+    }
 };
 
 // The holes' histories for retarded gravity (Simulation.histories,
@@ -227,6 +251,7 @@ gx_solver* gx_solver_new(void) {
     memset(s->hmeta, 0, sizeof(Meta));
     s->stream = s->own;
     for (auto& e : s->ev) (void)hipEventCreate(&e);
+    for (auto& e : s->gev) (void)hipEventCreate(&e);
     const char* sw = getenv("GX_SPLIT_WALK");
     s->splitWalk = sw && sw[0] == '1';
     const char* di = getenv("GX_DRAG_ITERATIONS");
@@ -243,21 +268,11 @@ void gx_solver_free(gx_solver* s) {
     (void)hipStreamSynchronize(s->stream);
     if (s->graphExec) (void)hipGraphExecDestroy(s->graphExec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
-    DevBuf* bufs[] = {&s->x,        &s->y,       &s->z,       &s->m,         &s->kind,      &s->alive,
-                      &s->ax,       &s->ay,      &s->az,      &s->keys,      &s->keys2,     &s->order,
-                      &s->order2,   &s->rank,    &s->holes,   &s->hist,    &s->digitTotal, &s->blockStar,
-                      &s->blockHole,
-                      &s->spos,     &s->trav,    &s->m64,     &s->m32,       &s->m16,       &s->meta,
-                      &s->partial,  &s->lfull,   &s->lpart,   &s->lleaf,     &s->lcounts,   &s->instCfg,
-                      &s->boxEnc,   &s->starCount, &s->starOff, &s->groupOff, &s->root,      &s->half,
-                      &s->actS,     &s->blockAct, &s->tOff,    &s->tlist,     &s->activeMask, &s->rung,
-                      &s->rungHist, &s->diag,     &s->nearFlag, &s->nearList, &s->tileCnt,  &s->splitScalars,
-                      &s->farF,     &s->fastF,    &s->dragFlag, &s->dragList, &s->dragD,    &s->dragS0,
-                      &s->dragS1,   &s->dragSig,  &s->dragGuess, &s->dragSeg, &s->dragSegStart, &s->dragSegMode,
-                      &s->dragSpec, &s->dragScal, &s->dragCount};
-    for (DevBuf* b : bufs)
+    for (DevBuf* b : s->buffers())
         if (b->p) (void)hipFree(b->p);
     for (auto& e : s->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : s->gev)
         if (e) (void)hipEventDestroy(e);
     if (s->instCopied) (void)hipEventDestroy(s->instCopied);
     if (s->instHost) (void)hipHostFree(s->instHost);
@@ -281,8 +296,23 @@ int gx_solver_config(gx_solver* s, double theta, int monopole_only, int walk, in
 int gx_solver_tune(gx_solver* s, int group_size, int persistent) {
     if (!s) return set_err(GX_ERR_ARG, "null solver");
     if (group_size != 32 && group_size != 64) return set_err(GX_ERR_ARG, "group size %d not in {32, 64}", group_size);
+    if (persistent < 0 || persistent > 2) return set_err(GX_ERR_ARG, "persistent %d not in {0, 1, 2}", persistent);
     s->gsize = group_size;
-    s->persistent = persistent ? 1 : 0;
+    s->persistent = persistent;
+    return GX_OK;
+}
+
+int gx_solver_reserve(gx_solver* s, int64_t nodes, int32_t full, int32_t partial, int32_t leaves) {
+    if (!s) return set_err(GX_ERR_ARG, "null solver");
+    if (nodes < 0 || nodes >= (1LL << 28) || full < 0 || partial < 0 || leaves < 0)
+        return set_err(GX_ERR_ARG, "bad capacities");
+    if (nodes > 0) {
+        s->capNodes = nodes;
+        s->nodesSet = true;
+    }
+    if (full > 0) s->cf = full;
+    if (partial > 0) s->cp = partial;
+    if (leaves > 0) s->cl = leaves;
     return GX_OK;
 }
 
@@ -371,9 +401,15 @@ int reserve(gx_solver* s, long long n) {
         (rc = ensure(s->holes, n * 4)) ||
         (rc = ensure(s->hist, (size_t)256 * std::max(nt, 1LL) * 4)) || (rc = ensure(s->digitTotal, 256 * 4)) ||
         (rc = ensure(s->blockStar, std::max(nb, 1LL) * 4)) || (rc = ensure(s->blockHole, std::max(nb, 1LL) * 4)) ||
-        (rc = ensure(s->spos, std::max(n, 1LL) * 32)) || (rc = ensure(s->meta, sizeof(Meta))))
+        (rc = ensure(s->spos, std::max(n, 1LL) * 32)))
         return rc;
-    long long wantNodes = std::max(4096LL, n);
+    if (!s->meta.p) {  // allocated once; every field starts at zero (the sticky word included)
+        if ((rc = ensure(s->meta, sizeof(Meta)))) return rc;
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, (unsigned long long*)s->meta.p,
+                           (int)(sizeof(Meta) / 8), 0ull);
+        HIPCHK(hipGetLastError());
+    }
+    const long long wantNodes = s->nodesSet ? 1 : std::max(4096LL, n);
     if (s->capNodes < wantNodes) s->capNodes = wantNodes;
     if (s->capNodes >= (1LL << 28)) return set_err(GX_ERR_CAPACITY, "more than 2^28 nodes");
     if ((rc = ensure(s->trav, (size_t)s->capNodes * sizeof(TravNode))) ||
@@ -381,15 +417,19 @@ int reserve(gx_solver* s, long long n) {
         (rc = ensure(s->m32, (size_t)s->capNodes * sizeof(Mult32))) ||
         (rc = ensure(s->m16, (size_t)s->capNodes * sizeof(Mult16))))
         return rc;
-    // ensure() may have over-allocated: use all of it (node indices stay below 2^28)
-    s->capNodes = (long long)std::min({s->trav.bytes / sizeof(TravNode), s->m64.bytes / sizeof(Mult64),
-                                       s->m32.bytes / sizeof(Mult32), s->m16.bytes / sizeof(Mult16)});
+    // ensure() may have over-allocated: use all of it (node indices stay below
+    // 2^28), unless the caller set the capacity (gx_solver_reserve)
+    if (!s->nodesSet)
+        s->capNodes = (long long)std::min({s->trav.bytes / sizeof(TravNode), s->m64.bytes / sizeof(Mult64),
+                                           s->m32.bytes / sizeof(Mult32), s->m16.bytes / sizeof(Mult16)});
     s->capNodes = std::min(s->capNodes, (1LL << 28) - 1);
     return GX_OK;
 }
 
+// Phase events: with timing on, between the phases.  The first and the last
+// (the whole step, for gx_last_seconds) are recorded on every eager step.
 void mark(gx_solver* s, int i) {
-    if (s->timing) (void)hipEventRecord(s->ev[i], s->stream);
+    if ((s->timing || i == 0 || i == kPhases) && !s->capturing) (void)hipEventRecord(s->ev[i], s->stream);
 }
 
 // collect (build.go:62-113): stars in state order into order[], alive holes
@@ -496,7 +536,9 @@ int enqueue_active(gx_solver* s, const StepArgs& a) {
     return GX_OK;
 }
 
-int enqueue_walk(gx_solver* s, int walk, const StepArgs& a) {
+// accumulate: the walk ends a step, so its flags go into Meta::sticky (not
+// for the FP64 walk that repeats an overflowed FP32 walk's work).
+int enqueue_walk(gx_solver* s, int walk, const StepArgs& a, bool accumulate = true) {
     hipStream_t st = s->stream;
     Meta* meta = (Meta*)s->meta.p;
     Batch batch = make_batch(s);
@@ -524,14 +566,17 @@ int enqueue_walk(gx_solver* s, int walk, const StepArgs& a) {
                       batch, ng, s->eps0, !s->monopole, lists, a.ax, a.ay, a.az, st,
                       s->timing ? s->ev[7] : nullptr, s->splitWalk,
                       // the work queue pays once there are more groups than resident
-                      // workgroups; below that, one workgroup per group
-                      s->persistent && ng > s->persistentBlocks ? s->persistentBlocks : 0);
+                      // workgroups; below that, one workgroup per group (persistent 1).
+                      // persistent 2 always takes the queue.
+                      (s->persistent == 2 || (s->persistent == 1 && ng > s->persistentBlocks))
+                          ? s->persistentBlocks : 0);
     } else {
         hipLaunchKernelGGL(k_walk64, dim3(blocks(n, 256)), dim3(256), 0, st, (const double4*)s->spos.p,
                            (const TravNode*)s->trav.p, (const Mult64*)s->m64.p, (const int32_t*)s->order.p, meta,
                            batch, !s->monopole, a.ax, a.ay, a.az);
     }
-    hipLaunchKernelGGL(k_zero_holes, dim3(64), dim3(256), 0, st, (const int32_t*)s->holes.p, meta, a.ax, a.ay, a.az);
+    hipLaunchKernelGGL(k_zero_holes, dim3(64), dim3(256), 0, st, (const int32_t*)s->holes.p, meta, a.ax, a.ay, a.az,
+                       accumulate ? 1 : 0);
     HIPCHK(hipGetLastError());
     return GX_OK;
 }
@@ -543,10 +588,11 @@ int enqueue_step(gx_solver* s, const StepArgs& a) {
     if (rc) return rc;
     if (a.active && (rc = enqueue_active(s, a))) return rc;
     if ((rc = enqueue_walk(s, s->walk, a))) return rc;
-    if (s->timing) HIPCHK(hipEventRecord(s->ev[kPhases], s->stream));
+    mark(s, kPhases);
     HIPCHK(hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, s->stream));
     s->lastWalk = s->walk;
     s->pending = true;
+    s->lastEvents = kStepEvents;
     return GX_OK;
 }
 
@@ -556,10 +602,20 @@ int finish_step(gx_solver* s) {
     s->pending = false;
     HIPCHK(hipStreamSynchronize(s->stream));
     const Meta& h = *s->hmeta;
+    // GPU time of the step (LastGPUSeconds, gpu_darwin.go:61,155): the first
+    // and last events of an eager step, or those around a run of graph
+    // replays (the mean per replay)
+    {
+        float total = 0;
+        const hipEvent_t a = s->lastEvents == kGraphEvents ? s->gev[0] : s->ev[0];
+        const hipEvent_t b = s->lastEvents == kGraphEvents ? s->gev[1] : s->ev[kPhases];
+        if (s->lastEvents != kNoEvents && hipEventElapsedTime(&total, a, b) == hipSuccess)
+            s->lastSeconds = total * 1e-3 / (s->lastEvents == kGraphEvents ? std::max(1, s->graphSteps) : 1);
+        s->lastEvents = kNoEvents;
+    }
     if (s->timing) {
         float total = 0;
         (void)hipEventElapsedTime(&total, s->ev[0], s->ev[kPhases]);
-        s->lastSeconds = total * 1e-3;
         for (int i = 0; i < 6; i++) (void)hipEventElapsedTime(&s->phase[i], s->ev[i], s->ev[i + 1]);
         (void)hipEventElapsedTime(&s->phase[6], s->ev[6], s->ev[kPhases]);
         s->phase[7] = total;
@@ -582,7 +638,13 @@ int finish_step(gx_solver* s) {
     s->stats.pp_interactions = (int64_t)h.pp - (int64_t)h.ntargets;
     s->stats.n_targets = h.ntargets;
     s->stats.max_stack = h.maxStack;
-    s->stats.flags = h.flags;
+    // this step's flags and those of earlier steps (graph replays) not yet reported
+    const int flags = h.flags | (int)h.sticky;
+    s->stats.flags = flags;
+    if (h.sticky) {  // reported now: clear it on the device (a kernel, never a captured memset)
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, &((Meta*)s->meta.p)->sticky, 1, 0ull);
+        HIPCHK(hipGetLastError());
+    }
     s->stats.depth = depth;
     s->stats.fallbacks = s->fallbacks;
     s->stats.list_entries = (int64_t)h.listEntries;
@@ -590,9 +652,9 @@ int finish_step(gx_solver* s) {
     s->stats.mixed_nodes = (int64_t)h.mixed;
     for (int i = 0; i < 3; i++) s->stats.max_list[i] = h.maxList[i];
     s->stats.max_list[3] = 0;
-    if (h.flags & kFlagNodeOverflow)
+    if (flags & kFlagNodeOverflow)
         return set_err(GX_ERR_CAPACITY, "node arena of %lld nodes too small", s->capNodes);
-    if (h.flags & kFlagDepth) return set_err(GX_ERR_CAPACITY, "tree deeper than the levels built");
+    if (flags & kFlagDepth) return set_err(GX_ERR_CAPACITY, "tree deeper than the levels built");
     s->levels = std::min(kMortonBits, depth + 2);
     return GX_OK;
 }
@@ -629,7 +691,7 @@ int run_dev(gx_solver* s, const StepArgs& a) {
             // the FP32 walk's LDS stack was too small for this tree: redo the
             // walk with the FP64 per-particle walk (same interaction set)
             s->fallbacks++;
-            if ((rc = enqueue_walk(s, GX_WALK_FP64, a))) return rc;
+            if ((rc = enqueue_walk(s, GX_WALK_FP64, a, false))) return rc;
             HIPCHK(hipStreamSynchronize(s->stream));
             s->stats.fallbacks = s->fallbacks;
         }
@@ -1533,7 +1595,7 @@ int gx_sync(gx_solver* s) {
     }
     int rc = finish_step(s);
     if (rc) return rc;
-    if (s->hmeta->flags & (kFlagStackOverflow | kFlagListOverflow))
+    if (s->stats.flags & (kFlagStackOverflow | kFlagListOverflow))
         return set_err(GX_ERR_CAPACITY, "walk stack or list overflow; use gx_accelerations_dev");
     return GX_OK;
 }
@@ -1639,6 +1701,7 @@ int gx_graph_capture_step(gx_solver* s, double* x, double* y, double* z, double*
     s->graph = nullptr;
     const int timing = s->timing;
     s->timing = 0;  // no event records inside the graph
+    s->capturing = true;
     s->allLevels = true;  // replays cannot repeat a step with more levels
     HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     rc = kick(s, vx, vy, vz, ax, ay, az, n, 0.5 * dt);
@@ -1650,8 +1713,10 @@ int gx_graph_capture_step(gx_solver* s, double* x, double* y, double* z, double*
     hipGraph_t g = nullptr;
     hipError_t ec = hipStreamEndCapture(s->stream, &g);
     s->timing = timing;
+    s->capturing = false;
     s->allLevels = false;
     s->pending = false;
+    s->lastEvents = kNoEvents;
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -1660,14 +1725,25 @@ int gx_graph_capture_step(gx_solver* s, double* x, double* y, double* z, double*
         return set_err(GX_ERR_HIP, "graph capture failed: %s", hipGetErrorString(mc != hipSuccess ? mc : ec));
     s->graph = g;
     HIPCHK(hipGraphInstantiate(&s->graphExec, g, nullptr, nullptr, 0));
+    s->graphBufs = s->buffer_ptrs();
     s->lastWalk = s->walk;
     return GX_OK;
 }
 
 int gx_graph_launch(gx_solver* s, int32_t steps) {
     if (!s || !s->graphExec || steps < 0) return set_err(GX_ERR_ARG, "no captured step");
+    // a later eager call that needed larger buffers freed the ones the graph uses
+    if (s->buffer_ptrs() != s->graphBufs)
+        return set_err(GX_ERR_ARG, "the captured step's buffers were reallocated by a larger step; capture again");
+    if (steps == 0) return GX_OK;
+    // an unsynchronised earlier run: its flags stay in the sticky word and are
+    // reported with this run's
+    HIPCHK(hipEventRecord(s->gev[0], s->stream));
     for (int i = 0; i < steps; i++) HIPCHK(hipGraphLaunch(s->graphExec, s->stream));
-    if (steps > 0) s->pending = true;
+    HIPCHK(hipEventRecord(s->gev[1], s->stream));
+    s->graphSteps = steps;
+    s->lastEvents = kGraphEvents;
+    s->pending = true;
     return GX_OK;
 }
 

@@ -318,7 +318,6 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
     hist[(size_t)tid * ntiles + blockIdx.x] = h[tid];
 }
 
-// One workgroup per digit: exclusive scan over the tiles, total per digit.
 // One workgroup per digit: exclusive scan of the digit's counts over the
 // tiles, 1,024 consecutive tiles at a time (coalesced loads, a block scan and
 // a running carry); the digit's total.
@@ -566,8 +565,11 @@ __device__ void topo_level(int L, const uint64_t* __restrict__ keys, TravNode* _
     const long long cnt = meta->levelCount[L];
     const long long nextBase = start + cnt;
     if (checkOnly) {
-        if (L >= kMortonBits) return;
-        for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
+        // After a node overflow the level's count includes children that got
+        // no slot (their indices are >= cap): nothing of this tree is used.
+        if (L >= kMortonBits || (meta->flags & kFlagNodeOverflow)) return;
+        const long long have = min(cnt, cap - start);
+        for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < have;
              t += (long long)gridDim.x * blockDim.x)
             if (trav[start + t].hi - trav[start + t].lo > kLeafSize) atomicOr(&meta->flags, (int)kFlagDepth);
         return;
@@ -794,9 +796,12 @@ __global__ void k_tree_finish(Meta* meta) {
     meta->nnodes = total;
 }
 
-__global__ void k_zero_holes(const int32_t* __restrict__ holes, const Meta* __restrict__ meta, double* ax,
-                             double* ay, double* az) {
+// Zeroes the holes' accelerations (bh.go:87-89, 99-101); with accumulate the
+// step's flags also go into the sticky word (the last kernel of a step).
+__global__ void k_zero_holes(const int32_t* __restrict__ holes, Meta* __restrict__ meta, double* ax,
+                             double* ay, double* az, int accumulate) {
     long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (accumulate && h == 0 && meta->flags) atomicOr(&meta->sticky, (unsigned long long)meta->flags);
     for (; h < meta->nholes; h += (long long)gridDim.x * blockDim.x) {
         int32_t i = holes[h];
         ax[i] = 0;

@@ -110,6 +110,11 @@ struct Meta {
     int flags;              // Flag bits
     int maxStack;
     int maxList[3];         // longest full-cell, partial-cell and leaf list of a group
+    // Flags of every step since the host last read them: each force step ORs
+    // its flags in at its end (k_zero_holes), k_step_init leaves it alone, and
+    // the host clears it once it has reported it.  Replays of a captured step
+    // thus keep an overflow of an earlier replay.
+    unsigned long long sticky;
 };
 
 enum Flag : int {

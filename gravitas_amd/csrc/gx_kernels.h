@@ -39,7 +39,7 @@ __global__ void k_moments_level(int L, const double4* spos, TravNode* trav, Mult
 __global__ void k_moments_leaves(const double4* spos, TravNode* trav, Mult64* m64, Mult32* m32, Mult16* m16,
                                  const Meta* meta, Batch batch);
 __global__ void k_tree_finish(Meta* meta);
-__global__ void k_zero_holes(const int32_t* holes, const Meta* meta, double* ax, double* ay, double* az);
+__global__ void k_zero_holes(const int32_t* holes, Meta* meta, double* ax, double* ay, double* az, int accumulate);
 __global__ void k_kick(long long n, double* vx, double* vy, double* vz, const double* ax, const double* ay,
                        const double* az, double h);
 __global__ void k_drift(long long n, double* px, double* py, double* pz, const double* vx, const double* vy,

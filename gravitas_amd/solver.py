@@ -173,7 +173,12 @@ class Solver:
         w = {"fp32": _lib.GX_WALK_FP32, "fp64": _lib.GX_WALK_FP64, "fp32-q16": _lib.GX_WALK_FP32_Q16}[self.walk]
         _lib.check(self._L.gx_solver_config(self._h, float(self.theta), int(self.monopole_only), w,
                                             int(self.timing)))
-        _lib.check(self._L.gx_solver_tune(self._h, int(self.group_size), int(bool(self.persistent))))
+        _lib.check(self._L.gx_solver_tune(self._h, int(self.group_size), int(self.persistent)))
+
+    def reserve(self, nodes=0, full=0, partial=0, leaves=0):
+        """Starting capacities of the node arena and the walk lists (0 keeps
+        one); they still grow on demand."""
+        _lib.check(self._L.gx_solver_reserve(self._h, int(nodes), int(full), int(partial), int(leaves)))
 
     def set_stream(self, stream):
         """Launch on the given hipStream_t (a pointer-sized int or None)."""

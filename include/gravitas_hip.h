@@ -72,11 +72,18 @@ void gx_solver_free(gx_solver *s);
 int gx_solver_config(gx_solver *s, double theta, int monopole_only, int walk, int timing);
 
 /* Walk tuning (no counterpart in the reference): stars per walk group (32 or
- * 64; one wavefront per group) and whether the FP32 walk runs as a
- * persistent work queue over resident workgroups (1, the default: used when
- * there are more groups than resident workgroups) or one workgroup per group
- * (0).  Used by the autotuner. */
+ * 64; one wavefront per group) and the FP32 walk's work queue over resident
+ * workgroups: 1 (the default) takes it when there are more groups than
+ * resident workgroups, 0 never (one workgroup per group), 2 always. */
 int gx_solver_tune(gx_solver *s, int group_size, int persistent);
+
+/* Starting capacities (no counterpart in the reference): the node arena in
+ * nodes and the FP32 walk's per-group lists (full-mask cells, partial-mask
+ * cells, opened leaves), each > 0 to set, 0 to keep.  They grow on demand:
+ * a step that overflows one is repeated with it enlarged, so small values
+ * cost repeats, not results.  Buffers change size at the next step; a step
+ * captured by gx_graph_capture_step must then be captured again. */
+int gx_solver_reserve(gx_solver *s, int64_t nodes, int32_t full, int32_t partial, int32_t leaves);
 
 /* Uses the given hipStream_t (passed as void*) instead of the solver's own
  * stream; NULL restores the solver's stream. */
@@ -141,8 +148,12 @@ int gx_batch_step_dev(gx_solver *s, double *x, double *y, double *z, double *vx,
 /* Captures one KDK step of the batch (kick, drift, force step, kick) into a
  * hipGraph on the solver's stream, after one eager force evaluation that
  * sizes every buffer; gx_graph_launch replays it `steps` times without host
- * work between steps, and gx_sync waits for the replays.  The arrays, the
- * instance layout and the parameters are fixed at capture. */
+ * work between steps, and gx_sync waits for the replays and reports an
+ * overflow of any of them (GX_ERR_CAPACITY: the forces of that replay are
+ * incomplete; run an eager step, which grows the buffers, and capture
+ * again).  The arrays, the instance layout and the parameters are fixed at
+ * capture; gx_graph_launch returns GX_ERR_ARG once a later eager call has
+ * reallocated the buffers the graph uses. */
 int gx_graph_capture_step(gx_solver *s, double *x, double *y, double *z, double *vx, double *vy, double *vz,
                           const double *mass, const uint8_t *kind, const uint8_t *alive, const int64_t *offsets,
                           int32_t count, const double *theta, const double *eps, int dark_matter_off, double dt,
@@ -339,7 +350,10 @@ int gx_direct_sample_dev(gx_solver *s, const double *x, const double *y, const d
                          const uint8_t *kind, int64_t n, const int64_t *targets, int64_t nt, double eps,
                          int dark_matter_off, double *out);
 
-/* LastGPUSeconds (gpu_darwin.go:155): GPU time of the last step, 0 before. */
+/* LastGPUSeconds (gpu_darwin.go:155): GPU time of the last force step
+ * (build and walk, from its first kernel to its last; after gx_graph_launch,
+ * the mean per replay), recorded on every step whatever the timing setting;
+ * 0 before the first step. */
 double gx_last_seconds(const gx_solver *s);
 
 typedef struct {

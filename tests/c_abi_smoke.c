@@ -8,6 +8,8 @@
  *   - two stars, m = 1 at x = 1 and m = 2 at x = 2, eps = 0: the
  *     accelerations are (2, 0, 0) and (-1, 0, 0) (bh_test.go:137-147);
  *   - a hole gets 0 and a dead body keeps its value (bh.go:87-101);
+ *   - gx_last_seconds is 0 before the first step and positive after one
+ *     with timing off (LastGPUSeconds, gpu_darwin.go:155);
  *   - a KDK step on device arrays gives the same positions as one computed
  *     here from the step's own formulas (sim.go:302-343), using the forces
  *     returned by the host entry point;
@@ -46,6 +48,7 @@ int main(void) {
     gx_solver *s = gx_solver_new();
     if (!s) return fail("gx_solver_new");
     if (gx_solver_config(s, 0.5, 0, GX_WALK_FP64, 0) != GX_OK) return fail("config");
+    if (gx_last_seconds(s) != 0) return fail("LastGPUSeconds must be 0 before the first step");
 
     /* two stars, a hole and a dead body */
     double x[4] = {1, 2, 5, 7}, y[4] = {0, 0, 0, 0}, z[4] = {0, 0, 0, 0}, m[4] = {1, 2, 3, 4};
@@ -56,6 +59,8 @@ int main(void) {
     if (ax[0] != 2 || ax[1] != -1 || ay[0] != 0 || az[1] != 0) return fail("two-star known answer");
     if (ax[2] != 0 || ay[2] != 0 || az[2] != 0) return fail("hole must get 0");
     if (ax[3] != 42 || ay[3] != 42 || az[3] != 42) return fail("dead body must be untouched");
+    /* LastGPUSeconds (gpu_darwin.go:155) with timing off, as the Go binding configures it */
+    if (!(gx_last_seconds(s) > 0)) return fail("LastGPUSeconds must be positive after a step");
 
     /* errors are status codes */
     if (gx_accelerations(s, NULL, y, z, m, kind, alive, 4, 0.0, 0, ax, ay, az) != GX_ERR_ARG)

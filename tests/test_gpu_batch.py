@@ -85,7 +85,7 @@ def test_batch_fp32_matches_and_variants_agree(gpu, batch):
     ref = ds64.acc_host()
     results = {}
     for gsize in (64, 32):
-        for persistent in (False, True):
+        for persistent in (0, 2):  # one workgroup per group; the work queue (forced: 2)
             say(f"[batch] fp32 group={gsize} persistent={persistent}")
             ds = DeviceState.from_host(state)
             s = Solver(walk="fp32", group_size=gsize, persistent=persistent)
@@ -100,8 +100,8 @@ def test_batch_fp32_matches_and_variants_agree(gpu, batch):
             assert e[len(e) // 2] <= 1e-5 and e[-1] <= 1e-3
             results[(gsize, persistent)] = acc
     # the work queue does not change the arithmetic of a group
-    assert np.array_equal(results[(64, False)], results[(64, True)])
-    assert np.array_equal(results[(32, False)], results[(32, True)])
+    assert np.array_equal(results[(64, 0)], results[(64, 2)])
+    assert np.array_equal(results[(32, 0)], results[(32, 2)])
 
 
 def test_single_system_batch_equals_plain_call(gpu):
@@ -123,7 +123,7 @@ def test_batch_step_and_graph_replay_equal_eager(gpu, batch):
         se = Solver(walk=walk)
         se.batch_accelerations(eager, offsets, THETAS, eps)
         graphed = DeviceState.from_host(state)
-        sg = Solver(walk=walk, persistent=True)
+        sg = Solver(walk=walk, persistent=2)
         sg.graph_capture_step(graphed, offsets, THETAS, eps, dt)  # also computes the initial forces
         for rep in range(3):
             se.batch_step(eager, offsets, THETAS, eps, dt, acc_valid=True)

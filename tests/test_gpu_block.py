@@ -69,7 +69,7 @@ def test_active_forces_equal_full_evaluation(gpu, encounter):
     st64 = s64.stats()
     assert st64["n_targets"] == int(sel.sum())
     for walk, (p50, mx) in (("fp32", (1e-5, 1e-3)), ("fp32-q16", (1e-4, 1e-2))):
-        for gsize, persistent in ((64, False), (32, True)):
+        for gsize, persistent in ((64, 0), (32, 2), (64, 2)):  # 2 forces the work queue (active groups)
             say(f"[block] active set, {walk} group={gsize} persistent={persistent}")
             d = DeviceState.from_host(s)
             sv = Solver(theta=THETA, walk=walk, group_size=gsize, persistent=persistent)
