@@ -217,6 +217,8 @@ def bench_disk_halo(args):
     ds = DeviceState.from_host(state)
     params = Params(softening=eps)
     solver = Solver(theta=theta, walk="fp32", timing=True)
+    log("autotune the walk")
+    table = solver.autotune(ds, [0, n], theta, eps, reps=3)
     log(f"timing {args.steps} steps after {args.warmup} warm-up")
     ms, phases, st = timed_steps(solver, ds, params, args.steps, args.warmup)
     acc32 = ds.acc_host()
@@ -233,6 +235,8 @@ def bench_disk_halo(args):
     out["interactions_per_s"] = (st["pp_interactions"] + st["pc_interactions"]) / (ms * 1e-3)
     out.update(walk_summary(n, st, phases))
     out["roofline"]["traffic"] = measured_traffic("k_walk32", "disk-halo-1m", n, theta)
+    out["walk_autotune"] = {"chosen": tune_name(min(table, key=lambda r: r["ms"])),
+                            "ms_per_walk": {tune_name(r): r["ms"] for r in table}}
     if not args.no_errors:
         log("FP64 walk and direct-sum sample")
         s64 = Solver(theta=theta, walk="fp64", timing=True)
@@ -355,6 +359,11 @@ def bench_clumps(args):
     return out
 
 
+def tune_name(r):
+    return (f"group{r['group_size']}-{'queue' if r['persistent'] == 2 else 'grid'}"
+            f"-stack{r['stack']}-staging{r['parts']}")
+
+
 def batched_measure(steps, warmup):
     """256 independent Plummer/Hernquist systems, N log-uniform in [1024, 65536]
     (seeded), theta 0.6, eps 0.01 N^(-1/3), one arena.  Autotunes the walk
@@ -371,25 +380,14 @@ def batched_measure(steps, warmup):
     n = int(offsets[-1])
     theta, dt = 0.6, 1e-3
     log(f"batched: {len(sizes)} systems, n={n}")
-    # autotune on eager batched force steps
-    tune = {}
-    for gsize in (64, 32):
-        for pers in (0, 2):
-            s = Solver(walk="fp32", group_size=gsize, persistent=pers)
-            ds = DeviceState.from_host(state)
-            s.batch_accelerations(ds, offsets, theta, eps)
-            synchronize()
-            t0 = time.perf_counter()
-            for _ in range(5):
-                s.batch_accelerations(ds, offsets, theta, eps)
-            synchronize()
-            tune[f"group{gsize}{'-persistent' if pers else ''}"] = (time.perf_counter() - t0) / 5 * 1e3
-    best = min(tune, key=tune.get)
-    gsize, pers = (32 if best.startswith("group32") else 64), (2 if best.endswith("persistent") else 0)
-    log(f"autotune {tune} -> {best}")
-    # graph-captured KDK step, replayed
-    solver = Solver(walk="fp32", group_size=gsize, persistent=pers, timing=True)
+    # the library's autotuner on the batch's walk (group size, work queue,
+    # stack and staging sizes), then the chosen walk in a captured KDK step
+    solver = Solver(walk="fp32", timing=True)
     ds = DeviceState.from_host(state)
+    table = solver.autotune(ds, offsets, theta, eps, reps=3)
+    tune = {tune_name(r): r["ms"] for r in table}
+    best = tune_name(min(table, key=lambda r: r["ms"]))
+    log(f"autotune -> {best}")
     solver.graph_capture_step(ds, offsets, theta, eps, dt)
     solver.graph_launch(max(1, warmup))
     solver.sync()
@@ -422,7 +420,7 @@ def batched_measure(steps, warmup):
         "ms_per_step": ms_graph, "steps_per_s": 1e3 / ms_graph, "steps_timed": steps,
         "separate_launches": {"ms_per_step": ms_sep, "steps_per_s": 1e3 / ms_sep, "steps_timed": nsep},
         "batched_over_separate": ms_sep / ms_graph,
-        "autotune_ms_per_force_step": tune,
+        "autotune_ms_per_walk": tune,
         "interactions_per_particle": {"pp": st["pp_interactions"] / n, "pc": st["pc_interactions"] / n}}
     return n, ms_graph, summary
 
@@ -437,7 +435,7 @@ def bench_batched(args):
     out["steps_per_s"] = summary["steps_per_s"]
     out["separate_launches"] = summary["separate_launches"]
     out["batched_over_separate"] = summary["batched_over_separate"]
-    out["autotune_ms_per_force_step"] = summary["autotune_ms_per_force_step"]
+    out["autotune_ms_per_walk"] = summary["autotune_ms_per_walk"]
     out["interactions_per_particle"] = summary["interactions_per_particle"]
     return out
 

@@ -25,7 +25,7 @@ GX_WALK_FP32_Q16 = 2
 # Every symbol include/gravitas_hip.h declares.
 EXPORTS = (
     "gx_init", "gx_name", "gx_error", "gx_solver_new", "gx_solver_free", "gx_solver_config",
-    "gx_solver_set_stream", "gx_solver_reserve", "gx_accelerations", "gx_accelerations_dev", "gx_enqueue_accelerations_dev",
+    "gx_solver_set_stream", "gx_solver_reserve", "gx_solver_tune_walk", "gx_solver_autotune", "gx_accelerations", "gx_accelerations_dev", "gx_enqueue_accelerations_dev",
     "gx_sync", "gx_potential_energy_dev", "gx_direct_sample_dev", "gx_last_seconds", "gx_get_stats",
     "gx_phase_ms", "gx_export", "gx_step_dev", "gx_kick_dev", "gx_drift_dev", "gx_solver_tune",
     "gx_batch_accelerations_dev", "gx_batch_step_dev", "gx_graph_capture_step", "gx_graph_launch",
@@ -53,6 +53,12 @@ class GxStats(ctypes.Structure):
         ("rounds", ctypes.c_int64), ("mixed_nodes", ctypes.c_int64),
         ("max_list", ctypes.c_int32 * 4), ("n_targets", ctypes.c_int64),
     ]
+
+
+class GxTuneEntry(ctypes.Structure):
+    _fields_ = [("group_size", ctypes.c_int32), ("persistent", ctypes.c_int32), ("stack", ctypes.c_int32),
+                ("parts", ctypes.c_int32), ("ms", ctypes.c_float), ("pp_interactions", ctypes.c_int64),
+                ("pc_interactions", ctypes.c_int64)]
 
 
 class GxDragStats(ctypes.Structure):
@@ -128,6 +134,9 @@ def load():
     L.gx_drift_dev.argtypes = [P, P, P, P, P, P, P, I64, D]
     L.gx_solver_tune.argtypes = [P, I, I]
     L.gx_solver_reserve.argtypes = [P, I64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    L.gx_solver_tune_walk.argtypes = [P, I, I, I, I]
+    L.gx_solver_autotune.argtypes = [P, P, P, P, P, P, P, P, ctypes.c_int32, P, P, I, P, P, P, ctypes.c_int32,
+                                     ctypes.POINTER(GxTuneEntry), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
     L.gx_batch_accelerations_dev.argtypes = [P, P, P, P, P, P, P, P, ctypes.c_int32, P, P, I, P, P, P]
     L.gx_batch_step_dev.argtypes = [P, P, P, P, P, P, P, P, P, P, P, ctypes.c_int32, P, P, I, D, I, P, P, P]
     L.gx_graph_capture_step.argtypes = [P, P, P, P, P, P, P, P, P, P, P, ctypes.c_int32, P, P, I, D, P, P, P]

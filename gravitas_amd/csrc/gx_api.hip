@@ -85,7 +85,9 @@ struct gx_solver {
     int timing = 0;
     int gsize = 64;           // stars per walk group
     int persistent = 1;       // FP32 walk work queue: 0 never, 1 when groups outnumber resident workgroups, 2 always
-    int persistentBlocks = 0;
+    int walkConfig = kWalkDefault;  // stack and staging sizes of the FP32 walk (kWalkConfig)
+    int cus = 0;                    // compute units of the device
+    int blocksPerCu[kWalkConfigs] = {};  // resident walk workgroups per CU, per configuration
     bool splitWalk = false;   // GX_SPLIT_WALK=1: traversal and evaluation as two kernels
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
@@ -136,6 +138,14 @@ struct gx_solver {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     std::vector<void*> graphBufs;  // the arena's buffers when the step was captured
+    // autotuner choices per workload (body count, instances, walk, monopole)
+    struct Tuned {
+        long long n;
+        int B, walk, monopole;
+        int gsize, persistent, config;
+        std::vector<gx_tune_entry> table;
+    };
+    std::vector<Tuned> tuned;
     bool nodesSet = false;  // gx_solver_reserve set the node capacity: no automatic floor
 
     // Every device buffer the solver owns.
@@ -256,10 +266,11 @@ gx_solver* gx_solver_new(void) {
     s->splitWalk = sw && sw[0] == '1';
     const char* di = getenv("GX_DRAG_ITERATIONS");
     if (di && di[0]) s->dragIters = std::max(0, atoi(di));
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
-        s->persistentBlocks = cus * std::max(walk32_blocks_per_cu(), 1);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        s->cus = 0;
+    for (int c = 0; c < kWalkConfigs; c++) s->blocksPerCu[c] = std::max(walk32_blocks_per_cu(c), 1);
     return s;
 }
 
@@ -300,6 +311,18 @@ int gx_solver_tune(gx_solver* s, int group_size, int persistent) {
     s->gsize = group_size;
     s->persistent = persistent;
     return GX_OK;
+}
+
+int gx_solver_tune_walk(gx_solver* s, int group_size, int persistent, int stack, int parts) {
+    int rc = gx_solver_tune(s, group_size, persistent);
+    if (rc) return rc;
+    for (int c = 0; c < kWalkConfigs; c++)
+        if (kWalkConfig[c].stack == stack && kWalkConfig[c].parts == parts) {
+            s->walkConfig = c;
+            return GX_OK;
+        }
+    return set_err(GX_ERR_ARG, "no walk built with stack %d and staging %d (stack 384 or 768, staging 128, 256 or 384)",
+                   stack, parts);
 }
 
 int gx_solver_reserve(gx_solver* s, int64_t nodes, int32_t full, int32_t partial, int32_t leaves) {
@@ -549,27 +572,33 @@ int enqueue_walk(gx_solver* s, int walk, const StepArgs& a, bool accumulate = tr
     const long long n = a.n;
     if (walk == GX_WALK_FP32 || walk == GX_WALK_FP32_Q16) {
         const long long ng = (n + s->gsize - 1) / s->gsize + s->B;
+        const int cfg = s->walkConfig;
+        const long long resident = (long long)std::max(s->cus, 1) * s->blocksPerCu[cfg];
+        // the work queue pays once there are more groups than resident
+        // workgroups; below that, one workgroup per group (persistent 1).
+        // persistent 2 always takes the queue.
+        const bool queue = !s->splitWalk && (s->persistent == 2 || (s->persistent == 1 && ng > resident));
+        // one list region per resident workgroup with the queue, else per group
+        const long long regions = queue ? resident : ng;
         int rc;
-        if ((rc = ensure(s->lfull, (size_t)ng * s->cf * 4)) || (rc = ensure(s->lpart, (size_t)ng * s->cp * 16)) ||
-            (rc = ensure(s->lleaf, (size_t)ng * s->cl * 16)) || (rc = ensure(s->lcounts, (size_t)ng * 16)))
+        if ((rc = ensure(s->lfull, (size_t)regions * s->cf * 4)) ||
+            (rc = ensure(s->lpart, (size_t)regions * s->cp * 12)) ||
+            (rc = ensure(s->lleaf, (size_t)regions * s->cl * 12)) ||
+            (rc = ensure(s->lcounts, (size_t)regions * sizeof(GroupCounts))))
             return rc;
         Lists lists;
         lists.full = (uint32_t*)s->lfull.p;
-        lists.part = (uint4*)s->lpart.p;
-        lists.leaf = (uint4*)s->lleaf.p;
-        lists.counts = (int4*)s->lcounts.p;
+        lists.part = (uint3*)s->lpart.p;
+        lists.leaf = (uint3*)s->lleaf.p;
+        lists.counts = (GroupCounts*)s->lcounts.p;
         lists.cf = s->cf;
         lists.cp = s->cp;
         lists.cl = s->cl;
+        lists.perSlot = queue ? 1 : 0;
         launch_walk32((const double4*)s->spos.p, (const TravNode*)s->trav.p, (const Mult32*)s->m32.p,
                       walk == GX_WALK_FP32_Q16 ? (const Mult16*)s->m16.p : nullptr, (const int32_t*)s->order.p, meta,
                       batch, ng, s->eps0, !s->monopole, lists, a.ax, a.ay, a.az, st,
-                      s->timing ? s->ev[7] : nullptr, s->splitWalk,
-                      // the work queue pays once there are more groups than resident
-                      // workgroups; below that, one workgroup per group (persistent 1).
-                      // persistent 2 always takes the queue.
-                      (s->persistent == 2 || (s->persistent == 1 && ng > s->persistentBlocks))
-                          ? s->persistentBlocks : 0);
+                      s->timing ? s->ev[7] : nullptr, s->splitWalk, queue ? (int)resident : 0, cfg);
     } else {
         hipLaunchKernelGGL(k_walk64, dim3(blocks(n, 256)), dim3(256), 0, st, (const double4*)s->spos.p,
                            (const TravNode*)s->trav.p, (const Mult64*)s->m64.p, (const int32_t*)s->order.p, meta,
@@ -1679,6 +1708,100 @@ int gx_batch_step_dev(gx_solver* s, double* x, double* y, double* z, double* vx,
     const StepArgs a = step_args(x, y, z, mass, kind, alive, offsets[count], dark_matter_off, ax, ay, az);
     if ((rc = check_arrays(a)) || (rc = set_instances(s, count, offsets, theta, 0, eps, 0))) return rc;
     return kdk(s, a, vx, vy, vz, dt, acc_valid);
+}
+
+int gx_solver_autotune(gx_solver* s, const double* x, const double* y, const double* z, const double* mass,
+                       const uint8_t* kind, const uint8_t* alive, const int64_t* offsets, int32_t count,
+                       const double* theta, const double* eps, int dark_matter_off, double* ax, double* ay,
+                       double* az, int32_t reps, gx_tune_entry* table, int32_t table_len, int32_t* entries) {
+    if (!s) return set_err(GX_ERR_ARG, "null solver");
+    int rc;
+    if ((rc = check_batch(offsets, count, theta, eps))) return rc;
+    if (reps < 1 || reps > 1000 || table_len < 0 || (table_len > 0 && !table))
+        return set_err(GX_ERR_ARG, "bad repetitions or table");
+    if (entries) *entries = 0;
+    const StepArgs a = step_args(x, y, z, mass, kind, alive, offsets[count], dark_matter_off, ax, ay, az);
+    if ((rc = check_arrays(a)) || (rc = set_instances(s, count, offsets, theta, 0, eps, 0))) return rc;
+    if (s->walk == GX_WALK_FP64 || a.n == 0) return GX_OK;  // nothing to tune
+    auto report = [&](const std::vector<gx_tune_entry>& t) {
+        const int k = std::min<int>((int)t.size(), table_len);
+        for (int i = 0; i < k; i++) table[i] = t[(size_t)i];
+        if (entries) *entries = (int32_t)t.size();
+    };
+    for (const auto& c : s->tuned)  // a workload of this shape was tuned already
+        if (c.n == a.n && c.B == count && c.walk == s->walk && c.monopole == s->monopole) {
+            s->gsize = c.gsize;
+            s->persistent = c.persistent;
+            s->walkConfig = c.config;
+            report(c.table);
+            return GX_OK;
+        }
+    const int timing = s->timing;
+    s->timing = 0;
+    std::vector<gx_tune_entry> t;
+    for (int gsize : {64, 32}) {
+        s->gsize = gsize;
+        s->persistent = 1;
+        s->walkConfig = kWalkDefault;
+        if ((rc = run_dev(s, a))) break;  // the tree and this group size's walk groups
+        // the step allocated Meta on a fresh solver: take the pointer only now
+        Meta* meta = (Meta*)s->meta.p;
+        if (!meta) {
+            rc = set_err(GX_ERR_HIP, "no step metadata after a force step");
+            break;
+        }
+        for (int cfg = 0; cfg < kWalkConfigs && !rc; cfg++)
+            for (int pers : {0, 2}) {
+                s->walkConfig = cfg;
+                s->persistent = pers;
+                bool done = false;
+                for (int attempt = 0; attempt < 4 && !done; attempt++) {
+                    // one untimed walk, then reps timed ones, on the same tree
+                    for (int r = 0; r <= reps && !rc; r++) {
+                        if (r == 1) (void)hipEventRecord(s->gev[0], s->stream);
+                        hipLaunchKernelGGL(k_walk_reset, dim3(1), dim3(64), 0, s->stream, meta);
+                        rc = enqueue_walk(s, s->walk, a, false);
+                    }
+                    if (rc) break;
+                    (void)hipEventRecord(s->gev[1], s->stream);
+                    HIPCHK(hipMemcpyAsync(s->hmeta, meta, sizeof(Meta), hipMemcpyDeviceToHost, s->stream));
+                    HIPCHK(hipStreamSynchronize(s->stream));
+                    const Meta& h = *s->hmeta;
+                    if (h.flags & kFlagListOverflow) {  // this group size needs longer lists: grow, time again
+                        (void)regrow(s);
+                        continue;
+                    }
+                    float ms = 0;
+                    HIPCHK(hipEventElapsedTime(&ms, s->gev[0], s->gev[1]));
+                    gx_tune_entry e;
+                    e.group_size = gsize;
+                    e.persistent = pers;
+                    e.stack = kWalkConfig[cfg].stack;
+                    e.parts = kWalkConfig[cfg].parts;
+                    e.ms = (h.flags & kFlagStackOverflow) ? INFINITY : ms / (float)reps;  // a stack overflow disqualifies
+                    e.pp_interactions = (int64_t)h.pp - (int64_t)h.ntargets;
+                    e.pc_interactions = (int64_t)h.pc;
+                    t.push_back(e);
+                    done = true;
+                }
+            }
+        if (rc) break;
+    }
+    s->timing = timing;
+    s->lastEvents = kNoEvents;
+    if (rc) return rc;
+    size_t best = 0;
+    for (size_t i = 1; i < t.size(); i++)
+        if (t[i].ms < t[best].ms) best = i;
+    if (t.empty() || !(t[best].ms < INFINITY)) return set_err(GX_ERR_CAPACITY, "no walk candidate completed");
+    s->gsize = t[best].group_size;
+    s->persistent = t[best].persistent;
+    for (int c = 0; c < kWalkConfigs; c++)
+        if (kWalkConfig[c].stack == t[best].stack && kWalkConfig[c].parts == t[best].parts) s->walkConfig = c;
+    s->tuned.push_back({a.n, count, s->walk, s->monopole, s->gsize, s->persistent, s->walkConfig, t});
+    report(t);
+    // leave the forces of the chosen walk in ax, ay, az
+    return run_dev(s, a);
 }
 
 int gx_graph_capture_step(gx_solver* s, double* x, double* y, double* z, double* vx, double* vy, double* vz,

@@ -1043,6 +1043,18 @@ __global__ void k_fill_u64(unsigned long long* p, int count, unsigned long long 
     if (i < count) p[i] = v;
 }
 
+// The walk's counters and flags, before a walk that repeats one on the same
+// tree (the autotuner's timed walks).
+__global__ void k_walk_reset(Meta* meta) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    meta->pp = meta->pc = 0;
+    meta->groupNext = 0;
+    meta->listEntries = meta->rounds = meta->mixed = 0;
+    meta->maxStack = 0;
+    meta->maxList[0] = meta->maxList[1] = meta->maxList[2] = 0;
+    meta->flags &= ~(kFlagStackOverflow | kFlagListOverflow);
+}
+
 // ---------------------------------------------------------------------------
 // FP64 diagnostics, internal/sim/diag.go: per workgroup partial sums, in a
 // fixed order, of

@@ -55,6 +55,7 @@ __global__ void k_block_kick(long long n, double* vx, double* vy, double* vz, co
                              const double* az, const uint8_t* kind, const uint8_t* alive, int8_t* rung, int now,
                              int full, int maxRung, double T, double crit, unsigned long long* hist);
 __global__ void k_fill_u64(unsigned long long* p, int count, unsigned long long v);
+__global__ void k_walk_reset(Meta* meta);
 constexpr int kDiagValues = 11;
 constexpr int kDiagBlocks = 1024;
 __global__ void k_diag(long long n, const double* px, const double* py, const double* pz, const double* vx,
@@ -189,24 +190,48 @@ __global__ void k_split_drift_far(long long n, const uint8_t* kind, const uint8_
 // gx_walk32.hip
 // Per-group interaction lists written by the FP32 walk's traversal and read by
 // its evaluation: full-mask cells as bare node indices, partial-mask cells and
-// opened leaves as (node, 0, mask lo, mask hi).  Group g owns entries
-// [g*cap, (g+1)*cap) of each list.
+// opened leaves as (node, mask lo, mask hi).  A region holds one group's
+// lists: leaves opened by every star fill its leaf list from the front, the
+// others from the back.  Region r owns entries [r*cap, (r+1)*cap) of each
+// list; with perSlot the region is the workgroup's (the work queue walks one
+// group at a time per workgroup, so the lists of all groups need only one
+// region per resident workgroup), otherwise the group's.
+struct GroupCounts {
+    int nf, np;      // full-mask and partial-mask cells
+    int nlf, nlp;    // leaves opened by every star, by some
+    int nps;         // stars in the leaves opened by some
+    int overflow;    // the traversal stack overflowed
+    int pad[2];
+};
 struct Lists {
     uint32_t* full;
-    uint4* part;
-    uint4* leaf;
-    int4* counts;  // per group: full, partial, leaves, stack overflow
-    int cf, cp, cl;  // capacities per group
+    uint3* part;
+    uint3* leaf;
+    GroupCounts* counts;
+    int cf, cp, cl;  // capacities per region
+    int perSlot;
 };
+
+// Compile-time sizes of the FP32 walk: traversal stack entries and stars
+// staged per evaluation flush.  kWalkConfigs lists the instantiated pairs;
+// the autotuner picks among them (gx_solver_autotune).
+struct WalkConfig {
+    int stack, parts;
+};
+constexpr int kWalkConfigs = 6;
+constexpr WalkConfig kWalkConfig[kWalkConfigs] = {{384, 128}, {384, 256}, {384, 384},
+                                                  {768, 128}, {768, 256}, {768, 384}};
+constexpr int kWalkDefault = 4;  // 768 entries, 256 stars
 
 // m16 non-null selects the FP16-stored quadrupoles; eps0 marks a batch with an
 // unsoftened instance; persistentBlocks > 0 runs the fused walk as a work queue
-// over that many resident workgroups.
+// over that many resident workgroups; config indexes kWalkConfig.
 void launch_walk32(const double4* spos, const TravNode* trav, const Mult32* m32, const Mult16* m16,
                    const int32_t* order, Meta* meta, const Batch& batch, long long ngroupsUpper, bool eps0,
                    int quad, const Lists& lists, double* ax, double* ay, double* az, hipStream_t stream,
-                   hipEvent_t mid, bool split, int persistentBlocks);
-// Resident workgroups per CU of the fused FP32 walk (for the persistent grid).
-int walk32_blocks_per_cu();
+                   hipEvent_t mid, bool split, int persistentBlocks, int config);
+// Resident workgroups per CU of the fused FP32 walk of a configuration (for
+// the persistent grid).
+int walk32_blocks_per_cu(int config);
 
 }  // namespace gx

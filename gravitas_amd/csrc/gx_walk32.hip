@@ -15,13 +15,16 @@
 // some lanes and opened leaves (with their lane masks).
 //
 // Evaluation (eval_group): cells are staged 64 at a time in LDS as
-// (COM - c_g, m, Q) in FP32, stars of opened leaves 256 at a time as
+// (COM - c_g, m, Q) in FP32, stars of opened leaves PARTS at a time as
 // (x - c_g, m), where c_g is the group's bounding-box centre; coordinates
 // are made group-relative in FP64 before rounding, so FP32 only carries
 // offsets that are small for near interactions.  Entries that apply to every
 // lane are evaluated by all lanes in a uniform loop; for the others the 64
 // lane masks of a chunk are transposed, and each lane loops over its own
-// entries only.
+// entries only.  Partial-mask entries are dealt to chunks (cells) and flushes
+// (leaves) with a stride across the whole list: neighbouring entries have
+// similar masks, so dealing them apart evens out the per-lane loop lengths
+// of each chunk or flush.
 //
 // Interactions follow bh.go:117-133 (cells, monopole + quadrupole) and
 // bh.go:145-150 (stars, Plummer softening).  Each chunk is summed in FP32 and
@@ -33,9 +36,7 @@
 
 namespace gx {
 
-constexpr int kStack = 768;   // traversal stack entries: (first child, child count, lane mask)
 constexpr int kChunk = 64;    // cells staged per evaluation chunk
-constexpr int kParts = 256;   // stars staged per evaluation chunk
 constexpr double kMargin = 1e-12;  // >> FP64 rounding of d2; box tests stay conservative
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
@@ -256,38 +257,43 @@ __device__ __forceinline__ long long xcd_group(const Batch& batch) {
 // ---------------------------------------------------------------------------
 // traversal: one wavefront per group, lists to HBM
 // ---------------------------------------------------------------------------
+template <int STACK>
 struct TravLDS {
-    uint64_t stkMask[kStack];
-    uint32_t stkRange[kStack];  // first | (count - 1) << 28
+    uint64_t stkMask[STACK];
+    uint32_t stkRange[STACK];  // first | (count - 1) << 28
     uint64_t popMask[kWave];
     int popNode[kWave];
 };
 
+template <int PARTS>
 struct EvalLDS {
     float4 cell[kChunk * 3];
-    float4 part[kParts];
-    uint64_t pmask[kParts];
+    float4 part[PARTS];
+    uint64_t pmask[PARTS];
 };
 
+template <int STACK, int PARTS>
 union WalkLDS {
-    TravLDS t;
-    EvalLDS e;
+    TravLDS<STACK> t;
+    EvalLDS<PARTS> e;
 };
 
-// Writes group G's interaction lists; returns false on a stack overflow.
-__device__ bool trav_group(const Group& G, TravLDS& S, const TravNode* __restrict__ trav, Meta* meta,
-                           const Lists& lists) {
+// Writes group G's interaction lists into region r; returns false on a stack
+// or list overflow.
+template <int STACK>
+__device__ bool trav_group(const Group& G, long long r, TravLDS<STACK>& S, const TravNode* __restrict__ trav,
+                           Meta* meta, const Lists& lists) {
     uint64_t* stkMask = S.stkMask;
     uint32_t* stkRange = S.stkRange;
     uint64_t* popMask = S.popMask;
     int* popNode = S.popNode;
     const int lane = lane_id();
     const double4 p = G.p;
-    uint32_t* __restrict__ fullOut = lists.full + (size_t)G.g * lists.cf;
-    uint4* __restrict__ partOut = lists.part + (size_t)G.g * lists.cp;
-    uint4* __restrict__ leafOut = lists.leaf + (size_t)G.g * lists.cl;
+    uint32_t* __restrict__ fullOut = lists.full + (size_t)r * lists.cf;
+    uint3* __restrict__ partOut = lists.part + (size_t)r * lists.cp;
+    uint3* __restrict__ leafOut = lists.leaf + (size_t)r * lists.cl;
     unsigned long long npp = 0, npc = 0;
-    int nf = 0, np = 0, nl = 0, sp = 1, maxsp = 1, nrounds = 0, nmixed = 0;
+    int nf = 0, np = 0, nlf = 0, nlp = 0, nps = 0, sp = 1, maxsp = 1, nrounds = 0, nmixed = 0;
     bool overflow = false;
     if (lane == 0) {
         stkRange[0] = (uint32_t)G.root;  // the instance's root, one node
@@ -387,27 +393,36 @@ __device__ bool trav_group(const Group& G, TravLDS& S, const TravNode* __restric
             if (pos < lists.cf) fullOut[pos] = (uint32_t)node;
         } else if (isCell) {
             const int pos = np + mbcnt(bp);
-            if (pos < lists.cp) partOut[pos] = make_uint4((uint32_t)node, 0u, (uint32_t)amask, (uint32_t)(amask >> 32));
+            if (pos < lists.cp) partOut[pos] = make_uint3((uint32_t)node, (uint32_t)amask, (uint32_t)(amask >> 32));
         }
         if (isCell) npc += (unsigned long long)__popcll(amask);
         nf += __popcll(bf);
         np += __popcll(bp);
 
-        // opened leaves -> leaf list
+        // opened leaves -> leaf list: opened by every star from the front,
+        // by some from the back
         const bool isLeaf = omask != 0 && tfirst < 0;
-        const uint64_t bl = __ballot(isLeaf);
-        if (isLeaf) {
-            const int pos = nl + mbcnt(bl);
-            if (pos < lists.cl) leafOut[pos] = make_uint4((uint32_t)node, 0u, (uint32_t)omask, (uint32_t)(omask >> 32));
-            npp += (unsigned long long)__popcll(omask) * (unsigned long long)(thi - tlo);
+        const bool leafAll = isLeaf && omask == G.gmask;
+        const uint64_t blf = __ballot(leafAll);
+        const uint64_t blp = __ballot(isLeaf && !leafAll);
+        if (leafAll) {
+            const int pos = nlf + mbcnt(blf);
+            if (pos < lists.cl) leafOut[pos] = make_uint3((uint32_t)node, (uint32_t)omask, (uint32_t)(omask >> 32));
+        } else if (isLeaf) {
+            const int pos = nlp + mbcnt(blp);
+            if (pos < lists.cl)
+                leafOut[lists.cl - 1 - pos] = make_uint3((uint32_t)node, (uint32_t)omask, (uint32_t)(omask >> 32));
+            nps += thi - tlo;
         }
-        nl += __popcll(bl);
+        if (isLeaf) npp += (unsigned long long)__popcll(omask) * (unsigned long long)(thi - tlo);
+        nlf += __popcll(blf);
+        nlp += __popcll(blp);
 
         // opened internal nodes -> one stack entry with the whole child range
         const bool openInt = omask != 0 && tfirst >= 0;
         const uint64_t pb = __ballot(openInt);
         const int npush = __popcll(pb);
-        if (sp + npush > kStack) {
+        if (sp + npush > STACK) {
             overflow = true;
             break;
         }
@@ -424,45 +439,84 @@ __device__ bool trav_group(const Group& G, TravLDS& S, const TravNode* __restric
 
     npp = wave_sum_u64(npp);
     npc = wave_sum_u64(npc);
+    nps = wave_sum_i(nps);
+    const bool listOverflow = nf > lists.cf || np > lists.cp || nlf + nlp > lists.cl;
     if (lane == 0) {
-        lists.counts[G.g] = make_int4(nf, np, nl, overflow ? 1 : 0);
+        GroupCounts c;
+        c.nf = nf; c.np = np; c.nlf = nlf; c.nlp = nlp; c.nps = nps;
+        c.overflow = overflow ? 1 : 0;
+        c.pad[0] = c.pad[1] = 0;
+        lists.counts[r] = c;
         atomicAdd(&meta->pp, npp);
         atomicAdd(&meta->pc, npc);
         atomicMax(&meta->maxStack, maxsp);
         atomicMax(&meta->maxList[0], nf);
         atomicMax(&meta->maxList[1], np);
-        atomicMax(&meta->maxList[2], nl);
+        atomicMax(&meta->maxList[2], nlf + nlp);
         atomicAdd(&meta->listEntries, (unsigned long long)(nf + np));
         atomicAdd(&meta->rounds, (unsigned long long)nrounds);
         atomicAdd(&meta->mixed, (unsigned long long)nmixed);
         if (overflow) atomicOr(&meta->flags, (int)kFlagStackOverflow);
-        if (nf > lists.cf || np > lists.cp || nl > lists.cl) atomicOr(&meta->flags, (int)kFlagListOverflow);
+        if (listOverflow) atomicOr(&meta->flags, (int)kFlagListOverflow);
     }
-    return !overflow && nf <= lists.cf && np <= lists.cp && nl <= lists.cl;
+    return !overflow && !listOverflow;
 }
 
 // ---------------------------------------------------------------------------
 // evaluation: one wavefront per group, lists from HBM
 // ---------------------------------------------------------------------------
-template <bool QUAD, bool EPS0, bool Q16>
-__device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict__ spos,
+struct Sums {
+    float sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;  // Kahan sums
+    __device__ void add(float ax, float ay, float az) {
+        kahan_add(sx, cx, ax);
+        kahan_add(sy, cy, ay);
+        kahan_add(sz, cz, az);
+    }
+};
+
+// Stars of up to 64 leaf ranges (one per lane: lcur, lcnt stars, mask lmask)
+// into the staging buffer, which is evaluated (flush) whenever it is full.
+template <bool MASKED, int PARTS, typename Flush>
+__device__ __forceinline__ void stage_leaves(EvalLDS<PARTS>& S, int& nst, int lcur, int lcnt, uint64_t lmask,
+                                             const double4* __restrict__ spos, const Group& G, Flush flush) {
+    while (__ballot(lcnt > 0)) {
+        const int room = PARTS - nst;
+        const int incl = wave_incl_scan(lcnt);
+        const int w = max(0, min(lcnt, room - (incl - lcnt)));
+        const int at = nst + incl - lcnt;
+        for (int i = 0; i < w; i++) {
+            const double4 q = spos[lcur + i];
+            S.part[at + i] = make_float4((float)(q.x - G.cgx), (float)(q.y - G.cgy), (float)(q.z - G.cgz),
+                                         (float)q.w);
+            if (MASKED) S.pmask[at + i] = lmask;
+        }
+        nst += min(room, __shfl(incl, 63, kWave));
+        lcur += w;
+        lcnt -= w;
+        if (nst == PARTS) flush();
+    }
+}
+
+template <bool QUAD, bool EPS0, bool Q16, int PARTS>
+__device__ void eval_group(const Group& G, long long r, EvalLDS<PARTS>& S, const double4* __restrict__ spos,
                            const TravNode* __restrict__ trav, const Mult32* __restrict__ m32,
                            const Mult16* __restrict__ m16, const int32_t* __restrict__ order, const Lists& lists,
                            double* __restrict__ oax, double* __restrict__ oay, double* __restrict__ oaz) {
     const float eps2 = G.eps2;
     float4* cell = S.cell;
-    float4* part = S.part;
-    uint64_t* pmask = S.pmask;
     const int lane = lane_id();
     const float px = (float)(G.p.x - G.cgx), py = (float)(G.p.y - G.cgy), pz = (float)(G.p.z - G.cgz);
-    int4 cnts = lists.counts[G.g];
-    cnts.x = __builtin_amdgcn_readfirstlane(cnts.x);  // list lengths are uniform over the wave
-    cnts.y = __builtin_amdgcn_readfirstlane(cnts.y);
-    cnts.z = __builtin_amdgcn_readfirstlane(cnts.z);
-    const uint32_t* __restrict__ fullIn = lists.full + (size_t)G.g * lists.cf;
-    const uint4* __restrict__ partIn = lists.part + (size_t)G.g * lists.cp;
-    const uint4* __restrict__ leafIn = lists.leaf + (size_t)G.g * lists.cl;
-    float sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;  // Kahan sums
+    const GroupCounts cnts = lists.counts[r];
+    // list lengths are uniform over the wave: they live in SGPRs
+    const int nfc = __builtin_amdgcn_readfirstlane(cnts.nf);
+    const int npc = __builtin_amdgcn_readfirstlane(cnts.np);
+    const int nlf = __builtin_amdgcn_readfirstlane(cnts.nlf);
+    const int nlp = __builtin_amdgcn_readfirstlane(cnts.nlp);
+    const int nps = __builtin_amdgcn_readfirstlane(cnts.nps);
+    const uint32_t* __restrict__ fullIn = lists.full + (size_t)r * lists.cf;
+    const uint3* __restrict__ partIn = lists.part + (size_t)r * lists.cp;
+    const uint3* __restrict__ leafIn = lists.leaf + (size_t)r * lists.cl;
+    Sums sum;
 
     // Cell chunks are software-pipelined: while chunk i is evaluated from LDS,
     // the node data of chunk i+1 and the indices of chunk i+2 are in flight.
@@ -493,8 +547,8 @@ __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict
     };
     auto index_of = [&](bool partial, int i, uint64_t& m) -> int {
         if (partial) {
-            const uint4 e = partIn[i];
-            m = (uint64_t)e.z | ((uint64_t)e.w << 32);
+            const uint3 e = partIn[i];
+            m = (uint64_t)e.y | ((uint64_t)e.z << 32);
             return (int)e.x;
         }
         m = G.gmask;
@@ -502,12 +556,10 @@ __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict
     };
     // Full cells go in list order, chunk c holding entries [64c, 64c + 64).
     // Partial cells are dealt to the chunks with a stride (chunk c holds
-    // entries c, c + nch, c + 2 nch, ...): consecutive entries have similar
-    // masks, and spreading them evens out how many cells each lane of a chunk
-    // evaluates in its own loop.
+    // entries c, c + nch, c + 2 nch, ...).
     for (int pass = 0; pass < 2; pass++) {
         const bool partial = pass == 1;
-        const int n = partial ? cnts.y : cnts.x;
+        const int n = partial ? npc : nfc;
         if (n == 0) continue;
         const int nch = (n + kChunk - 1) / kChunk;
         auto entry = [&](int c, int j) { return partial ? c + nch * j : c * kChunk + j; };
@@ -534,81 +586,82 @@ __device__ void eval_group(const Group& G, EvalLDS& S, const double4* __restrict
                 eval_cells_own<QUAD>(cell, transpose_bits(mine ? cur.m : 0ull), px, py, pz, eps2, ax, ay, az);
             else
                 eval_cells<QUAD>(cell, cnt, px, py, pz, eps2, ax, ay, az);
-            kahan_add(sx, cx, ax);
-            kahan_add(sy, cy, ay);
-            kahan_add(sz, cz, az);
+            sum.add(ax, ay, az);
             __syncthreads();
             cur = nxt;
         }
     }
-    // Stars of opened leaves (large leaves are split over chunks).  Stars of
-    // leaves opened by every lane fill the staging buffer from the front and
-    // are evaluated without masks; the others fill it from the back with
-    // their leaf's mask.
-    int nfull = 0, nmask = 0;
-    auto flush_parts = [&]() {
+
+    // Stars of opened leaves (large leaves are split over flushes).
+    int nst = 0;
+    auto flush_all = [&]() {  // stars every lane uses
         __syncthreads();
         float ax = 0, ay = 0, az = 0;
-        eval_parts<EPS0>(part, 0, nfull, px, py, pz, eps2, ax, ay, az);
-        eval_parts_own<EPS0>(part, pmask, kParts - nmask, nmask, px, py, pz, eps2, ax, ay, az);
-        kahan_add(sx, cx, ax);
-        kahan_add(sy, cy, ay);
-        kahan_add(sz, cz, az);
-        nfull = nmask = 0;
+        eval_parts<EPS0>(S.part, 0, nst, px, py, pz, eps2, ax, ay, az);
+        sum.add(ax, ay, az);
+        nst = 0;
         __syncthreads();
     };
-    for (int c0 = 0; c0 < cnts.z; c0 += kWave) {
-        const int cnt = min(kWave, cnts.z - c0);
+    auto flush_masked = [&]() {  // stars of leaves some lanes opened: per-lane loops
+        __syncthreads();
+        float ax = 0, ay = 0, az = 0;
+        eval_parts_own<EPS0>(S.part, S.pmask, 0, nst, px, py, pz, eps2, ax, ay, az);
+        sum.add(ax, ay, az);
+        nst = 0;
+        __syncthreads();
+    };
+    // leaves opened by every star, in list order
+    for (int c0 = 0; c0 < nlf; c0 += kWave) {
         int lcur = 0, lcnt = 0;
-        uint64_t lmask = 0;
-        if (lane < cnt) {
-            const uint4 e = leafIn[c0 + lane];
+        if (c0 + lane < nlf) {
+            const uint3 e = leafIn[c0 + lane];
             lcur = trav[e.x].lo;
             lcnt = trav[e.x].hi - lcur;
-            lmask = (uint64_t)e.z | ((uint64_t)e.w << 32);
         }
-        const bool all = lmask == G.gmask;
-        while (__ballot(lcnt > 0)) {
-            const int room = kParts - nfull - nmask;
-            const int incl = wave_incl_scan(lcnt);
-            const int w = max(0, min(lcnt, room - (incl - lcnt)));
-            const int wf = all ? w : 0, wm = all ? 0 : w;
-            const int ef = wave_incl_scan(wf) - wf, em = wave_incl_scan(wm) - wm;
-            for (int i = 0; i < w; i++) {
-                const double4 q = spos[lcur + i];
-                const float4 v = make_float4((float)(q.x - G.cgx), (float)(q.y - G.cgy), (float)(q.z - G.cgz),
-                                             (float)q.w);
-                if (all) {
-                    part[nfull + ef + i] = v;
-                } else {
-                    const int slot = kParts - 1 - (nmask + em + i);
-                    part[slot] = v;
-                    pmask[slot] = lmask;
-                }
-            }
-            nfull += wave_sum_i(wf);
-            nmask += wave_sum_i(wm);
-            lcur += w;
-            lcnt -= w;
-            if (nfull + nmask == kParts) flush_parts();
-        }
+        stage_leaves<false, PARTS>(S, nst, lcur, lcnt, 0ull, spos, G, flush_all);
     }
-    flush_parts();
+    if (nst > 0) flush_all();
+    // Leaves opened by some stars (stored from the back): flush f takes the
+    // leaves f, f + nfl, f + 2 nfl, ..., so every flush mixes leaves from
+    // all around the group and each lane's share of a flush's stars is close
+    // to its share of the whole list.
+    const int nfl = (nps + PARTS - 1) / PARTS;
+    for (int f = 0; f < nfl; f++) {
+        for (int c0 = 0; f + nfl * c0 < nlp; c0 += kWave) {
+            const int j = f + nfl * (c0 + lane);
+            int lcur = 0, lcnt = 0;
+            uint64_t lmask = 0;
+            if (j < nlp) {
+                const uint3 e = leafIn[lists.cl - 1 - j];
+                lcur = trav[e.x].lo;
+                lcnt = trav[e.x].hi - lcur;
+                lmask = (uint64_t)e.y | ((uint64_t)e.z << 32);
+            }
+            stage_leaves<true, PARTS>(S, nst, lcur, lcnt, lmask, spos, G, flush_masked);
+        }
+        if (nst > 0) flush_masked();
+    }
     if (G.valid) {
         const int32_t s = order[G.k];
-        oax[s] = (double)sx - (double)cx;
-        oay[s] = (double)sy - (double)cy;
-        oaz[s] = (double)sz - (double)cz;
+        oax[s] = (double)sum.sx - (double)sum.cx;
+        oay[s] = (double)sum.sy - (double)sum.cy;
+        oaz[s] = (double)sum.sz - (double)sum.cz;
     }
 }
 
+// The two phases as separate kernels (GX_SPLIT_WALK=1, for timing them
+// apart): every group keeps its own list region between the launches.
+constexpr int kSplitStack = kWalkConfig[kWalkDefault].stack;
+constexpr int kSplitParts = kWalkConfig[kWalkDefault].parts;
+
 __global__ __launch_bounds__(64) void k_trav32(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
                                                Meta* meta, Batch batch, Lists lists) {
-    __shared__ TravLDS S;
+    __shared__ TravLDS<kSplitStack> S;
     if (meta->flags & kFlagTreeIncomplete) return;
     Group G;
-    if (!load_group(G, spos, batch, xcd_group(batch))) return;
-    trav_group(G, S, trav, meta, lists);
+    const long long g = xcd_group(batch);
+    if (!load_group(G, spos, batch, g)) return;
+    trav_group<kSplitStack>(G, g, S, trav, meta, lists);
 }
 
 template <bool QUAD, bool EPS0, bool Q16>
@@ -617,26 +670,28 @@ __global__ __launch_bounds__(64) void k_eval32(const double4* __restrict__ spos,
                                                const int32_t* __restrict__ order, Meta* meta, Batch batch,
                                                Lists lists, double* __restrict__ oax, double* __restrict__ oay,
                                                double* __restrict__ oaz) {
-    __shared__ EvalLDS S;
+    __shared__ EvalLDS<kSplitParts> S;
     if (meta->flags & (kFlagTreeIncomplete | kFlagStackOverflow | kFlagListOverflow)) return;
     Group G;
-    if (!load_group(G, spos, batch, xcd_group(batch))) return;
-    eval_group<QUAD, EPS0, Q16>(G, S, spos, trav, m32, m16, order, lists, oax, oay, oaz);
+    const long long g = xcd_group(batch);
+    if (!load_group(G, spos, batch, g)) return;
+    eval_group<QUAD, EPS0, Q16, kSplitParts>(G, g, S, spos, trav, m32, m16, order, lists, oax, oay, oaz);
 }
 
 // Traversal and evaluation of one group in one wavefront: the latency-bound
 // traversal of some waves overlaps the VALU-bound evaluation of others on the
-// same SIMD.  The lists still go through HBM (they stay in L2/MALL between
+// same SIMD.  The lists still go through memory (they stay in L2/MALL between
 // the two phases), so LDS holds only one phase's buffers at a time.  With
 // persistent set, a grid of resident waves takes groups from a work queue
-// (one launch for any number of groups and instances).
-template <bool QUAD, bool EPS0, bool Q16>
+// (one launch for any number of groups and instances), and each workgroup
+// reuses one list region (lists.perSlot).
+template <bool QUAD, bool EPS0, bool Q16, int STACK, int PARTS>
 __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
                                                const Mult32* __restrict__ m32, const Mult16* __restrict__ m16,
                                                const int32_t* __restrict__ order, Meta* meta, Batch batch,
                                                Lists lists, int persistent, double* __restrict__ oax,
                                                double* __restrict__ oay, double* __restrict__ oaz) {
-    __shared__ WalkLDS S;
+    __shared__ WalkLDS<STACK, PARTS> S;
     if (meta->flags & kFlagTreeIncomplete) return;
     long long g = persistent ? -1 : xcd_group(batch);
     for (;;) {
@@ -647,17 +702,51 @@ __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos,
         }
         Group G;
         if (!load_group(G, spos, batch, g)) return;
-        const bool ok = trav_group(G, S.t, trav, meta, lists);
+        const long long r = lists.perSlot ? (long long)blockIdx.x : g;
+        const bool ok = trav_group<STACK>(G, r, S.t, trav, meta, lists);
         __syncthreads();  // lists written by this wave's lanes are read back by others
-        if (ok) eval_group<QUAD, EPS0, Q16>(G, S.e, spos, trav, m32, m16, order, lists, oax, oay, oaz);
+        if (ok) eval_group<QUAD, EPS0, Q16, PARTS>(G, r, S.e, spos, trav, m32, m16, order, lists, oax, oay, oaz);
         if (!persistent) return;
         __syncthreads();
     }
 }
 
-int walk32_blocks_per_cu() {
+namespace {
+
+using WalkKernel = void (*)(const double4*, const TravNode*, const Mult32*, const Mult16*, const int32_t*, Meta*,
+                            Batch, Lists, int, double*, double*, double*);
+
+// variant: 0 quadrupole, 1 quadrupole + FP16-stored, 2 monopole; x2 for eps0
+template <int C>
+constexpr WalkKernel walk_kernel(int variant) {
+    constexpr int st = kWalkConfig[C].stack, pa = kWalkConfig[C].parts;
+    switch (variant) {
+        case 0: return k_walk32<true, false, false, st, pa>;
+        case 1: return k_walk32<true, false, true, st, pa>;
+        case 2: return k_walk32<false, false, false, st, pa>;
+        case 3: return k_walk32<true, true, false, st, pa>;
+        case 4: return k_walk32<true, true, true, st, pa>;
+        default: return k_walk32<false, true, false, st, pa>;
+    }
+}
+
+WalkKernel walk_kernel_of(int config, int variant) {
+    switch (config) {
+        case 0: return walk_kernel<0>(variant);
+        case 1: return walk_kernel<1>(variant);
+        case 2: return walk_kernel<2>(variant);
+        case 3: return walk_kernel<3>(variant);
+        case 4: return walk_kernel<4>(variant);
+        default: return walk_kernel<5>(variant);
+    }
+}
+
+}  // namespace
+
+int walk32_blocks_per_cu(int config) {
     int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_walk32<true, false, false>, 64, 0) !=
+    if (config < 0 || config >= kWalkConfigs) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)walk_kernel_of(config, 0), 64, 0) !=
         hipSuccess)
         return 0;
     return per;
@@ -666,36 +755,34 @@ int walk32_blocks_per_cu() {
 void launch_walk32(const double4* spos, const TravNode* trav, const Mult32* m32, const Mult16* m16,
                    const int32_t* order, Meta* meta, const Batch& batch, long long ngroupsUpper, bool eps0,
                    int quad, const Lists& lists, double* ax, double* ay, double* az, hipStream_t stream,
-                   hipEvent_t mid, bool split, int persistentBlocks) {
+                   hipEvent_t mid, bool split, int persistentBlocks, int config) {
     if (ngroupsUpper <= 0) return;
     const long long per = (ngroupsUpper + 7) / 8;
     const bool persistent = !split && persistentBlocks > 0;
     dim3 grid((unsigned)(persistent ? persistentBlocks : per * 8)), block(64);
     const bool q16 = m16 != nullptr;
-    const int pers = persistent ? 1 : 0;
     if (split) {
         hipLaunchKernelGGL(k_trav32, grid, block, 0, stream, spos, trav, meta, batch, lists);
         if (mid) (void)hipEventRecord(mid, stream);
+#define GX_EVAL(Q, E, H) \
+    hipLaunchKernelGGL((k_eval32<Q, E, H>), grid, block, 0, stream, spos, trav, m32, m16, order, meta, batch, lists, \
+                       ax, ay, az)
+        if (quad) {
+            if (q16) {
+                if (eps0) GX_EVAL(true, true, true); else GX_EVAL(true, false, true);
+            } else {
+                if (eps0) GX_EVAL(true, true, false); else GX_EVAL(true, false, false);
+            }
+        } else {
+            if (eps0) GX_EVAL(false, true, false); else GX_EVAL(false, false, false);
+        }
+#undef GX_EVAL
+        return;
     }
-#define GX_LAUNCH(Q, E, H)                                                                                    \
-    do {                                                                                                      \
-        if (split)                                                                                            \
-            hipLaunchKernelGGL((k_eval32<Q, E, H>), grid, block, 0, stream, spos, trav, m32, m16, order, meta, batch, \
-                               lists, ax, ay, az);                                                            \
-        else                                                                                                  \
-            hipLaunchKernelGGL((k_walk32<Q, E, H>), grid, block, 0, stream, spos, trav, m32, m16, order, meta, batch, \
-                               lists, pers, ax, ay, az);                                                      \
-    } while (0)
-#define GX_LAUNCH_E(Q, H) \
-    if (eps0) GX_LAUNCH(Q, true, H); else GX_LAUNCH(Q, false, H)
-    if (quad) {
-        if (q16) { GX_LAUNCH_E(true, true); } else { GX_LAUNCH_E(true, false); }
-    } else {
-        GX_LAUNCH_E(false, false);
-    }
-#undef GX_LAUNCH_E
-#undef GX_LAUNCH
-    if (!split && mid) (void)hipEventRecord(mid, stream);
+    const int variant = (quad ? (q16 ? 1 : 0) : 2) + (eps0 ? 3 : 0);
+    hipLaunchKernelGGL(walk_kernel_of(config, variant), grid, block, 0, stream, spos, trav, m32, m16, order, meta,
+                       batch, lists, persistent ? 1 : 0, ax, ay, az);
+    if (mid) (void)hipEventRecord(mid, stream);
 }
 
 }  // namespace gx

@@ -145,7 +145,7 @@ class Solver:
     """
 
     def __init__(self, theta: float = 0.0, monopole_only: bool = False, walk: str = "fp32", timing: bool = False,
-                 group_size: int = 64, persistent: bool = True):
+                 group_size: int = 64, persistent: int = 1, stack: int = 768, parts: int = 256):
         reason = gpu_available()
         if reason is not None:
             raise RuntimeError(f"GPU backend unavailable: {reason}")
@@ -160,6 +160,8 @@ class Solver:
         self.timing = timing
         self.group_size = group_size
         self.persistent = persistent
+        self.stack = stack
+        self.parts = parts
 
     def __del__(self):
         try:
@@ -173,7 +175,27 @@ class Solver:
         w = {"fp32": _lib.GX_WALK_FP32, "fp64": _lib.GX_WALK_FP64, "fp32-q16": _lib.GX_WALK_FP32_Q16}[self.walk]
         _lib.check(self._L.gx_solver_config(self._h, float(self.theta), int(self.monopole_only), w,
                                             int(self.timing)))
-        _lib.check(self._L.gx_solver_tune(self._h, int(self.group_size), int(self.persistent)))
+        _lib.check(self._L.gx_solver_tune_walk(self._h, int(self.group_size), int(self.persistent), int(self.stack),
+                                               int(self.parts)))
+
+    def autotune(self, s: DeviceState, offsets, theta, eps, reps: int = 3, dark_matter_off=False) -> list:
+        """The library's walk autotuner on the systems packed in s (see
+        gx_solver_autotune): keeps the fastest candidate (group size, work
+        queue, stack and staging sizes) for this solver and returns the
+        timing table; s.ax/ay/az hold the chosen walk's forces."""
+        self._configure()
+        keep, off, b, th, ep = self._batch_args(offsets, theta, eps)
+        table = (_lib.GxTuneEntry * 64)()
+        n = ctypes.c_int32()
+        _lib.check(self._L.gx_solver_autotune(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.m), _ptr(s.kind), _ptr(s.alive), off, b, th, ep,
+            int(dark_matter_off), _ptr(s.ax), _ptr(s.ay), _ptr(s.az), int(reps), table, 64, ctypes.byref(n)))
+        rows = [{f: getattr(table[i], f) for f, _ in _lib.GxTuneEntry._fields_} for i in range(min(n.value, 64))]
+        if rows:
+            best = min(rows, key=lambda r: r["ms"])
+            self.group_size, self.persistent = best["group_size"], best["persistent"]
+            self.stack, self.parts = best["stack"], best["parts"]
+        return rows
 
     def reserve(self, nodes=0, full=0, partial=0, leaves=0):
         """Starting capacities of the node arena and the walk lists (0 keeps

@@ -85,6 +85,35 @@ int gx_solver_tune(gx_solver *s, int group_size, int persistent);
  * captured by gx_graph_capture_step must then be captured again. */
 int gx_solver_reserve(gx_solver *s, int64_t nodes, int32_t full, int32_t partial, int32_t leaves);
 
+/* Walk tuning with the FP32 walk's compile-time sizes (no counterpart in the
+ * reference): as gx_solver_tune, plus the traversal stack (entries, 384 or
+ * 768) and the stars staged per evaluation flush (128, 256 or 384); other
+ * pairs are GX_ERR_ARG.  Every choice gives the same interaction set. */
+int gx_solver_tune_walk(gx_solver *s, int group_size, int persistent, int stack, int parts);
+
+/* One timed candidate of the walk autotuner. */
+typedef struct {
+    int32_t group_size, persistent, stack, parts;
+    float ms;                 /* mean GPU time of one walk (INFINITY: stack overflow) */
+    int64_t pp_interactions;  /* the candidate's counts (equal for every candidate) */
+    int64_t pc_interactions;
+} gx_tune_entry;
+
+/* Walk autotuner (no counterpart in the reference): on the given bodies
+ * (device arrays and instances, as gx_batch_accelerations_dev) builds the
+ * tree, then times reps FP32 walks of it for every candidate -- group size
+ * 64 and 32, each stack/staging pair of gx_solver_tune_walk, work queue
+ * never (0) and always (2) -- and keeps the fastest in the solver.  The
+ * choice is cached per workload shape (body count, instance count, walk,
+ * monopole): a second call for the same shape applies it without timing.
+ * Fills up to table_len entries of table and *entries with the candidate
+ * count; ends with one force step of the chosen walk into ax, ay, az.  The
+ * FP64 walk has nothing to tune (*entries = 0). */
+int gx_solver_autotune(gx_solver *s, const double *x, const double *y, const double *z, const double *mass,
+                       const uint8_t *kind, const uint8_t *alive, const int64_t *offsets, int32_t count,
+                       const double *theta, const double *eps, int dark_matter_off, double *ax, double *ay,
+                       double *az, int32_t reps, gx_tune_entry *table, int32_t table_len, int32_t *entries);
+
 /* Uses the given hipStream_t (passed as void*) instead of the solver's own
  * stream; NULL restores the solver's stream. */
 int gx_solver_set_stream(gx_solver *s, void *stream);
