@@ -6,4 +6,4 @@ package builds it (``gravitas_amd.build``) and mirrors the reference's
 hydro's ``sph.Solver`` (``gravitas_amd.hydro``).
 """
 from .hydro import GasParams, GasState, Hydro, HydroOut  # noqa: F401
-from .solver import DeviceState, Params, Retard, Solver, gpu_available, gpu_name  # noqa: F401
+from .solver import DeviceState, Params, Retard, Solver, accelerations_noalloc, gpu_available, gpu_name  # noqa: F401

@@ -34,11 +34,12 @@ EXPORTS = (
     "gx_hydro_forces", "gx_hydro_get_stats", "gx_hydro_phase_ms", "gx_hydro_export",
     "gx_split_step_spin_dev", "gx_drag_outer_dev", "gx_drag_fast_dev", "gx_drag_get_stats",
     "gx_retard_new", "gx_retard_free", "gx_retard_count", "gx_retard_hole_appeared_dev", "gx_retard_source_pos_dev",
-    "gx_split_step_retarded_dev",
+    "gx_split_step_retarded_dev", "gx_accelerations_noalloc",
 )
 
 GX_GRAVITY = {"instant": 0, "retarded": 1, "retarded+v": 2}  # sim.GravityMode, internal/sim/retarded.go:21-28
 
+GX_NOALLOC_MAX_BODIES = 65536
 GX_BLOCK_MAX_LEVELS = 8
 GX_BLOCK_MAX_TICKS = 128
 GX_DIAG_VALUES = 11
@@ -135,6 +136,7 @@ def load():
     L.gx_solver_tune.argtypes = [P, I, I]
     L.gx_solver_reserve.argtypes = [P, I64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     L.gx_solver_tune_walk.argtypes = [P, I, I, I, I]
+    L.gx_accelerations_noalloc.argtypes = [P, P, P, P, P, P, I64, D, D, I, I, P, P, P, P]
     L.gx_solver_autotune.argtypes = [P, P, P, P, P, P, P, P, ctypes.c_int32, P, P, I, P, P, P, ctypes.c_int32,
                                      ctypes.POINTER(GxTuneEntry), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
     L.gx_batch_accelerations_dev.argtypes = [P, P, P, P, P, P, P, P, ctypes.c_int32, P, P, I, P, P, P]

@@ -52,8 +52,15 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// Set while gx_accelerations_noalloc runs: its buffers are slices of a
+// static scratch array, and a step that needs more must fail, not allocate.
+thread_local bool t_noAlloc = false;
+
 int ensure(DevBuf& b, size_t bytes) {
     if (bytes <= b.bytes && b.p) return GX_OK;
+    if (t_noAlloc)
+        return set_err(GX_ERR_CAPACITY, "a buffer of %zu bytes exceeds the no-allocation scratch (slice of %zu)", bytes,
+                       b.bytes);
     if (b.p) HIPCHK(hipFree(b.p));
     b.p = nullptr;
     size_t want = std::max<size_t>(bytes + bytes / 4, 256);
@@ -147,6 +154,11 @@ struct gx_solver {
     };
     std::vector<Tuned> tuned;
     bool nodesSet = false;  // gx_solver_reserve set the node capacity: no automatic floor
+    // the solver of gx_accelerations_noalloc: list capacities are fixed (an
+    // overflow repeats the walk in FP64 instead of growing them), the work
+    // queue uses at most maxSlots list regions, no events, no cached uploads
+    bool fixedCaps = false;
+    int maxSlots = 0;
 
     // Every device buffer the solver owns.
     std::vector<DevBuf*> buffers() {
@@ -360,7 +372,7 @@ int set_instances(gx_solver* s, int B, const int64_t* off, const double* theta, 
     if (B < 1) return set_err(GX_ERR_ARG, "need at least one instance");
     if (B > 65536) return set_err(GX_ERR_ARG, "at most 65536 instances");
     const size_t bytes = (size_t)(B + 1) * 8 + (size_t)B * 16;
-    HIPCHK(hipEventSynchronize(s->instCopied));  // the previous upload has read the staging
+    if (s->instCopied) HIPCHK(hipEventSynchronize(s->instCopied));  // the previous upload has read the staging
     if (s->instHostBytes < bytes) {
         if (s->instHost) HIPCHK(hipHostFree(s->instHost));
         s->instHost = nullptr;
@@ -387,7 +399,10 @@ int set_instances(gx_solver* s, int B, const int64_t* off, const double* theta, 
         (rc = ensure(s->groupOff, (size_t)(B + 1) * 8)) || (rc = ensure(s->root, (size_t)B * 4)) ||
         (rc = ensure(s->half, (size_t)B * 8)))
         return rc;
-    if (s->B != B || s->instLast.size() != bytes || memcmp(s->instLast.data(), s->instHost, bytes) != 0) {
+    if (!s->instCopied) {  // no upload cache (gx_accelerations_noalloc): copy and wait
+        HIPCHK(hipMemcpyAsync(s->instCfg.p, s->instHost, bytes, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    } else if (s->B != B || s->instLast.size() != bytes || memcmp(s->instLast.data(), s->instHost, bytes) != 0) {
         HIPCHK(hipMemcpyAsync(s->instCfg.p, s->instHost, bytes, hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipEventRecord(s->instCopied, s->stream));
         s->instLast.assign(s->instHost, s->instHost + bytes);
@@ -573,7 +588,8 @@ int enqueue_walk(gx_solver* s, int walk, const StepArgs& a, bool accumulate = tr
     if (walk == GX_WALK_FP32 || walk == GX_WALK_FP32_Q16) {
         const long long ng = (n + s->gsize - 1) / s->gsize + s->B;
         const int cfg = s->walkConfig;
-        const long long resident = (long long)std::max(s->cus, 1) * s->blocksPerCu[cfg];
+        long long resident = (long long)std::max(s->cus, 1) * s->blocksPerCu[cfg];
+        if (s->maxSlots > 0) resident = std::min<long long>(resident, s->maxSlots);
         // the work queue pays once there are more groups than resident
         // workgroups; below that, one workgroup per group (persistent 1).
         // persistent 2 always takes the queue.
@@ -621,7 +637,7 @@ int enqueue_step(gx_solver* s, const StepArgs& a) {
     HIPCHK(hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, s->stream));
     s->lastWalk = s->walk;
     s->pending = true;
-    s->lastEvents = kStepEvents;
+    s->lastEvents = s->capturing ? kNoEvents : kStepEvents;
     return GX_OK;
 }
 
@@ -699,7 +715,7 @@ bool regrow(gx_solver* s) {
         s->levels = kMortonBits;
         return true;
     }
-    if (h.flags & kFlagListOverflow) {
+    if ((h.flags & kFlagListOverflow) && !s->fixedCaps) {
         s->cf = std::max(s->cf, (int)(h.maxList[0] * 1.25) + 64);
         s->cp = std::max(s->cp, (int)(h.maxList[1] * 1.25) + 64);
         s->cl = std::max(s->cl, (int)(h.maxList[2] * 1.25) + 64);
@@ -716,9 +732,10 @@ int run_dev(gx_solver* s, const StepArgs& a) {
         rc = finish_step(s);
         if (rc && rc != GX_ERR_CAPACITY) return rc;
         if (regrow(s)) continue;
-        if (s->hmeta->flags & kFlagStackOverflow) {
-            // the FP32 walk's LDS stack was too small for this tree: redo the
-            // walk with the FP64 per-particle walk (same interaction set)
+        if (s->hmeta->flags & (kFlagStackOverflow | (s->fixedCaps ? kFlagListOverflow : 0))) {
+            // the FP32 walk's LDS stack (or a fixed list) was too small for this
+            // tree: redo the walk with the FP64 per-particle walk (same
+            // interaction set)
             s->fallbacks++;
             if ((rc = enqueue_walk(s, GX_WALK_FP64, a, false))) return rc;
             HIPCHK(hipStreamSynchronize(s->stream));
@@ -1983,6 +2000,157 @@ int gx_phase_ms(const gx_solver* s, float* ms, int n) {
     int k = std::min(n, kPhases);
     for (int i = 0; i < k; i++) ms[i] = s->phase[i];
     return k;
+}
+
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Force step without allocation: every buffer of the step is a slice of one
+// static device array per GPU (gx_accelerations_noalloc).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr long long kNaBodies = GX_NOALLOC_MAX_BODIES;
+constexpr long long kNaNodes = kNaBodies + kNaBodies / 4;  // clustered trees need more than 0.2 nodes per star
+constexpr int kNaSlots = 256;                             // list regions of the work queue
+constexpr int kNaFull = 3072, kNaPart = 3072, kNaLeaf = 768;  // fixed list capacities per region
+constexpr long long kNaTiles = (kNaBodies + kSortTile - 1) / kSortTile;
+constexpr long long kNaCollect = (kNaBodies + kCollectTile - 1) / kCollectTile;
+
+constexpr size_t na_round(size_t b) { return (b + 255) / 256 * 256; }
+
+// slice sizes, in the order they are carved
+constexpr size_t kNaSizes[] = {
+    na_round(8 * kNaBodies), na_round(8 * kNaBodies),    // keys, keys2
+    na_round(4 * kNaBodies), na_round(4 * kNaBodies),    // order, order2
+    na_round(4 * kNaBodies), na_round(4 * kNaBodies),    // rank, holes
+    na_round(256 * 4 * kNaTiles), na_round(256 * 4),     // hist, digitTotal
+    na_round(4 * kNaCollect), na_round(4 * kNaCollect),  // blockStar, blockHole
+    na_round(32 * kNaBodies), na_round(sizeof(Meta)),    // spos, meta
+    na_round(sizeof(TravNode) * kNaNodes), na_round(sizeof(Mult64) * kNaNodes),  // trav, m64
+    na_round(sizeof(Mult32) * kNaNodes), na_round(sizeof(Mult16) * kNaNodes),    // m32, m16
+    na_round(4ull * kNaSlots * kNaFull), na_round(12ull * kNaSlots * kNaPart),    // lfull, lpart
+    na_round(12ull * kNaSlots * kNaLeaf), na_round(sizeof(GroupCounts) * kNaSlots),  // lleaf, lcounts
+    256, 256, 256, 256, 256, 256, 256,  // instCfg, boxEnc, starCount, starOff, groupOff, root, half (one instance)
+};
+constexpr size_t na_total() {
+    size_t t = 0;
+    for (size_t b : kNaSizes) t += b;
+    return t;
+}
+static_assert(na_total() <= (size_t)GX_NOALLOC_SCRATCH_BYTES, "the no-allocation scratch is too small");
+
+}  // namespace
+
+// The scratch: one copy per GPU, held for the life of the process.  Host code
+// never uses this name as an address; it asks each device for the array's
+// address (hipGetSymbolAddress) and uses that.
+static __device__ __align__(256) unsigned char scratch[GX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kNaDevices = 16;
+
+struct NoAllocDevice {
+    std::once_flag once;
+    int rc = GX_OK;
+    char msg[256] = "";
+    std::mutex lock;  // calls on one device are serialised on the host
+    gx_solver s;
+    Meta hmeta;
+    alignas(64) char inst[256];
+};
+NoAllocDevice g_na[kNaDevices];
+
+// Carves the device's scratch into the solver's buffers (once per device,
+// with that device current).
+void na_setup(NoAllocDevice& d) {
+    unsigned char* base = nullptr;
+    hipError_t e = hipGetSymbolAddress((void**)&base, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !base) {
+        d.rc = GX_ERR_HIP;
+        snprintf(d.msg, sizeof d.msg, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        return;
+    }
+    gx_solver& s = d.s;
+    DevBuf* order[] = {&s.keys, &s.keys2, &s.order, &s.order2, &s.rank, &s.holes, &s.hist, &s.digitTotal,
+                       &s.blockStar, &s.blockHole, &s.spos, &s.meta, &s.trav, &s.m64, &s.m32, &s.m16,
+                       &s.lfull, &s.lpart, &s.lleaf, &s.lcounts, &s.instCfg, &s.boxEnc, &s.starCount,
+                       &s.starOff, &s.groupOff, &s.root, &s.half};
+    static_assert(sizeof(order) / sizeof(order[0]) == sizeof(kNaSizes) / sizeof(kNaSizes[0]), "one size per slice");
+    size_t off = 0;
+    for (size_t i = 0; i < sizeof(kNaSizes) / sizeof(kNaSizes[0]); i++) {
+        order[i]->p = base + off;
+        order[i]->bytes = kNaSizes[i];
+        off += kNaSizes[i];
+    }
+    s.hmeta = &d.hmeta;
+    s.instHost = d.inst;
+    s.instHostBytes = sizeof d.inst;
+    s.instCopied = nullptr;  // no upload cache
+    s.capturing = true;      // no events
+    s.fixedCaps = true;
+    s.allLevels = true;      // no repeat for a deeper tree
+    s.persistent = 2;
+    s.maxSlots = kNaSlots;
+    s.cf = kNaFull;
+    s.cp = kNaPart;
+    s.cl = kNaLeaf;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&s.cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        s.cus = 1;
+    for (int c = 0; c < kWalkConfigs; c++) s.blocksPerCu[c] = std::max(walk32_blocks_per_cu(c), 1);
+    // the step metadata starts at zero (its sticky flags above all)
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, nullptr, (unsigned long long*)s.meta.p,
+                       (int)(sizeof(Meta) / 8), 0ull);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) {
+        d.rc = GX_ERR_HIP;
+        snprintf(d.msg, sizeof d.msg, "scratch set-up: %s", hipGetErrorString(e));
+    }
+}
+
+struct NoAllocScope {
+    NoAllocScope() { t_noAlloc = true; }
+    ~NoAllocScope() { t_noAlloc = false; }
+};
+
+}  // namespace
+
+extern "C" {
+
+int gx_accelerations_noalloc(const double* x, const double* y, const double* z, const double* mass,
+                             const uint8_t* kind, const uint8_t* alive, int64_t n, double theta, double eps,
+                             int dark_matter_off, int walk, double* ax, double* ay, double* az, void* stream) {
+    if (n < 0 || n > kNaBodies)
+        return set_err(GX_ERR_ARG, "n = %lld: the no-allocation entry point takes at most %lld bodies", (long long)n,
+                       kNaBodies);
+    if (!(theta >= 0) || walk < GX_WALK_FP32 || walk > GX_WALK_FP32_Q16)
+        return set_err(GX_ERR_ARG, "bad theta %g or walk %d", theta, walk);
+    int rc = device_ready();
+    if (rc) return rc;
+    if (n == 0) return GX_OK;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNaDevices) return set_err(GX_ERR_ARG, "device %d beyond the %d with a scratch", dev, kNaDevices);
+    NoAllocDevice& d = g_na[dev];
+    std::call_once(d.once, [&d] { na_setup(d); });
+    if (d.rc) return set_err(d.rc, "%s", d.msg);
+    std::lock_guard<std::mutex> guard(d.lock);
+    NoAllocScope scope;
+    gx_solver* s = &d.s;
+    s->stream = (hipStream_t)stream;
+    s->theta = theta;
+    s->monopole = 0;
+    s->walk = walk;
+    s->timing = 0;
+    s->pending = false;
+    const StepArgs a = step_args(x, y, z, mass, kind, alive, n, dark_matter_off, ax, ay, az);
+    if ((rc = check_arrays(a)) || (rc = set_single(s, n, theta, eps))) return rc;
+    return run_dev(s, a);
 }
 
 }  // extern "C"

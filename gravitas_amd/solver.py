@@ -48,6 +48,18 @@ def gpu_available():
     return _avail
 
 
+def accelerations_noalloc(s: "DeviceState", p: "Params", theta: float = 0.0, walk: str = "fp32", stream=None):
+    """Accelerations of s (at most GX_NOALLOC_MAX_BODIES bodies) through the
+    entry point that allocates no device memory (gx_accelerations_noalloc):
+    every buffer is a slice of the library's static per-GPU scratch."""
+    L = _lib.load()
+    w = {"fp32": _lib.GX_WALK_FP32, "fp64": _lib.GX_WALK_FP64, "fp32-q16": _lib.GX_WALK_FP32_Q16}[walk]
+    _lib.check(L.gx_accelerations_noalloc(
+        _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.m), _ptr(s.kind), _ptr(s.alive), len(s), float(theta),
+        float(p.softening), int(p.dark_matter_off), w, _ptr(s.ax), _ptr(s.ay), _ptr(s.az),
+        ctypes.c_void_p(stream) if stream else None))
+
+
 def gpu_name() -> str:
     if gpu_available() is not None:
         return ""

@@ -379,6 +379,28 @@ int gx_direct_sample_dev(gx_solver *s, const double *x, const double *y, const d
                          const uint8_t *kind, int64_t n, const int64_t *targets, int64_t nt, double eps,
                          int dark_matter_off, double *out);
 
+/* Force step without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid allocation).  As gx_accelerations_dev on device
+ * arrays of at most GX_NOALLOC_MAX_BODIES bodies, with theta (0: 0.9), the
+ * softening and the walk (GX_WALK_*) given per call, launched on `stream`
+ * (a hipStream_t as void*, NULL: the default stream) and synchronous.  Every
+ * buffer of the step is a slice of one static device array of
+ * GX_NOALLOC_SCRATCH_BYTES per GPU, which each GPU the library runs on holds
+ * until the process ends; no hipMalloc, hipFree or other allocating call is
+ * made.  The FP32 walk's lists have fixed capacities: a group that overflows
+ * them is walked again by the FP64 walk (same interaction set).  A tree that
+ * needs more nodes than the scratch holds is GX_ERR_CAPACITY; more than
+ * GX_NOALLOC_MAX_BODIES bodies is GX_ERR_ARG.  Calls share their GPU's
+ * scratch, and each leaves its contents for the next: calls on one GPU are
+ * serialised (a later call waits for an earlier one); work that overlaps a
+ * call on the same GPU and touches its scratch races with it. */
+#define GX_NOALLOC_MAX_BODIES 65536
+#define GX_NOALLOC_SCRATCH_BYTES (34u << 20)
+int gx_accelerations_noalloc(const double *x, const double *y, const double *z, const double *mass,
+                             const uint8_t *kind, const uint8_t *alive, int64_t n, double theta, double eps,
+                             int dark_matter_off, int walk, double *ax, double *ay, double *az, void *stream);
+
 /* LastGPUSeconds (gpu_darwin.go:155): GPU time of the last force step
  * (build and walk, from its first kernel to its last; after gx_graph_launch,
  * the mean per replay), recorded on every step whatever the timing setting;
