@@ -153,13 +153,17 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
     return v;
 }
-// Inclusive prefix sum across the wave (Hillis-Steele with shuffles).
+// Inclusive prefix sum across the wave (every lane active), with DPP moves:
+// shifts by 1, 2, 4 and 8 within each row of 16 lanes, then the last lane of
+// rows 0 and 2 added to rows 1 and 3, and the last lane of row 1 to rows 2
+// and 3.  Lanes a move has no source for add the old value 0.
 __device__ __forceinline__ int wave_incl_scan(int v) {
-    int lane = lane_id();
-    for (int d = 1; d < kWave; d <<= 1) {
-        int t = __shfl_up(v, d, kWave);
-        if (lane >= d) v += t;
-    }
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
     return v;
 }
 

@@ -476,25 +476,46 @@ struct Sums {
 
 // Stars of up to 64 leaf ranges (one per lane: lcur, lcnt stars, mask lmask)
 // into the staging buffer, which is evaluated (flush) whenever it is full.
+// All lanes copy: star j of the batch goes to lane j % 64, which finds the
+// range holding it by a binary search over the ranges' exclusive prefix sums
+// (tables in the cell buffer, which is free while leaves are staged).
 template <bool MASKED, int PARTS, typename Flush>
 __device__ __forceinline__ void stage_leaves(EvalLDS<PARTS>& S, int& nst, int lcur, int lcnt, uint64_t lmask,
                                              const double4* __restrict__ spos, const Group& G, Flush flush) {
-    while (__ballot(lcnt > 0)) {
-        const int room = PARTS - nst;
-        const int incl = wave_incl_scan(lcnt);
-        const int w = max(0, min(lcnt, room - (incl - lcnt)));
-        const int at = nst + incl - lcnt;
-        for (int i = 0; i < w; i++) {
-            const double4 q = spos[lcur + i];
-            S.part[at + i] = make_float4((float)(q.x - G.cgx), (float)(q.y - G.cgy), (float)(q.z - G.cgz),
+    int* tStart = (int*)S.cell;  // exclusive prefix of the counts
+    int* tCur = tStart + kWave;  // first star of each range
+    uint64_t* tMask = (uint64_t*)(tCur + kWave);
+    const int lane = lane_id();
+    const int incl = wave_incl_scan(lcnt);
+    const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, kWave));
+    if (total == 0) return;
+    tStart[lane] = incl - lcnt;
+    tCur[lane] = lcur;
+    if (MASKED) tMask[lane] = lmask;
+    __syncthreads();
+    for (int done = 0; done < total;) {
+        const int take = min(PARTS - nst, total - done);
+        for (int j0 = 0; j0 < take; j0 += kWave) {
+            if (j0 + lane < take) {
+                const int j = done + j0 + lane;
+                // the last range starting at or before j (empty ranges share
+                // the next one's start and are passed over)
+                int o = 0;
+#pragma unroll
+                for (int step = kWave / 2; step > 0; step >>= 1)
+                    if (tStart[o + step] <= j) o += step;
+                const double4 q = spos[tCur[o] + (j - tStart[o])];
+                const int at = nst + j0 + lane;
+                S.part[at] = make_float4((float)(q.x - G.cgx), (float)(q.y - G.cgy), (float)(q.z - G.cgz),
                                          (float)q.w);
-            if (MASKED) S.pmask[at + i] = lmask;
+                if (MASKED) S.pmask[at] = tMask[o];
+            }
         }
-        nst += min(room, __shfl(incl, 63, kWave));
-        lcur += w;
-        lcnt -= w;
+        nst += take;
+        done += take;
         if (nst == PARTS) flush();
     }
+    __syncthreads();  // the tables are rewritten by the next batch
 }
 
 template <bool QUAD, bool EPS0, bool Q16, int PARTS>
