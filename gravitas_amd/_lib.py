@@ -34,7 +34,7 @@ EXPORTS = (
     "gx_hydro_forces", "gx_hydro_get_stats", "gx_hydro_phase_ms", "gx_hydro_export",
     "gx_split_step_spin_dev", "gx_drag_outer_dev", "gx_drag_fast_dev", "gx_drag_get_stats",
     "gx_retard_new", "gx_retard_free", "gx_retard_count", "gx_retard_hole_appeared_dev", "gx_retard_source_pos_dev",
-    "gx_split_step_retarded_dev", "gx_accelerations_noalloc",
+    "gx_split_step_retarded_dev", "gx_accelerations_noalloc", "gx_potential_exact_dev",
 )
 
 GX_GRAVITY = {"instant": 0, "retarded": 1, "retarded+v": 2}  # sim.GravityMode, internal/sim/retarded.go:21-28
@@ -125,6 +125,7 @@ def load():
         getattr(L, f).restype = I
     L.gx_sync.argtypes = [P]
     L.gx_potential_energy_dev.argtypes = [P, P, P, P, P, P, P, I64, D, I, D, P]
+    L.gx_potential_exact_dev.argtypes = [P, P, P, P, P, P, I64, D, I, D, P]
     L.gx_direct_sample_dev.argtypes = [P, P, P, P, P, P, I64, P, I64, D, I, P]
     L.gx_last_seconds.argtypes = [P]
     L.gx_last_seconds.restype = D

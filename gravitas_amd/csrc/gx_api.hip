@@ -1964,6 +1964,36 @@ int gx_direct_sample_dev(gx_solver* s, const double* x, const double* y, const d
     return GX_OK;
 }
 
+int gx_potential_exact_dev(gx_solver* s, const double* x, const double* y, const double* z, const double* mass,
+                           const uint8_t* kind, int64_t n, double eps, int dark_matter_off, double c, double* out) {
+    if (!s || !out) return set_err(GX_ERR_ARG, "null argument");
+    if (n < 0 || n > (1LL << 31) - 1) return set_err(GX_ERR_ARG, "bad n");
+    if (n > 0 && (!x || !y || !z || !mass || !kind)) return set_err(GX_ERR_ARG, "null array");
+    if (s->pending) (void)finish_step(s);
+    *out = 0;
+    if (n < 2) return GX_OK;
+    const long long rowBlocks = (n + 255) / 256;
+    // enough workgroups to fill the device; slices of at least 1,024 columns
+    int slices = (int)std::max(1LL, std::min<long long>(4096 / rowBlocks + 1, (n + 1023) / 1024));
+    int rc;
+    if ((rc = ensure(s->partial, (size_t)slices * n * 8))) return rc;
+    hipLaunchKernelGGL(k_potential_exact, dim3((unsigned)rowBlocks, (unsigned)slices), dim3(256), 0, s->stream, x, y,
+                       z, mass, kind, (long long)n, eps * eps, dark_matter_off, c, slices, (double*)s->partial.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h((size_t)slices * n);
+    HIPCHK(hipMemcpyAsync(h.data(), s->partial.p, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    // fixed order: rows in state order, each row's slices in column order
+    double sum = 0;
+    for (long long i = 0; i < n; i++) {
+        double row = 0;
+        for (int k = 0; k < slices; k++) row += h[(size_t)k * n + i];
+        sum += row;
+    }
+    *out = sum;
+    return GX_OK;
+}
+
 double gx_last_seconds(const gx_solver* s) { return s ? s->lastSeconds : 0; }
 
 int gx_get_stats(const gx_solver* s, gx_stats* out) {

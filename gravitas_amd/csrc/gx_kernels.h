@@ -76,6 +76,9 @@ __global__ void k_direct_sample(const double* x, const double* y, const double* 
                                 const uint8_t* kind, long long n, const long long* targets, int nt, double eps2,
                                 int dmOff, int splits, double* partial);
 __global__ void k_direct_reduce(const double* partial, int nt, int splits, double* out);
+__global__ void k_potential_exact(const double* x, const double* y, const double* z, const double* m,
+                                  const uint8_t* kind, long long n, double eps2, int dmOff, double c, int slices,
+                                  double* partial);
 
 // gx_drag.hip: frame dragging (internal/sim/framedrag.go)
 struct DragArgs {

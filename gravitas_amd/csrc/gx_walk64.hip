@@ -249,6 +249,61 @@ __global__ __launch_bounds__(256) void k_direct_sample(const double* __restrict_
     }
 }
 
+// Exact pair-model potential, PotentialEnergyParallel (diag.go:44-64): every
+// pair i < j of the state, alive or not, with source masses (dark matter off
+// pulls nothing) and PairRS from the raw masses (Paczynski-Wiita star-hole
+// pairs for c > 0).  A workgroup owns 256 rows i and one slice of the columns
+// j; partial[slice][i] holds row i's sum over the slice, in column order.
+__global__ __launch_bounds__(256) void k_potential_exact(const double* __restrict__ x, const double* __restrict__ y,
+                                                         const double* __restrict__ z, const double* __restrict__ m,
+                                                         const uint8_t* __restrict__ kind, long long n, double eps2,
+                                                         int dmOff, double c, int slices,
+                                                         double* __restrict__ partial) {
+    __shared__ double4 src[kDirTile];  // x, y, z, source mass
+    __shared__ double raw[kDirTile];   // mass (for the hole's radius)
+    __shared__ int knd[kDirTile];
+    const int tid = threadIdx.x;
+    const long long i = (long long)blockIdx.x * 256 + tid;
+    const long long per = (n + slices - 1) / slices;
+    const long long jlo = (long long)blockIdx.y * per;
+    const long long jhi = min(n, jlo + per);
+    double xi = 0, yi = 0, zi = 0, mi = 0, rmi = 0;
+    int ki = 0;
+    if (i < n) {
+        xi = x[i];
+        yi = y[i];
+        zi = z[i];
+        ki = kind[i];
+        rmi = m[i];
+        mi = (dmOff && ki == kDark) ? 0.0 : rmi;
+    }
+    double part = 0;
+    // columns j > i only: the workgroup's first row bounds where its tiles start
+    const long long first = max(jlo, (long long)blockIdx.x * 256 + 1);
+    for (long long j0 = first; j0 < jhi; j0 += kDirTile) {
+        const long long j = j0 + tid;
+        __syncthreads();
+        if (j < jhi) {
+            const int kj = kind[j];
+            const double mj = m[j];
+            src[tid] = make_double4(x[j], y[j], z[j], (dmOff && kj == kDark) ? 0.0 : mj);
+            raw[tid] = mj;
+            knd[tid] = kj;
+        }
+        __syncthreads();
+        const int cnt = (int)min((long long)kDirTile, jhi - j0);
+        if (i < n)
+            for (int u = 0; u < cnt; u++) {
+                if (j0 + u <= i) continue;
+                const double4 q = src[u];
+                const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+                const double rs = pair_rs(ki, rmi, knd[u], raw[u], c);
+                part += pair_potential(mi, q.w, dx * dx + dy * dy + dz * dz, eps2, rs);
+            }
+    }
+    if (i < n) partial[(size_t)blockIdx.y * n + i] = part;
+}
+
 __global__ void k_direct_reduce(const double* __restrict__ partial, int nt, int splits, double* __restrict__ out) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;

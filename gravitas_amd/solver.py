@@ -436,6 +436,15 @@ class Solver:
             float(p.softening), int(p.dark_matter_off), float(p.c), ctypes.byref(out)))
         return out.value
 
+    def potential_exact(self, s: DeviceState, p: Params) -> float:
+        """PotentialEnergyParallel (diag.go:44-64): the exact pair-model
+        potential over every pair, FP64 on the GPU."""
+        out = ctypes.c_double()
+        _lib.check(self._L.gx_potential_exact_dev(
+            self._h, _ptr(s.px), _ptr(s.py), _ptr(s.pz), _ptr(s.m), _ptr(s.kind), len(s), float(p.softening),
+            int(p.dark_matter_off), float(p.c), ctypes.byref(out)))
+        return out.value
+
     # -- FP64 direct-sum oracle kernel (direct.go:25-48) ------------------------------
     def direct_sample(self, s: DeviceState, p: Params, targets):
         """FP64 direct-sum accelerations of the given target indices (host array

@@ -372,6 +372,17 @@ int gx_potential_energy_dev(gx_solver *s, const double *x, const double *y, cons
                             const uint8_t *kind, const uint8_t *alive, int64_t n, double eps, int dark_matter_off,
                             double c, double *out);
 
+/* PotentialEnergyParallel (internal/sim/diag.go:44-64): the exact potential
+ * energy of the pair model (pair.go) over every pair of the n bodies (alive
+ * or not, as the reference's loop), FP64 on the GPU: softened Newtonian
+ * pairs, Paczynski-Wiita star-hole pairs for c > 0, dark matter with
+ * dark_matter_off pulling nothing.  TotalEnergy (diag.go:31-41) uses it up to
+ * 30,000 bodies and the tree estimate (gx_potential_energy_dev) above.  The
+ * rows' sums are added in state order (the reference adds chunk sums in
+ * completion order). */
+int gx_potential_exact_dev(gx_solver *s, const double *x, const double *y, const double *z, const double *mass,
+                           const uint8_t *kind, int64_t n, double eps, int dark_matter_off, double c, double *out);
+
 /* FP64 direct sum (internal/sim/direct/direct.go:25-48) for nt target
  * indices (device int64) against all n bodies (device arrays); out is a
  * device array of nt*3 doubles (ax, ay, az per target).  Synchronous. */

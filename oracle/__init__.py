@@ -83,7 +83,8 @@ def lib():
         L.or_direct.argtypes = [_D, _D, _D, _D, _U8, ctypes.c_int64, _I64, ctypes.c_int64, ctypes.c_double,
                                 ctypes.c_int, ctypes.c_int, _D, _D, _D]
         L.or_potential_exact.restype = ctypes.c_double
-        L.or_potential_exact.argtypes = [_D, _D, _D, _D, _U8, ctypes.c_int64, ctypes.c_double, ctypes.c_int]
+        L.or_potential_exact.argtypes = [_D, _D, _D, _D, _U8, ctypes.c_int64, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_double]
         L.or_radix_sort.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int32),
                                     ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int32), ctypes.c_int64]
         L.or_morton_key.restype = ctypes.c_uint64
@@ -312,10 +313,12 @@ def direct(x, y, z, m, kind=None, eps=0.0, dm_off=False, targets=None, workers=1
     return np.stack([ax, ay, az], axis=1)
 
 
-def potential_exact(x, y, z, m, kind=None, eps=0.0, dm_off=False):
-    """Exact softened star-star potential energy (diag.go:44-64, no holes)."""
+def potential_exact(x, y, z, m, kind=None, eps=0.0, dm_off=False, c=0.0):
+    """PotentialEnergyParallel (diag.go:44-64): the exact pair-model potential
+    over all pairs (Paczynski-Wiita star-hole pairs for c > 0)."""
     x, y, z, m, kind, _ = _state_arrays(x, y, z, m, kind, None)
-    return float(lib().or_potential_exact(_dp(x), _dp(y), _dp(z), _dp(m), _u8(kind), len(x), eps, int(dm_off)))
+    return float(lib().or_potential_exact(_dp(x), _dp(y), _dp(z), _dp(m), _u8(kind), len(x), eps, int(dm_off),
+                                          float(c)))
 
 
 def pair_force(m, r2, eps2, rs):

@@ -640,18 +640,23 @@ void or_drift(int64_t n, double *px, double *py, double *pz, const double *vx, c
     }
 }
 
-/* Exact pair potential of stars (no holes, c <= 0): diag.go:44-64 restricted
- * to the star-star pair model. */
+/* PotentialEnergyParallel, diag.go:44-64: the exact pair model over every
+ * pair i < j of the state (alive or not, as the reference's loop), source
+ * masses (dark matter off pulls nothing), PairRS from the raw masses.  The
+ * reference adds chunk sums in completion order; here rows in order. */
+static double pair_rs(int ki, double mi, int kj, double mj, double c);
 double or_potential_exact(const double *px, const double *py, const double *pz, const double *mass,
-                          const uint8_t *kind, int64_t n, double eps, int dm_off) {
+                          const uint8_t *kind, int64_t n, double eps, int dm_off, double c) {
     double eps2 = eps * eps, sum = 0;
     for (int64_t i = 0; i < n; i++) {
         double mi = (kind[i] == KIND_DARK && dm_off) ? 0 : mass[i];
+        double xi = px[i], yi = py[i], zi = pz[i];
         double part = 0;
         for (int64_t j = i + 1; j < n; j++) {
             double mj = (kind[j] == KIND_DARK && dm_off) ? 0 : mass[j];
-            double dx = px[j] - px[i], dy = py[j] - py[i], dz = pz[j] - pz[i];
-            part += pair_potential(mi, mj, dx * dx + dy * dy + dz * dz, eps2, 0);
+            double dx = px[j] - xi, dy = py[j] - yi, dz = pz[j] - zi;
+            double rs = pair_rs(kind[i], mass[i], kind[j], mass[j], c);
+            part += pair_potential(mi, mj, dx * dx + dy * dy + dz * dz, eps2, rs);
         }
         sum += part;
     }

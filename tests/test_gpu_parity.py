@@ -261,6 +261,33 @@ def test_potential_energy_with_holes_matches_oracle(gpu, c):
     assert abs(u - want) <= 1e-12 * abs(want)
 
 
+@pytest.mark.parametrize("c", [0.0, 3.0])
+def test_exact_potential_matches_oracle(gpu, c):
+    """PotentialEnergyParallel (diag.go:44-64) on the GPU: every pair of the
+    state (dead bodies included, as the reference's loop), holes with
+    Paczynski-Wiita pairs for c > 0, dark matter off; within 1e-12 of the
+    oracle (the rows' partial sums are added in another order), and equal to
+    the tree potential at tiny theta when every body is alive."""
+    s = ics.plummer(7000, 63)
+    s["kind"][[5, 2500, 6999]] = ics.KIND_HOLE
+    s["m"][[5, 2500, 6999]] = [0.05, 0.02, 0.03]
+    s["kind"][3000:3500] = ics.KIND_DARK
+    s["alive"] = np.ones(len(s["px"]), np.uint8)
+    s["alive"][[10, 4000]] = 0
+    for dm_off in (False, True):
+        say(f"[parity] exact potential, c={c}, dark matter off {dm_off}")
+        ds = DeviceState.from_host(s)
+        p = Params(softening=0.01, c=c, dark_matter_off=dm_off)
+        u = Solver().potential_exact(ds, p)
+        want = oracle.potential_exact(s["px"], s["py"], s["pz"], s["m"], s["kind"], eps=0.01, dm_off=dm_off, c=c)
+        assert abs(u - want) <= 1e-12 * abs(want)
+    s["alive"][:] = 1
+    ds = DeviceState.from_host(s)
+    p = Params(softening=0.01, c=c)
+    tree = Solver(theta=1e-9).potential_energy(ds, p)
+    assert abs(Solver().potential_exact(ds, p) - tree) <= 1e-10 * abs(tree)
+
+
 def test_repeat_is_deterministic(gpu):
     s = ics.clumps(20000, 8, 71, scale=0.01)
     say("[parity] determinism")

@@ -361,6 +361,34 @@ def test_pair_model_closed_forms():
     assert abs(flo - fhi) <= 1e-9 * fhi
 
 
+def test_exact_potential_closed_forms():
+    """PotentialEnergyParallel on two and three bodies: a star and a hole at
+    r >= 1.5 rs give -m1 m2 / (r - rs) with rs = 2 m_hole / c^2; inside, the
+    linear continuation; two holes and two stars are Newtonian; dark matter
+    switched off has source mass 0, so its pairs vanish, and dark matter
+    pairs with a hole are Paczynski-Wiita (pair.go:13-22)."""
+    K = np.array([ics.KIND_STAR, ics.KIND_HOLE], np.uint8)
+    x, z = np.array([0.0, 3.0]), np.array([0.0, 4.0])
+    m = np.array([2.0, 5.0])
+    c = 10.0
+    rs = 2 * 5.0 / c ** 2
+    got = oracle.potential_exact(x, np.zeros(2), z, m, K, eps=0.0, c=c)
+    assert abs(got - (-10.0 / (5.0 - rs))) <= 1e-15 * 10
+    # inside 1.5 rs (c small): m1 m2 (-2/rs + 4/rs^2 (r - 1.5 rs))
+    c2 = 1.5
+    rs2 = 2 * 5.0 / c2 ** 2
+    assert 5.0 < 1.5 * rs2
+    got2 = oracle.potential_exact(x, np.zeros(2), z, m, K, eps=0.0, c=c2)
+    assert abs(got2 - 10.0 * (-2 / rs2 + 4 / rs2 ** 2 * (5.0 - 1.5 * rs2))) <= 1e-14 * abs(got2)
+    for kinds in ([ics.KIND_HOLE, ics.KIND_HOLE], [ics.KIND_STAR, ics.KIND_STAR]):
+        assert oracle.potential_exact(x, np.zeros(2), z, m, np.array(kinds, np.uint8), eps=0.0, c=c) == -2.0
+    kd = np.array([ics.KIND_DARK, ics.KIND_HOLE, ics.KIND_STAR], np.uint8)
+    x3, m3 = np.array([0.0, 3.0, 7.0]), np.array([2.0, 5.0, 1.0])
+    on = oracle.potential_exact(x3, np.zeros(3), np.zeros(3), m3, kd, eps=0.0, c=c)
+    off = oracle.potential_exact(x3, np.zeros(3), np.zeros(3), m3, kd, eps=0.0, dm_off=True, c=c)
+    assert abs(off - (-5.0 / (4.0 - rs))) <= 1e-15 and abs(on - (off - 10.0 / (3.0 - rs) - 2.0 / 7.0)) <= 1e-14
+
+
 def _pair_model_potential(px, py, pz, m, kind, eps, c):
     """diag.go:44-64 in numpy: the sum over all pairs of PairPotential with
     PairRS (pair.go:13-47): Newtonian for star-star and hole-hole pairs,
@@ -392,6 +420,10 @@ def test_potential_energy_with_holes_matches_pair_model(c):
     want = _pair_model_potential(s["px"], s["py"], s["pz"], s["m"], s["kind"], 0.02, c)
     got = oracle.Solver(theta=1e-9).potential_energy(s["px"], s["py"], s["pz"], s["m"], s["kind"], eps=0.02, c=c)
     assert abs(got - want) <= 1e-10 * abs(want)
+    # PotentialEnergyParallel (diag.go:44-64), the exact pair loop: the same
+    # value as the tiny-theta tree walk with its separate hole pairs
+    exact = oracle.potential_exact(s["px"], s["py"], s["pz"], s["m"], s["kind"], eps=0.02, c=c)
+    assert abs(exact - got) <= 1e-10 * abs(want)
     # the PW terms matter at this c: the Newtonian value differs
     if c > 0:
         newt = _pair_model_potential(s["px"], s["py"], s["pz"], s["m"], s["kind"], 0.02, 0.0)
