@@ -101,7 +101,7 @@ struct gx_solver {
     // host-API staging
     DevBuf x, y, z, m, kind, alive, ax, ay, az;
     // arena
-    DevBuf keys, keys2, order, order2, rank, holes, hist, digitTotal, blockStar, blockHole, spos;
+    DevBuf keys, keys2, order, order2, rank, holes, hist, digitTotal, blockStar, blockHole, blockBox, spos;
     DevBuf trav, m64, m32, m16, meta, partial;
     DevBuf lfull, lpart, lleaf, lcounts;
     // active sets and block steps
@@ -164,7 +164,7 @@ struct gx_solver {
     std::vector<DevBuf*> buffers() {
         return {&x,        &y,       &z,        &m,          &kind,         &alive,        &ax,        &ay,
                 &az,       &keys,    &keys2,    &order,      &order2,       &rank,         &holes,     &hist,
-                &digitTotal, &blockStar, &blockHole, &spos,   &trav,         &m64,          &m32,       &m16,
+                &digitTotal, &blockStar, &blockHole, &blockBox, &spos,   &trav,         &m64,          &m32,       &m16,
                 &meta,     &partial, &lfull,    &lpart,      &lleaf,        &lcounts,      &instCfg,   &boxEnc,
                 &starCount, &starOff, &groupOff, &root,      &half,         &actS,         &blockAct,  &tOff,
                 &tlist,    &activeMask, &rung,  &rungHist,   &diag,         &nearFlag,     &nearList,  &tileCnt,
@@ -439,6 +439,7 @@ int reserve(gx_solver* s, long long n) {
         (rc = ensure(s->holes, n * 4)) ||
         (rc = ensure(s->hist, (size_t)256 * std::max(nt, 1LL) * 4)) || (rc = ensure(s->digitTotal, 256 * 4)) ||
         (rc = ensure(s->blockStar, std::max(nb, 1LL) * 4)) || (rc = ensure(s->blockHole, std::max(nb, 1LL) * 4)) ||
+        (rc = ensure(s->blockBox, std::max(nb, 1LL) * 48)) ||
         (rc = ensure(s->spos, std::max(n, 1LL) * 32)))
         return rc;
     if (!s->meta.p) {  // allocated once; every field starts at zero (the sticky word included)
@@ -481,9 +482,9 @@ void enqueue_collect(gx_solver* s, const StepArgs& a) {
     const Batch batch = make_batch(s);
     hipLaunchKernelGGL(k_step_init, dim3(blocks(s->B, 256)), dim3(256), 0, st, meta, batch);
     hipLaunchKernelGGL(k_collect_count, dim3(blocks(n, kCollectTile)), dim3(256), 0, st, a.x, a.y, a.z, a.kind,
-                       a.alive, n, (int*)s->blockStar.p, (int*)s->blockHole.p, batch);
+                       a.alive, n, (int*)s->blockStar.p, (int*)s->blockHole.p, (double*)s->blockBox.p, batch);
     hipLaunchKernelGGL(k_collect_scan, dim3(1), dim3(1024), 0, st, (int)std::max(nb, 1LL), (int*)s->blockStar.p,
-                       (int*)s->blockHole.p, meta, batch);
+                       (int*)s->blockHole.p, (const double*)s->blockBox.p, meta, batch);
     hipLaunchKernelGGL(k_collect_scatter, dim3(blocks(n, kCollectTile)), dim3(256), 0, st, a.kind, a.alive, n,
                        (const int*)s->blockStar.p, (const int*)s->blockHole.p, (int32_t*)s->order.p,
                        (int32_t*)s->holes.p);
@@ -503,7 +504,7 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
     enqueue_collect(s, a);
     mark(s, 1);
     hipLaunchKernelGGL(k_keys, dim3(blocks(n, 256)), dim3(256), 0, st, a.x, a.y, a.z, (const int32_t*)s->order.p,
-                       meta, batch, (uint64_t*)s->keys.p);
+                       meta, batch, (uint64_t*)s->keys.p, (TravNode*)s->trav.p);
     mark(s, 2);
     // 8 passes over the 63-bit keys, then (several instances) 2 passes over the
     // instance id, most significant last; an even count ends in keys/order.
@@ -530,18 +531,20 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
                        a.x, a.y, a.z, a.m, a.kind, a.alive, n, (const int32_t*)s->rank.p, a.dmOff,
                        (double4*)s->spos.p);
     mark(s, 4);
-    hipLaunchKernelGGL(k_tree_init, dim3(blocks(s->B, 256)), dim3(256), 0, st, meta, batch, (TravNode*)s->trav.p);
+    // levels 0 .. Lmax-1 are split (the roots were set by k_keys), creating
+    // nodes down to level Lmax, which no launch splits: its nodes are leaves,
+    // and k_moments_leaves checks that none of them needs splitting
     const int Lmax = s->allLevels ? kMortonBits : s->levels;
-    for (int L = 0; L <= Lmax; L++)
+    for (int L = 0; L < Lmax; L++)
         hipLaunchKernelGGL(k_topo_level, dim3(kTreeGrid * 256 / kTopoThreads), dim3(kTopoThreads), 0, st, L,
-                           (const uint64_t*)s->keys.p,
-                           (TravNode*)s->trav.p, meta, s->capNodes, L == Lmax && Lmax < kMortonBits ? 1 : 0);
-    hipLaunchKernelGGL(k_tree_finish, dim3(1), dim3(64), 0, st, meta);
+                           (const uint64_t*)s->keys.p, (TravNode*)s->trav.p, meta, s->capNodes);
     mark(s, 5);
-    // all leaves first, then the internal nodes level by level, bottom up
+    // all leaves first (and the node count), then the internal nodes level by
+    // level, bottom up
     hipLaunchKernelGGL(k_moments_leaves, dim3(kTreeGrid), dim3(256), 0, st, (const double4*)s->spos.p,
-                       (TravNode*)s->trav.p, (Mult64*)s->m64.p, (Mult32*)s->m32.p, (Mult16*)s->m16.p, meta, batch);
-    for (int L = Lmax; L >= 0; L--)
+                       (TravNode*)s->trav.p, (Mult64*)s->m64.p, (Mult32*)s->m32.p, (Mult16*)s->m16.p, meta, batch,
+                       Lmax);
+    for (int L = Lmax - 1; L >= 0; L--)
         hipLaunchKernelGGL(k_moments_level, dim3(kTreeGrid), dim3(256), 0, st, L, (const double4*)s->spos.p,
                            (TravNode*)s->trav.p, (Mult64*)s->m64.p, (Mult32*)s->m32.p, (Mult16*)s->m16.p, meta,
                            batch);
@@ -2057,6 +2060,7 @@ constexpr size_t kNaSizes[] = {
     na_round(4 * kNaBodies), na_round(4 * kNaBodies),    // rank, holes
     na_round(256 * 4 * kNaTiles), na_round(256 * 4),     // hist, digitTotal
     na_round(4 * kNaCollect), na_round(4 * kNaCollect),  // blockStar, blockHole
+    na_round(48 * kNaCollect),                           // blockBox
     na_round(32 * kNaBodies), na_round(sizeof(Meta)),    // spos, meta
     na_round(sizeof(TravNode) * kNaNodes), na_round(sizeof(Mult64) * kNaNodes),  // trav, m64
     na_round(sizeof(Mult32) * kNaNodes), na_round(sizeof(Mult16) * kNaNodes),    // m32, m16
@@ -2105,7 +2109,7 @@ void na_setup(NoAllocDevice& d) {
     }
     gx_solver& s = d.s;
     DevBuf* order[] = {&s.keys, &s.keys2, &s.order, &s.order2, &s.rank, &s.holes, &s.hist, &s.digitTotal,
-                       &s.blockStar, &s.blockHole, &s.spos, &s.meta, &s.trav, &s.m64, &s.m32, &s.m16,
+                       &s.blockStar, &s.blockHole, &s.blockBox, &s.spos, &s.meta, &s.trav, &s.m64, &s.m32, &s.m16,
                        &s.lfull, &s.lpart, &s.lleaf, &s.lcounts, &s.instCfg, &s.boxEnc, &s.starCount,
                        &s.starOff, &s.groupOff, &s.root, &s.half};
     static_assert(sizeof(order) / sizeof(order[0]) == sizeof(kNaSizes) / sizeof(kNaSizes[0]), "one size per slice");

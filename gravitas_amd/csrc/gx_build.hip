@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_collect_count(const double* __restrict_
                                                        const uint8_t* __restrict__ kind,
                                                        const uint8_t* __restrict__ alive, long long n,
                                                        int* __restrict__ blockStar, int* __restrict__ blockHole,
-                                                       Batch batch) {
+                                                       double* __restrict__ blockBox, Batch batch) {
     __shared__ double sbox[6][4];
     __shared__ int scnt[2][4];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -133,7 +133,11 @@ __global__ __launch_bounds__(256) void k_collect_count(const double* __restrict_
         }
         blockStar[blockIdx.x] = c0;
         blockHole[blockIdx.x] = c1;
-        if (b0 == b1) inst_accumulate(batch, b0, c0, b, b + 3);
+        if (batch.B == 1) {  // one instance: k_collect_scan reduces the blocks' boxes (no contended atomics)
+            for (int c = 0; c < 6; c++) blockBox[6 * (size_t)blockIdx.x + c] = b[c];
+        } else if (b0 == b1) {
+            inst_accumulate(batch, b0, c0, b, b + 3);
+        }
     }
 }
 
@@ -167,20 +171,48 @@ __device__ long long block_scan_1024(long long v, long long* s, long long* total
 // compaction of collect) and of the per-instance star and group counts;
 // instance roots and root cells.
 __global__ __launch_bounds__(1024) void k_collect_scan(int nb, int* __restrict__ blockStar,
-                                                       int* __restrict__ blockHole, Meta* meta, Batch batch) {
+                                                       int* __restrict__ blockHole, const double* __restrict__ blockBox,
+                                                       Meta* meta, Batch batch) {
     __shared__ long long ss[1024];
+    __shared__ double sb[6][kScanThreads / kWave];
     const int tid = threadIdx.x;
     long long total;
     {
         const int per = (nb + 1023) / 1024;
         const int lo = tid * per, hi = min(nb, lo + per);
         long long s0 = 0, s1 = 0;
+        double bx[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
         for (int i = lo; i < hi; i++) {
             s0 += blockStar[i];
             s1 += blockHole[i];
+            if (batch.B == 1)
+                for (int c = 0; c < 3; c++) {
+                    bx[c] = fmin(bx[c], blockBox[6 * (size_t)i + c]);
+                    bx[3 + c] = fmax(bx[3 + c], blockBox[6 * (size_t)i + 3 + c]);
+                }
+        }
+        if (batch.B == 1) {  // the one instance's box: wave, then workgroup reduction
+            for (int c = 0; c < 3; c++) {
+                bx[c] = wave_min_d(bx[c]);
+                bx[3 + c] = wave_max_d(bx[3 + c]);
+            }
+            if ((tid & (kWave - 1)) == 0)
+                for (int c = 0; c < 6; c++) sb[c][tid / kWave] = bx[c];
+            __syncthreads();
+            if (tid == 0) {
+                for (int w = 1; w < 1024 / kWave; w++)
+                    for (int c = 0; c < 3; c++) {
+                        bx[c] = fmin(bx[c], sb[c][w]);
+                        bx[3 + c] = fmax(bx[3 + c], sb[3 + c][w]);
+                    }
+                for (int c = 0; c < 6; c++) batch.boxEnc[c] = enc_d(bx[c]);
+            }
         }
         long long run0 = block_scan_1024(s0, ss, &total);
-        if (tid == 0) meta->nstars = meta->ntargets = total;
+        if (tid == 0) {
+            meta->nstars = meta->ntargets = total;
+            if (batch.B == 1) batch.starCount[0] = (unsigned long long)total;
+        }
         long long run1 = block_scan_1024(s1, ss, &total);
         if (tid == 0) meta->nholes = total;
         for (int i = lo; i < hi; i++) {
@@ -268,11 +300,35 @@ __device__ __forceinline__ uint64_t cell_of(double v, double lo, double scale) {
     return (uint64_t)fmin(2097151.0, t);
 }
 
+// Root cell of instance b (none for an instance without stars): centre = box
+// min + half, the half side in crit2 while building.
+__device__ void tree_root(const Batch& batch, int b, TravNode* __restrict__ trav) {
+    const int r = batch.root[b];
+    if (r < 0) return;
+    const double half = batch.half[b];
+    TravNode t;
+    t.x = dec_d(batch.boxEnc[6 * b]) + half;
+    t.y = dec_d(batch.boxEnc[6 * b + 1]) + half;
+    t.z = dec_d(batch.boxEnc[6 * b + 2]) + half;
+    t.crit2 = half;
+    t.first = -1;
+    t.next = -1;
+    t.lo = (int32_t)batch.starOff[b];
+    t.hi = (int32_t)batch.starOff[b + 1];
+    t.nchild = 0;
+    t.level = 0;
+    t.parent = -1;
+    t.inst = b;
+    trav[r] = t;
+}
+
 __global__ __launch_bounds__(256) void k_keys(const double* __restrict__ x, const double* __restrict__ y,
                                               const double* __restrict__ z, const int32_t* __restrict__ order,
                                               const Meta* __restrict__ meta, Batch batch,
-                                              uint64_t* __restrict__ keys) {
+                                              uint64_t* __restrict__ keys, TravNode* __restrict__ trav) {
     long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    // the instances' root cells, anchored at the box's min corner (build.go:134-157)
+    for (long long b = k; b < batch.B; b += (long long)gridDim.x * 256) tree_root(batch, (int)b, trav);
     if (k >= meta->nstars) return;
     int32_t i = order[k];
     const int b = upper_index(batch.bodyOff, batch.B, i);
@@ -470,27 +526,6 @@ __global__ __launch_bounds__(256) void k_scatter_spos(const double* __restrict__
 // ---------------------------------------------------------------------------
 // Root cell of every instance with stars: anchored at the instance's bounding
 // box min corner, centre = min + half (build.go:43-49).
-__global__ void k_tree_init(Meta* meta, Batch batch, TravNode* __restrict__ trav) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= batch.B) return;
-    const int r = batch.root[b];
-    if (r < 0) return;
-    const double half = batch.half[b];
-    TravNode t;
-    t.x = dec_d(batch.boxEnc[6 * b]) + half;
-    t.y = dec_d(batch.boxEnc[6 * b + 1]) + half;
-    t.z = dec_d(batch.boxEnc[6 * b + 2]) + half;
-    t.crit2 = half;
-    t.first = -1;
-    t.next = -1;
-    t.lo = (int32_t)batch.starOff[b];
-    t.hi = (int32_t)batch.starOff[b + 1];
-    t.nchild = 0;
-    t.level = 0;
-    t.parent = -1;
-    t.inst = b;
-    trav[r] = t;
-}
 
 // Octant boundaries of the sorted range [lo, hi) at digit shift: b[o] is the
 // first index whose digit is >= o (lowerBound, build.go:340-350).  Result in
@@ -556,24 +591,12 @@ __device__ int octant_bounds(const uint64_t* __restrict__ keys, int lo, int hi, 
 }
 
 // Splits every node of level L (one wavefront per node, grid-stride over the
-// level) and appends the children to level L+1.  With checkOnly the level is
-// the deepest one built for this step: a node there that would be split
-// raises kFlagDepth (the step is then repeated with every level).
+// level) and appends the children to level L+1.
 __device__ void topo_level(int L, const uint64_t* __restrict__ keys, TravNode* __restrict__ trav, Meta* meta,
-                           long long cap, int checkOnly) {
+                           long long cap) {
     const long long start = meta->levelStart[L];
     const long long cnt = meta->levelCount[L];
     const long long nextBase = start + cnt;
-    if (checkOnly) {
-        // After a node overflow the level's count includes children that got
-        // no slot (their indices are >= cap): nothing of this tree is used.
-        if (L >= kMortonBits || (meta->flags & kFlagNodeOverflow)) return;
-        const long long have = min(cnt, cap - start);
-        for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < have;
-             t += (long long)gridDim.x * blockDim.x)
-            if (trav[start + t].hi - trav[start + t].lo > kLeafSize) atomicOr(&meta->flags, (int)kFlagDepth);
-        return;
-    }
     if (blockIdx.x == 0 && threadIdx.x == 0) meta->levelStart[L + 1] = nextBase;
     __shared__ int wcnt[kTopoThreads / kWave];
     __shared__ long long wbase;
@@ -656,9 +679,8 @@ __device__ void topo_level(int L, const uint64_t* __restrict__ keys, TravNode* _
 }
 
 __global__ __launch_bounds__(kTopoThreads) void k_topo_level(int L, const uint64_t* __restrict__ keys,
-                                                    TravNode* __restrict__ trav, Meta* meta, long long cap,
-                                                    int checkOnly) {
-    topo_level(L, keys, trav, meta, cap, checkOnly);
+                                                    TravNode* __restrict__ trav, Meta* meta, long long cap) {
+    topo_level(L, keys, trav, meta, cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -776,25 +798,28 @@ __global__ __launch_bounds__(256) void k_moments_level(int L, const double4* __r
 // depend only on its stars, so the level-by-level launches that follow only
 // carry the internal nodes (short sums over at most eight children) and no
 // longer wait, level after level, on the leaves' sums over up to 24 stars.
-// The nodes of all levels are numbered contiguously from 0 (k_tree_finish
-// has counted them).
+// The nodes of all levels are numbered contiguously from 0.
+// It also counts the nodes (meta->nnodes) and, when the levels built stop at
+// Lmax < 21, checks that no node of level Lmax needs splitting (a deeper tree
+// raises kFlagDepth and the step is repeated with every level).
 __global__ __launch_bounds__(256) void k_moments_leaves(const double4* __restrict__ spos, TravNode* __restrict__ trav,
                                                         Mult64* __restrict__ m64, Mult32* __restrict__ m32,
-                                                        Mult16* __restrict__ m16, const Meta* __restrict__ meta,
-                                                        Batch batch) {
+                                                        Mult16* __restrict__ m16, Meta* __restrict__ meta,
+                                                        Batch batch, int Lmax) {
     if (meta->flags & kFlagTreeIncomplete) return;
-    const long long cnt = meta->nnodes;
+    long long cnt = 0;
+    for (int l = 0; l <= kMaxLevels; l++) cnt += meta->levelCount[l];
+    if (blockIdx.x == 0 && threadIdx.x == 0) meta->nnodes = cnt;
+    const bool check = Lmax < kMortonBits;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
          t += (long long)gridDim.x * blockDim.x)
-        if (trav[t].first < 0) node_moments(t, spos, trav, m64, m32, m16, batch.theta);
+        if (trav[t].first < 0) {
+            if (check && trav[t].level == Lmax && trav[t].hi - trav[t].lo > kLeafSize)
+                atomicOr(&meta->flags, (int)kFlagDepth);
+            node_moments(t, spos, trav, m64, m32, m16, batch.theta);
+        }
 }
 
-__global__ void k_tree_finish(Meta* meta) {
-    if (threadIdx.x != 0) return;
-    long long total = 0;
-    for (int l = 0; l <= kMaxLevels; l++) total += meta->levelCount[l];
-    meta->nnodes = total;
-}
 
 // Zeroes the holes' accelerations (bh.go:87-89, 99-101); with accumulate the
 // step's flags also go into the sticky word (the last kernel of a step).

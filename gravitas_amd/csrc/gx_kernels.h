@@ -15,12 +15,14 @@ constexpr int kSortTile = 2048;     // keys per sort tile (256 threads x 8; the 
 // gx_build.hip
 __global__ void k_step_init(Meta* meta, Batch batch);
 __global__ void k_collect_count(const double* x, const double* y, const double* z, const uint8_t* kind,
-                                const uint8_t* alive, long long n, int* blockStar, int* blockHole, Batch batch);
-__global__ void k_collect_scan(int nb, int* blockStar, int* blockHole, Meta* meta, Batch batch);
+                                const uint8_t* alive, long long n, int* blockStar, int* blockHole, double* blockBox,
+                                Batch batch);
+__global__ void k_collect_scan(int nb, int* blockStar, int* blockHole, const double* blockBox, Meta* meta,
+                               Batch batch);
 __global__ void k_collect_scatter(const uint8_t* kind, const uint8_t* alive, long long n, const int* starOff,
                                   const int* holeOff, int32_t* order, int32_t* holes);
 __global__ void k_keys(const double* x, const double* y, const double* z, const int32_t* order, const Meta* meta,
-                       Batch batch, uint64_t* keys);
+                       Batch batch, uint64_t* keys, TravNode* trav);
 __global__ void k_sort_hist(const uint64_t* keys, const int32_t* vals, const Meta* meta, int shift, int instPass,
                             Batch batch, uint32_t* hist, int ntiles);
 constexpr int kScanThreads = 1024;  // threads of the per-digit scan over the tiles
@@ -31,14 +33,12 @@ __global__ void k_sort_scatter(const uint64_t* kin, const int32_t* vin, uint64_t
 __global__ void k_scatter_spos(const double* x, const double* y, const double* z, const double* m,
                                const uint8_t* kind, const uint8_t* alive, long long n, const int32_t* rank,
                                int dmOff, double4* spos);
-__global__ void k_tree_init(Meta* meta, Batch batch, TravNode* trav);
 constexpr int kTopoThreads = 1024;  // threads per topology workgroup (16 wavefronts share one slot atomic)
-__global__ void k_topo_level(int L, const uint64_t* keys, TravNode* trav, Meta* meta, long long cap, int checkOnly);
+__global__ void k_topo_level(int L, const uint64_t* keys, TravNode* trav, Meta* meta, long long cap);
 __global__ void k_moments_level(int L, const double4* spos, TravNode* trav, Mult64* m64, Mult32* m32,
                                 Mult16* m16, const Meta* meta, Batch batch);
 __global__ void k_moments_leaves(const double4* spos, TravNode* trav, Mult64* m64, Mult32* m32, Mult16* m16,
-                                 const Meta* meta, Batch batch);
-__global__ void k_tree_finish(Meta* meta);
+                                 Meta* meta, Batch batch, int Lmax);
 __global__ void k_zero_holes(const int32_t* holes, Meta* meta, double* ax, double* ay, double* az, int accumulate);
 __global__ void k_kick(long long n, double* vx, double* vy, double* vz, const double* ax, const double* ay,
                        const double* az, double h);
