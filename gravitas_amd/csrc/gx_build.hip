@@ -43,6 +43,7 @@ __global__ void k_step_init(Meta* meta, Batch batch) {
         meta->maxList[0] = meta->maxList[1] = meta->maxList[2] = 0;
         meta->listEntries = meta->rounds = meta->mixed = 0;
         meta->groupNext = 0;
+        for (int q = 0; q < 8; q++) meta->xcdNext[q] = 0;
         meta->ntargets = 0;
     }
     if (b < batch.B) {
@@ -1074,6 +1075,7 @@ __global__ void k_walk_reset(Meta* meta) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     meta->pp = meta->pc = 0;
     meta->groupNext = 0;
+    for (int q = 0; q < 8; q++) meta->xcdNext[q] = 0;
     meta->listEntries = meta->rounds = meta->mixed = 0;
     meta->maxStack = 0;
     meta->maxList[0] = meta->maxList[1] = meta->maxList[2] = 0;

@@ -715,11 +715,27 @@ __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos,
     __shared__ WalkLDS<STACK, PARTS> S;
     if (meta->flags & kFlagTreeIncomplete) return;
     long long g = persistent ? -1 : xcd_group(batch);
+    // The work queue is split in 8, one contiguous eighth of the groups per
+    // XCD (workgroups are dealt round-robin to the XCDs): each XCD's L2 sees
+    // the node records of one region of space.  An XCD that runs out helps
+    // the next one.
+    const long long ng = batch.groupOff[batch.B];
+    const long long per = (ng + 7) / 8;
+    int q = blockIdx.x % 8, tried = 0;
     for (;;) {
         if (persistent) {
-            unsigned long long next = 0;
-            if (lane_id() == 0) next = atomicAdd(&meta->groupNext, 1ull);
-            g = (long long)__shfl((long long)next, 0, kWave);
+            long long next = ng;
+            if (lane_id() == 0)
+                while (tried < 8) {
+                    const long long t = (long long)q * per + (long long)atomicAdd(&meta->xcdNext[q], 1ull);
+                    if (t < min(ng, (long long)(q + 1) * per)) {
+                        next = t;
+                        break;
+                    }
+                    q = (q + 1) % 8;
+                    tried++;
+                }
+            g = __shfl(next, 0, kWave);
         }
         Group G;
         if (!load_group(G, spos, batch, g)) return;
