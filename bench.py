@@ -43,7 +43,7 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector, datasheet
 HBM_PEAK_GBS = 8000.0
 FLOPS_PP = 18              # bh.go:145-150 counting rule (add/sub/mul/div/sqrt = 1, fma = 2)
 FLOPS_PC = 50              # bh.go:114-131
-ENCOUNTER_T = 0.12          # global step of the two-disk encounter: 1-5 % of the stars on the top rung
+ENCOUNTER_TOP = 0.03        # two-disk encounter: the global step puts this fraction of the stars on the top rung
 FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector, datasheet
 # SPH counting rule (add/sub/mul/div/sqrt = 1; comparisons, min/max not counted):
 # a force pair 62 (sph.go:219-246 without the viscosity branch), + 19 when it
@@ -445,7 +445,10 @@ def bench_encounter(args):
     parabolic orbit (pericentre 5 R_d, 30 deg inclination), theta 0.5, eps
     0.02, FP32 walk with FP16-stored quadrupoles.  Each global step T runs
     block time steps on 4 power-of-two rungs (8 ticks): every tick drifts all
-    bodies and evaluates only those whose step ends there.  Energy (kinetic
+    bodies and evaluates only those whose step ends there.  T is 4 x the 3 %
+    quantile of the initial wanted steps sqrt(2 eta eps / |a|), so 3 % of the
+    stars start on the top rung (the odd ticks evaluate about that share).
+    Energy (kinetic
     + FP64 tree potential) and momentum are measured in FP64 on the GPU before
     and after the timed steps."""
     from gravitas_amd import DeviceState, Params, Solver, ics
@@ -474,8 +477,10 @@ def bench_encounter(args):
         full[walk] = n / ((time.perf_counter() - t0) / 3)
     solver.accelerations(ds, params)
     a = np.linalg.norm(acc0, axis=1)
-    T = ENCOUNTER_T if args.T is None else args.T
     dt_wanted = np.sqrt(2 * eta * eps / a[a > 0])
+    # a star goes to the top rung (step T/8) when its wanted step is below T/4:
+    # T = 4 x the 3 % quantile puts 3 % of the stars there at the start
+    T = float(4 * np.quantile(dt_wanted, ENCOUNTER_TOP)) if args.T is None else args.T
     out_err = {}
     if not args.no_errors:
         log("FP64 walk and direct-sum sample of the initial forces")

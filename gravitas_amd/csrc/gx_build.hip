@@ -92,17 +92,37 @@ __global__ __launch_bounds__(256) void k_collect_count(const double* __restrict_
         }
     }
     // instances of the tile's first and last body: one instance -> block-wide
-    // reduction and one set of atomics; several -> atomics per star
+    // reduction and one set of atomics; several -> each wave reduces every
+    // instance of each of its 64-body chunks and flushes one set of atomics per
+    // instance and chunk (per-star atomics on a few shared addresses serialise)
     const long long last = min(n, base + kCollectTile) - 1;
     const int b0 = last >= base ? upper_index(batch.bodyOff, batch.B, base) : 0;
     const int b1 = last >= base ? upper_index(batch.bodyOff, batch.B, last) : 0;
     if (b0 != b1) {
         for (int j = 0; j < kCollectTile / 256; j++) {
-            long long i = base + j * 256 + tid;
-            if (i >= n) break;
-            if (alive[i] && kind[i] != kHole) {
-                const double p[3] = {x[i], y[i], z[i]};
-                inst_accumulate(batch, upper_index(batch.bodyOff, batch.B, i), 1, p, p);
+            const long long c0 = base + j * 256 + w * 64;  // the wave's chunk
+            if (c0 >= n) break;
+            const long long c1 = min(n, c0 + 64) - 1, i = c0 + lane;
+            const bool in = i <= c1 && alive[i] && kind[i] != kHole;
+            double p[3] = {0.0, 0.0, 0.0};
+            int bi = -1;
+            if (in) {
+                p[0] = x[i];
+                p[1] = y[i];
+                p[2] = z[i];
+                bi = upper_index(batch.bodyOff, batch.B, i);
+            }
+            const int lo = upper_index(batch.bodyOff, batch.B, c0), hi = upper_index(batch.bodyOff, batch.B, c1);
+            for (int b = lo; b <= hi; b++) {
+                const bool m = bi == b;
+                const int cc = wave_sum_i(m ? 1 : 0);
+                if (cc == 0) continue;
+                double bmn[3], bmx[3];
+                for (int c = 0; c < 3; c++) {
+                    bmn[c] = wave_min_d(m ? p[c] : INFINITY);
+                    bmx[c] = wave_max_d(m ? p[c] : -INFINITY);
+                }
+                if (lane == 0) inst_accumulate(batch, b, cc, bmn, bmx);
             }
         }
     }
