@@ -116,9 +116,9 @@ struct gx_solver {
     unsigned long long* hhist = nullptr;  // pinned mirror of hist
     int cf = 2048, cp = 512, cl = 256;  // FP32-walk list capacities per group (grow on demand)
     long long capNodes = 0;
-    // Deepest tree level built: the last step's depth + 2 (a deeper tree
-    // raises kFlagDepth and the step is repeated with every level); all
-    // levels while a graph is captured.
+    // Levels of internal-node moments launched (levels-1 .. 0): the last
+    // step's depth + 2 (a deeper tree raises kFlagDepth and the step is
+    // repeated with every level); all levels while a graph is captured.
     int levels = kMortonBits;
     bool allLevels = false;
     // instances: device config (offsets | theta | eps2) and per-instance results
@@ -504,7 +504,7 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
     enqueue_collect(s, a);
     mark(s, 1);
     hipLaunchKernelGGL(k_keys, dim3(blocks(n, 256)), dim3(256), 0, st, a.x, a.y, a.z, (const int32_t*)s->order.p,
-                       meta, batch, (uint64_t*)s->keys.p, (TravNode*)s->trav.p);
+                       meta, batch, (uint64_t*)s->keys.p);
     mark(s, 2);
     // 8 passes over the 63-bit keys, then (several instances) 2 passes over the
     // instance id, most significant last; an even count ends in keys/order.
@@ -531,16 +531,22 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
                        a.x, a.y, a.z, a.m, a.kind, a.alive, n, (const int32_t*)s->rank.p, a.dmOff,
                        (double4*)s->spos.p);
     mark(s, 4);
-    // levels 0 .. Lmax-1 are split (the roots were set by k_keys), creating
-    // nodes down to level Lmax, which no launch splits: its nodes are leaves,
-    // and k_moments_leaves checks that none of them needs splitting
-    const int Lmax = s->allLevels ? kMortonBits : s->levels;
-    for (int L = 0; L < Lmax; L++)
-        hipLaunchKernelGGL(k_topo_level, dim3(kTreeGrid * 256 / kTopoThreads), dim3(kTopoThreads), 0, st, L,
-                           (const uint64_t*)s->keys.p, (TravNode*)s->trav.p, meta, s->capNodes);
+    // the whole topology at any depth: chains and per-level counts per tile,
+    // their scan per level, node records, then ends and links (the sort's
+    // spare key and value arrays hold the chains and each chain's first node)
+    hipLaunchKernelGGL(k_topo_count, dim3(ntiles), dim3(256), 0, st, (const uint64_t*)s->keys.p, meta, batch,
+                       (uint16_t*)s->keys2.p, (uint32_t*)s->hist.p, ntiles);
+    hipLaunchKernelGGL(k_sort_scan, dim3(kMaxLevels), dim3(kScanThreads), 0, st, (uint32_t*)s->hist.p, ntiles,
+                       (uint32_t*)s->digitTotal.p);
+    hipLaunchKernelGGL(k_topo_nodes, dim3(ntiles), dim3(256), 0, st, (const uint64_t*)s->keys.p,
+                       (const uint16_t*)s->keys2.p, (const uint32_t*)s->hist.p, (const uint32_t*)s->digitTotal.p, meta,
+                       batch, (TravNode*)s->trav.p, (int32_t*)s->order2.p, s->capNodes, ntiles);
+    hipLaunchKernelGGL(k_topo_links, dim3(kTreeGrid), dim3(256), 0, st, (const uint64_t*)s->keys.p,
+                       (TravNode*)s->trav.p, (const int32_t*)s->order2.p, meta, batch);
     mark(s, 5);
     // all leaves first (and the node count), then the internal nodes level by
-    // level, bottom up
+    // level, bottom up, from level Lmax-1 (the last step's depth + 1)
+    const int Lmax = s->allLevels ? kMortonBits : s->levels;
     hipLaunchKernelGGL(k_moments_leaves, dim3(kTreeGrid), dim3(256), 0, st, (const double4*)s->spos.p,
                        (TravNode*)s->trav.p, (Mult64*)s->m64.p, (Mult32*)s->m32.p, (Mult16*)s->m16.p, meta, batch,
                        Lmax);
@@ -703,6 +709,9 @@ int finish_step(gx_solver* s) {
     if (flags & kFlagNodeOverflow)
         return set_err(GX_ERR_CAPACITY, "node arena of %lld nodes too small", s->capNodes);
     if (flags & kFlagDepth) return set_err(GX_ERR_CAPACITY, "tree deeper than the levels built");
+    if (flags & kFlagTopology)
+        return set_err(GX_ERR_PIPELINE, "tree links failed their check (0x%x at node %lld of %lld; flags 0x%x sticky 0x%llx)",
+                       h.topoWhy, h.topoNode, h.nnodes, h.flags, h.sticky);
     s->levels = std::min(kMortonBits, depth + 2);
     return GX_OK;
 }
@@ -1921,6 +1930,9 @@ int gx_potential_energy_dev(gx_solver* s, const double* x, const double* y, cons
             s->levels = kMortonBits;
             continue;
         }
+        if (s->hmeta->flags & kFlagTopology)
+            return set_err(GX_ERR_PIPELINE, "tree links failed their check (0x%x at node %lld of %lld)",
+                           s->hmeta->topoWhy, s->hmeta->topoNode, s->hmeta->nnodes);
         const long long ns = s->hmeta->nstars, nh = s->hmeta->nholes;
         if (nh > 0) {
             hipLaunchKernelGGL(k_hole_potential, dim3((unsigned)nh), dim3(256), 0, s->stream, (const double4*)s->spos.p,

@@ -11,13 +11,14 @@
 //   topology         internal/sim/bh/build.go:134-172, lowerBound :340-350
 //   moments          internal/sim/bh/build.go:176-217, quadrupole.add :326-337
 //
-// The reference builds top-down recursively; here each tree level is one
-// launch (one wavefront per node finds the octant boundaries with a 10-ary
-// search on the sorted keys), and moments are one launch per level from the
-// deepest level up.  Children of a node are allocated contiguously and in
-// octant order, and skip links follow the reference's rule, so the walk order
-// -- which alone determines the forces -- equals the reference's.  Only the
-// numbering of nodes differs (breadth-first instead of depth-first).
+// The reference builds top-down recursively; here the node set follows from
+// the sorted keys in four launches at any depth (each star's chain of nodes
+// from the keys within leafSize+1 of it, see the topology section), and
+// moments are one launch for all leaves, then one per level from the deepest
+// level up.  Children of a node are stored contiguously and in octant order,
+// and skip links follow the reference's rule, so the walk order -- which
+// alone determines the forces -- equals the reference's.  Only the numbering
+// of nodes differs (level by level instead of depth-first).
 #include <hip/hip_fp16.h>
 
 #include "gx_common.h"
@@ -45,6 +46,8 @@ __global__ void k_step_init(Meta* meta, Batch batch) {
         meta->groupNext = 0;
         for (int q = 0; q < 8; q++) meta->xcdNext[q] = 0;
         meta->ntargets = 0;
+        meta->topoWhy = 0;
+        meta->topoNode = 1ll << 62;
     }
     if (b < batch.B) {
         for (int c = 0; c < 3; c++) {
@@ -321,35 +324,11 @@ __device__ __forceinline__ uint64_t cell_of(double v, double lo, double scale) {
     return (uint64_t)fmin(2097151.0, t);
 }
 
-// Root cell of instance b (none for an instance without stars): centre = box
-// min + half, the half side in crit2 while building.
-__device__ void tree_root(const Batch& batch, int b, TravNode* __restrict__ trav) {
-    const int r = batch.root[b];
-    if (r < 0) return;
-    const double half = batch.half[b];
-    TravNode t;
-    t.x = dec_d(batch.boxEnc[6 * b]) + half;
-    t.y = dec_d(batch.boxEnc[6 * b + 1]) + half;
-    t.z = dec_d(batch.boxEnc[6 * b + 2]) + half;
-    t.crit2 = half;
-    t.first = -1;
-    t.next = -1;
-    t.lo = (int32_t)batch.starOff[b];
-    t.hi = (int32_t)batch.starOff[b + 1];
-    t.nchild = 0;
-    t.level = 0;
-    t.parent = -1;
-    t.inst = b;
-    trav[r] = t;
-}
-
 __global__ __launch_bounds__(256) void k_keys(const double* __restrict__ x, const double* __restrict__ y,
                                               const double* __restrict__ z, const int32_t* __restrict__ order,
                                               const Meta* __restrict__ meta, Batch batch,
-                                              uint64_t* __restrict__ keys, TravNode* __restrict__ trav) {
+                                              uint64_t* __restrict__ keys) {
     long long k = (long long)blockIdx.x * 256 + threadIdx.x;
-    // the instances' root cells, anchored at the box's min corner (build.go:134-157)
-    for (long long b = k; b < batch.B; b += (long long)gridDim.x * 256) tree_root(batch, (int)b, trav);
     if (k >= meta->nstars) return;
     int32_t i = order[k];
     const int b = upper_index(batch.bodyOff, batch.B, i);
@@ -543,165 +522,321 @@ __global__ __launch_bounds__(256) void k_scatter_spos(const double* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// topology
+// topology: buildNode/split, build.go:134-172 (lowerBound :340-350)
 // ---------------------------------------------------------------------------
-// Root cell of every instance with stars: anchored at the instance's bounding
-// box min corner, centre = min + half (build.go:43-49).
+// The reference splits a node of more than leafSize stars into its non-empty
+// octants, down to level mortonBits.  Restated over the sorted keys: let
+// delta_i be the number of leading 3-bit digits keys i and i+1 share (-1
+// between two instances and past the ends).  The stars sharing a level-L
+// prefix are a run whose interior deltas are all >= L (a "level-L run"); a
+// run is a node when its parent run (level L-1) has more than leafSize stars,
+// and a node splits when it has more than leafSize stars and L < mortonBits.
+// A run of more than leafSize stars holding star s contains a window of
+// leafSize+1 consecutive stars that holds s, so with
+//   m(j) = min(delta_j .. delta_{j+leafSize-1})   (stars j .. j+leafSize share m(j) digits)
+//   M(s) = max(m(j), j = s-leafSize .. s)
+// the level-L run holding s has more than leafSize stars iff L <= M(s).  The
+// runs starting at star s are those of levels delta_{s-1}+1 .. mortonBits, so
+// the nodes starting at s are a chain over the levels
+//   Llo(s) = delta_{s-1} + 1  ..  Lhi(s) = min(M(s) + 1, mortonBits),
+// every one of them split except the last.  Each star's chain follows from
+// the keys within leafSize+1 of it, so the whole topology takes four
+// launches at any depth instead of one per level:
+//   k_topo_count   chains (Llo, Lhi) of a tile of stars; per-level node counts
+//   k_sort_scan    per level, the exclusive scan of the counts over the tiles
+//   k_topo_nodes   node records: level by level, each level in star order
+//                  (children of a node are contiguous and in octant order);
+//                  centres by the reference's halving from the root centre;
+//                  leaf ends from the deltas
+//   k_topo_links   split nodes' ends (a search on the keys), skip links (the
+//                  node after a subtree is the chain that starts at its end),
+//                  child counts and parents.
+constexpr int kTopoTile = kSortTile;           // stars per topology tile (256 threads x 8)
+constexpr int kTopoPre = kLeafSize;            // deltas before the tile: delta_{ts-24} ..
+constexpr int kTopoDeltas = kTopoTile + kTopoPre + kLeafSize - 1;  // .. delta_{te+22}
+constexpr int kTopoMins = kTopoTile + kTopoPre;                    // m(j), j = ts-24 .. te-1
+constexpr uint16_t kNoChain = kMaxLevels;      // Llo = 22 > Lhi = 0: no node starts here
 
-// Octant boundaries of the sorted range [lo, hi) at digit shift: b[o] is the
-// first index whose digit is >= o (lowerBound, build.go:340-350).  Result in
-// lane o (o = 1..7) of the returned value.  Whole wavefront cooperates.
-__device__ int octant_bounds(const uint64_t* __restrict__ keys, int lo, int hi, int shift) {
-    const int lane = lane_id();
-    const int R = hi - lo;
-    int result = 0;
-    if (R <= 64) {
-        int d = 8;
-        if (lane < R) d = (int)((keys[lo + lane] >> shift) & 7);
-        for (int o = 1; o < 8; o++) {
-            int b = lo + __popcll(__ballot(d < o));
-            if (lane == o) result = b;
-        }
-        return result;
+struct TopoLDS {
+    uint64_t key[kTopoDeltas + 1];  // keys ts-24 .. te+23
+    int8_t delta[kTopoDeltas];
+    int8_t mwin[kTopoMins];
+    uint8_t brk[kTopoDeltas];       // an instance starts right after delta index w
+};
+
+// Deltas of the window around tile ts (delta index w = i - (ts - 24)).
+__device__ void topo_deltas(TopoLDS& S, const uint64_t* __restrict__ keys, long long n, long long ts,
+                            const Batch& batch) {
+    const int tid = threadIdx.x;
+    const long long i0 = ts - kTopoPre;
+    for (int w = tid; w <= kTopoDeltas; w += 256) {
+        const long long i = i0 + w;
+        S.key[w] = i >= 0 && i < n ? keys[i] : 0ull;
+        if (w < kTopoDeltas) S.brk[w] = 0;
     }
-    if (R <= 8 * kWave) {
-        // one pass: every lane reads up to 8 keys; boundary o = lo + #(digit < o)
-        int c = 0;
-        const int jn = (R + kWave - 1) / kWave;  // rows of 64 keys, uniform over the wave
-        for (int j = 0; j < jn; j++) {
-            const int k = lo + lane + kWave * j;
-            const int d = k < hi ? (int)((keys[k] >> shift) & 7) : 8;
-#pragma unroll
-            for (int o = 1; o < 8; o++) {
-                const int b = __popcll(__ballot(d < o));
-                if (lane == o) c += b;
-            }
+    __syncthreads();
+    if (batch.B > 1) {
+        // instance starts i+1 = starOff[b], b = 1 .. B-1, inside the window
+        const long long v = i0;  // first b with starOff[b] > i0
+        int b = v < 0 ? 1 : upper_index(batch.starOff, batch.B + 1, v) + 1;
+        b = max(b, 1);
+        for (int bb = b + tid; bb < batch.B; bb += 256) {
+            const long long w = batch.starOff[bb] - 1 - i0;
+            if (w >= kTopoDeltas) break;
+            if (w >= 0) S.brk[w] = 1;
         }
-        return lane >= 1 && lane <= 7 ? lo + c : 0;
+        __syncthreads();
     }
-    // 7 groups of 9 lanes; group g searches octant o = g + 1 with a 10-ary search.
-    const int g = lane / 9, s = lane % 9, o = g + 1;
-    int a = lo, b = hi;  // answer in [a, b]
-    bool active = lane < 63;
-    for (;;) {
-        bool open = active && a < b;
-        if (__ballot(open) == 0) break;
-        long long span = b - a;
-        int q = a + (int)((span * (s + 1)) / 10);
-        bool pred = open && (int)((keys[q] >> shift) & 7) >= o;
-        uint64_t bal = __ballot(pred);
-        if (open) {
-            uint32_t gb = (uint32_t)(bal >> (g * 9)) & 0x1ffu;
-            int na, nb;
-            if (gb) {
-                int ss = __builtin_ctz(gb);
-                nb = a + (int)((span * (ss + 1)) / 10);
-                na = ss > 0 ? a + (int)((span * ss) / 10) + 1 : a;
-            } else {
-                na = a + (int)((span * 9) / 10) + 1;
-                nb = b;
-            }
-            a = na;
-            b = nb;
+    for (int w = tid; w < kTopoDeltas; w += 256) {
+        const long long i = i0 + w;
+        int d = -1;
+        if (i >= 0 && i < n - 1 && !S.brk[w]) {
+            const uint64_t x = S.key[w] ^ S.key[w + 1];
+            d = x ? (__clzll((long long)x) - 1) / 3 : kMortonBits;  // 63-bit keys: bit 63 is 0
         }
+        S.delta[w] = (int8_t)d;
     }
-    // lane o needs the answer of group o-1 (held by its lane 9*(o-1))
-    int src = lane >= 1 && lane <= 7 ? 9 * (lane - 1) : 0;
-    int v = __shfl(a, src, kWave);
-    return lane >= 1 && lane <= 7 ? v : 0;
+    __syncthreads();
 }
 
-// Splits every node of level L (one wavefront per node, grid-stride over the
-// level) and appends the children to level L+1.
-__device__ void topo_level(int L, const uint64_t* __restrict__ keys, TravNode* __restrict__ trav, Meta* meta,
-                           long long cap) {
-    const long long start = meta->levelStart[L];
-    const long long cnt = meta->levelCount[L];
-    const long long nextBase = start + cnt;
-    if (blockIdx.x == 0 && threadIdx.x == 0) meta->levelStart[L + 1] = nextBase;
-    __shared__ int wcnt[kTopoThreads / kWave];
-    __shared__ long long wbase;
-    __shared__ int stop;
-    if (threadIdx.x == 0) stop = L >= kMortonBits || (meta->flags & kFlagNodeOverflow);
+__device__ __forceinline__ void topo_fail(Meta* meta, int why, long long node) {
+    atomicOr(&meta->flags, (int)kFlagTopology);
+    atomicOr(&meta->topoWhy, why);
+    atomicMin(&meta->topoNode, node);
+}
+
+// Chain of star s = ts + u (u < kTopoTile), from the window's deltas and m().
+__device__ __forceinline__ uint16_t topo_chain(const TopoLDS& S, int u, bool star) {
+    if (!star) return kNoChain;
+    int M = -1;
+    for (int j = u; j <= u + kLeafSize; j++) M = max(M, (int)S.mwin[j]);  // m(s-24) .. m(s)
+    const int lo = S.delta[u + kTopoPre - 1] + 1;
+    const int hi = min(M + 1, kMortonBits);
+    return lo <= hi ? (uint16_t)(lo | hi << 8) : kNoChain;
+}
+
+__global__ __launch_bounds__(256) void k_topo_count(const uint64_t* __restrict__ keys, Meta* __restrict__ meta,
+                                                    Batch batch, uint16_t* __restrict__ chain,
+                                                    uint32_t* __restrict__ tileCount, int ntiles) {
+    __shared__ TopoLDS S;
+    __shared__ uint32_t cnt[kMaxLevels];
+    const int tid = threadIdx.x, lane = lane_id();
+    const long long n = meta->nstars;
+    const long long ts = (long long)blockIdx.x * kTopoTile;
+    if (tid < kMaxLevels) cnt[tid] = 0;
+    if (ts < n) {
+        topo_deltas(S, keys, n, ts, batch);
+        // the runs assume keys sorted within each instance
+        for (int u = tid; u < kTopoTile; u += 256) {
+            const int w = u + kTopoPre;
+            if (ts + u + 1 < n && !S.brk[w] && S.key[w] > S.key[w + 1]) topo_fail(meta, 64, ts + u);
+        }
+        for (int u = tid; u < kTopoMins; u += 256) {
+            int m = kMortonBits;
+            for (int k = 0; k < kLeafSize; k++) m = min(m, (int)S.delta[u + k]);
+            S.mwin[u] = (int8_t)m;
+        }
+        __syncthreads();
+        for (int r = 0; r < kTopoTile / 256; r++) {
+            const int u = r * 256 + tid;
+            const long long s = ts + u;
+            const uint16_t c = topo_chain(S, u, s < n);
+            if (s < n) chain[s] = c;
+            const int lo = c & 255, hi = c >> 8;
+            for (int L = 0; L < kMaxLevels; L++) {
+                const uint64_t b = __ballot(lo <= L && L <= hi);
+                if (lane == 0 && b) atomicAdd(&cnt[L], (uint32_t)__popcll(b));
+            }
+        }
+    }
     __syncthreads();
-    if (stop) return;  // the same decision for every wave of the workgroup
-    const int lane = lane_id();
-    const int w = threadIdx.x / kWave;  // kTopoThreads / 64 waves per workgroup
-    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
-    const long long iters = (cnt + nwaves - 1) / nwaves;
-    const int shift = 3 * (kMortonBits - 1 - L);
-    // Every wave of the workgroup runs the same number of iterations, so the
-    // children of the workgroup's (up to) 16 nodes get their slots from one
-    // atomic on the level counter (the one address every workgroup adds to:
-    // the fewer atomics, the less they queue behind each other).
-    for (long long it = 0; it < iters; it++) {
-        const long long wv = it * nwaves + ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-        const long long ni = start + wv;
-        int lo = 0, hi = 0;
-        if (wv < cnt) {
-            lo = trav[ni].lo;
-            hi = trav[ni].hi;
+    if (tid < kMaxLevels) tileCount[(size_t)tid * ntiles + blockIdx.x] = cnt[tid];
+}
+
+__global__ __launch_bounds__(256) void k_topo_nodes(const uint64_t* __restrict__ keys,
+                                                    const uint16_t* __restrict__ chain,
+                                                    const uint32_t* __restrict__ tileOff,
+                                                    const uint32_t* __restrict__ levelTotal, Meta* meta, Batch batch,
+                                                    TravNode* __restrict__ trav, int32_t* __restrict__ chainFirst,
+                                                    long long cap, int ntiles) {
+    constexpr int R = kTopoTile / 256, W = 256 / kWave;
+    __shared__ TopoLDS S;
+    __shared__ uint32_t pre[R][W][kMaxLevels];  // exclusive prefix of the (r, wave) counts, in star order
+    __shared__ long long base[kMaxLevels + 1];  // level start + this tile's offset in the level
+    const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
+    if (tid == 0) {
+        long long acc = 0;
+        for (int L = 0; L < kMaxLevels; L++) {
+            base[L] = acc;
+            acc += levelTotal[L];
         }
-        const bool split = wv < cnt && hi - lo > kLeafSize;  // otherwise a leaf: first = -1 already
-        int bo = 0, bnext = 0, nchild = 0;
-        uint64_t ne = 0;
-        if (split) {
-            const int b = octant_bounds(keys, lo, hi, shift);
-            // bounds[0] = lo, bounds[8] = hi
-            bo = lane == 0 ? lo : (lane == 8 ? hi : b);
-            bnext = __shfl_down(bo, 1, kWave);
-            ne = __ballot(lane < 8 && bnext > bo);
-            nchild = __popcll(ne);
+        base[kMaxLevels] = acc;
+        if (blockIdx.x == 0) {
+            for (int L = 0; L < kMaxLevels; L++) {
+                meta->levelStart[L] = base[L];
+                meta->levelCount[L] = levelTotal[L];
+            }
+            meta->levelStart[kMaxLevels] = acc;
+            meta->levelCount[kMaxLevels] = 0;
+            meta->nnodes = acc;
+            if (acc > cap) atomicOr(&meta->flags, (int)kFlagNodeOverflow);
         }
-        if (lane == 0) wcnt[w] = nchild;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int tot = 0;
-            for (int k = 0; k < kTopoThreads / kWave; k++) tot += wcnt[k];
-            wbase = tot ? (long long)atomicAdd((unsigned long long*)&meta->levelCount[L + 1], (unsigned long long)tot)
-                        : 0;
+    }
+    __syncthreads();
+    const long long n = meta->nstars;
+    const long long ts = (long long)blockIdx.x * kTopoTile;
+    if (base[kMaxLevels] > cap || ts >= n) return;
+    if (tid < kMaxLevels) base[tid] += tileOff[(size_t)tid * ntiles + blockIdx.x];
+    topo_deltas(S, keys, n, ts, batch);  // (its first barrier also publishes base)
+    uint16_t c[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const long long s = ts + r * 256 + tid;
+        c[r] = s < n ? chain[s] : kNoChain;
+        const int lo = c[r] & 255, hi = c[r] >> 8;
+        for (int L = 0; L < kMaxLevels; L++) {
+            const uint64_t b = __ballot(lo <= L && L <= hi);
+            if (lane == 0) pre[r][w][L] = (uint32_t)__popcll(b);
         }
-        __syncthreads();
-        long long firstc = nextBase + wbase;
-        for (int k = 0; k < w; k++) firstc += wcnt[k];
-        __syncthreads();  // wcnt and wbase are rewritten by the next iteration
-        if (!split) continue;
-        if (firstc + nchild > cap) {
-            if (lane == 0) atomicOr(&meta->flags, (int)kFlagNodeOverflow);
+    }
+    __syncthreads();
+    if (tid < kMaxLevels) {
+        uint32_t acc = 0;
+        for (int r = 0; r < R; r++)
+            for (int ww = 0; ww < W; ww++) {
+                const uint32_t v = pre[r][ww][tid];
+                pre[r][ww][tid] = acc;
+                acc += v;
+            }
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1;
+    const long long base_end = base[kMaxLevels];  // the node count
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int u = r * 256 + tid;
+        const long long s = ts + u;
+        const int lo = c[r] & 255, hi = c[r] >> 8;
+        const bool has = lo <= hi;
+        const uint64_t key = S.key[u + kTopoPre];
+        const int b = has && batch.B > 1 ? upper_index(batch.starOff, batch.B, s) : 0;
+        const double half0 = batch.half[b];
+        double cx = dec_d(batch.boxEnc[6 * b]) + half0;
+        double cy = dec_d(batch.boxEnc[6 * b + 1]) + half0;
+        double cz = dec_d(batch.boxEnc[6 * b + 2]) + half0;
+        double half = half0;
+        // level by level (wave-uniform, so the ballots that rank the star's
+        // node in each level are taken by every lane); the node of level L
+        // needs the index of level L+1 (its first child)
+        auto rank = [&](int L) -> int32_t {
+            const uint64_t bl = __ballot(lo <= L && L <= hi);
+            return (int32_t)(base[L] + pre[r][w][L] + (uint32_t)__popcll(bl & below));
+        };
+        int32_t cur = rank(0);
+        for (int L = 0; L < kMaxLevels; L++) {
+            if (__ballot(has && L <= hi) == 0) break;
+            const int32_t nxt = L + 1 < kMaxLevels ? rank(L + 1) : -1;
+            if (lo <= L && L <= hi) {
+                TravNode t;
+                t.x = cx;
+                t.y = cy;
+                t.z = cz;
+                t.crit2 = half;
+                t.first = L < hi ? nxt : -1;
+                t.next = -1;
+                t.lo = (int32_t)s;
+                t.hi = -1;
+                if (L == hi && L < kMortonBits) {
+                    // a leaf of at most leafSize stars: its end is the first
+                    // star after s that starts a run of level <= L
+                    for (int k = 1; k <= kLeafSize; k++)
+                        if (S.delta[u + kTopoPre - 1 + k] < L) {
+                            t.hi = (int32_t)(s + k);
+                            break;
+                        }
+                }
+                t.nchild = 0;
+                t.level = L;
+                t.parent = -1;
+                t.inst = b;
+                if (cur >= 0 && cur < base_end) {  // always (the counts of k_topo_count), checked anyway
+                    trav[cur] = t;
+                    if (L == lo) chainFirst[s] = cur;
+                } else {
+                    topo_fail(meta, 1, cur);
+                }
+            }
+            // the child cell holding this star: octant digit L, buildNode's offsets
+            const int oct = (int)((key >> (3 * (kMortonBits - 1 - min(L, kMortonBits - 1)))) & 7);
+            const double q = half / 2;
+            cx = cx + (((oct >> 2) & 1) ? 1.0 : -1.0) * q;
+            cy = cy + (((oct >> 1) & 1) ? 1.0 : -1.0) * q;
+            cz = cz + ((oct & 1) ? 1.0 : -1.0) * q;
+            half = q;
+            cur = nxt;
+        }
+    }
+}
+
+// End of a run: the first index in (lo, end) whose key differs from keys[lo]
+// above bit shift (end if none); a galloping then a binary search.
+__device__ __forceinline__ int32_t run_end(const uint64_t* __restrict__ keys, int32_t lo, int32_t end, int shift) {
+    const uint64_t k0 = keys[lo];
+    auto same = [&](int32_t j) { return ((keys[j] ^ k0) >> shift) == 0; };
+    int32_t a = lo, d = 1;
+    while (a + d < end && same(a + d)) {
+        a += d;
+        d *= 2;
+    }
+    int32_t b = min(end, a + d);
+    while (b - a > 1) {
+        const int32_t mid = a + (b - a) / 2;
+        if (same(mid))
+            a = mid;
+        else
+            b = mid;
+    }
+    return b;
+}
+
+__global__ __launch_bounds__(256) void k_topo_links(const uint64_t* __restrict__ keys, TravNode* __restrict__ trav,
+                                                    const int32_t* __restrict__ chainFirst, Meta* meta,
+                                                    Batch batch) {
+    if (meta->flags & kFlagNodeOverflow) return;
+    const long long nn = meta->nnodes;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nn;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int32_t lo = trav[t].lo, first = trav[t].first, L = trav[t].level, inst = trav[t].inst;
+        if (inst < 0 || inst >= batch.B || L < 0 || L > kMortonBits || lo < batch.starOff[inst] ||
+            lo >= batch.starOff[inst + 1]) {  // a record k_topo_nodes did not write (never expected)
+            topo_fail(meta, 2, t);
             continue;
         }
-        const double cx = trav[ni].x, cy = trav[ni].y, cz = trav[ni].z, half = trav[ni].crit2;
-        const int pnext = trav[ni].next, pinst = trav[ni].inst;
-        if (lane < 8 && bnext > bo) {
-            const int rank = __popcll(ne & ((1ull << lane) - 1));
-            const long long c = firstc + rank;
-            const double q = half / 2;
-            const int oct = lane;
-            TravNode t;
-            t.x = cx + (((oct >> 2) & 1) ? 1.0 : -1.0) * q;
-            t.y = cy + (((oct >> 1) & 1) ? 1.0 : -1.0) * q;
-            t.z = cz + ((oct & 1) ? 1.0 : -1.0) * q;
-            t.crit2 = q;
-            t.first = -1;
-            t.next = rank < nchild - 1 ? (int32_t)(c + 1) : pnext;
-            t.lo = bo;
-            t.hi = bnext;
-            t.nchild = 0;
-            t.level = L + 1;
-            t.parent = (int32_t)ni;
-            t.inst = pinst;
-            trav[c] = t;
+        int32_t hi = trav[t].hi;
+        const int32_t end = (int32_t)batch.starOff[inst + 1];
+        if (hi < 0) hi = L == 0 ? end : run_end(keys, lo, end, 3 * (kMortonBits - L));
+        const int32_t next = hi >= end ? -1 : chainFirst[hi];
+        // every link must lead to a node that starts where it should (the
+        // walks follow them): a first child at lo one level down, a skip link
+        // at hi; anything else stops the step before any walk
+        int why = hi <= lo || hi > end ? 4 : 0;
+        if (next != -1 && (next < 0 || next >= nn || trav[next].lo != hi)) why |= 8;
+        if (first >= 0 && L == kMortonBits) why |= 32;
+        int nchild = 0;
+        if (first >= 0 && !why) {
+            const long long cend = meta->levelStart[L + 1] + meta->levelCount[L + 1];
+            if (first >= cend || trav[first].lo != lo || trav[first].level != L + 1) why |= 16;
+            if (!why) {
+                nchild = 1;
+                while (nchild < 8 && first + nchild < cend && trav[first + nchild].lo < hi) nchild++;
+                for (int c = 0; c < nchild; c++) trav[first + c].parent = (int32_t)t;
+            }
         }
-        if (lane == 0) {
-            trav[ni].first = (int32_t)firstc;
-            trav[ni].nchild = nchild;
-        }
+        if (why) topo_fail(meta, why, t);
+        trav[t].hi = hi;
+        trav[t].next = next;
+        trav[t].nchild = nchild;
     }
-}
-
-__global__ __launch_bounds__(kTopoThreads) void k_topo_level(int L, const uint64_t* __restrict__ keys,
-                                                    TravNode* __restrict__ trav, Meta* meta, long long cap) {
-    topo_level(L, keys, trav, meta, cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -820,9 +955,10 @@ __global__ __launch_bounds__(256) void k_moments_level(int L, const double4* __r
 // carry the internal nodes (short sums over at most eight children) and no
 // longer wait, level after level, on the leaves' sums over up to 24 stars.
 // The nodes of all levels are numbered contiguously from 0.
-// It also counts the nodes (meta->nnodes) and, when the levels built stop at
-// Lmax < 21, checks that no node of level Lmax needs splitting (a deeper tree
-// raises kFlagDepth and the step is repeated with every level).
+// It also counts the nodes (meta->nnodes) and checks that no split node lies
+// at level Lmax or below, which the per-level launches (levels Lmax-1 .. 0)
+// would not reach (a deeper tree raises kFlagDepth and the step is repeated
+// with every level).
 __global__ __launch_bounds__(256) void k_moments_leaves(const double4* __restrict__ spos, TravNode* __restrict__ trav,
                                                         Mult64* __restrict__ m64, Mult32* __restrict__ m32,
                                                         Mult16* __restrict__ m16, Meta* __restrict__ meta,
@@ -831,14 +967,13 @@ __global__ __launch_bounds__(256) void k_moments_leaves(const double4* __restric
     long long cnt = 0;
     for (int l = 0; l <= kMaxLevels; l++) cnt += meta->levelCount[l];
     if (blockIdx.x == 0 && threadIdx.x == 0) meta->nnodes = cnt;
-    const bool check = Lmax < kMortonBits;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
-         t += (long long)gridDim.x * blockDim.x)
-        if (trav[t].first < 0) {
-            if (check && trav[t].level == Lmax && trav[t].hi - trav[t].lo > kLeafSize)
-                atomicOr(&meta->flags, (int)kFlagDepth);
+         t += (long long)gridDim.x * blockDim.x) {
+        if (trav[t].first < 0)
             node_moments(t, spos, trav, m64, m32, m16, batch.theta);
-        }
+        else if (trav[t].level >= Lmax)
+            atomicOr(&meta->flags, (int)kFlagDepth);
+    }
 }
 
 
@@ -1084,9 +1219,10 @@ __global__ __launch_bounds__(256) void k_block_kick(long long n, double* __restr
     }
 }
 
+// p[0 .. count) = v, for any count and launch shape (grid-stride: a single
+// 64-thread block also fills a Meta of more than 64 words).
 __global__ void k_fill_u64(unsigned long long* p, int count, unsigned long long v) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) p[i] = v;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
 // The walk's counters and flags, before a walk that repeats one on the same

@@ -116,6 +116,12 @@ struct Meta {
     // the host clears it once it has reported it.  Replays of a captured step
     // thus keep an overflow of an earlier replay.
     unsigned long long sticky;
+    // kFlagTopology: which link checks failed (OR of 1 record index, 2 record
+    // fields, 4 end, 8 skip link, 16 first child, 32 split at the last level,
+    // 64 keys out of order)
+    // and the lowest node that failed one
+    int topoWhy;
+    long long topoNode;
 };
 
 enum Flag : int {
@@ -123,9 +129,10 @@ enum Flag : int {
     kFlagStackOverflow = 2,  // FP32 walk stack too small
     kFlagListOverflow = 4,   // FP32 walk lists too small
     kFlagDepth = 8,          // the tree needs more levels than were built
+    kFlagTopology = 16,      // a node's links failed their check (never expected)
 };
 // The tree of this step is incomplete: later kernels skip their work.
-constexpr int kFlagTreeIncomplete = kFlagNodeOverflow | kFlagDepth;
+constexpr int kFlagTreeIncomplete = kFlagNodeOverflow | kFlagDepth | kFlagTopology;
 
 // ---------------------------------------------------------------------------
 // wavefront helpers

@@ -22,7 +22,7 @@ __global__ void k_collect_scan(int nb, int* blockStar, int* blockHole, const dou
 __global__ void k_collect_scatter(const uint8_t* kind, const uint8_t* alive, long long n, const int* starOff,
                                   const int* holeOff, int32_t* order, int32_t* holes);
 __global__ void k_keys(const double* x, const double* y, const double* z, const int32_t* order, const Meta* meta,
-                       Batch batch, uint64_t* keys, TravNode* trav);
+                       Batch batch, uint64_t* keys);
 __global__ void k_sort_hist(const uint64_t* keys, const int32_t* vals, const Meta* meta, int shift, int instPass,
                             Batch batch, uint32_t* hist, int ntiles);
 constexpr int kScanThreads = 1024;  // threads of the per-digit scan over the tiles
@@ -33,8 +33,12 @@ __global__ void k_sort_scatter(const uint64_t* kin, const int32_t* vin, uint64_t
 __global__ void k_scatter_spos(const double* x, const double* y, const double* z, const double* m,
                                const uint8_t* kind, const uint8_t* alive, long long n, const int32_t* rank,
                                int dmOff, double4* spos);
-constexpr int kTopoThreads = 1024;  // threads per topology workgroup (16 wavefronts share one slot atomic)
-__global__ void k_topo_level(int L, const uint64_t* keys, TravNode* trav, Meta* meta, long long cap);
+__global__ void k_topo_count(const uint64_t* keys, Meta* meta, Batch batch, uint16_t* chain,
+                             uint32_t* tileCount, int ntiles);
+__global__ void k_topo_nodes(const uint64_t* keys, const uint16_t* chain, const uint32_t* tileOff,
+                             const uint32_t* levelTotal, Meta* meta, Batch batch, TravNode* trav, int32_t* chainFirst,
+                             long long cap, int ntiles);
+__global__ void k_topo_links(const uint64_t* keys, TravNode* trav, const int32_t* chainFirst, Meta* meta, Batch batch);
 __global__ void k_moments_level(int L, const double4* spos, TravNode* trav, Mult64* m64, Mult32* m32,
                                 Mult16* m16, const Meta* meta, Batch batch);
 __global__ void k_moments_leaves(const double4* spos, TravNode* trav, Mult64* m64, Mult32* m32, Mult16* m16,

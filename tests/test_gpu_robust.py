@@ -103,3 +103,21 @@ def test_last_gpu_seconds_with_timing_off(gpu):
     s.graph_launch(3)
     s.sync()
     assert 0.0 < s.last_gpu_seconds() < 1.0
+
+
+def test_fresh_solver_metadata_starts_clean(gpu):
+    """A new solver's step metadata (flags, counters and the sticky flag word
+    at its end) starts at zero whatever the memory held before: blocks filled
+    with set bits are freed first, so the allocator is likely to hand them out
+    again, and every solver's first step must then find no flag set."""
+    from gravitas_amd.hip import DeviceArray
+    junk = [DeviceArray((size,), np.uint8) for size in (512, 1024, 2048, 4096) for _ in range(256)]
+    for j in junk:
+        j.fill(0xFF)
+    del junk, j
+    say("[robust] eight solvers on freed memory full of set bits")
+    for k in range(8):
+        sv = Solver(walk="fp32" if k % 2 else "fp64")
+        ds = DeviceState.from_host(ics.plummer(4000, 20 + k))
+        sv.accelerations(ds, Params(softening=0.01))
+        assert sv.stats()["flags"] == 0
