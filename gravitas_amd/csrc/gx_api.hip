@@ -485,9 +485,9 @@ void enqueue_collect(gx_solver* s, const StepArgs& a) {
                        a.alive, n, (int*)s->blockStar.p, (int*)s->blockHole.p, (double*)s->blockBox.p, batch);
     hipLaunchKernelGGL(k_collect_scan, dim3(1), dim3(1024), 0, st, (int)std::max(nb, 1LL), (int*)s->blockStar.p,
                        (int*)s->blockHole.p, (const double*)s->blockBox.p, meta, batch);
-    hipLaunchKernelGGL(k_collect_scatter, dim3(blocks(n, kCollectTile)), dim3(256), 0, st, a.kind, a.alive, n,
-                       (const int*)s->blockStar.p, (const int*)s->blockHole.p, (int32_t*)s->order.p,
-                       (int32_t*)s->holes.p);
+    hipLaunchKernelGGL(k_collect_scatter, dim3(blocks(n, kCollectTile)), dim3(256), 0, st, a.x, a.y, a.z, a.kind,
+                       a.alive, n, (const int*)s->blockStar.p, (const int*)s->blockHole.p, batch,
+                       (int32_t*)s->order.p, (int32_t*)s->holes.p, (uint64_t*)s->keys.p);
 }
 
 // Build the trees of all instances (collect .. moments).  theta is baked into
@@ -502,9 +502,7 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
     const int ntiles = std::max(1, (int)((n + kSortTile - 1) / kSortTile));
     mark(s, 0);
     enqueue_collect(s, a);
-    mark(s, 1);
-    hipLaunchKernelGGL(k_keys, dim3(blocks(n, 256)), dim3(256), 0, st, a.x, a.y, a.z, (const int32_t*)s->order.p,
-                       meta, batch, (uint64_t*)s->keys.p);
+    mark(s, 1);  // the keys are written by the collect scatter (phase "keys" stays empty)
     mark(s, 2);
     // 8 passes over the 63-bit keys, then (several instances) 2 passes over the
     // instance id, most significant last; an even count ends in keys/order.

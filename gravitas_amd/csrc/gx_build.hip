@@ -175,20 +175,48 @@ __device__ __forceinline__ double box_side(const Batch& batch, int b) {
 
 // Exclusive scan of per-thread values v (1024 threads) in LDS; returns the
 // prefix before this thread and the total in *total.
-__device__ long long block_scan_1024(long long v, long long* s, long long* total) {
-    const int tid = threadIdx.x;
-    s[tid] = v;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-        long long a = tid >= d ? s[tid - d] : 0;
-        __syncthreads();
-        s[tid] += a;
-        __syncthreads();
+// Exclusive scans of K values per thread over a 1,024-thread workgroup, all
+// K at once: shuffles within each wavefront, then the 16 wavefront totals
+// (two barriers instead of twenty per value).  v becomes the exclusive
+// prefix, total the sum over the workgroup.
+template <int K>
+__device__ void block_scan_1024(long long (&v)[K], long long (&total)[K],
+                                long long (*ws)[kScanThreads / kWave + 1]) {
+    constexpr int W = kScanThreads / kWave;
+    const int lane = lane_id(), w = threadIdx.x / kWave;
+    long long incl[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        incl[k] = v[k];
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const long long t = __shfl_up(incl[k], d, kWave);
+            if (lane >= d) incl[k] += t;
+        }
+        if (lane == kWave - 1) ws[k][w] = incl[k];
     }
-    const long long incl = s[tid];
-    *total = s[1023];
     __syncthreads();
-    return incl - v;
+    if (w == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const long long x = lane < W ? ws[k][lane] : 0;
+            long long xi = x;
+#pragma unroll
+            for (int d = 1; d < W; d <<= 1) {
+                const long long t = __shfl_up(xi, d, kWave);
+                if (lane >= d) xi += t;
+            }
+            if (lane < W) ws[k][lane] = xi - x;  // exclusive prefix of the wavefront totals
+            if (lane == W - 1) ws[k][W] = xi;    // the workgroup total
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        total[k] = ws[k][W];
+        v[k] = ws[k][w] + incl[k] - v[k];
+    }
+    __syncthreads();  // ws is reused by the next scan
 }
 
 // Single workgroup: exclusive scans of the per-block counts (the stable
@@ -197,18 +225,17 @@ __device__ long long block_scan_1024(long long v, long long* s, long long* total
 __global__ __launch_bounds__(1024) void k_collect_scan(int nb, int* __restrict__ blockStar,
                                                        int* __restrict__ blockHole, const double* __restrict__ blockBox,
                                                        Meta* meta, Batch batch) {
-    __shared__ long long ss[1024];
+    __shared__ long long ws[3][kScanThreads / kWave + 1];
     __shared__ double sb[6][kScanThreads / kWave];
     const int tid = threadIdx.x;
-    long long total;
     {
         const int per = (nb + 1023) / 1024;
         const int lo = tid * per, hi = min(nb, lo + per);
-        long long s0 = 0, s1 = 0;
+        long long v[2] = {0, 0}, total[2];
         double bx[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
         for (int i = lo; i < hi; i++) {
-            s0 += blockStar[i];
-            s1 += blockHole[i];
+            v[0] += blockStar[i];
+            v[1] += blockHole[i];
             if (batch.B == 1)
                 for (int c = 0; c < 3; c++) {
                     bx[c] = fmin(bx[c], blockBox[6 * (size_t)i + c]);
@@ -232,13 +259,13 @@ __global__ __launch_bounds__(1024) void k_collect_scan(int nb, int* __restrict__
                 for (int c = 0; c < 6; c++) batch.boxEnc[c] = enc_d(bx[c]);
             }
         }
-        long long run0 = block_scan_1024(s0, ss, &total);
+        block_scan_1024<2>(v, total, ws);
         if (tid == 0) {
-            meta->nstars = meta->ntargets = total;
-            if (batch.B == 1) batch.starCount[0] = (unsigned long long)total;
+            meta->nstars = meta->ntargets = total[0];
+            meta->nholes = total[1];
+            if (batch.B == 1) batch.starCount[0] = (unsigned long long)total[0];
         }
-        long long run1 = block_scan_1024(s1, ss, &total);
-        if (tid == 0) meta->nholes = total;
+        long long run0 = v[0], run1 = v[1];
         for (int i = lo; i < hi; i++) {
             const int c0 = blockStar[i], c1 = blockHole[i];
             blockStar[i] = (int)run0;
@@ -247,23 +274,25 @@ __global__ __launch_bounds__(1024) void k_collect_scan(int nb, int* __restrict__
             run1 += c1;
         }
     }
+    __syncthreads();  // starCount[0] (one instance) is read below
     {
         const int B = batch.B;
         const int per = (B + 1023) / 1024;
         const int lo = tid * per, hi = min(B, lo + per);
-        long long cs = 0, cg = 0, cr = 0;
+        long long v[3] = {0, 0, 0}, total[3];  // stars, groups, roots
         for (int b = lo; b < hi; b++) {
             const long long c = (long long)batch.starCount[b];
-            cs += c;
-            cg += (c + batch.gsize - 1) / batch.gsize;
-            cr += c > 0;
+            v[0] += c;
+            v[1] += (c + batch.gsize - 1) / batch.gsize;
+            v[2] += c > 0;
         }
-        long long rs = block_scan_1024(cs, ss, &total);
-        if (tid == 0) batch.starOff[B] = total;
-        long long rg = block_scan_1024(cg, ss, &total);
-        if (tid == 0) batch.groupOff[B] = total;
-        long long rr = block_scan_1024(cr, ss, &total);
-        if (tid == 0) meta->levelCount[0] = total;
+        block_scan_1024<3>(v, total, ws);
+        if (tid == 0) {
+            batch.starOff[B] = total[0];
+            batch.groupOff[B] = total[1];
+            meta->levelCount[0] = total[2];
+        }
+        long long rs = v[0], rg = v[1], rr = v[2];
         for (int b = lo; b < hi; b++) {
             const long long c = (long long)batch.starCount[b];
             batch.starOff[b] = rs;
@@ -277,12 +306,33 @@ __global__ __launch_bounds__(1024) void k_collect_scan(int nb, int* __restrict__
     }
 }
 
-// Stable compaction in state order: order[] gets the stars, holes[] the holes.
-__global__ __launch_bounds__(256) void k_collect_scatter(const uint8_t* __restrict__ kind,
+// ---------------------------------------------------------------------------
+// keys: build.go:19-31 (written by k_collect_scatter at each star's slot)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t cell_of(double v, double lo, double scale) {
+    double t = fmax(0.0, (v - lo) * scale);
+    return (uint64_t)fmin(2097151.0, t);
+}
+
+__device__ __forceinline__ uint64_t morton_key(double x, double y, double z, const Batch& batch, int b) {
+    const double side = box_side(batch, b);
+    const double scale = (double)(1 << kMortonBits) / side;
+    const uint64_t cx = cell_of(x, dec_d(batch.boxEnc[6 * b]), scale);
+    const uint64_t cy = cell_of(y, dec_d(batch.boxEnc[6 * b + 1]), scale);
+    const uint64_t cz = cell_of(z, dec_d(batch.boxEnc[6 * b + 2]), scale);
+    return spread21(cx) << 2 | spread21(cy) << 1 | spread21(cz);
+}
+
+// Stable compaction in state order: order[] gets the stars, holes[] the
+// holes, and keys[] each star's Morton key at its slot (the boxes are final).
+__global__ __launch_bounds__(256) void k_collect_scatter(const double* __restrict__ x, const double* __restrict__ y,
+                                                         const double* __restrict__ z,
+                                                         const uint8_t* __restrict__ kind,
                                                          const uint8_t* __restrict__ alive, long long n,
                                                          const int* __restrict__ starOff,
-                                                         const int* __restrict__ holeOff,
-                                                         int32_t* __restrict__ order, int32_t* __restrict__ holes) {
+                                                         const int* __restrict__ holeOff, Batch batch,
+                                                         int32_t* __restrict__ order, int32_t* __restrict__ holes,
+                                                         uint64_t* __restrict__ keys) {
     __shared__ int ssum[2][256];
     const int tid = threadIdx.x;
     constexpr int per = kCollectTile / 256;
@@ -311,33 +361,14 @@ __global__ __launch_bounds__(256) void k_collect_scatter(const uint8_t* __restri
     int os = starOff[blockIdx.x] + ssum[0][tid] - cs;
     int oh = holeOff[blockIdx.x] + ssum[1][tid] - ch;
     for (int j = 0; j < per; j++) {
-        if (smask >> j & 1) order[os++] = (int32_t)(base + j);
-        if (hmask >> j & 1) holes[oh++] = (int32_t)(base + j);
+        const long long i = base + j;
+        if (smask >> j & 1) {
+            const int b = batch.B > 1 ? upper_index(batch.bodyOff, batch.B, i) : 0;
+            keys[os] = morton_key(x[i], y[i], z[i], batch, b);
+            order[os++] = (int32_t)i;
+        }
+        if (hmask >> j & 1) holes[oh++] = (int32_t)i;
     }
-}
-
-// ---------------------------------------------------------------------------
-// keys: build.go:19-31
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t cell_of(double v, double lo, double scale) {
-    double t = fmax(0.0, (v - lo) * scale);
-    return (uint64_t)fmin(2097151.0, t);
-}
-
-__global__ __launch_bounds__(256) void k_keys(const double* __restrict__ x, const double* __restrict__ y,
-                                              const double* __restrict__ z, const int32_t* __restrict__ order,
-                                              const Meta* __restrict__ meta, Batch batch,
-                                              uint64_t* __restrict__ keys) {
-    long long k = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (k >= meta->nstars) return;
-    int32_t i = order[k];
-    const int b = upper_index(batch.bodyOff, batch.B, i);
-    double side = box_side(batch, b);
-    double scale = (double)(1 << kMortonBits) / side;
-    uint64_t cx = cell_of(x[i], dec_d(batch.boxEnc[6 * b]), scale);
-    uint64_t cy = cell_of(y[i], dec_d(batch.boxEnc[6 * b + 1]), scale);
-    uint64_t cz = cell_of(z[i], dec_d(batch.boxEnc[6 * b + 2]), scale);
-    keys[k] = spread21(cx) << 2 | spread21(cy) << 1 | spread21(cz);
 }
 
 // ---------------------------------------------------------------------------
@@ -560,8 +591,16 @@ constexpr uint16_t kNoChain = kMaxLevels;      // Llo = 22 > Lhi = 0: no node st
 struct TopoLDS {
     uint64_t key[kTopoDeltas + 1];  // keys ts-24 .. te+23
     int8_t delta[kTopoDeltas];
-    int8_t mwin[kTopoMins];
     uint8_t brk[kTopoDeltas];       // an instance starts right after delta index w
+};
+
+// The sliding windows of k_topo_count (van Herk / Gil-Werman: extremes from
+// the start and from the end of fixed blocks of the window's length, so each
+// window is one or two lookups).
+struct TopoWin {
+    int8_t pre[kTopoDeltas], suf[kTopoDeltas];  // min within blocks of leafSize deltas
+    int8_t mwin[kTopoMins];                     // m(j)
+    int8_t mpre[kTopoMins], msuf[kTopoMins];    // max within blocks of leafSize+1 m values
 };
 
 // Deltas of the window around tile ts (delta index w = i - (ts - 24)).
@@ -605,11 +644,37 @@ __device__ __forceinline__ void topo_fail(Meta* meta, int why, long long node) {
     atomicMin(&meta->topoNode, node);
 }
 
-// Chain of star s = ts + u (u < kTopoTile), from the window's deltas and m().
-__device__ __forceinline__ uint16_t topo_chain(const TopoLDS& S, int u, bool star) {
+// m(j) for j = ts-24 .. te-1 (window index u = j - (ts-24)), then the block
+// maxima that give M(s) = max(m(s-24) .. m(s)).
+__device__ void topo_windows(const TopoLDS& S, TopoWin& V) {
+    const int tid = threadIdx.x;
+    constexpr int BW = kLeafSize, BM = kLeafSize + 1;
+    for (int b = tid; b * BW < kTopoDeltas; b += 256) {
+        const int a = b * BW, e = min(a + BW, kTopoDeltas);
+        int v = 127;
+        for (int i = a; i < e; i++) V.pre[i] = (int8_t)(v = min(v, (int)S.delta[i]));
+        v = 127;
+        for (int i = e - 1; i >= a; i--) V.suf[i] = (int8_t)(v = min(v, (int)S.delta[i]));
+    }
+    __syncthreads();
+    for (int u = tid; u < kTopoMins; u += 256)  // min(delta[u .. u+23])
+        V.mwin[u] = u % BW == 0 ? V.suf[u] : (int8_t)min((int)V.suf[u], (int)V.pre[u + BW - 1]);
+    __syncthreads();
+    for (int b = tid; b * BM < kTopoMins; b += 256) {
+        const int a = b * BM, e = min(a + BM, kTopoMins);
+        int v = -128;
+        for (int i = a; i < e; i++) V.mpre[i] = (int8_t)(v = max(v, (int)V.mwin[i]));
+        v = -128;
+        for (int i = e - 1; i >= a; i--) V.msuf[i] = (int8_t)(v = max(v, (int)V.mwin[i]));
+    }
+    __syncthreads();
+}
+
+// Chain of star s = ts + u (u < kTopoTile), from the window's deltas and M(s).
+__device__ __forceinline__ uint16_t topo_chain(const TopoLDS& S, const TopoWin& V, int u, bool star) {
     if (!star) return kNoChain;
-    int M = -1;
-    for (int j = u; j <= u + kLeafSize; j++) M = max(M, (int)S.mwin[j]);  // m(s-24) .. m(s)
+    constexpr int BM = kLeafSize + 1;
+    const int M = u % BM == 0 ? V.msuf[u] : max((int)V.msuf[u], (int)V.mpre[u + BM - 1]);  // m(s-24) .. m(s)
     const int lo = S.delta[u + kTopoPre - 1] + 1;
     const int hi = min(M + 1, kMortonBits);
     return lo <= hi ? (uint16_t)(lo | hi << 8) : kNoChain;
@@ -619,8 +684,9 @@ __global__ __launch_bounds__(256) void k_topo_count(const uint64_t* __restrict__
                                                     Batch batch, uint16_t* __restrict__ chain,
                                                     uint32_t* __restrict__ tileCount, int ntiles) {
     __shared__ TopoLDS S;
+    __shared__ TopoWin V;
     __shared__ uint32_t cnt[kMaxLevels];
-    const int tid = threadIdx.x, lane = lane_id();
+    const int tid = threadIdx.x;
     const long long n = meta->nstars;
     const long long ts = (long long)blockIdx.x * kTopoTile;
     if (tid < kMaxLevels) cnt[tid] = 0;
@@ -631,22 +697,14 @@ __global__ __launch_bounds__(256) void k_topo_count(const uint64_t* __restrict__
             const int w = u + kTopoPre;
             if (ts + u + 1 < n && !S.brk[w] && S.key[w] > S.key[w + 1]) topo_fail(meta, 64, ts + u);
         }
-        for (int u = tid; u < kTopoMins; u += 256) {
-            int m = kMortonBits;
-            for (int k = 0; k < kLeafSize; k++) m = min(m, (int)S.delta[u + k]);
-            S.mwin[u] = (int8_t)m;
-        }
-        __syncthreads();
+        topo_windows(S, V);
         for (int r = 0; r < kTopoTile / 256; r++) {
             const int u = r * 256 + tid;
             const long long s = ts + u;
-            const uint16_t c = topo_chain(S, u, s < n);
+            const uint16_t c = topo_chain(S, V, u, s < n);
             if (s < n) chain[s] = c;
             const int lo = c & 255, hi = c >> 8;
-            for (int L = 0; L < kMaxLevels; L++) {
-                const uint64_t b = __ballot(lo <= L && L <= hi);
-                if (lane == 0 && b) atomicAdd(&cnt[L], (uint32_t)__popcll(b));
-            }
+            for (int L = lo; L <= hi; L++) atomicAdd(&cnt[L], 1u);  // about one node per chain
         }
     }
     __syncthreads();
@@ -661,7 +719,9 @@ __global__ __launch_bounds__(256) void k_topo_nodes(const uint64_t* __restrict__
                                                     long long cap, int ntiles) {
     constexpr int R = kTopoTile / 256, W = 256 / kWave;
     __shared__ TopoLDS S;
-    __shared__ uint32_t pre[R][W][kMaxLevels];  // exclusive prefix of the (r, wave) counts, in star order
+    __shared__ uint32_t list[kTopoTile];         // the tile's chains in star order: u << 16 | chain
+    __shared__ uint32_t wpos[R][W + 1];          // compaction offsets of the (r, wave) runs
+    __shared__ uint32_t pre[R][W][kMaxLevels];  // exclusive prefix of the (chunk, wave) counts, in list order
     __shared__ long long base[kMaxLevels + 1];  // level start + this tile's offset in the level
     const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
     if (tid == 0) {
@@ -688,35 +748,64 @@ __global__ __launch_bounds__(256) void k_topo_nodes(const uint64_t* __restrict__
     if (base[kMaxLevels] > cap || ts >= n) return;
     if (tid < kMaxLevels) base[tid] += tileOff[(size_t)tid * ntiles + blockIdx.x];
     topo_deltas(S, keys, n, ts, batch);  // (its first barrier also publishes base)
+    // the stars that start a chain (about one in seven), compacted in star
+    // order, so that the level loops below run over them only
+    const uint64_t below = (1ull << lane) - 1;
     uint16_t c[R];
+    uint64_t hb[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const long long s = ts + r * 256 + tid;
         c[r] = s < n ? chain[s] : kNoChain;
-        const int lo = c[r] & 255, hi = c[r] >> 8;
+        hb[r] = __ballot((c[r] & 255) <= (c[r] >> 8));
+        if (lane == 0) wpos[r][w] = (uint32_t)__popcll(hb[r]);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (int r = 0; r < R; r++)
+            for (int ww = 0; ww < W; ww++) {
+                const uint32_t v = wpos[r][ww];
+                wpos[r][ww] = acc;
+                acc += v;
+            }
+        wpos[0][W] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++)
+        if ((hb[r] >> lane) & 1)
+            list[wpos[r][w] + (uint32_t)__popcll(hb[r] & below)] = (uint32_t)(r * 256 + tid) << 16 | c[r];
+    const int m = (int)wpos[0][W];
+    const int Q = (m + 255) / 256;
+    __syncthreads();
+    for (int q = 0; q < Q; q++) {
+        const int e = q * 256 + tid;
+        const uint32_t ce = e < m ? list[e] & 0xffffu : kNoChain;
+        const int lo = ce & 255, hi = ce >> 8;
         for (int L = 0; L < kMaxLevels; L++) {
             const uint64_t b = __ballot(lo <= L && L <= hi);
-            if (lane == 0) pre[r][w][L] = (uint32_t)__popcll(b);
+            if (lane == 0) pre[q][w][L] = (uint32_t)__popcll(b);
         }
     }
     __syncthreads();
     if (tid < kMaxLevels) {
         uint32_t acc = 0;
-        for (int r = 0; r < R; r++)
+        for (int q = 0; q < Q; q++)
             for (int ww = 0; ww < W; ww++) {
-                const uint32_t v = pre[r][ww][tid];
-                pre[r][ww][tid] = acc;
+                const uint32_t v = pre[q][ww][tid];
+                pre[q][ww][tid] = acc;
                 acc += v;
             }
     }
     __syncthreads();
-    const uint64_t below = (1ull << lane) - 1;
     const long long base_end = base[kMaxLevels];  // the node count
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int u = r * 256 + tid;
+    for (int q = 0; q < Q; q++) {
+        const int e = q * 256 + tid;
+        const uint32_t le = e < m ? list[e] : kNoChain;
+        const int u = (int)(le >> 16);
         const long long s = ts + u;
-        const int lo = c[r] & 255, hi = c[r] >> 8;
+        const int lo = le & 255, hi = (le >> 8) & 255;
         const bool has = lo <= hi;
         const uint64_t key = S.key[u + kTopoPre];
         const int b = has && batch.B > 1 ? upper_index(batch.starOff, batch.B, s) : 0;
@@ -730,7 +819,7 @@ __global__ __launch_bounds__(256) void k_topo_nodes(const uint64_t* __restrict__
         // needs the index of level L+1 (its first child)
         auto rank = [&](int L) -> int32_t {
             const uint64_t bl = __ballot(lo <= L && L <= hi);
-            return (int32_t)(base[L] + pre[r][w][L] + (uint32_t)__popcll(bl & below));
+            return (int32_t)(base[L] + pre[q][w][L] + (uint32_t)__popcll(bl & below));
         };
         int32_t cur = rank(0);
         for (int L = 0; L < kMaxLevels; L++) {
@@ -768,11 +857,11 @@ __global__ __launch_bounds__(256) void k_topo_nodes(const uint64_t* __restrict__
             }
             // the child cell holding this star: octant digit L, buildNode's offsets
             const int oct = (int)((key >> (3 * (kMortonBits - 1 - min(L, kMortonBits - 1)))) & 7);
-            const double q = half / 2;
-            cx = cx + (((oct >> 2) & 1) ? 1.0 : -1.0) * q;
-            cy = cy + (((oct >> 1) & 1) ? 1.0 : -1.0) * q;
-            cz = cz + ((oct & 1) ? 1.0 : -1.0) * q;
-            half = q;
+            const double qh = half / 2;
+            cx = cx + (((oct >> 2) & 1) ? 1.0 : -1.0) * qh;
+            cy = cy + (((oct >> 1) & 1) ? 1.0 : -1.0) * qh;
+            cz = cz + ((oct & 1) ? 1.0 : -1.0) * qh;
+            half = qh;
             cur = nxt;
         }
     }
@@ -852,9 +941,11 @@ __device__ __forceinline__ void quad_add(double q[6], double m, double dx, doubl
     q[5] += m * 3 * dy * dz;
 }
 
-// Moments of node ni from its stars (leaf) or its finished children, in the
+// Moments of node ni from its stars (LEAF) or its finished children, in the
 // reference's order (finish, build.go:176-217).  While building, the node's
-// record holds the cell centre in x,y,z and the half side in crit2.
+// record holds the cell centre in x,y,z and the half side in crit2.  (Two
+// instances, so the leaf kernel carries no registers of the other branch.)
+template <bool LEAF>
 __device__ void node_moments(long long ni, const double4* __restrict__ spos, TravNode* __restrict__ trav,
                              Mult64* __restrict__ m64, Mult32* __restrict__ m32, Mult16* __restrict__ m16,
                              const double* __restrict__ thetas) {
@@ -864,7 +955,7 @@ __device__ void node_moments(long long ni, const double4* __restrict__ spos, Tra
     double x = 0, y = 0, z = 0, m = 0;
     // the sums keep the reference's order; unrolling only lets the loads of
     // several stars be in flight together
-    if (nd.first < 0) {
+    if (LEAF) {
 #pragma unroll 4
         for (int j = nd.lo; j < nd.hi; j++) {
             double4 p = spos[j];
@@ -893,7 +984,7 @@ __device__ void node_moments(long long ni, const double4* __restrict__ spos, Tra
         z = cz;
     }
     double q[6] = {0, 0, 0, 0, 0, 0};
-    if (nd.first < 0) {
+    if (LEAF) {
 #pragma unroll 4
         for (int j = nd.lo; j < nd.hi; j++) {
             double4 p = spos[j];
@@ -938,6 +1029,9 @@ __device__ void node_moments(long long ni, const double4* __restrict__ spos, Tra
 }
 
 // Internal nodes of level L (their children, one level down, are done).
+// (Finishing two levels per launch, each thread doing its node's split
+// children first, measured 2.6x slower: the per-thread chains of dependent
+// loads cost more than the launches they save.)
 __global__ __launch_bounds__(256) void k_moments_level(int L, const double4* __restrict__ spos,
                                                        TravNode* __restrict__ trav, Mult64* __restrict__ m64,
                                                        Mult32* __restrict__ m32, Mult16* __restrict__ m16,
@@ -947,7 +1041,7 @@ __global__ __launch_bounds__(256) void k_moments_level(int L, const double4* __r
     if (meta->flags & kFlagTreeIncomplete) return;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
          t += (long long)gridDim.x * blockDim.x)
-        if (trav[start + t].first >= 0) node_moments(start + t, spos, trav, m64, m32, m16, batch.theta);
+        if (trav[start + t].first >= 0) node_moments<false>(start + t, spos, trav, m64, m32, m16, batch.theta);
 }
 
 // Every leaf of the tree, whatever its level, in one launch: a leaf's moments
@@ -970,7 +1064,7 @@ __global__ __launch_bounds__(256) void k_moments_leaves(const double4* __restric
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
          t += (long long)gridDim.x * blockDim.x) {
         if (trav[t].first < 0)
-            node_moments(t, spos, trav, m64, m32, m16, batch.theta);
+            node_moments<true>(t, spos, trav, m64, m32, m16, batch.theta);
         else if (trav[t].level >= Lmax)
             atomicOr(&meta->flags, (int)kFlagDepth);
     }
@@ -1027,15 +1121,15 @@ __global__ __launch_bounds__(256) void k_active_count(const uint8_t* __restrict_
 
 // One workgroup: exclusive scan of the tile counts; the target count.
 __global__ __launch_bounds__(1024) void k_active_scan(int nb, int* __restrict__ blockAct, Meta* meta) {
-    __shared__ long long ss[1024];
+    __shared__ long long ws[1][kScanThreads / kWave + 1];
     const int tid = threadIdx.x;
     const int per = (nb + 1023) / 1024;
     const int lo = tid * per, hi = min(nb, lo + per);
-    long long s0 = 0;
-    for (int i = lo; i < hi; i++) s0 += blockAct[i];
-    long long total;
-    long long run = block_scan_1024(s0, ss, &total);
-    if (tid == 0) meta->ntargets = total;
+    long long v[1] = {0}, total[1];
+    for (int i = lo; i < hi; i++) v[0] += blockAct[i];
+    block_scan_1024<1>(v, total, ws);
+    if (tid == 0) meta->ntargets = total[0];
+    long long run = v[0];
     for (int i = lo; i < hi; i++) {
         const int c = blockAct[i];
         blockAct[i] = (int)run;
@@ -1064,16 +1158,16 @@ __global__ __launch_bounds__(256) void k_active_inst(const uint8_t* __restrict__
 
 // One workgroup: walk groups per instance from its target count.
 __global__ __launch_bounds__(1024) void k_active_groups(Batch batch, const long long* __restrict__ tOff) {
-    __shared__ long long ss[1024];
+    __shared__ long long ws[1][kScanThreads / kWave + 1];
     const int tid = threadIdx.x;
     const int B = batch.B;
     const int per = (B + 1023) / 1024;
     const int lo = tid * per, hi = min(B, lo + per);
-    long long cg = 0;
-    for (int b = lo; b < hi; b++) cg += (tOff[b + 1] - tOff[b] + batch.gsize - 1) / batch.gsize;
-    long long total;
-    long long rg = block_scan_1024(cg, ss, &total);
-    if (tid == 0) batch.groupOff[B] = total;
+    long long v[1] = {0}, total[1];
+    for (int b = lo; b < hi; b++) v[0] += (tOff[b + 1] - tOff[b] + batch.gsize - 1) / batch.gsize;
+    block_scan_1024<1>(v, total, ws);
+    if (tid == 0) batch.groupOff[B] = total[0];
+    long long rg = v[0];
     for (int b = lo; b < hi; b++) {
         batch.groupOff[b] = rg;
         rg += (tOff[b + 1] - tOff[b] + batch.gsize - 1) / batch.gsize;

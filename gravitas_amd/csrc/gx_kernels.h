@@ -9,7 +9,7 @@ namespace gx {
 int set_error(int code, const char* fmt, ...);
 int device_ready();
 
-constexpr int kCollectTile = 4096;  // bodies per collect workgroup (256 threads x 16)
+constexpr int kCollectTile = 1024;  // bodies per collect workgroup (256 threads x 4)
 constexpr int kSortTile = 2048;     // keys per sort tile (256 threads x 8; the scatter stages a tile in LDS)
 
 // gx_build.hip
@@ -19,10 +19,9 @@ __global__ void k_collect_count(const double* x, const double* y, const double* 
                                 Batch batch);
 __global__ void k_collect_scan(int nb, int* blockStar, int* blockHole, const double* blockBox, Meta* meta,
                                Batch batch);
-__global__ void k_collect_scatter(const uint8_t* kind, const uint8_t* alive, long long n, const int* starOff,
-                                  const int* holeOff, int32_t* order, int32_t* holes);
-__global__ void k_keys(const double* x, const double* y, const double* z, const int32_t* order, const Meta* meta,
-                       Batch batch, uint64_t* keys);
+__global__ void k_collect_scatter(const double* x, const double* y, const double* z, const uint8_t* kind,
+                                  const uint8_t* alive, long long n, const int* starOff, const int* holeOff,
+                                  Batch batch, int32_t* order, int32_t* holes, uint64_t* keys);
 __global__ void k_sort_hist(const uint64_t* keys, const int32_t* vals, const Meta* meta, int shift, int instPass,
                             Batch batch, uint32_t* hist, int ntiles);
 constexpr int kScanThreads = 1024;  // threads of the per-digit scan over the tiles
