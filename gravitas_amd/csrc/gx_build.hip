@@ -225,53 +225,88 @@ __device__ void block_scan_1024(long long (&v)[K], long long (&total)[K],
 __global__ __launch_bounds__(1024) void k_collect_scan(int nb, int* __restrict__ blockStar,
                                                        int* __restrict__ blockHole, const double* __restrict__ blockBox,
                                                        Meta* meta, Batch batch) {
-    __shared__ long long ws[3][kScanThreads / kWave + 1];
-    __shared__ double sb[6][kScanThreads / kWave];
-    const int tid = threadIdx.x;
+    constexpr int W = kScanThreads / kWave;
+    __shared__ long long ws[3][W + 1];
+    __shared__ double sb[6][W];
+    const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
     {
-        const int per = (nb + 1023) / 1024;
-        const int lo = tid * per, hi = min(nb, lo + per);
+        // wave w takes the blocks [w*chunk, (w+1)*chunk) in slices of 64
+        // consecutive ones (coalesced loads, a wavefront scan per slice)
+        const int chunk = (nb + W - 1) / W;
+        const int lo = min(nb, w * chunk), hi = min(nb, lo + chunk);
         long long v[2] = {0, 0}, total[2];
         double bx[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-        for (int i = lo; i < hi; i++) {
-            v[0] += blockStar[i];
-            v[1] += blockHole[i];
-            if (batch.B == 1)
-                for (int c = 0; c < 3; c++) {
-                    bx[c] = fmin(bx[c], blockBox[6 * (size_t)i + c]);
-                    bx[3 + c] = fmax(bx[3 + c], blockBox[6 * (size_t)i + 3 + c]);
-                }
+        for (int i0 = lo; i0 < hi; i0 += kWave) {
+            const int i = i0 + lane;
+            if (i < hi) {
+                v[0] += blockStar[i];
+                v[1] += blockHole[i];
+                if (batch.B == 1)
+                    for (int c = 0; c < 3; c++) {
+                        bx[c] = fmin(bx[c], blockBox[6 * (size_t)i + c]);
+                        bx[3 + c] = fmax(bx[3 + c], blockBox[6 * (size_t)i + 3 + c]);
+                    }
+            }
         }
         if (batch.B == 1) {  // the one instance's box: wave, then workgroup reduction
             for (int c = 0; c < 3; c++) {
                 bx[c] = wave_min_d(bx[c]);
                 bx[3 + c] = wave_max_d(bx[3 + c]);
             }
-            if ((tid & (kWave - 1)) == 0)
-                for (int c = 0; c < 6; c++) sb[c][tid / kWave] = bx[c];
-            __syncthreads();
-            if (tid == 0) {
-                for (int w = 1; w < 1024 / kWave; w++)
+            if (lane == 0)
+                for (int c = 0; c < 6; c++) sb[c][w] = bx[c];
+        }
+        // the waves' totals, scanned across the workgroup (lane sums of a
+        // wave reduce to its total; the wave's own order is rebuilt below)
+        long long t[2] = {(long long)wave_sum_u64((unsigned long long)v[0]),
+                          (long long)wave_sum_u64((unsigned long long)v[1])};
+        if (lane == 0) {
+            ws[0][w] = t[0];
+            ws[1][w] = t[1];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            long long a0 = 0, a1 = 0;
+            for (int ww = 0; ww < W; ww++) {
+                const long long c0 = ws[0][ww], c1 = ws[1][ww];
+                ws[0][ww] = a0;
+                ws[1][ww] = a1;
+                a0 += c0;
+                a1 += c1;
+            }
+            ws[0][W] = a0;
+            ws[1][W] = a1;
+            if (batch.B == 1) {
+                double b6[6];
+                for (int c = 0; c < 6; c++) b6[c] = sb[c][0];
+                for (int ww = 1; ww < W; ww++)
                     for (int c = 0; c < 3; c++) {
-                        bx[c] = fmin(bx[c], sb[c][w]);
-                        bx[3 + c] = fmax(bx[3 + c], sb[3 + c][w]);
+                        b6[c] = fmin(b6[c], sb[c][ww]);
+                        b6[3 + c] = fmax(b6[3 + c], sb[3 + c][ww]);
                     }
-                for (int c = 0; c < 6; c++) batch.boxEnc[c] = enc_d(bx[c]);
+                for (int c = 0; c < 6; c++) batch.boxEnc[c] = enc_d(b6[c]);
             }
         }
-        block_scan_1024<2>(v, total, ws);
+        __syncthreads();
+        total[0] = ws[0][W];
+        total[1] = ws[1][W];
         if (tid == 0) {
             meta->nstars = meta->ntargets = total[0];
             meta->nholes = total[1];
             if (batch.B == 1) batch.starCount[0] = (unsigned long long)total[0];
         }
-        long long run0 = v[0], run1 = v[1];
-        for (int i = lo; i < hi; i++) {
-            const int c0 = blockStar[i], c1 = blockHole[i];
-            blockStar[i] = (int)run0;
-            blockHole[i] = (int)run1;
-            run0 += c0;
-            run1 += c1;
+        // exclusive offsets, slice by slice in block order
+        long long run0 = ws[0][w], run1 = ws[1][w];
+        for (int i0 = lo; i0 < hi; i0 += kWave) {
+            const int i = i0 + lane;
+            const int c0 = i < hi ? blockStar[i] : 0, c1 = i < hi ? blockHole[i] : 0;
+            const int p0 = wave_incl_scan(c0), p1 = wave_incl_scan(c1);
+            if (i < hi) {
+                blockStar[i] = (int)(run0 + p0 - c0);
+                blockHole[i] = (int)(run1 + p1 - c1);
+            }
+            run0 += __shfl(p0, kWave - 1, kWave);
+            run1 += __shfl(p1, kWave - 1, kWave);
         }
     }
     __syncthreads();  // starCount[0] (one instance) is read below
@@ -325,6 +360,7 @@ __device__ __forceinline__ uint64_t morton_key(double x, double y, double z, con
 
 // Stable compaction in state order: order[] gets the stars, holes[] the
 // holes, and keys[] each star's Morton key at its slot (the boxes are final).
+// Body i = base + j*256 + tid (coalesced); ranks from ballots, in body order.
 __global__ __launch_bounds__(256) void k_collect_scatter(const double* __restrict__ x, const double* __restrict__ y,
                                                          const double* __restrict__ z,
                                                          const uint8_t* __restrict__ kind,
@@ -333,41 +369,45 @@ __global__ __launch_bounds__(256) void k_collect_scatter(const double* __restric
                                                          const int* __restrict__ holeOff, Batch batch,
                                                          int32_t* __restrict__ order, int32_t* __restrict__ holes,
                                                          uint64_t* __restrict__ keys) {
-    __shared__ int ssum[2][256];
-    const int tid = threadIdx.x;
-    constexpr int per = kCollectTile / 256;
-    long long base = (long long)blockIdx.x * kCollectTile + (long long)tid * per;
-    uint32_t smask = 0, hmask = 0;
+    constexpr int per = kCollectTile / 256, W = 256 / kWave;
+    __shared__ int wcnt[2][per][W];
+    const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
+    const long long base = (long long)blockIdx.x * kCollectTile;
+    uint64_t bs[per], bh[per];
+#pragma unroll
     for (int j = 0; j < per; j++) {
-        long long i = base + j;
-        if (i < n && alive[i]) {
-            if (kind[i] == kHole)
-                hmask |= 1u << j;
-            else
-                smask |= 1u << j;
+        const long long i = base + j * 256 + tid;
+        const bool a = i < n && alive[i];
+        const bool hole = a && kind[i] == kHole;
+        bs[j] = __ballot(a && !hole);
+        bh[j] = __ballot(hole);
+        if (lane == 0) {
+            wcnt[0][j][w] = __popcll(bs[j]);
+            wcnt[1][j][w] = __popcll(bh[j]);
         }
     }
-    int cs = __popc(smask), ch = __popc(hmask);
-    ssum[0][tid] = cs;
-    ssum[1][tid] = ch;
     __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {
-        int a0 = tid >= d ? ssum[0][tid - d] : 0, a1 = tid >= d ? ssum[1][tid - d] : 0;
-        __syncthreads();
-        ssum[0][tid] += a0;
-        ssum[1][tid] += a1;
-        __syncthreads();
+    if (tid < 2) {  // exclusive prefix over (slice, wave), in body order
+        int acc = tid == 0 ? starOff[blockIdx.x] : holeOff[blockIdx.x];
+        for (int j = 0; j < per; j++)
+            for (int ww = 0; ww < W; ww++) {
+                const int c = wcnt[tid][j][ww];
+                wcnt[tid][j][ww] = acc;
+                acc += c;
+            }
     }
-    int os = starOff[blockIdx.x] + ssum[0][tid] - cs;
-    int oh = holeOff[blockIdx.x] + ssum[1][tid] - ch;
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1;
+#pragma unroll
     for (int j = 0; j < per; j++) {
-        const long long i = base + j;
-        if (smask >> j & 1) {
+        const long long i = base + j * 256 + tid;
+        if ((bs[j] >> lane) & 1) {
+            const int os = wcnt[0][j][w] + __popcll(bs[j] & below);
             const int b = batch.B > 1 ? upper_index(batch.bodyOff, batch.B, i) : 0;
             keys[os] = morton_key(x[i], y[i], z[i], batch, b);
-            order[os++] = (int32_t)i;
+            order[os] = (int32_t)i;
         }
-        if (hmask >> j & 1) holes[oh++] = (int32_t)i;
+        if ((bh[j] >> lane) & 1) holes[wcnt[1][j][w] + __popcll(bh[j] & below)] = (int32_t)i;
     }
 }
 
