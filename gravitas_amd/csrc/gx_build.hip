@@ -484,11 +484,10 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict
                                                       const uint32_t* __restrict__ hist,
                                                       const uint32_t* __restrict__ digitTotal, int ntiles,
                                                       int32_t* __restrict__ rankOut) {
+    constexpr int R = kSortTile / 256, W = 256 / kWave;
     __shared__ uint32_t globalBase[256];  // output index of digit d's first element of this tile
     __shared__ uint32_t localStart[256];  // staging slot of digit d's first element
-    __shared__ uint32_t running[256];
-    __shared__ uint32_t waveCnt[4][256];
-    __shared__ uint32_t waveOff[2][4][256];
+    __shared__ uint16_t cnt[R][W][256];   // per (round, wave): elements of digit d, then their first slot
     __shared__ uint32_t scan[2][256];
     __shared__ uint64_t skey[kSortTile];
     __shared__ int32_t sval[kSortTile];
@@ -499,11 +498,21 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict
     // counts (differences of the per-tile prefixes k_sort_scan left in hist)
     const uint32_t tot = digitTotal[tid];
     const uint32_t pre = hist[(size_t)tid * ntiles + blockIdx.x];
-    const uint32_t cnt = ((int)blockIdx.x + 1 < ntiles ? hist[(size_t)tid * ntiles + blockIdx.x + 1] : tot) - pre;
+    const uint32_t tc = ((int)blockIdx.x + 1 < ntiles ? hist[(size_t)tid * ntiles + blockIdx.x + 1] : tot) - pre;
     scan[0][tid] = tot;
-    scan[1][tid] = cnt;
-    for (int ww = 0; ww < 4; ww++) waveCnt[ww][tid] = 0;
-    running[tid] = 0;
+    scan[1][tid] = tc;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+        for (int ww = 0; ww < W; ww++) cnt[r][ww][tid] = 0;
+    const long long base = (long long)blockIdx.x * kSortTile;
+    uint64_t keyr[R];  // the whole tile is loaded up front: one memory latency, not R
+    int32_t valr[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const long long idx = base + r * 256 + tid;
+        keyr[r] = idx < nst ? kin[idx] : 0;
+        valr[r] = idx < nst ? vin[idx] : 0;
+    }
     __syncthreads();
     for (int d = 1; d < 256; d <<= 1) {
         const uint32_t a0 = tid >= d ? scan[0][tid - d] : 0, a1 = tid >= d ? scan[1][tid - d] : 0;
@@ -513,52 +522,46 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict
         __syncthreads();
     }
     globalBase[tid] = scan[0][tid] - tot + pre;
-    localStart[tid] = scan[1][tid] - cnt;
-    __syncthreads();
-    const long long base = (long long)blockIdx.x * kSortTile;
-    constexpr int R = kSortTile / 256;
-    uint64_t keyr[R];  // the whole tile is loaded up front: one memory latency, not R
-    int32_t valr[R];
+    localStart[tid] = scan[1][tid] - tc;
+    // ranks within each (round, wave) from ballots: all rounds first, so one
+    // barrier, one pass over the (round, wave) counts per digit and one more
+    // barrier place the whole tile (stable: round, wave, lane order)
+    uint32_t digr[R];
+    int rankr[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const long long idx = base + r * 256 + tid;
-        keyr[r] = idx < nst ? kin[idx] : 0;
-        valr[r] = idx < nst ? vin[idx] : 0;
-    }
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        long long idx = base + r * 256 + tid;
-        bool valid = idx < nst;
-        uint64_t key = keyr[r];
-        int32_t val = valr[r];
-        uint32_t dig = valid ? sort_digit(key, val, shift, instPass, batch) : 0u;
+        const bool valid = idx < nst;
+        const uint32_t dig = valid ? sort_digit(keyr[r], valr[r], shift, instPass, batch) : 0u;
         uint64_t match = __ballot(valid);
         for (int b = 0; b < 8; b++) {
-            bool bit = (dig >> b) & 1u;
-            uint64_t bal = __ballot(bit);
+            const bool bit = (dig >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
             match &= bit ? bal : ~bal;
         }
-        int rank = mbcnt(match);
-        if (valid && rank == 0) waveCnt[w][dig] = (uint32_t)__popcll(match);
-        __syncthreads();
-        {
-            uint32_t run = running[tid];
-            for (int ww = 0; ww < 4; ww++) {
-                uint32_t c = waveCnt[ww][tid];
-                waveOff[r & 1][ww][tid] = run;
-                run += c;
-                waveCnt[ww][tid] = 0;
-            }
-            running[tid] = run;
-        }
-        __syncthreads();
-        if (valid) {
-            const uint32_t slot = localStart[dig] + waveOff[r & 1][w][dig] + (uint32_t)rank;
-            skey[slot] = key;
-            sval[slot] = val;
-            sdig[slot] = (uint8_t)dig;
-        }
+        rankr[r] = valid ? mbcnt(match) : -1;
+        digr[r] = dig;
+        if (valid && rankr[r] == 0) cnt[r][w][dig] = (uint16_t)__popcll(match);
     }
+    __syncthreads();
+    {
+        uint32_t run = localStart[tid];
+        for (int r = 0; r < R; r++)
+            for (int ww = 0; ww < W; ww++) {
+                const uint32_t c = cnt[r][ww][tid];
+                cnt[r][ww][tid] = (uint16_t)run;
+                run += c;
+            }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++)
+        if (rankr[r] >= 0) {
+            const uint32_t slot = (uint32_t)cnt[r][w][digr[r]] + (uint32_t)rankr[r];
+            skey[slot] = keyr[r];
+            sval[slot] = valr[r];
+            sdig[slot] = (uint8_t)digr[r];
+        }
     __syncthreads();
     const int count = (int)min((long long)kSortTile, nst - base);
     for (int j = tid; j < count; j += 256) {
