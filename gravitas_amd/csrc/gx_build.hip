@@ -43,7 +43,6 @@ __global__ void k_step_init(Meta* meta, Batch batch) {
         meta->maxStack = 0;
         meta->maxList[0] = meta->maxList[1] = meta->maxList[2] = 0;
         meta->listEntries = meta->rounds = meta->mixed = 0;
-        meta->groupNext = 0;
         for (int q = 0; q < 8; q++) meta->xcdNext[q] = 0;
         meta->ntargets = 0;
         meta->topoWhy = 0;
@@ -1367,7 +1366,6 @@ __global__ void k_fill_u64(unsigned long long* p, int count, unsigned long long 
 __global__ void k_walk_reset(Meta* meta) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     meta->pp = meta->pc = 0;
-    meta->groupNext = 0;
     for (int q = 0; q < 8; q++) meta->xcdNext[q] = 0;
     meta->listEntries = meta->rounds = meta->mixed = 0;
     meta->maxStack = 0;

@@ -102,7 +102,6 @@ struct Meta {
     long long levelCount[kMaxLevels + 1];
     unsigned long long pp;  // particle-particle interactions (incl. self pairs)
     unsigned long long pc;  // particle-cell interactions
-    unsigned long long groupNext;    // (unused)
     unsigned long long xcdNext[8];   // work queues of the persistent walk, one per XCD
     unsigned long long listEntries;  // cell list entries over all groups (FP32 walk)
     unsigned long long rounds;       // FP32 walk traversal rounds over all groups
