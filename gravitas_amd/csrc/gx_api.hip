@@ -1,12 +1,13 @@
 // gx_api.hip -- C ABI and step orchestration (host side of the backend).
 //
 // One force step enqueues, on one stream and with no host synchronisation in
-// between: step init, collect (3 launches), keys, the radix passes (3
-// launches each), gather, one topology launch per tree level, one moment
-// launch per level, the walk and the hole fix-up.  Sizes that only the device
-// knows (stars per instance, nodes per level, walk groups) are read by the
-// kernels from the Meta record and the instance arrays; grids are sized from
-// the body count n and the instance count B.  A single system is B = 1.
+// between: step init, collect and keys (3 launches), the radix passes (3
+// launches each), the sorted copies, the topology (4 launches), the leaf
+// moments and one moment launch per internal level, the walk and the hole
+// fix-up.  Sizes that only the device knows (stars per instance, nodes,
+// walk groups) are read by the kernels from the Meta record and the instance
+// arrays; grids are sized from the body count n and the instance count B.  A
+// single system is B = 1.
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -516,7 +517,7 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
         const int shift = inst ? 8 * (pass - 8) : 8 * pass;
         hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(256), 0, st, kin, vin, meta, shift, inst, batch,
                            (uint32_t*)s->hist.p, ntiles);
-        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(kScanThreads), 0, st, (uint32_t*)s->hist.p, ntiles,
+        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(kRowScanThreads), 0, st, (uint32_t*)s->hist.p, ntiles,
                            (uint32_t*)s->digitTotal.p);
         hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(256), 0, st, kin, vin, kout, vout, meta, shift, inst,
                            batch, (const uint32_t*)s->hist.p, (const uint32_t*)s->digitTotal.p, ntiles,
@@ -532,13 +533,16 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
     // the whole topology at any depth: chains and per-level counts per tile,
     // their scan per level, node records, then ends and links (the sort's
     // spare key and value arrays hold the chains and each chain's first node)
-    hipLaunchKernelGGL(k_topo_count, dim3(ntiles), dim3(256), 0, st, (const uint64_t*)s->keys.p, meta, batch,
-                       (uint16_t*)s->keys2.p, (uint32_t*)s->hist.p, ntiles);
-    hipLaunchKernelGGL(k_sort_scan, dim3(kMaxLevels), dim3(kScanThreads), 0, st, (uint32_t*)s->hist.p, ntiles,
+    static_assert(kMaxLevels * ((kSortTile + kTopoTile - 1) / kTopoTile) <= 256,
+                  "the per-level counts of the topology tiles fit in the sort's histogram");
+    const int ttiles = std::max(1, (int)((n + kTopoTile - 1) / kTopoTile));
+    hipLaunchKernelGGL(k_topo_count, dim3(ttiles), dim3(256), 0, st, (const uint64_t*)s->keys.p, meta, batch,
+                       (uint16_t*)s->keys2.p, (uint32_t*)s->hist.p, ttiles);
+    hipLaunchKernelGGL(k_sort_scan, dim3(kMaxLevels), dim3(kRowScanThreads), 0, st, (uint32_t*)s->hist.p, ttiles,
                        (uint32_t*)s->digitTotal.p);
-    hipLaunchKernelGGL(k_topo_nodes, dim3(ntiles), dim3(256), 0, st, (const uint64_t*)s->keys.p,
+    hipLaunchKernelGGL(k_topo_nodes, dim3(ttiles), dim3(256), 0, st, (const uint64_t*)s->keys.p,
                        (const uint16_t*)s->keys2.p, (const uint32_t*)s->hist.p, (const uint32_t*)s->digitTotal.p, meta,
-                       batch, (TravNode*)s->trav.p, (int32_t*)s->order2.p, s->capNodes, ntiles);
+                       batch, (TravNode*)s->trav.p, (int32_t*)s->order2.p, s->capNodes, ttiles);
     hipLaunchKernelGGL(k_topo_links, dim3(kTreeGrid), dim3(256), 0, st, (const uint64_t*)s->keys.p,
                        (TravNode*)s->trav.p, (const int32_t*)s->order2.p, meta, batch);
     mark(s, 5);

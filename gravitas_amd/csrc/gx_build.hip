@@ -264,27 +264,31 @@ __global__ __launch_bounds__(1024) void k_collect_scan(int nb, int* __restrict__
             ws[1][w] = t[1];
         }
         __syncthreads();
-        if (tid == 0) {
-            long long a0 = 0, a1 = 0;
-            for (int ww = 0; ww < W; ww++) {
-                const long long c0 = ws[0][ww], c1 = ws[1][ww];
-                ws[0][ww] = a0;
-                ws[1][ww] = a1;
-                a0 += c0;
-                a1 += c1;
+        if (w == 0) {  // the first wave scans the wave totals and reduces the wave boxes
+            const long long c0 = lane < W ? ws[0][lane] : 0, c1 = lane < W ? ws[1][lane] : 0;
+            long long p0 = c0, p1 = c1;
+#pragma unroll
+            for (int d = 1; d < W; d <<= 1) {
+                const long long t0 = __shfl_up(p0, d, kWave), t1 = __shfl_up(p1, d, kWave);
+                if (lane >= d) {
+                    p0 += t0;
+                    p1 += t1;
+                }
             }
-            ws[0][W] = a0;
-            ws[1][W] = a1;
-            if (batch.B == 1) {
-                double b6[6];
-                for (int c = 0; c < 6; c++) b6[c] = sb[c][0];
-                for (int ww = 1; ww < W; ww++)
-                    for (int c = 0; c < 3; c++) {
-                        b6[c] = fmin(b6[c], sb[c][ww]);
-                        b6[3 + c] = fmax(b6[3 + c], sb[3 + c][ww]);
-                    }
-                for (int c = 0; c < 6; c++) batch.boxEnc[c] = enc_d(b6[c]);
+            if (lane < W) {
+                ws[0][lane] = p0 - c0;
+                ws[1][lane] = p1 - c1;
             }
+            if (lane == W - 1) {
+                ws[0][W] = p0;
+                ws[1][W] = p1;
+            }
+            if (batch.B == 1)
+                for (int c = 0; c < 6; c++) {
+                    const double v = lane < W ? sb[c][lane] : (c < 3 ? INFINITY : -INFINITY);
+                    const double r = c < 3 ? wave_min_d(v) : wave_max_d(v);
+                    if (lane == 0) batch.boxEnc[c] = enc_d(r);
+                }
         }
         __syncthreads();
         total[0] = ws[0][W];
@@ -445,28 +449,92 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
 }
 
 // One workgroup per digit: exclusive scan of the digit's counts over the
-// tiles, 1,024 consecutive tiles at a time (coalesced loads, a block scan and
-// a running carry); the digit's total.
-__global__ __launch_bounds__(kScanThreads) void k_sort_scan(uint32_t* __restrict__ hist, int ntiles,
-                                                            uint32_t* __restrict__ digitTotal) {
-    __shared__ uint32_t s[kScanThreads];
-    const int tid = threadIdx.x;
+// tiles; the digit's total.  Rows of up to 1,024 tiles: thread t scans the
+// run [t per, (t+1) per) (per <= 4) in registers, and the run sums are scanned
+// with shuffles and one LDS step.  Longer rows go in chunks of 2,048 counts:
+// a chunk is loaded coalesced into LDS (padded by one word in 32 against
+// bank conflicts), thread t takes the counts [8t, 8t+8), the thread sums are
+// scanned as above, the offsets go back through LDS and out coalesced; a
+// running carry joins the chunks.
+__global__ __launch_bounds__(kRowScanThreads) void k_sort_scan(uint32_t* __restrict__ hist, int ntiles,
+                                                               uint32_t* __restrict__ digitTotal) {
+    constexpr int T = kRowScanThreads, K = 8, C = T * K, W = T / kWave;
+    __shared__ uint32_t buf[C + C / 32];
+    __shared__ uint32_t ws[W];
+    const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
     uint32_t* row = hist + (size_t)blockIdx.x * ntiles;
-    uint32_t carry = 0;
-    for (int base = 0; base < ntiles; base += kScanThreads) {
-        const int i = base + tid;
-        const uint32_t c = i < ntiles ? row[i] : 0u;
-        s[tid] = c;
-        __syncthreads();
-        for (int d = 1; d < kScanThreads; d <<= 1) {
-            const uint32_t v = tid >= d ? s[tid - d] : 0u;
-            __syncthreads();
-            s[tid] += v;
-            __syncthreads();
+    if (ntiles <= 4 * T) {
+        const int per = (ntiles + T - 1) / T;
+        const int lo = min(tid * per, ntiles), hi = min(lo + per, ntiles);
+        uint32_t sum = 0;
+        for (int i = lo; i < hi; i++) sum += row[i];
+        uint32_t incl = sum;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t t = __shfl_up(incl, d, kWave);
+            if (lane >= d) incl += t;
         }
-        if (i < ntiles) row[i] = carry + s[tid] - c;
-        carry += s[kScanThreads - 1];
-        __syncthreads();  // s is rewritten by the next chunk
+        if (lane == kWave - 1) ws[w] = incl;
+        __syncthreads();
+        uint32_t off = incl - sum, total = 0;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            const uint32_t x = ws[k];
+            off += k < w ? x : 0u;
+            total += x;
+        }
+        for (int i = lo; i < hi; i++) {
+            const uint32_t c = row[i];
+            row[i] = off;
+            off += c;
+        }
+        if (tid == 0) digitTotal[blockIdx.x] = total;
+        return;
+    }
+    uint32_t carry = 0;
+    for (int base = 0; base < ntiles; base += C) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int j = k * T + tid, i = base + j;
+            buf[j + (j >> 5)] = i < ntiles ? row[i] : 0u;
+        }
+        __syncthreads();
+        uint32_t c[K], sum = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int j = tid * K + k;
+            c[k] = buf[j + (j >> 5)];
+            sum += c[k];
+        }
+        uint32_t incl = sum;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t t = __shfl_up(incl, d, kWave);
+            if (lane >= d) incl += t;
+        }
+        if (lane == kWave - 1) ws[w] = incl;
+        __syncthreads();
+        uint32_t off = carry + incl - sum, tot = 0;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            const uint32_t x = ws[k];
+            off += k < w ? x : 0u;
+            tot += x;
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int j = tid * K + k;
+            buf[j + (j >> 5)] = off;
+            off += c[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int j = k * T + tid, i = base + j;
+            if (i < ntiles) row[i] = buf[j + (j >> 5)];
+        }
+        carry += tot;
+        __syncthreads();  // buf and ws are rewritten by the next chunk
     }
     if (tid == 0) digitTotal[blockIdx.x] = carry;
 }
@@ -487,7 +555,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict
     __shared__ uint32_t globalBase[256];  // output index of digit d's first element of this tile
     __shared__ uint32_t localStart[256];  // staging slot of digit d's first element
     __shared__ uint16_t cnt[R][W][256];   // per (round, wave): elements of digit d, then their first slot
-    __shared__ uint32_t scan[2][256];
+    __shared__ uint32_t wsum[2][W];       // per wave: sums of its 64 digits' totals and tile counts
     __shared__ uint64_t skey[kSortTile];
     __shared__ int32_t sval[kSortTile];
     __shared__ uint8_t sdig[kSortTile];
@@ -498,8 +566,6 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict
     const uint32_t tot = digitTotal[tid];
     const uint32_t pre = hist[(size_t)tid * ntiles + blockIdx.x];
     const uint32_t tc = ((int)blockIdx.x + 1 < ntiles ? hist[(size_t)tid * ntiles + blockIdx.x + 1] : tot) - pre;
-    scan[0][tid] = tot;
-    scan[1][tid] = tc;
 #pragma unroll
     for (int r = 0; r < R; r++)
         for (int ww = 0; ww < W; ww++) cnt[r][ww][tid] = 0;
@@ -512,16 +578,32 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint64_t* __restrict
         keyr[r] = idx < nst ? kin[idx] : 0;
         valr[r] = idx < nst ? vin[idx] : 0;
     }
-    __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {
-        const uint32_t a0 = tid >= d ? scan[0][tid - d] : 0, a1 = tid >= d ? scan[1][tid - d] : 0;
-        __syncthreads();
-        scan[0][tid] += a0;
-        scan[1][tid] += a1;
-        __syncthreads();
+    // exclusive scans over the digits (thread = digit): shuffles within each
+    // wavefront, then the wavefronts' sums (the barrier also orders the
+    // zeroed counts before the ranks below)
+    const int lane = lane_id();
+    uint32_t a0 = tot, a1 = tc;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t t0 = __shfl_up(a0, d, kWave), t1 = __shfl_up(a1, d, kWave);
+        if (lane >= d) {
+            a0 += t0;
+            a1 += t1;
+        }
     }
-    globalBase[tid] = scan[0][tid] - tot + pre;
-    localStart[tid] = scan[1][tid] - tc;
+    if (lane == kWave - 1) {
+        wsum[0][w] = a0;
+        wsum[1][w] = a1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < W; k++)
+        if (k < w) {
+            a0 += wsum[0][k];
+            a1 += wsum[1][k];
+        }
+    globalBase[tid] = a0 - tot + pre;
+    localStart[tid] = a1 - tc;
     // ranks within each (round, wave) from ballots: all rounds first, so one
     // barrier, one pass over the (round, wave) counts per digit and one more
     // barrier place the whole tile (stable: round, wave, lane order)
@@ -624,7 +706,6 @@ __global__ __launch_bounds__(256) void k_scatter_spos(const double* __restrict__
 //   k_topo_links   split nodes' ends (a search on the keys), skip links (the
 //                  node after a subtree is the chain that starts at its end),
 //                  child counts and parents.
-constexpr int kTopoTile = kSortTile;           // stars per topology tile (256 threads x 8)
 constexpr int kTopoPre = kLeafSize;            // deltas before the tile: delta_{ts-24} ..
 constexpr int kTopoDeltas = kTopoTile + kTopoPre + kLeafSize - 1;  // .. delta_{te+22}
 constexpr int kTopoMins = kTopoTile + kTopoPre;                    // m(j), j = ts-24 .. te-1

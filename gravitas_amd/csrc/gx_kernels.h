@@ -11,6 +11,7 @@ int device_ready();
 
 constexpr int kCollectTile = 1024;  // bodies per collect workgroup (256 threads x 4)
 constexpr int kSortTile = 2048;     // keys per sort tile (256 threads x 8; the scatter stages a tile in LDS)
+constexpr int kTopoTile = 1024;     // stars per topology tile (256 threads x 4)
 
 // gx_build.hip
 __global__ void k_step_init(Meta* meta, Batch batch);
@@ -24,7 +25,8 @@ __global__ void k_collect_scatter(const double* x, const double* y, const double
                                   Batch batch, int32_t* order, int32_t* holes, uint64_t* keys);
 __global__ void k_sort_hist(const uint64_t* keys, const int32_t* vals, const Meta* meta, int shift, int instPass,
                             Batch batch, uint32_t* hist, int ntiles);
-constexpr int kScanThreads = 1024;  // threads of the per-digit scan over the tiles
+constexpr int kScanThreads = 1024;  // threads of the one-workgroup scans (collect, active sets)
+constexpr int kRowScanThreads = 256;  // threads of the per-digit scan over the tiles
 __global__ void k_sort_scan(uint32_t* hist, int ntiles, uint32_t* digitTotal);
 __global__ void k_sort_scatter(const uint64_t* kin, const int32_t* vin, uint64_t* kout, int32_t* vout,
                                const Meta* meta, int shift, int instPass, Batch batch, const uint32_t* hist,

@@ -1014,7 +1014,7 @@ int hydro_run(gx_hydro* H, const double* x, const double* y, const double* z, co
     for (int pass = 0; pass < 8; pass++) {
         hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(256), 0, st, kin, vin, sm, 8 * pass, 0, batch,
                            (uint32_t*)H->hist.p, ntiles);
-        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(kScanThreads), 0, st, (uint32_t*)H->hist.p, ntiles,
+        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(kRowScanThreads), 0, st, (uint32_t*)H->hist.p, ntiles,
                            (uint32_t*)H->digitTotal.p);
         hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(256), 0, st, kin, vin, kout, vout, sm, 8 * pass, 0,
                            batch, (const uint32_t*)H->hist.p, (const uint32_t*)H->digitTotal.p, ntiles,
