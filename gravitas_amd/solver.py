@@ -497,7 +497,7 @@ class Solver:
         FP32 walk, "walk" is traversal + evaluation."""
         buf = (ctypes.c_float * 9)()
         k = self._L.gx_phase_ms(self._h, buf, 9)
-        names = ("collect", "keys", "sort", "gather", "topology", "moments", "walk", "total", "evaluate")
+        names = ("collect", "keys", "sort", "sorted_copies", "topology", "moments", "walk", "total", "evaluate")
         d = {names[i]: float(buf[i]) for i in range(k)}
         if d.get("evaluate", 0.0) > 0.0:
             d["traverse"] = d["walk"]

@@ -435,9 +435,11 @@ typedef struct {
 int gx_get_stats(const gx_solver *s, gx_stats *out);
 
 /* Per-phase GPU milliseconds of the last step when timing is on:
- * 0 collect, 1 keys, 2 sort, 3 gather, 4 topology, 5 moments, 6 walk (for
- * the FP32 walk: its traversal kernel), 7 total, 8 FP32 walk evaluation
- * kernel (0 for the FP64 walk).  Returns the number of phases written. */
+ * 0 collect (which also writes the Morton keys), 1 keys (empty since), 2 sort,
+ * 3 sorted copies, 4 topology, 5 moments, 6 walk (for the FP32 walk run as
+ * two kernels: its traversal kernel), 7 total, 8 FP32 walk evaluation kernel
+ * (0 unless the walk runs as two kernels).  Returns the number of phases
+ * written. */
 int gx_phase_ms(const gx_solver *s, float *ms, int n);
 
 /* Test hook with no counterpart in the reference: copies count items of
