@@ -544,7 +544,7 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
                        (const uint16_t*)s->keys2.p, (const uint32_t*)s->hist.p, (const uint32_t*)s->digitTotal.p, meta,
                        batch, (TravNode*)s->trav.p, (int32_t*)s->order2.p, s->capNodes, ttiles);
     hipLaunchKernelGGL(k_topo_links, dim3(kTreeGrid), dim3(256), 0, st, (const uint64_t*)s->keys.p,
-                       (TravNode*)s->trav.p, (const int32_t*)s->order2.p, meta, batch);
+                       (const uint16_t*)s->keys2.p, (TravNode*)s->trav.p, (const int32_t*)s->order2.p, meta, batch);
     mark(s, 5);
     // all leaves first (and the node count), then the internal nodes level by
     // level, bottom up, from level Lmax-1 (the last step's depth + 1)

@@ -990,28 +990,67 @@ __global__ __launch_bounds__(256) void k_topo_nodes(const uint64_t* __restrict__
     }
 }
 
-// End of a run: the first index in (lo, end) whose key differs from keys[lo]
-// above bit shift (end if none); a galloping then a binary search.
-__device__ __forceinline__ int32_t run_end(const uint64_t* __restrict__ keys, int32_t lo, int32_t end, int shift) {
+// End of the level-L run that starts at star lo, for a split node (its run
+// holds more than kLeafSize stars).  The stars inside the run start chains at
+// levels above L only, and the first star after it starts one at level L or
+// below, so the end is the first star from a = lo + kLeafSize + 1 on whose
+// chain starts at a level <= L.  It is looked up in the chains of the 64
+// stars from a (nine aligned 16-byte loads in flight together, each 32-bit
+// word tested for both its stars at once: bit 15 of 0x8000 + L - first is
+// set iff first <= L), then, for a longer run, by a galloping search on the
+// keys with steps from 64 up.  The chain array has room past n (it lives in
+// an 8-byte-per-star buffer); stars at or past end are never taken.
+__device__ __forceinline__ int32_t split_end(const uint64_t* __restrict__ keys, const uint16_t* __restrict__ chain,
+                                             int32_t lo, int32_t end, int L) {
+    const int32_t a = lo + kLeafSize + 1;
+    if (a >= end) return end;
+    const int32_t a0 = a & ~7;
+    uint4 v[9];
+    const uint4* p = reinterpret_cast<const uint4*>(chain + a0);
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[k] = p[k];
+    const uint32_t c = (0x8000u + (uint32_t)L) * 0x00010001u;
+    uint64_t m0 = 0;  // bit i: star a0 + i starts a chain at a level <= L (i < 64)
+    uint32_t m1 = 0;  // the same for a0 + 64 + i (i < 8)
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const uint32_t w[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            const uint32_t y = (c - (w[h] & 0x00FF00FFu)) & 0x80008000u;
+            const uint32_t two = (y >> 15 & 1u) | (y >> 30 & 2u);
+            if (k < 8)
+                m0 |= (uint64_t)two << (8 * k + 2 * h);
+            else
+                m1 |= two << (2 * h);
+        }
+    }
+    const int sh = a - a0;  // 0..7: drop the stars before a, keep the window [a, a + 64)
+    const uint64_t win = (m0 >> sh) | (sh ? (uint64_t)m1 << (64 - sh) : 0ull);
+    if (win) return min(end, a + (int32_t)__builtin_ctzll(win));
+    if (a + 64 >= end) return end;
+    // the run holds a .. a + 63: gallop on the keys from there
+    const int shift = 3 * (kMortonBits - L);
     const uint64_t k0 = keys[lo];
     auto same = [&](int32_t j) { return ((keys[j] ^ k0) >> shift) == 0; };
-    int32_t a = lo, d = 1;
-    while (a + d < end && same(a + d)) {
-        a += d;
+    int32_t x = a + 63, d = 64;
+    while (x + d < end && same(x + d)) {
+        x += d;
         d *= 2;
     }
-    int32_t b = min(end, a + d);
-    while (b - a > 1) {
-        const int32_t mid = a + (b - a) / 2;
+    int32_t y = min(end, x + d);
+    while (y - x > 1) {
+        const int32_t mid = x + (y - x) / 2;
         if (same(mid))
-            a = mid;
+            x = mid;
         else
-            b = mid;
+            y = mid;
     }
-    return b;
+    return y;
 }
 
-__global__ __launch_bounds__(256) void k_topo_links(const uint64_t* __restrict__ keys, TravNode* __restrict__ trav,
+__global__ __launch_bounds__(256) void k_topo_links(const uint64_t* __restrict__ keys,
+                                                    const uint16_t* __restrict__ chain, TravNode* __restrict__ trav,
                                                     const int32_t* __restrict__ chainFirst, Meta* meta,
                                                     Batch batch) {
     if (meta->flags & kFlagNodeOverflow) return;
@@ -1026,7 +1065,7 @@ __global__ __launch_bounds__(256) void k_topo_links(const uint64_t* __restrict__
         }
         int32_t hi = trav[t].hi;
         const int32_t end = (int32_t)batch.starOff[inst + 1];
-        if (hi < 0) hi = L == 0 ? end : run_end(keys, lo, end, 3 * (kMortonBits - L));
+        if (hi < 0) hi = L == 0 ? end : split_end(keys, chain, lo, end, L);
         const int32_t next = hi >= end ? -1 : chainFirst[hi];
         // every link must lead to a node that starts where it should (the
         // walks follow them): a first child at lo one level down, a skip link
@@ -1039,8 +1078,15 @@ __global__ __launch_bounds__(256) void k_topo_links(const uint64_t* __restrict__
             const long long cend = meta->levelStart[L + 1] + meta->levelCount[L + 1];
             if (first >= cend || trav[first].lo != lo || trav[first].level != L + 1) why |= 16;
             if (!why) {
+                // the children are the level-(L+1) nodes from first that start
+                // before hi (at most eight; their starts ascend): all eight
+                // starts are loaded at once
+                int32_t clo[8];
+#pragma unroll
+                for (int c = 1; c < 8; c++) clo[c] = first + c < cend ? trav[first + c].lo : INT32_MAX;
                 nchild = 1;
-                while (nchild < 8 && first + nchild < cend && trav[first + nchild].lo < hi) nchild++;
+#pragma unroll
+                for (int c = 1; c < 8; c++) nchild += clo[c] < hi;
                 for (int c = 0; c < nchild; c++) trav[first + c].parent = (int32_t)t;
             }
         }

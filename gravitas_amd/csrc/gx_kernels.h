@@ -39,7 +39,8 @@ __global__ void k_topo_count(const uint64_t* keys, Meta* meta, Batch batch, uint
 __global__ void k_topo_nodes(const uint64_t* keys, const uint16_t* chain, const uint32_t* tileOff,
                              const uint32_t* levelTotal, Meta* meta, Batch batch, TravNode* trav, int32_t* chainFirst,
                              long long cap, int ntiles);
-__global__ void k_topo_links(const uint64_t* keys, TravNode* trav, const int32_t* chainFirst, Meta* meta, Batch batch);
+__global__ void k_topo_links(const uint64_t* keys, const uint16_t* chain, TravNode* trav, const int32_t* chainFirst,
+                             Meta* meta, Batch batch);
 __global__ void k_moments_level(int L, const double4* spos, TravNode* trav, Mult64* m64, Mult32* m32,
                                 Mult16* m16, const Meta* meta, Batch batch);
 __global__ void k_moments_leaves(const double4* spos, TravNode* trav, Mult64* m64, Mult32* m32, Mult16* m16,

@@ -61,6 +61,36 @@ def chain_tree(keys, off):
     return sorted(nodes)
 
 
+def chain_first_levels(keys, off):
+    """Per star, the level its chain of nodes starts at (delta_{s-1}+1), or
+    22 when no node starts there: what the GPU keeps per star."""
+    n = len(keys)
+    out = np.full(n, BITS + 1, np.int64)
+    for L, s, e in chain_tree(keys, off):
+        out[s] = min(out[s], L)
+    return out
+
+
+def split_ends(keys, off):
+    """The end of every split node by the rule of the links pass: the first
+    star from start + LEAF + 1 on whose chain starts at the node's level or
+    above (a smaller level number), or the instance's end."""
+    first = chain_first_levels(keys, off)
+    inst_end = {}
+    for b in range(len(off) - 1):
+        for s in range(int(off[b]), int(off[b + 1])):
+            inst_end[s] = int(off[b + 1])
+    got = []
+    for L, s, e in split_tree(keys, off):
+        if e - s > LEAF and L < BITS and L > 0:
+            end = inst_end[s]
+            j = s + LEAF + 1
+            while j < end and first[j] > L:
+                j += 1
+            got.append((L, s, min(j, end), e))
+    return got
+
+
 def key_sets(seed):
     rng = np.random.default_rng(seed)
     for trial in range(12):
@@ -97,3 +127,10 @@ def test_edge_cases():
                       (np.sort(np.random.default_rng(4).integers(0, 1 << 63, 60, dtype=np.uint64)), [0, 30, 30, 60])):
         off = np.asarray(off, np.int64)
         assert chain_tree(keys, off) == split_tree(keys, off)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_split_node_ends_from_the_chains(seed):
+    for keys, off in key_sets(seed):
+        for L, s, got, want in split_ends(keys, off):
+            assert got == want, (L, s, got, want)
