@@ -361,7 +361,7 @@ def bench_clumps(args):
 
 def tune_name(r):
     return (f"group{r['group_size']}-{'queue' if r['persistent'] == 2 else 'grid'}"
-            f"-stack{r['stack']}-staging{r['parts']}")
+            f"-stack{r['stack']}-staging{r['parts']}" + (f"-waves{r['waves']}" if r['waves'] != 4 else ""))
 
 
 def batched_measure(steps, warmup):

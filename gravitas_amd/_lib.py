@@ -58,7 +58,7 @@ class GxStats(ctypes.Structure):
 
 class GxTuneEntry(ctypes.Structure):
     _fields_ = [("group_size", ctypes.c_int32), ("persistent", ctypes.c_int32), ("stack", ctypes.c_int32),
-                ("parts", ctypes.c_int32), ("ms", ctypes.c_float), ("pp_interactions", ctypes.c_int64),
+                ("parts", ctypes.c_int32), ("waves", ctypes.c_int32), ("ms", ctypes.c_float), ("pp_interactions", ctypes.c_int64),
                 ("pc_interactions", ctypes.c_int64)]
 
 
@@ -136,7 +136,7 @@ def load():
     L.gx_drift_dev.argtypes = [P, P, P, P, P, P, P, I64, D]
     L.gx_solver_tune.argtypes = [P, I, I]
     L.gx_solver_reserve.argtypes = [P, I64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
-    L.gx_solver_tune_walk.argtypes = [P, I, I, I, I]
+    L.gx_solver_tune_walk.argtypes = [P, I, I, I, I, I]
     L.gx_accelerations_noalloc.argtypes = [P, P, P, P, P, P, I64, D, D, I, I, P, P, P, P]
     L.gx_solver_autotune.argtypes = [P, P, P, P, P, P, P, P, ctypes.c_int32, P, P, I, P, P, P, ctypes.c_int32,
                                      ctypes.POINTER(GxTuneEntry), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]

@@ -326,16 +326,18 @@ int gx_solver_tune(gx_solver* s, int group_size, int persistent) {
     return GX_OK;
 }
 
-int gx_solver_tune_walk(gx_solver* s, int group_size, int persistent, int stack, int parts) {
+int gx_solver_tune_walk(gx_solver* s, int group_size, int persistent, int stack, int parts, int waves) {
     int rc = gx_solver_tune(s, group_size, persistent);
     if (rc) return rc;
     for (int c = 0; c < kWalkConfigs; c++)
-        if (kWalkConfig[c].stack == stack && kWalkConfig[c].parts == parts) {
+        if (kWalkConfig[c].stack == stack && kWalkConfig[c].parts == parts && kWalkConfig[c].waves == waves) {
             s->walkConfig = c;
             return GX_OK;
         }
-    return set_err(GX_ERR_ARG, "no walk built with stack %d and staging %d (stack 384 or 768, staging 128, 256 or 384)",
-                   stack, parts);
+    return set_err(GX_ERR_ARG,
+                   "no walk built with stack %d, staging %d and %d waves (stack 384 or 768, staging 128, 256 or "
+                   "384, 4 waves; stack 384 with staging 128 also 5 or 6 waves)",
+                   stack, parts, waves);
 }
 
 int gx_solver_reserve(gx_solver* s, int64_t nodes, int32_t full, int32_t partial, int32_t leaves) {
@@ -346,6 +348,8 @@ int gx_solver_reserve(gx_solver* s, int64_t nodes, int32_t full, int32_t partial
         s->capNodes = nodes;
         s->nodesSet = true;
     }
+    if (full > kMaxListCap || partial > kMaxListCap || leaves > kMaxListCap)
+        return set_err(GX_ERR_ARG, "list capacities above %d", kMaxListCap);
     if (full > 0) s->cf = full;
     if (partial > 0) s->cp = partial;
     if (leaves > 0) s->cl = leaves;
@@ -730,9 +734,15 @@ bool regrow(gx_solver* s) {
         return true;
     }
     if ((h.flags & kFlagListOverflow) && !s->fixedCaps) {
-        s->cf = std::max(s->cf, (int)(h.maxList[0] * 1.25) + 64);
-        s->cp = std::max(s->cp, (int)(h.maxList[1] * 1.25) + 64);
-        s->cl = std::max(s->cl, (int)(h.maxList[2] * 1.25) + 64);
+        // list entries are addressed with 24-bit products in the walk
+        auto grown = [](int cap, int seen) {
+            return (int)std::min<long long>(kMaxListCap, std::max<long long>(cap, (long long)(seen * 1.25) + 64));
+        };
+        const int cf = grown(s->cf, h.maxList[0]), cp = grown(s->cp, h.maxList[1]), cl = grown(s->cl, h.maxList[2]);
+        if (cf == s->cf && cp == s->cp && cl == s->cl) return false;  // at the limit: the FP64 walk takes over
+        s->cf = cf;
+        s->cp = cp;
+        s->cl = cl;
         return true;
     }
     return false;
@@ -746,7 +756,7 @@ int run_dev(gx_solver* s, const StepArgs& a) {
         rc = finish_step(s);
         if (rc && rc != GX_ERR_CAPACITY) return rc;
         if (regrow(s)) continue;
-        if (s->hmeta->flags & (kFlagStackOverflow | (s->fixedCaps ? kFlagListOverflow : 0))) {
+        if (s->hmeta->flags & (kFlagStackOverflow | kFlagListOverflow)) {
             // the FP32 walk's LDS stack (or a fixed list) was too small for this
             // tree: redo the walk with the FP64 per-particle walk (same
             // interaction set)
@@ -1809,6 +1819,7 @@ int gx_solver_autotune(gx_solver* s, const double* x, const double* y, const dou
                     e.persistent = pers;
                     e.stack = kWalkConfig[cfg].stack;
                     e.parts = kWalkConfig[cfg].parts;
+                    e.waves = kWalkConfig[cfg].waves;
                     e.ms = (h.flags & kFlagStackOverflow) ? INFINITY : ms / (float)reps;  // a stack overflow disqualifies
                     e.pp_interactions = (int64_t)h.pp - (int64_t)h.ntargets;
                     e.pc_interactions = (int64_t)h.pc;
@@ -1828,7 +1839,9 @@ int gx_solver_autotune(gx_solver* s, const double* x, const double* y, const dou
     s->gsize = t[best].group_size;
     s->persistent = t[best].persistent;
     for (int c = 0; c < kWalkConfigs; c++)
-        if (kWalkConfig[c].stack == t[best].stack && kWalkConfig[c].parts == t[best].parts) s->walkConfig = c;
+        if (kWalkConfig[c].stack == t[best].stack && kWalkConfig[c].parts == t[best].parts &&
+            kWalkConfig[c].waves == t[best].waves)
+            s->walkConfig = c;
     s->tuned.push_back({a.n, count, s->walk, s->monopole, s->gsize, s->persistent, s->walkConfig, t});
     report(t);
     // leave the forces of the chosen walk in ax, ay, az

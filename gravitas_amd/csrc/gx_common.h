@@ -144,6 +144,14 @@ __device__ __forceinline__ int mbcnt(uint64_t m) {
 
 __device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, kWave); }
 
+// A double that is the same in every active lane, moved to scalar registers.
+__device__ __forceinline__ double uniform_d(double v) {
+    const long long u = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return __longlong_as_double((long long)((uint64_t)hi << 32 | lo));
+}
+
 __device__ __forceinline__ double wave_min_d(double v) {
     for (int m = 32; m >= 1; m >>= 1) v = fmin(v, shfl_xor_d(v, m));
     return v;
@@ -171,6 +179,22 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
     v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
     v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
     v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return v;
+}
+
+// v_writelane_b32: lane `lane` of the result is `value` (uniform), the other
+// lanes keep `old` (the LLVM intrinsic, which has no clang builtin here).
+extern "C" __device__ int gx_llvm_writelane(int value, int lane, int old) __asm("llvm.amdgcn.writelane");
+
+// Inclusive running maximum across the wave (every lane active, values >= 0),
+// with the same DPP moves as wave_incl_scan.
+__device__ __forceinline__ int wave_incl_max(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));  // row_bcast:15 into rows 1, 3
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));  // row_bcast:31 into rows 2, 3
     return v;
 }
 

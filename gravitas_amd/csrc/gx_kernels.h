@@ -212,6 +212,9 @@ struct GroupCounts {
     int overflow;    // the traversal stack overflowed
     int pad[2];
 };
+// Largest per-group list capacity (entries are addressed with 24-bit products).
+constexpr int kMaxListCap = (1 << 24) - 1;
+
 struct Lists {
     uint32_t* full;
     uint3* part;
@@ -221,16 +224,20 @@ struct Lists {
     int perSlot;
 };
 
-// Compile-time sizes of the FP32 walk: traversal stack entries and stars
-// staged per evaluation flush.  kWalkConfigs lists the instantiated pairs;
-// the autotuner picks among them (gx_solver_autotune).
+// Compile-time shapes of the FP32 walk: traversal stack entries, stars staged
+// per evaluation flush, and the wavefronts per SIMD the register allocation
+// must leave room for (4 is what the walk needs unconstrained; 5 and 6 make
+// the compiler fit 96 and 80 VGPRs, with a few spills, and only pay where the
+// LDS of a workgroup lets that many stay resident: the small stack and
+// staging).  kWalkConfigs lists the instances; the autotuner picks among them
+// (gx_solver_autotune).
 struct WalkConfig {
-    int stack, parts;
+    int stack, parts, waves;
 };
-constexpr int kWalkConfigs = 6;
-constexpr WalkConfig kWalkConfig[kWalkConfigs] = {{384, 128}, {384, 256}, {384, 384},
-                                                  {768, 128}, {768, 256}, {768, 384}};
-constexpr int kWalkDefault = 4;  // 768 entries, 256 stars
+constexpr int kWalkConfigs = 8;
+constexpr WalkConfig kWalkConfig[kWalkConfigs] = {{384, 128, 4}, {384, 256, 4}, {384, 384, 4}, {768, 128, 4},
+                                                  {768, 256, 4}, {768, 384, 4}, {384, 128, 5}, {384, 128, 6}};
+constexpr int kWalkDefault = 4;  // 768 entries, 256 stars, 4 waves
 
 // m16 non-null selects the FP16-stored quadrupoles; eps0 marks a batch with an
 // unsoftened instance; persistentBlocks > 0 runs the fused walk as a work queue

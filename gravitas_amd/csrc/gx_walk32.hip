@@ -82,7 +82,7 @@ template <bool QUAD>
 __device__ __forceinline__ void eval_cells(const float4* __restrict__ cell, int ncell, float px, float py, float pz,
                                            float eps2, float& ax, float& ay, float& az) {
     ncell = __builtin_amdgcn_readfirstlane(ncell);  // uniform: the loop counter lives in SGPRs
-#pragma unroll 2
+#pragma unroll 4
     for (int e = 0; e < ncell; e++)
         cell_force<QUAD>(cell[3 * e], cell[3 * e + 1], cell[3 * e + 2], px, py, pz, eps2, ax, ay, az);
 }
@@ -201,6 +201,18 @@ __device__ __forceinline__ void eval_parts_own(const float4* __restrict__ part, 
     }
 }
 
+// Element i (0 <= i < 2^24, the list capacities' bound) of an array whose base
+// is uniform, through a 32-bit byte offset: a scalar base plus one vector
+// offset register from a 24-bit multiply, no 64-bit address math.
+template <typename T>
+__device__ __forceinline__ T& at32(T* base, int i) {
+    return *(T*)((char*)base + __umul24((uint32_t)i, (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ const T& at32(const T* base, int i) {
+    return *(const T*)((const char*)base + __umul24((uint32_t)i, (uint32_t)sizeof(T)));
+}
+
 // Group prologue shared by both kernels: the group's stars (up to gsize
 // consecutive targets of one instance in Morton order: all its stars, or the
 // active ones of an active set), FP64 bounding box and centre c_g, the
@@ -223,22 +235,23 @@ __device__ __forceinline__ bool load_group(Group& G, const double4* __restrict__
     G.inst = upper_index(batch.groupOff, batch.B, g);
     const long long base = batch.tOff[G.inst] + (g - batch.groupOff[G.inst]) * batch.gsize;
     const long long count = min((long long)batch.gsize, batch.tOff[G.inst + 1] - base);
-    G.root = batch.root[G.inst];
-    G.eps2 = (float)batch.eps2[G.inst];
-    G.half = batch.half[G.inst];
+    G.root = __builtin_amdgcn_readfirstlane(batch.root[G.inst]);
+    G.eps2 = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, (float)batch.eps2[G.inst])));
+    G.half = uniform_d(batch.half[G.inst]);
     G.valid = lane_id() < count;
     G.k = target_star(batch, G.valid ? base + lane_id() : base);
     G.p = spos[G.k];
     G.gmask = __ballot(G.valid);
-    G.bminx = wave_min_d(G.valid ? G.p.x : INFINITY);
-    G.bmaxx = wave_max_d(G.valid ? G.p.x : -INFINITY);
-    G.bminy = wave_min_d(G.valid ? G.p.y : INFINITY);
-    G.bmaxy = wave_max_d(G.valid ? G.p.y : -INFINITY);
-    G.bminz = wave_min_d(G.valid ? G.p.z : INFINITY);
-    G.bmaxz = wave_max_d(G.valid ? G.p.z : -INFINITY);
-    G.cgx = 0.5 * (G.bminx + G.bmaxx);
-    G.cgy = 0.5 * (G.bminy + G.bmaxy);
-    G.cgz = 0.5 * (G.bminz + G.bmaxz);
+    // the box and its centre are the same in every lane: scalar registers
+    G.bminx = uniform_d(wave_min_d(G.valid ? G.p.x : INFINITY));
+    G.bmaxx = uniform_d(wave_max_d(G.valid ? G.p.x : -INFINITY));
+    G.bminy = uniform_d(wave_min_d(G.valid ? G.p.y : INFINITY));
+    G.bmaxy = uniform_d(wave_max_d(G.valid ? G.p.y : -INFINITY));
+    G.bminz = uniform_d(wave_min_d(G.valid ? G.p.z : INFINITY));
+    G.bmaxz = uniform_d(wave_max_d(G.valid ? G.p.z : -INFINITY));
+    G.cgx = uniform_d(0.5 * (G.bminx + G.bmaxx));
+    G.cgy = uniform_d(0.5 * (G.bminy + G.bmaxy));
+    G.cgz = uniform_d(0.5 * (G.bminz + G.bmaxz));
     return true;
 }
 
@@ -261,8 +274,7 @@ template <int STACK>
 struct TravLDS {
     uint64_t stkMask[STACK];
     uint32_t stkRange[STACK];  // first | (count - 1) << 28
-    uint64_t popMask[kWave];
-    int popNode[kWave];
+    int popOwner[kWave];       // lane whose range the popped slot starts
 };
 
 template <int PARTS>
@@ -285,8 +297,7 @@ __device__ bool trav_group(const Group& G, long long r, TravLDS<STACK>& S, const
                            Meta* meta, const Lists& lists) {
     uint64_t* stkMask = S.stkMask;
     uint32_t* stkRange = S.stkRange;
-    uint64_t* popMask = S.popMask;
-    int* popNode = S.popNode;
+    int* popOwner = S.popOwner;
     const int lane = lane_id();
     const double4 p = G.p;
     uint32_t* __restrict__ fullOut = lists.full + (size_t)r * lists.cf;
@@ -315,10 +326,6 @@ __device__ bool trav_group(const Group& G, long long r, TravLDS<STACK>& S, const
         const int excl0 = incl0 - cnt;
         const int take = max(0, min(cnt, kWave - excl0));
         const int rfirst = (int)(rng & 0x0fffffffu);
-        for (int j = 0; j < take; j++) {
-            popNode[excl0 + j] = rfirst + j;
-            popMask[excl0 + j] = rmask;
-        }
         const int nfull = __popcll(__ballot(cnt > 0 && take == cnt));
         const int npop = min(kWave, __shfl(incl0, 63, kWave));
         // a partly taken range stays on the stack with its remaining children
@@ -326,13 +333,24 @@ __device__ bool trav_group(const Group& G, long long r, TravLDS<STACK>& S, const
             stkRange[sp - 1 - lane] = (uint32_t)(rfirst + take) | ((uint32_t)(cnt - take - 1) << 28);
         sp -= nfull;
         nrounds++;
+        // Slot j of the round holds node rfirst + (j - excl0) of the last lane
+        // whose run starts at or before j: each taking lane marks its first
+        // slot, and a running maximum over the slots spreads the marks (the
+        // lanes' runs ascend with the lane), so no lane loops over its run.
+        popOwner[lane] = 0;
         __syncthreads();
+        if (take > 0) popOwner[excl0] = lane;
+        __syncthreads();
+        const int owner = wave_incl_max(popOwner[lane]);
         const bool has = lane < npop;
+        const int ofirst = __shfl(rfirst, owner, kWave);
+        const int ostart = __shfl(excl0, owner, kWave);
+        const uint64_t omaskAll = (uint64_t)__shfl((long long)rmask, owner, kWave);
         int node = 0;
         uint64_t mask = 0;
         if (has) {
-            node = popNode[lane];
-            mask = popMask[lane];
+            node = ofirst + (lane - ostart);
+            mask = omaskAll;
         }
         double tx = 0, ty = 0, tz = 0, tc = 0;
         int tfirst = -1, tnchild = 0, tlo = 0, thi = 0;
@@ -360,7 +378,9 @@ __device__ bool trav_group(const Group& G, long long r, TravLDS<STACK>& S, const
         uint64_t omask = cls == 2 ? mask : 0;
         const uint64_t mixed = __ballot(cls == 3);
         nmixed += __popcll(mixed);
-        uint64_t accMine = 0;  // lane src: the ballot of its mixed node
+        // lane src: the ballot of its mixed node, written into that lane from
+        // the scalar registers (two writelanes per node)
+        int accLo = 0, accHi = 0;
         for (uint64_t todo = mixed; todo;) {
             const int src = __builtin_ctzll(todo);
             todo &= todo - 1;
@@ -374,11 +394,12 @@ __device__ bool trav_group(const Group& G, long long r, TravLDS<STACK>& S, const
             const double d2 = dx * dx + dy * dy + dz * dz;
             const uint64_t acc = __ballot(d2 >= ncrit2);
             // lane src keeps the ballot and applies its node's lane mask after
-            // the loop (no lane reads of the mask, one select instead of two)
-            if (lane == src) accMine = acc;
+            // the loop (no lane reads of the mask)
+            accLo = gx_llvm_writelane((int)(uint32_t)acc, src, accLo);
+            accHi = gx_llvm_writelane((int)(uint32_t)(acc >> 32), src, accHi);
         }
         if (cls == 3) {
-            const uint64_t acc = accMine & mask;
+            const uint64_t acc = ((uint64_t)(uint32_t)accHi << 32 | (uint32_t)accLo) & mask;
             amask = acc;
             omask = mask & ~acc;
         }
@@ -390,10 +411,10 @@ __device__ bool trav_group(const Group& G, long long r, TravLDS<STACK>& S, const
         const uint64_t bp = __ballot(isCell && !isFull);
         if (isFull) {
             const int pos = nf + mbcnt(bf);
-            if (pos < lists.cf) fullOut[pos] = (uint32_t)node;
+            if (pos < lists.cf) at32(fullOut, pos) = (uint32_t)node;
         } else if (isCell) {
             const int pos = np + mbcnt(bp);
-            if (pos < lists.cp) partOut[pos] = make_uint3((uint32_t)node, (uint32_t)amask, (uint32_t)(amask >> 32));
+            if (pos < lists.cp) at32(partOut, pos) = make_uint3((uint32_t)node, (uint32_t)amask, (uint32_t)(amask >> 32));
         }
         if (isCell) npc += (unsigned long long)__popcll(amask);
         nf += __popcll(bf);
@@ -407,14 +428,14 @@ __device__ bool trav_group(const Group& G, long long r, TravLDS<STACK>& S, const
         const uint64_t blp = __ballot(isLeaf && !leafAll);
         if (leafAll) {
             const int pos = nlf + mbcnt(blf);
-            if (pos < lists.cl) leafOut[pos] = make_uint3((uint32_t)node, (uint32_t)omask, (uint32_t)(omask >> 32));
+            if (pos < lists.cl) at32(leafOut, pos) = make_uint3((uint32_t)node, (uint32_t)omask, (uint32_t)(omask >> 32));
         } else if (isLeaf) {
             const int pos = nlp + mbcnt(blp);
             if (pos < lists.cl)
-                leafOut[lists.cl - 1 - pos] = make_uint3((uint32_t)node, (uint32_t)omask, (uint32_t)(omask >> 32));
+                at32(leafOut, lists.cl - 1 - pos) = make_uint3((uint32_t)node, (uint32_t)omask, (uint32_t)(omask >> 32));
             nps += thi - tlo;
         }
-        if (isLeaf) npp += (unsigned long long)__popcll(omask) * (unsigned long long)(thi - tlo);
+        if (isLeaf) npp += (unsigned long long)(uint32_t)__popcll(omask) * (uint32_t)(thi - tlo);
         nlf += __popcll(blf);
         nlp += __popcll(blp);
 
@@ -437,8 +458,8 @@ __device__ bool trav_group(const Group& G, long long r, TravLDS<STACK>& S, const
         __syncthreads();
     }
 
-    npp = wave_sum_u64(npp);
-    npc = wave_sum_u64(npc);
+    const unsigned long long gpp = wave_sum_u64(npp);
+    const unsigned long long gpc = wave_sum_u64(npc);
     nps = wave_sum_i(nps);
     const bool listOverflow = nf > lists.cf || np > lists.cp || nlf + nlp > lists.cl;
     if (lane == 0) {
@@ -447,8 +468,8 @@ __device__ bool trav_group(const Group& G, long long r, TravLDS<STACK>& S, const
         c.overflow = overflow ? 1 : 0;
         c.pad[0] = c.pad[1] = 0;
         lists.counts[r] = c;
-        atomicAdd(&meta->pp, npp);
-        atomicAdd(&meta->pc, npc);
+        atomicAdd(&meta->pp, gpp);
+        atomicAdd(&meta->pc, gpc);
         atomicMax(&meta->maxStack, maxsp);
         atomicMax(&meta->maxList[0], nf);
         atomicMax(&meta->maxList[1], np);
@@ -568,12 +589,12 @@ __device__ void eval_group(const Group& G, long long r, EvalLDS<PARTS>& S, const
     };
     auto index_of = [&](bool partial, int i, uint64_t& m) -> int {
         if (partial) {
-            const uint3 e = partIn[i];
+            const uint3 e = at32(partIn, i);
             m = (uint64_t)e.y | ((uint64_t)e.z << 32);
             return (int)e.x;
         }
         m = G.gmask;
-        return (int)fullIn[i];
+        return (int)at32(fullIn, i);
     };
     // Full cells go in list order, chunk c holding entries [64c, 64c + 64).
     // Partial cells are dealt to the chunks with a stride (chunk c holds
@@ -635,7 +656,7 @@ __device__ void eval_group(const Group& G, long long r, EvalLDS<PARTS>& S, const
     for (int c0 = 0; c0 < nlf; c0 += kWave) {
         int lcur = 0, lcnt = 0;
         if (c0 + lane < nlf) {
-            const uint3 e = leafIn[c0 + lane];
+            const uint3 e = at32(leafIn, c0 + lane);
             lcur = trav[e.x].lo;
             lcnt = trav[e.x].hi - lcur;
         }
@@ -653,7 +674,7 @@ __device__ void eval_group(const Group& G, long long r, EvalLDS<PARTS>& S, const
             int lcur = 0, lcnt = 0;
             uint64_t lmask = 0;
             if (j < nlp) {
-                const uint3 e = leafIn[lists.cl - 1 - j];
+                const uint3 e = at32(leafIn, lists.cl - 1 - j);
                 lcur = trav[e.x].lo;
                 lcnt = trav[e.x].hi - lcur;
                 lmask = (uint64_t)e.y | ((uint64_t)e.z << 32);
@@ -706,8 +727,8 @@ __global__ __launch_bounds__(64) void k_eval32(const double4* __restrict__ spos,
 // persistent set, a grid of resident waves takes groups from a work queue
 // (one launch for any number of groups and instances), and each workgroup
 // reuses one list region (lists.perSlot).
-template <bool QUAD, bool EPS0, bool Q16, int STACK, int PARTS>
-__global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
+template <bool QUAD, bool EPS0, bool Q16, int STACK, int PARTS, int WAVES>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_walk32(const double4* __restrict__ spos, const TravNode* __restrict__ trav,
                                                const Mult32* __restrict__ m32, const Mult16* __restrict__ m16,
                                                const int32_t* __restrict__ order, Meta* meta, Batch batch,
                                                Lists lists, int persistent, double* __restrict__ oax,
@@ -735,7 +756,10 @@ __global__ __launch_bounds__(64) void k_walk32(const double4* __restrict__ spos,
                     q = (q + 1) % 8;
                     tried++;
                 }
-            g = __shfl(next, 0, kWave);
+            // lane 0's claim, in scalar registers (the list regions' bases
+            // and every per-group offset derived from it stay uniform)
+            g = (long long)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(next >> 32)) << 32) |
+                            (uint32_t)__builtin_amdgcn_readfirstlane((int)next));
         }
         Group G;
         if (!load_group(G, spos, batch, g)) return;
@@ -756,17 +780,18 @@ using WalkKernel = void (*)(const double4*, const TravNode*, const Mult32*, cons
 // variant: 0 quadrupole, 1 quadrupole + FP16-stored, 2 monopole; x2 for eps0
 template <int C>
 constexpr WalkKernel walk_kernel(int variant) {
-    constexpr int st = kWalkConfig[C].stack, pa = kWalkConfig[C].parts;
+    constexpr int st = kWalkConfig[C].stack, pa = kWalkConfig[C].parts, wv = kWalkConfig[C].waves;
     switch (variant) {
-        case 0: return k_walk32<true, false, false, st, pa>;
-        case 1: return k_walk32<true, false, true, st, pa>;
-        case 2: return k_walk32<false, false, false, st, pa>;
-        case 3: return k_walk32<true, true, false, st, pa>;
-        case 4: return k_walk32<true, true, true, st, pa>;
-        default: return k_walk32<false, true, false, st, pa>;
+        case 0: return k_walk32<true, false, false, st, pa, wv>;
+        case 1: return k_walk32<true, false, true, st, pa, wv>;
+        case 2: return k_walk32<false, false, false, st, pa, wv>;
+        case 3: return k_walk32<true, true, false, st, pa, wv>;
+        case 4: return k_walk32<true, true, true, st, pa, wv>;
+        default: return k_walk32<false, true, false, st, pa, wv>;
     }
 }
 
+static_assert(kWalkConfigs == 8, "walk_kernel_of dispatches every configuration");
 WalkKernel walk_kernel_of(int config, int variant) {
     switch (config) {
         case 0: return walk_kernel<0>(variant);
@@ -774,7 +799,9 @@ WalkKernel walk_kernel_of(int config, int variant) {
         case 2: return walk_kernel<2>(variant);
         case 3: return walk_kernel<3>(variant);
         case 4: return walk_kernel<4>(variant);
-        default: return walk_kernel<5>(variant);
+        case 5: return walk_kernel<5>(variant);
+        case 6: return walk_kernel<6>(variant);
+        default: return walk_kernel<7>(variant);
     }
 }
 

@@ -157,7 +157,8 @@ class Solver:
     """
 
     def __init__(self, theta: float = 0.0, monopole_only: bool = False, walk: str = "fp32", timing: bool = False,
-                 group_size: int = 64, persistent: int = 1, stack: int = 768, parts: int = 256):
+                 group_size: int = 64, persistent: int = 1, stack: int = 768, parts: int = 256,
+                 waves: int = 4):
         reason = gpu_available()
         if reason is not None:
             raise RuntimeError(f"GPU backend unavailable: {reason}")
@@ -174,6 +175,7 @@ class Solver:
         self.persistent = persistent
         self.stack = stack
         self.parts = parts
+        self.waves = waves
 
     def __del__(self):
         try:
@@ -188,7 +190,7 @@ class Solver:
         _lib.check(self._L.gx_solver_config(self._h, float(self.theta), int(self.monopole_only), w,
                                             int(self.timing)))
         _lib.check(self._L.gx_solver_tune_walk(self._h, int(self.group_size), int(self.persistent), int(self.stack),
-                                               int(self.parts)))
+                                               int(self.parts), int(self.waves)))
 
     def autotune(self, s: DeviceState, offsets, theta, eps, reps: int = 3, dark_matter_off=False) -> list:
         """The library's walk autotuner on the systems packed in s (see
@@ -206,7 +208,7 @@ class Solver:
         if rows:
             best = min(rows, key=lambda r: r["ms"])
             self.group_size, self.persistent = best["group_size"], best["persistent"]
-            self.stack, self.parts = best["stack"], best["parts"]
+            self.stack, self.parts, self.waves = best["stack"], best["parts"], best["waves"]
         return rows
 
     def reserve(self, nodes=0, full=0, partial=0, leaves=0):

@@ -85,15 +85,17 @@ int gx_solver_tune(gx_solver *s, int group_size, int persistent);
  * captured by gx_graph_capture_step must then be captured again. */
 int gx_solver_reserve(gx_solver *s, int64_t nodes, int32_t full, int32_t partial, int32_t leaves);
 
-/* Walk tuning with the FP32 walk's compile-time sizes (no counterpart in the
+/* Walk tuning with the FP32 walk's compile-time shapes (no counterpart in the
  * reference): as gx_solver_tune, plus the traversal stack (entries, 384 or
- * 768) and the stars staged per evaluation flush (128, 256 or 384); other
- * pairs are GX_ERR_ARG.  Every choice gives the same interaction set. */
-int gx_solver_tune_walk(gx_solver *s, int group_size, int persistent, int stack, int parts);
+ * 768), the stars staged per evaluation flush (128, 256 or 384) and the
+ * wavefronts per SIMD the registers are fitted for (4; the stack of 384 with
+ * the staging of 128 is also built for 5 and 6); other triples are
+ * GX_ERR_ARG.  Every choice gives the same interaction set. */
+int gx_solver_tune_walk(gx_solver *s, int group_size, int persistent, int stack, int parts, int waves);
 
 /* One timed candidate of the walk autotuner. */
 typedef struct {
-    int32_t group_size, persistent, stack, parts;
+    int32_t group_size, persistent, stack, parts, waves;
     float ms;                 /* mean GPU time of one walk (INFINITY: stack overflow) */
     int64_t pp_interactions;  /* the candidate's counts (equal for every candidate) */
     int64_t pc_interactions;
@@ -102,7 +104,7 @@ typedef struct {
 /* Walk autotuner (no counterpart in the reference): on the given bodies
  * (device arrays and instances, as gx_batch_accelerations_dev) builds the
  * tree, then times reps FP32 walks of it for every candidate -- group size
- * 64 and 32, each stack/staging pair of gx_solver_tune_walk, work queue
+ * 64 and 32, each walk shape of gx_solver_tune_walk, work queue
  * never (0) and always (2) -- and keeps the fastest in the solver.  The
  * choice is cached per workload shape (body count, instance count, walk,
  * monopole): a second call for the same shape applies it without timing.

@@ -13,7 +13,8 @@ from gravitas_amd import DeviceState, Params, Solver, ics
 
 pytestmark = pytest.mark.gpu
 
-STACKS, PARTS = (384, 768), (128, 256, 384)
+# every built walk shape: (traversal stack, staging, wavefronts per SIMD)
+SHAPES = tuple((st, pa, 4) for st in (384, 768) for pa in (128, 256, 384)) + ((384, 128, 5), (384, 128, 6))
 
 
 def say(msg):
@@ -53,21 +54,22 @@ def test_every_candidate_walks_the_fp64_interaction_set(gpu, workload):
         results = {}
         for gsize in (64, 32):
             for pers in (0, 2):
-                for stack in STACKS:
-                    for parts in PARTS:
-                        say(f"[tune] {name} group {gsize} queue {pers} stack {stack} staging {parts}")
-                        ds = DeviceState.from_host(state)
-                        sv = Solver(walk="fp32", group_size=gsize, persistent=pers, stack=stack, parts=parts)
-                        sv.batch_accelerations(ds, offsets, theta, ep)
-                        st = sv.stats()
-                        assert (st["pp_interactions"], st["pc_interactions"]) == counts
-                        assert st["fallbacks"] == 0
-                        acc = ds.acc_host()
-                        check(acc, ref)
-                        results[(gsize, pers, stack, parts)] = acc
-        # the stack size and the work queue do not change a group's arithmetic
-        for (gsize, pers, stack, parts), acc in results.items():
-            assert np.array_equal(acc, results[(gsize, 0, 768, parts)])
+                for stack, parts, waves in SHAPES:
+                    say(f"[tune] {name} group {gsize} queue {pers} stack {stack} staging {parts} waves {waves}")
+                    ds = DeviceState.from_host(state)
+                    sv = Solver(walk="fp32", group_size=gsize, persistent=pers, stack=stack, parts=parts,
+                                waves=waves)
+                    sv.batch_accelerations(ds, offsets, theta, ep)
+                    st = sv.stats()
+                    assert (st["pp_interactions"], st["pc_interactions"]) == counts
+                    assert st["fallbacks"] == 0
+                    acc = ds.acc_host()
+                    check(acc, ref)
+                    results[(gsize, pers, stack, parts, waves)] = acc
+        # the stack size, the register budget and the work queue do not change
+        # a group's arithmetic
+        for (gsize, pers, stack, parts, waves), acc in results.items():
+            assert np.array_equal(acc, results[(gsize, 0, 768, parts, 4)])
 
 
 def test_bad_walk_sizes_are_argument_errors(gpu):
@@ -86,12 +88,12 @@ def test_autotuner_times_all_and_caches(gpu, workload):
     sv = Solver(theta=0.5, walk="fp32")
     say("[tune] autotune disk+halo")
     table = sv.autotune(ds, [0, n], 0.5, 0.02, reps=2)
-    assert len(table) == 2 * 2 * len(STACKS) * len(PARTS)
+    assert len(table) == 2 * 2 * len(SHAPES)
     assert all((r["pp_interactions"], r["pc_interactions"]) == counts for r in table)
     assert all(0 < r["ms"] < 1e3 for r in table)
     best = min(table, key=lambda r: r["ms"])
-    assert (sv.group_size, sv.persistent, sv.stack, sv.parts) == (best["group_size"], best["persistent"],
-                                                                 best["stack"], best["parts"])
+    assert (sv.group_size, sv.persistent, sv.stack, sv.parts, sv.waves) == (
+        best["group_size"], best["persistent"], best["stack"], best["parts"], best["waves"])
     check(ds.acc_host(), ref)  # the chosen walk's forces
     # the same shape again: the cached choice, the same table, no new timing
     again = sv.autotune(ds, [0, n], 0.5, 0.02, reps=2)
