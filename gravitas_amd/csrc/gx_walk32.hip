@@ -539,7 +539,7 @@ __device__ __forceinline__ void stage_leaves(EvalLDS<PARTS>& S, int& nst, int lc
     __syncthreads();  // the tables are rewritten by the next batch
 }
 
-template <bool QUAD, bool EPS0, bool Q16, int PARTS>
+template <bool QUAD, bool EPS0, bool Q16, int PARTS, bool PREFETCH>
 __device__ void eval_group(const Group& G, long long r, EvalLDS<PARTS>& S, const double4* __restrict__ spos,
                            const TravNode* __restrict__ trav, const Mult32* __restrict__ m32,
                            const Mult16* __restrict__ m16, const int32_t* __restrict__ order, const Lists& lists,
@@ -560,8 +560,9 @@ __device__ void eval_group(const Group& G, long long r, EvalLDS<PARTS>& S, const
     const uint3* __restrict__ leafIn = lists.leaf + (size_t)r * lists.cl;
     Sums sum;
 
-    // Cell chunks are software-pipelined: while chunk i is evaluated from LDS,
-    // the node data of chunk i+1 and the indices of chunk i+2 are in flight.
+    // With PREFETCH the cell chunks are software-pipelined: while chunk i is
+    // evaluated from LDS, the node data of chunk i+1 and the indices of chunk
+    // i+2 are in flight.
     struct Staged {
         float4 a, b;
         float2 c;
@@ -605,6 +606,31 @@ __device__ void eval_group(const Group& G, long long r, EvalLDS<PARTS>& S, const
         if (n == 0) continue;
         const int nch = (n + kChunk - 1) / kChunk;
         auto entry = [&](int c, int j) { return partial ? c + nch * j : c * kChunk + j; };
+        if (!PREFETCH) {
+            // at high occupancy the other wavefronts hide the fetch: no
+            // registers held for the next chunk
+            for (int c = 0; c < nch; c++) {
+                const bool mine = entry(c, lane) < n;
+                const int cnt = partial ? (n - c + nch - 1) / nch : min(kChunk, n - c * kChunk);
+                uint64_t m = 0;
+                if (mine) {
+                    const int idx = index_of(partial, entry(c, lane), m);
+                    const Staged d = fetch(idx, m);
+                    cell[3 * lane] = d.a;
+                    cell[3 * lane + 1] = d.b;
+                    cell[3 * lane + 2] = make_float4(d.c.x, d.c.y, 0.0f, 0.0f);
+                }
+                __syncthreads();
+                float ax = 0, ay = 0, az = 0;
+                if (partial)
+                    eval_cells_own<QUAD>(cell, transpose_bits(mine ? m : 0ull), px, py, pz, eps2, ax, ay, az);
+                else
+                    eval_cells<QUAD>(cell, cnt, px, py, pz, eps2, ax, ay, az);
+                sum.add(ax, ay, az);
+                __syncthreads();
+            }
+            continue;
+        }
         uint64_t m0 = 0, m1 = 0;
         int idx0 = entry(0, lane) < n ? index_of(partial, entry(0, lane), m0) : 0;
         int idx1 = nch > 1 && entry(1, lane) < n ? index_of(partial, entry(1, lane), m1) : 0;
@@ -717,7 +743,7 @@ __global__ __launch_bounds__(64) void k_eval32(const double4* __restrict__ spos,
     Group G;
     const long long g = xcd_group(batch);
     if (!load_group(G, spos, batch, g)) return;
-    eval_group<QUAD, EPS0, Q16, kSplitParts>(G, g, S, spos, trav, m32, m16, order, lists, oax, oay, oaz);
+    eval_group<QUAD, EPS0, Q16, kSplitParts, true>(G, g, S, spos, trav, m32, m16, order, lists, oax, oay, oaz);
 }
 
 // Traversal and evaluation of one group in one wavefront: the latency-bound
@@ -766,7 +792,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) voi
         const long long r = lists.perSlot ? (long long)blockIdx.x : g;
         const bool ok = trav_group<STACK>(G, r, S.t, trav, meta, lists);
         __syncthreads();  // lists written by this wave's lanes are read back by others
-        if (ok) eval_group<QUAD, EPS0, Q16, PARTS>(G, r, S.e, spos, trav, m32, m16, order, lists, oax, oay, oaz);
+        if (ok) eval_group<QUAD, EPS0, Q16, PARTS, (WAVES < 6)>(G, r, S.e, spos, trav, m32, m16, order, lists, oax, oay, oaz);
         if (!persistent) return;
         __syncthreads();
     }
@@ -791,7 +817,7 @@ constexpr WalkKernel walk_kernel(int variant) {
     }
 }
 
-static_assert(kWalkConfigs == 8, "walk_kernel_of dispatches every configuration");
+static_assert(kWalkConfigs == 9, "walk_kernel_of dispatches every configuration");
 WalkKernel walk_kernel_of(int config, int variant) {
     switch (config) {
         case 0: return walk_kernel<0>(variant);
@@ -801,7 +827,8 @@ WalkKernel walk_kernel_of(int config, int variant) {
         case 4: return walk_kernel<4>(variant);
         case 5: return walk_kernel<5>(variant);
         case 6: return walk_kernel<6>(variant);
-        default: return walk_kernel<7>(variant);
+        case 7: return walk_kernel<7>(variant);
+        default: return walk_kernel<8>(variant);
     }
 }
 

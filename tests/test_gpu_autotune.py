@@ -14,7 +14,8 @@ from gravitas_amd import DeviceState, Params, Solver, ics
 pytestmark = pytest.mark.gpu
 
 # every built walk shape: (traversal stack, staging, wavefronts per SIMD)
-SHAPES = tuple((st, pa, 4) for st in (384, 768) for pa in (128, 256, 384)) + ((384, 128, 5), (384, 128, 6))
+SHAPES = tuple((st, pa, 4) for st in (384, 768) for pa in (128, 256, 384)) + ((384, 128, 5), (384, 128, 6),
+                                                                              (384, 96, 7))
 
 
 def say(msg):
@@ -67,9 +68,12 @@ def test_every_candidate_walks_the_fp64_interaction_set(gpu, workload):
                     check(acc, ref)
                     results[(gsize, pers, stack, parts, waves)] = acc
         # the stack size, the register budget and the work queue do not change
-        # a group's arithmetic
+        # a group's arithmetic (the staging size does: it sets the flushes)
+        first = {}
+        for stack, parts, waves in SHAPES:
+            first.setdefault(parts, (stack, waves))
         for (gsize, pers, stack, parts, waves), acc in results.items():
-            assert np.array_equal(acc, results[(gsize, 0, 768, parts, 4)])
+            assert np.array_equal(acc, results[(gsize, 0, first[parts][0], parts, first[parts][1])])
 
 
 def test_bad_walk_sizes_are_argument_errors(gpu):
