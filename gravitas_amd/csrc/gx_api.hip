@@ -102,7 +102,7 @@ struct gx_solver {
     // host-API staging
     DevBuf x, y, z, m, kind, alive, ax, ay, az;
     // arena
-    DevBuf keys, keys2, order, order2, rank, holes, hist, digitTotal, blockStar, blockHole, blockBox, spos;
+    DevBuf keys, keys2, order, order2, rank, holes, hist, digitTotal, blockStar, blockHole, blockBox, spos, instOf;
     DevBuf trav, m64, m32, m16, meta, partial;
     DevBuf lfull, lpart, lleaf, lcounts;
     // active sets and block steps
@@ -165,7 +165,7 @@ struct gx_solver {
     std::vector<DevBuf*> buffers() {
         return {&x,        &y,       &z,        &m,          &kind,         &alive,        &ax,        &ay,
                 &az,       &keys,    &keys2,    &order,      &order2,       &rank,         &holes,     &hist,
-                &digitTotal, &blockStar, &blockHole, &blockBox, &spos,   &trav,         &m64,          &m32,       &m16,
+                &digitTotal, &blockStar, &blockHole, &blockBox, &spos,   &instOf,       &trav,         &m64,          &m32,       &m16,
                 &meta,     &partial, &lfull,    &lpart,      &lleaf,        &lcounts,      &instCfg,   &boxEnc,
                 &starCount, &starOff, &groupOff, &root,      &half,         &actS,         &blockAct,  &tOff,
                 &tlist,    &activeMask, &rung,  &rungHist,   &diag,         &nearFlag,     &nearList,  &tileCnt,
@@ -432,6 +432,7 @@ Batch make_batch(gx_solver* s) {
     b.half = (double*)s->half.p;
     b.tOff = b.starOff;
     b.tlist = nullptr;
+    b.instOf = s->B > 1 ? (const uint16_t*)s->instOf.p : nullptr;
     return b;
 }
 
@@ -445,7 +446,7 @@ int reserve(gx_solver* s, long long n) {
         (rc = ensure(s->hist, (size_t)256 * std::max(nt, 1LL) * 4)) || (rc = ensure(s->digitTotal, 256 * 4)) ||
         (rc = ensure(s->blockStar, std::max(nb, 1LL) * 4)) || (rc = ensure(s->blockHole, std::max(nb, 1LL) * 4)) ||
         (rc = ensure(s->blockBox, std::max(nb, 1LL) * 48)) ||
-        (rc = ensure(s->spos, std::max(n, 1LL) * 32)))
+        (rc = ensure(s->spos, std::max(n, 1LL) * 32)) || (s->B > 1 && (rc = ensure(s->instOf, n * 2))))
         return rc;
     if (!s->meta.p) {  // allocated once; every field starts at zero (the sticky word included)
         if ((rc = ensure(s->meta, sizeof(Meta)))) return rc;
@@ -476,10 +477,10 @@ void mark(gx_solver* s, int i) {
     if ((s->timing || i == 0 || i == kPhases) && !s->capturing) (void)hipEventRecord(s->ev[i], s->stream);
 }
 
-// collect (build.go:62-113): stars in state order into order[], alive holes
-// into holes[], counts and boxes into Meta and the instance arrays.  Needs
-// reserve() first.
-void enqueue_collect(gx_solver* s, const StepArgs& a) {
+// collect (build.go:62-113): stars in state order into order[] (order2[]
+// with spare, for an odd number of sort passes), alive holes into holes[],
+// counts and boxes into Meta and the instance arrays.  Needs reserve() first.
+void enqueue_collect(gx_solver* s, const StepArgs& a, bool spare) {
     const long long n = a.n;
     const long long nb = (n + kCollectTile - 1) / kCollectTile;
     hipStream_t st = s->stream;
@@ -492,7 +493,8 @@ void enqueue_collect(gx_solver* s, const StepArgs& a) {
                        (int*)s->blockHole.p, (const double*)s->blockBox.p, meta, batch);
     hipLaunchKernelGGL(k_collect_scatter, dim3(blocks(n, kCollectTile)), dim3(256), 0, st, a.x, a.y, a.z, a.kind,
                        a.alive, n, (const int*)s->blockStar.p, (const int*)s->blockHole.p, batch,
-                       (int32_t*)s->order.p, (int32_t*)s->holes.p, (uint64_t*)s->keys.p);
+                       (int32_t*)(spare ? s->order2.p : s->order.p), (int32_t*)s->holes.p,
+                       (uint64_t*)(spare ? s->keys2.p : s->keys.p), (uint16_t*)s->instOf.p);
 }
 
 // Build the trees of all instances (collect .. moments).  theta is baked into
@@ -505,17 +507,20 @@ int enqueue_build(gx_solver* s, const StepArgs& a) {
     Meta* meta = (Meta*)s->meta.p;
     const Batch batch = make_batch(s);
     const int ntiles = std::max(1, (int)((n + kSortTile - 1) / kSortTile));
+    // 8 passes over the 63-bit keys, then (several instances) 1 or 2 passes
+    // over the instance id (up to 256 instances, up to 65,536), most
+    // significant last; the collect writes into the spare arrays when the
+    // count is odd, so that the sorted pairs end in keys/order
+    const int passes = 8 + (s->B > 256 ? 2 : s->B > 1 ? 1 : 0);
+    const bool odd = passes & 1;
     mark(s, 0);
-    enqueue_collect(s, a);
+    enqueue_collect(s, a, odd);
     mark(s, 1);  // the keys are written by the collect scatter (phase "keys" stays empty)
     mark(s, 2);
-    // 8 passes over the 63-bit keys, then (several instances) 2 passes over the
-    // instance id, most significant last; an even count ends in keys/order.
-    uint64_t* kin = (uint64_t*)s->keys.p;
-    uint64_t* kout = (uint64_t*)s->keys2.p;
-    int32_t* vin = (int32_t*)s->order.p;
-    int32_t* vout = (int32_t*)s->order2.p;
-    const int passes = s->B > 1 ? 10 : 8;
+    uint64_t* kin = (uint64_t*)(odd ? s->keys2.p : s->keys.p);
+    uint64_t* kout = (uint64_t*)(odd ? s->keys.p : s->keys2.p);
+    int32_t* vin = (int32_t*)(odd ? s->order2.p : s->order.p);
+    int32_t* vout = (int32_t*)(odd ? s->order.p : s->order2.p);
     for (int pass = 0; pass < passes; pass++) {
         const int inst = pass >= 8;
         const int shift = inst ? 8 * (pass - 8) : 8 * pass;
@@ -1264,7 +1269,7 @@ int split_step(gx_solver* s, StepArgs a, double* vx, double* vy, double* vz, dou
     const unsigned grid = (unsigned)std::min<long long>(blocks(n, 256), 4096);
     const double eps2 = eps * eps;
     // the alive holes in state order (collectHoles, sim.go:291-299)
-    enqueue_collect(s, a);
+    enqueue_collect(s, a, false);
     HIPCHK(hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     const long long nh = s->hmeta->nholes;
@@ -1500,7 +1505,7 @@ int gx_retard_source_pos_dev(gx_solver* s, gx_retard* r, const double* x, const 
     StepArgs a = step_args(x, y, z, nullptr, kind, alive, n, 0, nullptr, nullptr, nullptr);
     int rc;
     if ((rc = set_single(s, n, s->theta, 0.0)) || (rc = reserve(s, n))) return rc;
-    enqueue_collect(s, a);
+    enqueue_collect(s, a, false);
     HIPCHK(hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     const long long nh = s->hmeta->nholes;
@@ -1531,7 +1536,7 @@ static int drag_setup(gx_solver* s, const double* x, const double* y, const doub
     int rc;
     if ((rc = set_single(s, n, s->theta, eps)) || (rc = reserve(s, n))) return rc;
     const StepArgs a = step_args(x, y, z, mass, kind, alive, n, 0, nullptr, nullptr, nullptr);
-    enqueue_collect(s, a);
+    enqueue_collect(s, a, false);
     HIPCHK(hipMemcpyAsync(s->hmeta, s->meta.p, sizeof(Meta), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     *nh = s->hmeta->nholes;

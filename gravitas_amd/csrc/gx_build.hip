@@ -371,7 +371,7 @@ __global__ __launch_bounds__(256) void k_collect_scatter(const double* __restric
                                                          const int* __restrict__ starOff,
                                                          const int* __restrict__ holeOff, Batch batch,
                                                          int32_t* __restrict__ order, int32_t* __restrict__ holes,
-                                                         uint64_t* __restrict__ keys) {
+                                                         uint64_t* __restrict__ keys, uint16_t* __restrict__ instOf) {
     constexpr int per = kCollectTile / 256, W = 256 / kWave;
     __shared__ int wcnt[2][per][W];
     const int tid = threadIdx.x, lane = lane_id(), w = tid / kWave;
@@ -409,6 +409,7 @@ __global__ __launch_bounds__(256) void k_collect_scatter(const double* __restric
             const int b = batch.B > 1 ? upper_index(batch.bodyOff, batch.B, i) : 0;
             keys[os] = morton_key(x[i], y[i], z[i], batch, b);
             order[os] = (int32_t)i;
+            if (batch.B > 1) instOf[i] = (uint16_t)b;
         }
         if ((bh[j] >> lane) & 1) holes[wcnt[1][j][w] + __popcll(bh[j] & below)] = (int32_t)i;
     }
@@ -418,10 +419,11 @@ __global__ __launch_bounds__(256) void k_collect_scatter(const double* __restric
 // Stable LSD radix sort of (key, order) pairs, 8-bit digits.
 // ---------------------------------------------------------------------------
 // Digit of a sort element: 8 bits of the Morton key, or (instance passes, the
-// last ones) 8 bits of the instance of the element's body.
+// last ones) 8 bits of the instance of the element's body, which the collect
+// wrote per body.
 __device__ __forceinline__ uint32_t sort_digit(uint64_t key, int32_t val, int shift, bool instPass,
                                                const Batch& batch) {
-    if (instPass) return ((uint32_t)upper_index(batch.bodyOff, batch.B, val) >> shift) & 255u;
+    if (instPass) return ((uint32_t)batch.instOf[val] >> shift) & 255u;
     return (uint32_t)(key >> shift) & 255u;
 }
 

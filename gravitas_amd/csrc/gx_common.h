@@ -65,6 +65,9 @@ struct Batch {
     // an active set tlist is null, tOff is starOff and target t is star t.
     const long long* tOff;
     const int32_t* tlist;
+    // Instance of each body (B > 1; written by the collect for the sort's
+    // instance passes), null for a single system.
+    const uint16_t* instOf;
 };
 
 // Sorted star index of walk target t.

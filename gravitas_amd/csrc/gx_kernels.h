@@ -22,7 +22,7 @@ __global__ void k_collect_scan(int nb, int* blockStar, int* blockHole, const dou
                                Batch batch);
 __global__ void k_collect_scatter(const double* x, const double* y, const double* z, const uint8_t* kind,
                                   const uint8_t* alive, long long n, const int* starOff, const int* holeOff,
-                                  Batch batch, int32_t* order, int32_t* holes, uint64_t* keys);
+                                  Batch batch, int32_t* order, int32_t* holes, uint64_t* keys, uint16_t* instOf);
 __global__ void k_sort_hist(const uint64_t* keys, const int32_t* vals, const Meta* meta, int shift, int instPass,
                             Batch batch, uint32_t* hist, int ntiles);
 constexpr int kScanThreads = 1024;  // threads of the one-workgroup scans (collect, active sets)
