@@ -157,8 +157,8 @@ class Solver:
     """
 
     def __init__(self, theta: float = 0.0, monopole_only: bool = False, walk: str = "fp32", timing: bool = False,
-                 group_size: int = 64, persistent: int = 1, stack: int = 768, parts: int = 256,
-                 waves: int = 4):
+                 group_size: int = 64, persistent: int = 1, stack: int = 384, parts: int = 128,
+                 waves: int = 5):
         reason = gpu_available()
         if reason is not None:
             raise RuntimeError(f"GPU backend unavailable: {reason}")

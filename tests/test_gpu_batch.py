@@ -150,3 +150,31 @@ def test_batch_step_and_graph_replay_equal_eager(gpu, batch):
         oracle.kick_drift_kick(st, acc, dt)
         for k in ("px", "vy"):
             assert np.array_equal(got[k][sl], st[k]), (b, k)
+
+
+def test_batch_of_more_than_256_systems_fp64_equals_oracle(gpu):
+    # more than 256 instances: the sort orders them with two passes over the
+    # instance id (one pass up to 256); small systems, some empty
+    rng = np.random.default_rng(77)
+    sizes = [int(n) for n in rng.integers(0, 300, 300)]
+    sizes[5] = sizes[17] = 0
+    state, offsets = make_batch(sizes, 500)
+    thetas = [0.6] * len(sizes)
+    eps = [0.01 * max(n, 1) ** (-1.0 / 3.0) for n in sizes]
+    say(f"[batch] fp64 {len(sizes)} systems, n={offsets[-1]}")
+    want = np.zeros((len(state["px"]), 3))
+    for b, n in enumerate(sizes):
+        if n == 0:
+            continue
+        sl = slice(offsets[b], offsets[b + 1])
+        o = oracle.Solver(theta=thetas[b], workers=1)
+        want[sl] = o.accelerations(state["px"][sl], state["py"][sl], state["pz"][sl], state["m"][sl],
+                                   state["kind"][sl], eps=eps[b])
+    ds = DeviceState.from_host(state)
+    Solver(walk="fp64").batch_accelerations(ds, offsets, thetas, eps)
+    assert np.array_equal(ds.acc_host(), want)
+    ds32 = DeviceState.from_host(state)
+    Solver(walk="fp32").batch_accelerations(ds32, offsets, thetas, eps)
+    pulled = np.linalg.norm(want, axis=1) > 0  # a lone star feels nothing
+    e = np.sort(ics.rel_errors(ds32.acc_host()[pulled], want[pulled]))
+    assert e[len(e) // 2] <= 1e-5 and e[-1] <= 1e-3
