@@ -352,35 +352,31 @@ __device__ bool trav_group(const Group& G, long long r, TravLDS<STACK>& S, const
             node = ofirst + (lane - ostart);
             mask = omaskAll;
         }
-        double tx = 0, ty = 0, tz = 0, tc = 0;
         int tfirst = -1, tnchild = 0, tlo = 0, thi = 0;
         int cls = 0;  // 1 accept for all, 2 open for all, 3 decide per star
         if (has) {
             const TravNode t = trav[node];
-            tx = t.x; ty = t.y; tz = t.z; tc = t.crit2;
             tfirst = t.first; tnchild = t.nchild; tlo = t.lo; thi = t.hi;
-            const double ex = fmax(0.0, fmax(G.bminx - tx, tx - G.bmaxx));
-            const double ey = fmax(0.0, fmax(G.bminy - ty, ty - G.bmaxy));
-            const double ez = fmax(0.0, fmax(G.bminz - tz, tz - G.bmaxz));
-            const double dmin2 = ex * ex + ey * ey + ez * ez;
-            const double fx = fmax(fabs(tx - G.bminx), fabs(tx - G.bmaxx));
-            const double fy = fmax(fabs(ty - G.bminy), fabs(ty - G.bmaxy));
-            const double fz = fmax(fabs(tz - G.bminz), fabs(tz - G.bmaxz));
-            const double dmax2 = fx * fx + fy * fy + fz * fz;
-            if (dmin2 >= tc * (1 + kMargin))
-                cls = 1;
-            else if (dmax2 * (1 + kMargin) < tc)
-                cls = 2;
-            else
-                cls = 3;
+            // nearest and farthest distance from the group's box to the
+            // node's centre of mass; this classification is the walk's own
+            // (the margin keeps it conservative, fused or not), only the
+            // per-star test below must be the reference's arithmetic
+            const double lx = G.bminx - t.x, hx = t.x - G.bmaxx;
+            const double ly = G.bminy - t.y, hy = t.y - G.bmaxy;
+            const double lz = G.bminz - t.z, hz = t.z - G.bmaxz;
+            const double ex = fmax(0.0, fmax(lx, hx)), ey = fmax(0.0, fmax(ly, hy)), ez = fmax(0.0, fmax(lz, hz));
+            const double fx = fmax(fabs(lx), fabs(hx)), fy = fmax(fabs(ly), fabs(hy)), fz = fmax(fabs(lz), fabs(hz));
+            const double dmin2 = fma(ex, ex, fma(ey, ey, ez * ez));
+            const double dmax2 = fma(fx, fx, fma(fy, fy, fz * fz));
+            cls = dmin2 >= t.crit2 * (1 + kMargin) ? 1 : dmax2 * (1 + kMargin) < t.crit2 ? 2 : 3;
         }
-        uint64_t amask = cls == 1 ? mask : 0;
-        uint64_t omask = cls == 2 ? mask : 0;
         const uint64_t mixed = __ballot(cls == 3);
         nmixed += __popcll(mixed);
         // lane src: the ballot of its mixed node, written into that lane from
         // the scalar registers (two writelanes per node)
-        int accLo = 0, accHi = 0;
+        // (a node every star accepts starts from an all-ones ballot, one they
+        // all open from zero)
+        int accLo = cls == 1 ? -1 : 0, accHi = accLo;
         for (uint64_t todo = mixed; todo;) {
             const int src = __builtin_ctzll(todo);
             todo &= todo - 1;
@@ -398,11 +394,8 @@ __device__ bool trav_group(const Group& G, long long r, TravLDS<STACK>& S, const
             accLo = gx_llvm_writelane((int)(uint32_t)acc, src, accLo);
             accHi = gx_llvm_writelane((int)(uint32_t)(acc >> 32), src, accHi);
         }
-        if (cls == 3) {
-            const uint64_t acc = ((uint64_t)(uint32_t)accHi << 32 | (uint32_t)accLo) & mask;
-            amask = acc;
-            omask = mask & ~acc;
-        }
+        const uint64_t acc = (uint64_t)(uint32_t)accHi << 32 | (uint32_t)accLo;
+        const uint64_t amask = acc & mask, omask = mask & ~acc;
 
         // accepted cells -> full or partial list
         const bool isCell = amask != 0;
