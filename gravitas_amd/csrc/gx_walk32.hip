@@ -397,50 +397,55 @@ __device__ bool trav_group(const Group& G, long long r, TravLDS<STACK>& S, const
         const uint64_t acc = (uint64_t)(uint32_t)accHi << 32 | (uint32_t)accLo;
         const uint64_t amask = acc & mask, omask = mask & ~acc;
 
+        // The classes of the round's nodes as wave masks (compares straight
+        // into scalar registers; each lane's class is read back from the mask
+        // by the branch itself, with no vector instruction).
+        const uint64_t bcell = __ballot(amask != 0);
+        const uint64_t bfull = __ballot(amask == G.gmask) & bcell;
+        const uint64_t bpart = bcell & ~bfull;
+        const uint64_t bopen = __ballot(omask != 0);
+        const uint64_t bleaf = __ballot(tfirst < 0) & bopen;
+        const uint64_t blf = __ballot(omask == G.gmask) & bleaf;
+        const uint64_t blp = bleaf & ~blf;
+        const uint64_t pb = bopen & ~bleaf;  // opened internal nodes
+
         // accepted cells -> full or partial list
-        const bool isCell = amask != 0;
-        const bool isFull = isCell && amask == G.gmask;
-        const uint64_t bf = __ballot(isFull);
-        const uint64_t bp = __ballot(isCell && !isFull);
-        if (isFull) {
-            const int pos = nf + mbcnt(bf);
+        if (__builtin_amdgcn_inverse_ballot_w64(bfull)) {
+            const int pos = nf + mbcnt(bfull);
             if (pos < lists.cf) at32(fullOut, pos) = (uint32_t)node;
-        } else if (isCell) {
-            const int pos = np + mbcnt(bp);
-            if (pos < lists.cp) at32(partOut, pos) = make_uint3((uint32_t)node, (uint32_t)amask, (uint32_t)(amask >> 32));
         }
-        if (isCell) npc += (unsigned long long)__popcll(amask);
-        nf += __popcll(bf);
-        np += __popcll(bp);
+        if (__builtin_amdgcn_inverse_ballot_w64(bpart)) {
+            const int pos = np + mbcnt(bpart);
+            if (pos < lists.cp) at32(partOut, pos) = make_uint3((uint32_t)amask, (uint32_t)(amask >> 32), (uint32_t)node);
+        }
+        if (__builtin_amdgcn_inverse_ballot_w64(bcell)) npc += (unsigned long long)__popcll(amask);
+        nf += __popcll(bfull);
+        np += __popcll(bpart);
 
         // opened leaves -> leaf list: opened by every star from the front,
         // by some from the back
-        const bool isLeaf = omask != 0 && tfirst < 0;
-        const bool leafAll = isLeaf && omask == G.gmask;
-        const uint64_t blf = __ballot(leafAll);
-        const uint64_t blp = __ballot(isLeaf && !leafAll);
-        if (leafAll) {
+        if (__builtin_amdgcn_inverse_ballot_w64(blf)) {
             const int pos = nlf + mbcnt(blf);
-            if (pos < lists.cl) at32(leafOut, pos) = make_uint3((uint32_t)node, (uint32_t)omask, (uint32_t)(omask >> 32));
-        } else if (isLeaf) {
+            if (pos < lists.cl) at32(leafOut, pos) = make_uint3((uint32_t)omask, (uint32_t)(omask >> 32), (uint32_t)node);
+        }
+        if (__builtin_amdgcn_inverse_ballot_w64(blp)) {
             const int pos = nlp + mbcnt(blp);
             if (pos < lists.cl)
-                at32(leafOut, lists.cl - 1 - pos) = make_uint3((uint32_t)node, (uint32_t)omask, (uint32_t)(omask >> 32));
+                at32(leafOut, lists.cl - 1 - pos) = make_uint3((uint32_t)omask, (uint32_t)(omask >> 32), (uint32_t)node);
             nps += thi - tlo;
         }
-        if (isLeaf) npp += (unsigned long long)(uint32_t)__popcll(omask) * (uint32_t)(thi - tlo);
+        if (__builtin_amdgcn_inverse_ballot_w64(bleaf))
+            npp += (unsigned long long)(uint32_t)__popcll(omask) * (uint32_t)(thi - tlo);
         nlf += __popcll(blf);
         nlp += __popcll(blp);
 
         // opened internal nodes -> one stack entry with the whole child range
-        const bool openInt = omask != 0 && tfirst >= 0;
-        const uint64_t pb = __ballot(openInt);
         const int npush = __popcll(pb);
         if (sp + npush > STACK) {
             overflow = true;
             break;
         }
-        if (openInt) {
+        if (__builtin_amdgcn_inverse_ballot_w64(pb)) {
             // later lanes end lower, so lane 0's children are popped first
             const int slot = sp + npush - 1 - mbcnt(pb);
             stkRange[slot] = (uint32_t)tfirst | ((uint32_t)(tnchild - 1) << 28);
@@ -584,8 +589,8 @@ __device__ void eval_group(const Group& G, long long r, EvalLDS<PARTS>& S, const
     auto index_of = [&](bool partial, int i, uint64_t& m) -> int {
         if (partial) {
             const uint3 e = at32(partIn, i);
-            m = (uint64_t)e.y | ((uint64_t)e.z << 32);
-            return (int)e.x;
+            m = (uint64_t)e.x | ((uint64_t)e.y << 32);
+            return (int)e.z;
         }
         m = G.gmask;
         return (int)at32(fullIn, i);
@@ -676,8 +681,8 @@ __device__ void eval_group(const Group& G, long long r, EvalLDS<PARTS>& S, const
         int lcur = 0, lcnt = 0;
         if (c0 + lane < nlf) {
             const uint3 e = at32(leafIn, c0 + lane);
-            lcur = trav[e.x].lo;
-            lcnt = trav[e.x].hi - lcur;
+            lcur = trav[e.z].lo;
+            lcnt = trav[e.z].hi - lcur;
         }
         stage_leaves<false, PARTS>(S, nst, lcur, lcnt, 0ull, spos, G, flush_all);
     }
@@ -694,9 +699,9 @@ __device__ void eval_group(const Group& G, long long r, EvalLDS<PARTS>& S, const
             uint64_t lmask = 0;
             if (j < nlp) {
                 const uint3 e = at32(leafIn, lists.cl - 1 - j);
-                lcur = trav[e.x].lo;
-                lcnt = trav[e.x].hi - lcur;
-                lmask = (uint64_t)e.y | ((uint64_t)e.z << 32);
+                lcur = trav[e.z].lo;
+                lcnt = trav[e.z].hi - lcur;
+                lmask = (uint64_t)e.x | ((uint64_t)e.y << 32);
             }
             stage_leaves<true, PARTS>(S, nst, lcur, lcnt, lmask, spos, G, flush_masked);
         }
