@@ -87,27 +87,53 @@ __device__ __forceinline__ void eval_cells(const float4* __restrict__ cell, int 
         cell_force<QUAD>(cell[3 * e], cell[3 * e + 1], cell[3 * e + 2], px, py, pz, eps2, ax, ay, az);
 }
 
+// One butterfly stage of the bit-matrix transposes on a 32-bit word: lanes l
+// and l ^ s swap blocks of s bits.  The lower lane keeps the blocks of m and
+// sends its other blocks down by s; the upper lane keeps the other blocks
+// and sends the blocks of m up by s.  A shift of the sent blocks never
+// carries a bit past the word's end, so it is a rotation by a per-lane
+// amount (one v_alignbit_b32), and the lane's own part is w ^ t.
+struct Stage {
+    uint32_t nm, rot;  // blocks to send, rotation right that moves them
+    __device__ __forceinline__ Stage(int lane, int s, uint32_t m)
+        : nm((lane & s) ? m : ~m), rot((lane & s) ? 32 - s : s) {}
+    __device__ __forceinline__ uint32_t apply(uint32_t w, int s) const {
+        const uint32_t t = w & nm;
+        const uint32_t send = __builtin_amdgcn_alignbit(t, t, rot);
+        return (w ^ t) | (uint32_t)__shfl_xor((int)send, s, kWave);
+    }
+};
+
 // Transpose of the 64 x 64 bit matrix whose row l is held by lane l: returns
 // column l (bit e set iff bit l of lane e's row is set).  Six butterfly
-// stages, each swapping off-diagonal blocks with the lane l ^ s.
+// stages: the first exchanges whole words (the lower half of the lanes keeps
+// its low words and takes its partners' low words as high words, the upper
+// half the reverse), the other five run on both words with the same blocks.
 __device__ __forceinline__ uint64_t transpose_bits(uint64_t x) {
     const int lane = lane_id();
-    constexpr uint64_t keep[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
-                                  0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-        const int sft = 32 >> k;
-        const uint64_t m = keep[k];
-        const uint64_t y = (uint64_t)__shfl_xor((long long)x, sft, kWave);
-        x = (lane & sft) ? ((x & ~m) | ((y & ~m) >> sft)) : ((x & m) | ((y & m) << sft));
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    {
+        const bool up = (lane & 32) != 0;
+        const uint32_t recv = (uint32_t)__shfl_xor((int)(up ? lo : hi), 32, kWave);
+        if (up)
+            lo = recv;
+        else
+            hi = recv;
     }
-    return x;
+    constexpr uint32_t keep[5] = {0x0000FFFFu, 0x00FF00FFu, 0x0F0F0F0Fu, 0x33333333u, 0x55555555u};
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const int sft = 16 >> k;
+        const Stage st(lane, sft, keep[k]);
+        lo = st.apply(lo, sft);
+        hi = st.apply(hi, sft);
+    }
+    return (uint64_t)hi << 32 | lo;
 }
 
 // The same for a 32 x 64 matrix whose rows sit on lanes 0-31 (lanes 32-63
 // pass 0): column l has 32 bits.  The first stage only moves the high words
-// of lanes 0-31 to lanes 32-63; the other five run on 32-bit words, each lane
-// sending the block its partner keeps (no divergent branches).
+// of lanes 0-31 to lanes 32-63; the other five run on 32-bit words.
 __device__ __forceinline__ uint32_t transpose_bits32(uint64_t x) {
     const int lane = lane_id();
     const uint32_t hi = __shfl_xor((int)(uint32_t)(x >> 32), 32, kWave);
@@ -116,11 +142,7 @@ __device__ __forceinline__ uint32_t transpose_bits32(uint64_t x) {
 #pragma unroll
     for (int k = 0; k < 5; k++) {
         const int sft = 16 >> k;
-        const uint32_t m = keep[k];
-        const bool up = (lane & sft) != 0;
-        const uint32_t send = up ? (w & m) << sft : (w & ~m) >> sft;
-        const uint32_t kept = up ? (w & ~m) : (w & m);
-        w = kept | (uint32_t)__shfl_xor((int)send, sft, kWave);
+        w = Stage(lane, sft, keep[k]).apply(w, sft);
     }
     return w;
 }
