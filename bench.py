@@ -70,6 +70,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=16.0,
                     help="budget of the CPU baseline samples (half on all host threads, half on one)")
     ap.add_argument("--T", type=float, default=None, help="two-disk-4m: override the global step")
+    ap.add_argument("--shape", default=None,
+                    help="disk-halo-1m: walk shape 'stack,staging,waves' (64-star groups, work queue) instead of "
+                         "the autotuner's choice, so that profiler passes all run the same kernel")
     ap.add_argument("--calibrate", action="store_true",
                     help="also launch 3 drifts with dt = 0 over the resident bodies (k_drift reads 48 and writes "
                          "24 bytes per body): the known-size kernel that scales HBM counters")
@@ -216,9 +219,16 @@ def bench_disk_halo(args):
     state = ics.disk_halo(n, seed=2, eps=eps)
     ds = DeviceState.from_host(state)
     params = Params(softening=eps)
-    solver = Solver(theta=theta, walk="fp32", timing=True)
-    log("autotune the walk")
-    table = solver.autotune(ds, [0, n], theta, eps, reps=3)
+    if args.shape:
+        stack, parts, waves = (int(v) for v in args.shape.split(","))
+        solver = Solver(theta=theta, walk="fp32", timing=True, group_size=64, persistent=2, stack=stack,
+                        parts=parts, waves=waves)
+        table = []
+        log(f"walk shape {args.shape} (no autotune)")
+    else:
+        solver = Solver(theta=theta, walk="fp32", timing=True)
+        log("autotune the walk")
+        table = solver.autotune(ds, [0, n], theta, eps, reps=3)
     log(f"timing {args.steps} steps after {args.warmup} warm-up")
     ms, phases, st = timed_steps(solver, ds, params, args.steps, args.warmup)
     acc32 = ds.acc_host()
@@ -235,8 +245,12 @@ def bench_disk_halo(args):
     out["interactions_per_s"] = (st["pp_interactions"] + st["pc_interactions"]) / (ms * 1e-3)
     out.update(walk_summary(n, st, phases))
     out["roofline"]["traffic"] = measured_traffic("k_walk32", "disk-halo-1m", n, theta)
-    out["walk_autotune"] = {"chosen": tune_name(min(table, key=lambda r: r["ms"])),
-                            "ms_per_walk": {tune_name(r): r["ms"] for r in table}}
+    if table:
+        out["walk_autotune"] = {"chosen": tune_name(min(table, key=lambda r: r["ms"])),
+                                "ms_per_walk": {tune_name(r): r["ms"] for r in table}}
+    else:
+        out["walk_shape"] = {"stack": solver.stack, "staging": solver.parts, "waves": solver.waves,
+                             "group": 64, "queue": True}
     if not args.no_errors:
         log("FP64 walk and direct-sum sample")
         s64 = Solver(theta=theta, walk="fp64", timing=True)
