@@ -1116,62 +1116,95 @@ __device__ __forceinline__ void quad_add(double q[6], double m, double dx, doubl
 // reference's order (finish, build.go:176-217).  While building, the node's
 // record holds the cell centre in x,y,z and the half side in crit2.  (Two
 // instances, so the leaf kernel carries no registers of the other branch.)
+// The stars of a leaf (up to kLeafSize) or the children of a node (up to 8)
+// are all loaded into registers before the sums, so that a node waits for one
+// round of memory latency instead of one per term; the sums themselves keep
+// the reference's order.
 template <bool LEAF>
-__device__ void node_moments(long long ni, const double4* __restrict__ spos, TravNode* __restrict__ trav,
-                             Mult64* __restrict__ m64, Mult32* __restrict__ m32, Mult16* __restrict__ m16,
-                             const double* __restrict__ thetas) {
-    TravNode nd = trav[ni];
+__device__ void node_moments(long long ni, TravNode nd, const double4* __restrict__ spos,
+                             TravNode* __restrict__ trav, Mult64* __restrict__ m64, Mult32* __restrict__ m32,
+                             Mult16* __restrict__ m16, const double* __restrict__ thetas) {
     const double theta = thetas[nd.inst];
     const double cx = nd.x, cy = nd.y, cz = nd.z, half = nd.crit2;
     double x = 0, y = 0, z = 0, m = 0;
-    // the sums keep the reference's order; unrolling only lets the loads of
-    // several stars be in flight together
-    if (LEAF) {
-#pragma unroll 4
-        for (int j = nd.lo; j < nd.hi; j++) {
-            double4 p = spos[j];
-            double mj = p.w;
-            x = x + mj * p.x;
-            y = y + mj * p.y;
-            z = z + mj * p.z;
-            m = m + mj;
-        }
-    } else {
-        for (int c = nd.first; c < nd.first + nd.nchild; c++) {
-            double cm = m64[c].m;
-            x = x + cm * trav[c].x;
-            y = y + cm * trav[c].y;
-            z = z + cm * trav[c].z;
-            m = m + cm;
-        }
-    }
-    if (m > 0) {
-        x = x / m;
-        y = y / m;
-        z = z / m;
-    } else {
-        x = cx;
-        y = cy;
-        z = cz;
-    }
     double q[6] = {0, 0, 0, 0, 0, 0};
-    if (LEAF) {
-#pragma unroll 4
-        for (int j = nd.lo; j < nd.hi; j++) {
+    if constexpr (LEAF) {
+        // every node holds at least one star; a leaf at the deepest level may
+        // hold more than kLeafSize (coincident stars), the rest follow in a loop
+        constexpr int P = kLeafSize;
+        const int lo = nd.lo, n = nd.hi - nd.lo;
+        double4 s[P];
+#pragma unroll
+        for (int k = 0; k < P; k++) s[k] = spos[lo + min(k, n - 1)];
+#pragma unroll
+        for (int k = 0; k < P; k++)
+            if (k < n) {
+                x = x + s[k].w * s[k].x;
+                y = y + s[k].w * s[k].y;
+                z = z + s[k].w * s[k].z;
+                m = m + s[k].w;
+            }
+        for (int j = lo + P; j < lo + n; j++) {
+            double4 p = spos[j];
+            x = x + p.w * p.x;
+            y = y + p.w * p.y;
+            z = z + p.w * p.z;
+            m = m + p.w;
+        }
+        if (m > 0) {
+            x = x / m;
+            y = y / m;
+            z = z / m;
+        } else {
+            x = cx;
+            y = cy;
+            z = cz;
+        }
+#pragma unroll
+        for (int k = 0; k < P; k++)
+            if (k < n) quad_add(q, s[k].w, s[k].x - x, s[k].y - y, s[k].z - z);
+        for (int j = lo + P; j < lo + n; j++) {
             double4 p = spos[j];
             quad_add(q, p.w, p.x - x, p.y - y, p.z - z);
         }
     } else {
-        for (int c = nd.first; c < nd.first + nd.nchild; c++) {
-            const Mult64 cm = m64[c];
-            q[0] = q[0] + cm.qxx;
-            q[1] = q[1] + cm.qyy;
-            q[2] = q[2] + cm.qzz;
-            q[3] = q[3] + cm.qxy;
-            q[4] = q[4] + cm.qxz;
-            q[5] = q[5] + cm.qyz;
-            quad_add(q, cm.m, trav[c].x - x, trav[c].y - y, trav[c].z - z);
+        const int first = nd.first, n = nd.nchild;  // 1 .. 8
+        Mult64 cm[8];
+        double4 cp[8];  // the children's centres of mass (x, y, z; w unused)
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int c = first + min(k, n - 1);
+            cm[k] = m64[c];
+            cp[k] = *(const double4*)&trav[c];
         }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (k < n) {
+                x = x + cm[k].m * cp[k].x;
+                y = y + cm[k].m * cp[k].y;
+                z = z + cm[k].m * cp[k].z;
+                m = m + cm[k].m;
+            }
+        if (m > 0) {
+            x = x / m;
+            y = y / m;
+            z = z / m;
+        } else {
+            x = cx;
+            y = cy;
+            z = cz;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (k < n) {
+                q[0] = q[0] + cm[k].qxx;
+                q[1] = q[1] + cm[k].qyy;
+                q[2] = q[2] + cm[k].qzz;
+                q[3] = q[3] + cm[k].qxy;
+                q[4] = q[4] + cm[k].qxz;
+                q[5] = q[5] + cm[k].qyz;
+                quad_add(q, cm[k].m, cp[k].x - x, cp[k].y - y, cp[k].z - z);
+            }
     }
     double delta = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
     double crit = 2 * half / theta + delta;
@@ -1210,9 +1243,15 @@ __global__ __launch_bounds__(256) void k_moments_level(int L, const double4* __r
     const long long start = meta->levelStart[L];
     const long long cnt = meta->levelCount[L];
     if (meta->flags & kFlagTreeIncomplete) return;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
-         t += (long long)gridDim.x * blockDim.x)
-        if (trav[start + t].first >= 0) node_moments<false>(start + t, spos, trav, m64, m32, m16, batch.theta);
+    // node t of the level goes to thread t / gridDim.x of workgroup
+    // t % gridDim.x: a level's internal nodes (a few hundred to a few
+    // thousand) are spread over every CU instead of packed into the first
+    // workgroups, so more of their gathers are in flight at once
+    for (long long t = (long long)threadIdx.x * gridDim.x + blockIdx.x; t < cnt;
+         t += (long long)gridDim.x * blockDim.x) {
+        const TravNode nd = trav[start + t];
+        if (nd.first >= 0) node_moments<false>(start + t, nd, spos, trav, m64, m32, m16, batch.theta);
+    }
 }
 
 // Every leaf of the tree, whatever its level, in one launch: a leaf's moments
@@ -1234,9 +1273,10 @@ __global__ __launch_bounds__(256) void k_moments_leaves(const double4* __restric
     if (blockIdx.x == 0 && threadIdx.x == 0) meta->nnodes = cnt;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
          t += (long long)gridDim.x * blockDim.x) {
-        if (trav[t].first < 0)
-            node_moments<true>(t, spos, trav, m64, m32, m16, batch.theta);
-        else if (trav[t].level >= Lmax)
+        const TravNode nd = trav[t];
+        if (nd.first < 0)
+            node_moments<true>(t, nd, spos, trav, m64, m32, m16, batch.theta);
+        else if (nd.level >= Lmax)
             atomicOr(&meta->flags, (int)kFlagDepth);
     }
 }
