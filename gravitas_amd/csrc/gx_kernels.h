@@ -226,11 +226,11 @@ struct Lists {
 
 // Compile-time shapes of the FP32 walk: traversal stack entries, stars staged
 // per evaluation flush, and the wavefronts per SIMD the register allocation
-// must leave room for.  4 is what the walk needs unconstrained; 5, 6 and 7
-// make the compiler fit 96, 80 and 72 VGPRs (from 6 on without the cell
-// prefetch, and at 7 with a few spills), which only pays where the LDS of a
-// workgroup lets that many stay resident: the stack of 384 with the staging
-// of 128 (6 KiB, up to 26 wavefronts per CU) or of 96 (5.3 KiB, 30).
+// must leave room for.  4 is what the walk needs unconstrained; 5 and 6
+// make the compiler fit 96 and 80 VGPRs (at 6 without the cell prefetch and
+// with a few spills), which only pays where the LDS of a workgroup lets that
+// many stay resident: the stack of 384 with the staging of 256 (7 KiB, up to
+// 22 wavefronts per CU) or of 128 (4.75 KiB, 32).
 // kWalkConfigs lists the instances; the autotuner picks among them
 // (gx_solver_autotune).
 struct WalkConfig {
@@ -239,8 +239,8 @@ struct WalkConfig {
 constexpr int kWalkConfigs = 9;
 constexpr WalkConfig kWalkConfig[kWalkConfigs] = {{384, 128, 4}, {384, 256, 4}, {384, 384, 4},
                                                   {768, 128, 4}, {768, 256, 4}, {768, 384, 4},
-                                                  {384, 128, 5}, {384, 128, 6}, {384, 96, 7}};
-constexpr int kWalkDefault = 6;  // 384 entries, 128 stars, 5 waves
+                                                  {384, 128, 5}, {384, 128, 6}, {384, 256, 5}};
+constexpr int kWalkDefault = 8;  // 384 entries, 256 stars, 5 waves
 
 // m16 non-null selects the FP16-stored quadrupoles; eps0 marks a batch with an
 // unsoftened instance; persistentBlocks > 0 runs the fused walk as a work queue

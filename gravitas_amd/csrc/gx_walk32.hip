@@ -299,11 +299,24 @@ struct TravLDS {
     int popOwner[kWave];       // lane whose range the popped slot starts
 };
 
+// The stars of opened leaves staged for a flush, their lane masks, and the
+// staging tables of up to 64 leaf ranges (exclusive prefix of the counts,
+// first star and lane mask of each range).
 template <int PARTS>
-struct EvalLDS {
-    float4 cell[kChunk * 3];
+struct StarStage {
     float4 part[PARTS];
     uint64_t pmask[PARTS];
+    uint64_t tMask[kWave];
+    int tStart[kWave];
+    int tCur[kWave];
+};
+
+// A group evaluates its cells first, then its stars: the cell chunk and the
+// star staging share the memory.
+template <int PARTS>
+union EvalLDS {
+    float4 cell[kChunk * 3];
+    StarStage<PARTS> s;
 };
 
 template <int STACK, int PARTS>
@@ -519,13 +532,13 @@ struct Sums {
 // into the staging buffer, which is evaluated (flush) whenever it is full.
 // All lanes copy: star j of the batch goes to lane j % 64, which finds the
 // range holding it by a binary search over the ranges' exclusive prefix sums
-// (tables in the cell buffer, which is free while leaves are staged).
+// (tables beside the staged stars).
 template <bool MASKED, int PARTS, typename Flush>
 __device__ __forceinline__ void stage_leaves(EvalLDS<PARTS>& S, int& nst, int lcur, int lcnt, uint64_t lmask,
                                              const double4* __restrict__ spos, const Group& G, Flush flush) {
-    int* tStart = (int*)S.cell;  // exclusive prefix of the counts
-    int* tCur = tStart + kWave;  // first star of each range
-    uint64_t* tMask = (uint64_t*)(tCur + kWave);
+    int* tStart = S.s.tStart;
+    int* tCur = S.s.tCur;
+    uint64_t* tMask = S.s.tMask;
     const int lane = lane_id();
     const int incl = wave_incl_scan(lcnt);
     const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, kWave));
@@ -547,9 +560,9 @@ __device__ __forceinline__ void stage_leaves(EvalLDS<PARTS>& S, int& nst, int lc
                     if (tStart[o + step] <= j) o += step;
                 const double4 q = spos[tCur[o] + (j - tStart[o])];
                 const int at = nst + j0 + lane;
-                S.part[at] = make_float4((float)(q.x - G.cgx), (float)(q.y - G.cgy), (float)(q.z - G.cgz),
+                S.s.part[at] = make_float4((float)(q.x - G.cgx), (float)(q.y - G.cgy), (float)(q.z - G.cgz),
                                          (float)q.w);
-                if (MASKED) S.pmask[at] = tMask[o];
+                if (MASKED) S.s.pmask[at] = tMask[o];
             }
         }
         nst += take;
@@ -685,7 +698,7 @@ __device__ void eval_group(const Group& G, long long r, EvalLDS<PARTS>& S, const
     auto flush_all = [&]() {  // stars every lane uses
         __syncthreads();
         float ax = 0, ay = 0, az = 0;
-        eval_parts<EPS0>(S.part, 0, nst, px, py, pz, eps2, ax, ay, az);
+        eval_parts<EPS0>(S.s.part, 0, nst, px, py, pz, eps2, ax, ay, az);
         sum.add(ax, ay, az);
         nst = 0;
         __syncthreads();
@@ -693,7 +706,7 @@ __device__ void eval_group(const Group& G, long long r, EvalLDS<PARTS>& S, const
     auto flush_masked = [&]() {  // stars of leaves some lanes opened: per-lane loops
         __syncthreads();
         float ax = 0, ay = 0, az = 0;
-        eval_parts_own<EPS0>(S.part, S.pmask, 0, nst, px, py, pz, eps2, ax, ay, az);
+        eval_parts_own<EPS0>(S.s.part, S.s.pmask, 0, nst, px, py, pz, eps2, ax, ay, az);
         sum.add(ax, ay, az);
         nst = 0;
         __syncthreads();

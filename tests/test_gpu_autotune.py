@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 # every built walk shape: (traversal stack, staging, wavefronts per SIMD)
 SHAPES = tuple((st, pa, 4) for st in (384, 768) for pa in (128, 256, 384)) + ((384, 128, 5), (384, 128, 6),
-                                                                              (384, 96, 7))
+                                                                              (384, 256, 5))
 
 
 def say(msg):
