@@ -335,8 +335,9 @@ int gx_solver_tune_walk(gx_solver* s, int group_size, int persistent, int stack,
             return GX_OK;
         }
     return set_err(GX_ERR_ARG,
-                   "no walk built with stack %d, staging %d and %d waves (stack 384 or 768, staging 128, 256 or "
-                   "384, 4 waves; stack 384 with staging 128 and 5 or 6 waves, or staging 256 and 5 waves)",
+                   "no walk built with stack %d, staging %d and %d waves (4 waves: stack 384 with staging 128, "
+                   "256 or 384, stack 768 with 256 or 384; 5 waves: stack 384 with 128, 256 or 384; 6 waves: "
+                   "stack 384 with 256)",
                    stack, parts, waves);
 }
 

@@ -229,8 +229,8 @@ struct Lists {
 // must leave room for.  4 is what the walk needs unconstrained; 5 and 6
 // make the compiler fit 96 and 80 VGPRs (at 6 without the cell prefetch and
 // with a few spills), which only pays where the LDS of a workgroup lets that
-// many stay resident: the stack of 384 with the staging of 256 (7 KiB, up to
-// 22 wavefronts per CU) or of 128 (4.75 KiB, 32).
+// many stay resident: the stack of 384 with the staging of 128, 256 or 384
+// (4.75, 5.25 and 7.4 KiB, up to 32, 30 and 21 wavefronts per CU).
 // kWalkConfigs lists the instances; the autotuner picks among them
 // (gx_solver_autotune).
 struct WalkConfig {
@@ -238,8 +238,8 @@ struct WalkConfig {
 };
 constexpr int kWalkConfigs = 9;
 constexpr WalkConfig kWalkConfig[kWalkConfigs] = {{384, 128, 4}, {384, 256, 4}, {384, 384, 4},
-                                                  {768, 128, 4}, {768, 256, 4}, {768, 384, 4},
-                                                  {384, 128, 5}, {384, 128, 6}, {384, 256, 5}};
+                                                  {384, 256, 6}, {768, 256, 4}, {768, 384, 4},
+                                                  {384, 128, 5}, {384, 384, 5}, {384, 256, 5}};
 constexpr int kWalkDefault = 8;  // 384 entries, 256 stars, 5 waves
 
 // m16 non-null selects the FP16-stored quadrupoles; eps0 marks a batch with an

@@ -195,16 +195,16 @@ __device__ __forceinline__ void eval_parts(const float4* __restrict__ part, int 
 }
 
 // Stars [first, first + npart) of the staging buffer, each for the lanes of
-// its leaf's mask pmask[]: blocks of 32 stars, masks transposed (lanes 0-31
+// its leaf's mask rmask[range[]]: blocks of 32 stars, masks transposed (lanes 0-31
 // hold the rows, so every column fits in 32 bits), and each lane loops over
 // its own stars.  A 32-bit column costs four VALU instructions of loop
 // control per star against nine for a 64-bit one; the lane that does the most
 // work is mostly the same in every block, so smaller blocks add little
 // imbalance.
 template <bool EPS0>
-__device__ __forceinline__ void eval_parts_own(const float4* __restrict__ part, const uint64_t* __restrict__ pmask,
-                                               int first, int npart, float px, float py, float pz, float eps2,
-                                               float& ax, float& ay, float& az) {
+__device__ __forceinline__ void eval_parts_own(const float4* __restrict__ part, const uint8_t* __restrict__ range,
+                                               const uint64_t* __restrict__ rmask, int first, int npart, float px,
+                                               float py, float pz, float eps2, float& ax, float& ay, float& az) {
     constexpr int kBlk = 32;
     first = __builtin_amdgcn_readfirstlane(first);
     npart = __builtin_amdgcn_readfirstlane(npart);
@@ -213,7 +213,7 @@ __device__ __forceinline__ void eval_parts_own(const float4* __restrict__ part, 
     const int lane = lane_id();
     for (int b = 0; b < nb; b++) {
         const int mine = b + nb * lane;
-        uint32_t col = transpose_bits32(lane < kBlk && mine < npart ? pmask[first + mine] : 0ull);
+        uint32_t col = transpose_bits32(lane < kBlk && mine < npart ? rmask[range[first + mine]] : 0ull);
         const float4* blk = part + first + b;
         while (col) {
             const int e = __builtin_ctz(col);
@@ -299,16 +299,17 @@ struct TravLDS {
     int popOwner[kWave];       // lane whose range the popped slot starts
 };
 
-// The stars of opened leaves staged for a flush, their lane masks, and the
-// staging tables of up to 64 leaf ranges (exclusive prefix of the counts,
-// first star and lane mask of each range).
+// The stars of opened leaves staged for a flush, the staging tables of up to
+// 64 leaf ranges (lane mask, exclusive prefix of the counts and first star of
+// each range), and for the stars of partly opened leaves the range each came
+// from (its lane mask is the range's: a byte per star instead of the mask).
 template <int PARTS>
 struct StarStage {
     float4 part[PARTS];
-    uint64_t pmask[PARTS];
     uint64_t tMask[kWave];
     int tStart[kWave];
     int tCur[kWave];
+    uint8_t range[PARTS];
 };
 
 // A group evaluates its cells first, then its stars: the cell chunk and the
@@ -532,7 +533,8 @@ struct Sums {
 // into the staging buffer, which is evaluated (flush) whenever it is full.
 // All lanes copy: star j of the batch goes to lane j % 64, which finds the
 // range holding it by a binary search over the ranges' exclusive prefix sums
-// (tables beside the staged stars).
+// (tables beside the staged stars).  MASKED: the staged stars refer to this
+// call's table of masks, so they are evaluated before the call returns.
 template <bool MASKED, int PARTS, typename Flush>
 __device__ __forceinline__ void stage_leaves(EvalLDS<PARTS>& S, int& nst, int lcur, int lcnt, uint64_t lmask,
                                              const double4* __restrict__ spos, const Group& G, Flush flush) {
@@ -562,13 +564,14 @@ __device__ __forceinline__ void stage_leaves(EvalLDS<PARTS>& S, int& nst, int lc
                 const int at = nst + j0 + lane;
                 S.s.part[at] = make_float4((float)(q.x - G.cgx), (float)(q.y - G.cgy), (float)(q.z - G.cgz),
                                          (float)q.w);
-                if (MASKED) S.s.pmask[at] = tMask[o];
+                if (MASKED) S.s.range[at] = (uint8_t)o;
             }
         }
         nst += take;
         done += take;
         if (nst == PARTS) flush();
     }
+    if (MASKED && nst > 0) flush();
     __syncthreads();  // the tables are rewritten by the next batch
 }
 
@@ -706,7 +709,7 @@ __device__ void eval_group(const Group& G, long long r, EvalLDS<PARTS>& S, const
     auto flush_masked = [&]() {  // stars of leaves some lanes opened: per-lane loops
         __syncthreads();
         float ax = 0, ay = 0, az = 0;
-        eval_parts_own<EPS0>(S.s.part, S.s.pmask, 0, nst, px, py, pz, eps2, ax, ay, az);
+        eval_parts_own<EPS0>(S.s.part, S.s.range, S.s.tMask, 0, nst, px, py, pz, eps2, ax, ay, az);
         sum.add(ax, ay, az);
         nst = 0;
         __syncthreads();
@@ -740,7 +743,6 @@ __device__ void eval_group(const Group& G, long long r, EvalLDS<PARTS>& S, const
             }
             stage_leaves<true, PARTS>(S, nst, lcur, lcnt, lmask, spos, G, flush_masked);
         }
-        if (nst > 0) flush_masked();
     }
     if (G.valid) {
         const int32_t s = order[G.k];

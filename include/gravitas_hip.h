@@ -88,10 +88,11 @@ int gx_solver_reserve(gx_solver *s, int64_t nodes, int32_t full, int32_t partial
 /* Walk tuning with the FP32 walk's compile-time shapes (no counterpart in the
  * reference): as gx_solver_tune, plus the traversal stack (entries, 384 or
  * 768), the stars staged per evaluation flush (128, 256 or 384) and the
- * wavefronts per SIMD the registers are fitted for (4; the stack of 384 is
- * also built with the staging of 128 for 5 and 6 and with a staging of 256
- * for 5); other triples are GX_ERR_ARG.  Every choice gives the same
- * interaction set. */
+ * wavefronts per SIMD the registers are fitted for.  Built: 4 wavefronts with
+ * the stack of 384 and staging 128, 256 or 384, and with the stack of 768 and
+ * staging 256 or 384; 5 wavefronts with the stack of 384 and staging 128, 256
+ * or 384; 6 with the stack of 384 and staging 256.  Other triples are
+ * GX_ERR_ARG.  Every choice gives the same interaction set. */
 int gx_solver_tune_walk(gx_solver *s, int group_size, int persistent, int stack, int parts, int waves);
 
 /* One timed candidate of the walk autotuner. */

@@ -14,8 +14,8 @@ from gravitas_amd import DeviceState, Params, Solver, ics
 pytestmark = pytest.mark.gpu
 
 # every built walk shape: (traversal stack, staging, wavefronts per SIMD)
-SHAPES = tuple((st, pa, 4) for st in (384, 768) for pa in (128, 256, 384)) + ((384, 128, 5), (384, 128, 6),
-                                                                              (384, 256, 5))
+SHAPES = ((384, 128, 4), (384, 256, 4), (384, 384, 4), (768, 256, 4), (768, 384, 4), (384, 128, 5), (384, 256, 5),
+          (384, 384, 5), (384, 256, 6))
 
 
 def say(msg):
