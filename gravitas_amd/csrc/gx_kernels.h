@@ -240,7 +240,7 @@ constexpr int kWalkConfigs = 9;
 constexpr WalkConfig kWalkConfig[kWalkConfigs] = {{384, 128, 4}, {384, 256, 4}, {384, 384, 4},
                                                   {384, 256, 6}, {768, 256, 4}, {768, 384, 4},
                                                   {384, 128, 5}, {384, 384, 5}, {384, 256, 5}};
-constexpr int kWalkDefault = 8;  // 384 entries, 256 stars, 5 waves
+constexpr int kWalkDefault = 7;  // 384 entries, 384 stars, 5 waves
 
 // m16 non-null selects the FP16-stored quadrupoles; eps0 marks a batch with an
 // unsoftened instance; persistentBlocks > 0 runs the fused walk as a work queue

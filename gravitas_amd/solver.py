@@ -157,7 +157,7 @@ class Solver:
     """
 
     def __init__(self, theta: float = 0.0, monopole_only: bool = False, walk: str = "fp32", timing: bool = False,
-                 group_size: int = 64, persistent: int = 1, stack: int = 384, parts: int = 256,
+                 group_size: int = 64, persistent: int = 1, stack: int = 384, parts: int = 384,
                  waves: int = 5):
         reason = gpu_available()
         if reason is not None:
