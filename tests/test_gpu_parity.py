@@ -296,6 +296,19 @@ def test_repeat_is_deterministic(gpu):
     assert np.array_equal(a, b)
 
 
+def test_split_walk_equals_fused_walk(gpu, monkeypatch):
+    # GX_SPLIT_WALK=1 runs the traversal and the evaluation as two kernels
+    # (for timing them apart): the same lists and sums, so the same bits
+    s = ics.clumps(20000, 8, 73, scale=0.01)
+    say("[parity] split walk")
+    fused = run_gpu(s, 0.5, 1e-3, "fp32")
+    monkeypatch.setenv("GX_SPLIT_WALK", "1")
+    split = run_gpu(s, 0.5, 1e-3, "fp32")
+    assert np.array_equal(fused["acc"], split["acc"])
+    for k in ("pp_interactions", "pc_interactions"):
+        assert fused["stats"][k] == split["stats"][k]
+
+
 def test_deeper_tree_after_shallow_one_is_rebuilt_in_full(gpu):
     """The solver builds as many levels as the previous tree needed (+2); a
     deeper tree in the next call must be detected and built in full."""
